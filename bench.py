@@ -1,0 +1,197 @@
+"""Benchmark: residual Mcell-updates/s of the MI355X residual path.
+
+Workload (one GPU): 3D Euler, 100^3 hexes split into 6,000,000 tets on a periodic
+[0,10]^3 box (vertices jittered +-5% from seed 2, cells RCM-ordered on upload),
+isentropic vortex (strength 5) advected at M=0.5, second-order MUSCL with LSQ
+gradients and the Venkatakrishnan limiter (K=5), HLLC, classical RK4, CFL 0.8.
+A step is one RK4 step = 4 residual evaluations (time step on the device
+included); Mcell-updates/s = cells * 4 * steps / time.  `value` is the FP64
+form; the FP32+FP64-accumulate form and its density error after the same steps
+are reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 100] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "residual Mcell-updates/s, FP64 vs FP32+refine at stated err; adjoint/primal ratio"
+HBM_PEAK_GBS = 8000.0
+
+
+def byte_model(nc, nf, nb, dim=3, nv=5, mixed=False, rk4=True):
+    """Algorithmic HBM bytes per launch of the two residual kernels (compulsory traffic, each
+    array once per pass, int32 indices): gradient+limiter pass and flux+gather+RK-update pass."""
+    ps = pg = px = 4 if mixed else 8
+    E = 2 * nf - nb
+    b_grad = nc * (nv * ps + 8 * dim + dim * (dim + 1) // 2 * pg + dim * nv * pg + 4) + 8 * nf + 4 * E
+    b_flux = nc * (nv * ps + 8 + 4) + nf * (dim * px + 8) + 4 * E
+    b_flux += nc * (dim * nv * pg + 8 * dim) + nf * dim * px
+    b_rk = nc * nv * (22 + 0.75 * ps) if rk4 else 0.0
+    return dict(grad=b_grad, flux=b_flux + b_rk, residual_only=b_grad + b_flux, total=b_grad + b_flux + b_rk)
+
+
+def run_gpu(m, s, g, u, steps, warmup, precision):
+    from typhon_amd.solver import Solver
+    from typhon_amd import scheme as S
+    sc = dict(s)
+    sc["precision"] = precision
+    solver = Solver(m, sc, g)
+    solver.set_state(u)
+    solver.advance(warmup)
+    solver.sync()
+    solver.timing(True)  # reset and enable per-kernel events
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+    sync()
+    solver.sync()
+    t0 = time.perf_counter()
+    solver.advance(steps)
+    solver.sync()
+    sync()
+    t1 = time.perf_counter()
+    tm = solver.timing(False)
+    state = solver.get_state()
+    info = solver.mesh_info()
+    solver.close()
+    return t1 - t0, tm, state, info
+
+
+def cpu_baseline(n_cpu, budget_s=20.0):
+    """Oracle C restatement (1 thread) on a bounded sample of the same workload."""
+    import cases as K
+    m, s, g, u = K.vortex_case(n_cpu, L=10.0)
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        dt = o.timestep(s["cfl"])
+        o.rk_step(dt)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s or steps >= 50:
+            break
+    el = time.perf_counter() - t0
+    rate = m.ncell_int * 4 * steps / el / 1e6
+    return dict(value=rate, unit="Mcell-updates/s", cores=1, kind="port",
+                sample=f"oracle C restatement, 1 thread, same scheme on {n_cpu}^3 hexes x 6 = {m.ncell_int} "
+                       f"periodic tets, {steps} RK4 steps ({el:.1f} s)")
+
+
+def load_traffic(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=100, help="hexes per box edge (6 n^3 tets)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=20)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (PMC byte-counter calibration)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nbytes = args.copy_calibration << 20
+        x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}), flush=True)
+        return
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no inter-GPU path is built); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    import cases as K
+    from typhon_amd import scheme as S
+
+    t0 = time.perf_counter()
+    m, s, g, u = K.vortex_case(args.n, L=10.0)
+    print(f"# mesh: {m.ncell_int} tets, {m.nface} faces (reference form), built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    t64, tm64, st64, info = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_FP64)
+    print(f"# fp64: {t64:.3f} s for {args.steps} steps; kernels {tm64}", file=sys.stderr, flush=True)
+    nc = m.ncell_int
+    nf_dev = info["nface"]
+    nb_dev = info["nbface"]
+    evals = 4 * args.steps
+    value = nc * evals / t64 / 1e6
+    bm = byte_model(nc, nf_dev, nb_dev, mixed=False)
+    # dominant kernel and its roofline
+    kern = "flux" if tm64["flux_ms"] >= tm64["grad_ms"] else "grad"
+    ms_avg = tm64[f"{kern}_ms"] / max(1, tm64[f"{kern}_n"])
+    ach = bm[kern] / (ms_avg * 1e-3) / 1e9
+    traffic = load_traffic(os.path.join(ROOT, "profiles", f"traffic_{kern}_c2_fp64.json"))
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t64 / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered periodic tet mesh, isentropic vortex initial field)",
+        "config": {"workload": f"3D Euler MUSCL (LSQ gradients, Venkatakrishnan K=5) + HLLC + RK4, "
+                               f"{args.n}^3 hexes -> {nc} periodic jittered tets, isentropic vortex",
+                   "cells": nc, "faces": nf_dev, "evals_per_step": 4, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": f"k_{'flux' if kern == 'flux' else 'gradient'}",
+                     "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "algorithmic_bytes_per_launch": int(bm[kern]),
+                     "avg_launch_ms": round(ms_avg, 4)},
+        "kernels_fp64": {"grad_ms_avg": round(tm64["grad_ms"] / max(1, tm64["grad_n"]), 4),
+                         "flux_ms_avg": round(tm64["flux_ms"] / max(1, tm64["flux_n"]), 4),
+                         "dt_ms_avg": round(tm64["dt_ms"] / max(1, tm64["dt_n"]), 4),
+                         "bytes_per_cell_eval": round(bm["total"] / nc, 1)},
+    }
+    if not args.no_mixed:
+        tmx, tmm, stmx, _ = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_MIXED)
+        bmm = byte_model(nc, nf_dev, nb_dev, mixed=True)
+        rho64, rhomx = st64[0], stmx[0]
+        err_l2 = float(np.sqrt(np.mean((rhomx - rho64) ** 2)) / np.sqrt(np.mean(rho64 ** 2)))
+        err_inf = float(np.abs(rhomx - rho64).max() / np.abs(rho64).max())
+        fl = tmm["flux_ms"] / max(1, tmm["flux_n"])
+        out["mixed"] = {"value": round(nc * evals / tmx / 1e6, 3), "unit": "Mcell-updates/s",
+                        "dtype": "fp32 flux/gradient/stage state, fp64 U0/accumulator",
+                        "speedup_vs_fp64": round(t64 / tmx, 3),
+                        "density_rel_err_l2": err_l2, "density_rel_err_linf": err_inf,
+                        "after_steps": args.steps + args.warmup,
+                        "flux_ms_avg": round(fl, 4),
+                        "grad_ms_avg": round(tmm["grad_ms"] / max(1, tmm["grad_n"]), 4),
+                        "flux_achieved_GBs": round(bmm["flux"] / (fl * 1e-3) / 1e9, 1)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

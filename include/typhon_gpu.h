@@ -1,0 +1,140 @@
+/*
+ * typhon_gpu.h -- C-ABI of the MI355X (gfx950) residual path for Typhon.
+ *
+ * Drop-in boundary for the body of calc_rhs and the RK stage loop
+ * (SOURCE/MAIN/calc_rhs.f90:10, SOURCE/MAIN/treelevel_rungekutta.f90:8).  The
+ * caller (Typhon's Fortran driver through iso_c_binding, or the Python host
+ * mirror in typhon_amd/) owns every host array; the library copies what it
+ * needs once and keeps mesh and state resident in HBM.  Plain pointers and
+ * sizes only.  Every function returns 0 on success, a TY_ERR_* code otherwise;
+ * ty_last_error() gives the message (no exceptions cross the ABI).
+ *
+ * Threading: one host thread per context, one HIP stream per context.
+ */
+#ifndef TYPHON_GPU_H
+#define TYPHON_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TY_OK 0
+#define TY_ERR_ARG 1          /* bad argument or call order */
+#define TY_ERR_HIP 2          /* HIP runtime error */
+#define TY_ERR_STATE 3        /* non-positive or NaN internal energy (calc_ns_timestep.f90:89-100) */
+#define TY_ERR_UNSUPPORTED 4  /* scheme/BC combination not provided by this library */
+#define TY_ERR_NOT_CONVERGED 5
+
+#define TY_PREC_FP64 0        /* everything in FP64 */
+#define TY_PREC_MIXED 1       /* FP32 stage state, gradients, fluxes, geometry; FP64 U0/accumulator */
+
+typedef struct ty_ctx ty_ctx;
+
+/* One boundary condition: umesh%boco(ib) (CFDTOOLS/Mesh/USTMESH.f90:31-52) with
+ * defsolver%boco(idef)%typ_boco (SOURCE/MODCOM/VARCOM.f90:75-93) and its
+ * resolved uniform parameters (setboco_ns_*.f90). */
+typedef struct {
+    int32_t type;            /* 10 symmetry, 11 periodic, 31 supersonic inlet, 35 subsonic outlet,
+                                36 supersonic outlet */
+    int32_t nface;
+    const int32_t* iface;    /* umesh%boco(ib)%iface(1:nface), 1-based */
+    const int32_t* ipart;    /* periodic only: boco%gridcon%i_param(1:nface), 1-based partner faces */
+    double param[8];         /* 31: ptot, ttot, mach, dir_x, dir_y, dir_z; 35: pstat */
+} ty_boco_desc;
+
+/* The unstructured mesh as Typhon holds it after init_world (SOURCE/MAIN/main.f90:95):
+ * st_ustmesh (USTMESH.f90:31-52), st_mesh (MESHBASE.f90:44-59). */
+typedef struct {
+    int32_t dim;             /* 2 or 3 */
+    int32_t ncell_int;       /* umesh%ncell_int */
+    int32_t ncell;           /* umesh%ncell (interior + ghost cells) */
+    int32_t nface;           /* umesh%nface */
+    int32_t nface_int;       /* umesh%nface_int */
+    const int32_t* facecell; /* umesh%facecell%fils(1:nface,1:2), column-major, 1-based */
+    const double* face;      /* umesh%mesh%iface(1:nface,1,1): st_face {normale(3), centre(3), surface} */
+    const double* centre;    /* umesh%mesh%centre(1:ncell,1,1): v3d */
+    const double* volume;    /* umesh%mesh%volume(1:ncell,1,1) */
+    int32_t nboco;
+    const ty_boco_desc* boco;
+} ty_mesh_desc;
+
+/* st_espece (SOURCE/EQNS/EQNS.f90:60-70) and the fluid model (def_model_ns.f90) */
+typedef struct {
+    double gamma;
+    double r_const;
+    double prandtl;
+    int32_t typ_visc;        /* 11 Sutherland, 13 constant dynamic, 14 constant kinematic */
+    double visc_dyn, visc_kin, tref, tsuth;
+    int32_t typ_fluid;       /* 10 Euler, 15 laminar Navier-Stokes (EQNS.f90:27-32) */
+} ty_gas;
+
+/* mnu_spat / mnu_time (SOURCE/PARAM/MENU_NUM.f90:155-231) with resolved integer IDs */
+typedef struct {
+    int32_t sch_hyp;         /* 5 Rusanov, 10 Roe, 21 HLLE, 22 HLLEB, 25 HLLC, 26 HLLCB */
+    int32_t sch_dis;         /* 1 none, 5 cell-average, 10 cell-full */
+    char method;             /* 'N' none, 'M' MUSCL, 'F' MUSCL-FAST, 'U' MUSCL-UNS */
+    char limiter;            /* 'X','M','A','V','S','K','L'; 'W' = Venkatakrishnan cell limiter */
+    int32_t gradmeth;        /* 5 none, 10 LSQ, 12 weighted LSQ */
+    int32_t postlimiter;     /* 0 none, 20 Barth, 21 super-Barth */
+    double venkat_k;         /* Venkatakrishnan K (eps^2 = (K h)^3) */
+    int32_t nstage;          /* rk%nstage (1..8) */
+    const double* rk_coef;   /* rk%coef(1:nstage,1:nstage), column-major */
+    double cfl;              /* deftime%stabnb */
+    int32_t precision;       /* TY_PREC_FP64 or TY_PREC_MIXED */
+} ty_scheme;
+
+/* context */
+int ty_create(ty_ctx** ctx, int device);
+int ty_destroy(ty_ctx* ctx);
+const char* ty_last_error(const ty_ctx* ctx);
+
+/* mesh upload once (after init_world); builds the device layout: RCM-ordered
+ * cells, periodic face pairs merged, cell->face slot lists.  Replaces the
+ * mesh side of calc_rhs (calc_rhs.f90:48-145). */
+int ty_mesh_upload(ty_ctx* ctx, const ty_mesh_desc* mesh);
+
+/* model and scheme: the resolved mnu_solver / mnu_spat / mnu_time menus */
+int ty_set_scheme(ty_ctx* ctx, const ty_gas* gas, const ty_scheme* scheme);
+
+/* conservative state of interior cells: etatcons tabscal(1)=rho, tabscal(2)=rhoE,
+ * tabvect(1)=momentum v3d (DEFFIELD.f90:32-52); host arrays in the caller's cell order */
+int ty_state_set(ty_ctx* ctx, const double* rho, const double* rhoE, const double* mom);
+int ty_state_get(ty_ctx* ctx, double* rho, double* rhoE, double* mom);
+
+/* global time step: calc_zonetimestep with stab_cond (ZONE/calc_zonetimestep.f90:57-126,
+ * EQNS/calc_ns_timestep.f90:52-102): dt = min(min_c CFL 2V/sum(A)/(|u|+a), dtmax).
+ * Returns TY_ERR_STATE on a non-positive or NaN sound speed. */
+int ty_timestep(ty_ctx* ctx, double dtmax, double* dt);
+
+/* one residual evaluation, the reference convention -dt/V sum_f F.A on interior cells
+ * (calc_rhs.f90:10 + flux_to_res.f90:126-142) for the current state; outputs in the
+ * caller's cell order (res_mom is v3d) */
+int ty_residual(ty_ctx* ctx, double dt, double* res_rho, double* res_rhoE, double* res_mom);
+
+/* one explicit RK step with a given global dt: treelevel_rungekutta
+ * (MAIN/treelevel_rungekutta.f90:50-113) then update_field (ZONE/update_field.f90:38) */
+int ty_rk_step(ty_ctx* ctx, double dt);
+
+/* nsteps device-resident steps, each: time step on the device (min with dtmax) then
+ * the RK step, no host synchronisation in between (integration_cyclezone.f90:80-151).
+ * dt_last, if not NULL, receives the last step's dt (synchronises). */
+int ty_advance(ty_ctx* ctx, int32_t nsteps, double dtmax, double* dt_last);
+
+/* wait for queued work; reports a bad state found by device-side time steps */
+int ty_sync(ty_ctx* ctx);
+
+/* per-kernel timing for benchmarks (no reference counterpart): when enabled, every
+ * gradient, flux and time-step launch is bracketed by HIP events on the context's
+ * stream.  Returns the summed milliseconds and launch counts recorded since the last
+ * call (ms[3], count[3]: gradient, flux, time step) and sets the enable flag. */
+int ty_timing(ty_ctx* ctx, int32_t enable, double* ms, int32_t* count);
+
+/* layout facts for the caller / harness: number of slots per cell, merged face count */
+int ty_mesh_info(const ty_ctx* ctx, int32_t* ncell, int32_t* nface_dev, int32_t* nslot,
+                 int32_t* nbface_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,968 @@
+/*
+ * typhon_oracle.c -- CPU restatement (FP64, single thread) of Typhon's
+ * Euler/NS residual + explicit RK path.  TEST INFRASTRUCTURE ONLY: the parity
+ * checker of the HIP path, never linked into the product.
+ *
+ * Each function cites the reference routine it restates (paths relative to the
+ * reference root).  Arithmetic follows the reference's operation order where
+ * that is cheap to do; the face->cell scatter runs in face order instead of the
+ * reference's greedy colour order (SOURCE/MAIN/flux_to_res.f90:90-113), which
+ * changes sums only at the 1e-16 relative level.
+ *
+ * min/max follow the semantics AMD flang gives the Fortran intrinsics
+ * (MAX(a,b) = a>b ? a : b, MIN(a,b) = a<b ? a : b); this matters for the NaN
+ * ratios of postlimit_barth.
+ */
+#include "typhon_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define FMAX(a, b) ((a) > (b) ? (a) : (b))
+#define FMIN(a, b) ((a) < (b) ? (a) : (b))
+#define NV 5
+
+struct or_ctx {
+    or_config c;
+    int nc, ni, nf, nfi;
+    double* cons;  /* [5][ncell] */
+    double* prim;  /* [5][ncell] rho p u v w */
+    double* grad;  /* [15][ncell]: (var*3+dir) */
+    double* qL;    /* [5][nface] */
+    double* qR;
+    double* flux;  /* [5][nface] */
+    double* res;   /* [5][ncell] */
+    double* lsq;   /* [9][ncell_int] cholesky lower part, column-major 3x3 */
+    int lsq_done;
+    double* dtloc; /* [ncell] */
+    double* ratio; /* [ncell] */
+};
+
+#define CONS(o, v, i) ((o)->cons[(size_t)(v) * (o)->nc + (i)])
+#define PRIM(o, v, i) ((o)->prim[(size_t)(v) * (o)->nc + (i)])
+#define GRAD(o, v, d, i) ((o)->grad[(size_t)((v) * 3 + (d)) * (o)->nc + (i)])
+#define QL(o, v, f) ((o)->qL[(size_t)(v) * (o)->nf + (f)])
+#define QR(o, v, f) ((o)->qR[(size_t)(v) * (o)->nf + (f)])
+#define FLX(o, v, f) ((o)->flux[(size_t)(v) * (o)->nf + (f)])
+#define RES(o, v, i) ((o)->res[(size_t)(v) * (o)->nc + (i)])
+
+static double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+or_ctx* or_create(const or_config* cfg)
+{
+    or_ctx* o = (or_ctx*)calloc(1, sizeof(or_ctx));
+    o->c = *cfg;
+    o->nc = cfg->ncell;
+    o->ni = cfg->ncell_int;
+    o->nf = cfg->nface;
+    o->nfi = cfg->nface_int;
+    o->cons = (double*)calloc((size_t)NV * o->nc, sizeof(double));
+    o->prim = (double*)calloc((size_t)NV * o->nc, sizeof(double));
+    o->grad = (double*)calloc((size_t)15 * o->nc, sizeof(double));
+    o->qL = (double*)calloc((size_t)NV * o->nf, sizeof(double));
+    o->qR = (double*)calloc((size_t)NV * o->nf, sizeof(double));
+    o->flux = (double*)calloc((size_t)NV * o->nf, sizeof(double));
+    o->res = (double*)calloc((size_t)NV * o->nc, sizeof(double));
+    o->lsq = (double*)calloc((size_t)9 * (o->ni > 0 ? o->ni : 1), sizeof(double));
+    o->dtloc = (double*)calloc((size_t)o->nc, sizeof(double));
+    o->ratio = (double*)calloc((size_t)o->nc, sizeof(double));
+    /* ghost cells start as a copy of a neutral state so that calc_varprim on them
+       stays finite (the reference computes it on ghosts too, calc_varprim_ns.f90:37) */
+    for (int i = 0; i < o->nc; ++i) { CONS(o, 0, i) = 1.0; CONS(o, 1, i) = 1.0; }
+    return o;
+}
+
+void or_destroy(or_ctx* o)
+{
+    if (!o) return;
+    free(o->cons); free(o->prim); free(o->grad); free(o->qL); free(o->qR);
+    free(o->flux); free(o->res); free(o->lsq); free(o->dtloc); free(o->ratio);
+    free(o);
+}
+
+void or_set_cons(or_ctx* o, const double* cons)
+{
+    for (int v = 0; v < NV; ++v)
+        memcpy(&CONS(o, v, 0), cons + (size_t)v * o->ni, sizeof(double) * o->ni);
+}
+
+void or_get_cons(const or_ctx* o, double* cons)
+{
+    for (int v = 0; v < NV; ++v)
+        memcpy(cons + (size_t)v * o->ni, &o->cons[(size_t)v * o->nc], sizeof(double) * o->ni);
+}
+
+const double* or_prim(const or_ctx* o) { return o->prim; }
+const double* or_grad(const or_ctx* o) { return o->grad; }
+const double* or_face_left(const or_ctx* o) { return o->qL; }
+const double* or_face_right(const or_ctx* o) { return o->qR; }
+const double* or_flux(const or_ctx* o) { return o->flux; }
+
+/* ---------------------------------------------------------------------------
+ * A2: calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:37-57), all cells */
+static void varprim(or_ctx* o)
+{
+    double g1 = o->c.gamma - 1.0;
+    for (int i = 0; i < o->nc; ++i) {
+        double rho = CONS(o, 0, i);
+        double u = CONS(o, 2, i) / rho, v = CONS(o, 3, i) / rho, w = CONS(o, 4, i) / rho;
+        double ec = .5 * rho * (u * u + v * v + w * w);
+        PRIM(o, 0, i) = rho;
+        PRIM(o, 1, i) = g1 * (CONS(o, 1, i) - ec);
+        PRIM(o, 2, i) = u; PRIM(o, 3, i) = v; PRIM(o, 4, i) = w;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * A3: physical boundary states.  q[5] = (rho, p, u, v, w) from the interior
+ * (or left face) state to the ghost (or right face) state. */
+
+/* pi_ti_mach_dir2nspri (SOURCE/EQNS/EQNS.f90:139-160) with dir/|dir|
+   (setboco_ns_inlet_sup.f90:104) */
+static void bc_inlet_sup(const or_config* c, const double* prm, double* q)
+{
+    double g1 = c->gamma - 1.0;
+    double mach = prm[2];
+    double dx = prm[3], dy = prm[4], dz = prm[5];
+    double dn = sqrt(dx * dx + dy * dy + dz * dz);
+    dx /= dn; dy /= dn; dz /= dn;
+    double fm = 1.0 / (1.0 + .5 * g1 * mach * mach);
+    double ts = prm[1] * fm;
+    double a = sqrt(c->gamma * c->rgas * ts);
+    double p = prm[0] * pow(fm, c->gamma / g1);
+    double rho = p / (c->rgas * ts);
+    double ma = mach * a;
+    q[0] = rho; q[1] = p; q[2] = ma * dx; q[3] = ma * dy; q[4] = ma * dz;
+}
+
+/* setboco_ns_outlet_sub.f90:57-95: nspri2pi_ti_mach_dir (EQNS.f90:197-219) of the
+   interior state, then pi_ti_ps_dir2nspri (EQNS.f90:166-191) with imposed ps */
+static void bc_outlet_sub(const or_config* c, const double* prm, const double* qi, double* q)
+{
+    double g = c->gamma, g1 = g - 1.0, r = c->rgas;
+    double rho = qi[0], p = qi[1], u = qi[2], v = qi[3], w = qi[4];
+    double fmv = u * u + v * v + w * w;
+    double sq = sqrt(fmv);
+    double dx = u / sq, dy = v / sq, dz = w / sq;
+    fmv = fmv / (g * p) * rho;
+    fmv = 1.0 + 0.5 * g1 * fmv;
+    double pi = p * pow(fmv, g / g1);
+    double ti = p / r / rho * fmv;
+    double ps = prm[0];
+    double fm = pow(pi / ps, g1 / g);
+    if (fm < 1.0) fm = 1.0;
+    double rhoo = ps / (r * ti) * fm;
+    double vm = sqrt((fm - 1.0) * 2.0 / g1) * sqrt(g * ps / rhoo);
+    q[0] = rhoo; q[1] = ps; q[2] = vm * dx; q[3] = vm * dy; q[4] = vm * dz;
+}
+
+/* calcboco_ust_sym.f90:46-79 (wall velocity zero): scalars copied, velocity mirrored */
+static void bc_sym(const double* n, const double* qi, double* q)
+{
+    double vn = qi[2] * n[0] + qi[3] * n[1] + qi[4] * n[2];
+    double k = 2.0 * vn;
+    q[0] = qi[0]; q[1] = qi[1];
+    q[2] = qi[2] - k * n[0]; q[3] = qi[3] - k * n[1]; q[4] = qi[4] - k * n[2];
+}
+
+static void bc_state(const or_ctx* o, const or_boco* b, int f, const double* qi, double* q)
+{
+    switch (b->type) {
+    case OR_BC_SYM: bc_sym(o->c.fnorm + 3 * (size_t)f, qi, q); break;
+    case OR_BC_INLET_SUP: bc_inlet_sup(&o->c, b->param, q); break;
+    case OR_BC_OUTLET_SUB: bc_outlet_sub(&o->c, b->param, qi, q); break;
+    case OR_BC_OUTLET_SUP:
+    default:
+        for (int v = 0; v < NV; ++v) q[v] = qi[v];
+        break;
+    }
+}
+
+/* calcboco_gen mode bccon_cell_state (calcboco_gen.f90:46-88, calcboco_ust.f90:44-105,
+   periodic copy calcboco_connect_per_match.f90:61-84) */
+static void boco_cell_state(or_ctx* o)
+{
+    double qi[NV], q[NV];
+    for (int ib = 0; ib < o->c.nboco; ++ib) {
+        const or_boco* b = &o->c.boco[ib];
+        for (int k = 0; k < b->nface; ++k) {
+            int f = b->iface[k];
+            int ig = o->c.ic2[f];
+            int is = (b->type == OR_BC_PERIOD) ? o->c.ic1[b->ipart[k]] : o->c.ic1[f];
+            for (int v = 0; v < NV; ++v) qi[v] = PRIM(o, v, is);
+            if (b->type == OR_BC_PERIOD) { for (int v = 0; v < NV; ++v) q[v] = qi[v]; }
+            else bc_state(o, b, f, qi, q);
+            for (int v = 0; v < NV; ++v) PRIM(o, v, ig) = q[v];
+        }
+    }
+}
+
+/* bccon_face_state: right face state from left face state (calc_rhs.f90:109) */
+static void boco_face_state(or_ctx* o)
+{
+    double qi[NV], q[NV];
+    for (int ib = 0; ib < o->c.nboco; ++ib) {
+        const or_boco* b = &o->c.boco[ib];
+        for (int k = 0; k < b->nface; ++k) {
+            int f = b->iface[k];
+            int is = (b->type == OR_BC_PERIOD) ? b->ipart[k] : f;
+            for (int v = 0; v < NV; ++v) qi[v] = QL(o, v, is);
+            if (b->type == OR_BC_PERIOD) { for (int v = 0; v < NV; ++v) q[v] = qi[v]; }
+            else bc_state(o, b, f, qi, q);
+            for (int v = 0; v < NV; ++v) QR(o, v, f) = q[v];
+        }
+    }
+}
+
+/* bccon_cell_grad: ghost gradients.  Symmetry mirrors (calcboco_ust_sym.f90:65-77:
+   vectors v-2(v.n)n, tensor S.T.S with S=I-2nn^T, VEC3D.f90:679-694); the NS
+   inlet/outlet routines copy (setboco_ns_inlet_sup.f90:140-149); periodic copies. */
+static void boco_cell_grad(or_ctx* o)
+{
+    for (int ib = 0; ib < o->c.nboco; ++ib) {
+        const or_boco* b = &o->c.boco[ib];
+        for (int k = 0; k < b->nface; ++k) {
+            int f = b->iface[k];
+            int ig = o->c.ic2[f];
+            int is = (b->type == OR_BC_PERIOD) ? o->c.ic1[b->ipart[k]] : o->c.ic1[f];
+            if (b->type == OR_BC_SYM) {
+                const double* n = o->c.fnorm + 3 * (size_t)f;
+                for (int v = 0; v < 2; ++v) {
+                    double g[3] = {GRAD(o, v, 0, is), GRAD(o, v, 1, is), GRAD(o, v, 2, is)};
+                    double k2 = 2.0 * (g[0] * n[0] + g[1] * n[1] + g[2] * n[2]);
+                    for (int d = 0; d < 3; ++d) GRAD(o, v, d, ig) = g[d] - k2 * n[d];
+                }
+                double S[3][3], T[3][3], M[3][3];
+                S[0][0] = 1.0 - 2.0 * n[0] * n[0]; S[1][1] = 1.0 - 2.0 * n[1] * n[1];
+                S[2][2] = 1.0 - 2.0 * n[2] * n[2];
+                S[0][1] = -2.0 * n[0] * n[1]; S[0][2] = -2.0 * n[0] * n[2];
+                S[1][0] = S[0][1]; S[1][2] = -2.0 * n[1] * n[2];
+                S[2][0] = S[0][2]; S[2][1] = S[1][2];
+                for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) T[i][j] = GRAD(o, 2 + i, j, is);
+                for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+                    double s = 0; for (int l = 0; l < 3; ++l) s += T[i][l] * S[l][j]; M[i][j] = s; }
+                for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+                    double s = 0; for (int l = 0; l < 3; ++l) s += S[i][l] * M[l][j];
+                    GRAD(o, 2 + i, j, ig) = s; }
+            } else {
+                for (int v = 0; v < 15; ++v) o->grad[(size_t)v * o->nc + ig] = o->grad[(size_t)v * o->nc + is];
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * A4: precalc_grad_lsq (SOURCE/MGRID/precalc_grad_lsq.f90:60-155) and
+ * cholesky_decomp (SOURCE/MATH/MATRIX.f90:42-70) */
+static void face_dcg(const or_ctx* o, int f, double* d)
+{
+    const double* c1 = o->c.ccent + 3 * (size_t)o->c.ic1[f];
+    const double* c2 = o->c.ccent + 3 * (size_t)o->c.ic2[f];
+    d[0] = c2[0] - c1[0]; d[1] = c2[1] - c1[1]; d[2] = c2[2] - c1[2];
+    if (o->c.gradmeth == OR_GRAD_LSQW) {
+        double s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        d[0] /= s; d[1] /= s; d[2] /= s;
+    }
+}
+
+static void cholesky_decomp(double* a /* 3x3 column-major a[i+3j] */)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j) {
+            double sum = a[i + 3 * j];
+            for (int k = i - 1; k >= 0; --k) sum = sum - a[i + 3 * k] * a[j + 3 * k];
+            if (i == j) a[i + 3 * i] = sqrt(sum);
+            else a[j + 3 * i] = sum / a[i + 3 * i];
+        }
+}
+
+static void precalc_lsq(or_ctx* o)
+{
+    memset(o->lsq, 0, sizeof(double) * 9 * (size_t)o->ni);
+    for (int f = 0; f < o->nf; ++f) {
+        double d[3]; face_dcg(o, f, d);
+        double m[9];
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m[i + 3 * j] = d[i] * d[j];
+        m[0] = d[0] * d[0]; m[4] = d[1] * d[1]; m[8] = d[2] * d[2];
+        int c1 = o->c.ic1[f];
+        for (int k = 0; k < 9; ++k) o->lsq[9 * (size_t)c1 + k] += m[k];
+        if (f < o->nfi) {
+            int c2 = o->c.ic2[f];
+            for (int k = 0; k < 9; ++k) o->lsq[9 * (size_t)c2 + k] += m[k];
+        }
+    }
+    if (o->c.dim == 2)
+        for (int i = 0; i < o->ni; ++i) o->lsq[9 * (size_t)i + 8] += 1.0;
+    for (int i = 0; i < o->ni; ++i) cholesky_decomp(o->lsq + 9 * (size_t)i);
+    o->lsq_done = 1;
+}
+
+/* A5: calc_gradient (SOURCE/MGRID/calc_gradient.f90:75-192) + cholesky_solve
+   (MATRIX.f90:79-111).  Order of variables: rho, p, then velocity components
+   (tensor rows mat(i,1:3) = grad u_i). */
+static void calc_gradient(or_ctx* o)
+{
+    if (!o->lsq_done) precalc_lsq(o);
+    int ni = o->ni;
+    double* rhs = (double*)calloc((size_t)15 * ni, sizeof(double)); /* [cell][var][3] */
+    for (int f = 0; f < o->nf; ++f) {
+        double d[3]; face_dcg(o, f, d);
+        int c1 = o->c.ic1[f], c2 = o->c.ic2[f];
+        for (int v = 0; v < NV; ++v) {
+            double ds = PRIM(o, v, c2) - PRIM(o, v, c1);
+            double* r1 = rhs + ((size_t)c1 * NV + v) * 3;
+            r1[0] += ds * d[0]; r1[1] += ds * d[1]; r1[2] += ds * d[2];
+            if (f < o->nfi) {
+                double* r2 = rhs + ((size_t)c2 * NV + v) * 3;
+                r2[0] += ds * d[0]; r2[1] += ds * d[1]; r2[2] += ds * d[2];
+            }
+        }
+    }
+    for (int i = 0; i < ni; ++i) {
+        const double* a = o->lsq + 9 * (size_t)i;
+        for (int v = 0; v < NV; ++v) {
+            double* b = rhs + ((size_t)i * NV + v) * 3;
+            for (int r = 0; r < 3; ++r) {
+                for (int j = 0; j < r; ++j) b[r] = b[r] - a[r + 3 * j] * b[j];
+                b[r] = b[r] / a[r + 3 * r];
+            }
+            for (int r = 2; r >= 0; --r) {
+                for (int j = r + 1; j < 3; ++j) b[r] = b[r] - a[j + 3 * r] * b[j];
+                b[r] = b[r] / a[r + 3 * r];
+            }
+            for (int d = 0; d < 3; ++d) GRAD(o, v, d, i) = b[d];
+        }
+    }
+    free(rhs);
+}
+
+/* ---------------------------------------------------------------------------
+ * A6: limiters (SOURCE/MGRID/LIMITER.f90:72-339) */
+static double sgn1(double x) { return copysign(1.0, x); }
+
+double or_limiter(char lim, double x1, double x2)
+{
+    double r = 0.0;
+    switch (lim) {
+    case 'M': { /* minmod :72 */
+        double s = .5 * (sgn1(x1) + sgn1(x2));
+        r = s * FMIN(fabs(x1), fabs(x2));
+        break; }
+    case 'A': { /* albada :122 */
+        double s = x1 * x2;
+        r = (s <= 0.0) ? 0.0 : s * (x1 + x2) / (x1 * x1 + x2 * x2);
+        break; }
+    case 'V': { /* vleer :175 */
+        double s = x1 * x2;
+        r = (s <= 0.0) ? 0.0 : 2.0 * s / (x1 + x2);
+        break; }
+    case 'S': /* superbee :227 */
+        if (x1 > 0.0 && x2 > 0.0) r = FMAX(FMIN(2.0 * x1, x2), FMIN(2.0 * x2, x1));
+        if (x1 < 0.0 && x2 < 0.0) r = FMIN(FMAX(2.0 * x1, x2), FMAX(2.0 * x2, x1));
+        break;
+    case 'K': /* kim3 :277 */
+        if (x1 > 0.0 && x2 > 0.0) { r = (2.0 * x1 + x2) / 3.0; r = FMIN(FMIN(2.0 * x1, 2.0 * x2), r); }
+        if (x1 < 0.0 && x2 < 0.0) { r = (2.0 * x1 + x2) / 3.0; r = FMAX(FMAX(2.0 * x1, 2.0 * x2), r); }
+        break;
+    case 'L': { /* lim03 :339 */
+        double eta = (x1 * x1 + x2 * x2) / 0.01 / 0.01;
+        if (eta < 1.0) r = (2.0 * x2 + x1) / 3.0;
+        else if (x1 > 0.0 && x2 > 0.0) { r = (2.0 * x2 + x1) / 3.0; r = FMIN(FMIN(1.6 * x2, 2.0 * x1), r); }
+        else if (x1 < 0.0 && x2 < 0.0) { r = (2.0 * x2 + x1) / 3.0; r = FMAX(FMAX(1.6 * x2, 2.0 * x1), r); }
+        else if (x1 < 0.0 && x2 > 0.0) { r = (2.0 * x2 + x1) / 3.0; r = FMAX(0.0, FMIN(-0.5 * x1, r)); }
+        else if (x1 > 0.0 && x2 < 0.0) { r = (2.0 * x2 + x1) / 3.0; r = FMIN(0.0, FMAX(-0.5 * x1, r)); }
+        break; }
+    default: r = 0.0;
+    }
+    return r;
+}
+
+/* slope correction shared by tvdgradstr (MUSCL, :35-58) and tvdgraduns (MUSCL-UNS):
+   returns the increment added to the projected slope s. */
+static double muscl_corr(char method, char lim, double s, double lr)
+{
+    if (lim == 'X') return lr - s;
+    if (method == 'M') {
+        if (lim == 'K') return or_limiter(lim, lr, 2.0 * s - lr) - s;
+        return or_limiter(lim, 2.0 * s - lr, lr) - s;
+    }
+    /* MUSCL-UNS (tvdgraduns_scal.f90:35-57) */
+    if (lim == 'K') return or_limiter(lim, lr, s) - s;
+    return or_limiter(lim, s, lr) - s;
+}
+
+/* MUSCL-FAST slope (tvdgradfst_scal.f90:35-60): limited slope along LR */
+static double fast_slope(char lim, double s, double lr)
+{
+    if (lim == 'X') return lr;
+    if (lim == 'K') return or_limiter(lim, lr, 2.0 * s - lr);
+    return or_limiter(lim, 2.0 * s - lr, lr);
+}
+
+/* calc_hres_states (SOURCE/MAIN/calc_hres_states.f90:40-96) -> hres_ns_muscl
+   (hres_ns_muscl.f90:51-143), hres_ns_musclfast (:51-144), hres_ns_muscluns, or
+   distrib_field for first order; all faces, ghost cells on the right side. */
+static void hres_states(or_ctx* o)
+{
+    const or_config* c = &o->c;
+    for (int f = 0; f < o->nf; ++f) {
+        int cl = c->ic1[f], cr = c->ic2[f];
+        if (c->method == 'N') {
+            for (int v = 0; v < NV; ++v) { QL(o, v, f) = PRIM(o, v, cl); QR(o, v, f) = PRIM(o, v, cr); }
+            continue;
+        }
+        const double* xl = c->ccent + 3 * (size_t)cl;
+        const double* xr = c->ccent + 3 * (size_t)cr;
+        const double* xf = c->fcent + 3 * (size_t)f;
+        double u[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+        double LF[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+        double RF[3] = {xf[0] - xr[0], xf[1] - xr[1], xf[2] - xr[2]};
+        if (c->method == 'F') {
+            double kl = sqrt(LF[0] * LF[0] + LF[1] * LF[1] + LF[2] * LF[2]);
+            double kr = -sqrt(RF[0] * RF[0] + RF[1] * RF[1] + RF[2] * RF[2]);
+            double dlr = kl - kr;
+            u[0] /= dlr; u[1] /= dlr; u[2] /= dlr;
+            for (int v = 0; v < NV; ++v) {
+                double lr = (PRIM(o, v, cr) - PRIM(o, v, cl)) / dlr;
+                double gl[3] = {GRAD(o, v, 0, cl), GRAD(o, v, 1, cl), GRAD(o, v, 2, cl)};
+                double gr[3] = {GRAD(o, v, 0, cr), GRAD(o, v, 1, cr), GRAD(o, v, 2, cr)};
+                double sl = (c->limiter == 'X') ? 0.0 : dot3(gl, u);
+                double sr = (c->limiter == 'X') ? 0.0 : dot3(gr, u);
+                QL(o, v, f) = PRIM(o, v, cl) + fast_slope(c->limiter, sl, lr) * kl;
+                QR(o, v, f) = PRIM(o, v, cr) + fast_slope(c->limiter, sr, lr) * kr;
+            }
+        } else { /* 'M' or 'U' */
+            double dlr = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            u[0] /= dlr; u[1] /= dlr; u[2] /= dlr;
+            for (int v = 0; v < NV; ++v) {
+                double lr = (PRIM(o, v, cr) - PRIM(o, v, cl)) / dlr;
+                for (int side = 0; side < 2; ++side) {
+                    int ci = side ? cr : cl;
+                    const double* cf = side ? RF : LF;
+                    double g[3] = {GRAD(o, v, 0, ci), GRAD(o, v, 1, ci), GRAD(o, v, 2, ci)};
+                    double s = dot3(g, u);
+                    double ds = muscl_corr(c->method, c->limiter, s, lr);
+                    g[0] = g[0] + ds * u[0]; g[1] = g[1] + ds * u[1]; g[2] = g[2] + ds * u[2];
+                    double dq = dot3(g, cf);
+                    if (side) QR(o, v, f) = PRIM(o, v, cr) + dq;
+                    else QL(o, v, f) = PRIM(o, v, cl) + dq;
+                }
+            }
+        }
+    }
+}
+
+/* A7: postlimit_barth (SOURCE/MGRID/postlimit_barth.f90:50-223), face order,
+   separate ratios for rho, p and the velocity vector (all three components). */
+static void postlimit_barth(or_ctx* o, double klim)
+{
+    const or_config* c = &o->c;
+    static const int groups[3][2] = {{0, 1}, {1, 2}, {2, 5}};
+    for (int g = 0; g < 3; ++g) {
+        for (int i = 0; i < o->nc; ++i) o->ratio[i] = 1.0;
+        for (int f = 0; f < o->nf; ++f) {
+            int cl = c->ic1[f], cr = c->ic2[f];
+            for (int v = groups[g][0]; v < groups[g][1]; ++v) {
+                double lr = PRIM(o, v, cr) - PRIM(o, v, cl);
+                double l = lr / (QL(o, v, f) - PRIM(o, v, cl));
+                l = FMAX(klim * l, 0.0);
+                o->ratio[cl] = FMIN(o->ratio[cl], l);
+            }
+            for (int v = groups[g][0]; v < groups[g][1]; ++v) {
+                double lr = PRIM(o, v, cr) - PRIM(o, v, cl);
+                double l = -lr / (QR(o, v, f) - PRIM(o, v, cr));
+                l = FMAX(klim * l, 0.0);
+                o->ratio[cr] = FMIN(o->ratio[cr], l);
+            }
+        }
+        for (int f = 0; f < o->nf; ++f) {
+            int cl = c->ic1[f], cr = c->ic2[f];
+            for (int v = groups[g][0]; v < groups[g][1]; ++v) {
+                QL(o, v, f) = PRIM(o, v, cl) + o->ratio[cl] * (QL(o, v, f) - PRIM(o, v, cl));
+                QR(o, v, f) = PRIM(o, v, cr) + o->ratio[cr] * (QR(o, v, f) - PRIM(o, v, cr));
+            }
+        }
+    }
+}
+
+/* A8: Venkatakrishnan limiter (no reference counterpart; Venkatakrishnan, J. Comput.
+   Phys. 118 (1995) 120-130, eq. 4.4 smooth min with eps^2 = (K h)^3, h = V^(1/dim)).
+   Written as phi = [(dm^2+eps2) + 2 d2 dm] / [dm^2 + 2 d2^2 + dm d2 + eps2]
+   where dm = dmax (d2 > 0) or dmin (d2 < 0); phi = 1 when d2 == 0. */
+double or_venkat_phi(double dmax, double dmin, double d2, double eps2)
+{
+    if (d2 == 0.0) return 1.0;
+    double dm = (d2 > 0.0) ? dmax : dmin;
+    double num = (dm * dm + eps2) + 2.0 * d2 * dm;
+    double den = dm * dm + 2.0 * d2 * d2 + dm * d2 + eps2;
+    if (den <= 0.0) return 1.0;
+    double phi = num / den;
+    return FMIN(1.0, FMAX(0.0, phi));
+}
+
+/* Venkatakrishnan cell limiter applied to the LSQ gradient, then a plain
+   unlimited MUSCL face extrapolation q_f = q_c + phi grad(q).(x_f - x_c). */
+static void venkat_states(or_ctx* o)
+{
+    const or_config* c = &o->c;
+    int nc = o->nc;
+    double* qmax = (double*)malloc(sizeof(double) * NV * (size_t)nc);
+    double* qmin = (double*)malloc(sizeof(double) * NV * (size_t)nc);
+    double* phi = (double*)malloc(sizeof(double) * NV * (size_t)nc);
+    for (int v = 0; v < NV; ++v)
+        for (int i = 0; i < nc; ++i) {
+            qmax[(size_t)v * nc + i] = PRIM(o, v, i);
+            qmin[(size_t)v * nc + i] = PRIM(o, v, i);
+            phi[(size_t)v * nc + i] = 1.0;
+        }
+    for (int f = 0; f < o->nf; ++f) {
+        int cl = c->ic1[f], cr = c->ic2[f];
+        for (int v = 0; v < NV; ++v) {
+            double a = PRIM(o, v, cl), b = PRIM(o, v, cr);
+            size_t il = (size_t)v * nc + cl, ir = (size_t)v * nc + cr;
+            qmax[il] = FMAX(qmax[il], b); qmin[il] = FMIN(qmin[il], b);
+            if (f < o->nfi) { qmax[ir] = FMAX(qmax[ir], a); qmin[ir] = FMIN(qmin[ir], a); }
+        }
+    }
+    double k3 = c->venkat_k * c->venkat_k * c->venkat_k;
+    for (int f = 0; f < o->nf; ++f) {
+        for (int side = 0; side < 2; ++side) {
+            if (side == 1 && f >= o->nfi) continue;
+            int ci = side ? c->ic2[f] : c->ic1[f];
+            const double* xc = c->ccent + 3 * (size_t)ci;
+            const double* xf = c->fcent + 3 * (size_t)f;
+            double r[3] = {xf[0] - xc[0], xf[1] - xc[1], xf[2] - xc[2]};
+            double h = (c->dim == 2) ? sqrt(c->cvol[ci]) : cbrt(c->cvol[ci]);
+            double eps2 = k3 * h * h * h;
+            for (int v = 0; v < NV; ++v) {
+                double g[3] = {GRAD(o, v, 0, ci), GRAD(o, v, 1, ci), GRAD(o, v, 2, ci)};
+                double d2 = dot3(g, r);
+                size_t ii = (size_t)v * nc + ci;
+                double p = or_venkat_phi(qmax[ii] - PRIM(o, v, ci), qmin[ii] - PRIM(o, v, ci), d2, eps2);
+                phi[ii] = FMIN(phi[ii], p);
+            }
+        }
+    }
+    for (int f = 0; f < o->nf; ++f) {
+        int cl = c->ic1[f], cr = c->ic2[f];
+        for (int side = 0; side < 2; ++side) {
+            int ci = side ? cr : cl;
+            const double* xc = c->ccent + 3 * (size_t)ci;
+            const double* xf = c->fcent + 3 * (size_t)f;
+            double r[3] = {xf[0] - xc[0], xf[1] - xc[1], xf[2] - xc[2]};
+            for (int v = 0; v < NV; ++v) {
+                double g[3] = {GRAD(o, v, 0, ci), GRAD(o, v, 1, ci), GRAD(o, v, 2, ci)};
+                double q = PRIM(o, v, ci) + phi[(size_t)v * nc + ci] * dot3(g, r);
+                if (side) QR(o, v, f) = q; else QL(o, v, f) = q;
+            }
+        }
+    }
+    free(qmax); free(qmin); free(phi);
+}
+
+/* ---------------------------------------------------------------------------
+ * A9: numerical fluxes per unit area.  q = (rho, p, u, v, w). */
+static void roe_state(double g, const double* qL, const double* qR, double* rho, double* vel, double* p)
+{
+    /* calc_roe_states (SOURCE/EQNS/calc_roe_states.f90:35-53) */
+    double gig1 = g / (g - 1.0);
+    double t1 = sqrt(qR[0] / qL[0]);
+    *rho = qL[0] * t1;
+    double kl = 1.0 / (1.0 + t1);
+    double kr = t1 * kl;
+    for (int d = 0; d < 3; ++d) vel[d] = kl * qL[2 + d] + kr * qR[2 + d];
+    double hl = gig1 * qL[1] / qL[0] + .5 * (qL[2] * qL[2] + qL[3] * qL[3] + qL[4] * qL[4]);
+    double hr = gig1 * qR[1] / qR[0] + .5 * (qR[2] * qR[2] + qR[3] * qR[3] + qR[4] * qR[4]);
+    double h = kl * hl + kr * hr;
+    *p = (h - .5 * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2])) * (*rho) / gig1;
+}
+
+void or_flux_face(const or_config* c, const double* qL, const double* qR, const double* n, double* F)
+{
+    double g = c->gamma, ig1 = 1.0 / (g - 1.0);
+    double vnl = qL[2] * n[0] + qL[3] * n[1] + qL[4] * n[2];
+    double vnr = qR[2] * n[0] + qR[3] * n[1] + qR[4] * n[2];
+    double rel = ig1 * qL[1] + .5 * qL[0] * (qL[2] * qL[2] + qL[3] * qL[3] + qL[4] * qL[4]);
+    double rer = ig1 * qR[1] + .5 * qR[0] * (qR[2] * qR[2] + qR[3] * qR[3] + qR[4] * qR[4]);
+    switch (c->sch_hyp) {
+    case OR_SCH_RUSANOV: { /* calc_flux_rusanov.f90:64-83 */
+        double al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
+        double ray = FMAX(fabs(vnl) + al, fabs(vnr) + ar);
+        F[0] = .5 * ((vnl + ray) * qL[0] + (vnr - ray) * qR[0]);
+        F[1] = .5 * ((vnl + ray) * rel + (vnr - ray) * rer + vnl * qL[1] + vnr * qR[1]);
+        for (int d = 0; d < 3; ++d)
+            F[2 + d] = .5 * (((vnl + ray) * qL[0]) * qL[2 + d] + ((vnr - ray) * qR[0]) * qR[2 + d]
+                             + (qL[1] + qR[1]) * n[d]);
+        return; }
+    case OR_SCH_HLLE: case OR_SCH_HLLEB: { /* calc_flux_hlle.f90:109-143 */
+        double beta = (c->sch_hyp == OR_SCH_HLLE) ? 1.0 : sqrt((g - 1.0) / (2 * g));
+        double al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
+        double rrho, rv[3], rp; roe_state(g, qL, qR, &rrho, rv, &rp);
+        double vm = rv[0] * n[0] + rv[1] * n[1] + rv[2] * n[2];
+        double am = sqrt(g * rp / rrho);
+        double sl = FMIN(FMIN(0.0, vnl - beta * al), vm - am);
+        double sr = FMAX(FMAX(0.0, vnr + beta * ar), vm + am);
+        double iks = 1.0 / (sr - sl);
+        double kl = sr * iks, kr = -sl * iks, ku = sl * sr * iks;
+        F[0] = (kl * vnl - ku) * qL[0] + (kr * vnr + ku) * qR[0];
+        F[1] = (kl * vnl - ku) * rel + (kr * vnr + ku) * rer + (kl * vnl * qL[1] + kr * vnr * qR[1]);
+        for (int d = 0; d < 3; ++d)
+            F[2 + d] = ((kl * vnl - ku) * qL[0]) * qL[2 + d] + ((kr * vnr + ku) * qR[0]) * qR[2 + d]
+                       + (kl * qL[1] + kr * qR[1]) * n[d];
+        return; }
+    case OR_SCH_ROE: { /* textbook Roe with Harten entropy fix (no reference counterpart:
+                          calc_flux_inviscid.f90:77-78 stops on ROE) */
+        double rrho, rv[3], rp; roe_state(g, qL, qR, &rrho, rv, &rp);
+        double H = (g / (g - 1.0)) * rp / rrho + .5 * (rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+        double a = sqrt(g * rp / rrho);
+        double vn = rv[0] * n[0] + rv[1] * n[1] + rv[2] * n[2];
+        double drho = qR[0] - qL[0], dp = qR[1] - qL[1], dvn = vnr - vnl;
+        double dv[3] = {qR[2] - qL[2], qR[3] - qL[3], qR[4] - qL[4]};
+        double l1 = fabs(vn - a), l2 = fabs(vn), l3 = fabs(vn + a);
+        double del = 0.1 * a;
+        if (l1 < del) l1 = .5 * (l1 * l1 / del + del);
+        if (l3 < del) l3 = .5 * (l3 * l3 / del + del);
+        double w1 = l1 * (dp - rrho * a * dvn) / (2 * a * a);
+        double w2 = l2 * (drho - dp / (a * a));
+        double w3 = l3 * (dp + rrho * a * dvn) / (2 * a * a);
+        double w4 = l2 * rrho;
+        double dvt[3] = {dv[0] - dvn * n[0], dv[1] - dvn * n[1], dv[2] - dvn * n[2]};
+        double FL[5], FR[5];
+        FL[0] = qL[0] * vnl; FR[0] = qR[0] * vnr;
+        FL[1] = vnl * (rel + qL[1]); FR[1] = vnr * (rer + qR[1]);
+        for (int d = 0; d < 3; ++d) { FL[2 + d] = qL[0] * vnl * qL[2 + d] + qL[1] * n[d];
+                                      FR[2 + d] = qR[0] * vnr * qR[2 + d] + qR[1] * n[d]; }
+        double D[5];
+        D[0] = w1 + w2 + w3;
+        D[1] = w1 * (H - vn * a) + w2 * .5 * (rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2])
+               + w3 * (H + vn * a) + w4 * (rv[0] * dvt[0] + rv[1] * dvt[1] + rv[2] * dvt[2]);
+        for (int d = 0; d < 3; ++d)
+            D[2 + d] = w1 * (rv[d] - a * n[d]) + w2 * rv[d] + w3 * (rv[d] + a * n[d]) + w4 * dvt[d];
+        for (int k = 0; k < 5; ++k) F[k] = .5 * (FL[k] + FR[k] - D[k]);
+        return; }
+    case OR_SCH_HLLC: case OR_SCH_HLLCB: default: { /* calc_flux_hllc.f90:61-171 */
+        double beta = (c->sch_hyp == OR_SCH_HLLCB) ? sqrt((g - 1.0) / (2 * g)) : 1.0;
+        double al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
+        double rrho, rv[3], rp; roe_state(g, qL, qR, &rrho, rv, &rp);
+        double vm = rv[0] * n[0] + rv[1] * n[1] + rv[2] * n[2];
+        double am = sqrt(g * rp / rrho);
+        double sl = FMIN(vnl - beta * al, vm - am);
+        double sr = FMAX(vnr + beta * ar, vm + am);
+        if (sl >= 0.0) {
+            F[0] = vnl * qL[0]; F[1] = vnl * (rel + qL[1]);
+            for (int d = 0; d < 3; ++d) F[2 + d] = (vnl * qL[0]) * qL[2 + d] + qL[1] * n[d];
+        } else if (sr <= 0.0) {
+            F[0] = vnr * qR[0]; F[1] = vnr * (rer + qR[1]);
+            for (int d = 0; d < 3; ++d) F[2 + d] = (vnr * qR[0]) * qR[2 + d] + qR[1] * n[d];
+        } else {
+            double rqL = qL[0] * (sl - vnl), rqR = qR[0] * (sr - vnr);
+            double sst = (rqR * vnr - rqL * vnl - qR[1] + qL[1]) / (rqR - rqL);
+            double iks, rst, pst, rvst[3], rest;
+            if (sst >= 0.0) {
+                iks = 1.0 / (sl - sst); rst = rqL * iks; pst = qL[1] - rqL * (vnl - sst);
+                for (int d = 0; d < 3; ++d) rvst[d] = iks * ((rqL * qL[2 + d]) + (pst - qL[1]) * n[d]);
+                rest = iks * (rel * (sl - vnl) - qL[1] * vnl + pst * sst);
+            } else {
+                iks = 1.0 / (sr - sst); rst = rqR * iks; pst = qR[1] - rqR * (vnr - sst);
+                for (int d = 0; d < 3; ++d) rvst[d] = iks * ((rqR * qR[2 + d]) + (pst - qR[1]) * n[d]);
+                rest = iks * (rer * (sr - vnr) - qR[1] * vnr + pst * sst);
+            }
+            F[0] = sst * rst; F[1] = sst * (rest + pst);
+            for (int d = 0; d < 3; ++d) F[2 + d] = (sst * rvst[d]) + (pst * n[d]);
+        }
+        return; }
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * A10: viscous fluxes, calc_flux_viscous_cell (SOURCE/EQNS/calc_flux_viscous_cell.f90:75-158)
+ * with interp_facegradient_vect (SOURCE/MESH/interpface_gradient_vect.f90:59-81),
+ * interp_facegradn_scal (SOURCE/MESH/interpface_gradn_scal.f90:59-91) and
+ * calc_viscosity (SOURCE/EQNS/EQNS.f90:225-265).  Cell states and cell gradients
+ * are used on both sides (integration_ns_ust.f90:136-159); ghost cells carry the
+ * cell_state / cell_grad boundary values. */
+double or_viscosity(const or_config* c, double rho, double T)
+{
+    switch (c->visc_law) {
+    case OR_VISC_SUTH: return c->visc_dyn * pow(T / c->tref, 1.5) * (c->tref + c->tsuth) / (T + c->tsuth);
+    case OR_VISC_KINCST: return c->visc_kin * rho;
+    case OR_VISC_DYNCST: default: return c->visc_dyn;
+    }
+}
+
+static void viscous_fluxes(or_ctx* o)
+{
+    const or_config* c = &o->c;
+    double r = c->rgas;
+    double cp = c->gamma * r / (c->gamma - 1);
+    for (int f = 0; f < o->nf; ++f) {
+        int cl = c->ic1[f], cr = c->ic2[f];
+        const double* xl = c->ccent + 3 * (size_t)cl;
+        const double* xr = c->ccent + 3 * (size_t)cr;
+        const double* xf = c->fcent + 3 * (size_t)f;
+        const double* n = c->fnorm + 3 * (size_t)f;
+        double a[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+        double b[3] = {xf[0] - xr[0], xf[1] - xr[1], xf[2] - xr[2]};
+        double dHL = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        double dHR = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        double id = 1.0 / (dHL + dHR);
+        dHL = id * dHL; dHR = id * dHR;
+        double vLR[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+        double rhoL = PRIM(o, 0, cl), rhoR = PRIM(o, 0, cr);
+        double pL = PRIM(o, 1, cl), pR = PRIM(o, 1, cr);
+        double vL[3] = {PRIM(o, 2, cl), PRIM(o, 3, cl), PRIM(o, 4, cl)};
+        double vR[3] = {PRIM(o, 2, cr), PRIM(o, 3, cr), PRIM(o, 4, cr)};
+        double velH[3];
+        for (int d = 0; d < 3; ++d) velH[d] = dHR * vL[d] + dHL * vR[d];
+        double TL = pL / (rhoL * r), TR = pR / (rhoR * r);
+        double TH = dHR * TL + dHL * TR;
+        double rhoH = dHR * rhoL + dHL * rhoR;
+        double gTL[3], gTR[3];
+        double kl1 = 1.0 / (rhoL * r), kl2 = pL / (rhoL * rhoL * r);
+        double kr1 = 1.0 / (rhoR * r), kr2 = pR / (rhoR * rhoR * r);
+        for (int d = 0; d < 3; ++d) {
+            gTL[d] = kl1 * GRAD(o, 1, d, cl) - kl2 * GRAD(o, 0, d, cl);
+            gTR[d] = kr1 * GRAD(o, 1, d, cr) - kr2 * GRAD(o, 0, d, cr);
+        }
+        double mu = or_viscosity(c, rhoH, TH);
+        /* face velocity gradient G(i,j) = d u_i / d x_j */
+        double G[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) G[i][j] = dHL * GRAD(o, 2 + i, j, cr) + dHR * GRAD(o, 2 + i, j, cl);
+        double dLR2 = vLR[0] * vLR[0] + vLR[1] * vLR[1] + vLR[2] * vLR[2];
+        if (c->sch_dis == OR_DIS_CELLFULL) {
+            double kv[3] = {(1.0 / dLR2) * vLR[0], (1.0 / dLR2) * vLR[1], (1.0 / dLR2) * vLR[2]};
+            for (int i = 0; i < 3; ++i) {
+                double gv = G[i][0] * vLR[0] + G[i][1] * vLR[1] + G[i][2] * vLR[2];
+                double fa = vR[i] - vL[i] - gv;
+                for (int j = 0; j < 3; ++j) G[i][j] = G[i][j] + fa * kv[j];
+            }
+        }
+        double gTH;
+        double gavg[3] = {dHL * gTR[0] + dHR * gTL[0], dHL * gTR[1] + dHR * gTL[1], dHL * gTR[2] + dHR * gTL[2]};
+        if (c->sch_dis == OR_DIS_CELLFULL) {
+            double pscal = (vLR[0] * n[0] + vLR[1] * n[1] + vLR[2] * n[2]) / dLR2;
+            double fcomp = pscal * (TR - TL);
+            double vi[3] = {n[0] - pscal * vLR[0], n[1] - pscal * vLR[1], n[2] - pscal * vLR[2]};
+            gTH = 1.0 * fcomp + dot3(gavg, vi);
+        } else {
+            gTH = dot3(gavg, n);
+        }
+        double S[3][3];
+        double tr = G[0][0] + G[1][1] + G[2][2];
+        double addv = -(2.0 / 3.0) * tr;
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) S[i][j] = G[i][j] + G[j][i];
+        for (int i = 0; i < 3; ++i) S[i][i] = S[i][i] + addv;
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) S[i][j] = mu * S[i][j];
+        double sn[3];
+        for (int i = 0; i < 3; ++i) sn[i] = S[i][0] * n[0] + S[i][1] * n[1] + S[i][2] * n[2];
+        double svn = dot3(sn, velH);
+        for (int d = 0; d < 3; ++d) FLX(o, 2 + d, f) = FLX(o, 2 + d, f) - sn[d];
+        FLX(o, 1, f) = FLX(o, 1, f) - svn;
+        double conduct = mu * cp / c->prandtl;
+        FLX(o, 1, f) = FLX(o, 1, f) - conduct * gTH;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * calc_rhs (SOURCE/MAIN/calc_rhs.f90:48-145) for one grid with global dt. */
+static void calc_rhs_internal(or_ctx* o, double dt)
+{
+    const or_config* c = &o->c;
+    varprim(o);
+    int second = (c->method != 'N');
+    int needgrad = second || c->viscous;
+    if (needgrad) {
+        boco_cell_state(o);
+        calc_gradient(o);
+        boco_cell_grad(o);
+    }
+    if (second && c->venkat) {
+        venkat_states(o);
+    } else {
+        hres_states(o);
+        if (c->postlimiter == OR_POSTLIM_BARTH) postlimit_barth(o, 0.5);
+        else if (c->postlimiter == OR_POSTLIM_SUPERBARTH) postlimit_barth(o, 1.0);
+    }
+    boco_face_state(o);
+    /* integration_ns_ust (EQNS/integration_ns_ust.f90:87-192): inviscid fluxes */
+    for (int f = 0; f < o->nf; ++f) {
+        double ql[NV], qr[NV], F[NV];
+        for (int v = 0; v < NV; ++v) { ql[v] = QL(o, v, f); qr[v] = QR(o, v, f); }
+        or_flux_face(c, ql, qr, c->fnorm + 3 * (size_t)f, F);
+        for (int v = 0; v < NV; ++v) FLX(o, v, f) = F[v];
+    }
+    if (c->viscous) viscous_fluxes(o);
+    /* flux_to_res (SOURCE/MAIN/flux_to_res.f90:47-142) */
+    for (int v = 0; v < NV; ++v)
+        for (int f = 0; f < o->nf; ++f) FLX(o, v, f) = c->fsurf[f] * FLX(o, v, f);
+    memset(o->res, 0, sizeof(double) * NV * (size_t)o->nc);
+    for (int f = 0; f < o->nf; ++f) {
+        int c1 = c->ic1[f], c2 = c->ic2[f];
+        for (int v = 0; v < NV; ++v) {
+            RES(o, v, c1) = RES(o, v, c1) - FLX(o, v, f);
+            RES(o, v, c2) = RES(o, v, c2) + FLX(o, v, f);
+        }
+    }
+    for (int i = 0; i < o->ni; ++i) {
+        double coef = dt / c->cvol[i];
+        for (int v = 0; v < NV; ++v) RES(o, v, i) = coef * RES(o, v, i);
+    }
+}
+
+void or_calc_rhs(or_ctx* o, double dt, double* res)
+{
+    calc_rhs_internal(o, dt);
+    if (res)
+        for (int v = 0; v < NV; ++v) memcpy(res + (size_t)v * o->ni, &RES(o, v, 0), sizeof(double) * o->ni);
+}
+
+/* A13: calc_ns_timestep (SOURCE/EQNS/calc_ns_timestep.f90:52-102) and
+   calc_zonetimestep (SOURCE/ZONE/calc_zonetimestep.f90:57-126, stab_cond). */
+double or_timestep(or_ctx* o, double cfl, double dtmax)
+{
+    const or_config* c = &o->c;
+    int ni = o->ni;
+    double gg1 = c->gamma * (c->gamma - 1.0);
+    for (int i = 0; i < ni; ++i) o->dtloc[i] = 0.0;
+    for (int f = 0; f < o->nf; ++f) {
+        int c1 = c->ic1[f], c2 = c->ic2[f];
+        o->dtloc[c1] = o->dtloc[c1] + c->fsurf[f];
+        if (c2 < ni) o->dtloc[c2] = o->dtloc[c2] + c->fsurf[f];
+    }
+    double dt = HUGE_VAL;
+    int bad = 0;
+    for (int i = 0; i < ni; ++i) {
+        double d = cfl * 2.0 * c->cvol[i] / o->dtloc[i];
+        double irho = 1.0 / CONS(o, 0, i);
+        double rv2 = CONS(o, 2, i) * CONS(o, 2, i) + CONS(o, 3, i) * CONS(o, 3, i) + CONS(o, 4, i) * CONS(o, 4, i);
+        double a2 = (CONS(o, 1, i) - .5 * rv2 * irho) * gg1 * irho;
+        if (!(a2 > 0.0)) bad++;
+        d = d / (sqrt(rv2) * irho + sqrt(a2));
+        o->dtloc[i] = d;
+        dt = FMIN(dt, d);
+    }
+    if (bad) return -1.0;
+    dt = FMIN(dt, dtmax);
+    return dt;
+}
+
+/* A14: treelevel_rungekutta (SOURCE/MAIN/treelevel_rungekutta.f90:50-113) and
+   update_field (SOURCE/ZONE/update_field.f90:38) with global dt. */
+void or_rk_step(or_ctx* o, double dt)
+{
+    const or_config* c = &o->c;
+    int ns = c->nstage, nc = o->nc;
+    size_t sz = (size_t)NV * nc;
+    double* u0 = (double*)malloc(sizeof(double) * sz);
+    double* rhs = (double*)malloc(sizeof(double) * sz * (size_t)ns);
+    memcpy(u0, o->cons, sizeof(double) * sz);
+    for (int s = 1; s <= ns; ++s) {
+        if (s >= 2) {
+            memcpy(o->cons, u0, sizeof(double) * sz);
+            for (int j = 1; j <= s - 1; ++j) {
+                double a = c->coef[(s - 2) * 8 + (j - 1)];
+                const double* r = rhs + (size_t)(j - 1) * sz;
+                for (size_t k = 0; k < sz; ++k) o->cons[k] = o->cons[k] + a * r[k];
+            }
+        }
+        calc_rhs_internal(o, dt);
+        memcpy(rhs + (size_t)(s - 1) * sz, o->res, sizeof(double) * sz);
+    }
+    /* residu = coef(n,1) R1 + sum coef(n,j) Rj ; U = U0 + residu */
+    double b1 = c->coef[(ns - 1) * 8 + 0];
+    for (size_t k = 0; k < sz; ++k) o->res[k] = rhs[k] * b1;
+    for (int j = 2; j <= ns; ++j) {
+        double b = c->coef[(ns - 1) * 8 + (j - 1)];
+        const double* r = rhs + (size_t)(j - 1) * sz;
+        for (size_t k = 0; k < sz; ++k) o->res[k] = o->res[k] + b * r[k];
+    }
+    for (size_t k = 0; k < sz; ++k) o->cons[k] = u0[k] + o->res[k];
+    free(u0); free(rhs);
+}
+
+/* ---------------------------------------------------------------------------
+ * A1: MESHGEOM restatement (CFDTOOLS/Mesh/MESHGEOM.f90:37-461). */
+int or_meshgeom(int dim, int nvtx, const double* vtx, int nface, const int32_t* fvtx,
+                int32_t* ic1, int32_t* ic2, int ncell_int,
+                double* fnorm, double* fcent, double* fsurf, double* ccent, double* cvol)
+{
+    (void)nvtx;
+    /* calc_ust_face :37-139 */
+    for (int f = 0; f < nface; ++f) {
+        const int32_t* fv = fvtx + 4 * (size_t)f;
+        int ns = 0; while (ns < 4 && fv[ns] >= 0) ns++;
+        const double* v1 = vtx + 3 * (size_t)fv[0];
+        const double* v2 = vtx + 3 * (size_t)fv[1];
+        double nrm[3], surf, cen[3];
+        if (dim == 2) {
+            double e[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+            nrm[0] = 0.0 * e[2] - 1.0 * e[1]; nrm[1] = 1.0 * e[0] - 0.0 * e[2]; nrm[2] = 0.0 * e[1] - 0.0 * e[0];
+            surf = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            for (int d = 0; d < 3; ++d) cen[d] = .5 * (v1[d] + v2[d]);
+        } else {
+            const double* v3 = vtx + 3 * (size_t)fv[2];
+            double a[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+            double b[3] = {v3[0] - v1[0], v3[1] - v1[1], v3[2] - v1[2]};
+            double n1[3] = {.5 * (a[1] * b[2] - a[2] * b[1]), .5 * (a[2] * b[0] - a[0] * b[2]), .5 * (a[0] * b[1] - a[1] * b[0])};
+            if (ns == 3) {
+                for (int d = 0; d < 3; ++d) { nrm[d] = n1[d]; cen[d] = (v1[d] + v2[d] + v3[d]) / 3.0; }
+                surf = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            } else {
+                const double* v4 = vtx + 3 * (size_t)fv[3];
+                double s1 = sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
+                double c4[3] = {v4[0] - v1[0], v4[1] - v1[1], v4[2] - v1[2]};
+                double n2[3] = {.5 * (b[1] * c4[2] - b[2] * c4[1]), .5 * (b[2] * c4[0] - b[0] * c4[2]), .5 * (b[0] * c4[1] - b[1] * c4[0])};
+                double s2 = sqrt(n2[0] * n2[0] + n2[1] * n2[1] + n2[2] * n2[2]);
+                for (int d = 0; d < 3; ++d) {
+                    double cg1 = (v1[d] + v2[d] + v3[d]) / 3.0, cg2 = (v1[d] + v3[d] + v4[d]) / 3.0;
+                    nrm[d] = n1[d] + n2[d];
+                    cen[d] = (s1 * cg1 + s2 * cg2) / (s1 + s2);
+                }
+                surf = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            }
+        }
+        for (int d = 0; d < 3; ++d) { fnorm[3 * (size_t)f + d] = nrm[d] / surf; fcent[3 * (size_t)f + d] = cen[d]; }
+        fsurf[f] = surf;
+    }
+    /* calc_ust_midcell :148-197 */
+    double* mid = (double*)calloc(3 * (size_t)ncell_int, sizeof(double));
+    int* nb = (int*)calloc((size_t)ncell_int, sizeof(int));
+    for (int f = 0; f < nface; ++f) {
+        int a = ic1[f], b = ic2[f];
+        for (int d = 0; d < 3; ++d) mid[3 * (size_t)a + d] += fcent[3 * (size_t)f + d];
+        nb[a]++;
+        if (b >= 0 && b < ncell_int) { for (int d = 0; d < 3; ++d) mid[3 * (size_t)b + d] += fcent[3 * (size_t)f + d]; nb[b]++; }
+    }
+    for (int i = 0; i < ncell_int; ++i) for (int d = 0; d < 3; ++d) mid[3 * (size_t)i + d] /= (double)nb[i];
+    /* calc_ust_elemvol :215-277 and calc_ust_cell :286-331 */
+    double kcg = (dim == 2) ? 1. / 3.0 : 1. / 4.0;
+    double kvol = (dim == 2) ? 1. / 2.0 : 1. / 3.0;
+    for (int i = 0; i < ncell_int; ++i) { cvol[i] = 0.0; ccent[3 * (size_t)i] = ccent[3 * (size_t)i + 1] = ccent[3 * (size_t)i + 2] = 0.0; }
+    for (int f = 0; f < nface; ++f) {
+        for (int side = 0; side < 2; ++side) {
+            int ci = side ? ic2[f] : ic1[f];
+            if (side && !(ci >= 0 && ci < ncell_int)) continue;
+            const double* fc = fcent + 3 * (size_t)f;
+            const double* fn = fnorm + 3 * (size_t)f;
+            double kh[3] = {fc[0] - mid[3 * (size_t)ci], fc[1] - mid[3 * (size_t)ci + 1], fc[2] - mid[3 * (size_t)ci + 2]};
+            double h = fabs(kh[0] * fn[0] + kh[1] * fn[1] + kh[2] * fn[2]);
+            double vol = kvol * h * fsurf[f];
+            for (int d = 0; d < 3; ++d) ccent[3 * (size_t)ci + d] += vol * (fc[d] - kcg * kh[d]);
+            cvol[ci] += vol;
+        }
+    }
+    for (int i = 0; i < ncell_int; ++i) for (int d = 0; d < 3; ++d) ccent[3 * (size_t)i + d] /= cvol[i];
+    /* calc_ust_checkface :362-410 */
+    for (int f = 0; f < nface; ++f) {
+        if (ic2[f] >= 0 && ic1[f] > ic2[f]) { int t = ic1[f]; ic1[f] = ic2[f]; ic2[f] = t; }
+        const double* a = ccent + 3 * (size_t)ic1[f];
+        double v12[3];
+        if (ic2[f] >= 0) { const double* b = ccent + 3 * (size_t)ic2[f]; for (int d = 0; d < 3; ++d) v12[d] = b[d] - a[d]; }
+        else { const double* b = fcent + 3 * (size_t)f; for (int d = 0; d < 3; ++d) v12[d] = b[d] - a[d]; }
+        double* n = fnorm + 3 * (size_t)f;
+        if (v12[0] * n[0] + v12[1] * n[1] + v12[2] * n[2] < 0.0) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
+    }
+    free(mid); free(nb);
+    return 0;
+}

@@ -1,0 +1,190 @@
+"""Parity of the HIP residual path (through the C-ABI) with the oracle.
+
+FP64 bar: 1e-12 relative (max-norm per field) on residuals and on states after
+a few RK steps -- the summation order differs (per-cell slot order on the GPU,
+face order in the oracle, colour order in the reference), so results agree to
+a few ulps, not bitwise.  Mixed precision (FP32 fluxes/gradients, FP64
+accumulation) is checked at the tolerance written in each test.
+"""
+import numpy as np
+import pytest
+
+import cases as K
+from typhon_amd import scheme as S
+from typhon_amd.solver import Solver
+from typhon_amd._lib import TyError
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def _pair(mk):
+    m, s, g, u = mk()
+    gpu = Solver(m, s, g)
+    gpu.set_state(u)
+    orc = K.make_oracle(m, s, g)
+    orc.set_cons(u)
+    return m, s, g, u, gpu, orc
+
+
+def _residual_parity(mk, tol=TOL64):
+    m, s, g, u, gpu, orc = _pair(mk)
+    dt_g = gpu.timestep()
+    dt_o = orc.timestep(s["cfl"])
+    assert abs(dt_g - dt_o) <= 1e-13 * dt_o
+    rg = gpu.calc_rhs(dt_o)
+    ro = orc.calc_rhs(dt_o)
+    nv = m.dim + 2
+    err = K.rel_err(rg[:nv], ro[:nv])
+    assert err <= tol, f"residual rel err {err:.3e}"
+    return gpu, orc
+
+
+def _step_parity(mk, nsteps=5, tol=TOL64):
+    m, s, g, u, gpu, orc = _pair(mk)
+    for _ in range(nsteps):
+        dt = orc.timestep(s["cfl"])
+        gpu.rk_step(dt)
+        orc.rk_step(dt)
+    nv = m.dim + 2
+    err = K.rel_err(gpu.get_state()[:nv], orc.get_cons()[:nv])
+    assert err <= tol, f"state rel err after {nsteps} steps {err:.3e}"
+    return gpu, orc
+
+
+# --- first order (config: 2D channel, HLLC, RK2) ------------------------------
+
+def test_channel_first_order_residual():
+    _residual_parity(lambda: K.channel_case(32, 16))
+
+
+def test_channel_first_order_full_size_steps():
+    _step_parity(lambda: K.channel_case(256, 128), nsteps=5)
+
+
+@pytest.mark.parametrize("sch", [S.SCH_HLLC, S.SCH_HLLCB, S.SCH_HLLE, S.SCH_HLLEB, S.SCH_RUSANOV, S.SCH_ROE])
+def test_fluxes_residual(sch):
+    _residual_parity(lambda: K.channel_case(24, 12, sch_hyp=sch))
+
+
+# --- second order reference MUSCL family ---------------------------------------
+
+@pytest.mark.parametrize("method", ["M", "F", "U"])
+@pytest.mark.parametrize("lim", ["X", "M", "A", "V", "S", "K", "L"])
+def test_muscl_family_channel(method, lim):
+    _residual_parity(lambda: K.channel_case(24, 12, method=method, limiter=lim))
+
+
+@pytest.mark.parametrize("method", ["M", "F", "U"])
+def test_muscl_periodic_jittered(method):
+    _residual_parity(lambda: K.periodic_quad_case(16, 12, jitter=0.2, method=method, limiter="A"))
+
+
+@pytest.mark.parametrize("post", [S.POSTLIM_BARTH, S.POSTLIM_SUPERBARTH])
+@pytest.mark.parametrize("method", ["M", "U"])
+def test_barth_postlimiter(method, post):
+    _residual_parity(lambda: K.channel_case(24, 12, method=method, limiter="X", postlimiter=post))
+    _residual_parity(lambda: K.vortex_case(4, L=4.0, method=method, limiter="X", postlimiter=post))
+
+
+def test_weighted_lsq():
+    _residual_parity(lambda: K.channel_case(24, 12, method="M", limiter="M", gradmeth=S.GRAD_LSQW))
+
+
+# --- 3D tets, Venkatakrishnan, RK4 ----------------------------------------------
+
+def test_vortex_venkat_residual():
+    _residual_parity(lambda: K.vortex_case(6, L=6.0))
+
+
+def test_vortex_venkat_steps():
+    _step_parity(lambda: K.vortex_case(6, L=6.0), nsteps=3)
+
+
+def test_vortex_reference_variant_steps():
+    # the reference-expressible form of the 3D case: MUSCL + van Albada
+    _step_parity(lambda: K.vortex_case(5, L=5.0, limiter="A"), nsteps=2)
+
+
+@pytest.mark.parametrize("tps", [S.TPS_EXPL, S.TPS_RK2, S.TPS_RK2SSP, S.TPS_RK3SSP, S.TPS_RK4,
+                                 S.TPS_LSRK25BB, S.TPS_LSRK26BB, S.TPS_LSRK12BS, S.TPS_LSRK13BS])
+def test_rk_methods(tps):
+    _step_parity(lambda: K.periodic_quad_case(12, 10, jitter=0.1, tps_meth=tps), nsteps=3)
+
+
+# --- Navier-Stokes ----------------------------------------------------------------
+
+@pytest.mark.parametrize("dis", [S.DIS_CELLAVG2, S.DIS_CELLFULL])
+def test_tgv_viscous_residual(dis):
+    _residual_parity(lambda: K.tgv_case(6, sch_dis=dis))
+
+
+def test_tgv_viscous_steps():
+    _step_parity(lambda: K.tgv_case(6), nsteps=3)
+
+
+def test_viscous_second_order_jittered():
+    _residual_parity(lambda: K.tgv_case(5, jitter=0.1, method="M", limiter="W"))
+
+
+# --- mixed precision ----------------------------------------------------------------
+
+def test_mixed_precision_residual():
+    # FP32 flux/gradient arithmetic: residual within 2e-5 relative of the FP64 oracle
+    _residual_parity(lambda: K.vortex_case(6, L=6.0, precision=S.PREC_MIXED), tol=2e-5)
+
+
+def test_mixed_precision_steps():
+    # FP64 accumulation: density within 1e-6 relative of FP64 after 20 RK4 steps
+    m, s, g, u = K.vortex_case(6, L=6.0)
+    a = Solver(m, s, g)
+    s2 = dict(s)
+    s2["precision"] = S.PREC_MIXED
+    b = Solver(m, s2, g)
+    a.set_state(u)
+    b.set_state(u)
+    for _ in range(20):
+        dt = a.timestep()
+        a.rk_step(dt)
+        b.rk_step(dt)
+    ra, rb = a.get_state()[0], b.get_state()[0]
+    assert np.abs(ra - rb).max() / np.abs(ra).max() <= 1e-6
+
+
+# --- driver behaviour ------------------------------------------------------------------
+
+def test_advance_matches_host_loop():
+    m, s, g, u = K.vortex_case(5, L=5.0)
+    a = Solver(m, s, g)
+    b = Solver(m, s, g)
+    a.set_state(u)
+    b.set_state(u)
+    for _ in range(3):
+        b.rk_step(b.timestep())
+    dt = a.advance(3, want_dt=True)
+    assert dt > 0
+    assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_bitwise_repeatable():
+    m, s, g, u = K.vortex_case(5, L=5.0)
+    out = []
+    for _ in range(2):
+        a = Solver(m, s, g)
+        a.set_state(u)
+        a.advance(3)
+        out.append(a.get_state())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_bad_state_is_reported():
+    # calc_ns_timestep.f90:89-100: a cell with rhoE < 1/2 rho |u|^2 stops the run
+    m, s, g, u = K.channel_case(16, 8)
+    u = u.copy()
+    u[1, 5] = 0.1 * 0.5 * u[2, 5] ** 2 / u[0, 5]
+    a = Solver(m, s, g)
+    a.set_state(u)
+    with pytest.raises(TyError) as e:
+        a.timestep()
+    assert e.value.code == 3

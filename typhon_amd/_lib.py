@@ -1,0 +1,81 @@
+"""ctypes binding of libtyphon_gpu.so (include/typhon_gpu.h).
+
+The library must be built in-tree (python -m typhon_amd.build); loading fails
+loudly otherwise -- there is no CPU fallback."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libtyphon_gpu.so")
+
+I32P = C.POINTER(C.c_int32)
+F64P = C.POINTER(C.c_double)
+
+EXPORTS = ["ty_create", "ty_destroy", "ty_last_error", "ty_mesh_upload", "ty_set_scheme", "ty_state_set",
+           "ty_state_get", "ty_timestep", "ty_residual", "ty_rk_step", "ty_advance", "ty_sync", "ty_timing",
+           "ty_mesh_info"]
+
+
+class TyBocoDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("nface", C.c_int32), ("iface", I32P), ("ipart", I32P),
+                ("param", C.c_double * 8)]
+
+
+class TyMeshDesc(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("ncell_int", C.c_int32), ("ncell", C.c_int32), ("nface", C.c_int32),
+                ("nface_int", C.c_int32), ("facecell", I32P), ("face", F64P), ("centre", F64P),
+                ("volume", F64P), ("nboco", C.c_int32), ("boco", C.POINTER(TyBocoDesc))]
+
+
+class TyGas(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("r_const", C.c_double), ("prandtl", C.c_double),
+                ("typ_visc", C.c_int32), ("visc_dyn", C.c_double), ("visc_kin", C.c_double),
+                ("tref", C.c_double), ("tsuth", C.c_double), ("typ_fluid", C.c_int32)]
+
+
+class TyScheme(C.Structure):
+    _fields_ = [("sch_hyp", C.c_int32), ("sch_dis", C.c_int32), ("method", C.c_char), ("limiter", C.c_char),
+                ("gradmeth", C.c_int32), ("postlimiter", C.c_int32), ("venkat_k", C.c_double),
+                ("nstage", C.c_int32), ("rk_coef", F64P), ("cfl", C.c_double), ("precision", C.c_int32)]
+
+
+class TyError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"typhon_gpu error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load the in-tree HIP library (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} is missing: build it with `python -m typhon_amd.build`")
+    lib = C.CDLL(LIBPATH)
+    vp = C.c_void_p
+    lib.ty_create.argtypes = [C.POINTER(vp), C.c_int]
+    lib.ty_destroy.argtypes = [vp]
+    lib.ty_last_error.restype = C.c_char_p
+    lib.ty_last_error.argtypes = [vp]
+    lib.ty_mesh_upload.argtypes = [vp, C.POINTER(TyMeshDesc)]
+    lib.ty_set_scheme.argtypes = [vp, C.POINTER(TyGas), C.POINTER(TyScheme)]
+    lib.ty_state_set.argtypes = [vp, F64P, F64P, F64P]
+    lib.ty_state_get.argtypes = [vp, F64P, F64P, F64P]
+    lib.ty_timestep.argtypes = [vp, C.c_double, F64P]
+    lib.ty_residual.argtypes = [vp, C.c_double, F64P, F64P, F64P]
+    lib.ty_rk_step.argtypes = [vp, C.c_double]
+    lib.ty_advance.argtypes = [vp, C.c_int32, C.c_double, F64P]
+    lib.ty_sync.argtypes = [vp]
+    lib.ty_timing.argtypes = [vp, C.c_int32, F64P, I32P]
+    lib.ty_mesh_info.argtypes = [vp, I32P, I32P, I32P, I32P]
+    for name in EXPORTS:
+        if name != "ty_last_error":
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib

@@ -1,0 +1,926 @@
+// Host side of the C-ABI declared in include/typhon_gpu.h.
+//
+// Mesh upload turns Typhon's st_ustmesh (ghost cells, periodic boundaries as
+// ghost copies, CFDTOOLS/Mesh/USTMESH.f90:31-52) into the device layout:
+//   - interior cells renumbered by reverse Cuthill-McKee,
+//   - periodic face pairs merged into one two-sided face whose far side is
+//     translated by a shift from a small table (init_connect_grid.f90:226-231),
+//   - internal faces sorted by their lower cell, boundary faces last,
+//   - per-cell face slots in ELL form [slot][cell].
+// Everything stays resident; only dt and optional monitors come back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/typhon_gpu.h"
+#include "ty_internal.h"
+
+using namespace ty;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct ty_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // host mesh (device order)
+    int dim = 0, n = 0, ns = 0, nf = 0, nfi = 0, nb = 0;
+    std::vector<int> perm, iperm;  // perm[new] = caller interior index
+    std::vector<int> nbr, fw;      // [ns][n]
+    std::vector<double> fn, fc, fs;  // [3][nf], [3][nf], [nf]
+    std::vector<double> cc, vol, sumA;  // [3][n], [n], [n]
+    std::vector<double> gc;  // [3][nb]
+    std::vector<int> bboco;  // [nb]
+    std::vector<double> shift;  // [MAX_SHIFT][3]
+    int nshift = 1;
+    std::vector<ty_boco_desc> bocos;  // types and params (pointers not kept)
+    bool mesh_ok = false;
+    // scheme
+    bool scheme_ok = false;
+    ty_gas gas{};
+    ty_scheme sch{};
+    std::vector<double> rk;  // coef(i,j) row-major [ns][ns]
+    int nv = 0;
+    bool mixed = false;
+    Physics ph{};
+    // device buffers
+    std::vector<void*> owned;
+    int* d_nbr = nullptr;
+    int* d_fw = nullptr;
+    void* d_geo = nullptr;  // one allocation for the geometry of the active precision
+    Geo<double> geo_d{};
+    Geo<float> geo_f{};
+    int* d_bboco = nullptr;
+    BocoDev* d_boco = nullptr;
+    double* d_vol = nullptr;
+    double* d_dtfac = nullptr;
+    double* d_u0 = nullptr;
+    void* d_us[2] = {nullptr, nullptr};
+    int cur = 0;
+    double* d_acc = nullptr;
+    double* d_rstore = nullptr;
+    int nstore = 0;
+    void* d_grad = nullptr;
+    void* d_ratio = nullptr;
+    double* d_res = nullptr;
+    double* d_part = nullptr;
+    int* d_bad = nullptr;
+    double* d_dt = nullptr;
+    int* d_err = nullptr;
+    bool state_ok = false;
+    double h_dt = 0.0;
+    // per-kernel event timing (bench)
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_grad, ev_flux, ev_dt;
+};
+
+namespace {
+
+int fail(ty_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(ctx, TY_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void free_state(ty_ctx* c)
+{
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->owned) hipFree(p);
+    c->owned.clear();
+    c->d_nbr = nullptr;
+    c->d_fw = nullptr;
+    c->d_geo = nullptr;
+    c->d_bboco = nullptr;
+    c->d_boco = nullptr;
+    c->d_vol = c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
+    c->d_us[0] = c->d_us[1] = nullptr;
+    c->d_grad = c->d_ratio = nullptr;
+    c->d_bad = c->d_err = nullptr;
+    for (auto* v : {&c->ev_grad, &c->ev_flux, &c->ev_dt})
+        for (auto& e : *v) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    c->ev_grad.clear();
+    c->ev_flux.clear();
+    c->ev_dt.clear();
+}
+
+template <typename T>
+int dalloc(ty_ctx* c, T** p, size_t count)
+{
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(c, TY_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->owned.push_back(q);
+    *p = static_cast<T*>(q);
+    return TY_OK;
+}
+
+template <typename T>
+int upload(ty_ctx* c, T** p, const std::vector<T>& h)
+{
+    int rc = dalloc(c, p, h.size());
+    if (rc) return rc;
+    if (!h.empty()) HIPCHK(c, hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return TY_OK;
+}
+
+// reverse Cuthill-McKee on the interior-cell graph
+std::vector<int> rcm(int n, const std::vector<int>& xadj, const std::vector<int>& adj)
+{
+    std::vector<int> deg(n);
+    for (int i = 0; i < n; ++i) deg[i] = xadj[i + 1] - xadj[i];
+    std::vector<int> mark(n, -1), order;
+    order.reserve(n);
+    std::vector<int> q(n), lvl(n);
+    int stamp = 0, scan = 0;
+    std::vector<char> done(n, 0);
+    auto bfs_far = [&](int s) {
+        // BFS from s over unfinished nodes; returns min-degree node of the last level
+        ++stamp;
+        int h = 0, t = 0;
+        q[t++] = s;
+        mark[s] = stamp;
+        lvl[s] = 0;
+        int last = 0;
+        while (h < t) {
+            int u = q[h++];
+            last = lvl[u];
+            for (int k = xadj[u]; k < xadj[u + 1]; ++k) {
+                int v = adj[k];
+                if (done[v] || mark[v] == stamp) continue;
+                mark[v] = stamp;
+                lvl[v] = lvl[u] + 1;
+                q[t++] = v;
+            }
+        }
+        int best = s;
+        for (int i = 0; i < t; ++i)
+            if (lvl[q[i]] == last && deg[q[i]] < deg[best]) best = q[i];
+        return best;
+    };
+    std::vector<int> nb;
+    while ((int)order.size() < n) {
+        while (done[scan]) ++scan;
+        int s = bfs_far(bfs_far(scan));
+        size_t head = order.size();
+        order.push_back(s);
+        done[s] = 1;
+        while (head < order.size()) {
+            int u = order[head++];
+            nb.clear();
+            for (int k = xadj[u]; k < xadj[u + 1]; ++k)
+                if (!done[adj[k]]) { nb.push_back(adj[k]); done[adj[k]] = 1; }
+            std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+            for (int v : nb) order.push_back(v);
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
+
+// cholesky_decomp (SOURCE/MATH/MATRIX.f90:42-70), a(i,j) column-major 3x3
+void cholesky3(double* a)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j) {
+            double sum = a[i + 3 * j];
+            for (int k = i - 1; k >= 0; --k) sum = sum - a[i + 3 * k] * a[j + 3 * k];
+            if (i == j) a[i + 3 * i] = std::sqrt(sum);
+            else a[j + 3 * i] = sum / a[i + 3 * i];
+        }
+}
+
+}  // namespace
+
+// The C-ABI functions below take C linkage from their declarations in typhon_gpu.h.
+
+int ty_create(ty_ctx** ctx, int device)
+{
+    if (!ctx) return TY_ERR_ARG;
+    *ctx = nullptr;
+    ty_ctx* c = new ty_ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return TY_ERR_HIP;
+    }
+    *ctx = c;
+    return TY_OK;
+}
+
+int ty_destroy(ty_ctx* c)
+{
+    if (!c) return TY_OK;
+    hipSetDevice(c->device);
+    free_state(c);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return TY_OK;
+}
+
+const char* ty_last_error(const ty_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
+{
+    if (!c || !m) return TY_ERR_ARG;
+    if (m->dim != 2 && m->dim != 3) return fail(c, TY_ERR_ARG, "dim must be 2 or 3");
+    const int ni = m->ncell_int, nc = m->ncell, nf = m->nface, nfi = m->nface_int;
+    if (ni <= 0 || nc < ni || nf <= 0 || nfi < 0 || nfi > nf) return fail(c, TY_ERR_ARG, "bad mesh sizes");
+    hipSetDevice(c->device);
+    free_state(c);
+    c->mesh_ok = c->scheme_ok = c->state_ok = false;
+    c->dim = m->dim;
+    c->n = ni;
+    std::vector<int> ic1(nf), ic2(nf);
+    for (int f = 0; f < nf; ++f) {
+        ic1[f] = m->facecell[f] - 1;
+        ic2[f] = m->facecell[(size_t)nf + f] - 1;
+        if (ic1[f] < 0 || ic1[f] >= ni) return fail(c, TY_ERR_ARG, "face " + std::to_string(f + 1) + ": bad ic1");
+        if (f < nfi && (ic2[f] < 0 || ic2[f] >= ni))
+            return fail(c, TY_ERR_ARG, "internal face " + std::to_string(f + 1) + ": bad ic2");
+        if (f >= nfi && (ic2[f] < ni || ic2[f] >= nc))
+            return fail(c, TY_ERR_ARG, "boundary face " + std::to_string(f + 1) + ": ic2 is not a ghost cell");
+    }
+    auto F = [&](int f, int k) { return m->face[(size_t)7 * f + k]; };
+    // boundary faces -> (boco, partner)
+    std::vector<int> fboco(nf, -1), fpart(nf, -1);
+    c->bocos.assign(m->boco, m->boco + m->nboco);
+    if (m->nboco > MAX_BOCO) return fail(c, TY_ERR_UNSUPPORTED, "too many boundary conditions");
+    for (int b = 0; b < m->nboco; ++b) {
+        const ty_boco_desc& d = m->boco[b];
+        c->bocos[b].iface = nullptr;
+        c->bocos[b].ipart = nullptr;
+        for (int k = 0; k < d.nface; ++k) {
+            int f = d.iface[k] - 1;
+            if (f < nfi || f >= nf) return fail(c, TY_ERR_ARG, "boco face outside the boundary face range");
+            if (fboco[f] >= 0) return fail(c, TY_ERR_ARG, "face in two boundary conditions");
+            fboco[f] = b;
+            if (d.type == 11) {
+                if (!d.ipart) return fail(c, TY_ERR_ARG, "periodic boco without partner faces");
+                int p = d.ipart[k] - 1;
+                if (p < nfi || p >= nf) return fail(c, TY_ERR_ARG, "bad periodic partner face");
+                fpart[f] = p;
+            } else if (d.type != 10 && d.type != 31 && d.type != 35 && d.type != 36) {
+                return fail(c, TY_ERR_UNSUPPORTED, "boundary condition type " + std::to_string(d.type) +
+                                                       " is not provided by this library");
+            }
+        }
+    }
+    for (int f = nfi; f < nf; ++f)
+        if (fboco[f] < 0) return fail(c, TY_ERR_ARG, "boundary face " + std::to_string(f + 1) + " has no boco");
+    for (int f = nfi; f < nf; ++f)
+        if (fpart[f] >= 0 && fpart[fpart[f]] != f) return fail(c, TY_ERR_ARG, "periodic partners are not mutual");
+    // device faces before renumbering: owner, other (or -1 boundary), source face, shift id
+    struct DF { int a, b, src, sid, bidx; };
+    std::vector<DF> faces;
+    faces.reserve(nf);
+    c->shift.assign(MAX_SHIFT * 3, 0.0);
+    c->nshift = 1;
+    for (int f = 0; f < nfi; ++f) faces.push_back({ic1[f], ic2[f], f, 0, -1});
+    std::vector<DF> bfaces;
+    for (int f = nfi; f < nf; ++f) {
+        if (fpart[f] >= 0) {
+            int p = fpart[f];
+            if (f > p) continue;
+            double T[3] = {F(f, 3) - F(p, 3), F(f, 4) - F(p, 4), F(f, 5) - F(p, 5)};
+            double tn = std::sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
+            int sid = -1;
+            for (int s = 1; s < c->nshift; ++s) {
+                double dx = T[0] - c->shift[3 * s], dy = T[1] - c->shift[3 * s + 1], dz = T[2] - c->shift[3 * s + 2];
+                if (std::sqrt(dx * dx + dy * dy + dz * dz) <= 1e-9 * (1.0 + tn)) { sid = s; break; }
+            }
+            if (sid < 0) {
+                if (c->nshift >= MAX_SHIFT) return fail(c, TY_ERR_UNSUPPORTED, "too many periodic translations");
+                sid = c->nshift++;
+                for (int d = 0; d < 3; ++d) c->shift[3 * sid + d] = T[d];
+            }
+            faces.push_back({ic1[f], ic1[p], f, sid, -1});
+        } else {
+            bfaces.push_back({ic1[f], -1, f, 0, fboco[f]});
+        }
+    }
+    // RCM over interior cells
+    std::vector<int> xadj(ni + 1, 0), adj;
+    for (auto& d : faces) { xadj[d.a + 1]++; xadj[d.b + 1]++; }
+    for (int i = 0; i < ni; ++i) xadj[i + 1] += xadj[i];
+    adj.resize(xadj[ni]);
+    {
+        std::vector<int> pos(xadj.begin(), xadj.end() - 1);
+        for (auto& d : faces) { adj[pos[d.a]++] = d.b; adj[pos[d.b]++] = d.a; }
+    }
+    c->perm = rcm(ni, xadj, adj);
+    c->iperm.assign(ni, 0);
+    for (int i = 0; i < ni; ++i) c->iperm[c->perm[i]] = i;
+    for (auto& d : faces) { d.a = c->iperm[d.a]; d.b = c->iperm[d.b]; }
+    for (auto& d : bfaces) d.a = c->iperm[d.a];
+    std::stable_sort(faces.begin(), faces.end(), [](const DF& x, const DF& y) {
+        int xl = std::min(x.a, x.b), yl = std::min(y.a, y.b);
+        if (xl != yl) return xl < yl;
+        return std::max(x.a, x.b) < std::max(y.a, y.b);
+    });
+    std::stable_sort(bfaces.begin(), bfaces.end(), [](const DF& x, const DF& y) { return x.a < y.a; });
+    c->nfi = (int)faces.size();
+    c->nb = (int)bfaces.size();
+    c->nf = c->nfi + c->nb;
+    const int n = ni, ndf = c->nf;
+    c->fn.assign((size_t)3 * ndf, 0.0);
+    c->fc.assign((size_t)3 * ndf, 0.0);
+    c->fs.assign(ndf, 0.0);
+    c->gc.assign((size_t)3 * c->nb, 0.0);
+    c->bboco.assign(c->nb, 0);
+    std::vector<std::vector<std::pair<int, int>>> slots(n);  // (nbr, word)
+    auto put_face = [&](int df, int src) {
+        for (int d = 0; d < 3; ++d) {
+            c->fn[(size_t)d * ndf + df] = F(src, d);
+            c->fc[(size_t)d * ndf + df] = F(src, 3 + d);
+        }
+        c->fs[df] = F(src, 6);
+    };
+    for (int k = 0; k < c->nfi; ++k) {
+        const DF& d = faces[k];
+        put_face(k, d.src);
+        int w = k | (d.sid << FACE_BITS);
+        slots[d.a].push_back({d.b, w});
+        slots[d.b].push_back({d.a, (int)((unsigned)w | 0x80000000u)});
+    }
+    for (int k = 0; k < c->nb; ++k) {
+        const DF& d = bfaces[k];
+        int df = c->nfi + k;
+        put_face(df, d.src);
+        int g = ic2[d.src];
+        for (int e = 0; e < 3; ++e) c->gc[(size_t)e * c->nb + k] = m->centre[(size_t)3 * g + e];
+        c->bboco[k] = d.bidx;
+        slots[d.a].push_back({-(k + 1), df});
+    }
+    if ((unsigned)c->nf > FACE_MASK) return fail(c, TY_ERR_UNSUPPORTED, "too many faces for the slot word");
+    int ns = 0;
+    for (auto& s : slots) ns = std::max<int>(ns, (int)s.size());
+    if (ns > MAX_SLOTS) return fail(c, TY_ERR_UNSUPPORTED, "more than 8 faces per cell");
+    c->ns = ns;
+    c->nbr.assign((size_t)ns * n, 0);
+    c->fw.assign((size_t)ns * n, SLOT_UNUSED);
+    c->sumA.assign(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (size_t k = 0; k < slots[i].size(); ++k) {
+            c->nbr[k * n + i] = slots[i][k].first;
+            c->fw[k * n + i] = slots[i][k].second;
+            c->sumA[i] += c->fs[(unsigned)slots[i][k].second & FACE_MASK];
+        }
+    c->cc.assign((size_t)3 * n, 0.0);
+    c->vol.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        int o = c->perm[i];
+        for (int d = 0; d < 3; ++d) c->cc[(size_t)d * n + i] = m->centre[(size_t)3 * o + d];
+        c->vol[i] = m->volume[o];
+    }
+    int rc;
+    if ((rc = upload(c, &c->d_nbr, c->nbr)) || (rc = upload(c, &c->d_fw, c->fw)) ||
+        (rc = upload(c, &c->d_bboco, c->bboco)) || (rc = upload(c, &c->d_vol, c->vol)))
+        return rc;
+    c->mesh_ok = true;
+    return TY_OK;
+}
+
+int ty_mesh_info(const ty_ctx* c, int32_t* ncell, int32_t* nface_dev, int32_t* nslot, int32_t* nbface_dev)
+{
+    if (!c || !c->mesh_ok) return TY_ERR_ARG;
+    if (ncell) *ncell = c->n;
+    if (nface_dev) *nface_dev = c->nf;
+    if (nslot) *nslot = c->ns;
+    if (nbface_dev) *nbface_dev = c->nb;
+    return TY_OK;
+}
+
+namespace {
+
+template <typename R>
+int build_geo(ty_ctx* c, Geo<R>& g)
+{
+    const int n = c->n, nf = c->nf, nb = c->nb, dim = c->dim;
+    std::vector<R> h;
+    // layout: fn(3nf) fs(nf) fc(3nf) cc(3n) gc(3nb) lsq(6n) eps2(n) shift(3*MAX_SHIFT)
+    size_t o_fn = 0, o_fs = o_fn + 3 * (size_t)nf, o_fc = o_fs + nf, o_cc = o_fc + 3 * (size_t)nf;
+    size_t o_gc = o_cc + 3 * (size_t)n, o_lsq = o_gc + 3 * (size_t)nb, o_eps = o_lsq + 6 * (size_t)n;
+    size_t o_sh = o_eps + n, tot = o_sh + 3 * MAX_SHIFT;
+    h.assign(tot, R(0));
+    for (size_t i = 0; i < 3 * (size_t)nf; ++i) { h[o_fn + i] = (R)c->fn[i]; h[o_fc + i] = (R)c->fc[i]; }
+    for (int i = 0; i < nf; ++i) h[o_fs + i] = (R)c->fs[i];
+    for (size_t i = 0; i < 3 * (size_t)n; ++i) h[o_cc + i] = (R)c->cc[i];
+    for (size_t i = 0; i < 3 * (size_t)nb; ++i) h[o_gc + i] = (R)c->gc[i];
+    for (int i = 0; i < 3 * MAX_SHIFT; ++i) h[o_sh + i] = (R)c->shift[i];
+    // LSQ matrices (precalc_grad_lsq.f90:60-155) and their Cholesky factors, in FP64
+    const bool w = c->sch.gradmeth == 12;
+    const double K = c->sch.venkat_k;
+    for (int i = 0; i < n; ++i) {
+        double A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double x[3] = {c->cc[i], c->cc[(size_t)n + i], c->cc[(size_t)2 * n + i]};
+        for (int k = 0; k < c->ns; ++k) {
+            int wd = c->fw[(size_t)k * n + i];
+            if (wd == SLOT_UNUSED) continue;
+            int nbv = c->nbr[(size_t)k * n + i];
+            double d[3];
+            if (nbv >= 0) {
+                int sid = ((unsigned)wd >> FACE_BITS) & (MAX_SHIFT - 1);
+                bool other = wd < 0;
+                const double* T = &c->shift[3 * sid];
+                for (int e = 0; e < 3; ++e) {
+                    double xo = c->cc[(size_t)e * n + nbv];
+                    d[e] = other ? (x[e] + T[e]) - xo : (xo + T[e]) - x[e];
+                }
+            } else {
+                int bi = -nbv - 1;
+                for (int e = 0; e < 3; ++e) d[e] = c->gc[(size_t)e * nb + bi] - x[e];
+            }
+            if (w) {
+                double s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                for (int e = 0; e < 3; ++e) d[e] /= s;
+            }
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) A[a + 3 * b] += d[a] * d[b];
+        }
+        if (dim == 2) A[8] += 1.0;
+        cholesky3(A);
+        // l11 l21 l31 l22 l32 l33
+        const double L[6] = {A[0], A[1], A[2], A[4], A[5], A[8]};
+        for (int k = 0; k < 6; ++k) h[o_lsq + (size_t)k * n + i] = (R)L[k];
+        const double h3 = dim == 3 ? c->vol[i] : std::pow(c->vol[i], 1.5);
+        h[o_eps + i] = (R)(K * K * K * h3);
+    }
+    R* d = nullptr;
+    int rc = dalloc(c, &d, tot);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(d, h.data(), tot * sizeof(R), hipMemcpyHostToDevice));
+    g.fnx = d + o_fn;
+    g.fny = d + o_fn + nf;
+    g.fnz = d + o_fn + 2 * (size_t)nf;
+    g.fs = d + o_fs;
+    g.fcx = d + o_fc;
+    g.fcy = d + o_fc + nf;
+    g.fcz = d + o_fc + 2 * (size_t)nf;
+    g.ccx = d + o_cc;
+    g.ccy = d + o_cc + n;
+    g.ccz = d + o_cc + 2 * (size_t)n;
+    g.gcx = d + o_gc;
+    g.gcy = d + o_gc + nb;
+    g.gcz = d + o_gc + 2 * (size_t)nb;
+    g.lsq = d + o_lsq;
+    g.eps2 = d + o_eps;
+    g.shift = d + o_sh;
+    g.bboco = c->d_bboco;
+    return TY_OK;
+}
+
+}  // namespace
+
+int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
+{
+    if (!c || !gas || !s) return TY_ERR_ARG;
+    if (!c->mesh_ok) return fail(c, TY_ERR_ARG, "ty_set_scheme before ty_mesh_upload");
+    if (s->nstage < 1 || s->nstage > MAX_STAGES || !s->rk_coef) return fail(c, TY_ERR_ARG, "bad RK tableau");
+    const int sch = s->sch_hyp;
+    if (sch != 5 && sch != 10 && sch != 21 && sch != 22 && sch != 25 && sch != 26)
+        return fail(c, TY_ERR_UNSUPPORTED, "numerical flux " + std::to_string(sch) + " not provided");
+    if (!strchr("NMFU", s->method) || s->method == 0) return fail(c, TY_ERR_UNSUPPORTED, "high-resolution method");
+    if (!strchr("XMAVSKLW", s->limiter) || s->limiter == 0) return fail(c, TY_ERR_UNSUPPORTED, "limiter");
+    if (s->postlimiter != 0 && s->postlimiter != 20 && s->postlimiter != 21)
+        return fail(c, TY_ERR_UNSUPPORTED, "post-limiter");
+    if (s->limiter == 'W' && s->postlimiter != 0)
+        return fail(c, TY_ERR_UNSUPPORTED, "Venkatakrishnan limiter with a post-limiter");
+    const bool viscous = gas->typ_fluid == 15;
+    if (gas->typ_fluid != 10 && !viscous) return fail(c, TY_ERR_UNSUPPORTED, "fluid model");
+    if (viscous && s->sch_dis != 5 && s->sch_dis != 10)
+        return fail(c, TY_ERR_UNSUPPORTED, "dissipative flux scheme");
+    if ((s->method != 'N' || viscous) && s->gradmeth != 10 && s->gradmeth != 12)
+        return fail(c, TY_ERR_UNSUPPORTED, "gradient method (LSQ or W-LSQ required)");
+    if (s->precision != TY_PREC_FP64 && s->precision != TY_PREC_MIXED) return fail(c, TY_ERR_ARG, "precision");
+    hipSetDevice(c->device);
+    // keep the mesh buffers, drop the rest
+    hipStreamSynchronize(c->stream);
+    {
+        std::vector<void*> keep;
+        for (void* p : c->owned) {
+            if (p == c->d_nbr || p == c->d_fw || p == c->d_bboco || p == c->d_vol) keep.push_back(p);
+            else hipFree(p);
+        }
+        c->owned = keep;
+    }
+    c->gas = *gas;
+    c->sch = *s;
+    const int ns = s->nstage;
+    c->rk.assign((size_t)ns * ns, 0.0);
+    for (int i = 0; i < ns; ++i)
+        for (int j = 0; j < ns; ++j) c->rk[(size_t)i * ns + j] = s->rk_coef[(size_t)j * ns + i];
+    c->sch.rk_coef = nullptr;
+    c->nv = c->dim + 2;
+    c->mixed = s->precision == TY_PREC_MIXED;
+    Physics& ph = c->ph;
+    ph.gamma = gas->gamma;
+    ph.rgas = gas->r_const;
+    ph.prandtl = gas->prandtl;
+    ph.visc_law = gas->typ_visc;
+    ph.visc_dyn = gas->visc_dyn;
+    ph.visc_kin = gas->visc_kin;
+    ph.tref = gas->tref;
+    ph.tsuth = gas->tsuth;
+    ph.viscous = viscous;
+    ph.sch_dis = s->sch_dis;
+    ph.sch_hyp = sch;
+    ph.method = s->method;
+    ph.limiter = s->method == 'N' ? 'X' : s->limiter;
+    ph.lsqw = s->gradmeth == 12;
+    ph.barth = s->method != 'N' && s->limiter != 'W' && s->postlimiter != 0;
+    ph.klim = s->postlimiter == 21 ? 1.0 : 0.5;
+    // boundary table: supersonic inlet state from pi_ti_mach_dir2nspri (EQNS.f90:139-160)
+    std::vector<BocoDev> tab(std::max<size_t>(1, c->bocos.size()));
+    for (size_t b = 0; b < c->bocos.size(); ++b) {
+        const ty_boco_desc& d = c->bocos[b];
+        BocoDev& t = tab[b];
+        memset(&t, 0, sizeof(t));
+        switch (d.type) {
+        case 10: t.kind = BK_SYM; break;
+        case 35: t.kind = BK_OUTLET_SUB; t.param[0] = d.param[0]; break;
+        case 36: t.kind = BK_OUTLET_SUP; break;
+        case 31: {
+            t.kind = BK_INLET_SUP;
+            const double g = gas->gamma, g1 = g - 1.0, r = gas->r_const;
+            double dx = d.param[3], dy = d.param[4], dz = d.param[5];
+            const double dn = std::sqrt(dx * dx + dy * dy + dz * dz);
+            dx /= dn; dy /= dn; dz /= dn;
+            const double mach = d.param[2];
+            const double fm = 1.0 / (1.0 + .5 * g1 * mach * mach);
+            const double ts = d.param[1] * fm;
+            const double a = std::sqrt(g * r * ts);
+            const double p = d.param[0] * std::pow(fm, g / g1);
+            const double rho = p / (r * ts);
+            const double ma = mach * a;
+            t.param[0] = rho;
+            t.param[1] = p;
+            t.param[2] = ma * dx;
+            t.param[3] = ma * dy;
+            if (c->dim == 3) t.param[4] = ma * dz;
+            break;
+        }
+        default: t.kind = BK_OUTLET_SUP;
+        }
+    }
+    int rc;
+    if ((rc = upload(c, &c->d_boco, tab))) return rc;
+    if (c->mixed) rc = build_geo<float>(c, c->geo_f);
+    else rc = build_geo<double>(c, c->geo_d);
+    if (rc) return rc;
+    std::vector<double> dtfac(c->n);
+    for (int i = 0; i < c->n; ++i) dtfac[i] = s->cfl * 2.0 * c->vol[i] / c->sumA[i];
+    if ((rc = upload(c, &c->d_dtfac, dtfac))) return rc;
+    // RK storage plan: R_s is kept if a later non-final stage state uses it off the diagonal
+    c->nstore = 0;
+    const size_t cells = (size_t)c->nv * c->n;
+    int nstore = 0;
+    for (int sidx = 1; sidx <= ns; ++sidx)
+        for (int t = sidx + 1; t < ns; ++t)
+            if (c->rk[(size_t)(t - 1) * ns + (sidx - 1)] != 0.0) { ++nstore; break; }
+    c->nstore = nstore;
+    const size_t rb = c->mixed ? sizeof(float) : sizeof(double);
+    void* p = nullptr;
+    if ((rc = dalloc(c, &c->d_u0, cells)) || (rc = dalloc(c, &c->d_acc, cells)) ||
+        (rc = dalloc(c, &c->d_rstore, cells * std::max(1, nstore))) || (rc = dalloc(c, &c->d_res, cells)))
+        return rc;
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(c, hipMalloc(&p, cells * rb));
+        c->owned.push_back(p);
+        c->d_us[k] = p;
+    }
+    HIPCHK(c, hipMalloc(&p, (size_t)c->nv * c->dim * c->n * rb));
+    c->owned.push_back(p);
+    c->d_grad = p;
+    HIPCHK(c, hipMalloc(&p, (size_t)3 * c->n * rb));
+    c->owned.push_back(p);
+    c->d_ratio = p;
+    const int nblk = grid_blocks(c->n);
+    if ((rc = dalloc(c, &c->d_part, nblk)) || (rc = dalloc(c, &c->d_bad, nblk)) || (rc = dalloc(c, &c->d_dt, 1)) ||
+        (rc = dalloc(c, &c->d_err, 1)))
+        return rc;
+    HIPCHK(c, hipMemset(c->d_err, 0, sizeof(int)));
+    c->cur = 0;
+    c->scheme_ok = true;
+    c->state_ok = false;
+    return TY_OK;
+}
+
+int ty_state_set(ty_ctx* c, const double* rho, const double* rhoE, const double* mom)
+{
+    if (!c || !rho || !rhoE || !mom) return TY_ERR_ARG;
+    if (!c->scheme_ok) return fail(c, TY_ERR_ARG, "ty_state_set before ty_set_scheme");
+    hipSetDevice(c->device);
+    const int n = c->n, nv = c->nv;
+    std::vector<double> h((size_t)nv * n);
+    for (int i = 0; i < n; ++i) {
+        int o = c->perm[i];
+        h[i] = rho[o];
+        h[(size_t)n + i] = rhoE[o];
+        for (int d = 0; d < c->dim; ++d) h[(size_t)(2 + d) * n + i] = mom[(size_t)3 * o + d];
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_u0, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->cur = 0;
+    hipError_t e = c->mixed ? launch_cast<float>(c->d_u0, (float*)c->d_us[0], (long)nv * n, c->stream)
+                            : launch_cast<double>(c->d_u0, (double*)c->d_us[0], (long)nv * n, c->stream);
+    HIPCHK(c, e);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->state_ok = true;
+    return TY_OK;
+}
+
+int ty_state_get(ty_ctx* c, double* rho, double* rhoE, double* mom)
+{
+    if (!c || !rho || !rhoE || !mom) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    hipSetDevice(c->device);
+    const int n = c->n, nv = c->nv;
+    std::vector<double> h((size_t)nv * n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_u0, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) {
+        int o = c->perm[i];
+        rho[o] = h[i];
+        rhoE[o] = h[(size_t)n + i];
+        for (int d = 0; d < 3; ++d) mom[(size_t)3 * o + d] = d < c->dim ? h[(size_t)(2 + d) * n + i] : 0.0;
+    }
+    return TY_OK;
+}
+
+namespace {
+
+template <typename R>
+KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
+{
+    KArgs<R> a;
+    memset(&a, 0, sizeof(a));
+    a.n = c->n;
+    a.ns = c->ns;
+    a.nbr = c->d_nbr;
+    a.fw = c->d_fw;
+    a.g = g;
+    a.boco = c->d_boco;
+    a.vol = c->d_vol;
+    a.dtfac = c->d_dtfac;
+    a.ph = c->ph;
+    a.us = (const R*)c->d_us[c->cur];
+    a.us_next = (R*)c->d_us[c->cur ^ 1];
+    a.u0 = c->d_u0;
+    a.acc = c->d_acc;
+    a.rstore = c->d_rstore;
+    a.grad = (const R*)c->d_grad;
+    a.grad_out = (R*)c->d_grad;
+    a.ratio = (const R*)c->d_ratio;
+    a.ratio_out = (R*)c->d_ratio;
+    a.res_out = c->d_res;
+    a.d_dt = c->d_dt;
+    a.store_slot = -1;
+    for (int j = 0; j < MAX_STAGES; ++j) a.read_slot[j] = -1;
+    return a;
+}
+
+template <typename R>
+hipError_t timed(ty_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>& ev, hipError_t (*fn)(const KArgs<R>&, hipStream_t),
+                 const KArgs<R>& a)
+{
+    if (!c->timing) return fn(a, c->stream);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, c->stream);
+    hipError_t e = fn(a, c->stream);
+    hipEventRecord(e1, c->stream);
+    ev.push_back({e0, e1});
+    return e;
+}
+
+// residual kernels of one evaluation on the current stage state
+template <int DIM, typename R>
+hipError_t eval_kernels(ty_ctx* c, KArgs<R>& a)
+{
+    hipError_t e = hipSuccess;
+    const bool grad = c->ph.method != 'N' || c->ph.viscous;
+    if (grad) {
+        e = timed<R>(c, c->ev_grad, &launch_gradient<DIM, R>, a);
+        if (e != hipSuccess) return e;
+        if (c->ph.barth) {
+            e = launch_barth<DIM, R>(a, c->stream);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return timed<R>(c, c->ev_flux, &launch_flux<DIM, R>, a);
+}
+
+template <int DIM, typename R>
+hipError_t rk_stages(ty_ctx* c, const Geo<R>& g)
+{
+    const int ns = c->sch.nstage;
+    auto coef = [&](int i, int j) { return c->rk[(size_t)(i - 1) * ns + (j - 1)]; };
+    // storage slots
+    int slot_of[MAX_STAGES + 1];
+    int nslot = 0;
+    for (int s = 1; s <= ns; ++s) {
+        slot_of[s] = -1;
+        for (int t = s + 1; t < ns; ++t)
+            if (coef(t, s) != 0.0) { slot_of[s] = nslot++; break; }
+    }
+    for (int s = 1; s <= ns; ++s) {
+        KArgs<R> a = make_args<R>(c, g);
+        a.mode = 0;
+        a.stage = s;
+        a.nstage = ns;
+        a.b = coef(ns, s);
+        a.store_slot = slot_of[s];
+        if (s < ns) {
+            for (int j = 1; j <= s; ++j) a.a[j - 1] = coef(s, j);
+            for (int j = 1; j < s; ++j) a.read_slot[j - 1] = coef(s, j) != 0.0 ? slot_of[j] : -1;
+        }
+        hipError_t e = eval_kernels<DIM, R>(c, a);
+        if (e != hipSuccess) return e;
+        c->cur ^= 1;
+    }
+    return hipSuccess;
+}
+
+int run_step(ty_ctx* c)
+{
+    hipError_t e;
+    if (c->mixed) e = c->dim == 2 ? rk_stages<2, float>(c, c->geo_f) : rk_stages<3, float>(c, c->geo_f);
+    else e = c->dim == 2 ? rk_stages<2, double>(c, c->geo_d) : rk_stages<3, double>(c, c->geo_d);
+    if (e != hipSuccess) return fail(c, TY_ERR_HIP, std::string("RK stage launch: ") + hipGetErrorString(e));
+    return TY_OK;
+}
+
+int launch_dt(ty_ctx* c, double dtmax)
+{
+    DtArgs d;
+    d.n = c->n;
+    d.u0 = c->d_u0;
+    d.dtfac = c->d_dtfac;
+    d.gamma = c->gas.gamma;
+    d.cfl = c->sch.cfl;
+    d.part = c->d_part;
+    d.bad = c->d_bad;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, c->stream);
+    }
+    hipError_t e = c->dim == 2 ? launch_timestep<2>(d, dtmax, c->d_dt, c->d_err, c->stream)
+                               : launch_timestep<3>(d, dtmax, c->d_dt, c->d_err, c->stream);
+    if (c->timing) {
+        hipEventRecord(e1, c->stream);
+        c->ev_dt.push_back({e0, e1});
+    }
+    if (e != hipSuccess) return fail(c, TY_ERR_HIP, std::string("time step launch: ") + hipGetErrorString(e));
+    return TY_OK;
+}
+
+int check_err(ty_ctx* c)
+{
+    int h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (h) {
+        hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream);
+        hipStreamSynchronize(c->stream);
+        return fail(c, TY_ERR_STATE, "negative or NaN internal energy (calc_ns_timestep)");
+    }
+    return TY_OK;
+}
+
+}  // namespace
+
+int ty_timestep(ty_ctx* c, double dtmax, double* dt)
+{
+    if (!c || !dt) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    hipSetDevice(c->device);
+    int rc = launch_dt(c, dtmax);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(&c->h_dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    rc = check_err(c);
+    if (rc) return rc;
+    *dt = c->h_dt;
+    return TY_OK;
+}
+
+int ty_residual(ty_ctx* c, double dt, double* res_rho, double* res_rhoE, double* res_mom)
+{
+    if (!c || !res_rho || !res_rhoE || !res_mom) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    hipSetDevice(c->device);
+    c->h_dt = dt;
+    HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipError_t e;
+    if (c->mixed) {
+        KArgs<float> a = make_args<float>(c, c->geo_f);
+        a.mode = 1;
+        e = c->dim == 2 ? eval_kernels<2, float>(c, a) : eval_kernels<3, float>(c, a);
+    } else {
+        KArgs<double> a = make_args<double>(c, c->geo_d);
+        a.mode = 1;
+        e = c->dim == 2 ? eval_kernels<2, double>(c, a) : eval_kernels<3, double>(c, a);
+    }
+    HIPCHK(c, e);
+    const int n = c->n, nv = c->nv;
+    std::vector<double> h((size_t)nv * n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_res, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) {
+        int o = c->perm[i];
+        res_rho[o] = h[i];
+        res_rhoE[o] = h[(size_t)n + i];
+        for (int d = 0; d < 3; ++d) res_mom[(size_t)3 * o + d] = d < c->dim ? h[(size_t)(2 + d) * n + i] : 0.0;
+    }
+    return TY_OK;
+}
+
+int ty_rk_step(ty_ctx* c, double dt)
+{
+    if (!c) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    hipSetDevice(c->device);
+    c->h_dt = dt;
+    HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = run_step(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TY_OK;
+}
+
+int ty_advance(ty_ctx* c, int32_t nsteps, double dtmax, double* dt_last)
+{
+    if (!c || nsteps < 0) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    hipSetDevice(c->device);
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = launch_dt(c, dtmax);
+        if (rc) return rc;
+        rc = run_step(c);
+        if (rc) return rc;
+    }
+    if (dt_last) {
+        HIPCHK(c, hipMemcpyAsync(&c->h_dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        int rc = check_err(c);
+        if (rc) return rc;
+        *dt_last = c->h_dt;
+    }
+    return TY_OK;
+}
+
+int ty_sync(ty_ctx* c)
+{
+    if (!c) return TY_ERR_ARG;
+    hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->d_err) return check_err(c);
+    return TY_OK;
+}
+
+// Per-kernel timing for the benchmark harness: when enabled, every gradient,
+// flux and time-step launch is bracketed by HIP events on the context stream.
+int ty_timing(ty_ctx* c, int32_t enable, double* ms, int32_t* count)
+{
+    if (!c) return TY_ERR_ARG;
+    hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<std::pair<hipEvent_t, hipEvent_t>>* lists[3] = {&c->ev_grad, &c->ev_flux, &c->ev_dt};
+    for (int k = 0; k < 3; ++k) {
+        double tot = 0.0;
+        for (auto& e : *lists[k]) {
+            float t = 0.f;
+            hipEventElapsedTime(&t, e.first, e.second);
+            tot += t;
+            hipEventDestroy(e.first);
+            hipEventDestroy(e.second);
+        }
+        if (ms) ms[k] = tot;
+        if (count) count[k] = (int32_t)lists[k]->size();
+        lists[k]->clear();
+    }
+    c->timing = enable != 0;
+    return TY_OK;
+}

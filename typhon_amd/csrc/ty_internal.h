@@ -1,0 +1,116 @@
+// Internal declarations shared by the HIP kernels (ty_kernels.hip) and the
+// host side of the C-ABI (ty_host.cpp).  gfx950 only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ty {
+
+constexpr int NVMAX = 5;         // conserved variables (rho, rhoE, rho u, rho v, rho w)
+constexpr int MAX_SLOTS = 8;     // faces per cell
+constexpr int MAX_STAGES = 8;    // RK stages
+constexpr int FACE_BITS = 26;    // face id bits of a slot word
+constexpr uint32_t FACE_MASK = (1u << FACE_BITS) - 1u;
+constexpr int SHIFT_BITS = 5;    // periodic shift id bits
+constexpr int MAX_SHIFT = 1 << SHIFT_BITS;
+constexpr int SLOT_UNUSED = -1;  // slot word of an empty slot
+constexpr int BLOCK = 256;       // threads per block (4 wavefronts of 64)
+constexpr int MAX_BOCO = 64;
+
+// boundary kinds handled on the device (the reference's bc_* codes map onto these)
+enum BcKind : int { BK_SYM = 0, BK_INLET_SUP = 1, BK_OUTLET_SUB = 2, BK_OUTLET_SUP = 3 };
+
+struct BocoDev {
+    int kind;
+    int pad;
+    double param[8];  // BK_INLET_SUP: precomputed ghost primitive state (rho, p, u, v, w);
+                      // BK_OUTLET_SUB: param[0] = imposed static pressure
+};
+
+// Device geometry in the arithmetic precision R.  Face arrays are indexed by the
+// device face id, cell arrays by the RCM cell index, boundary arrays by the
+// boundary face index.  Normals point from the face's owner (the reference's
+// ic1 side) to the other side.
+template <typename R>
+struct Geo {
+    const R *fnx, *fny, *fnz, *fs;  // [nf]
+    const R *fcx, *fcy, *fcz;       // [nf] face centre in the owner's frame
+    const R *ccx, *ccy, *ccz;       // [n]
+    const R *gcx, *gcy, *gcz;       // [nb] ghost centroid (init_ustboco_ghost*.f90)
+    const int* bboco;               // [nb] boco index
+    const R* lsq;                   // [6][n] Cholesky factors l11 l21 l31 l22 l32 l33
+    const R* eps2;                  // [n] Venkatakrishnan (K h)^3
+    const R* shift;                 // [MAX_SHIFT][3] periodic translation of the other side
+};
+
+struct Physics {
+    double gamma, rgas, prandtl;
+    int visc_law;
+    double visc_dyn, visc_kin, tref, tsuth;
+    int viscous;
+    int sch_dis;
+    int sch_hyp;
+    char method;   // 'N','M','F','U'
+    char limiter;  // 'X','M','A','V','S','K','L','W'
+    int lsqw;      // weighted LSQ
+    int barth;     // post-limiter on
+    double klim;
+};
+
+// Everything one launch of the residual kernels needs, passed by value.
+template <typename R>
+struct KArgs {
+    int n, ns, nblk_real;
+    const int* nbr;  // [ns][n] >=0 neighbour cell, <0: -(boundary face + 1)
+    const int* fw;   // [ns][n] slot word: face id | shift id << 26 | other-side << 31; -1 unused
+    Geo<R> g;
+    const BocoDev* boco;
+    const double* vol;    // [n]
+    const double* dtfac;  // [n] 2 V / sum(A)
+    Physics ph;
+    // state
+    const R* us;          // [nv][n] stage state (conservative)
+    R* us_next;           // [nv][n]
+    double* u0;           // [nv][n]
+    double* acc;          // [nv][n]
+    double* rstore;       // [nstore][nv][n]
+    const R* grad;        // [nv*dim][n] (limited when the Venkatakrishnan limiter is on)
+    R* grad_out;
+    const R* ratio;       // [3][n] Barth ratios (rho, p, velocity)
+    R* ratio_out;
+    double* res_out;      // [nv][n] residual-only mode
+    const double* d_dt;   // device time step
+    // RK bookkeeping for this stage
+    int mode;             // 0: stage update, 1: residual only
+    int stage, nstage;
+    double a[MAX_STAGES];     // coef(stage, 1..stage) for the next stage state
+    double b;                 // coef(nstage, stage)
+    int store_slot;           // store R_stage into rstore[store_slot] (-1: no)
+    int read_slot[MAX_STAGES];// rstore slot of R_j, j < stage (-1: not stored / zero coefficient)
+};
+
+struct DtArgs {
+    int n, nblk_real;
+    const double* u0;
+    const double* dtfac;
+    double gamma, cfl;
+    double* part;  // [nblocks]
+    int* bad;      // [nblocks]
+};
+
+// launchers (ty_kernels.hip)
+template <int DIM, typename R>
+hipError_t launch_gradient(const KArgs<R>& a, hipStream_t s);
+template <int DIM, typename R>
+hipError_t launch_barth(const KArgs<R>& a, hipStream_t s);
+template <int DIM, typename R>
+hipError_t launch_flux(const KArgs<R>& a, hipStream_t s);
+template <int DIM>
+hipError_t launch_timestep(const DtArgs& a, double dtmax, double* d_dt, int* d_err, hipStream_t s);
+template <typename R>
+hipError_t launch_cast(const double* src, R* dst, long n, hipStream_t s);
+
+int grid_blocks(int n);  // XCD-padded block count
+
+}  // namespace ty

@@ -1,0 +1,1027 @@
+// HIP kernels of the residual path for gfx950 (MI355X).
+//
+// Layout: one work-item per interior cell (RCM order).  Each cell walks its
+// face slots (ELL, [slot][cell] so a wavefront's slot loads coalesce), gathers
+// the neighbour's state, and evaluates every face flux in the face's canonical
+// orientation (owner = the reference's ic1 side, geometry in the owner's frame),
+// so the two cells of a face compute bit-identical fluxes.  The residual is a
+// deterministic per-cell sum in slot order: no atomics, no face-flux array.
+// Fluxes and reconstruction follow the reference routines cited per function.
+#include "ty_internal.h"
+
+namespace ty {
+
+// AMD flang's MIN/MAX (a<b?a:b, a>b?a:b): second operand unless the first wins.
+template <typename R>
+__device__ __forceinline__ R fmin_(R a, R b) { return a < b ? a : b; }
+template <typename R>
+__device__ __forceinline__ R fmax_(R a, R b) { return a > b ? a : b; }
+
+__device__ __forceinline__ double dpow(double a, double b) { return pow(a, b); }
+__device__ __forceinline__ float dpow(float a, float b) { return powf(a, b); }
+
+int grid_blocks(int n)
+{
+    int nb = (n + BLOCK - 1) / BLOCK;
+    return (nb + 7) & ~7;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD
+// owns one contiguous range of RCM-ordered cells and its L2 sees the
+// neighbours of its own cells.
+__device__ __forceinline__ int xcd_block()
+{
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void load_prim(const R* __restrict__ us, int n, int c, R g1, R* q)
+{
+    // calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:47-52)
+    const R rho = us[c];
+    const R rhoE = us[n + c];
+    R vel[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) vel[d] = us[(size_t)(2 + d) * n + c] / rho;
+    R sq = vel[0] * vel[0];
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) sq += vel[d] * vel[d];
+    const R ec = R(.5) * rho * sq;
+    q[0] = rho;
+    q[1] = g1 * (rhoE - ec);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) q[2 + d] = vel[d];
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ R dotd(const R* a, const R* b)
+{
+    R s = a[0] * b[0];
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) s += a[d] * b[d];
+    return s;
+}
+
+// Boundary states (cell_state and face_state modes use the same map):
+// symmetry calcboco_ust_sym.f90:46-79, supersonic inlet setboco_ns_inlet_sup.f90
+// (state precomputed on the host from pi_ti_mach_dir2nspri, EQNS.f90:139-160),
+// subsonic outlet setboco_ns_outlet_sub.f90:57-95 (EQNS.f90:166-219),
+// supersonic outlet setboco_ns_outlet_sup.f90 (copy).
+template <int DIM, typename R>
+__device__ __forceinline__ void bc_state(const BocoDev& b, const R* n, const R* qi, R* qo, const Physics& ph)
+{
+    constexpr int NV = DIM + 2;
+    switch (b.kind) {
+    case BK_SYM: {
+        const R vn = dotd<DIM>(qi + 2, n);
+        const R k = R(2) * vn;
+        qo[0] = qi[0];
+        qo[1] = qi[1];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) qo[2 + d] = qi[2 + d] - k * n[d];
+        break;
+    }
+    case BK_INLET_SUP:
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qo[v] = R(b.param[v]);
+        break;
+    case BK_OUTLET_SUB: {
+        const R g = R(ph.gamma), g1 = g - R(1), r = R(ph.rgas);
+        R fmv = dotd<DIM>(qi + 2, qi + 2);
+        const R sq = sqrt(fmv);
+        R dir[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) dir[d] = qi[2 + d] / sq;
+        fmv = fmv / (g * qi[1]) * qi[0];
+        fmv = R(1) + R(.5) * g1 * fmv;
+        const R pi = qi[1] * dpow(fmv, g / g1);
+        const R ti = qi[1] / r / qi[0] * fmv;
+        const R ps = R(b.param[0]);
+        R fm = dpow(pi / ps, g1 / g);
+        if (fm < R(1)) fm = R(1);
+        const R rho = ps / (r * ti) * fm;
+        const R vm = sqrt((fm - R(1)) * R(2) / g1) * sqrt(g * ps / rho);
+        qo[0] = rho;
+        qo[1] = ps;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) qo[2 + d] = vm * dir[d];
+        break;
+    }
+    default:
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qo[v] = qi[v];
+    }
+}
+
+// ghost-cell gradients for the viscous flux (cell_grad mode): symmetry mirrors
+// (calcboco_ust_sym.f90:65-77 with calc_symn, VEC3D.f90:679-694), others copy.
+template <int DIM, typename R>
+__device__ __forceinline__ void bc_grad(const BocoDev& b, const R* n, const R* gi, R* go)
+{
+    constexpr int NV = DIM + 2;
+    if (b.kind != BK_SYM) {
+#pragma unroll
+        for (int k = 0; k < NV * DIM; ++k) go[k] = gi[k];
+        return;
+    }
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+        const R k2 = R(2) * dotd<DIM>(gi + v * DIM, n);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) go[v * DIM + d] = gi[v * DIM + d] - k2 * n[d];
+    }
+    R S[DIM][DIM], M[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) S[i][j] = (i == j ? R(1) : R(0)) - R(2) * n[i] * n[j];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) {
+            R s = R(0);
+#pragma unroll
+            for (int l = 0; l < DIM; ++l) s += gi[(2 + i) * DIM + l] * S[l][j];
+            M[i][j] = s;
+        }
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) {
+            R s = R(0);
+#pragma unroll
+            for (int l = 0; l < DIM; ++l) s += S[i][l] * M[l][j];
+            go[(2 + i) * DIM + j] = s;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// numerical fluxes per unit area, primitive inputs (rho, p, u..)
+
+template <int DIM, typename R>
+__device__ __forceinline__ void roe_avg(R g, const R* qL, const R* qR, R& rho, R* vel, R& p)
+{
+    // calc_roe_states (SOURCE/EQNS/calc_roe_states.f90:35-53)
+    const R gig1 = g / (g - R(1));
+    const R t1 = sqrt(qR[0] / qL[0]);
+    rho = qL[0] * t1;
+    const R kl = R(1) / (R(1) + t1);
+    const R kr = t1 * kl;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) vel[d] = kl * qL[2 + d] + kr * qR[2 + d];
+    const R hl = gig1 * qL[1] / qL[0] + R(.5) * dotd<DIM>(qL + 2, qL + 2);
+    const R hr = gig1 * qR[1] / qR[0] + R(.5) * dotd<DIM>(qR + 2, qR + 2);
+    const R h = kl * hl + kr * hr;
+    p = (h - R(.5) * dotd<DIM>(vel, vel)) * rho / gig1;
+}
+
+template <int DIM, typename R>
+__device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* F)
+{
+    const R ig1 = R(1) / (g - R(1));
+    const R vnl = dotd<DIM>(qL + 2, n);
+    const R vnr = dotd<DIM>(qR + 2, n);
+    const R rel = ig1 * qL[1] + R(.5) * qL[0] * dotd<DIM>(qL + 2, qL + 2);
+    const R rer = ig1 * qR[1] + R(.5) * qR[0] * dotd<DIM>(qR + 2, qR + 2);
+    if (sch == 5) {  // Rusanov: calc_flux_rusanov.f90:64-83
+        const R al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
+        const R ray = fmax_(fabs(vnl) + al, fabs(vnr) + ar);
+        F[0] = R(.5) * ((vnl + ray) * qL[0] + (vnr - ray) * qR[0]);
+        F[1] = R(.5) * ((vnl + ray) * rel + (vnr - ray) * rer + vnl * qL[1] + vnr * qR[1]);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            F[2 + d] = R(.5) * (((vnl + ray) * qL[0]) * qL[2 + d] + ((vnr - ray) * qR[0]) * qR[2 + d]
+                                + (qL[1] + qR[1]) * n[d]);
+        return;
+    }
+    R rrho, rv[DIM], rp;
+    roe_avg<DIM>(g, qL, qR, rrho, rv, rp);
+    const R vm = dotd<DIM>(rv, n);
+    const R am = sqrt(g * rp / rrho);
+    if (sch == 21 || sch == 22) {  // HLLE(B): calc_flux_hlle.f90:109-143
+        const R beta = (sch == 21) ? R(1) : sqrt((g - R(1)) / (R(2) * g));
+        const R al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
+        const R sl = fmin_(fmin_(R(0), vnl - beta * al), vm - am);
+        const R sr = fmax_(fmax_(R(0), vnr + beta * ar), vm + am);
+        const R iks = R(1) / (sr - sl);
+        const R kl = sr * iks, kr = -sl * iks, ku = sl * sr * iks;
+        F[0] = (kl * vnl - ku) * qL[0] + (kr * vnr + ku) * qR[0];
+        F[1] = (kl * vnl - ku) * rel + (kr * vnr + ku) * rer + (kl * vnl * qL[1] + kr * vnr * qR[1]);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            F[2 + d] = ((kl * vnl - ku) * qL[0]) * qL[2 + d] + ((kr * vnr + ku) * qR[0]) * qR[2 + d]
+                       + (kl * qL[1] + kr * qR[1]) * n[d];
+        return;
+    }
+    if (sch == 10) {  // Roe with Harten entropy fix (the reference parses ROE but stops on it)
+        const R H = (g / (g - R(1))) * rp / rrho + R(.5) * dotd<DIM>(rv, rv);
+        const R a = am;
+        const R drho = qR[0] - qL[0], dp = qR[1] - qL[1], dvn = vnr - vnl;
+        R l1 = fabs(vm - a), l2 = fabs(vm), l3 = fabs(vm + a);
+        const R del = R(0.1) * a;
+        if (l1 < del) l1 = R(.5) * (l1 * l1 / del + del);
+        if (l3 < del) l3 = R(.5) * (l3 * l3 / del + del);
+        const R w1 = l1 * (dp - rrho * a * dvn) / (R(2) * a * a);
+        const R w2 = l2 * (drho - dp / (a * a));
+        const R w3 = l3 * (dp + rrho * a * dvn) / (R(2) * a * a);
+        const R w4 = l2 * rrho;
+        R dvt[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) dvt[d] = (qR[2 + d] - qL[2 + d]) - dvn * n[d];
+        F[0] = R(.5) * (qL[0] * vnl + qR[0] * vnr - (w1 + w2 + w3));
+        F[1] = R(.5) * (vnl * (rel + qL[1]) + vnr * (rer + qR[1])
+                        - (w1 * (H - vm * a) + w2 * R(.5) * dotd<DIM>(rv, rv) + w3 * (H + vm * a)
+                           + w4 * dotd<DIM>(rv, dvt)));
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            F[2 + d] = R(.5) * (qL[0] * vnl * qL[2 + d] + qL[1] * n[d] + qR[0] * vnr * qR[2 + d] + qR[1] * n[d]
+                                - (w1 * (rv[d] - a * n[d]) + w2 * rv[d] + w3 * (rv[d] + a * n[d]) + w4 * dvt[d]));
+        return;
+    }
+    // HLLC (Batten), HLLCB: calc_flux_hllc.f90:61-171
+    const R beta = (sch == 26) ? sqrt((g - R(1)) / (R(2) * g)) : R(1);
+    const R al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
+    const R sl = fmin_(vnl - beta * al, vm - am);
+    const R sr = fmax_(vnr + beta * ar, vm + am);
+    if (sl >= R(0)) {
+        F[0] = vnl * qL[0];
+        F[1] = vnl * (rel + qL[1]);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) F[2 + d] = (vnl * qL[0]) * qL[2 + d] + qL[1] * n[d];
+    } else if (sr <= R(0)) {
+        F[0] = vnr * qR[0];
+        F[1] = vnr * (rer + qR[1]);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) F[2 + d] = (vnr * qR[0]) * qR[2 + d] + qR[1] * n[d];
+    } else {
+        const R rqL = qL[0] * (sl - vnl), rqR = qR[0] * (sr - vnr);
+        const R sst = (rqR * vnr - rqL * vnl - qR[1] + qL[1]) / (rqR - rqL);
+        // star state of the side of S* (calc_flux_hllc.f90:149-161)
+        const bool left = sst >= R(0);
+        const R* q = left ? qL : qR;
+        const R rq = left ? rqL : rqR;
+        const R vn = left ? vnl : vnr;
+        const R s = left ? sl : sr;
+        const R re = left ? rel : rer;
+        const R iks = R(1) / (s - sst);
+        const R rst = rq * iks;
+        const R pst = q[1] - rq * (vn - sst);
+        const R rest = iks * (re * (s - vn) - q[1] * vn + pst * sst);
+        F[0] = sst * rst;
+        F[1] = sst * (rest + pst);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const R rvst = iks * ((rq * q[2 + d]) + (pst - q[1]) * n[d]);
+            F[2 + d] = (sst * rvst) + (pst * n[d]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reconstruction: limiters (SOURCE/MGRID/LIMITER.f90:72-339)
+
+template <typename R>
+__device__ __forceinline__ R limiter(char lim, R x1, R x2)
+{
+    R r = R(0);
+    switch (lim) {
+    case 'M': {
+        const R s = R(.5) * (copysign(R(1), x1) + copysign(R(1), x2));
+        r = s * fmin_(fabs(x1), fabs(x2));
+        break;
+    }
+    case 'A': {
+        const R s = x1 * x2;
+        r = (s <= R(0)) ? R(0) : s * (x1 + x2) / (x1 * x1 + x2 * x2);
+        break;
+    }
+    case 'V': {
+        const R s = x1 * x2;
+        r = (s <= R(0)) ? R(0) : R(2) * s / (x1 + x2);
+        break;
+    }
+    case 'S':
+        if (x1 > R(0) && x2 > R(0)) r = fmax_(fmin_(R(2) * x1, x2), fmin_(R(2) * x2, x1));
+        if (x1 < R(0) && x2 < R(0)) r = fmin_(fmax_(R(2) * x1, x2), fmax_(R(2) * x2, x1));
+        break;
+    case 'K':
+        if (x1 > R(0) && x2 > R(0)) { r = (R(2) * x1 + x2) / R(3); r = fmin_(fmin_(R(2) * x1, R(2) * x2), r); }
+        if (x1 < R(0) && x2 < R(0)) { r = (R(2) * x1 + x2) / R(3); r = fmax_(fmax_(R(2) * x1, R(2) * x2), r); }
+        break;
+    case 'L': {
+        const R eta = (x1 * x1 + x2 * x2) / R(0.01) / R(0.01);
+        if (eta < R(1)) r = (R(2) * x2 + x1) / R(3);
+        else if (x1 > R(0) && x2 > R(0)) { r = (R(2) * x2 + x1) / R(3); r = fmin_(fmin_(R(1.6) * x2, R(2) * x1), r); }
+        else if (x1 < R(0) && x2 < R(0)) { r = (R(2) * x2 + x1) / R(3); r = fmax_(fmax_(R(1.6) * x2, R(2) * x1), r); }
+        else if (x1 < R(0) && x2 > R(0)) { r = (R(2) * x2 + x1) / R(3); r = fmax_(R(0), fmin_(R(-0.5) * x1, r)); }
+        else if (x1 > R(0) && x2 < R(0)) { r = (R(2) * x2 + x1) / R(3); r = fmin_(R(0), fmax_(R(-0.5) * x1, r)); }
+        break;
+    }
+    default: break;
+    }
+    return r;
+}
+
+// One side's face value for the MUSCL family: hres_ns_muscl.f90:51-143 with
+// tvdgradstr_*.f90:35-58, hres_ns_musclfast.f90:51-144 with tvdgradfst_*.f90:35-60,
+// hres_ns_muscluns with tvdgraduns_*.f90:35-57.  g is this side's gradient of q,
+// u the (possibly unnormalised) L->R vector, lr the L->R slope, cf the cell->face
+// vector (MUSCL, MUSCL-UNS) and kcf its signed length (MUSCL-FAST).
+template <int DIM, typename R>
+__device__ __forceinline__ R muscl_side(char method, char lim, R q, const R* g, const R* u, R lr, const R* cf, R kcf)
+{
+    if (method == 'F') {
+        R s;
+        if (lim == 'X') s = lr;
+        else {
+            const R sp = dotd<DIM>(g, u);
+            s = (lim == 'K') ? limiter(lim, lr, R(2) * sp - lr) : limiter(lim, R(2) * sp - lr, lr);
+        }
+        return q + s * kcf;
+    }
+    const R s = dotd<DIM>(g, u);
+    R ds;
+    if (lim == 'X') ds = lr - s;
+    else if (method == 'M') ds = ((lim == 'K') ? limiter(lim, lr, R(2) * s - lr) : limiter(lim, R(2) * s - lr, lr)) - s;
+    else ds = ((lim == 'K') ? limiter(lim, lr, s) : limiter(lim, s, lr)) - s;
+    R gp[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) gp[d] = g[d] + ds * u[d];
+    return q + dotd<DIM>(gp, cf);
+}
+
+// Face states of one face in canonical orientation for the MUSCL family.
+// side: 0 left only, 1 right only, 2 both.
+template <int DIM, typename R>
+__device__ __forceinline__ void muscl_face(const Physics& ph, const R* qL, const R* qR, const R* gL, const R* gR,
+                                           const R* xL, const R* xR, const R* xf, R* qLf, R* qRf, int side)
+{
+    constexpr int NV = DIM + 2;
+    R u[DIM], LF[DIM], RF[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        u[d] = xR[d] - xL[d];
+        LF[d] = xf[d] - xL[d];
+        RF[d] = xf[d] - xR[d];
+    }
+    R dlr, kl = R(0), kr = R(0);
+    if (ph.method == 'F') {
+        kl = sqrt(dotd<DIM>(LF, LF));
+        kr = -sqrt(dotd<DIM>(RF, RF));
+        dlr = kl - kr;
+    } else {
+        dlr = sqrt(dotd<DIM>(u, u));
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) u[d] = u[d] / dlr;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const R lr = (qR[v] - qL[v]) / dlr;
+        if (side != 1) qLf[v] = muscl_side<DIM>(ph.method, ph.limiter, qL[v], gL + v * DIM, u, lr, LF, kl);
+        if (side != 0) qRf[v] = muscl_side<DIM>(ph.method, ph.limiter, qR[v], gR + v * DIM, u, lr, RF, kr);
+    }
+}
+
+// Venkatakrishnan smooth limiter function (J. Comput. Phys. 118 (1995) 120-130)
+template <typename R>
+__device__ __forceinline__ R venkat_phi(R dmax, R dmin, R d2, R eps2)
+{
+    if (d2 == R(0)) return R(1);
+    const R dm = (d2 > R(0)) ? dmax : dmin;
+    const R num = (dm * dm + eps2) + R(2) * d2 * dm;
+    const R den = dm * dm + R(2) * d2 * d2 + dm * d2 + eps2;
+    if (den <= R(0)) return R(1);
+    return fmin_(R(1), fmax_(R(0), num / den));
+}
+
+// ---------------------------------------------------------------------------
+// slot decoding and neighbour positions
+
+struct SlotInfo {
+    int nb, f, sid;
+    bool other;
+};
+
+__device__ __forceinline__ SlotInfo decode(int w, int nb)
+{
+    SlotInfo s;
+    s.nb = nb;
+    s.f = (int)((unsigned)w & FACE_MASK);
+    s.sid = (int)(((unsigned)w >> FACE_BITS) & (MAX_SHIFT - 1));
+    s.other = w < 0;
+    return s;
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void cell_x(const Geo<R>& g, int c, R* x)
+{
+    x[0] = g.ccx[c];
+    x[1] = g.ccy[c];
+    if (DIM == 3) x[DIM - 1] = g.ccz[c];
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void face_n(const Geo<R>& g, int f, R* n)
+{
+    n[0] = g.fnx[f];
+    n[1] = g.fny[f];
+    if (DIM == 3) n[DIM - 1] = g.fnz[f];
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void face_x(const Geo<R>& g, int f, R* x)
+{
+    x[0] = g.fcx[f];
+    x[1] = g.fcy[f];
+    if (DIM == 3) x[DIM - 1] = g.fcz[f];
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void load_grad(const R* __restrict__ grad, int n, int c, R* gv)
+{
+#pragma unroll
+    for (int k = 0; k < (DIM + 2) * DIM; ++k) gv[k] = grad[(size_t)k * n + c];
+}
+
+// ---------------------------------------------------------------------------
+// K1: LSQ gradients (calc_gradient.f90:75-192 + cholesky_solve MATRIX.f90:79-111),
+// optionally limited by Venkatakrishnan.  Neighbour set = face neighbours incl.
+// ghost cells built by the cell_state boundary map.
+
+template <int DIM, typename R>
+__global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
+{
+    constexpr int NV = DIM + 2;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const int n = a.n;
+    const R g1 = R(a.ph.gamma - 1.0);
+    R qc[NV], xc[DIM];
+    load_prim<DIM>(a.us, n, c, g1, qc);
+    cell_x<DIM>(a.g, c, xc);
+    R b[NV][DIM];
+    R qmax[NV], qmin[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        qmax[v] = qc[v];
+        qmin[v] = qc[v];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) b[v][d] = R(0);
+    }
+    for (int k = 0; k < a.ns; ++k) {
+        const int w = a.fw[(size_t)k * n + c];
+        if (w == SLOT_UNUSED) continue;
+        const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
+        R d[DIM], dq[NV], qo[NV];
+        if (s.nb >= 0) {
+            load_prim<DIM>(a.us, n, s.nb, g1, qo);
+            R xo[DIM];
+            cell_x<DIM>(a.g, s.nb, xo);
+            const R* T = a.g.shift + 3 * s.sid;
+            // owner frame: d = x_other + T - x_owner, dq = q_other - q_owner
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) d[e] = s.other ? (xc[e] + T[e]) - xo[e] : (xo[e] + T[e]) - xc[e];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) dq[v] = s.other ? qc[v] - qo[v] : qo[v] - qc[v];
+        } else {
+            const int bi = -s.nb - 1;
+            const BocoDev bc = a.boco[a.g.bboco[bi]];
+            R nf[DIM];
+            face_n<DIM>(a.g, s.f, nf);
+            bc_state<DIM>(bc, nf, qc, qo, a.ph);
+            R xg[DIM];
+            xg[0] = a.g.gcx[bi];
+            xg[1] = a.g.gcy[bi];
+            if (DIM == 3) xg[DIM - 1] = a.g.gcz[bi];
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) d[e] = xg[e] - xc[e];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) dq[v] = qo[v] - qc[v];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            qmax[v] = fmax_(qmax[v], qo[v]);
+            qmin[v] = fmin_(qmin[v], qo[v]);
+        }
+        if (a.ph.lsqw) {
+            const R s2 = dotd<DIM>(d, d);
+            const R s4 = s2 * s2;
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) d[e] = d[e] / s4;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) b[v][e] += dq[v] * d[e];
+    }
+    // Cholesky solve with the factors of A (precalc_grad_lsq.f90:60-155)
+    const R l11 = a.g.lsq[c], l21 = a.g.lsq[(size_t)n + c], l22 = a.g.lsq[(size_t)3 * n + c];
+    R gr[NV][DIM];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        if (DIM == 3) {
+            const R l31 = a.g.lsq[(size_t)2 * n + c], l32 = a.g.lsq[(size_t)4 * n + c], l33 = a.g.lsq[(size_t)5 * n + c];
+            R y1 = b[v][0] / l11;
+            R y2 = (b[v][1] - l21 * y1) / l22;
+            R y3 = ((b[v][DIM - 1] - l31 * y1) - l32 * y2) / l33;
+            R x3 = y3 / l33;
+            R x2 = (y2 - l32 * x3) / l22;
+            R x1 = ((y1 - l21 * x2) - l31 * x3) / l11;
+            gr[v][0] = x1;
+            gr[v][1] = x2;
+            gr[v][DIM - 1] = x3;
+        } else {
+            R y1 = b[v][0] / l11;
+            R y2 = (b[v][1] - l21 * y1) / l22;
+            R x2 = y2 / l22;
+            R x1 = (y1 - l21 * x2) / l11;
+            gr[v][0] = x1;
+            gr[v][1] = x2;
+        }
+    }
+    if (a.ph.limiter == 'W') {
+        const R eps2 = a.g.eps2[c];
+        R phi[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) phi[v] = R(1);
+        for (int k = 0; k < a.ns; ++k) {
+            const int w = a.fw[(size_t)k * n + c];
+            if (w == SLOT_UNUSED) continue;
+            const SlotInfo s = decode(w, 0);
+            R xf[DIM], r[DIM];
+            face_x<DIM>(a.g, s.f, xf);
+            const R* T = a.g.shift + 3 * s.sid;
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) r[e] = (s.other ? xf[e] - T[e] : xf[e]) - xc[e];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const R d2 = dotd<DIM>(gr[v], r);
+                phi[v] = fmin_(phi[v], venkat_phi(qmax[v] - qc[v], qmin[v] - qc[v], d2, eps2));
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) gr[v][e] = phi[v] * gr[v][e];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) a.grad_out[(size_t)(v * DIM + e) * n + c] = gr[v][e];
+}
+
+// ---------------------------------------------------------------------------
+// shared per-face evaluation: left/right cell data in canonical orientation
+
+template <int DIM, typename R>
+struct FaceCtx {
+    R qL[DIM + 2], qR[DIM + 2];
+    R xL[DIM], xR[DIM], xf[DIM], n[DIM];
+    int owner, other;  // other < 0: boundary (ghost)
+    int bi;
+};
+
+template <int DIM, typename R>
+__device__ __forceinline__ void face_ctx(const KArgs<R>& a, int c, const R* qc, const R* xc, const SlotInfo& s,
+                                         FaceCtx<DIM, R>& F)
+{
+    constexpr int NV = DIM + 2;
+    const R g1 = R(a.ph.gamma - 1.0);
+    face_n<DIM>(a.g, s.f, F.n);
+    face_x<DIM>(a.g, s.f, F.xf);
+    if (s.nb >= 0) {
+        R qo[NV], xo[DIM];
+        load_prim<DIM>(a.us, a.n, s.nb, g1, qo);
+        cell_x<DIM>(a.g, s.nb, xo);
+        const R* T = a.g.shift + 3 * s.sid;
+        if (!s.other) {
+            F.owner = c;
+            F.other = s.nb;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { F.qL[v] = qc[v]; F.qR[v] = qo[v]; }
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) { F.xL[e] = xc[e]; F.xR[e] = xo[e] + T[e]; }
+        } else {
+            F.owner = s.nb;
+            F.other = c;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { F.qL[v] = qo[v]; F.qR[v] = qc[v]; }
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) { F.xL[e] = xo[e]; F.xR[e] = xc[e] + T[e]; }
+        }
+        F.bi = -1;
+    } else {
+        F.owner = c;
+        F.other = -1;
+        F.bi = -s.nb - 1;
+        const BocoDev bc = a.boco[a.g.bboco[F.bi]];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F.qL[v] = qc[v];
+        bc_state<DIM>(bc, F.n, qc, F.qR, a.ph);  // ghost cell state (cell_state mode)
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) F.xL[e] = xc[e];
+        F.xR[0] = a.g.gcx[F.bi];
+        F.xR[1] = a.g.gcy[F.bi];
+        if (DIM == 3) F.xR[DIM - 1] = a.g.gcz[F.bi];
+    }
+}
+
+// K2 (MUSCL family with a Barth post-limiter only): per-cell ratios
+// (postlimit_barth.f90:50-223), groups rho, p, velocity vector.
+template <int DIM, typename R>
+__global__ __launch_bounds__(BLOCK) void k_barth(KArgs<R> a)
+{
+    constexpr int NV = DIM + 2;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const int n = a.n;
+    const R g1 = R(a.ph.gamma - 1.0);
+    const R klim = R(a.ph.klim);
+    R qc[NV], xc[DIM], gc[NV * DIM];
+    load_prim<DIM>(a.us, n, c, g1, qc);
+    cell_x<DIM>(a.g, c, xc);
+    load_grad<DIM>(a.grad, n, c, gc);
+    R ratio[3] = {R(1), R(1), R(1)};
+    for (int k = 0; k < a.ns; ++k) {
+        const int w = a.fw[(size_t)k * n + c];
+        if (w == SLOT_UNUSED) continue;
+        const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
+        FaceCtx<DIM, R> F;
+        face_ctx<DIM>(a, c, qc, xc, s, F);
+        R qf[NV], dummy[NV];
+        const bool left = (F.owner == c) && !s.other;
+        if (left) muscl_face<DIM>(a.ph, F.qL, F.qR, gc, gc, F.xL, F.xR, F.xf, qf, dummy, 0);
+        else muscl_face<DIM>(a.ph, F.qL, F.qR, gc, gc, F.xL, F.xR, F.xf, dummy, qf, 1);
+        const R* qo = left ? F.qR : F.qL;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R l = (qo[v] - qc[v]) / (qf[v] - qc[v]);
+            const R lr = fmax_(klim * l, R(0));
+            const int gi = v < 2 ? v : 2;
+            ratio[gi] = fmin_(ratio[gi], lr);
+        }
+    }
+    // in 2D the reference also limits the (identically zero) z velocity: 0/0 -> max(NaN,0) = 0
+    if (DIM == 2) ratio[2] = fmin_(ratio[2], R(0));
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a.ratio_out[(size_t)i * n + c] = ratio[i];
+}
+
+// viscous flux of one face, calc_flux_viscous_cell.f90:75-158 (subtracted from F)
+template <int DIM, typename R>
+__device__ void viscous_face(const Physics& ph, const FaceCtx<DIM, R>& F, const R* gL, const R* gR, R* flux)
+{
+    const R r = R(ph.rgas);
+    const R cp = R(ph.gamma) * r / (R(ph.gamma) - R(1));
+    R a[DIM], b[DIM], vLR[DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) { a[e] = F.xf[e] - F.xL[e]; b[e] = F.xf[e] - F.xR[e]; vLR[e] = F.xR[e] - F.xL[e]; }
+    R dHL = sqrt(dotd<DIM>(a, a)), dHR = sqrt(dotd<DIM>(b, b));
+    const R id = R(1) / (dHL + dHR);
+    dHL = id * dHL;
+    dHR = id * dHR;
+    R velH[DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) velH[e] = dHR * F.qL[2 + e] + dHL * F.qR[2 + e];
+    const R TL = F.qL[1] / (F.qL[0] * r), TR = F.qR[1] / (F.qR[0] * r);
+    const R TH = dHR * TL + dHL * TR;
+    const R rhoH = dHR * F.qL[0] + dHL * F.qR[0];
+    R gTL[DIM], gTR[DIM];
+    const R kl1 = R(1) / (F.qL[0] * r), kl2 = F.qL[1] / (F.qL[0] * F.qL[0] * r);
+    const R kr1 = R(1) / (F.qR[0] * r), kr2 = F.qR[1] / (F.qR[0] * F.qR[0] * r);
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) {
+        gTL[e] = kl1 * gL[DIM + e] - kl2 * gL[e];
+        gTR[e] = kr1 * gR[DIM + e] - kr2 * gR[e];
+    }
+    R mu;
+    if (ph.visc_law == 11) mu = R(ph.visc_dyn) * dpow(TH / R(ph.tref), R(1.5)) * (R(ph.tref) + R(ph.tsuth)) / (TH + R(ph.tsuth));
+    else if (ph.visc_law == 14) mu = R(ph.visc_kin) * rhoH;
+    else mu = R(ph.visc_dyn);
+    R G[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) G[i][j] = dHL * gR[(2 + i) * DIM + j] + dHR * gL[(2 + i) * DIM + j];
+    const R dLR2 = dotd<DIM>(vLR, vLR);
+    R gavg[DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) gavg[e] = dHL * gTR[e] + dHR * gTL[e];
+    R gTH;
+    if (ph.sch_dis == 10) {
+        R kv[DIM];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) kv[e] = (R(1) / dLR2) * vLR[e];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            const R fa = F.qR[2 + i] - F.qL[2 + i] - dotd<DIM>(G[i], vLR);
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) G[i][j] = G[i][j] + fa * kv[j];
+        }
+        const R pscal = dotd<DIM>(vLR, F.n) / dLR2;
+        const R fcomp = pscal * (TR - TL);
+        R vi[DIM];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) vi[e] = F.n[e] - pscal * vLR[e];
+        gTH = fcomp + dotd<DIM>(gavg, vi);
+    } else {
+        gTH = dotd<DIM>(gavg, F.n);
+    }
+    R tr = G[0][0];
+#pragma unroll
+    for (int i = 1; i < DIM; ++i) tr += G[i][i];
+    const R addv = -(R(2) / R(3)) * tr;
+    R sn[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        R s = R(0);
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) {
+            R sij = G[i][j] + G[j][i];
+            if (i == j) sij += addv;
+            s += (mu * sij) * F.n[j];
+        }
+        sn[i] = s;
+    }
+    const R svn = dotd<DIM>(sn, velH);
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) flux[2 + e] = flux[2 + e] - sn[e];
+    flux[1] = flux[1] - svn;
+    const R conduct = mu * cp / R(ph.prandtl);
+    flux[1] = flux[1] - conduct * gTH;
+}
+
+// ---------------------------------------------------------------------------
+// K3: fluxes + deterministic gather + fused RK stage update.
+// calc_rhs.f90:48-145 (BC face_state: calcboco_gen bccon_face_state), integration_ns_ust.f90,
+// flux_to_res.f90:47-142, treelevel_rungekutta.f90:60-105, update_field.f90:38.
+
+template <int DIM, typename R, bool SECOND>
+__global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
+{
+    constexpr int NV = DIM + 2;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const int n = a.n;
+    const R g1 = R(a.ph.gamma - 1.0);
+    const R gam = R(a.ph.gamma);
+    R qc[NV], xc[DIM];
+    load_prim<DIM>(a.us, n, c, g1, qc);
+    cell_x<DIM>(a.g, c, xc);
+    R res[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) res[v] = R(0);
+    const bool venk = a.ph.limiter == 'W';
+    for (int k = 0; k < a.ns; ++k) {
+        const int w = a.fw[(size_t)k * n + c];
+        if (w == SLOT_UNUSED) continue;
+        const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
+        FaceCtx<DIM, R> F;
+        face_ctx<DIM>(a, c, qc, xc, s, F);
+        R qLf[NV], qRf[NV];
+        R gL[SECOND ? NV * DIM : 1], gR[SECOND ? NV * DIM : 1];
+        if (SECOND) {
+            load_grad<DIM>(a.grad, n, F.owner, gL);
+            if (F.other >= 0) load_grad<DIM>(a.grad, n, F.other, gR);
+        }
+        if (!SECOND || a.ph.method == 'N') {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { qLf[v] = F.qL[v]; qRf[v] = F.qR[v]; }
+        } else {
+            if (venk) {
+                R rl[DIM], rr[DIM];
+#pragma unroll
+                for (int e = 0; e < DIM; ++e) { rl[e] = F.xf[e] - F.xL[e]; rr[e] = F.xf[e] - F.xR[e]; }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    qLf[v] = F.qL[v] + dotd<DIM>(gL + v * DIM, rl);
+                    if (F.other >= 0) qRf[v] = F.qR[v] + dotd<DIM>(gR + v * DIM, rr);
+                }
+            } else {
+                muscl_face<DIM>(a.ph, F.qL, F.qR, gL, gR, F.xL, F.xR, F.xf, qLf, qRf, F.other >= 0 ? 2 : 0);
+                if (a.ph.barth) {
+                    const R rL[3] = {a.ratio[F.owner], a.ratio[(size_t)n + F.owner], a.ratio[(size_t)2 * n + F.owner]};
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) qLf[v] = F.qL[v] + rL[v < 2 ? v : 2] * (qLf[v] - F.qL[v]);
+                    if (F.other >= 0) {
+                        const R rR[3] = {a.ratio[F.other], a.ratio[(size_t)n + F.other], a.ratio[(size_t)2 * n + F.other]};
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) qRf[v] = F.qR[v] + rR[v < 2 ? v : 2] * (qRf[v] - F.qR[v]);
+                    }
+                }
+            }
+        }
+        const BocoDev* bcp = nullptr;
+        if (F.other < 0) {  // face_state boundary map on the right face state
+            bcp = &a.boco[a.g.bboco[F.bi]];
+            const BocoDev bc = *bcp;
+            bc_state<DIM>(bc, F.n, qLf, qRf, a.ph);
+        }
+        R flux[NV];
+        num_flux<DIM>(a.ph.sch_hyp, gam, qLf, qRf, F.n, flux);
+        if (SECOND && a.ph.viscous) {
+            if (F.other < 0) {
+                const BocoDev bc = *bcp;
+                bc_grad<DIM>(bc, F.n, gL, gR);
+            }
+            viscous_face<DIM>(a.ph, F, gL, gR, flux);
+        }
+        const R A = a.g.fs[s.f];
+        // flux_to_res.f90:90-113: R(ic1) -= F.A, R(ic2) += F.A
+        if (s.other) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) res[v] = res[v] + A * flux[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) res[v] = res[v] - A * flux[v];
+        }
+    }
+    // flux_to_res.f90:126-142: R = dt/V * sum
+    const double dt = *a.d_dt;
+    const double coef = dt / a.vol[c];
+    double Rv[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Rv[v] = coef * (double)res[v];
+    if (a.mode == 1) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a.res_out[(size_t)v * n + c] = Rv[v];
+        return;
+    }
+    // RK bookkeeping (treelevel_rungekutta.f90:60-105)
+    double acc[NV];
+    if (a.stage == 1) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = Rv[v] * a.b;
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = a.acc[(size_t)v * n + c] + a.b * Rv[v];
+    }
+    if (a.store_slot >= 0) {
+        double* rs = a.rstore + (size_t)a.store_slot * NV * n;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) rs[(size_t)v * n + c] = Rv[v];
+    }
+    if (a.stage < a.nstage) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a.acc[(size_t)v * n + c] = acc[v];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double u = a.u0[(size_t)v * n + c];
+            for (int j = 0; j < a.stage - 1; ++j) {
+                const int sl = a.read_slot[j];
+                if (sl >= 0) u = u + a.a[j] * a.rstore[((size_t)sl * NV + v) * n + c];
+            }
+            u = u + a.a[a.stage - 1] * Rv[v];
+            a.us_next[(size_t)v * n + c] = (R)u;
+        }
+    } else {
+        // update_field.f90:38: U = U0 + residu
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double u = a.u0[(size_t)v * n + c] + acc[v];
+            a.u0[(size_t)v * n + c] = u;
+            a.us_next[(size_t)v * n + c] = (R)u;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// time step: calc_ns_timestep.f90:52-102, global min (calc_zonetimestep.f90:95)
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin_(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_dt_local(DtArgs a)
+{
+    constexpr int NV = DIM + 2;
+    __shared__ double smin[BLOCK / 64];
+    __shared__ int sbad[BLOCK / 64];
+    const int lb = xcd_block();
+    const int c = lb * BLOCK + threadIdx.x;
+    double dt = HUGE_VAL;
+    int bad = 0;
+    if (c < a.n) {
+        const int n = a.n;
+        const double gg1 = a.gamma * (a.gamma - 1.0);
+        const double rho = a.u0[c];
+        const double irho = 1.0 / rho;
+        double rv2 = a.u0[(size_t)2 * n + c] * a.u0[(size_t)2 * n + c];
+#pragma unroll
+        for (int d = 1; d < DIM; ++d) rv2 += a.u0[(size_t)(2 + d) * n + c] * a.u0[(size_t)(2 + d) * n + c];
+        const double a2 = (a.u0[(size_t)n + c] - .5 * rv2 * irho) * gg1 * irho;
+        if (!(a2 > 0.0)) bad = 1;
+        dt = a.dtfac[c];  // cfl * 2 V / sum(A) premultiplied by cfl on the host
+        dt = dt / (sqrt(rv2) * irho + sqrt(a2));
+        if (!(dt == dt)) bad = 1;
+        (void)NV;
+    }
+    dt = wave_min(dt);
+    bad = wave_sum(bad);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { smin[wid] = dt; sbad[wid] = bad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = smin[0];
+        int b = sbad[0];
+        for (int i = 1; i < BLOCK / 64; ++i) { m = fmin_(m, smin[i]); b += sbad[i]; }
+        a.part[lb] = m;
+        a.bad[lb] = b;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_dt_final(const double* part, const int* badp, int nb, double dtmax,
+                                                   double* d_dt, int* d_err)
+{
+    __shared__ double smin[16];
+    __shared__ int sbad[16];
+    double m = HUGE_VAL;
+    int b = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) { m = fmin_(m, part[i]); b += badp[i]; }
+    m = wave_min(m);
+    b = wave_sum(b);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { smin[wid] = m; sbad[wid] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = smin[0];
+        int bb = sbad[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) { r = fmin_(r, smin[i]); bb += sbad[i]; }
+        r = fmin_(r, dtmax);
+        *d_dt = r;
+        if (bb > 0) *d_err = 1;
+    }
+}
+
+template <typename R>
+__global__ void k_cast(const double* __restrict__ src, R* __restrict__ dst, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (R)src[i];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+template <int DIM, typename R>
+hipError_t launch_gradient(const KArgs<R>& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_gradient<DIM, R>), dim3(grid_blocks(a.n)), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+template <int DIM, typename R>
+hipError_t launch_barth(const KArgs<R>& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_barth<DIM, R>), dim3(grid_blocks(a.n)), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+template <int DIM, typename R>
+hipError_t launch_flux(const KArgs<R>& a, hipStream_t s)
+{
+    const bool second = a.ph.method != 'N' || a.ph.viscous;
+    if (second) hipLaunchKernelGGL((k_flux<DIM, R, true>), dim3(grid_blocks(a.n)), dim3(BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_flux<DIM, R, false>), dim3(grid_blocks(a.n)), dim3(BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+template <int DIM>
+hipError_t launch_timestep(const DtArgs& a, double dtmax, double* d_dt, int* d_err, hipStream_t s)
+{
+    const int nb = grid_blocks(a.n);
+    hipLaunchKernelGGL((k_dt_local<DIM>), dim3(nb), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1024), 0, s, (const double*)a.part, (const int*)a.bad, nb, dtmax,
+                       d_dt, d_err);
+    return hipGetLastError();
+}
+template <typename R>
+hipError_t launch_cast(const double* src, R* dst, long n, hipStream_t s)
+{
+    const int nb = (int)((n + 255) / 256);
+    hipLaunchKernelGGL((k_cast<R>), dim3(nb), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+#define TY_INST(DIM, R)                                                          \
+    template hipError_t launch_gradient<DIM, R>(const KArgs<R>&, hipStream_t); \
+    template hipError_t launch_barth<DIM, R>(const KArgs<R>&, hipStream_t);    \
+    template hipError_t launch_flux<DIM, R>(const KArgs<R>&, hipStream_t);
+TY_INST(2, double)
+TY_INST(3, double)
+TY_INST(2, float)
+TY_INST(3, float)
+template hipError_t launch_timestep<2>(const DtArgs&, double, double*, int*, hipStream_t);
+template hipError_t launch_timestep<3>(const DtArgs&, double, double*, int*, hipStream_t);
+template hipError_t launch_cast<double>(const double*, double*, long, hipStream_t);
+template hipError_t launch_cast<float>(const double*, float*, long, hipStream_t);
+
+}  // namespace ty

@@ -1,0 +1,453 @@
+"""Unstructured mesh in Typhon's own form, built on the host with numpy.
+
+This is the host side of the drop-in: it produces exactly the arrays the
+reference holds after ``init_world`` (SOURCE/MAIN/main.f90:95) and hands them to
+the C-ABI once.  Geometry restates CFDTOOLS/Mesh/MESHGEOM.f90:37-461, face
+ordering follows MESHCONNECT.f90:538-548 (internal faces first, then boundary
+faces grouped by boundary condition), ghost-cell geometry follows
+SOURCE/ZONE/init_ustboco_ghostcell.f90:65-73, init_ustboco_ghostface.f90:53-54
+and init_connect_grid.f90:226-231.
+
+All indices here are 0-based; ``ic2`` of a boundary face is its ghost cell
+(ncell_int + k).  The C-ABI converts to the reference's 1-based arrays.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# boundary condition codes (SOURCE/MODCOM/VARCOM.f90:75-93)
+BC_SYM = 10
+BC_PERIOD = 11
+BC_WALL_ADIAB = 20
+BC_WALL_ISOTH = 21
+BC_WALL_FLUX = 22
+BC_INLET_SUB = 30
+BC_INLET_SUP = 31
+BC_OUTLET_SUB = 35
+BC_OUTLET_SUP = 36
+
+# ghost-cell geometry kind per BC type (EQNS/MENU_NS.f90:91-114, PARAM/MENU_BOCO.f90:127-136)
+_GHOSTCELL = {BC_SYM, BC_WALL_ADIAB, BC_WALL_ISOTH, BC_WALL_FLUX}
+
+
+@dataclass
+class Boco:
+    """One boundary condition: ``umesh%boco(ib)`` plus its resolved parameters."""
+
+    family: str
+    type: int
+    iface: np.ndarray                     # 0-based face indices
+    ipart: np.ndarray | None = None       # periodic: 0-based partner face per face
+    param: np.ndarray = field(default_factory=lambda: np.zeros(8))
+
+
+@dataclass
+class UstMesh:
+    dim: int
+    ncell_int: int
+    ncell: int
+    nface: int
+    nface_int: int
+    ic1: np.ndarray      # int32 [nface]
+    ic2: np.ndarray      # int32 [nface]
+    fnorm: np.ndarray    # [nface,3]
+    fcent: np.ndarray    # [nface,3]
+    fsurf: np.ndarray    # [nface]
+    ccent: np.ndarray    # [ncell,3]
+    cvol: np.ndarray     # [ncell]
+    bocos: list
+    vtx: np.ndarray      # [nvtx,3]
+    fvtx: np.ndarray     # int32 [nface,4], -1 padded
+
+    def boco(self, family: str) -> Boco:
+        for b in self.bocos:
+            if b.family == family:
+                return b
+        raise KeyError(family)
+
+
+# ---------------------------------------------------------------------------
+# seeded counter-based RNG: SplitMix64 keyed by (seed, entity id, draw)
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    x = (x + np.uint64(0x9E3779B97F4A7C15)).astype(np.uint64)
+    z = x
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)).astype(np.uint64)
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)).astype(np.uint64)
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform_pm1(seed: int, ids: np.ndarray, draw: int) -> np.ndarray:
+    """Uniform [-1, 1) from SplitMix64(seed*2^32 ^ (3*id + draw))."""
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) << np.uint64(32)) ^ (ids.astype(np.uint64) * np.uint64(3) + np.uint64(draw))
+        z = _splitmix64(key)
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return 2.0 * u - 1.0
+
+
+# ---------------------------------------------------------------------------
+# local faces of each element type (vertex order defines the geometry split)
+
+_LOCAL_FACES = {
+    "quad": [(0, 1), (1, 2), (2, 3), (3, 0)],
+    "tri": [(0, 1), (1, 2), (2, 0)],
+    "tet": [(0, 2, 1), (0, 1, 3), (1, 2, 3), (0, 3, 2)],
+    "hex": [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)],
+}
+
+
+def _extract_faces(cells: np.ndarray, etype: str, wrap: np.ndarray | None):
+    """Unique faces of an element mesh.
+
+    Returns fvtx [nf,4] (first occurrence's vertex order), cell1, cell2 (-1 when
+    boundary), in order of first appearance (cell-major).  ``wrap`` maps vertex
+    ids to their periodic representative and is only used to build the periodic
+    pairing key of boundary faces."""
+    lf = _LOCAL_FACES[etype]
+    nloc = len(lf)
+    nvf = len(lf[0])
+    ncell = cells.shape[0]
+    fv = np.empty((ncell, nloc, 4), dtype=np.int64)
+    fv[:] = -1
+    for k, f in enumerate(lf):
+        for j, v in enumerate(f):
+            fv[:, k, j] = cells[:, v]
+    fv = fv.reshape(ncell * nloc, 4)
+    owner = np.repeat(np.arange(ncell, dtype=np.int64), nloc)
+    srt = np.sort(fv[:, :nvf], axis=1)
+    nvtx = int(cells.max()) + 1
+    bits = int(np.ceil(np.log2(max(nvtx, 2))))
+    if bits * nvf > 63:
+        raise ValueError("mesh too large for the face key")
+    key = np.zeros(srt.shape[0], dtype=np.int64)
+    for j in range(nvf):
+        key = (key << bits) | srt[:, j]
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    first = np.ones(ks.shape[0], dtype=bool)
+    first[1:] = ks[1:] != ks[:-1]
+    grp = np.cumsum(first) - 1
+    nuniq = int(grp[-1]) + 1
+    # first occurrence (lowest local index in cell-major order, stable sort)
+    fidx = order[first]
+    c1 = owner[fidx]
+    c2 = np.full(nuniq, -1, dtype=np.int64)
+    second = ~first
+    c2[grp[second]] = owner[order[second]]
+    if np.bincount(grp, minlength=nuniq).max() > 2:
+        raise ValueError("non-manifold mesh: a face is shared by more than two cells")
+    # order of first appearance
+    app = np.argsort(fidx, kind="stable")
+    fvtx = fv[fidx[app]]
+    c1 = c1[app]
+    c2 = c2[app]
+    pkey = None
+    if wrap is not None:
+        ws = np.sort(wrap[fvtx[:, :nvf]], axis=1)
+        pkey = np.zeros(ws.shape[0], dtype=np.int64)
+        for j in range(nvf):
+            pkey = (pkey << bits) | ws[:, j]
+    return fvtx, c1, c2, pkey
+
+
+# ---------------------------------------------------------------------------
+# geometry: CFDTOOLS/Mesh/MESHGEOM.f90
+
+def _face_geometry(dim: int, vtx: np.ndarray, fvtx: np.ndarray):
+    """calc_ust_face (MESHGEOM.f90:37-139)."""
+    nf = fvtx.shape[0]
+    v1 = vtx[fvtx[:, 0]]
+    v2 = vtx[fvtx[:, 1]]
+    if dim == 2:
+        e = v2 - v1
+        nrm = np.stack([-e[:, 1], e[:, 0], np.zeros(nf)], axis=1)
+        surf = np.sqrt(nrm[:, 0] * nrm[:, 0] + nrm[:, 1] * nrm[:, 1] + nrm[:, 2] * nrm[:, 2])
+        cen = .5 * (v1 + v2)
+    else:
+        v3 = vtx[fvtx[:, 2]]
+        a = v2 - v1
+        b = v3 - v1
+        n1 = .5 * np.cross(a, b)
+        quad = fvtx[:, 3] >= 0
+        nrm = n1.copy()
+        cen = (v1 + v2 + v3) / 3.0
+        if quad.any():
+            v4 = vtx[np.where(quad, fvtx[:, 3], fvtx[:, 0])]
+            s1 = np.sqrt(np.einsum("ij,ij->i", n1, n1))
+            n2 = .5 * np.cross(b, v4 - v1)
+            s2 = np.sqrt(np.einsum("ij,ij->i", n2, n2))
+            cg2 = (v1 + v3 + v4) / 3.0
+            qc = (s1[:, None] * cen + s2[:, None] * cg2) / (s1 + s2)[:, None]
+            nrm = np.where(quad[:, None], n1 + n2, n1)
+            cen = np.where(quad[:, None], qc, cen)
+        surf = np.sqrt(nrm[:, 0] * nrm[:, 0] + nrm[:, 1] * nrm[:, 1] + nrm[:, 2] * nrm[:, 2])
+    return nrm / surf[:, None], cen, surf
+
+
+def _cell_geometry(dim, ncell_int, ic1, ic2, fnorm, fcent, fsurf):
+    """calc_ust_midcell / calc_ust_elemvol / calc_ust_cell (MESHGEOM.f90:148-331)."""
+    inner = (ic2 >= 0) & (ic2 < ncell_int)
+    nbf = np.bincount(ic1, minlength=ncell_int) + np.bincount(ic2[inner], minlength=ncell_int)
+    mid = np.zeros((ncell_int, 3))
+    for d in range(3):
+        mid[:, d] = (np.bincount(ic1, weights=fcent[:, d], minlength=ncell_int)
+                     + np.bincount(ic2[inner], weights=fcent[inner, d], minlength=ncell_int))
+    mid /= nbf[:, None]
+    kcg, kvol = (1. / 3.0, 1. / 2.0) if dim == 2 else (1. / 4.0, 1. / 3.0)
+    vol = np.zeros(ncell_int)
+    cen = np.zeros((ncell_int, 3))
+    for side, cells, sel in ((0, ic1, slice(None)), (1, ic2[inner], inner)):
+        kh = fcent[sel] - mid[cells]
+        h = np.abs(np.einsum("ij,ij->i", kh, fnorm[sel]))
+        v = kvol * h * fsurf[sel]
+        cg = fcent[sel] - kcg * kh
+        vol += np.bincount(cells, weights=v, minlength=ncell_int)
+        for d in range(3):
+            cen[:, d] += np.bincount(cells, weights=v * cg[:, d], minlength=ncell_int)
+    cen /= vol[:, None]
+    return cen, vol
+
+
+def build_ustmesh(dim: int, vtx: np.ndarray, cells: np.ndarray, etype: str,
+                  face_family, bocos_spec: list, wrap: np.ndarray | None = None) -> UstMesh:
+    """Build Typhon's ``st_ustmesh`` from an element mesh.
+
+    face_family(fvtx_boundary, centres) -> array of family names per boundary face.
+    bocos_spec: list of dicts {family, type, param[, partner]} in boundary order;
+    a periodic boco names its partner family, paired through ``wrap``.
+    """
+    fvtx, c1, c2, pkey = _extract_faces(cells, etype, wrap)
+    ncell_int = cells.shape[0]
+    internal = c2 >= 0
+    # face geometry needs the face vertex list in hand
+    fn, fc, fs = _face_geometry(dim, vtx, fvtx)
+    fam = np.asarray(face_family(fvtx[~internal], fc[~internal]))
+    bidx = np.nonzero(~internal)[0]
+    # internal faces first, then boundary faces grouped by boco in spec order
+    groups = []
+    for spec in bocos_spec:
+        groups.append(bidx[fam == spec["family"]])
+    nb_listed = sum(len(g) for g in groups)
+    if nb_listed != len(bidx):
+        raise ValueError("boundary faces without a boundary condition family")
+    order = np.concatenate([np.nonzero(internal)[0]] + groups)
+    fvtx = fvtx[order].astype(np.int32)
+    c1 = c1[order]
+    c2 = c2[order]
+    fn, fc, fs = fn[order], fc[order], fs[order]
+    if pkey is not None:
+        pkey = pkey[order]
+    nface = order.shape[0]
+    nface_int = int(internal.sum())
+    # calc_ust_checkface (MESHGEOM.f90:362-410): ic1 < ic2 for internal faces
+    swap = (c2 >= 0) & (c1 > c2)
+    t = c1[swap].copy()
+    c1[swap] = c2[swap]
+    c2[swap] = t
+    ccent_i, cvol_i = _cell_geometry(dim, ncell_int, c1, c2, fn, fc, fs)
+    v12 = np.where((c2 >= 0)[:, None], ccent_i[np.maximum(c2, 0)], fc) - ccent_i[c1]
+    flip = np.einsum("ij,ij->i", v12, fn) < 0.0
+    fn[flip] = -fn[flip]
+    # ghost cells, one per boundary face, in boco order (init_connect_grid.f90:82-88)
+    nghost = nface - nface_int
+    ncell = ncell_int + nghost
+    ccent = np.zeros((ncell, 3))
+    cvol = np.zeros(ncell)
+    ccent[:ncell_int] = ccent_i
+    cvol[:ncell_int] = cvol_i
+    bocos = []
+    start = nface_int
+    spans = {}
+    for spec, g in zip(bocos_spec, groups):
+        faces = np.arange(start, start + len(g), dtype=np.int32)
+        spans[spec["family"]] = faces
+        start += len(g)
+    for spec in bocos_spec:
+        faces = spans[spec["family"]]
+        ghost = ncell_int + (faces - nface_int)
+        c2[faces] = ghost
+        btype = spec["type"]
+        inner = c1[faces]
+        ipart = None
+        if btype == BC_PERIOD:
+            pf = spans[spec["partner"]]
+            kp = {int(k): int(f) for k, f in zip(pkey[pf], pf)}
+            ipart = np.array([kp[int(k)] for k in pkey[faces]], dtype=np.int32)
+            icper = c1[ipart]
+            dfc = ccent_i[icper] - fc[ipart]
+            cvol[ghost] = cvol_i[icper]
+            ccent[ghost] = fc[faces] + dfc
+        elif btype == BC_SYM:
+            gc = ccent_i[inner]
+            n = fn[faces]
+            k = 2.0 * np.einsum("ij,ij->i", fc[faces] - gc, n)
+            cvol[ghost] = cvol_i[inner]
+            ccent[ghost] = gc + k[:, None] * n
+        elif btype in _GHOSTCELL:
+            cvol[ghost] = cvol_i[inner]
+            ccent[ghost] = 2.0 * fc[faces] - ccent_i[inner]
+        else:
+            cvol[ghost] = 0.0
+            ccent[ghost] = fc[faces]
+        param = np.zeros(8)
+        p = np.asarray(spec.get("param", []), dtype=np.float64)
+        param[:p.size] = p
+        bocos.append(Boco(spec["family"], btype, faces, ipart, param))
+    return UstMesh(dim, ncell_int, ncell, nface, nface_int, c1.astype(np.int32), c2.astype(np.int32),
+                   fn, fc, fs, ccent, cvol, bocos, vtx, fvtx)
+
+
+# ---------------------------------------------------------------------------
+# synthetic meshes
+
+def channel_quad(nx=256, ny=128, lx=1.0, ly=0.5, jitter=0.1, seed=1, bocos=None):
+    """2D channel of nx*ny quads, interior vertices jittered by +-jitter*h.
+
+    Families IMIN/IMAX (x=0/lx) and JMIN/JMAX (y=0/ly)."""
+    hx, hy = lx / nx, ly / ny
+    i, j = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    vid = (i * (ny + 1) + j).ravel()
+    x = (i * hx).ravel().astype(np.float64)
+    y = (j * hy).ravel().astype(np.float64)
+    interior = ((i > 0) & (i < nx) & (j > 0) & (j < ny)).ravel()
+    x = x + np.where(interior, jitter * hx * uniform_pm1(seed, vid, 0), 0.0)
+    y = y + np.where(interior, jitter * hy * uniform_pm1(seed, vid, 1), 0.0)
+    vtx = np.stack([x, y, np.zeros_like(x)], axis=1)
+    ci, cj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    v0 = ci * (ny + 1) + cj
+    cells = np.stack([v0, v0 + (ny + 1), v0 + (ny + 2), v0 + 1], axis=-1).reshape(-1, 4)
+    wrap = None
+    if bocos is None:
+        bocos = [dict(family="JMIN", type=BC_SYM), dict(family="JMAX", type=BC_SYM),
+                 dict(family="IMIN", type=BC_SYM), dict(family="IMAX", type=BC_SYM)]
+    if any(b["type"] == BC_PERIOD for b in bocos):
+        wi = np.where(i == nx, 0, i)
+        wj = np.where(j == ny, 0, j)
+        wrap = (wi * (ny + 1) + wj).ravel()
+
+    def fam(fv, fc):
+        iv = fv[:, 0] // (ny + 1)
+        jv = fv[:, 0] % (ny + 1)
+        iw = fv[:, 1] // (ny + 1)
+        jw = fv[:, 1] % (ny + 1)
+        out = np.empty(fv.shape[0], dtype=object)
+        out[(iv == 0) & (iw == 0)] = "IMIN"
+        out[(iv == nx) & (iw == nx)] = "IMAX"
+        out[(jv == 0) & (jw == 0)] = "JMIN"
+        out[(jv == ny) & (jw == ny)] = "JMAX"
+        return out
+
+    return build_ustmesh(2, vtx, cells, "quad", fam, bocos, wrap)
+
+
+def _box_vertices(n, L, jitter, seed, periodic):
+    h = L / n
+    i, j, k = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    if periodic:
+        wi, wj, wk = i % n, j % n, k % n
+        rid = (wi * n + wj) * n + wk
+        dx = jitter * h * uniform_pm1(seed, rid, 0)
+        dy = jitter * h * uniform_pm1(seed, rid, 1)
+        dz = jitter * h * uniform_pm1(seed, rid, 2)
+        wrap = ((wi * (n + 1) + wj) * (n + 1) + wk)
+    else:
+        vid = (i * (n + 1) + j) * (n + 1) + k
+        inner = (i > 0) & (i < n) & (j > 0) & (j < n) & (k > 0) & (k < n)
+        dx = np.where(inner, jitter * h * uniform_pm1(seed, vid, 0), 0.0)
+        dy = np.where(inner, jitter * h * uniform_pm1(seed, vid, 1), 0.0)
+        dz = np.where(inner, jitter * h * uniform_pm1(seed, vid, 2), 0.0)
+        wrap = None
+    vtx = np.stack([i * h + dx, j * h + dy, k * h + dz], axis=1)
+    return vtx, wrap
+
+
+def _box_family(n):
+    def fam(fv, fc):
+        vi = fv[:, :3] // ((n + 1) * (n + 1))
+        vj = (fv[:, :3] // (n + 1)) % (n + 1)
+        vk = fv[:, :3] % (n + 1)
+        out = np.empty(fv.shape[0], dtype=object)
+        for name, arr, val in (("IMIN", vi, 0), ("IMAX", vi, n), ("JMIN", vj, 0),
+                               ("JMAX", vj, n), ("KMIN", vk, 0), ("KMAX", vk, n)):
+            out[np.all(arr == val, axis=1)] = name
+        return out
+    return fam
+
+
+def _periodic_box_bocos():
+    return [dict(family="IMIN", type=BC_PERIOD, partner="IMAX"),
+            dict(family="IMAX", type=BC_PERIOD, partner="IMIN"),
+            dict(family="JMIN", type=BC_PERIOD, partner="JMAX"),
+            dict(family="JMAX", type=BC_PERIOD, partner="JMIN"),
+            dict(family="KMIN", type=BC_PERIOD, partner="KMAX"),
+            dict(family="KMAX", type=BC_PERIOD, partner="KMIN")]
+
+
+def box_tets(n=100, L=10.0, jitter=0.05, seed=2, periodic=True, bocos=None):
+    """n^3 hexes on [0,L]^3, each split into 6 tets about the main diagonal
+    (Kuhn/Freudenthal split, conforming between neighbours)."""
+    vtx, wrap = _box_vertices(n, L, jitter, seed, periodic)
+    ci, cj, ck = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+
+    def v(di, dj, dk):
+        return ((ci + di) * (n + 1) + (cj + dj)) * (n + 1) + (ck + dk)
+    h = [v(0, 0, 0), v(1, 0, 0), v(1, 1, 0), v(0, 1, 0), v(0, 0, 1), v(1, 0, 1), v(1, 1, 1), v(0, 1, 1)]
+    kuhn = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
+    cells = np.stack([np.stack([h[a], h[b], h[c], h[d]], axis=1) for a, b, c, d in kuhn], axis=1).reshape(-1, 4)
+    if bocos is None:
+        bocos = _periodic_box_bocos() if periodic else [dict(family=f, type=BC_SYM) for f in
+                                                         ("IMIN", "IMAX", "JMIN", "JMAX", "KMIN", "KMAX")]
+    return build_ustmesh(3, vtx, cells, "tet", _box_family(n), bocos, wrap)
+
+
+def box_hex(n=16, L=2 * np.pi, jitter=0.0, seed=3, periodic=True, bocos=None):
+    """n^3 hexes on [0,L]^3."""
+    vtx, wrap = _box_vertices(n, L, jitter, seed, periodic)
+    ci, cj, ck = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+
+    def v(di, dj, dk):
+        return ((ci + di) * (n + 1) + (cj + dj)) * (n + 1) + (ck + dk)
+    cells = np.stack([v(0, 0, 0), v(1, 0, 0), v(1, 1, 0), v(0, 1, 0),
+                      v(0, 0, 1), v(1, 0, 1), v(1, 1, 1), v(0, 1, 1)], axis=1)
+    if bocos is None:
+        bocos = _periodic_box_bocos() if periodic else [dict(family=f, type=BC_SYM) for f in
+                                                         ("IMIN", "IMAX", "JMIN", "JMAX", "KMIN", "KMAX")]
+    return build_ustmesh(3, vtx, cells, "hex", _box_family(n), bocos, wrap)
+
+
+def periodic_quad(nx=20, ny=20, lx=1.0, ly=1.0, jitter=0.0, seed=1):
+    """2D doubly periodic quad mesh (the NRG TIME/RUNGEKUTTA layout)."""
+    bocos = [dict(family="IMIN", type=BC_PERIOD, partner="IMAX"),
+             dict(family="IMAX", type=BC_PERIOD, partner="IMIN"),
+             dict(family="JMIN", type=BC_PERIOD, partner="JMAX"),
+             dict(family="JMAX", type=BC_PERIOD, partner="JMIN")]
+    hx, hy = lx / nx, ly / ny
+    i, j = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    wi = np.where(i == nx, 0, i)
+    wj = np.where(j == ny, 0, j)
+    rid = (wi * ny + wj).ravel()
+    x = (i * hx).ravel() + jitter * hx * uniform_pm1(seed, rid, 0)
+    y = (j * hy).ravel() + jitter * hy * uniform_pm1(seed, rid, 1)
+    vtx = np.stack([x, y, np.zeros_like(x)], axis=1)
+    ci, cj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    v0 = ci * (ny + 1) + cj
+    cells = np.stack([v0, v0 + (ny + 1), v0 + (ny + 2), v0 + 1], axis=-1).reshape(-1, 4)
+    wrap = (wi * (ny + 1) + wj).ravel()
+
+    def fam(fv, fc):
+        iv, jv = fv[:, 0] // (ny + 1), fv[:, 0] % (ny + 1)
+        iw, jw = fv[:, 1] // (ny + 1), fv[:, 1] % (ny + 1)
+        out = np.empty(fv.shape[0], dtype=object)
+        out[(iv == 0) & (iw == 0)] = "IMIN"
+        out[(iv == nx) & (iw == nx)] = "IMAX"
+        out[(jv == 0) & (jw == 0)] = "JMIN"
+        out[(jv == ny) & (jw == ny)] = "JMAX"
+        return out
+
+    return build_ustmesh(2, vtx, cells, "quad", fam, bocos, wrap)

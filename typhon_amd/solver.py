@@ -1,0 +1,154 @@
+"""Host-side mirror of Typhon's EQNS residual / time-integration interface.
+
+``Solver`` takes the mesh as Typhon holds it after init_world (a
+``typhon_amd.mesh.UstMesh``) and the resolved solver menus, and drives the
+MI355X residual path through the C-ABI (include/typhon_gpu.h):
+
+    calc_rhs(dt)      -> SOURCE/MAIN/calc_rhs.f90:10 (residual -dt/V sum F.A)
+    timestep(dtmax)   -> SOURCE/ZONE/calc_zonetimestep.f90:57 (stab_cond)
+    rk_step(dt)       -> SOURCE/MAIN/treelevel_rungekutta.f90:8 + update_field
+    advance(n, dtmax) -> n device-resident cycles of the two above
+
+Conservative state arrays are [5, ncell_int] (rho, rhoE, rho u, rho v, rho w)
+in the caller's cell order.  Errors from the library raise TyError, mirroring
+the reference's error_stop.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .scheme import default_gas, default_scheme, rk_tableau
+
+
+def _p64(a):
+    return a.ctypes.data_as(_lib.F64P)
+
+
+def _p32(a):
+    return a.ctypes.data_as(_lib.I32P)
+
+
+class Solver:
+    def __init__(self, mesh, scheme: dict | None = None, gas: dict | None = None, device: int = 0):
+        self.lib = _lib.load()
+        self.mesh = mesh
+        self.ni = mesh.ncell_int
+        self.h = C.c_void_p()
+        self._check(self.lib.ty_create(C.byref(self.h), device), "ty_create")
+        self._upload_mesh(mesh)
+        self.set_scheme(scheme or {}, gas or {})
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.ty_last_error(self.h).decode() if self.h else what
+            raise _lib.TyError(rc, f"{what}: {msg}")
+
+    def close(self):
+        if self.h:
+            self.lib.ty_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _upload_mesh(self, m):
+        fc = np.ascontiguousarray(np.concatenate([m.ic1 + 1, m.ic2 + 1]).astype(np.int32))
+        face = np.ascontiguousarray(np.column_stack([m.fnorm, m.fcent, m.fsurf]))
+        centre = np.ascontiguousarray(m.ccent, dtype=np.float64)
+        volume = np.ascontiguousarray(m.cvol, dtype=np.float64)
+        keep = [fc, face, centre, volume]
+        bocos = (_lib.TyBocoDesc * max(1, len(m.bocos)))()
+        for k, b in enumerate(m.bocos):
+            fi = np.ascontiguousarray(b.iface + 1, dtype=np.int32)
+            keep.append(fi)
+            bocos[k].type = b.type
+            bocos[k].nface = fi.shape[0]
+            bocos[k].iface = _p32(fi)
+            if b.ipart is not None:
+                pi = np.ascontiguousarray(b.ipart + 1, dtype=np.int32)
+                keep.append(pi)
+                bocos[k].ipart = _p32(pi)
+            for j in range(8):
+                bocos[k].param[j] = float(b.param[j])
+        d = _lib.TyMeshDesc(m.dim, m.ncell_int, m.ncell, m.nface, m.nface_int, _p32(fc), _p64(face),
+                            _p64(centre), _p64(volume), len(m.bocos), bocos)
+        self._check(self.lib.ty_mesh_upload(self.h, C.byref(d)), "ty_mesh_upload")
+        del keep
+
+    def set_scheme(self, scheme: dict, gas: dict):
+        s = default_scheme(**scheme)
+        g = default_gas(**gas)
+        coef = s.get("rk_coef")
+        coef = np.asarray(rk_tableau(s["tps_meth"]) if coef is None else coef, dtype=np.float64)
+        self.nstage = coef.shape[0]
+        cm = np.ascontiguousarray(coef.T)  # column-major rk%coef
+        sd = _lib.TyScheme(s["sch_hyp"], s["sch_dis"], s["method"].encode(), s["limiter"].encode(),
+                           s["gradmeth"], s["postlimiter"], s["venkat_k"], self.nstage, _p64(cm), s["cfl"],
+                           s["precision"])
+        gd = _lib.TyGas(g["gamma"], g["r_const"], g["prandtl"], g["typ_visc"], g["visc_dyn"], g["visc_kin"],
+                        g["tref"], g["tsuth"], g["typ_fluid"])
+        self._check(self.lib.ty_set_scheme(self.h, C.byref(gd), C.byref(sd)), "ty_set_scheme")
+        self.scheme, self.gas = s, g
+
+    # -- reference interface ----------------------------------------------
+    def set_state(self, cons):
+        cons = np.asarray(cons, dtype=np.float64).reshape(5, self.ni)
+        rho = np.ascontiguousarray(cons[0])
+        rhoe = np.ascontiguousarray(cons[1])
+        mom = np.ascontiguousarray(cons[2:5].T)
+        self._check(self.lib.ty_state_set(self.h, _p64(rho), _p64(rhoe), _p64(mom)), "ty_state_set")
+
+    def get_state(self):
+        rho, rhoe, mom = np.empty(self.ni), np.empty(self.ni), np.empty((self.ni, 3))
+        self._check(self.lib.ty_state_get(self.h, _p64(rho), _p64(rhoe), _p64(mom)), "ty_state_get")
+        return np.vstack([rho, rhoe, mom.T])
+
+    def timestep(self, dtmax=np.inf):
+        dt = C.c_double()
+        self._check(self.lib.ty_timestep(self.h, dtmax, C.byref(dt)), "ty_timestep")
+        return dt.value
+
+    def calc_rhs(self, dt):
+        rho, rhoe, mom = np.empty(self.ni), np.empty(self.ni), np.empty((self.ni, 3))
+        self._check(self.lib.ty_residual(self.h, dt, _p64(rho), _p64(rhoe), _p64(mom)), "ty_residual")
+        return np.vstack([rho, rhoe, mom.T])
+
+    def rk_step(self, dt):
+        self._check(self.lib.ty_rk_step(self.h, dt), "ty_rk_step")
+
+    def advance(self, nsteps, dtmax=np.inf, want_dt=False):
+        if want_dt:
+            dt = C.c_double()
+            self._check(self.lib.ty_advance(self.h, nsteps, dtmax, C.byref(dt)), "ty_advance")
+            return dt.value
+        self._check(self.lib.ty_advance(self.h, nsteps, dtmax, None), "ty_advance")
+        return None
+
+    def sync(self):
+        self._check(self.lib.ty_sync(self.h), "ty_sync")
+
+    def timing(self, enable=True):
+        ms = (C.c_double * 3)()
+        cnt = (C.c_int32 * 3)()
+        self._check(self.lib.ty_timing(self.h, 1 if enable else 0, ms, cnt), "ty_timing")
+        return dict(grad_ms=ms[0], flux_ms=ms[1], dt_ms=ms[2], grad_n=cnt[0], flux_n=cnt[1], dt_n=cnt[2])
+
+    def mesh_info(self):
+        v = [C.c_int32() for _ in range(4)]
+        self._check(self.lib.ty_mesh_info(self.h, *[C.byref(x) for x in v]), "ty_mesh_info")
+        return dict(ncell=v[0].value, nface=v[1].value, nslot=v[2].value, nbface=v[3].value)
+
+
+def prim_to_cons(rho, p, vel, gamma=1.4):
+    """calc_varcons_ns (SOURCE/EQNS/calc_varcons_ns.f90:35-45) on arrays: vel [3, n]."""
+    vel = np.asarray(vel, dtype=np.float64)
+    mom = rho * vel
+    rhoe = p / (gamma - 1.0) + .5 * rho * np.einsum("ij,ij->j", vel, vel)
+    return np.vstack([rho, rhoe, mom])
