@@ -41,12 +41,11 @@ def byte_model(nc, nf, nb, dim=3, nv=5, mixed=False, rk4=True):
     return dict(grad=b_grad, flux=b_flux + b_rk, residual_only=b_grad + b_flux, total=b_grad + b_flux + b_rk)
 
 
-def run_gpu(m, s, g, u, steps, warmup, precision):
+def run_gpu(m, s, g, u, steps, warmup, precision, order="hilbert"):
     from typhon_amd.solver import Solver
-    from typhon_amd import scheme as S
     sc = dict(s)
     sc["precision"] = precision
-    solver = Solver(m, sc, g)
+    solver = Solver(m, sc, g, cell_order=order)
     solver.set_state(u)
     solver.advance(warmup)
     solver.sync()
@@ -108,6 +107,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=20)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--order", default="hilbert", choices=["hilbert", "rcm", "none"],
+                    help="device cell order set at mesh upload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -133,7 +134,7 @@ def main():
     m, s, g, u = K.vortex_case(args.n, L=10.0)
     print(f"# mesh: {m.ncell_int} tets, {m.nface} faces (reference form), built in {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
-    t64, tm64, st64, info = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_FP64)
+    t64, tm64, st64, info = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_FP64, args.order)
     print(f"# fp64: {t64:.3f} s for {args.steps} steps; kernels {tm64}", file=sys.stderr, flush=True)
     nc = m.ncell_int
     nf_dev = info["nface"]
@@ -161,7 +162,8 @@ def main():
         "data": "synthetic (seeded jittered periodic tet mesh, isentropic vortex initial field)",
         "config": {"workload": f"3D Euler MUSCL (LSQ gradients, Venkatakrishnan K=5) + HLLC + RK4, "
                                f"{args.n}^3 hexes -> {nc} periodic jittered tets, isentropic vortex",
-                   "cells": nc, "faces": nf_dev, "evals_per_step": 4, "precision": "fp64"},
+                   "cells": nc, "faces": nf_dev, "evals_per_step": 4, "precision": "fp64",
+                   "cell_order": args.order},
         "roofline": {"bound": "hbm", "kernel": f"k_{'flux' if kern == 'flux' else 'gradient'}",
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
@@ -174,7 +176,7 @@ def main():
                          "bytes_per_cell_eval": round(bm["total"] / nc, 1)},
     }
     if not args.no_mixed:
-        tmx, tmm, stmx, _ = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_MIXED)
+        tmx, tmm, stmx, _ = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_MIXED, args.order)
         bmm = byte_model(nc, nf_dev, nb_dev, mixed=True)
         rho64, rhomx = st64[0], stmx[0]
         err_l2 = float(np.sqrt(np.mean((rhomx - rho64) ** 2)) / np.sqrt(np.mean(rho64 ** 2)))

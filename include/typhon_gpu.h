@@ -57,6 +57,8 @@ typedef struct {
     const double* volume;    /* umesh%mesh%volume(1:ncell,1,1) */
     int32_t nboco;
     const ty_boco_desc* boco;
+    int32_t cell_order;      /* device cell order (no reference counterpart): 0 reverse Cuthill-McKee,
+                                1 Hilbert curve through the cell centres, 2 the caller's order */
 } ty_mesh_desc;
 
 /* st_espece (SOURCE/EQNS/EQNS.f90:60-70) and the fluid model (def_model_ns.f90) */

@@ -254,13 +254,17 @@ static void boco_cell_grad(or_ctx* o)
 /* ---------------------------------------------------------------------------
  * A4: precalc_grad_lsq (SOURCE/MGRID/precalc_grad_lsq.f90:60-155) and
  * cholesky_decomp (SOURCE/MATH/MATRIX.f90:42-70) */
-static void face_dcg(const or_ctx* o, int f, double* d)
+/* weighted LSQ: the matrix uses d/|d|^2 (precalc_grad_lsq.f90:70-73) while the
+   right-hand side uses d/|d|^4 (calc_gradient.f90:94-101, sqrabs(dvec)**2), i.e. both
+   are the weight 1/|d|^4 of d d^T and dq d.  rhs selects the right-hand-side form. */
+static void face_dcg(const or_ctx* o, int f, double* d, int rhs)
 {
     const double* c1 = o->c.ccent + 3 * (size_t)o->c.ic1[f];
     const double* c2 = o->c.ccent + 3 * (size_t)o->c.ic2[f];
     d[0] = c2[0] - c1[0]; d[1] = c2[1] - c1[1]; d[2] = c2[2] - c1[2];
     if (o->c.gradmeth == OR_GRAD_LSQW) {
         double s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (rhs) s = s * s;
         d[0] /= s; d[1] /= s; d[2] /= s;
     }
 }
@@ -280,7 +284,7 @@ static void precalc_lsq(or_ctx* o)
 {
     memset(o->lsq, 0, sizeof(double) * 9 * (size_t)o->ni);
     for (int f = 0; f < o->nf; ++f) {
-        double d[3]; face_dcg(o, f, d);
+        double d[3]; face_dcg(o, f, d, 0);
         double m[9];
         for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m[i + 3 * j] = d[i] * d[j];
         m[0] = d[0] * d[0]; m[4] = d[1] * d[1]; m[8] = d[2] * d[2];
@@ -306,7 +310,7 @@ static void calc_gradient(or_ctx* o)
     int ni = o->ni;
     double* rhs = (double*)calloc((size_t)15 * ni, sizeof(double)); /* [cell][var][3] */
     for (int f = 0; f < o->nf; ++f) {
-        double d[3]; face_dcg(o, f, d);
+        double d[3]; face_dcg(o, f, d, 1);
         int c1 = o->c.ic1[f], c2 = o->c.ic2[f];
         for (int v = 0; v < NV; ++v) {
             double ds = PRIM(o, v, c2) - PRIM(o, v, c1);

@@ -1,0 +1,33 @@
+"""The reference's golden RK histories (NRG/TIME/RUNGEKUTTA/*/RHOERR.tmon) replayed by the oracle
+(CPU) and by the HIP path (GPU).  The files hold 8 significant digits; the check is 2e-7 relative
+per cycle over all 100 cycles."""
+import numpy as np
+import pytest
+
+import cases as K
+import nrg
+
+RTOL = 2e-7
+
+
+@pytest.mark.parametrize("case", list(nrg.CASES))
+def test_oracle_matches_nrg_golden(case):
+    m, s, g, u = nrg.setup(case)
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u)
+    res = nrg.replay(o.rk_step, lambda rem: o.timestep(s["cfl"], rem), lambda: o.get_cons()[0], m)
+    gold = nrg.golden(case)
+    np.testing.assert_allclose(res[:, 1], gold[:, 1], rtol=1e-7)
+    np.testing.assert_allclose(res[:, 2], gold[:, 2], rtol=RTOL)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["RK2", "RK4", "RK25BB"])
+def test_gpu_matches_nrg_golden(case):
+    from typhon_amd.solver import Solver
+    m, s, g, u = nrg.setup(case)
+    gpu = Solver(m, s, g)
+    gpu.set_state(u)
+    res = nrg.replay(gpu.rk_step, lambda rem: gpu.timestep(rem), lambda: gpu.get_state()[0], m)
+    gold = nrg.golden(case)
+    np.testing.assert_allclose(res[:, 2], gold[:, 2], rtol=RTOL)

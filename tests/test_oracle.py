@@ -1,0 +1,352 @@
+"""Pins of the oracle (CPU): closed forms, invariants and independent textbook forms.
+
+Each check is chosen so that a plausible mistake in the restatement (a dropped
+term, a wrong sign, a transposed operand, a wrong coefficient) fails it; none of
+them retypes the oracle's own formula.
+"""
+import numpy as np
+import pytest
+
+import cases as K
+from oracle import oracle as O
+from typhon_amd import mesh as M
+from typhon_amd import scheme as S
+
+G = 1.4
+SCHEMES = [S.SCH_HLLC, S.SCH_HLLCB, S.SCH_HLLE, S.SCH_HLLEB, S.SCH_RUSANOV, S.SCH_ROE]
+
+
+def phys_flux(q, n, g=G):
+    rho, p, u = q[0], q[1], np.asarray(q[2:5])
+    vn = u @ n
+    E = p / (g - 1) + .5 * rho * u @ u
+    return np.concatenate([[rho * vn, vn * (E + p)], rho * vn * u + p * n])
+
+
+def rand_state(rng, vmax=1.0):
+    return np.concatenate([[rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0)], rng.uniform(-vmax, vmax, 3)])
+
+
+def rand_normal(rng):
+    n = rng.normal(size=3)
+    return n / np.linalg.norm(n)
+
+
+# --- A9 numerical fluxes ---------------------------------------------------------------
+
+@pytest.mark.parametrize("sch", SCHEMES)
+def test_flux_consistency(sch):
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        q, n = rand_state(rng), rand_normal(rng)
+        f = O.flux_face(q, q, n, sch)
+        np.testing.assert_allclose(f, phys_flux(q, n), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("sch", SCHEMES)
+def test_flux_rotational_invariance_and_antisymmetry(sch):
+    rng = np.random.default_rng(2)
+    for _ in range(30):
+        qL, qR, n = rand_state(rng), rand_state(rng), rand_normal(rng)
+        f = O.flux_face(qL, qR, n, sch)
+        # F(qR, qL, -n) = -F(qL, qR, n)
+        np.testing.assert_allclose(O.flux_face(qR, qL, -n, sch), -f, rtol=1e-12, atol=1e-12)
+        # rotation: F(R qL, R qR, R n) = R F
+        A, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        rot = lambda q: np.concatenate([q[:2], A @ q[2:5]])  # noqa: E731
+        fr = O.flux_face(rot(qL), rot(qR), A @ n, sch)
+        np.testing.assert_allclose(fr, np.concatenate([f[:2], A @ f[2:5]]), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("sch", [S.SCH_HLLC, S.SCH_HLLCB, S.SCH_HLLE, S.SCH_HLLEB, S.SCH_ROE])
+def test_supersonic_upwinding(sch):
+    rng = np.random.default_rng(3)
+    n = np.array([1.0, 0.0, 0.0])
+    for _ in range(20):
+        qL, qR = rand_state(rng, 0.1), rand_state(rng, 0.1)
+        qL[2] += 4.0  # supersonic to the right (a ~ 1..2.4)
+        qR[2] += 4.0
+        np.testing.assert_allclose(O.flux_face(qL, qR, n, sch), phys_flux(qL, n), rtol=1e-12, atol=1e-12)
+        qL[2] -= 8.0
+        qR[2] -= 8.0
+        np.testing.assert_allclose(O.flux_face(qL, qR, n, sch), phys_flux(qR, n), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("sch", [S.SCH_HLLC, S.SCH_HLLCB, S.SCH_ROE])
+def test_stationary_contact_is_exact(sch):
+    # equal pressure, zero normal velocity, density jump and tangential shear: pure pressure flux
+    n = np.array([0.6, 0.8, 0.0])
+    t = np.array([-0.8, 0.6, 0.0])
+    qL = np.concatenate([[1.0, 0.7], 0.3 * t])
+    qR = np.concatenate([[3.0, 0.7], -0.5 * t])
+    f = O.flux_face(qL, qR, n, sch)
+    np.testing.assert_allclose(f, np.concatenate([[0.0, 0.0], 0.7 * n]), atol=1e-14)
+
+
+def toro_hllc(qL, qR, n, g=G):
+    """HLLC in Toro's conservative-star form (Toro, Riemann Solvers 3rd ed., eqs. 10.37-10.39)
+    with Batten's Roe-average wave speeds; an independent statement of the same flux."""
+    def cons(q):
+        rho, p, u = q[0], q[1], q[2:5]
+        return np.concatenate([[rho, p / (g - 1) + .5 * rho * u @ u], rho * u])
+    rl, rr = qL[0], qR[0]
+    sq = np.sqrt(rr / rl)
+    ut = (qL[2:5] + sq * qR[2:5]) / (1 + sq)
+    hl = g / (g - 1) * qL[1] / rl + .5 * qL[2:5] @ qL[2:5]
+    hr = g / (g - 1) * qR[1] / rr + .5 * qR[2:5] @ qR[2:5]
+    ht = (hl + sq * hr) / (1 + sq)
+    at = np.sqrt((g - 1) * (ht - .5 * ut @ ut))
+    ul, ur = qL[2:5] @ n, qR[2:5] @ n
+    al, ar = np.sqrt(g * qL[1] / rl), np.sqrt(g * qR[1] / rr)
+    sl = min(ul - al, ut @ n - at)
+    sr = max(ur + ar, ut @ n + at)
+    ss = (qR[1] - qL[1] + rl * ul * (sl - ul) - rr * ur * (sr - ur)) / (rl * (sl - ul) - rr * (sr - ur))
+    if sl >= 0:
+        return phys_flux(qL, n, g)
+    if sr <= 0:
+        return phys_flux(qR, n, g)
+    q, s, un = (qL, sl, ul) if ss >= 0 else (qR, sr, ur)
+    U = cons(q)
+    rho = q[0]
+    fac = rho * (s - un) / (s - ss)
+    E = U[1] / rho
+    ustar = q[2:5] + (ss - un) * n
+    Us = fac * np.concatenate([[1.0, E + (ss - un) * (ss + q[1] / (rho * (s - un)))], ustar])
+    return phys_flux(q, n, g) + s * (Us - U)
+
+
+def test_hllc_matches_toro_form():
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        qL, qR, n = rand_state(rng, 1.5), rand_state(rng, 1.5), rand_normal(rng)
+        np.testing.assert_allclose(O.flux_face(qL, qR, n, S.SCH_HLLC), toro_hllc(qL, qR, n), rtol=1e-11,
+                                   atol=1e-11)
+
+
+def test_roe_property():
+    """Roe: F = (F_L + F_R)/2 - |A~|(U_R - U_L)/2 with the Roe property A~ (U_R - U_L) = F_R - F_L.
+    With all eigenvalues positive and outside the entropy-fix band |A~| = A~, so F = F_L exactly:
+    this holds only if the wave strengths and eigenvectors are right."""
+    rng = np.random.default_rng(5)
+    n = np.array([0.0, 0.0, 1.0])
+    for _ in range(20):
+        qL, qR = rand_state(rng, 0.1), rand_state(rng, 0.1)
+        qL[4] += 3.0
+        qR[4] += 3.0
+        np.testing.assert_allclose(O.flux_face(qL, qR, n, S.SCH_ROE), phys_flux(qL, n), rtol=1e-12, atol=1e-12)
+
+
+# --- A6 limiters and A8 Venkatakrishnan ------------------------------------------------
+
+def test_limiter_closed_forms():
+    L = O.limiter
+    assert L("M", 1.0, 2.0) == 1.0 and L("M", -1.0, -3.0) == -1.0 and L("M", -1.0, 2.0) == 0.0
+    assert L("A", 2.0, 2.0) == 2.0 and L("A", 1.0, -1.0) == 0.0
+    assert abs(L("A", 1.0, 3.0) - 3.0 * 4.0 / 10.0) < 1e-15
+    assert abs(L("V", 1.0, 3.0) - 1.5) < 1e-15 and L("V", -1.0, 3.0) == 0.0
+    assert L("S", 1.0, 3.0) == 2.0 and L("S", 1.0, 1.5) == 1.5 and L("S", -1.0, -0.4) == -0.8
+    assert abs(L("K", 1.0, 1.0) - 1.0) < 1e-15 and L("K", 1.0, 10.0) == 2.0
+    # symmetric limiters
+    rng = np.random.default_rng(6)
+    for lim in "MAVS":
+        for _ in range(50):
+            a, b = rng.normal(size=2)
+            assert abs(L(lim, a, b) - L(lim, b, a)) < 1e-15
+            assert abs(L(lim, -a, -b) + L(lim, a, b)) < 1e-15
+            # TVD: result between 0 and 2 min(|a|,|b|), same sign as a and b
+            r = L(lim, a, b)
+            if a * b <= 0:
+                assert r == 0.0
+            else:
+                assert np.sign(r) == np.sign(a) and abs(r) <= 2 * min(abs(a), abs(b)) + 1e-15
+
+
+def test_venkat_properties():
+    rng = np.random.default_rng(7)
+    for _ in range(500):
+        dmax, dmin = abs(rng.normal()), -abs(rng.normal())
+        d2 = rng.normal() * 3
+        eps2 = abs(rng.normal()) * 1e-3
+        p = O.venkat_phi(dmax, dmin, d2, eps2)
+        assert 0.0 <= p <= 1.0
+        # without smoothing the limited extrapolation stays within the neighbour bounds
+        p0 = O.venkat_phi(dmax, dmin, d2, 0.0)
+        assert dmin - 1e-12 <= p0 * d2 <= dmax + 1e-12
+    assert O.venkat_phi(1.0, -1.0, 0.0, 0.0) == 1.0
+    # smooth region: |d2| << eps -> phi ~ 1
+    assert O.venkat_phi(1e-3, -1e-3, 1e-6, 1.0) > 0.999999
+
+
+# --- A1 geometry, A4/A5 gradients, A6 reconstruction on meshes ------------------------------
+
+def test_geometry_closed_forms():
+    m = M.channel_quad(8, 4, 1.0, 0.5, 0.0, 1)
+    assert np.allclose(m.cvol[:m.ncell_int], (1 / 8) * (0.5 / 4))
+    assert np.allclose(m.fsurf, 0.125)  # 1/8 x 0.5/4 cells: every face is 0.125 long
+    mt = M.box_tets(3, 1.0, 0.1, 4, periodic=False)
+    assert abs(mt.cvol[:mt.ncell_int].sum() - 1.0) < 1e-14
+    S_ = mt.fnorm * mt.fsurf[:, None]
+    acc = np.zeros((mt.ncell, 3))
+    np.add.at(acc, mt.ic1, S_)
+    np.add.at(acc, mt.ic2, -S_)
+    assert np.abs(acc[:mt.ncell_int]).max() < 1e-15
+    # normals point ic1 -> ic2
+    d = mt.ccent[mt.ic2] - mt.ccent[mt.ic1]
+    assert (np.einsum("ij,ij->i", d, mt.fnorm) > 0).all()
+
+
+def test_meshgeom_c_restatement_matches_builder():
+    for m in (M.channel_quad(6, 5, 1.0, 0.5, 0.2, 3), M.box_tets(3, 1.0, 0.1, 5, periodic=False),
+              M.box_hex(3, 1.0, 0.1, 6, periodic=False)):
+        ic2 = m.ic2.copy()
+        ic2[m.nface_int:] = -1
+        g = O.meshgeom(m.dim, m.vtx, m.fvtx, m.ic1, ic2, m.ncell_int)
+        np.testing.assert_allclose(g["fnorm"], m.fnorm, atol=1e-15)
+        np.testing.assert_allclose(g["fsurf"], m.fsurf, rtol=1e-14)
+        np.testing.assert_allclose(g["ccent"], m.ccent[:m.ncell_int], atol=1e-14)
+        np.testing.assert_allclose(g["cvol"], m.cvol[:m.ncell_int], rtol=1e-13)
+
+
+def _linear_field_oracle(m, scheme):
+    x = m.ccent[:m.ncell_int]
+    rho = 1.0 + 0.1 * x[:, 0] - 0.05 * x[:, 1] + 0.03 * x[:, 2]
+    p = 1.0 + 0.2 * x[:, 1]
+    vel = np.vstack([0.1 + 0.01 * x[:, 0], 0.02 * x[:, 1], np.zeros_like(rho) if m.dim == 2 else 0.01 * x[:, 2]])
+    from typhon_amd.solver import prim_to_cons
+    o = K.make_oracle(m, scheme, S.default_gas())
+    o.set_cons(prim_to_cons(rho, p, vel))
+    o.calc_rhs(1e-3)
+    return o
+
+
+def _deep_cells(m):
+    """interior cells whose face neighbours are all interior (not ghosts)"""
+    bad = np.zeros(m.ncell_int, dtype=bool)
+    bad[m.ic1[m.nface_int:]] = True
+    return ~bad
+
+
+@pytest.mark.parametrize("gm", [S.GRAD_LSQ, S.GRAD_LSQW])
+def test_lsq_exact_on_linear_fields(gm):
+    for m in (M.channel_quad(8, 6, 1.0, 0.5, 0.3, 8, [dict(family=f, type=M.BC_OUTLET_SUP)
+                                                       for f in ("JMIN", "JMAX", "IMIN", "IMAX")]),
+              M.box_tets(3, 1.0, 0.1, 9, periodic=False,
+                         bocos=[dict(family=f, type=M.BC_OUTLET_SUP)
+                                for f in ("IMIN", "IMAX", "JMIN", "JMAX", "KMIN", "KMAX")])):
+        sch = S.default_scheme(method="M", limiter="X", gradmeth=gm)
+        o = _linear_field_oracle(m, sch)
+        g = o.grad()[:, :m.ncell_int]
+        deep = _deep_cells(m)
+        assert deep.sum() > 0
+        exp = {0: (0.1, -0.05, 0.03 if m.dim == 3 else 0.0), 1: (0.0, 0.2, 0.0), 2: (0.01, 0.0, 0.0),
+               3: (0.0, 0.02, 0.0)}
+        for v, gv in exp.items():
+            for d in range(m.dim):
+                np.testing.assert_allclose(g[3 * v + d, deep], gv[d], atol=1e-12)
+
+
+def test_muscl_unlimited_reconstruction_exact_on_linear_fields():
+    m = M.box_tets(3, 1.0, 0.1, 10, periodic=False,
+                   bocos=[dict(family=f, type=M.BC_OUTLET_SUP) for f in ("IMIN", "IMAX", "JMIN", "JMAX", "KMIN", "KMAX")])
+    for method in ("M", "U"):
+        o = _linear_field_oracle(m, S.default_scheme(method=method, limiter="X"))
+        qL, qR = o.face_states()
+        deep = _deep_cells(m)
+        inner = np.nonzero((np.arange(m.nface) < m.nface_int) & deep[m.ic1] & deep[np.minimum(m.ic2, m.ncell_int - 1)])[0]
+        xf = m.fcent[inner]
+        rho_f = 1.0 + 0.1 * xf[:, 0] - 0.05 * xf[:, 1] + 0.03 * xf[:, 2]
+        np.testing.assert_allclose(qL[0, inner], rho_f, atol=1e-13)
+        np.testing.assert_allclose(qR[0, inner], rho_f, atol=1e-13)
+
+
+# --- A12 gather, A13 time step, A14 RK --------------------------------------------------------
+
+@pytest.mark.parametrize("scheme", [dict(method="N"), dict(method="M", limiter="A"), dict(method="F", limiter="M"),
+                                    dict(method="U", limiter="V", postlimiter=20), dict(method="M", limiter="W")])
+def test_freestream_preservation_and_conservation(scheme):
+    m = M.box_tets(3, 1.0, 0.15, 11, periodic=True)
+    sch = S.default_scheme(**scheme)
+    o = K.make_oracle(m, sch, S.default_gas())
+    n = m.ncell_int
+    from typhon_amd.solver import prim_to_cons
+    u = prim_to_cons(np.ones(n), np.ones(n), np.tile([[0.3], [-0.2], [0.1]], (1, n)))
+    o.set_cons(u)
+    r = o.calc_rhs(0.01)
+    assert np.abs(r).max() < 1e-15
+    # conservation on a non-uniform field: sum V R = 0 on a periodic mesh
+    x = m.ccent[:n]
+    u2 = prim_to_cons(1 + 0.2 * np.sin(2 * np.pi * x[:, 0]), 1 + 0.1 * np.cos(2 * np.pi * x[:, 1]),
+                      np.vstack([0.3 + 0.1 * np.sin(2 * np.pi * x[:, 2]), -0.2 * np.ones(n), 0.1 * np.ones(n)]))
+    o.set_cons(u2)
+    r = o.calc_rhs(0.01)
+    tot = (r * m.cvol[:n]).sum(axis=1)
+    assert np.abs(tot).max() < 1e-15 * max(1.0, np.abs(r * m.cvol[:n]).sum())
+
+
+def test_timestep_closed_form():
+    m = M.channel_quad(10, 5, 1.0, 0.5, 0.0, 1, [dict(family=f, type=M.BC_SYM) for f in ("JMIN", "JMAX", "IMIN", "IMAX")])
+    n = m.ncell_int
+    from typhon_amd.solver import prim_to_cons
+    o = K.make_oracle(m, S.default_scheme(), S.default_gas())
+    o.set_cons(prim_to_cons(np.full(n, 1.3), np.full(n, 0.9), np.tile([[0.4], [0.3], [0.0]], (1, n))))
+    h = 0.1
+    a = np.sqrt(G * 0.9 / 1.3)
+    exp = 0.8 * 2 * h * h / (4 * h) / (0.5 + a)
+    assert abs(o.timestep(0.8) - exp) < 1e-15
+    assert o.timestep(0.8, 1e-5) == 1e-5
+    bad = prim_to_cons(np.full(n, 1.3), np.full(n, 0.9), np.tile([[0.4], [0.3], [0.0]], (1, n)))
+    bad[1, 3] = 0.0
+    o.set_cons(bad)
+    assert o.timestep(0.8) < 0
+
+
+RK_ORDER = {S.TPS_EXPL: 1, S.TPS_RK2: 2, S.TPS_RK2SSP: 2, S.TPS_RK3SSP: 3, S.TPS_RK4: 4,
+            S.TPS_LSRK25BB: 2, S.TPS_LSRK26BB: 2, S.TPS_LSRK12BS: 1, S.TPS_LSRK13BS: 1}
+
+
+def rk_polynomial(coef, z):
+    """amplification factor of the reference's RK combination (treelevel_rungekutta.f90:60-113)
+    applied to dU/dt = lambda U, z = lambda dt"""
+    ns = coef.shape[0]
+    R = []
+    for s in range(ns):
+        U = 1.0 + sum(coef[s - 1, j] * R[j] for j in range(s)) if s > 0 else 1.0
+        R.append(z * U)
+    return 1.0 + sum(coef[ns - 1, j] * R[j] for j in range(ns))
+
+
+@pytest.mark.parametrize("tps", list(RK_ORDER))
+def test_rk_tableau_order(tps):
+    coef = S.rk_tableau(tps)
+    p = RK_ORDER[tps]
+    z = np.array([1e-2, 2e-2])
+    err = np.abs(np.array([rk_polynomial(coef, zz) for zz in z]) - np.exp(z))
+    # local error O(z^(p+1)): doubling z multiplies the error by ~2^(p+1)
+    ratio = err[1] / err[0]
+    assert abs(np.log2(ratio) - (p + 1)) < 0.15, (tps, ratio)
+
+
+@pytest.mark.parametrize("tps", list(RK_ORDER))
+def test_rk_step_on_advected_fourier_mode(tps):
+    """Uniform u and p, density wave along x on a periodic quad mesh, first-order HLLC: the
+    contact is exactly upwinded, so one RK step multiplies each Fourier mode of rho by
+    P(z), z = -(u dt/h)(1 - exp(-i k h)) -- a closed form for the whole oracle RK path."""
+    nx, ny = 16, 2
+    m = M.periodic_quad(nx, ny, 1.0, 1.0 / 8, 0.0, 1)
+    n = m.ncell_int
+    x = m.ccent[:n, 0]
+    kx = 2 * np.pi * 3
+    eps = 1e-3
+    rho = 1.0 + eps * np.cos(kx * x)
+    from typhon_amd.solver import prim_to_cons
+    u0 = prim_to_cons(rho, np.ones(n), np.tile([[0.5], [0.0], [0.0]], (1, n)))
+    coef = S.rk_tableau(tps)
+    o = K.make_oracle(m, S.default_scheme(method="N", tps_meth=tps, rk_coef=coef), S.default_gas())
+    o.set_cons(u0)
+    h = 1.0 / nx
+    dt = 0.4 * h / 0.5
+    o.rk_step(dt)
+    z = -(0.5 * dt / h) * (1 - np.exp(-1j * kx * h))
+    P = rk_polynomial(coef, z)
+    exp = 1.0 + eps * np.real(P * np.exp(1j * kx * x))
+    np.testing.assert_allclose(o.get_cons()[0], exp, atol=1e-14)

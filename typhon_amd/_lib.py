@@ -26,7 +26,8 @@ class TyBocoDesc(C.Structure):
 class TyMeshDesc(C.Structure):
     _fields_ = [("dim", C.c_int32), ("ncell_int", C.c_int32), ("ncell", C.c_int32), ("nface", C.c_int32),
                 ("nface_int", C.c_int32), ("facecell", I32P), ("face", F64P), ("centre", F64P),
-                ("volume", F64P), ("nboco", C.c_int32), ("boco", C.POINTER(TyBocoDesc))]
+                ("volume", F64P), ("nboco", C.c_int32), ("boco", C.POINTER(TyBocoDesc)),
+                ("cell_order", C.c_int32)]
 
 
 class TyGas(C.Structure):

@@ -4,7 +4,7 @@
 // ghost copies, CFDTOOLS/Mesh/USTMESH.f90:31-52) into the device layout:
 //   - interior cells renumbered by reverse Cuthill-McKee,
 //   - periodic face pairs merged into one two-sided face whose far side is
-//     translated by a shift from a small table (init_connect_grid.f90:226-231),
+//     translated by the pair's face-centre difference (init_connect_grid.f90:226-231),
 //   - internal faces sorted by their lower cell, boundary faces last,
 //   - per-cell face slots in ELL form [slot][cell].
 // Everything stays resident; only dt and optional monitors come back.
@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -43,8 +44,8 @@ struct ty_ctx {
     std::vector<double> cc, vol, sumA;  // [3][n], [n], [n]
     std::vector<double> gc;  // [3][nb]
     std::vector<int> bboco;  // [nb]
-    std::vector<double> shift;  // [MAX_SHIFT][3]
-    int nshift = 1;
+    std::vector<double> ps;  // [3][nper] periodic translations, faces per_base..per_base+nper-1
+    int per_base = 0, nper = 0;
     std::vector<ty_boco_desc> bocos;  // types and params (pointers not kept)
     bool mesh_ok = false;
     // scheme
@@ -73,7 +74,7 @@ struct ty_ctx {
     double* d_rstore = nullptr;
     int nstore = 0;
     void* d_grad = nullptr;
-    void* d_ratio = nullptr;
+    void* d_qf = nullptr;
     double* d_res = nullptr;
     double* d_part = nullptr;
     int* d_bad = nullptr;
@@ -113,7 +114,7 @@ void free_state(ty_ctx* c)
     c->d_boco = nullptr;
     c->d_vol = c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
     c->d_us[0] = c->d_us[1] = nullptr;
-    c->d_grad = c->d_ratio = nullptr;
+    c->d_grad = c->d_qf = nullptr;
     c->d_bad = c->d_err = nullptr;
     for (auto* v : {&c->ev_grad, &c->ev_flux, &c->ev_dt})
         for (auto& e : *v) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
@@ -194,6 +195,47 @@ std::vector<int> rcm(int n, const std::vector<int>& xadj, const std::vector<int>
         }
     }
     std::reverse(order.begin(), order.end());
+    return order;
+}
+
+// Hilbert-curve order of the cell centres (J. Skilling, "Programming the Hilbert
+// curve", AIP Conf. Proc. 707 (2004) 381: axes -> transposed index, then bit interleave)
+std::vector<int> hilbert_order(int n, const double* centre, int dim)
+{
+    const int bits = dim == 3 ? 21 : 31;
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < dim; ++d) {
+            lo[d] = std::min(lo[d], centre[(size_t)3 * i + d]);
+            hi[d] = std::max(hi[d], centre[(size_t)3 * i + d]);
+        }
+    const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], dim == 3 ? hi[2] - lo[2] : 0.0, 1e-300});
+    const double scale = (double)((1u << (bits - 1)) - 1) * 2.0 / span;
+    std::vector<std::pair<uint64_t, int>> key(n);
+    for (int i = 0; i < n; ++i) {
+        uint32_t X[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) X[d] = (uint32_t)((centre[(size_t)3 * i + d] - lo[d]) * scale);
+        const uint32_t M = 1u << (bits - 1);
+        for (uint32_t Q = M; Q > 1; Q >>= 1) {
+            const uint32_t P = Q - 1;
+            for (int d = 0; d < dim; ++d) {
+                if (X[d] & Q) X[0] ^= P;
+                else { uint32_t t = (X[0] ^ X[d]) & P; X[0] ^= t; X[d] ^= t; }
+            }
+        }
+        for (int d = 1; d < dim; ++d) X[d] ^= X[d - 1];
+        uint32_t t = 0;
+        for (uint32_t Q = M; Q > 1; Q >>= 1)
+            if (X[dim - 1] & Q) t ^= Q - 1;
+        for (int d = 0; d < dim; ++d) X[d] ^= t;
+        uint64_t h = 0;
+        for (int b = bits - 1; b >= 0; --b)
+            for (int d = 0; d < dim; ++d) h = (h << 1) | ((X[d] >> b) & 1u);
+        key[i] = {h, i};
+    }
+    std::stable_sort(key.begin(), key.end());
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = key[i].second;
     return order;
 }
 
@@ -291,64 +333,66 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         if (fboco[f] < 0) return fail(c, TY_ERR_ARG, "boundary face " + std::to_string(f + 1) + " has no boco");
     for (int f = nfi; f < nf; ++f)
         if (fpart[f] >= 0 && fpart[fpart[f]] != f) return fail(c, TY_ERR_ARG, "periodic partners are not mutual");
-    // device faces before renumbering: owner, other (or -1 boundary), source face, shift id
-    struct DF { int a, b, src, sid, bidx; };
-    std::vector<DF> faces;
-    faces.reserve(nf);
-    c->shift.assign(MAX_SHIFT * 3, 0.0);
-    c->nshift = 1;
-    for (int f = 0; f < nfi; ++f) faces.push_back({ic1[f], ic2[f], f, 0, -1});
-    std::vector<DF> bfaces;
+    // device faces before renumbering: owner, other (or -1 boundary), source face, periodic
+    // partner face (or -1), boco index
+    struct DF { int a, b, src, part, bidx; };
+    std::vector<DF> faces, pfaces, bfaces;
+    faces.reserve(nfi);
+    for (int f = 0; f < nfi; ++f) faces.push_back({ic1[f], ic2[f], f, -1, -1});
     for (int f = nfi; f < nf; ++f) {
         if (fpart[f] >= 0) {
             int p = fpart[f];
-            if (f > p) continue;
-            double T[3] = {F(f, 3) - F(p, 3), F(f, 4) - F(p, 4), F(f, 5) - F(p, 5)};
-            double tn = std::sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]);
-            int sid = -1;
-            for (int s = 1; s < c->nshift; ++s) {
-                double dx = T[0] - c->shift[3 * s], dy = T[1] - c->shift[3 * s + 1], dz = T[2] - c->shift[3 * s + 2];
-                if (std::sqrt(dx * dx + dy * dy + dz * dz) <= 1e-9 * (1.0 + tn)) { sid = s; break; }
-            }
-            if (sid < 0) {
-                if (c->nshift >= MAX_SHIFT) return fail(c, TY_ERR_UNSUPPORTED, "too many periodic translations");
-                sid = c->nshift++;
-                for (int d = 0; d < 3; ++d) c->shift[3 * sid + d] = T[d];
-            }
-            faces.push_back({ic1[f], ic1[p], f, sid, -1});
+            if (f < p) pfaces.push_back({ic1[f], ic1[p], f, p, -1});
         } else {
-            bfaces.push_back({ic1[f], -1, f, 0, fboco[f]});
+            bfaces.push_back({ic1[f], -1, f, -1, fboco[f]});
         }
     }
-    // RCM over interior cells
-    std::vector<int> xadj(ni + 1, 0), adj;
-    for (auto& d : faces) { xadj[d.a + 1]++; xadj[d.b + 1]++; }
-    for (int i = 0; i < ni; ++i) xadj[i + 1] += xadj[i];
-    adj.resize(xadj[ni]);
-    {
+    // cell order: RCM over the face graph, a Hilbert curve through the cell centres, or the caller's
+    std::vector<int> order;
+    if (m->cell_order == 1) {
+        order = hilbert_order(ni, m->centre, m->dim);
+    } else if (m->cell_order == 2) {
+        order.resize(ni);
+        for (int i = 0; i < ni; ++i) order[i] = i;
+    } else {
+        std::vector<int> xadj(ni + 1, 0), adj;
+        for (auto* L : {&faces, &pfaces})
+            for (auto& d : *L) { xadj[d.a + 1]++; xadj[d.b + 1]++; }
+        for (int i = 0; i < ni; ++i) xadj[i + 1] += xadj[i];
+        adj.resize(xadj[ni]);
         std::vector<int> pos(xadj.begin(), xadj.end() - 1);
-        for (auto& d : faces) { adj[pos[d.a]++] = d.b; adj[pos[d.b]++] = d.a; }
+        for (auto* L : {&faces, &pfaces})
+            for (auto& d : *L) { adj[pos[d.a]++] = d.b; adj[pos[d.b]++] = d.a; }
+        order = rcm(ni, xadj, adj);
     }
-    c->perm = rcm(ni, xadj, adj);
+    c->perm = order;
     c->iperm.assign(ni, 0);
     for (int i = 0; i < ni; ++i) c->iperm[c->perm[i]] = i;
-    for (auto& d : faces) { d.a = c->iperm[d.a]; d.b = c->iperm[d.b]; }
-    for (auto& d : bfaces) d.a = c->iperm[d.a];
-    std::stable_sort(faces.begin(), faces.end(), [](const DF& x, const DF& y) {
+    auto lowfirst = [](const DF& x, const DF& y) {
         int xl = std::min(x.a, x.b), yl = std::min(y.a, y.b);
         if (xl != yl) return xl < yl;
         return std::max(x.a, x.b) < std::max(y.a, y.b);
-    });
+    };
+    for (auto* L : {&faces, &pfaces})
+        for (auto& d : *L) { d.a = c->iperm[d.a]; d.b = c->iperm[d.b]; }
+    for (auto& d : bfaces) d.a = c->iperm[d.a];
+    std::stable_sort(faces.begin(), faces.end(), lowfirst);
+    std::stable_sort(pfaces.begin(), pfaces.end(), lowfirst);
     std::stable_sort(bfaces.begin(), bfaces.end(), [](const DF& x, const DF& y) { return x.a < y.a; });
-    c->nfi = (int)faces.size();
+    // device face ranges: internal | periodic pairs | physical boundary
+    c->per_base = (int)faces.size();
+    c->nper = (int)pfaces.size();
+    c->nfi = c->per_base + c->nper;
     c->nb = (int)bfaces.size();
     c->nf = c->nfi + c->nb;
     const int n = ni, ndf = c->nf;
+    if ((unsigned)ndf > FACE_MASK) return fail(c, TY_ERR_UNSUPPORTED, "too many faces for the slot word");
     c->fn.assign((size_t)3 * ndf, 0.0);
     c->fc.assign((size_t)3 * ndf, 0.0);
     c->fs.assign(ndf, 0.0);
     c->gc.assign((size_t)3 * c->nb, 0.0);
     c->bboco.assign(c->nb, 0);
+    c->ps.assign((size_t)3 * std::max(1, c->nper), 0.0);
     std::vector<std::vector<std::pair<int, int>>> slots(n);  // (nbr, word)
     auto put_face = [&](int df, int src) {
         for (int d = 0; d < 3; ++d) {
@@ -358,11 +402,18 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         c->fs[df] = F(src, 6);
     };
     for (int k = 0; k < c->nfi; ++k) {
-        const DF& d = faces[k];
+        const bool per = k >= c->per_base;
+        const DF& d = per ? pfaces[k - c->per_base] : faces[k];
         put_face(k, d.src);
-        int w = k | (d.sid << FACE_BITS);
-        slots[d.a].push_back({d.b, w});
-        slots[d.b].push_back({d.a, (int)((unsigned)w | 0x80000000u)});
+        unsigned w = (unsigned)k;
+        if (per) {
+            // far side seen through this face: x_far + (c_face - c_partner) (init_connect_grid.f90:226-231)
+            const int j = k - c->per_base;
+            for (int e = 0; e < 3; ++e) c->ps[(size_t)e * c->nper + j] = F(d.src, 3 + e) - F(d.part, 3 + e);
+            w |= PERIODIC_BIT;
+        }
+        slots[d.a].push_back({d.b, (int)w});
+        slots[d.b].push_back({d.a, (int)(w | 0x80000000u)});
     }
     for (int k = 0; k < c->nb; ++k) {
         const DF& d = bfaces[k];
@@ -373,19 +424,31 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         c->bboco[k] = d.bidx;
         slots[d.a].push_back({-(k + 1), df});
     }
-    if ((unsigned)c->nf > FACE_MASK) return fail(c, TY_ERR_UNSUPPORTED, "too many faces for the slot word");
     int ns = 0;
-    for (auto& s : slots) ns = std::max<int>(ns, (int)s.size());
+    for (auto& sl : slots) ns = std::max<int>(ns, (int)sl.size());
     if (ns > MAX_SLOTS) return fail(c, TY_ERR_UNSUPPORTED, "more than 8 faces per cell");
     c->ns = ns;
     c->nbr.assign((size_t)ns * n, 0);
     c->fw.assign((size_t)ns * n, SLOT_UNUSED);
     c->sumA.assign(n, 0.0);
+    // mirror slots: where the far cell keeps the same face
+    std::vector<unsigned char> slot_own(c->nfi, 0), slot_oth(c->nfi, 0);
     for (int i = 0; i < n; ++i)
         for (size_t k = 0; k < slots[i].size(); ++k) {
+            if (slots[i][k].first < 0) continue;
+            const unsigned w = (unsigned)slots[i][k].second;
+            (w & 0x80000000u ? slot_oth : slot_own)[w & FACE_MASK] = (unsigned char)k;
+        }
+    for (int i = 0; i < n; ++i)
+        for (size_t k = 0; k < slots[i].size(); ++k) {
+            unsigned w = (unsigned)slots[i][k].second;
+            if (slots[i][k].first >= 0) {
+                const unsigned mk = (w & 0x80000000u) ? slot_own[w & FACE_MASK] : slot_oth[w & FACE_MASK];
+                w |= mk << MIRROR_SHIFT;
+            }
             c->nbr[k * n + i] = slots[i][k].first;
-            c->fw[k * n + i] = slots[i][k].second;
-            c->sumA[i] += c->fs[(unsigned)slots[i][k].second & FACE_MASK];
+            c->fw[k * n + i] = (int)w;
+            c->sumA[i] += c->fs[w & FACE_MASK];
         }
     c->cc.assign((size_t)3 * n, 0.0);
     c->vol.assign(n, 0.0);
@@ -419,16 +482,17 @@ int build_geo(ty_ctx* c, Geo<R>& g)
 {
     const int n = c->n, nf = c->nf, nb = c->nb, dim = c->dim;
     std::vector<R> h;
-    // layout: fn(3nf) fs(nf) fc(3nf) cc(3n) gc(3nb) lsq(6n) eps2(n) shift(3*MAX_SHIFT)
+    const int np = std::max(1, c->nper);
+    // layout: fn(3nf) fs(nf) fc(3nf) cc(3n) gc(3nb) lsq(6n) eps2(n) ps(3 nper)
     size_t o_fn = 0, o_fs = o_fn + 3 * (size_t)nf, o_fc = o_fs + nf, o_cc = o_fc + 3 * (size_t)nf;
     size_t o_gc = o_cc + 3 * (size_t)n, o_lsq = o_gc + 3 * (size_t)nb, o_eps = o_lsq + 6 * (size_t)n;
-    size_t o_sh = o_eps + n, tot = o_sh + 3 * MAX_SHIFT;
+    size_t o_sh = o_eps + n, tot = o_sh + 3 * (size_t)np;
     h.assign(tot, R(0));
     for (size_t i = 0; i < 3 * (size_t)nf; ++i) { h[o_fn + i] = (R)c->fn[i]; h[o_fc + i] = (R)c->fc[i]; }
     for (int i = 0; i < nf; ++i) h[o_fs + i] = (R)c->fs[i];
     for (size_t i = 0; i < 3 * (size_t)n; ++i) h[o_cc + i] = (R)c->cc[i];
     for (size_t i = 0; i < 3 * (size_t)nb; ++i) h[o_gc + i] = (R)c->gc[i];
-    for (int i = 0; i < 3 * MAX_SHIFT; ++i) h[o_sh + i] = (R)c->shift[i];
+    for (size_t i = 0; i < 3 * (size_t)np; ++i) h[o_sh + i] = (R)c->ps[i];
     // LSQ matrices (precalc_grad_lsq.f90:60-155) and their Cholesky factors, in FP64
     const bool w = c->sch.gradmeth == 12;
     const double K = c->sch.venkat_k;
@@ -441,9 +505,12 @@ int build_geo(ty_ctx* c, Geo<R>& g)
             int nbv = c->nbr[(size_t)k * n + i];
             double d[3];
             if (nbv >= 0) {
-                int sid = ((unsigned)wd >> FACE_BITS) & (MAX_SHIFT - 1);
-                bool other = wd < 0;
-                const double* T = &c->shift[3 * sid];
+                const bool other = wd < 0;
+                double T[3] = {0.0, 0.0, 0.0};
+                if ((unsigned)wd & PERIODIC_BIT) {
+                    const int j = (int)((unsigned)wd & FACE_MASK) - c->per_base;
+                    for (int e = 0; e < 3; ++e) T[e] = c->ps[(size_t)e * c->nper + j];
+                }
                 for (int e = 0; e < 3; ++e) {
                     double xo = c->cc[(size_t)e * n + nbv];
                     d[e] = other ? (x[e] + T[e]) - xo : (xo + T[e]) - x[e];
@@ -486,7 +553,10 @@ int build_geo(ty_ctx* c, Geo<R>& g)
     g.gcz = d + o_gc + 2 * (size_t)nb;
     g.lsq = d + o_lsq;
     g.eps2 = d + o_eps;
-    g.shift = d + o_sh;
+    g.psx = d + o_sh;
+    g.psy = d + o_sh + np;
+    g.psz = d + o_sh + 2 * (size_t)np;
+    g.per_base = c->per_base;
     g.bboco = c->d_bboco;
     return TY_OK;
 }
@@ -610,12 +680,17 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
         c->owned.push_back(p);
         c->d_us[k] = p;
     }
-    HIPCHK(c, hipMalloc(&p, (size_t)c->nv * c->dim * c->n * rb));
-    c->owned.push_back(p);
-    c->d_grad = p;
-    HIPCHK(c, hipMalloc(&p, (size_t)3 * c->n * rb));
-    c->owned.push_back(p);
-    c->d_ratio = p;
+    c->d_grad = c->d_qf = nullptr;
+    if (viscous) {  // cell gradients for the viscous flux
+        HIPCHK(c, hipMalloc(&p, (size_t)c->nv * c->dim * c->n * rb));
+        c->owned.push_back(p);
+        c->d_grad = p;
+    }
+    if (s->method != 'N') {  // own-side face states of every slot
+        HIPCHK(c, hipMalloc(&p, (size_t)c->ns * c->nv * c->n * rb));
+        c->owned.push_back(p);
+        c->d_qf = p;
+    }
     const int nblk = grid_blocks(c->n);
     if ((rc = dalloc(c, &c->d_part, nblk)) || (rc = dalloc(c, &c->d_bad, nblk)) || (rc = dalloc(c, &c->d_dt, 1)) ||
         (rc = dalloc(c, &c->d_err, 1)))
@@ -691,8 +766,8 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     a.rstore = c->d_rstore;
     a.grad = (const R*)c->d_grad;
     a.grad_out = (R*)c->d_grad;
-    a.ratio = (const R*)c->d_ratio;
-    a.ratio_out = (R*)c->d_ratio;
+    a.qf = (const R*)c->d_qf;
+    a.qf_out = (R*)c->d_qf;  // null for first-order schemes
     a.res_out = c->d_res;
     a.d_dt = c->d_dt;
     a.store_slot = -1;
@@ -724,10 +799,6 @@ hipError_t eval_kernels(ty_ctx* c, KArgs<R>& a)
     if (grad) {
         e = timed<R>(c, c->ev_grad, &launch_gradient<DIM, R>, a);
         if (e != hipSuccess) return e;
-        if (c->ph.barth) {
-            e = launch_barth<DIM, R>(a, c->stream);
-            if (e != hipSuccess) return e;
-        }
     }
     return timed<R>(c, c->ev_flux, &launch_flux<DIM, R>, a);
 }

@@ -12,8 +12,8 @@ constexpr int MAX_SLOTS = 8;     // faces per cell
 constexpr int MAX_STAGES = 8;    // RK stages
 constexpr int FACE_BITS = 26;    // face id bits of a slot word
 constexpr uint32_t FACE_MASK = (1u << FACE_BITS) - 1u;
-constexpr int SHIFT_BITS = 5;    // periodic shift id bits
-constexpr int MAX_SHIFT = 1 << SHIFT_BITS;
+constexpr uint32_t PERIODIC_BIT = 1u << FACE_BITS;  // face joins periodic partners
+constexpr int MIRROR_SHIFT = 27;  // bits 27-29: slot of the same face in the far cell's list
 constexpr int SLOT_UNUSED = -1;  // slot word of an empty slot
 constexpr int BLOCK = 256;       // threads per block (4 wavefronts of 64)
 constexpr int MAX_BOCO = 64;
@@ -41,7 +41,9 @@ struct Geo {
     const int* bboco;               // [nb] boco index
     const R* lsq;                   // [6][n] Cholesky factors l11 l21 l31 l22 l32 l33
     const R* eps2;                  // [n] Venkatakrishnan (K h)^3
-    const R* shift;                 // [MAX_SHIFT][3] periodic translation of the other side
+    const R *psx, *psy, *psz;       // [nper] translation of the far side of periodic face
+                                    // per_base + k (face centre difference of the pair)
+    int per_base;
 };
 
 struct Physics {
@@ -63,7 +65,8 @@ template <typename R>
 struct KArgs {
     int n, ns, nblk_real;
     const int* nbr;  // [ns][n] >=0 neighbour cell, <0: -(boundary face + 1)
-    const int* fw;   // [ns][n] slot word: face id | shift id << 26 | other-side << 31; -1 unused
+    const int* fw;   // [ns][n] slot word: face id | periodic << 26 | mirror slot << 27 | other-side << 31;
+                     // -1 unused
     Geo<R> g;
     const BocoDev* boco;
     const double* vol;    // [n]
@@ -75,10 +78,10 @@ struct KArgs {
     double* u0;           // [nv][n]
     double* acc;          // [nv][n]
     double* rstore;       // [nstore][nv][n]
-    const R* grad;        // [nv*dim][n] (limited when the Venkatakrishnan limiter is on)
-    R* grad_out;
-    const R* ratio;       // [3][n] Barth ratios (rho, p, velocity)
-    R* ratio_out;
+    const R* grad;        // [nv*dim][n] LSQ gradients (viscous flux)
+    R* grad_out;          // written by K1 when not null
+    const R* qf;          // [ns][nv][n] own-side face states (second order)
+    R* qf_out;            // written by K1 when not null
     double* res_out;      // [nv][n] residual-only mode
     const double* d_dt;   // device time step
     // RK bookkeeping for this stage
@@ -102,8 +105,6 @@ struct DtArgs {
 // launchers (ty_kernels.hip)
 template <int DIM, typename R>
 hipError_t launch_gradient(const KArgs<R>& a, hipStream_t s);
-template <int DIM, typename R>
-hipError_t launch_barth(const KArgs<R>& a, hipStream_t s);
 template <int DIM, typename R>
 hipError_t launch_flux(const KArgs<R>& a, hipStream_t s);
 template <int DIM>

@@ -399,8 +399,8 @@ __device__ __forceinline__ R venkat_phi(R dmax, R dmin, R d2, R eps2)
 // slot decoding and neighbour positions
 
 struct SlotInfo {
-    int nb, f, sid;
-    bool other;
+    int nb, f;
+    bool per, other;
 };
 
 __device__ __forceinline__ SlotInfo decode(int w, int nb)
@@ -408,9 +408,24 @@ __device__ __forceinline__ SlotInfo decode(int w, int nb)
     SlotInfo s;
     s.nb = nb;
     s.f = (int)((unsigned)w & FACE_MASK);
-    s.sid = (int)(((unsigned)w >> FACE_BITS) & (MAX_SHIFT - 1));
+    s.per = ((unsigned)w & PERIODIC_BIT) != 0u;
     s.other = w < 0;
     return s;
+}
+
+// translation of the far side of a face (zero unless periodic)
+template <int DIM, typename R>
+__device__ __forceinline__ void face_shift(const Geo<R>& g, const SlotInfo& s, R* T)
+{
+    if (s.per) {
+        const int k = s.f - g.per_base;
+        T[0] = g.psx[k];
+        T[1] = g.psy[k];
+        if (DIM == 3) T[DIM - 1] = g.psz[k];
+    } else {
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) T[e] = R(0);
+    }
 }
 
 template <int DIM, typename R>
@@ -445,11 +460,102 @@ __device__ __forceinline__ void load_grad(const R* __restrict__ grad, int n, int
 }
 
 // ---------------------------------------------------------------------------
-// K1: LSQ gradients (calc_gradient.f90:75-192 + cholesky_solve MATRIX.f90:79-111),
-// optionally limited by Venkatakrishnan.  Neighbour set = face neighbours incl.
-// ghost cells built by the cell_state boundary map.
+// Far side of one face slot, in the face's canonical orientation (left = owner,
+// the reference's ic1 side; right = the other cell, or the ghost cell built by
+// the cell_state boundary map; positions in the owner's frame).
 
 template <int DIM, typename R>
+struct Side {
+    R qL[DIM + 2], qR[DIM + 2];
+    R xL[DIM], xR[DIM];
+    bool boundary;
+};
+
+template <int DIM, typename R>
+__device__ __forceinline__ void side_ctx(const KArgs<R>& a, int c, const R* qc, const R* xc, const SlotInfo& s,
+                                         const R* nf, Side<DIM, R>& S)
+{
+    constexpr int NV = DIM + 2;
+    const R g1 = R(a.ph.gamma - 1.0);
+    if (s.nb >= 0) {
+        R qo[NV], xo[DIM], T[DIM];
+        load_prim<DIM>(a.us, a.n, s.nb, g1, qo);
+        cell_x<DIM>(a.g, s.nb, xo);
+        face_shift<DIM>(a.g, s, T);
+        if (!s.other) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { S.qL[v] = qc[v]; S.qR[v] = qo[v]; }
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) { S.xL[e] = xc[e]; S.xR[e] = xo[e] + T[e]; }
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { S.qL[v] = qo[v]; S.qR[v] = qc[v]; }
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) { S.xL[e] = xo[e]; S.xR[e] = xc[e] + T[e]; }
+        }
+        S.boundary = false;
+    } else {
+        const int bi = -s.nb - 1;
+        const BocoDev bc = a.boco[a.g.bboco[bi]];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) S.qL[v] = qc[v];
+        bc_state<DIM>(bc, nf, qc, S.qR, a.ph);
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) S.xL[e] = xc[e];
+        S.xR[0] = a.g.gcx[bi];
+        S.xR[1] = a.g.gcy[bi];
+        if (DIM == 3) S.xR[DIM - 1] = a.g.gcz[bi];
+        S.boundary = true;
+    }
+}
+
+// One side's MUSCL-family face value from the canonical face data: this cell is
+// the left side when !other (hres_ns_muscl.f90:51-143 and its variants).
+template <int DIM, typename R>
+__device__ __forceinline__ void muscl_own(const Physics& ph, const Side<DIM, R>& S, const R* xf, const R* g,
+                                          bool other, R* qf)
+{
+    constexpr int NV = DIM + 2;
+    R u[DIM], CF[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        u[d] = S.xR[d] - S.xL[d];
+        CF[d] = other ? xf[d] - S.xR[d] : xf[d] - S.xL[d];
+    }
+    R dlr, kcf = R(0);
+    if (ph.method == 'F') {
+        R LF[DIM], RF[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) { LF[d] = xf[d] - S.xL[d]; RF[d] = xf[d] - S.xR[d]; }
+        const R kl = sqrt(dotd<DIM>(LF, LF));
+        const R kr = -sqrt(dotd<DIM>(RF, RF));
+        dlr = kl - kr;
+        kcf = other ? kr : kl;
+    } else {
+        dlr = sqrt(dotd<DIM>(u, u));
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) u[d] = u[d] / dlr;
+    const R* q = other ? S.qR : S.qL;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const R lr = (S.qR[v] - S.qL[v]) / dlr;
+        qf[v] = muscl_side<DIM>(ph.method, ph.limiter, q[v], g + v * DIM, u, lr, CF, kcf);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K1: LSQ gradients (calc_gradient.f90:75-192 + cholesky_solve MATRIX.f90:79-111)
+// and this cell's own-side face states for every face slot: MUSCL / MUSCL-FAST /
+// MUSCL-UNS with the reference limiters, Barth post-limitation
+// (postlimit_barth.f90:50-223, ratios of rho, p and the velocity vector), or the
+// Venkatakrishnan cell limiter.  Neighbour set = face neighbours incl. ghost
+// cells from the cell_state boundary map.
+
+// reconstruction performed by K1 after the gradient
+enum Rec : int { REC_NONE = 0, REC_VENKAT = 1, REC_MUSCL = 2, REC_MUSCL_BARTH = 3 };
+
+template <int DIM, typename R, int NSM, int REC>
 __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 {
     constexpr int NV = DIM + 2;
@@ -469,40 +575,26 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) b[v][d] = R(0);
     }
-    for (int k = 0; k < a.ns; ++k) {
+#pragma unroll
+    for (int k = 0; k < NSM; ++k) {
+        if (k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
         const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
-        R d[DIM], dq[NV], qo[NV];
-        if (s.nb >= 0) {
-            load_prim<DIM>(a.us, n, s.nb, g1, qo);
-            R xo[DIM];
-            cell_x<DIM>(a.g, s.nb, xo);
-            const R* T = a.g.shift + 3 * s.sid;
-            // owner frame: d = x_other + T - x_owner, dq = q_other - q_owner
+        R nf[DIM];
+        if (s.nb < 0) face_n<DIM>(a.g, s.f, nf);
+        Side<DIM, R> S;
+        side_ctx<DIM>(a, c, qc, xc, s, nf, S);
+        // owner frame: d = x_R - x_L, dq = q_R - q_L (precalc_grad_lsq.f90:62-73)
+        R d[DIM], dq[NV];
 #pragma unroll
-            for (int e = 0; e < DIM; ++e) d[e] = s.other ? (xc[e] + T[e]) - xo[e] : (xo[e] + T[e]) - xc[e];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) dq[v] = s.other ? qc[v] - qo[v] : qo[v] - qc[v];
-        } else {
-            const int bi = -s.nb - 1;
-            const BocoDev bc = a.boco[a.g.bboco[bi]];
-            R nf[DIM];
-            face_n<DIM>(a.g, s.f, nf);
-            bc_state<DIM>(bc, nf, qc, qo, a.ph);
-            R xg[DIM];
-            xg[0] = a.g.gcx[bi];
-            xg[1] = a.g.gcy[bi];
-            if (DIM == 3) xg[DIM - 1] = a.g.gcz[bi];
-#pragma unroll
-            for (int e = 0; e < DIM; ++e) d[e] = xg[e] - xc[e];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) dq[v] = qo[v] - qc[v];
-        }
+        for (int e = 0; e < DIM; ++e) d[e] = S.xR[e] - S.xL[e];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            qmax[v] = fmax_(qmax[v], qo[v]);
-            qmin[v] = fmin_(qmin[v], qo[v]);
+            dq[v] = S.qR[v] - S.qL[v];
+            const R qo = s.other ? S.qL[v] : S.qR[v];
+            qmax[v] = fmax_(qmax[v], qo);
+            qmin[v] = fmin_(qmin[v], qo);
         }
         if (a.ph.lsqw) {
             const R s2 = dotd<DIM>(d, d);
@@ -517,166 +609,123 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
     }
     // Cholesky solve with the factors of A (precalc_grad_lsq.f90:60-155)
     const R l11 = a.g.lsq[c], l21 = a.g.lsq[(size_t)n + c], l22 = a.g.lsq[(size_t)3 * n + c];
-    R gr[NV][DIM];
+    R gr[NV * DIM];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         if (DIM == 3) {
             const R l31 = a.g.lsq[(size_t)2 * n + c], l32 = a.g.lsq[(size_t)4 * n + c], l33 = a.g.lsq[(size_t)5 * n + c];
-            R y1 = b[v][0] / l11;
-            R y2 = (b[v][1] - l21 * y1) / l22;
-            R y3 = ((b[v][DIM - 1] - l31 * y1) - l32 * y2) / l33;
-            R x3 = y3 / l33;
-            R x2 = (y2 - l32 * x3) / l22;
-            R x1 = ((y1 - l21 * x2) - l31 * x3) / l11;
-            gr[v][0] = x1;
-            gr[v][1] = x2;
-            gr[v][DIM - 1] = x3;
+            const R y1 = b[v][0] / l11;
+            const R y2 = (b[v][1] - l21 * y1) / l22;
+            const R y3 = ((b[v][DIM - 1] - l31 * y1) - l32 * y2) / l33;
+            const R x3 = y3 / l33;
+            const R x2 = (y2 - l32 * x3) / l22;
+            const R x1 = ((y1 - l21 * x2) - l31 * x3) / l11;
+            gr[v * DIM + 0] = x1;
+            gr[v * DIM + 1] = x2;
+            gr[v * DIM + DIM - 1] = x3;
         } else {
-            R y1 = b[v][0] / l11;
-            R y2 = (b[v][1] - l21 * y1) / l22;
-            R x2 = y2 / l22;
-            R x1 = (y1 - l21 * x2) / l11;
-            gr[v][0] = x1;
-            gr[v][1] = x2;
+            const R y1 = b[v][0] / l11;
+            const R y2 = (b[v][1] - l21 * y1) / l22;
+            const R x2 = y2 / l22;
+            const R x1 = (y1 - l21 * x2) / l11;
+            gr[v * DIM + 0] = x1;
+            gr[v * DIM + 1] = x2;
         }
     }
-    if (a.ph.limiter == 'W') {
+    if (a.grad_out) {
+#pragma unroll
+        for (int k = 0; k < NV * DIM; ++k) a.grad_out[(size_t)k * n + c] = gr[k];
+    }
+    if (REC == REC_NONE) return;
+    if (REC == REC_VENKAT) {
+        // Venkatakrishnan: phi_v = min over faces, then q_f = q_c + phi grad(q).(x_f - x_c)
         const R eps2 = a.g.eps2[c];
         R phi[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) phi[v] = R(1);
-        for (int k = 0; k < a.ns; ++k) {
-            const int w = a.fw[(size_t)k * n + c];
-            if (w == SLOT_UNUSED) continue;
-            const SlotInfo s = decode(w, 0);
-            R xf[DIM], r[DIM];
-            face_x<DIM>(a.g, s.f, xf);
-            const R* T = a.g.shift + 3 * s.sid;
+        for (int pass = 0; pass < 2; ++pass) {
 #pragma unroll
-            for (int e = 0; e < DIM; ++e) r[e] = (s.other ? xf[e] - T[e] : xf[e]) - xc[e];
+            for (int k = 0; k < NSM; ++k) {
+                if (k >= a.ns) break;
+                const int w = a.fw[(size_t)k * n + c];
+                if (w == SLOT_UNUSED) continue;
+                const SlotInfo s = decode(w, 0);
+                R xf[DIM], T[DIM], rk[DIM];
+                face_x<DIM>(a.g, s.f, xf);
+                face_shift<DIM>(a.g, s, T);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const R d2 = dotd<DIM>(gr[v], r);
-                phi[v] = fmin_(phi[v], venkat_phi(qmax[v] - qc[v], qmin[v] - qc[v], d2, eps2));
+                for (int e = 0; e < DIM; ++e) rk[e] = (s.other ? xf[e] - T[e] : xf[e]) - xc[e];
+                if (pass == 0) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        const R d2 = dotd<DIM>(gr + v * DIM, rk);
+                        phi[v] = fmin_(phi[v], venkat_phi(qmax[v] - qc[v], qmin[v] - qc[v], d2, eps2));
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v)
+                        a.qf_out[((size_t)k * NV + v) * n + c] = qc[v] + phi[v] * dotd<DIM>(gr + v * DIM, rk);
+                }
             }
         }
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-#pragma unroll
-            for (int e = 0; e < DIM; ++e) gr[v][e] = phi[v] * gr[v][e];
+        return;
     }
-#pragma unroll
-    for (int v = 0; v < NV; ++v)
-#pragma unroll
-        for (int e = 0; e < DIM; ++e) a.grad_out[(size_t)(v * DIM + e) * n + c] = gr[v][e];
-}
-
-// ---------------------------------------------------------------------------
-// shared per-face evaluation: left/right cell data in canonical orientation
-
-template <int DIM, typename R>
-struct FaceCtx {
-    R qL[DIM + 2], qR[DIM + 2];
-    R xL[DIM], xR[DIM], xf[DIM], n[DIM];
-    int owner, other;  // other < 0: boundary (ghost)
-    int bi;
-};
-
-template <int DIM, typename R>
-__device__ __forceinline__ void face_ctx(const KArgs<R>& a, int c, const R* qc, const R* xc, const SlotInfo& s,
-                                         FaceCtx<DIM, R>& F)
-{
-    constexpr int NV = DIM + 2;
-    const R g1 = R(a.ph.gamma - 1.0);
-    face_n<DIM>(a.g, s.f, F.n);
-    face_x<DIM>(a.g, s.f, F.xf);
-    if (s.nb >= 0) {
-        R qo[NV], xo[DIM];
-        load_prim<DIM>(a.us, a.n, s.nb, g1, qo);
-        cell_x<DIM>(a.g, s.nb, xo);
-        const R* T = a.g.shift + 3 * s.sid;
-        if (!s.other) {
-            F.owner = c;
-            F.other = s.nb;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) { F.qL[v] = qc[v]; F.qR[v] = qo[v]; }
-#pragma unroll
-            for (int e = 0; e < DIM; ++e) { F.xL[e] = xc[e]; F.xR[e] = xo[e] + T[e]; }
-        } else {
-            F.owner = s.nb;
-            F.other = c;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) { F.qL[v] = qo[v]; F.qR[v] = qc[v]; }
-#pragma unroll
-            for (int e = 0; e < DIM; ++e) { F.xL[e] = xo[e]; F.xR[e] = xc[e] + T[e]; }
-        }
-        F.bi = -1;
-    } else {
-        F.owner = c;
-        F.other = -1;
-        F.bi = -s.nb - 1;
-        const BocoDev bc = a.boco[a.g.bboco[F.bi]];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) F.qL[v] = qc[v];
-        bc_state<DIM>(bc, F.n, qc, F.qR, a.ph);  // ghost cell state (cell_state mode)
-#pragma unroll
-        for (int e = 0; e < DIM; ++e) F.xL[e] = xc[e];
-        F.xR[0] = a.g.gcx[F.bi];
-        F.xR[1] = a.g.gcy[F.bi];
-        if (DIM == 3) F.xR[DIM - 1] = a.g.gcz[F.bi];
-    }
-}
-
-// K2 (MUSCL family with a Barth post-limiter only): per-cell ratios
-// (postlimit_barth.f90:50-223), groups rho, p, velocity vector.
-template <int DIM, typename R>
-__global__ __launch_bounds__(BLOCK) void k_barth(KArgs<R> a)
-{
-    constexpr int NV = DIM + 2;
-    const int c = xcd_block() * BLOCK + threadIdx.x;
-    if (c >= a.n) return;
-    const int n = a.n;
-    const R g1 = R(a.ph.gamma - 1.0);
+    // MUSCL family, optionally Barth-limited
+    constexpr bool barth = REC == REC_MUSCL_BARTH;
     const R klim = R(a.ph.klim);
-    R qc[NV], xc[DIM], gc[NV * DIM];
-    load_prim<DIM>(a.us, n, c, g1, qc);
-    cell_x<DIM>(a.g, c, xc);
-    load_grad<DIM>(a.grad, n, c, gc);
     R ratio[3] = {R(1), R(1), R(1)};
-    for (int k = 0; k < a.ns; ++k) {
+    R qfk[barth ? NSM : 1][NV];
+#pragma unroll
+    for (int k = 0; k < NSM; ++k) {
+        if (k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
         const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
-        FaceCtx<DIM, R> F;
-        face_ctx<DIM>(a, c, qc, xc, s, F);
-        R qf[NV], dummy[NV];
-        const bool left = (F.owner == c) && !s.other;
-        if (left) muscl_face<DIM>(a.ph, F.qL, F.qR, gc, gc, F.xL, F.xR, F.xf, qf, dummy, 0);
-        else muscl_face<DIM>(a.ph, F.qL, F.qR, gc, gc, F.xL, F.xR, F.xf, dummy, qf, 1);
-        const R* qo = left ? F.qR : F.qL;
+        R nf[DIM], xf[DIM];
+        face_x<DIM>(a.g, s.f, xf);
+        if (s.nb < 0) face_n<DIM>(a.g, s.f, nf);
+        Side<DIM, R> S;
+        side_ctx<DIM>(a, c, qc, xc, s, nf, S);
+        R* q = qfk[barth ? k : 0];
+        muscl_own<DIM>(a.ph, S, xf, gr, s.other, q);
+        if (barth) {
+            const R* qo = s.other ? S.qL : S.qR;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const R l = (qo[v] - qc[v]) / (qf[v] - qc[v]);
-            const R lr = fmax_(klim * l, R(0));
-            const int gi = v < 2 ? v : 2;
-            ratio[gi] = fmin_(ratio[gi], lr);
+            for (int v = 0; v < NV; ++v) {
+                const R l = (qo[v] - qc[v]) / (q[v] - qc[v]);
+                const R lr = fmax_(klim * l, R(0));
+                const int gi = v < 2 ? v : 2;
+                ratio[gi] = fmin_(ratio[gi], lr);
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) a.qf_out[((size_t)k * NV + v) * n + c] = q[v];
         }
     }
+    if (!barth) return;
     // in 2D the reference also limits the (identically zero) z velocity: 0/0 -> max(NaN,0) = 0
     if (DIM == 2) ratio[2] = fmin_(ratio[2], R(0));
 #pragma unroll
-    for (int i = 0; i < 3; ++i) a.ratio_out[(size_t)i * n + c] = ratio[i];
+    for (int k = 0; k < NSM; ++k) {
+        if (k >= a.ns) break;
+        if (a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            a.qf_out[((size_t)k * NV + v) * n + c] = qc[v] + ratio[v < 2 ? v : 2] * (qfk[barth ? k : 0][v] - qc[v]);
+    }
 }
 
-// viscous flux of one face, calc_flux_viscous_cell.f90:75-158 (subtracted from F)
+// viscous flux of one face, calc_flux_viscous_cell.f90:75-158 (subtracted from F);
+// S carries the cell states and positions, gL/gR the cell gradients, xf/nv the face.
 template <int DIM, typename R>
-__device__ void viscous_face(const Physics& ph, const FaceCtx<DIM, R>& F, const R* gL, const R* gR, R* flux)
+__device__ void viscous_face(const Physics& ph, const Side<DIM, R>& F, const R* xf, const R* nf, const R* gL,
+                             const R* gR, R* flux)
 {
     const R r = R(ph.rgas);
     const R cp = R(ph.gamma) * r / (R(ph.gamma) - R(1));
     R a[DIM], b[DIM], vLR[DIM];
 #pragma unroll
-    for (int e = 0; e < DIM; ++e) { a[e] = F.xf[e] - F.xL[e]; b[e] = F.xf[e] - F.xR[e]; vLR[e] = F.xR[e] - F.xL[e]; }
+    for (int e = 0; e < DIM; ++e) { a[e] = xf[e] - F.xL[e]; b[e] = xf[e] - F.xR[e]; vLR[e] = F.xR[e] - F.xL[e]; }
     R dHL = sqrt(dotd<DIM>(a, a)), dHR = sqrt(dotd<DIM>(b, b));
     const R id = R(1) / (dHL + dHR);
     dHL = id * dHL;
@@ -719,14 +768,14 @@ __device__ void viscous_face(const Physics& ph, const FaceCtx<DIM, R>& F, const 
 #pragma unroll
             for (int j = 0; j < DIM; ++j) G[i][j] = G[i][j] + fa * kv[j];
         }
-        const R pscal = dotd<DIM>(vLR, F.n) / dLR2;
+        const R pscal = dotd<DIM>(vLR, nf) / dLR2;
         const R fcomp = pscal * (TR - TL);
         R vi[DIM];
 #pragma unroll
-        for (int e = 0; e < DIM; ++e) vi[e] = F.n[e] - pscal * vLR[e];
+        for (int e = 0; e < DIM; ++e) vi[e] = nf[e] - pscal * vLR[e];
         gTH = fcomp + dotd<DIM>(gavg, vi);
     } else {
-        gTH = dotd<DIM>(gavg, F.n);
+        gTH = dotd<DIM>(gavg, nf);
     }
     R tr = G[0][0];
 #pragma unroll
@@ -740,7 +789,7 @@ __device__ void viscous_face(const Physics& ph, const FaceCtx<DIM, R>& F, const 
         for (int j = 0; j < DIM; ++j) {
             R sij = G[i][j] + G[j][i];
             if (i == j) sij += addv;
-            s += (mu * sij) * F.n[j];
+            s += (mu * sij) * nf[j];
         }
         sn[i] = s;
     }
@@ -753,11 +802,14 @@ __device__ void viscous_face(const Physics& ph, const FaceCtx<DIM, R>& F, const 
 }
 
 // ---------------------------------------------------------------------------
-// K3: fluxes + deterministic gather + fused RK stage update.
-// calc_rhs.f90:48-145 (BC face_state: calcboco_gen bccon_face_state), integration_ns_ust.f90,
+// K2: fluxes + deterministic gather + fused RK stage update.
+// calc_rhs.f90:48-145 (face_state boundary map on the right state), integration_ns_ust.f90,
 // flux_to_res.f90:47-142, treelevel_rungekutta.f90:60-105, update_field.f90:38.
+// FS: second order, own-side face states come from K1 and the far side's is read
+// at its mirror slot; otherwise first order on cell states.  VISC: add the
+// viscous flux from cell states and cell gradients of both sides.
 
-template <int DIM, typename R, bool SECOND>
+template <int DIM, typename R, bool FS, bool VISC, int NSM>
 __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
 {
     constexpr int NV = DIM + 2;
@@ -767,65 +819,70 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
     const R g1 = R(a.ph.gamma - 1.0);
     const R gam = R(a.ph.gamma);
     R qc[NV], xc[DIM];
-    load_prim<DIM>(a.us, n, c, g1, qc);
-    cell_x<DIM>(a.g, c, xc);
+    if (!FS || VISC) load_prim<DIM>(a.us, n, c, g1, qc);
+    if (VISC) cell_x<DIM>(a.g, c, xc);
+    R gcell[VISC ? NV * DIM : 1];
+    if (VISC) load_grad<DIM>(a.grad, n, c, gcell);
     R res[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) res[v] = R(0);
-    const bool venk = a.ph.limiter == 'W';
-    for (int k = 0; k < a.ns; ++k) {
+#pragma unroll
+    for (int k = 0; k < NSM; ++k) {
+        if (k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
         const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
-        FaceCtx<DIM, R> F;
-        face_ctx<DIM>(a, c, qc, xc, s, F);
+        R nf[DIM];
+        face_n<DIM>(a.g, s.f, nf);
         R qLf[NV], qRf[NV];
-        R gL[SECOND ? NV * DIM : 1], gR[SECOND ? NV * DIM : 1];
-        if (SECOND) {
-            load_grad<DIM>(a.grad, n, F.owner, gL);
-            if (F.other >= 0) load_grad<DIM>(a.grad, n, F.other, gR);
-        }
-        if (!SECOND || a.ph.method == 'N') {
+        if (FS) {
+            R own[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) { qLf[v] = F.qL[v]; qRf[v] = F.qR[v]; }
-        } else {
-            if (venk) {
-                R rl[DIM], rr[DIM];
+            for (int v = 0; v < NV; ++v) own[v] = a.qf[((size_t)k * NV + v) * n + c];
+            if (s.nb >= 0) {
+                const int mk = ((unsigned)w >> MIRROR_SHIFT) & (MAX_SLOTS - 1);
+                R far[NV];
 #pragma unroll
-                for (int e = 0; e < DIM; ++e) { rl[e] = F.xf[e] - F.xL[e]; rr[e] = F.xf[e] - F.xR[e]; }
+                for (int v = 0; v < NV; ++v) far[v] = a.qf[((size_t)mk * NV + v) * n + s.nb];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    qLf[v] = F.qL[v] + dotd<DIM>(gL + v * DIM, rl);
-                    if (F.other >= 0) qRf[v] = F.qR[v] + dotd<DIM>(gR + v * DIM, rr);
+                    qLf[v] = s.other ? far[v] : own[v];
+                    qRf[v] = s.other ? own[v] : far[v];
                 }
             } else {
-                muscl_face<DIM>(a.ph, F.qL, F.qR, gL, gR, F.xL, F.xR, F.xf, qLf, qRf, F.other >= 0 ? 2 : 0);
-                if (a.ph.barth) {
-                    const R rL[3] = {a.ratio[F.owner], a.ratio[(size_t)n + F.owner], a.ratio[(size_t)2 * n + F.owner]};
+                const BocoDev bc = a.boco[a.g.bboco[-s.nb - 1]];
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) qLf[v] = F.qL[v] + rL[v < 2 ? v : 2] * (qLf[v] - F.qL[v]);
-                    if (F.other >= 0) {
-                        const R rR[3] = {a.ratio[F.other], a.ratio[(size_t)n + F.other], a.ratio[(size_t)2 * n + F.other]};
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) qRf[v] = F.qR[v] + rR[v < 2 ? v : 2] * (qRf[v] - F.qR[v]);
-                    }
-                }
+                for (int v = 0; v < NV; ++v) qLf[v] = own[v];
+                bc_state<DIM>(bc, nf, qLf, qRf, a.ph);
             }
-        }
-        const BocoDev* bcp = nullptr;
-        if (F.other < 0) {  // face_state boundary map on the right face state
-            bcp = &a.boco[a.g.bboco[F.bi]];
-            const BocoDev bc = *bcp;
-            bc_state<DIM>(bc, F.n, qLf, qRf, a.ph);
+        } else {
+            if (s.nb >= 0) {
+                R qo[NV];
+                load_prim<DIM>(a.us, n, s.nb, g1, qo);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    qLf[v] = s.other ? qo[v] : qc[v];
+                    qRf[v] = s.other ? qc[v] : qo[v];
+                }
+            } else {
+                const BocoDev bc = a.boco[a.g.bboco[-s.nb - 1]];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qLf[v] = qc[v];
+                bc_state<DIM>(bc, nf, qLf, qRf, a.ph);
+            }
         }
         R flux[NV];
-        num_flux<DIM>(a.ph.sch_hyp, gam, qLf, qRf, F.n, flux);
-        if (SECOND && a.ph.viscous) {
-            if (F.other < 0) {
-                const BocoDev bc = *bcp;
-                bc_grad<DIM>(bc, F.n, gL, gR);
-            }
-            viscous_face<DIM>(a.ph, F, gL, gR, flux);
+        num_flux<DIM>(a.ph.sch_hyp, gam, qLf, qRf, nf, flux);
+        if (VISC) {
+            Side<DIM, R> S;
+            side_ctx<DIM>(a, c, qc, xc, s, nf, S);
+            R xf[DIM], go[NV * DIM];
+            face_x<DIM>(a.g, s.f, xf);
+            if (s.nb >= 0) load_grad<DIM>(a.grad, n, s.nb, go);
+            else bc_grad<DIM>(a.boco[a.g.bboco[-s.nb - 1]], nf, gcell, go);
+            const R* gL = s.other ? go : gcell;
+            const R* gR = s.other ? gcell : go;
+            viscous_face<DIM>(a.ph, S, xf, nf, gL, gR, flux);
         }
         const R A = a.g.fs[s.f];
         // flux_to_res.f90:90-113: R(ic1) -= F.A, R(ic2) += F.A
@@ -974,26 +1031,44 @@ __global__ void k_cast(const double* __restrict__ src, R* __restrict__ dst, long
 // ---------------------------------------------------------------------------
 // launchers
 
+template <int DIM, typename R, int NSM>
+static void gradient_variant(const KArgs<R>& a, hipStream_t s)
+{
+    const dim3 grid(grid_blocks(a.n)), block(BLOCK);
+    if (!a.qf_out) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_NONE>), grid, block, 0, s, a);
+    else if (a.ph.limiter == 'W') hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_VENKAT>), grid, block, 0, s, a);
+    else if (a.ph.barth) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL_BARTH>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL>), grid, block, 0, s, a);
+}
+
 template <int DIM, typename R>
 hipError_t launch_gradient(const KArgs<R>& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_gradient<DIM, R>), dim3(grid_blocks(a.n)), dim3(BLOCK), 0, s, a);
+    if (a.ns <= 4) gradient_variant<DIM, R, 4>(a, s);
+    else gradient_variant<DIM, R, MAX_SLOTS>(a, s);
     return hipGetLastError();
 }
-template <int DIM, typename R>
-hipError_t launch_barth(const KArgs<R>& a, hipStream_t s)
+
+template <int DIM, typename R, int NSM>
+static void flux_variant(const KArgs<R>& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_barth<DIM, R>), dim3(grid_blocks(a.n)), dim3(BLOCK), 0, s, a);
-    return hipGetLastError();
+    const dim3 grid(grid_blocks(a.n)), block(BLOCK);
+    const bool fs = a.ph.method != 'N';
+    const bool visc = a.ph.viscous != 0;
+    if (fs && visc) hipLaunchKernelGGL((k_flux<DIM, R, true, true, NSM>), grid, block, 0, s, a);
+    else if (fs) hipLaunchKernelGGL((k_flux<DIM, R, true, false, NSM>), grid, block, 0, s, a);
+    else if (visc) hipLaunchKernelGGL((k_flux<DIM, R, false, true, NSM>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_flux<DIM, R, false, false, NSM>), grid, block, 0, s, a);
 }
+
 template <int DIM, typename R>
 hipError_t launch_flux(const KArgs<R>& a, hipStream_t s)
 {
-    const bool second = a.ph.method != 'N' || a.ph.viscous;
-    if (second) hipLaunchKernelGGL((k_flux<DIM, R, true>), dim3(grid_blocks(a.n)), dim3(BLOCK), 0, s, a);
-    else hipLaunchKernelGGL((k_flux<DIM, R, false>), dim3(grid_blocks(a.n)), dim3(BLOCK), 0, s, a);
+    if (a.ns <= 4) flux_variant<DIM, R, 4>(a, s);
+    else flux_variant<DIM, R, MAX_SLOTS>(a, s);
     return hipGetLastError();
 }
+
 template <int DIM>
 hipError_t launch_timestep(const DtArgs& a, double dtmax, double* d_dt, int* d_err, hipStream_t s)
 {
@@ -1013,7 +1088,6 @@ hipError_t launch_cast(const double* src, R* dst, long n, hipStream_t s)
 
 #define TY_INST(DIM, R)                                                          \
     template hipError_t launch_gradient<DIM, R>(const KArgs<R>&, hipStream_t); \
-    template hipError_t launch_barth<DIM, R>(const KArgs<R>&, hipStream_t);    \
     template hipError_t launch_flux<DIM, R>(const KArgs<R>&, hipStream_t);
 TY_INST(2, double)
 TY_INST(3, double)

@@ -32,13 +32,16 @@ def _p32(a):
 
 
 class Solver:
-    def __init__(self, mesh, scheme: dict | None = None, gas: dict | None = None, device: int = 0):
+    CELL_ORDER = {"rcm": 0, "hilbert": 1, "none": 2}
+
+    def __init__(self, mesh, scheme: dict | None = None, gas: dict | None = None, device: int = 0,
+                 cell_order: str = "hilbert"):
         self.lib = _lib.load()
         self.mesh = mesh
         self.ni = mesh.ncell_int
         self.h = C.c_void_p()
         self._check(self.lib.ty_create(C.byref(self.h), device), "ty_create")
-        self._upload_mesh(mesh)
+        self._upload_mesh(mesh, self.CELL_ORDER[cell_order])
         self.set_scheme(scheme or {}, gas or {})
 
     # -- plumbing ---------------------------------------------------------
@@ -58,7 +61,7 @@ class Solver:
         except Exception:
             pass
 
-    def _upload_mesh(self, m):
+    def _upload_mesh(self, m, cell_order):
         fc = np.ascontiguousarray(np.concatenate([m.ic1 + 1, m.ic2 + 1]).astype(np.int32))
         face = np.ascontiguousarray(np.column_stack([m.fnorm, m.fcent, m.fsurf]))
         centre = np.ascontiguousarray(m.ccent, dtype=np.float64)
@@ -78,7 +81,7 @@ class Solver:
             for j in range(8):
                 bocos[k].param[j] = float(b.param[j])
         d = _lib.TyMeshDesc(m.dim, m.ncell_int, m.ncell, m.nface, m.nface_int, _p32(fc), _p64(face),
-                            _p64(centre), _p64(volume), len(m.bocos), bocos)
+                            _p64(centre), _p64(volume), len(m.bocos), bocos, cell_order)
         self._check(self.lib.ty_mesh_upload(self.h, C.byref(d)), "ty_mesh_upload")
         del keep
 
