@@ -1,0 +1,196 @@
+!------------------------------------------------------------------------------!
+! MODULE : TYPHON_GPU
+!
+! Function
+!   iso_c_binding interface of libtyphon_gpu.so (include/typhon_gpu.h) for the
+!   Typhon Fortran driver: derived types laid out as the C structs and bind(C)
+!   interfaces of every entry point.  A driver built with GPU support fills
+!   ty_mesh_desc from umesh after init_world (SOURCE/MAIN/main.f90:95) and
+!   replaces the body of calc_rhs / the RK stage loop with these calls.
+!------------------------------------------------------------------------------!
+module TYPHON_GPU
+
+use iso_c_binding
+
+implicit none
+
+integer(c_int), parameter :: TY_OK = 0, TY_ERR_ARG = 1, TY_ERR_HIP = 2, TY_ERR_STATE = 3, &
+                             TY_ERR_UNSUPPORTED = 4, TY_ERR_NOT_CONVERGED = 5
+integer(c_int32_t), parameter :: TY_PREC_FP64 = 0, TY_PREC_MIXED = 1
+
+! umesh%boco(ib) with its type code and resolved uniform parameters
+type, bind(C) :: ty_boco_desc
+  integer(c_int32_t) :: type
+  integer(c_int32_t) :: nface
+  type(c_ptr)        :: iface          ! int32 (1:nface), 1-based face indices
+  type(c_ptr)        :: ipart          ! periodic partner faces or c_null_ptr
+  real(c_double)     :: param(8)
+endtype ty_boco_desc
+
+! st_ustmesh after init_world
+type, bind(C) :: ty_mesh_desc
+  integer(c_int32_t) :: dim, ncell_int, ncell, nface, nface_int
+  type(c_ptr)        :: facecell       ! c_loc(umesh%facecell%fils(1,1)), (nface,2) column-major
+  type(c_ptr)        :: face           ! c_loc(umesh%mesh%iface(1,1,1)), st_face records
+  type(c_ptr)        :: centre         ! c_loc(umesh%mesh%centre(1,1,1)), v3d records
+  type(c_ptr)        :: volume         ! c_loc(umesh%mesh%volume(1,1,1))
+  integer(c_int32_t) :: nboco
+  type(c_ptr)        :: boco           ! c_loc of a ty_boco_desc array
+  integer(c_int32_t) :: cell_order     ! 0 RCM, 1 Hilbert, 2 caller order
+endtype ty_mesh_desc
+
+! st_espece and the fluid model
+type, bind(C) :: ty_gas
+  real(c_double)     :: gamma, r_const, prandtl
+  integer(c_int32_t) :: typ_visc
+  real(c_double)     :: visc_dyn, visc_kin, tref, tsuth
+  integer(c_int32_t) :: typ_fluid
+endtype ty_gas
+
+! mnu_spat / mnu_time
+type, bind(C) :: ty_scheme
+  integer(c_int32_t)     :: sch_hyp, sch_dis
+  character(kind=c_char) :: method, limiter
+  integer(c_int32_t)     :: gradmeth, postlimiter
+  real(c_double)         :: venkat_k
+  integer(c_int32_t)     :: nstage
+  type(c_ptr)            :: rk_coef    ! c_loc(deftime%rk%coef(1,1))
+  real(c_double)         :: cfl
+  integer(c_int32_t)     :: precision
+endtype ty_scheme
+
+! matrix-free implicit solve settings and statistics
+type, bind(C) :: ty_krylov
+  integer(c_int32_t) :: nkrylov, max_restart, max_outer, inner_precision, precond
+  real(c_double)     :: inner_tol, outer_tol
+endtype ty_krylov
+
+type, bind(C) :: ty_krylov_stats
+  integer(c_int32_t) :: outer_iters, inner_iters
+  real(c_double)     :: rel_res, rhs_norm
+endtype ty_krylov_stats
+
+interface
+
+  integer(c_int) function ty_create(ctx, device) bind(C, name="ty_create")
+    import :: c_int, c_ptr
+    type(c_ptr)           :: ctx
+    integer(c_int), value :: device
+  endfunction
+
+  integer(c_int) function ty_destroy(ctx) bind(C, name="ty_destroy")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  endfunction
+
+  type(c_ptr) function ty_last_error(ctx) bind(C, name="ty_last_error")
+    import :: c_ptr
+    type(c_ptr), value :: ctx
+  endfunction
+
+  integer(c_int) function ty_mesh_upload(ctx, mesh) bind(C, name="ty_mesh_upload")
+    import :: c_int, c_ptr, ty_mesh_desc
+    type(c_ptr), value :: ctx
+    type(ty_mesh_desc) :: mesh
+  endfunction
+
+  integer(c_int) function ty_set_scheme(ctx, gas, scheme) bind(C, name="ty_set_scheme")
+    import :: c_int, c_ptr, ty_gas, ty_scheme
+    type(c_ptr), value :: ctx
+    type(ty_gas)       :: gas
+    type(ty_scheme)    :: scheme
+  endfunction
+
+  integer(c_int) function ty_state_set(ctx, rho, rhoe, mom) bind(C, name="ty_state_set")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double)     :: rho(*), rhoe(*), mom(*)
+  endfunction
+
+  integer(c_int) function ty_state_get(ctx, rho, rhoe, mom) bind(C, name="ty_state_get")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value :: ctx
+    real(c_double)     :: rho(*), rhoe(*), mom(*)
+  endfunction
+
+  integer(c_int) function ty_timestep(ctx, dtmax, dt) bind(C, name="ty_timestep")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value    :: ctx
+    real(c_double), value :: dtmax
+    real(c_double)        :: dt
+  endfunction
+
+  integer(c_int) function ty_residual(ctx, dt, res_rho, res_rhoe, res_mom) bind(C, name="ty_residual")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value    :: ctx
+    real(c_double), value :: dt
+    real(c_double)        :: res_rho(*), res_rhoe(*), res_mom(*)
+  endfunction
+
+  integer(c_int) function ty_rk_step(ctx, dt) bind(C, name="ty_rk_step")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value    :: ctx
+    real(c_double), value :: dt
+  endfunction
+
+  integer(c_int) function ty_advance(ctx, nsteps, dtmax, dt_last) bind(C, name="ty_advance")
+    import :: c_int, c_int32_t, c_ptr, c_double
+    type(c_ptr), value        :: ctx
+    integer(c_int32_t), value :: nsteps
+    real(c_double), value     :: dtmax
+    real(c_double)            :: dt_last
+  endfunction
+
+  integer(c_int) function ty_sync(ctx) bind(C, name="ty_sync")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  endfunction
+
+  integer(c_int) function ty_timing(ctx, enable, ms, count) bind(C, name="ty_timing")
+    import :: c_int, c_int32_t, c_ptr, c_double
+    type(c_ptr), value        :: ctx
+    integer(c_int32_t), value :: enable
+    real(c_double)            :: ms(3)
+    integer(c_int32_t)        :: count(3)
+  endfunction
+
+  integer(c_int) function ty_mesh_info(ctx, ncell, nface_dev, nslot, nbface_dev) bind(C, name="ty_mesh_info")
+    import :: c_int, c_int32_t, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int32_t) :: ncell, nface_dev, nslot, nbface_dev
+  endfunction
+
+  integer(c_int) function ty_jacobian_diag(ctx, dt, diag) bind(C, name="ty_jacobian_diag")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value    :: ctx
+    real(c_double), value :: dt
+    real(c_double)        :: diag(*)
+  endfunction
+
+  integer(c_int) function ty_jvp(ctx, dt, v, jv) bind(C, name="ty_jvp")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value    :: ctx
+    real(c_double), value :: dt
+    real(c_double)        :: v(*), jv(*)
+  endfunction
+
+  integer(c_int) function ty_linear_solve(ctx, dt, kry, b, x, stats) bind(C, name="ty_linear_solve")
+    import :: c_int, c_ptr, c_double, ty_krylov, ty_krylov_stats
+    type(c_ptr), value    :: ctx
+    real(c_double), value :: dt
+    type(ty_krylov)       :: kry
+    real(c_double)        :: b(*), x(*)
+    type(ty_krylov_stats) :: stats
+  endfunction
+
+  integer(c_int) function ty_implicit_step(ctx, dt, kry, stats) bind(C, name="ty_implicit_step")
+    import :: c_int, c_ptr, c_double, ty_krylov, ty_krylov_stats
+    type(c_ptr), value    :: ctx
+    real(c_double), value :: dt
+    type(ty_krylov)       :: kry
+    type(ty_krylov_stats) :: stats
+  endfunction
+
+endinterface
+
+endmodule TYPHON_GPU

@@ -86,6 +86,30 @@ typedef struct {
     int32_t precision;       /* TY_PREC_FP64 or TY_PREC_MIXED */
 } ty_scheme;
 
+/* Matrix-free implicit solve (no reference counterpart: the reference's INVERSION=GMRESFREE
+ * path is dead code, SOURCE/MGRID/MATFREE.f90:73-158, gmres_free.f90:147-289).  The linear
+ * system of the linearised implicit step is (I - J) x = b with J = dR/dU and
+ * R(U) = -dt/V sum_f F.A (SOURCE/MAIN/calc_rhs.f90 convention); J v comes from finite
+ * differences of R, the preconditioner is block-Jacobi with the diagonal blocks of the
+ * reference's HLL Jacobian assembly (build_implicit_bdlu.f90:44-83). */
+typedef struct {
+    int32_t nkrylov;         /* GMRES restart length m (1..31) */
+    int32_t max_restart;     /* GMRES cycles per refinement pass */
+    int32_t max_outer;       /* FP64 refinement passes */
+    int32_t inner_precision; /* TY_PREC_MIXED: FP32 Krylov vectors, preconditioner and JVP;
+                                TY_PREC_FP64: all FP64 */
+    int32_t precond;         /* 0 none, 1 block-Jacobi */
+    double inner_tol;        /* relative residual reduction of each GMRES solve */
+    double outer_tol;        /* target ||b - (I - J) x|| / ||b|| (FP64 central-difference JVP) */
+} ty_krylov;
+
+typedef struct {
+    int32_t outer_iters;     /* refinement passes performed */
+    int32_t inner_iters;     /* total Arnoldi steps */
+    double rel_res;          /* final ||b - (I - J) x|| / ||b|| */
+    double rhs_norm;         /* ||b|| */
+} ty_krylov_stats;
+
 /* context */
 int ty_create(ty_ctx** ctx, int device);
 int ty_destroy(ty_ctx* ctx);
@@ -125,6 +149,27 @@ int ty_advance(ty_ctx* ctx, int32_t nsteps, double dtmax, double* dt_last);
 
 /* wait for queued work; reports a bad state found by device-side time steps */
 int ty_sync(ty_ctx* ctx);
+
+/* diagonal blocks I + dt/V sum_f A_f dF_f/dU_c of the implicit operator at the current state
+ * (init_implicit_bdlu.f90:37-49 + build_implicit_bdlu.f90:44-83 restricted to the diagonal, HLL
+ * face Jacobians calc_jac_hll.f90:46-57 at the face states of the residual).  diag: [ncell_int][5][5]
+ * row-major in the caller's cell order, variables (rho, rhoE, rho u, rho v, rho w); in 2D the
+ * z-momentum row and column hold the identity. */
+int ty_jacobian_diag(ty_ctx* ctx, double dt, double* diag);
+
+/* J v at the current state by central differences of the FP64 residual (v, jv: [5][ncell_int]
+ * in the caller's cell order) */
+int ty_jvp(ty_ctx* ctx, double dt, const double* v, double* jv);
+
+/* solve (I - J) x = b at the current state (b, x: [5][ncell_int], caller order) by FP64
+ * iterative refinement around preconditioned restarted GMRES; TY_ERR_NOT_CONVERGED if the
+ * outer target is not reached (x then holds the last iterate) */
+int ty_linear_solve(ty_ctx* ctx, double dt, const ty_krylov* kry, const double* b, double* x,
+                    ty_krylov_stats* stats);
+
+/* one linearised backward-Euler step: U <- U + x with (I - J) x = R(U) (calc_rhs.f90:133-140
+ * tstep_implicit convention) */
+int ty_implicit_step(ty_ctx* ctx, double dt, const ty_krylov* kry, ty_krylov_stats* stats);
 
 /* per-kernel timing for benchmarks (no reference counterpart): when enabled, every
  * gradient, flux and time-step launch is bracketed by HIP events on the context's

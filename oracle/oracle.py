@@ -65,6 +65,8 @@ def _load():
     lib.or_venkat_phi.argtypes = [C.c_double] * 4
     lib.or_viscosity.restype = C.c_double
     lib.or_viscosity.argtypes = [C.POINTER(OrConfig), C.c_double, C.c_double]
+    lib.or_jac_hll.argtypes = [C.POINTER(OrConfig), F64P, F64P, F64P, F64P, F64P]
+    lib.or_jac_diag.argtypes = [C.c_void_p, C.c_double, F64P]
     lib.or_meshgeom.restype = C.c_int
     lib.or_meshgeom.argtypes = [C.c_int, C.c_int, F64P, C.c_int, I32P, I32P, I32P, C.c_int,
                                 F64P, F64P, F64P, F64P, F64P]
@@ -200,6 +202,12 @@ class Oracle:
     def flux(self):
         return self.view("or_flux", 5, self.mesh.nface)
 
+    def jac_diag(self, dt):
+        """diagonal blocks [ncell_int, 5, 5] of I + dt/V dF.A/dU after calc_rhs(dt)"""
+        out = np.empty((self.mesh.ncell_int, 5, 5))
+        self.lib.or_jac_diag(self.h, dt, _p64(out))
+        return out
+
 
 def flux_face(qL, qR, n, sch_hyp=25, gamma=1.4):
     cfg, _ = make_config(None, dict(sch_hyp=sch_hyp), dict(gamma=gamma))
@@ -207,6 +215,14 @@ def flux_face(qL, qR, n, sch_hyp=25, gamma=1.4):
     a, b, c = (np.ascontiguousarray(x, dtype=np.float64) for x in (qL, qR, n))
     lib().or_flux_face(C.byref(cfg), _p64(a), _p64(b), _p64(c), _p64(out))
     return out
+
+
+def jac_hll(qL, qR, n, sch_hyp=25, gamma=1.4):
+    cfg, _ = make_config(None, dict(sch_hyp=sch_hyp), dict(gamma=gamma))
+    jl, jr = np.empty(25), np.empty(25)
+    a, b, c = (np.ascontiguousarray(x, dtype=np.float64) for x in (qL, qR, n))
+    lib().or_jac_hll(C.byref(cfg), _p64(a), _p64(b), _p64(c), _p64(jl), _p64(jr))
+    return jl.reshape(5, 5), jr.reshape(5, 5)
 
 
 def limiter(lim, x1, x2):

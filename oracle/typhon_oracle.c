@@ -970,3 +970,94 @@ int or_meshgeom(int dim, int nvtx, const double* vtx, int nface, const int32_t* 
     free(mid); free(nb);
     return 0;
 }
+
+/* ---------------------------------------------------------------------------
+ * A15: flux Jacobians and the diagonal blocks of the implicit operator.
+ * calc_jacflux (SOURCE/EQNS/calc_jacflux.f90:49-77): Euler flux Jacobian A(n) in
+ * (rho, rhoE, rho u, rho v, rho w) at a primitive state; calc_jac_hll
+ * (calc_jac_hll.f90:46-57) with the HLLC wave speeds clipped at zero
+ * (calc_flux_hllc.f90:185-189).  Blocks are row-major m[i*5+j] = dF_i/dU_j. */
+static void jacflux(double g, const double* q, const double* n, double vn, double* m)
+{
+    double g1 = g - 1.0, ig1 = 1.0 / g1;
+    double v2 = q[2] * q[2] + q[3] * q[3] + q[4] * q[4];
+    double en = ig1 * q[1] / q[0] + .5 * v2;
+    const double* u = q + 2;
+    for (int k = 0; k < 25; ++k) m[k] = 0.0;
+    m[0 * 5 + 0] = 0.0; m[0 * 5 + 1] = 0.0;
+    for (int j = 0; j < 3; ++j) m[0 * 5 + 2 + j] = n[j];
+    m[1 * 5 + 0] = vn * (g1 * v2 - g * en);
+    m[1 * 5 + 1] = vn * g;
+    for (int j = 0; j < 3; ++j) m[1 * 5 + 2 + j] = (g * en - .5 * g1 * v2) * n[j] - g1 * vn * u[j];
+    for (int i = 0; i < 3; ++i) {
+        m[(2 + i) * 5 + 0] = .5 * g1 * v2 * n[i] - vn * u[i];
+        m[(2 + i) * 5 + 1] = g1 * n[i];
+        for (int j = 0; j < 3; ++j) m[(2 + i) * 5 + 2 + j] = u[i] * n[j] - (g1 * n[i]) * u[j];
+        m[(2 + i) * 5 + 2 + i] += vn;
+    }
+}
+
+void or_jac_hll(const or_config* c, const double* qL, const double* qR, const double* n, double* jL, double* jR)
+{
+    double g = c->gamma;
+    double beta = (c->sch_hyp == OR_SCH_HLLCB) ? sqrt((g - 1.0) / (2 * g)) : 1.0;
+    double vnl = qL[2] * n[0] + qL[3] * n[1] + qL[4] * n[2];
+    double vnr = qR[2] * n[0] + qR[3] * n[1] + qR[4] * n[2];
+    double al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
+    double rrho, rv[3], rp; roe_state(g, qL, qR, &rrho, rv, &rp);
+    double vm = rv[0] * n[0] + rv[1] * n[1] + rv[2] * n[2];
+    double am = sqrt(g * rp / rrho);
+    double sl = FMIN(vnl - beta * al, vm - am);
+    double sr = FMAX(vnr + beta * ar, vm + am);
+    sl = FMIN(sl, 0.0);
+    sr = FMAX(sr, 0.0);
+    jacflux(g, qL, n, vnl, jL);
+    jacflux(g, qR, n, vnr, jR);
+    for (int i = 0; i < 5; ++i) { jL[i * 5 + i] = jL[i * 5 + i] - sl; jR[i * 5 + i] = jR[i * 5 + i] - sr; }
+    double kl = sr / (sr - sl), kr = -sl / (sr - sl);
+    for (int k = 0; k < 25; ++k) { jL[k] = kl * jL[k]; jR[k] = kr * jR[k]; }
+}
+
+/* Diagonal blocks of the implicit operator I + dt/V dF.A/dU of interior cells
+ * (init_implicit_bdlu.f90:37-49, build_implicit_bdlu.f90:44-83) from the face states
+ * of the last calc_rhs; viscous diagonal of calc_flux_viscous_cell.f90:149-158.
+ * diag: [ncell_int][25] row-major. */
+void or_jac_diag(or_ctx* o, double dt, double* diag)
+{
+    const or_config* c = &o->c;
+    int ni = o->ni;
+    for (int i = 0; i < ni; ++i)
+        for (int k = 0; k < 25; ++k) diag[(size_t)25 * i + k] = (k % 6 == 0) ? 1.0 : 0.0;
+    for (int f = 0; f < o->nf; ++f) {
+        double ql[NV], qr[NV], jL[25], jR[25];
+        for (int v = 0; v < NV; ++v) { ql[v] = QL(o, v, f); qr[v] = QR(o, v, f); }
+        const double* n = c->fnorm + 3 * (size_t)f;
+        or_jac_hll(c, ql, qr, n, jL, jR);
+        if (c->viscous) {
+            int cl = c->ic1[f], cr = c->ic2[f];
+            const double* xl = c->ccent + 3 * (size_t)cl;
+            const double* xr = c->ccent + 3 * (size_t)cr;
+            const double* xf = c->fcent + 3 * (size_t)f;
+            double a[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+            double b[3] = {xf[0] - xr[0], xf[1] - xr[1], xf[2] - xr[2]};
+            double dHL = sqrt(dot3(a, a)), dHR = sqrt(dot3(b, b));
+            double id0 = 1.0 / (dHL + dHR);
+            dHL = id0 * dHL; dHR = id0 * dHR;
+            double rL = PRIM(o, 0, cl), rR = PRIM(o, 0, cr);
+            double TL = PRIM(o, 1, cl) / (rL * c->rgas), TR = PRIM(o, 1, cr) / (rR * c->rgas);
+            double TH = dHR * TL + dHL * TR, rhoH = dHR * rL + dHL * rR;
+            double mu = or_viscosity(c, rhoH, TH);
+            double d[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+            double id = mu / (dHR * rL + dHL * rR) / sqrt(dot3(d, d));
+            for (int i = 0; i < 5; ++i) { jR[i * 5 + i] = jR[i * 5 + i] - id; jL[i * 5 + i] = jL[i * 5 + i] + id; }
+        }
+        double s = c->fsurf[f];
+        int c1 = c->ic1[f], c2 = c->ic2[f];
+        double k1 = dt / c->cvol[c1];
+        for (int k = 0; k < 25; ++k) diag[(size_t)25 * c1 + k] += (s * jL[k]) * k1;
+        if (f < o->nfi) {
+            double k2 = dt / c->cvol[c2];
+            for (int k = 0; k < 25; ++k) diag[(size_t)25 * c2 + k] -= (s * jR[k]) * k2;
+        }
+    }
+}

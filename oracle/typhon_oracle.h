@@ -137,6 +137,11 @@ double or_limiter(char lim, double x1, double x2);       /* LIMITER.f90 scalar f
 double or_venkat_phi(double dmax, double dmin, double d2, double eps2);
 double or_viscosity(const or_config* cfg, double rho, double T); /* EQNS.f90:225-265 */
 
+/* A15 pieces: HLL flux Jacobians of one face (row-major 5x5, dF/dU in (rho, rhoE, rho u, rho v, rho w))
+   and the diagonal blocks I + dt/V dF.A/dU of the interior cells after or_calc_rhs ([ncell_int][25]) */
+void or_jac_hll(const or_config* cfg, const double* qL, const double* qR, const double* n, double* jL, double* jR);
+void or_jac_diag(or_ctx* o, double dt, double* diag);
+
 /* MESHGEOM restatement (A1): geometry of faces given by vertex lists.
    fvtx [nface][4] 0-based vertex ids (-1 padded), ic1/ic2 in/out (swapped so ic1<ic2 when ic2>=0,
    ic2 = -1 on boundary faces), outputs face normal/centre/surface and cell centre/volume

@@ -350,3 +350,62 @@ def test_rk_step_on_advected_fourier_mode(tps):
     P = rk_polynomial(coef, z)
     exp = 1.0 + eps * np.real(P * np.exp(1j * kx * x))
     np.testing.assert_allclose(o.get_cons()[0], exp, atol=1e-14)
+
+
+# --- A15 face Jacobians ------------------------------------------------------------------------
+
+def cons_of(q, g=G):
+    rho, p, u = q[0], q[1], np.asarray(q[2:5])
+    return np.concatenate([[rho, p / (g - 1) + .5 * rho * u @ u], rho * u])
+
+
+def prim_of(U, g=G):
+    rho = U[0]
+    u = U[2:5] / rho
+    return np.concatenate([[rho, (g - 1) * (U[1] - .5 * rho * u @ u)], u])
+
+
+def test_jacflux_is_the_euler_flux_jacobian():
+    """For qL = qR the HLL weights sum to one, so jL + jR = A(q) = dF/dU; check it against central
+    differences of the physical flux in conservative variables and the homogeneity A(U) U = F(U)."""
+    rng = np.random.default_rng(12)
+    for _ in range(20):
+        q, n = rand_state(rng), rand_normal(rng)
+        jl, jr = O.jac_hll(q, q, n)
+        A = jl + jr
+        U = cons_of(q)
+        np.testing.assert_allclose(A @ U, phys_flux(q, n), rtol=1e-12, atol=1e-12)
+        fd = np.empty((5, 5))
+        for j in range(5):
+            h = 1e-6 * max(1.0, abs(U[j]))
+            e = np.zeros(5)
+            e[j] = h
+            fd[:, j] = (phys_flux(prim_of(U + e), n) - phys_flux(prim_of(U - e), n)) / (2 * h)
+        np.testing.assert_allclose(A, fd, rtol=1e-7, atol=1e-7)
+
+
+def test_hll_jacobian_weights():
+    """Supersonic to the right: both HLL speeds positive, clipped S_L = 0, so jL = A(qL) and jR = 0;
+    subsonic: jL = S_R/(S_R-S_L) (A_L - S_L I), the weights of calc_jac_hll."""
+    n = np.array([1.0, 0.0, 0.0])
+    qL = np.array([1.0, 1.0, 3.0, 0.1, 0.0])
+    qR = np.array([1.1, 1.2, 3.1, 0.0, 0.1])
+    jl, jr = O.jac_hll(qL, qR, n)
+    A = O.jac_hll(qL, qL, n)
+    np.testing.assert_allclose(jl, A[0] + A[1], rtol=1e-13, atol=1e-13)
+    assert np.abs(jr).max() == 0.0
+    # subsonic: jL = a (A_L - S_L I), jR = b (A_R - S_R I) with a = S_R/(S_R-S_L), b = -S_L/(S_R-S_L)
+    qL2 = np.array([1.0, 1.0, 0.2, 0.1, 0.0])
+    qR2 = np.array([1.3, 0.8, -0.1, 0.0, 0.1])
+    jl2, jr2 = O.jac_hll(qL2, qR2, n)
+    AL = sum(O.jac_hll(qL2, qL2, n))
+    AR = sum(O.jac_hll(qR2, qR2, n))
+    a = jl2[0, 2] / AL[0, 2]  # mass row: A[0, 2] = n_x, no diagonal shift
+    b = jr2[0, 2] / AR[0, 2]
+    assert abs(a + b - 1.0) < 1e-13
+    sl = AL[1, 1] - jl2[1, 1] / a  # recover the speeds from the diagonal shifts
+    sr = AR[1, 1] - jr2[1, 1] / b
+    assert sl < 0 < sr
+    assert abs(a - sr / (sr - sl)) < 1e-12
+    np.testing.assert_allclose(jl2, a * (AL - sl * np.eye(5)), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(jr2, b * (AR - sr * np.eye(5)), rtol=1e-12, atol=1e-12)

@@ -15,7 +15,7 @@ F64P = C.POINTER(C.c_double)
 
 EXPORTS = ["ty_create", "ty_destroy", "ty_last_error", "ty_mesh_upload", "ty_set_scheme", "ty_state_set",
            "ty_state_get", "ty_timestep", "ty_residual", "ty_rk_step", "ty_advance", "ty_sync", "ty_timing",
-           "ty_mesh_info"]
+           "ty_mesh_info", "ty_jacobian_diag", "ty_jvp", "ty_linear_solve", "ty_implicit_step"]
 
 
 class TyBocoDesc(C.Structure):
@@ -40,6 +40,17 @@ class TyScheme(C.Structure):
     _fields_ = [("sch_hyp", C.c_int32), ("sch_dis", C.c_int32), ("method", C.c_char), ("limiter", C.c_char),
                 ("gradmeth", C.c_int32), ("postlimiter", C.c_int32), ("venkat_k", C.c_double),
                 ("nstage", C.c_int32), ("rk_coef", F64P), ("cfl", C.c_double), ("precision", C.c_int32)]
+
+
+class TyKrylov(C.Structure):
+    _fields_ = [("nkrylov", C.c_int32), ("max_restart", C.c_int32), ("max_outer", C.c_int32),
+                ("inner_precision", C.c_int32), ("precond", C.c_int32), ("inner_tol", C.c_double),
+                ("outer_tol", C.c_double)]
+
+
+class TyKrylovStats(C.Structure):
+    _fields_ = [("outer_iters", C.c_int32), ("inner_iters", C.c_int32), ("rel_res", C.c_double),
+                ("rhs_norm", C.c_double)]
 
 
 class TyError(RuntimeError):
@@ -75,6 +86,10 @@ def load():
     lib.ty_sync.argtypes = [vp]
     lib.ty_timing.argtypes = [vp, C.c_int32, F64P, I32P]
     lib.ty_mesh_info.argtypes = [vp, I32P, I32P, I32P, I32P]
+    lib.ty_jacobian_diag.argtypes = [vp, C.c_double, F64P]
+    lib.ty_jvp.argtypes = [vp, C.c_double, F64P, F64P]
+    lib.ty_linear_solve.argtypes = [vp, C.c_double, C.POINTER(TyKrylov), F64P, F64P, C.POINTER(TyKrylovStats)]
+    lib.ty_implicit_step.argtypes = [vp, C.c_double, C.POINTER(TyKrylov), C.POINTER(TyKrylovStats)]
     for name in EXPORTS:
         if name != "ty_last_error":
             getattr(lib, name).restype = C.c_int

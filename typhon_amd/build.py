@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtyphon_gpu.so")
-SOURCES = ["ty_kernels.hip", "ty_host.cpp"]
+SOURCES = ["ty_kernels_f64.hip", "ty_kernels_f32.hip", "ty_krylov.hip", "ty_host.cpp"]
 ARCH = os.environ.get("TY_OFFLOAD_ARCH", "gfx950")
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 VALU ops
@@ -22,8 +22,8 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
-    for src in SOURCES:
+    objs, procs = [], []
+    for src in SOURCES:  # translation units compile in parallel
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
         cmd = ["hipcc", *HIPFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
@@ -31,8 +31,11 @@ def build(verbose: bool = False) -> str:
             cmd.insert(2, "hip")
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise RuntimeError(f"hipcc failed on {failed}")
     tmp = LIB + ".tmp"
     cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:

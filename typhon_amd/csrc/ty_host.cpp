@@ -20,6 +20,7 @@
 
 #include "../../include/typhon_gpu.h"
 #include "ty_internal.h"
+#include "ty_krylov.h"
 
 using namespace ty;
 
@@ -73,8 +74,17 @@ struct ty_ctx {
     double* d_acc = nullptr;
     double* d_rstore = nullptr;
     int nstore = 0;
-    void* d_grad = nullptr;
+    void* d_grad = nullptr;   // work set of the scheme precision (aliases work_grad/qf)
     void* d_qf = nullptr;
+    void* work_grad[2] = {nullptr, nullptr};  // per precision: [0] FP64, [1] FP32
+    void* work_qf[2] = {nullptr, nullptr};
+    // matrix-free implicit solve (allocated on first use)
+    bool kry_ok = false;
+    int kry_m = 0, kry_prec = -1;
+    double *k_b = nullptr, *k_x = nullptr, *k_r = nullptr, *k_r0 = nullptr, *k_r1 = nullptr, *k_r2 = nullptr;
+    double *k_pd = nullptr, *k_part = nullptr, *k_h = nullptr, *k_dx = nullptr;
+    float* k_pf = nullptr;
+    void *k_V = nullptr, *k_z = nullptr, *k_w = nullptr, *k_t = nullptr, *k_dinv = nullptr;
     double* d_res = nullptr;
     double* d_part = nullptr;
     int* d_bad = nullptr;
@@ -115,6 +125,10 @@ void free_state(ty_ctx* c)
     c->d_vol = c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
     c->d_us[0] = c->d_us[1] = nullptr;
     c->d_grad = c->d_qf = nullptr;
+    c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
+    c->kry_ok = false;
+    c->kry_m = 0;
+    c->kry_prec = -1;
     c->d_bad = c->d_err = nullptr;
     for (auto* v : {&c->ev_grad, &c->ev_flux, &c->ev_dt})
         for (auto& e : *v) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
@@ -563,6 +577,28 @@ int build_geo(ty_ctx* c, Geo<R>& g)
 
 }  // namespace
 
+namespace {
+
+// gradient / face-state work arrays of one arithmetic precision (0: FP64, 1: FP32)
+int ensure_work(ty_ctx* c, int prec)
+{
+    const size_t rb = prec ? sizeof(float) : sizeof(double);
+    void* p = nullptr;
+    if (c->ph.viscous && !c->work_grad[prec]) {  // cell gradients for the viscous flux
+        HIPCHK(c, hipMalloc(&p, (size_t)c->nv * c->dim * c->n * rb));
+        c->owned.push_back(p);
+        c->work_grad[prec] = p;
+    }
+    if (c->ph.method != 'N' && !c->work_qf[prec]) {  // own-side face states of every slot
+        HIPCHK(c, hipMalloc(&p, (size_t)c->ns * c->nv * c->n * rb));
+        c->owned.push_back(p);
+        c->work_qf[prec] = p;
+    }
+    return TY_OK;
+}
+
+}  // namespace
+
 int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
 {
     if (!c || !gas || !s) return TY_ERR_ARG;
@@ -656,9 +692,8 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     }
     int rc;
     if ((rc = upload(c, &c->d_boco, tab))) return rc;
-    if (c->mixed) rc = build_geo<float>(c, c->geo_f);
-    else rc = build_geo<double>(c, c->geo_d);
-    if (rc) return rc;
+    // both precisions: the FP32 geometry also serves the FP32 Krylov products
+    if ((rc = build_geo<float>(c, c->geo_f)) || (rc = build_geo<double>(c, c->geo_d))) return rc;
     std::vector<double> dtfac(c->n);
     for (int i = 0; i < c->n; ++i) dtfac[i] = s->cfl * 2.0 * c->vol[i] / c->sumA[i];
     if ((rc = upload(c, &c->d_dtfac, dtfac))) return rc;
@@ -680,17 +715,13 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
         c->owned.push_back(p);
         c->d_us[k] = p;
     }
-    c->d_grad = c->d_qf = nullptr;
-    if (viscous) {  // cell gradients for the viscous flux
-        HIPCHK(c, hipMalloc(&p, (size_t)c->nv * c->dim * c->n * rb));
-        c->owned.push_back(p);
-        c->d_grad = p;
-    }
-    if (s->method != 'N') {  // own-side face states of every slot
-        HIPCHK(c, hipMalloc(&p, (size_t)c->ns * c->nv * c->n * rb));
-        c->owned.push_back(p);
-        c->d_qf = p;
-    }
+    c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
+    c->kry_ok = false;
+    c->kry_m = 0;
+    c->kry_prec = -1;
+    if ((rc = ensure_work(c, c->mixed ? 1 : 0))) return rc;
+    c->d_grad = c->work_grad[c->mixed ? 1 : 0];
+    c->d_qf = c->work_qf[c->mixed ? 1 : 0];
     const int nblk = grid_blocks(c->n);
     if ((rc = dalloc(c, &c->d_part, nblk)) || (rc = dalloc(c, &c->d_bad, nblk)) || (rc = dalloc(c, &c->d_dt, 1)) ||
         (rc = dalloc(c, &c->d_err, 1)))
@@ -764,10 +795,11 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     a.u0 = c->d_u0;
     a.acc = c->d_acc;
     a.rstore = c->d_rstore;
-    a.grad = (const R*)c->d_grad;
-    a.grad_out = (R*)c->d_grad;
-    a.qf = (const R*)c->d_qf;
-    a.qf_out = (R*)c->d_qf;  // null for first-order schemes
+    const int prec = sizeof(R) == sizeof(float) ? 1 : 0;
+    a.grad = (const R*)c->work_grad[prec];
+    a.grad_out = (R*)c->work_grad[prec];
+    a.qf = (const R*)c->work_qf[prec];
+    a.qf_out = (R*)c->work_qf[prec];  // null for first-order schemes
     a.res_out = c->d_res;
     a.d_dt = c->d_dt;
     a.store_slot = -1;
@@ -993,5 +1025,397 @@ int ty_timing(ty_ctx* c, int32_t enable, double* ms, int32_t* count)
         lists[k]->clear();
     }
     c->timing = enable != 0;
+    return TY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-free implicit solve: (I - J) x = b, J = dR/dU, R = -dt/V sum F.A.
+// FP64 iterative refinement (central-difference JVP of the FP64 residual)
+// around restarted, right-preconditioned GMRES whose Krylov vectors, block-Jacobi
+// preconditioner and one-sided JVP run in the inner precision.  Classical
+// Gram-Schmidt is applied twice per Arnoldi step (two fused multi-dot/update
+// passes) instead of the reference's modified Gram-Schmidt (bdlu_gmres.f90:101-107).
+
+namespace {
+
+template <typename R>
+const Geo<R>& geo_of(ty_ctx* c);
+template <>
+const Geo<double>& geo_of<double>(ty_ctx* c) { return c->geo_d; }
+template <>
+const Geo<float>& geo_of<float>(ty_ctx* c) { return c->geo_f; }
+
+template <typename R>
+hipError_t eval_residual(ty_ctx* c, const R* state, double* out, R* dinv, double* diag)
+{
+    KArgs<R> a = make_args<R>(c, geo_of<R>(c));
+    a.us = state;
+    a.mode = 1;
+    a.res_out = out;
+    a.dinv_out = dinv;
+    a.diag_out = diag;
+    return c->dim == 2 ? eval_kernels<2, R>(c, a) : eval_kernels<3, R>(c, a);
+}
+
+int alloc_krylov(ty_ctx* c, int m, int prec)
+{
+    if (c->kry_ok && c->kry_m >= m && c->kry_prec == prec) return TY_OK;
+    int rc;
+    if ((rc = ensure_work(c, prec)) || (rc = ensure_work(c, 0))) return rc;
+    const long len = (long)c->nv * c->n;
+    const size_t pb = prec ? sizeof(float) : sizeof(double);
+    if (!c->kry_ok) {
+        if ((rc = dalloc(c, &c->k_b, len)) || (rc = dalloc(c, &c->k_x, len)) || (rc = dalloc(c, &c->k_r, len)) ||
+            (rc = dalloc(c, &c->k_r0, len)) || (rc = dalloc(c, &c->k_r1, len)) || (rc = dalloc(c, &c->k_r2, len)) ||
+            (rc = dalloc(c, &c->k_pd, len)) || (rc = dalloc(c, &c->k_dx, len)) || (rc = dalloc(c, &c->k_pf, len)) ||
+            (rc = dalloc(c, &c->k_part, (size_t)multidot_blocks(len) * MAXK)) || (rc = dalloc(c, &c->k_h, MAXK)))
+            return rc;
+    }
+    void* p = nullptr;
+    HIPCHK(c, hipMalloc(&p, (size_t)(m + 1) * len * pb));
+    c->owned.push_back(p);
+    c->k_V = p;
+    for (void** q : {&c->k_z, &c->k_w, &c->k_t}) {
+        HIPCHK(c, hipMalloc(&p, (size_t)len * pb));
+        c->owned.push_back(p);
+        *q = p;
+    }
+    HIPCHK(c, hipMalloc(&p, (size_t)c->nv * c->nv * c->n * pb));
+    c->owned.push_back(p);
+    c->k_dinv = p;
+    c->kry_ok = true;
+    c->kry_m = m;
+    c->kry_prec = prec;
+    return TY_OK;
+}
+
+template <typename T>
+int dot_host(ty_ctx* c, const T* a, const T* b, long len, double* out)
+{
+    HIPCHK(c, multidot<T>(a, b, 0, 1, len, c->k_part, c->k_h, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, c->k_h, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TY_OK;
+}
+
+template <typename P>
+P* pert_buffer(ty_ctx* c);
+template <>
+double* pert_buffer<double>(ty_ctx* c) { return c->k_pd; }
+template <>
+float* pert_buffer<float>(ty_ctx* c) { return c->k_pf; }
+
+// restarted GMRES in precision P on (I - J) d = rhs; d accumulates into c->k_dx (FP64)
+template <typename P>
+int gmres(ty_ctx* c, const double* rhs, const ty_krylov* k, double unorm, int* iters, double* rel)
+{
+    const long len = (long)c->nv * c->n;
+    const int m = k->nkrylov;
+    P* V = (P*)c->k_V;
+    P* z = (P*)c->k_z;
+    P* w = (P*)c->k_w;
+    P* t = (P*)c->k_t;
+    const P* dinv = (const P*)c->k_dinv;
+    P* pert = pert_buffer<P>(c);
+    const double sq = std::sqrt(sizeof(P) == sizeof(float) ? 1.1920928955078125e-07 : 2.220446049250313e-16);
+    int rc;
+    double bb;
+    if ((rc = dot_host<double>(c, rhs, rhs, len, &bb))) return rc;
+    const double beta0 = std::sqrt(bb);
+    *rel = 0.0;
+    if (beta0 == 0.0) return TY_OK;
+    const double target = k->inner_tol * beta0;
+    std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m), hh(MAXK), h2(MAXK);
+    double res = beta0;
+    HIPCHK(c, hipMemsetAsync(c->k_dx, 0, len * sizeof(double), c->stream));
+    auto A_apply = [&](const P* zin, P* wout) -> int {  // wout = zin - J zin
+        double zz;
+        int r2;
+        if ((r2 = dot_host<P>(c, zin, zin, len, &zz))) return r2;
+        const double zn = std::sqrt(zz / (double)len);
+        if (zn == 0.0) {
+            HIPCHK(c, hipMemcpyAsync(wout, zin, len * sizeof(P), hipMemcpyDeviceToDevice, c->stream));
+            return TY_OK;
+        }
+        const double eps = sq * (1.0 + unorm) / zn;
+        HIPCHK(c, (fill_state<P, P>(pert, c->d_u0, eps, zin, len, c->stream)));
+        HIPCHK(c, eval_residual<P>(c, pert, c->k_r1, nullptr, nullptr));
+        HIPCHK(c, apply_a<P>(wout, zin, c->k_r1, c->k_r0, 1.0 / eps, len, c->stream));
+        return TY_OK;
+    };
+    for (int cycle = 0; cycle < std::max(1, k->max_restart); ++cycle) {
+        double beta;
+        if (cycle == 0) {
+            beta = beta0;
+            HIPCHK(c, (scale<P, double>(V, rhs, 1.0 / beta, len, c->stream)));
+        } else {
+            // current residual rhs - (d - J d) from a one-sided JVP of the accumulated correction
+            double dd;
+            if ((rc = dot_host<double>(c, c->k_dx, c->k_dx, len, &dd))) return rc;
+            const double dn = std::sqrt(dd / (double)len);
+            const double eps = sq * (1.0 + unorm) / std::max(dn, 1e-300);
+            HIPCHK(c, (fill_state<P, double>(pert, c->d_u0, eps, c->k_dx, len, c->stream)));
+            HIPCHK(c, eval_residual<P>(c, pert, c->k_r1, nullptr, nullptr));
+            HIPCHK(c, outer_residual(c->k_r2, rhs, c->k_dx, c->k_r1, c->k_r0, 1.0 / eps, len, c->stream));
+            if ((rc = dot_host<double>(c, c->k_r2, c->k_r2, len, &bb))) return rc;
+            beta = std::sqrt(bb);
+            if (beta <= target) { res = beta; break; }
+            HIPCHK(c, (scale<P, double>(V, c->k_r2, 1.0 / beta, len, c->stream)));
+        }
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int jj = 0;
+        for (int j = 0; j < m; ++j) {
+            P* vj = V + (size_t)j * len;
+            const P* zj = vj;
+            if (k->precond) {
+                HIPCHK(c, block_apply<P>(c->nv, dinv, vj, z, c->n, c->stream));
+                zj = z;
+            }
+            if ((rc = A_apply(zj, w))) return rc;
+            // classical Gram-Schmidt, twice
+            for (int pass = 0; pass < 2; ++pass) {
+                double* hp = pass ? h2.data() : hh.data();
+                HIPCHK(c, multidot<P>(w, V, len, j + 1, len, c->k_part, c->k_h, c->stream));
+                HIPCHK(c, hipMemcpyAsync(hp, c->k_h, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(c, cgs_update<P>(w, V, len, c->k_h, j + 1, len, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+            }
+            double ww;
+            if ((rc = dot_host<P>(c, w, w, len, &ww))) return rc;
+            const double hn = std::sqrt(ww);
+            double* col = &H[(size_t)j * (m + 1)];
+            for (int i = 0; i <= j; ++i) col[i] = hh[i] + h2[i];
+            col[j + 1] = hn;
+            for (int i = 0; i < j; ++i) {
+                const double tmp = col[i];
+                col[i] = cs[i] * tmp + sn[i] * col[i + 1];
+                col[i + 1] = -sn[i] * tmp + cs[i] * col[i + 1];
+            }
+            const double rr = std::sqrt(col[j] * col[j] + col[j + 1] * col[j + 1]);
+            cs[j] = rr > 0 ? col[j] / rr : 1.0;
+            sn[j] = rr > 0 ? col[j + 1] / rr : 0.0;
+            col[j] = rr;
+            col[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            res = std::fabs(g[j + 1]);
+            ++*iters;
+            jj = j + 1;
+            if (res <= target || hn == 0.0) break;
+            if (j + 1 < m) HIPCHK(c, (scale<P, P>(V + (size_t)(j + 1) * len, w, 1.0 / hn, len, c->stream)));
+        }
+        // y = H^-1 g, correction d += M^-1 V y
+        for (int i = jj - 1; i >= 0; --i) {
+            double tmp = g[i];
+            for (int l = i + 1; l < jj; ++l) tmp -= H[(size_t)l * (m + 1) + i] * y[l];
+            y[i] = tmp / H[(size_t)i * (m + 1) + i];
+        }
+        HIPCHK(c, hipMemcpyAsync(c->k_h, y.data(), jj * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, lincomb<P>(t, V, len, c->k_h, jj, len, c->stream));
+        if (k->precond) {
+            HIPCHK(c, block_apply<P>(c->nv, dinv, t, z, c->n, c->stream));
+            HIPCHK(c, accumulate<P>(c->k_dx, z, len, c->stream));
+        } else {
+            HIPCHK(c, accumulate<P>(c->k_dx, t, len, c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (res <= target) break;
+    }
+    *rel = res / beta0;
+    return TY_OK;
+}
+
+// solve (I - J) x = k_b at the current state into k_x
+int solve_dev(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* st)
+{
+    if (k->nkrylov < 1 || k->nkrylov >= MAXK) return fail(c, TY_ERR_ARG, "nkrylov must be in 1..31");
+    const int prec = k->inner_precision == TY_PREC_MIXED ? 1 : 0;
+    int rc;
+    if ((rc = alloc_krylov(c, k->nkrylov, prec))) return rc;
+    const long len = (long)c->nv * c->n;
+    c->h_dt = dt;
+    HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    double uu, bb;
+    if ((rc = dot_host<double>(c, c->d_u0, c->d_u0, len, &uu)) || (rc = dot_host<double>(c, c->k_b, c->k_b, len, &bb)))
+        return rc;
+    const double unorm = std::sqrt(uu / (double)len);
+    const double bnorm = std::sqrt(bb);
+    // inner-precision residual at U (base of the one-sided JVP) and the preconditioner
+    if (prec) {
+        HIPCHK(c, (fill_state<float, double>(c->k_pf, c->d_u0, 0.0, c->d_u0, len, c->stream)));
+        HIPCHK(c, eval_residual<float>(c, c->k_pf, c->k_r0, k->precond ? (float*)c->k_dinv : nullptr, nullptr));
+    } else {
+        HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, 0.0, c->d_u0, len, c->stream)));
+        HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r0, k->precond ? (double*)c->k_dinv : nullptr, nullptr));
+    }
+    HIPCHK(c, hipMemsetAsync(c->k_x, 0, len * sizeof(double), c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->k_r, c->k_b, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    st->outer_iters = 0;
+    st->inner_iters = 0;
+    st->rhs_norm = bnorm;
+    st->rel_res = bnorm > 0 ? 1.0 : 0.0;
+    if (bnorm == 0.0) return TY_OK;
+    const double cb = std::cbrt(2.220446049250313e-16);
+    for (int outer = 0;; ++outer) {
+        if (outer > 0) {
+            // FP64 residual of the refinement with a central-difference JVP
+            double xx;
+            if ((rc = dot_host<double>(c, c->k_x, c->k_x, len, &xx))) return rc;
+            const double xn = std::sqrt(xx / (double)len);
+            const double eps = cb * (1.0 + unorm) / std::max(xn, 1e-300);
+            HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, eps, c->k_x, len, c->stream)));
+            HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r1, nullptr, nullptr));
+            HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, -eps, c->k_x, len, c->stream)));
+            HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r2, nullptr, nullptr));
+            HIPCHK(c, outer_residual(c->k_r, c->k_b, c->k_x, c->k_r1, c->k_r2, 0.5 / eps, len, c->stream));
+        }
+        double rr;
+        if ((rc = dot_host<double>(c, c->k_r, c->k_r, len, &rr))) return rc;
+        st->rel_res = std::sqrt(rr) / bnorm;
+        if (st->rel_res <= k->outer_tol || outer >= k->max_outer) break;
+        int it = 0;
+        double irel = 0.0;
+        if (prec) {
+            rc = gmres<float>(c, c->k_r, k, unorm, &it, &irel);
+        } else {
+            rc = gmres<double>(c, c->k_r, k, unorm, &it, &irel);
+        }
+        if (rc) return rc;
+        st->inner_iters += it;
+        st->outer_iters = outer + 1;
+        HIPCHK(c, accumulate<double>(c->k_x, c->k_dx, len, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return st->rel_res <= k->outer_tol ? TY_OK : TY_ERR_NOT_CONVERGED;
+}
+
+void host_to_dev_vec(ty_ctx* c, const double* h, std::vector<double>& d)
+{
+    const int n = c->n;
+    d.assign((size_t)c->nv * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        const int o = c->perm[i];
+        for (int v = 0; v < c->nv; ++v) d[(size_t)v * n + i] = h[(size_t)v * n + o];
+    }
+}
+
+void dev_to_host_vec(ty_ctx* c, const std::vector<double>& d, double* h)
+{
+    const int n = c->n;
+    for (int i = 0; i < n; ++i) {
+        const int o = c->perm[i];
+        for (int v = 0; v < 5; ++v) h[(size_t)v * n + o] = v < c->nv ? d[(size_t)v * n + i] : 0.0;
+    }
+}
+
+}  // namespace
+
+int ty_jacobian_diag(ty_ctx* c, double dt, double* diag)
+{
+    if (!c || !diag) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    hipSetDevice(c->device);
+    const int n = c->n, nv = c->nv;
+    double* d_diag = nullptr;
+    double* d_inv = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &d_diag, (size_t)nv * nv * n)) || (rc = dalloc(c, &d_inv, (size_t)nv * nv * n))) return rc;
+    c->h_dt = dt;
+    HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((rc = ensure_work(c, 0))) return rc;
+    HIPCHK(c, eval_residual<double>(c, c->d_u0, c->d_res, d_inv, d_diag));
+    std::vector<double> h((size_t)nv * nv * n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d_diag, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) {
+        const int o = c->perm[i];
+        double* out = diag + (size_t)25 * o;
+        for (int k = 0; k < 25; ++k) out[k] = (k % 6 == 0) ? 1.0 : 0.0;
+        for (int a = 0; a < nv; ++a)
+            for (int b = 0; b < nv; ++b) out[a * 5 + b] = h[(size_t)(a * nv + b) * n + i];
+    }
+    // release the temporaries
+    for (double* p : {d_diag, d_inv}) {
+        hipFree(p);
+        c->owned.erase(std::find(c->owned.begin(), c->owned.end(), (void*)p));
+    }
+    return TY_OK;
+}
+
+int ty_jvp(ty_ctx* c, double dt, const double* v, double* jv)
+{
+    if (!c || !v || !jv) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    hipSetDevice(c->device);
+    int rc;
+    if ((rc = alloc_krylov(c, std::max(1, c->kry_m), c->kry_prec < 0 ? 0 : c->kry_prec))) return rc;
+    const long len = (long)c->nv * c->n;
+    std::vector<double> h;
+    host_to_dev_vec(c, v, h);
+    HIPCHK(c, hipMemcpyAsync(c->k_x, h.data(), len * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->h_dt = dt;
+    HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    double uu, xx;
+    if ((rc = dot_host<double>(c, c->d_u0, c->d_u0, len, &uu)) || (rc = dot_host<double>(c, c->k_x, c->k_x, len, &xx)))
+        return rc;
+    const double unorm = std::sqrt(uu / (double)len), xn = std::sqrt(xx / (double)len);
+    const double eps = std::cbrt(2.220446049250313e-16) * (1.0 + unorm) / std::max(xn, 1e-300);
+    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, eps, c->k_x, len, c->stream)));
+    HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r1, nullptr, nullptr));
+    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, -eps, c->k_x, len, c->stream)));
+    HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r2, nullptr, nullptr));
+    // Jv = (r1 - r2) / (2 eps): reuse outer_residual with b = 0 and x = 0 -> r = (r1 - r2) s
+    HIPCHK(c, hipMemsetAsync(c->k_b, 0, len * sizeof(double), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->k_dx, 0, len * sizeof(double), c->stream));
+    HIPCHK(c, outer_residual(c->k_r, c->k_b, c->k_dx, c->k_r1, c->k_r2, 0.5 / eps, len, c->stream));
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->k_r, len * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_to_host_vec(c, h, jv);
+    return TY_OK;
+}
+
+int ty_linear_solve(ty_ctx* c, double dt, const ty_krylov* k, const double* b, double* x, ty_krylov_stats* st)
+{
+    if (!c || !k || !b || !x || !st) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    hipSetDevice(c->device);
+    int rc;
+    if ((rc = alloc_krylov(c, k->nkrylov, k->inner_precision == TY_PREC_MIXED ? 1 : 0))) return rc;
+    const long len = (long)c->nv * c->n;
+    std::vector<double> h;
+    host_to_dev_vec(c, b, h);
+    HIPCHK(c, hipMemcpyAsync(c->k_b, h.data(), len * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const int rs = solve_dev(c, dt, k, st);
+    if (rs && rs != TY_ERR_NOT_CONVERGED) return rs;
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->k_x, len * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_to_host_vec(c, h, x);
+    if (rs) return fail(c, rs, "implicit solve did not reach the requested tolerance");
+    return TY_OK;
+}
+
+int ty_implicit_step(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* st)
+{
+    if (!c || !k || !st) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    hipSetDevice(c->device);
+    int rc;
+    if ((rc = alloc_krylov(c, k->nkrylov, k->inner_precision == TY_PREC_MIXED ? 1 : 0))) return rc;
+    const long len = (long)c->nv * c->n;
+    c->h_dt = dt;
+    HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // b = R(U) in FP64 (calc_rhs.f90:119-145)
+    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, 0.0, c->d_u0, len, c->stream)));
+    HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_b, nullptr, nullptr));
+    const int rs = solve_dev(c, dt, k, st);
+    if (rs && rs != TY_ERR_NOT_CONVERGED) return rs;
+    // update_field.f90:38: U += dU, then refresh the stage-state copy
+    HIPCHK(c, accumulate<double>(c->d_u0, c->k_x, len, c->stream));
+    hipError_t e = c->mixed ? launch_cast<float>(c->d_u0, (float*)c->d_us[c->cur], len, c->stream)
+                            : launch_cast<double>(c->d_u0, (double*)c->d_us[c->cur], len, c->stream);
+    HIPCHK(c, e);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (rs) return fail(c, rs, "implicit solve did not reach the requested tolerance");
     return TY_OK;
 }

@@ -83,6 +83,8 @@ struct KArgs {
     const R* qf;          // [ns][nv][n] own-side face states (second order)
     R* qf_out;            // written by K1 when not null
     double* res_out;      // [nv][n] residual-only mode
+    R* dinv_out;          // [nv*nv][n] inverse diagonal blocks of the implicit operator (residual mode)
+    double* diag_out;     // [nv*nv][n] the diagonal blocks themselves (optional)
     const double* d_dt;   // device time step
     // RK bookkeeping for this stage
     int mode;             // 0: stage update, 1: residual only

@@ -7,6 +7,7 @@
 // so the two cells of a face compute bit-identical fluxes.  The residual is a
 // deterministic per-cell sum in slot order: no atomics, no face-flux array.
 // Fluxes and reconstruction follow the reference routines cited per function.
+#pragma once
 #include "ty_internal.h"
 
 namespace ty {
@@ -20,11 +21,6 @@ __device__ __forceinline__ R fmax_(R a, R b) { return a > b ? a : b; }
 __device__ __forceinline__ double dpow(double a, double b) { return pow(a, b); }
 __device__ __forceinline__ float dpow(float a, float b) { return powf(a, b); }
 
-int grid_blocks(int n)
-{
-    int nb = (n + BLOCK - 1) / BLOCK;
-    return (nb + 7) & ~7;
-}
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD
 // owns one contiguous range of RCM-ordered cells and its L2 sees the
@@ -802,6 +798,122 @@ __device__ void viscous_face(const Physics& ph, const Side<DIM, R>& F, const R* 
 }
 
 // ---------------------------------------------------------------------------
+// Face Jacobians for the implicit operator: calc_jacflux (calc_jacflux.f90:49-77), the
+// Euler flux Jacobian A(n) in (rho, rhoE, rho u..) at a primitive state, weighted by
+// calc_jac_hll (calc_jac_hll.f90:46-57) with the HLLC wave speeds clipped at zero
+// (calc_flux_hllc.f90:185-189).  Row-major NV x NV blocks, dF_i/dU_j.
+
+template <int DIM, typename R>
+__device__ __forceinline__ void jacflux(R g, const R* q, const R* n, R vn, R* m)
+{
+    constexpr int NV = DIM + 2;
+    const R g1 = g - R(1), ig1 = R(1) / g1;
+    const R v2 = dotd<DIM>(q + 2, q + 2);
+    const R en = ig1 * q[1] / q[0] + R(.5) * v2;
+    const R* u = q + 2;
+    m[0] = R(0);
+    m[1] = R(0);
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) m[2 + j] = n[j];
+    m[NV + 0] = vn * (g1 * v2 - g * en);
+    m[NV + 1] = vn * g;
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) m[NV + 2 + j] = (g * en - R(.5) * g1 * v2) * n[j] - g1 * vn * u[j];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        m[(2 + i) * NV + 0] = R(.5) * g1 * v2 * n[i] - vn * u[i];
+        m[(2 + i) * NV + 1] = g1 * n[i];
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) m[(2 + i) * NV + 2 + j] = u[i] * n[j] - (g1 * n[i]) * u[j] + (i == j ? vn : R(0));
+    }
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void jac_hll(const Physics& ph, const R* qL, const R* qR, const R* n, R* jL, R* jR)
+{
+    constexpr int NV = DIM + 2;
+    const R g = R(ph.gamma);
+    const R beta = (ph.sch_hyp == 26) ? sqrt((g - R(1)) / (R(2) * g)) : R(1);
+    const R vnl = dotd<DIM>(qL + 2, n), vnr = dotd<DIM>(qR + 2, n);
+    const R al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
+    R rrho, rv[DIM], rp;
+    roe_avg<DIM>(g, qL, qR, rrho, rv, rp);
+    const R vm = dotd<DIM>(rv, n);
+    const R am = sqrt(g * rp / rrho);
+    R sl = fmin_(vnl - beta * al, vm - am);
+    R sr = fmax_(vnr + beta * ar, vm + am);
+    sl = fmin_(sl, R(0));
+    sr = fmax_(sr, R(0));
+    jacflux<DIM>(g, qL, n, vnl, jL);
+    jacflux<DIM>(g, qR, n, vnr, jR);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) { jL[i * NV + i] -= sl; jR[i * NV + i] -= sr; }
+    const R kl = sr / (sr - sl), kr = -sl / (sr - sl);
+#pragma unroll
+    for (int k = 0; k < NV * NV; ++k) { jL[k] = kl * jL[k]; jR[k] = kr * jR[k]; }
+}
+
+// viscous diagonal of the face Jacobians (calc_flux_viscous_cell.f90:149-158)
+template <int DIM, typename R>
+__device__ __forceinline__ R visc_jac_diag(const Physics& ph, const Side<DIM, R>& F, const R* xf)
+{
+    const R r = R(ph.rgas);
+    R a[DIM], b[DIM], d[DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) { a[e] = xf[e] - F.xL[e]; b[e] = xf[e] - F.xR[e]; d[e] = F.xR[e] - F.xL[e]; }
+    R dHL = sqrt(dotd<DIM>(a, a)), dHR = sqrt(dotd<DIM>(b, b));
+    const R id0 = R(1) / (dHL + dHR);
+    dHL = id0 * dHL;
+    dHR = id0 * dHR;
+    const R TL = F.qL[1] / (F.qL[0] * r), TR = F.qR[1] / (F.qR[0] * r);
+    const R TH = dHR * TL + dHL * TR;
+    const R rhoH = dHR * F.qL[0] + dHL * F.qR[0];
+    R mu;
+    if (ph.visc_law == 11) mu = R(ph.visc_dyn) * dpow(TH / R(ph.tref), R(1.5)) * (R(ph.tref) + R(ph.tsuth)) / (TH + R(ph.tsuth));
+    else if (ph.visc_law == 14) mu = R(ph.visc_kin) * rhoH;
+    else mu = R(ph.visc_dyn);
+    return mu / (dHR * F.qL[0] + dHL * F.qR[0]) / sqrt(dotd<DIM>(d, d));
+}
+
+// in-register Gauss-Jordan inverse with partial pivoting (FP64)
+template <int NV>
+__device__ __forceinline__ void invert_block(double* m, double* inv)
+{
+#pragma unroll
+    for (int i = 0; i < NV * NV; ++i) inv[i] = (i % (NV + 1) == 0) ? 1.0 : 0.0;
+#pragma unroll
+    for (int col = 0; col < NV; ++col) {
+        int piv = col;
+        double best = fabs(m[col * NV + col]);
+#pragma unroll
+        for (int r = col + 1; r < NV; ++r) {
+            const double v = fabs(m[r * NV + col]);
+            if (v > best) { best = v; piv = r; }
+        }
+#pragma unroll
+        for (int r = col; r < NV; ++r) {
+            if (r == piv && piv != col) {
+#pragma unroll
+                for (int k = 0; k < NV; ++k) {
+                    double t = m[col * NV + k]; m[col * NV + k] = m[r * NV + k]; m[r * NV + k] = t;
+                    t = inv[col * NV + k]; inv[col * NV + k] = inv[r * NV + k]; inv[r * NV + k] = t;
+                }
+            }
+        }
+        const double ip = 1.0 / m[col * NV + col];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) { m[col * NV + k] *= ip; inv[col * NV + k] *= ip; }
+#pragma unroll
+        for (int r = 0; r < NV; ++r) {
+            if (r == col) continue;
+            const double f = m[r * NV + col];
+#pragma unroll
+            for (int k = 0; k < NV; ++k) { m[r * NV + k] -= f * m[col * NV + k]; inv[r * NV + k] -= f * inv[col * NV + k]; }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // K2: fluxes + deterministic gather + fused RK stage update.
 // calc_rhs.f90:48-145 (face_state boundary map on the right state), integration_ns_ust.f90,
 // flux_to_res.f90:47-142, treelevel_rungekutta.f90:60-105, update_field.f90:38.
@@ -809,7 +921,7 @@ __device__ void viscous_face(const Physics& ph, const Side<DIM, R>& F, const R* 
 // at its mirror slot; otherwise first order on cell states.  VISC: add the
 // viscous flux from cell states and cell gradients of both sides.
 
-template <int DIM, typename R, bool FS, bool VISC, int NSM>
+template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC>
 __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
 {
     constexpr int NV = DIM + 2;
@@ -820,7 +932,13 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
     const R gam = R(a.ph.gamma);
     R qc[NV], xc[DIM];
     if (!FS || VISC) load_prim<DIM>(a.us, n, c, g1, qc);
-    if (VISC) cell_x<DIM>(a.g, c, xc);
+    if (VISC || JAC) cell_x<DIM>(a.g, c, xc);
+    double D[JAC ? NV * NV : 1];
+    const double kdt = JAC ? *a.d_dt / a.vol[c] : 0.0;
+    if (JAC) {
+#pragma unroll
+        for (int k = 0; k < NV * NV; ++k) D[k] = (k % (NV + 1) == 0) ? 1.0 : 0.0;
+    }
     R gcell[VISC ? NV * DIM : 1];
     if (VISC) load_grad<DIM>(a.grad, n, c, gcell);
     R res[NV];
@@ -885,6 +1003,27 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
             viscous_face<DIM>(a.ph, S, xf, nf, gL, gR, flux);
         }
         const R A = a.g.fs[s.f];
+        if (JAC) {
+            // build_implicit_bdlu.f90:44-83: diag(ic1) += jacL.A dt/V, diag(ic2) -= jacR.A dt/V
+            R jL[NV * NV], jR[NV * NV];
+            jac_hll<DIM>(a.ph, qLf, qRf, nf, jL, jR);
+            if (VISC) {
+                Side<DIM, R> S;
+                side_ctx<DIM>(a, c, qc, xc, s, nf, S);
+                R xf[DIM];
+                face_x<DIM>(a.g, s.f, xf);
+                const R id = visc_jac_diag<DIM>(a.ph, S, xf);
+#pragma unroll
+                for (int i = 0; i < NV; ++i) { jR[i * NV + i] -= id; jL[i * NV + i] += id; }
+            }
+            if (s.other) {
+#pragma unroll
+                for (int k = 0; k < NV * NV; ++k) D[k] -= (double)(A * jR[k]) * kdt;
+            } else {
+#pragma unroll
+                for (int k = 0; k < NV * NV; ++k) D[k] += (double)(A * jL[k]) * kdt;
+            }
+        }
         // flux_to_res.f90:90-113: R(ic1) -= F.A, R(ic2) += F.A
         if (s.other) {
 #pragma unroll
@@ -903,6 +1042,16 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
     if (a.mode == 1) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) a.res_out[(size_t)v * n + c] = Rv[v];
+        if (JAC) {
+            if (a.diag_out) {
+#pragma unroll
+                for (int k = 0; k < NV * NV; ++k) a.diag_out[(size_t)k * n + c] = D[k];
+            }
+            double inv[NV * NV];
+            invert_block<NV>(D, inv);
+#pragma unroll
+            for (int k = 0; k < NV * NV; ++k) a.dinv_out[(size_t)k * n + c] = (R)inv[k];
+        }
         return;
     }
     // RK bookkeeping (treelevel_rungekutta.f90:60-105)
@@ -998,7 +1147,7 @@ __global__ __launch_bounds__(BLOCK) void k_dt_local(DtArgs a)
     }
 }
 
-__global__ __launch_bounds__(1024) void k_dt_final(const double* part, const int* badp, int nb, double dtmax,
+static __global__ __launch_bounds__(1024) void k_dt_final(const double* part, const int* badp, int nb, double dtmax,
                                                    double* d_dt, int* d_err)
 {
     __shared__ double smin[16];
@@ -1049,23 +1198,29 @@ hipError_t launch_gradient(const KArgs<R>& a, hipStream_t s)
     return hipGetLastError();
 }
 
-template <int DIM, typename R, int NSM>
+template <int DIM, typename R, int NSM, bool JAC>
 static void flux_variant(const KArgs<R>& a, hipStream_t s)
 {
     const dim3 grid(grid_blocks(a.n)), block(BLOCK);
     const bool fs = a.ph.method != 'N';
     const bool visc = a.ph.viscous != 0;
-    if (fs && visc) hipLaunchKernelGGL((k_flux<DIM, R, true, true, NSM>), grid, block, 0, s, a);
-    else if (fs) hipLaunchKernelGGL((k_flux<DIM, R, true, false, NSM>), grid, block, 0, s, a);
-    else if (visc) hipLaunchKernelGGL((k_flux<DIM, R, false, true, NSM>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_flux<DIM, R, false, false, NSM>), grid, block, 0, s, a);
+    if (fs && visc) hipLaunchKernelGGL((k_flux<DIM, R, true, true, NSM, JAC>), grid, block, 0, s, a);
+    else if (fs) hipLaunchKernelGGL((k_flux<DIM, R, true, false, NSM, JAC>), grid, block, 0, s, a);
+    else if (visc) hipLaunchKernelGGL((k_flux<DIM, R, false, true, NSM, JAC>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_flux<DIM, R, false, false, NSM, JAC>), grid, block, 0, s, a);
 }
 
 template <int DIM, typename R>
 hipError_t launch_flux(const KArgs<R>& a, hipStream_t s)
 {
-    if (a.ns <= 4) flux_variant<DIM, R, 4>(a, s);
-    else flux_variant<DIM, R, MAX_SLOTS>(a, s);
+    const bool jac = a.dinv_out != nullptr;
+    if (a.ns <= 4) {
+        if (jac) flux_variant<DIM, R, 4, true>(a, s);
+        else flux_variant<DIM, R, 4, false>(a, s);
+    } else {
+        if (jac) flux_variant<DIM, R, MAX_SLOTS, true>(a, s);
+        else flux_variant<DIM, R, MAX_SLOTS, false>(a, s);
+    }
     return hipGetLastError();
 }
 
@@ -1085,17 +1240,5 @@ hipError_t launch_cast(const double* src, R* dst, long n, hipStream_t s)
     hipLaunchKernelGGL((k_cast<R>), dim3(nb), dim3(256), 0, s, src, dst, n);
     return hipGetLastError();
 }
-
-#define TY_INST(DIM, R)                                                          \
-    template hipError_t launch_gradient<DIM, R>(const KArgs<R>&, hipStream_t); \
-    template hipError_t launch_flux<DIM, R>(const KArgs<R>&, hipStream_t);
-TY_INST(2, double)
-TY_INST(3, double)
-TY_INST(2, float)
-TY_INST(3, float)
-template hipError_t launch_timestep<2>(const DtArgs&, double, double*, int*, hipStream_t);
-template hipError_t launch_timestep<3>(const DtArgs&, double, double*, int*, hipStream_t);
-template hipError_t launch_cast<double>(const double*, double*, long, hipStream_t);
-template hipError_t launch_cast<float>(const double*, float*, long, hipStream_t);
 
 }  // namespace ty

@@ -1,0 +1,29 @@
+// Launchers of the Krylov vector kernels (ty_krylov.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ty {
+
+constexpr int MAXK = 32;  // Krylov basis size limit (GMRES restart length + 1)
+
+int multidot_blocks(long len);
+template <typename P, typename T>
+hipError_t fill_state(P* dst, const double* u, double a, const T* v, long len, hipStream_t s);
+template <typename T>
+hipError_t apply_a(T* w, const T* z, const double* r1, const double* r0, double sc, long len, hipStream_t s);
+template <typename T>
+hipError_t block_apply(int nv, const T* dinv, const T* v, T* z, int n, hipStream_t s);
+template <typename T>
+hipError_t multidot(const T* w, const T* V, long stride, int m, long len, double* partial, double* out, hipStream_t s);
+template <typename T>
+hipError_t cgs_update(T* w, const T* V, long stride, const double* h, int m, long len, hipStream_t s);
+template <typename T>
+hipError_t lincomb(T* out, const T* V, long stride, const double* y, int m, long len, hipStream_t s);
+template <typename T, typename U>
+hipError_t scale(T* dst, const U* src, double a, long len, hipStream_t s);
+template <typename U>
+hipError_t accumulate(double* dst, const U* src, long len, hipStream_t s);
+hipError_t outer_residual(double* r, const double* b, const double* x, const double* r1, const double* r0, double sc,
+                          long len, hipStream_t s);
+
+}  // namespace ty

@@ -1,0 +1,224 @@
+// Vector kernels of the matrix-free implicit solve (restarted GMRES with
+// classical Gram-Schmidt applied twice, block-Jacobi preconditioning, FD
+// Jacobian-vector products).  Vectors are [nv][n] arrays of length len = nv*n.
+// Reductions are two-pass (per-block partials in a fixed order, then one block),
+// so every dot product is bitwise repeatable.
+#include "ty_krylov.h"
+
+namespace ty {
+
+constexpr int KB = 256;  // threads per block for vector kernels
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// dst = (P)(u + a * v)
+template <typename P, typename T>
+__global__ void k_fill_state(P* __restrict__ dst, const double* __restrict__ u, double a, const T* __restrict__ v,
+                             long len)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) dst[i] = (P)(u[i] + a * (double)v[i]);
+}
+
+// w = z - (r1 - r0) * s     (A z = z - J z with J z from a one-sided difference)
+// central form: w = z - (r1 - r0) * s with r0 the residual at u - eps z
+template <typename T>
+__global__ void k_apply_a(T* __restrict__ w, const T* __restrict__ z, const double* __restrict__ r1,
+                          const double* __restrict__ r0, double s, long len)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) w[i] = (T)((double)z[i] - (r1[i] - r0[i]) * s);
+}
+
+// z = Dinv v per cell (nv x nv blocks, [nv*nv][n] row-major entries)
+template <int NV, typename T>
+__global__ void k_block_apply(const T* __restrict__ dinv, const T* __restrict__ v, T* __restrict__ z, int n)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    T x[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) x[j] = v[(size_t)j * n + c];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j < NV; ++j) s += dinv[(size_t)(i * NV + j) * n + c] * x[j];
+        z[(size_t)i * n + c] = s;
+    }
+}
+
+// partial[b][i] = sum over block b of w . V_i, i < m (m <= MAXK)
+template <typename T>
+__global__ __launch_bounds__(KB) void k_multidot(const T* __restrict__ w, const T* __restrict__ V, long stride,
+                                                 int m, long len, double* __restrict__ partial)
+{
+    __shared__ double red[KB / 64][MAXK];
+    double acc[MAXK];
+    for (int i = 0; i < m; ++i) acc[i] = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long)gridDim.x * blockDim.x) {
+        const double we = (double)w[e];
+        for (int i = 0; i < m; ++i) acc[i] += we * (double)V[(long)i * stride + e];
+    }
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = 0; i < m; ++i) {
+        const double s = wsum(acc[i]);
+        if (lane == 0) red[wid][i] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < m) {
+        double s = 0.0;
+        for (int k = 0; k < KB / 64; ++k) s += red[k][threadIdx.x];
+        partial[(long)blockIdx.x * MAXK + threadIdx.x] = s;
+    }
+}
+
+__global__ void k_reduce_partials(const double* __restrict__ partial, int nblk, int m, double* __restrict__ out)
+{
+    // one thread per output, fixed summation order
+    const int i = threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += partial[(long)b * MAXK + i];
+    out[i] = s;
+}
+
+// w -= sum_i h_i V_i
+template <typename T>
+__global__ void k_cgs_update(T* __restrict__ w, const T* __restrict__ V, long stride, const double* __restrict__ h,
+                             int m, long len)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= len) return;
+    double s = (double)w[e];
+    for (int i = 0; i < m; ++i) s -= h[i] * (double)V[(long)i * stride + e];
+    w[e] = (T)s;
+}
+
+// out = sum_i y_i V_i
+template <typename T>
+__global__ void k_lincomb(T* __restrict__ out, const T* __restrict__ V, long stride, const double* __restrict__ y,
+                          int m, long len)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= len) return;
+    double s = 0.0;
+    for (int i = 0; i < m; ++i) s += y[i] * (double)V[(long)i * stride + e];
+    out[e] = (T)s;
+}
+
+// dst = a * src (type conversion allowed)
+template <typename T, typename U>
+__global__ void k_scale(T* __restrict__ dst, const U* __restrict__ src, double a, long len)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < len) dst[e] = (T)(a * (double)src[e]);
+}
+
+// dst += src (double accumulator)
+template <typename U>
+__global__ void k_accumulate(double* __restrict__ dst, const U* __restrict__ src, long len)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < len) dst[e] += (double)src[e];
+}
+
+// r = b - (x - Jx) with Jx = (r1 - r0) s   (outer refinement residual, FP64)
+__global__ void k_outer_residual(double* __restrict__ r, const double* __restrict__ b, const double* __restrict__ x,
+                                 const double* __restrict__ r1, const double* __restrict__ r0, double s, long len)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < len) r[e] = b[e] - (x[e] - (r1[e] - r0[e]) * s);
+}
+
+static int nb_of(long len) { return (int)((len + KB - 1) / KB); }
+
+int multidot_blocks(long len)
+{
+    long nb = (len + KB - 1) / KB;
+    return (int)(nb < 1024 ? nb : 1024);
+}
+
+template <typename P, typename T>
+hipError_t fill_state(P* dst, const double* u, double a, const T* v, long len, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_fill_state<P, T>), dim3(nb_of(len)), dim3(KB), 0, s, dst, u, a, v, len);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t apply_a(T* w, const T* z, const double* r1, const double* r0, double sc, long len, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_apply_a<T>), dim3(nb_of(len)), dim3(KB), 0, s, w, z, r1, r0, sc, len);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t block_apply(int nv, const T* dinv, const T* v, T* z, int n, hipStream_t s)
+{
+    const int nb = (n + KB - 1) / KB;
+    if (nv == 4) hipLaunchKernelGGL((k_block_apply<4, T>), dim3(nb), dim3(KB), 0, s, dinv, v, z, n);
+    else hipLaunchKernelGGL((k_block_apply<5, T>), dim3(nb), dim3(KB), 0, s, dinv, v, z, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t multidot(const T* w, const T* V, long stride, int m, long len, double* partial, double* out, hipStream_t s)
+{
+    const int nb = multidot_blocks(len);
+    hipLaunchKernelGGL((k_multidot<T>), dim3(nb), dim3(KB), 0, s, w, V, stride, m, len, partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, (const double*)partial, nb, m, out);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t cgs_update(T* w, const T* V, long stride, const double* h, int m, long len, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cgs_update<T>), dim3(nb_of(len)), dim3(KB), 0, s, w, V, stride, h, m, len);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t lincomb(T* out, const T* V, long stride, const double* y, int m, long len, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_lincomb<T>), dim3(nb_of(len)), dim3(KB), 0, s, out, V, stride, y, m, len);
+    return hipGetLastError();
+}
+template <typename T, typename U>
+hipError_t scale(T* dst, const U* src, double a, long len, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_scale<T, U>), dim3(nb_of(len)), dim3(KB), 0, s, dst, src, a, len);
+    return hipGetLastError();
+}
+template <typename U>
+hipError_t accumulate(double* dst, const U* src, long len, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_accumulate<U>), dim3(nb_of(len)), dim3(KB), 0, s, dst, src, len);
+    return hipGetLastError();
+}
+hipError_t outer_residual(double* r, const double* b, const double* x, const double* r1, const double* r0, double sc,
+                          long len, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_outer_residual, dim3(nb_of(len)), dim3(KB), 0, s, r, b, x, r1, r0, sc, len);
+    return hipGetLastError();
+}
+
+#define TY_KINST(T)                                                                                       \
+    template hipError_t apply_a<T>(T*, const T*, const double*, const double*, double, long, hipStream_t); \
+    template hipError_t block_apply<T>(int, const T*, const T*, T*, int, hipStream_t);                     \
+    template hipError_t multidot<T>(const T*, const T*, long, int, long, double*, double*, hipStream_t);   \
+    template hipError_t cgs_update<T>(T*, const T*, long, const double*, int, long, hipStream_t);          \
+    template hipError_t lincomb<T>(T*, const T*, long, const double*, int, long, hipStream_t);             \
+    template hipError_t accumulate<T>(double*, const T*, long, hipStream_t);
+TY_KINST(float)
+TY_KINST(double)
+template hipError_t fill_state<float, float>(float*, const double*, double, const float*, long, hipStream_t);
+template hipError_t fill_state<float, double>(float*, const double*, double, const double*, long, hipStream_t);
+template hipError_t fill_state<double, double>(double*, const double*, double, const double*, long, hipStream_t);
+template hipError_t fill_state<double, float>(double*, const double*, double, const float*, long, hipStream_t);
+template hipError_t scale<float, float>(float*, const float*, double, long, hipStream_t);
+template hipError_t scale<float, double>(float*, const double*, double, long, hipStream_t);
+template hipError_t scale<double, double>(double*, const double*, double, long, hipStream_t);
+template hipError_t scale<double, float>(double*, const float*, double, long, hipStream_t);
+
+}  // namespace ty

@@ -344,6 +344,10 @@ def channel_quad(nx=256, ny=128, lx=1.0, ly=0.5, jitter=0.1, seed=1, bocos=None)
 
 
 def _box_vertices(n, L, jitter, seed, periodic):
+    if periodic and n < 3:
+        # partner faces are found through vertex ids taken modulo n: at n = 2 two different
+        # faces of one side carry the same set of wrapped ids
+        raise ValueError("a periodic box needs n >= 3")
     h = L / n
     i, j, k = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
     i, j, k = i.ravel(), j.ravel(), k.ravel()

@@ -143,10 +143,50 @@ class Solver:
         self._check(self.lib.ty_timing(self.h, 1 if enable else 0, ms, cnt), "ty_timing")
         return dict(grad_ms=ms[0], flux_ms=ms[1], dt_ms=ms[2], grad_n=cnt[0], flux_n=cnt[1], dt_n=cnt[2])
 
+    # -- implicit path -----------------------------------------------------
+    def jacobian_diag(self, dt):
+        """diagonal blocks [ncell_int, 5, 5] of I + dt/V d(sum F.A)/dU (HLL face Jacobians)"""
+        out = np.empty((self.ni, 5, 5))
+        self._check(self.lib.ty_jacobian_diag(self.h, dt, _p64(out)), "ty_jacobian_diag")
+        return out
+
+    def jvp(self, dt, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(5, self.ni)
+        out = np.empty((5, self.ni))
+        self._check(self.lib.ty_jvp(self.h, dt, _p64(v), _p64(out)), "ty_jvp")
+        return out
+
+    def linear_solve(self, dt, b, kry=None, allow_unconverged=False):
+        """solve (I - J) x = b at the current state; returns (x, stats)"""
+        kry = kry or krylov()
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(5, self.ni)
+        x = np.empty((5, self.ni))
+        st = _lib.TyKrylovStats()
+        rc = self.lib.ty_linear_solve(self.h, dt, C.byref(kry), _p64(b), _p64(x), C.byref(st))
+        if not (allow_unconverged and rc == 5):
+            self._check(rc, "ty_linear_solve")
+        return x, _stats(st)
+
+    def implicit_step(self, dt, kry=None):
+        kry = kry or krylov()
+        st = _lib.TyKrylovStats()
+        self._check(self.lib.ty_implicit_step(self.h, dt, C.byref(kry), C.byref(st)), "ty_implicit_step")
+        return _stats(st)
+
     def mesh_info(self):
         v = [C.c_int32() for _ in range(4)]
         self._check(self.lib.ty_mesh_info(self.h, *[C.byref(x) for x in v]), "ty_mesh_info")
         return dict(ncell=v[0].value, nface=v[1].value, nslot=v[2].value, nbface=v[3].value)
+
+
+def krylov(nkrylov=30, max_restart=4, max_outer=8, inner_precision=1, precond=1, inner_tol=1e-3, outer_tol=1e-8):
+    """Settings of the matrix-free implicit solve (ty_krylov): GMRES(nkrylov) with block-Jacobi,
+    FP32 inner iterations (inner_precision=1) inside FP64 refinement to outer_tol."""
+    return _lib.TyKrylov(nkrylov, max_restart, max_outer, inner_precision, precond, inner_tol, outer_tol)
+
+
+def _stats(st):
+    return dict(outer_iters=st.outer_iters, inner_iters=st.inner_iters, rel_res=st.rel_res, rhs_norm=st.rhs_norm)
 
 
 def prim_to_cons(rho, p, vel, gamma=1.4):
