@@ -90,6 +90,101 @@ def cpu_baseline(n_cpu, budget_s=20.0):
                        f"periodic tets, {steps} RK4 steps ({el:.1f} s)")
 
 
+PROFILE_CMDS = {
+    "trace": "rocprofv3 --kernel-trace --stats -f csv -d {d}/trace -o c2 -- python3 bench.py --steps 20 --warmup 2 "
+             "--no-cpu --no-mixed",
+    "fetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/fetch -o c2 -- python3 bench.py --steps 2 --warmup 1 --no-cpu "
+             "--no-mixed",
+    "write": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/write -o c2 -- python3 bench.py --steps 2 --warmup 1 --no-cpu "
+             "--no-mixed",
+    "cfetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/cfetch -o copy -- python3 bench.py --copy-calibration 1024",
+    "cwrite": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/cwrite -o copy -- python3 bench.py --copy-calibration 1024",
+}
+
+
+def _csv_rows(path):
+    import csv
+    with open(path) as f:
+        return list(csv.DictReader(f))
+
+
+def _kernel_key(name):
+    """short kernel key from a demangled HIP kernel name: 'k_flux<3, double, ...>' -> ('flux', 'double')"""
+    for key, tag in (("k_flux<", "flux"), ("k_gradient<", "grad"), ("k_dt_local<", "dt")):
+        i = name.find(key)
+        if i >= 0:
+            args = name[i + len(key):].split(">")[0].split(",")
+            return tag, args[1].strip() if len(args) > 1 else ""
+    return None, None
+
+
+def summarize_profiles(d, n=100):
+    """Turn rocprofv3 CSVs under d (made by the PROFILE_CMDS commands) into the committed text
+    summaries under profiles/: per-kernel durations, and HBM bytes per launch from FETCH_SIZE /
+    WRITE_SIZE scaled by what the same counters read for a plain copy of known size."""
+    nc = 6 * n ** 3
+    nf = 2 * nc  # periodic Kuhn-split box: every face is shared by two tets
+    bm = byte_model(nc, nf, 0, mixed=False)
+    stats = _csv_rows(os.path.join(d, "trace", "c2_kernel_stats.csv"))
+
+    def pmc(sub, prefix, counter):
+        rows = _csv_rows(os.path.join(d, sub, f"{prefix}_counter_collection.csv"))
+        out = {}
+        for r in rows:
+            if r["Counter_Name"] != counter:
+                continue
+            out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]) * 1024.0)
+        return out
+
+    cf, cw = pmc("cfetch", "copy", "FETCH_SIZE"), pmc("cwrite", "copy", "WRITE_SIZE")
+    copy_bytes = 1024 << 20
+
+    def copy_factor(tab):
+        vals = [v for k, vs in tab.items() if "copy" in k.lower() for v in vs if v > copy_bytes / 4]
+        return copy_bytes / (sum(vals) / len(vals)) if vals else None
+
+    f_fac, w_fac = copy_factor(cf), copy_factor(cw)
+    fe, wr = pmc("fetch", "c2", "FETCH_SIZE"), pmc("write", "c2", "WRITE_SIZE")
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    lines = ["# Residual kernels, 3D Euler MUSCL/LSQ/Venkatakrishnan + HLLC + RK4, 6,000,000 periodic tets (FP64 and FP32)",
+             "", "Commands (from the repository root, output directory D):", ""]
+    lines += ["    " + c.format(d="D") for c in PROFILE_CMDS.values()]
+    lines += ["    python3 bench.py --summarize D", "",
+              "## Kernel durations (rocprofv3 --kernel-trace --stats)", "",
+              "| kernel | calls | avg us | min us | max us | share % |", "|---|---|---|---|---|---|"]
+    for r in stats:
+        tag, prec = _kernel_key(r["Name"])
+        if tag is None:
+            continue
+        lines.append(f"| {r['Name'].replace('|', '/')} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    lines += ["", "## HBM traffic per launch (FP64 kernels)", "",
+              f"Copy calibration (1 GiB device copy): FETCH_SIZE factor {f_fac:.3f}, WRITE_SIZE factor {w_fac:.3f} "
+              "(bytes moved / counter bytes).", "",
+              "| kernel | FETCH_SIZE GB (raw) | read GB (scaled) | WRITE_SIZE GB (raw) | written GB (scaled) | "
+              "HBM GB/launch | algorithmic GB/launch | ratio |", "|---|---|---|---|---|---|---|---|"]
+    traffic = {}
+    for name, vals in fe.items():
+        tag, prec = _kernel_key(name)
+        if tag not in ("flux", "grad") or prec != "double":
+            continue
+        f_raw = sum(vals) / len(vals)
+        wv = wr.get(name, [])
+        w_raw = sum(wv) / len(wv) if wv else float("nan")
+        tot = f_raw * f_fac + w_raw * w_fac
+        alg = bm[tag]
+        traffic[tag] = dict(kernel=name, fetch_raw=f_raw, write_raw=w_raw, fetch_factor=f_fac, write_factor=w_fac,
+                            bytes_per_launch=int(tot), algorithmic_bytes=int(alg), launches=len(vals))
+        lines.append(f"| {name} | {f_raw / 1e9:.3f} | {f_raw * f_fac / 1e9:.3f} | {w_raw / 1e9:.3f} | "
+                     f"{w_raw * w_fac / 1e9:.3f} | {tot / 1e9:.3f} | {alg / 1e9:.3f} | {tot / alg:.2f} |")
+    with open(os.path.join(ROOT, "profiles", "residual_kernels_c2.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    for tag, t in traffic.items():
+        with open(os.path.join(ROOT, "profiles", f"traffic_{tag}_c2_fp64.json"), "w") as f:
+            json.dump(t, f, indent=1)
+    print("\n".join(lines))
+
+
 def load_traffic(path):
     try:
         with open(path) as f:
@@ -111,7 +206,12 @@ def main():
                     help="device cell order set at mesh upload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (PMC byte-counter calibration)")
+    ap.add_argument("--summarize", metavar="DIR", default=None,
+                    help="write the profiles/ summaries from rocprofv3 output under DIR and exit")
     args = ap.parse_args()
+    if args.summarize:
+        summarize_profiles(args.summarize, args.n)
+        return
     if args.copy_calibration:
         import torch
         nbytes = args.copy_calibration << 20
