@@ -191,6 +191,23 @@ interface
     type(ty_krylov_stats) :: stats
   endfunction
 
+  ! tangent-linear / adjoint of the first-order residual; pass c_null_ptr for absent arrays
+  integer(c_int) function ty_residual_tangent(ctx, dt, du, ds, dvol, dq, dres) bind(C, name="ty_residual_tangent")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value    :: ctx
+    real(c_double), value :: dt
+    type(c_ptr), value    :: du, ds, dvol, dq
+    real(c_double)        :: dres(*)
+  endfunction
+
+  integer(c_int) function ty_residual_adjoint(ctx, dt, lam, gu, gs, gvol, gq) bind(C, name="ty_residual_adjoint")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value    :: ctx
+    real(c_double), value :: dt
+    real(c_double)        :: lam(*), gu(*)
+    type(c_ptr), value    :: gs, gvol, gq
+  endfunction
+
 endinterface
 
 endmodule TYPHON_GPU

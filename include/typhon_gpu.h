@@ -171,6 +171,24 @@ int ty_linear_solve(ty_ctx* ctx, double dt, const ty_krylov* kry, const double* 
  * tstep_implicit convention) */
 int ty_implicit_step(ty_ctx* ctx, double dt, const ty_krylov* kry, ty_krylov_stats* stats);
 
+/* Tangent-linear residual (no reference counterpart: the reference has no derivative code).
+ * Exact forward-mode derivative of the first-order residual R = -dt/V sum_f F.A
+ * (calc_rhs.f90:10, flux_to_res.f90:47-142, first-order face states distrib_field.f90) at the
+ * current state and a fixed dt:
+ *   dres = dR/dU du + dR/dS ds + dR/dV dvol + dR/dq dq
+ * du, dres: [5][ncell_int] conservative (rho, rhoE, rho u, rho v, rho w), caller cell order;
+ * ds: [nface][3] direction of each face's area vector S = surface * normale (caller faces);
+ * dvol: [ncell_int] cell volumes; dq: [nboco][5] ghost primitive state (rho, p, u, v, w) of
+ * supersonic inlets (other bocos ignored).  Any input may be NULL (zero direction).
+ * TY_ERR_UNSUPPORTED unless the scheme is first order and inviscid. */
+int ty_residual_tangent(ty_ctx* ctx, double dt, const double* du, const double* ds, const double* dvol,
+                        const double* dq, double* dres);
+/* Adjoint residual (no reference counterpart): the transpose of ty_residual_tangent,
+ * gu = (dR/dU)^T lam, gs = (dR/dS)^T lam, gvol = (dR/dV)^T lam, gq = (dR/dq)^T lam, same
+ * layouts; gs, gvol and gq may be NULL.  Far faces of merged periodic pairs get zero in gs
+ * (the device uses the geometry of the pair's first face). */
+int ty_residual_adjoint(ty_ctx* ctx, double dt, const double* lam, double* gu, double* gs, double* gvol,
+                        double* gq);
 /* per-kernel timing for benchmarks (no reference counterpart): when enabled, every
  * gradient, flux and time-step launch is bracketed by HIP events on the context's
  * stream.  Returns the summed milliseconds and launch counts recorded since the last

@@ -114,6 +114,34 @@ def tgv_case(n=16, seed=3, mach=0.3, re=1e4, noise=0.01, jitter=0.0, **scheme):
     return m, S.default_scheme(**sch), gas, cons
 
 
+def bump_case(nx=63, ny=33, nz=20, inflow=(1.0, 0.5 * np.sqrt(GAMMA), 1.0), vtx=None, noise=0.01, seed=4,
+              p_out=None, **scheme):
+    """3D Euler tet channel [0,3]x[0,1]x[0,0.6] with the bottom-wall bump y_w = 0.05 sin^2(pi(x-1)),
+    x in [1,2]: inflow (rho, u, p) imposed at x=0 (supersonic-inlet Dirichlet), static pressure p at
+    x=3 (subsonic outlet; ``p_out`` if given), slip walls elsewhere; first-order HLLC, RK2
+    (reference midpoint).
+    Initial state: the inflow state with seeded relative noise (no exact ties in the flux's
+    wave-speed selection).  ``vtx`` replaces the vertex coordinates (displaced wall vertices)."""
+    rho, u, p = inflow
+    mach = u / np.sqrt(GAMMA * p / rho)
+    bocos = [dict(family="IMIN", type=M.BC_INLET_SUP, param=freestream_inlet(rho, p, mach)),
+             dict(family="IMAX", type=M.BC_OUTLET_SUB, param=[p if p_out is None else p_out]),
+             dict(family="JMIN", type=M.BC_SYM), dict(family="JMAX", type=M.BC_SYM),
+             dict(family="KMIN", type=M.BC_SYM), dict(family="KMAX", type=M.BC_SYM)]
+    m = M.bump_channel_tets(nx, ny, nz, bocos=bocos, vtx=vtx, seed=seed)
+    n = m.ncell_int
+    ids = np.arange(n)
+    vel = np.zeros((3, n))
+    vel[0] = u * (1.0 + noise * M.uniform_pm1(seed + 100, ids, 0))
+    vel[1] = u * noise * M.uniform_pm1(seed + 100, ids, 1)
+    vel[2] = u * noise * M.uniform_pm1(seed + 100, ids, 2)
+    cons = prim_to_cons(rho * (1.0 + noise * M.uniform_pm1(seed + 100, ids, 3)),
+                        p * (1.0 + noise * M.uniform_pm1(seed + 100, ids, 4)), vel)
+    sch = dict(sch_hyp=S.SCH_HLLC, method="N", tps_meth=S.TPS_RK2, cfl=0.8)
+    sch.update(scheme)
+    return m, S.default_scheme(**sch), S.default_gas(), cons
+
+
 def oracle_scheme(scheme: dict) -> dict:
     """Oracle configuration (oracle/typhon_oracle.h) for a resolved scheme dict."""
     coef = scheme.get("rk_coef")

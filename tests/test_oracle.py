@@ -409,3 +409,142 @@ def test_hll_jacobian_weights():
     assert abs(a - sr / (sr - sl)) < 1e-12
     np.testing.assert_allclose(jl2, a * (AL - sl * np.eye(5)), rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(jr2, b * (AR - sr * np.eye(5)), rtol=1e-12, atol=1e-12)
+
+
+# --- A10 viscous flux -----------------------------------------------------------------------
+
+def _visc_gas(mu, **kw):
+    g = S.default_gas(typ_fluid=S.EQ_NSLAM, typ_visc=S.VISC_DYNCST, visc_dyn=mu)
+    g.update(kw)
+    return g
+
+
+def _face_flux(m, sch, gas, u):
+    o = K.make_oracle(m, sch, gas)
+    o.set_cons(u)
+    o.calc_rhs(0.01)
+    return o.flux() / m.fsurf[None, :]
+
+
+def _deep_faces(m):
+    """internal faces whose two cells have only internal neighbours (exact LSQ gradients)"""
+    n = m.ncell_int
+    touches = np.zeros(n, dtype=bool)
+    touches[m.ic1[m.nface_int:]] = True
+    bad = touches.copy()
+    for f in range(m.nface_int):  # cells next to a boundary cell
+        a, b = m.ic1[f], m.ic2[f]
+        if touches[a] or touches[b]:
+            bad[a] = bad[b] = True
+    f = np.arange(m.nface_int)
+    return f[~bad[m.ic1[f]] & ~bad[m.ic2[f]]]
+
+
+@pytest.mark.parametrize("dis", [S.DIS_CELLAVG2, S.DIS_CELLFULL])
+def test_viscous_momentum_flux_exact_for_linear_velocity(dis):
+    """u = u0 + G x, uniform rho and p: the face stress is mu (G + G^T - 2/3 tr G I) exactly, so
+    the viscous momentum flux is -tau n; the energy flux -(tau n).u_H has u_H on the segment
+    between the two cell velocities (calc_flux_viscous_cell.f90:75-158)"""
+    from typhon_amd.solver import prim_to_cons
+    m = M.box_tets(5, 1.0, 0.15, 21, periodic=False)
+    n = m.ncell_int
+    x = m.ccent[:n]
+    Gm = np.array([[0.3, -0.2, 0.5], [0.1, -0.4, 0.2], [-0.3, 0.6, 0.25]])
+    vel = (np.array([0.2, -0.1, 0.05])[:, None] + Gm @ x.T)
+    u = prim_to_cons(np.ones(n), np.ones(n), vel)
+    sch = S.default_scheme(sch_dis=dis)
+    mu = 0.7
+    fv = _face_flux(m, sch, _visc_gas(mu), u) - _face_flux(m, sch, S.default_gas(), u)
+    f = _deep_faces(m)
+    assert f.size > 20
+    tau = mu * (Gm + Gm.T - 2.0 / 3.0 * np.trace(Gm) * np.eye(3))
+    nrm = m.fnorm[f]
+    tn = nrm @ tau.T
+    np.testing.assert_allclose(fv[2:5, f].T, -tn, rtol=0, atol=1e-13)
+    assert np.abs(fv[0, f]).max() < 1e-15
+    eL = -(tn * vel[:, m.ic1[f]].T).sum(1)
+    eR = -(tn * vel[:, m.ic2[f]].T).sum(1)
+    lo, hi = np.minimum(eL, eR) - 1e-13, np.maximum(eL, eR) + 1e-13
+    assert np.all((fv[1, f] >= lo) & (fv[1, f] <= hi))
+    # a uniform flow on a periodic mesh has no viscous flux anywhere
+    mp = M.box_tets(4, 1.0, 0.15, 21, periodic=True)
+    npc = mp.ncell_int
+    u0 = prim_to_cons(np.ones(npc), np.ones(npc), np.tile([[0.2], [-0.1], [0.05]], (1, npc)))
+    fv0 = _face_flux(mp, sch, _visc_gas(mu), u0) - _face_flux(mp, sch, S.default_gas(), u0)
+    assert np.abs(fv0).max() < 1e-14
+
+
+def test_viscous_heat_flux_converges_to_fourier_law():
+    """rho = 1, u = 0, T = T0 (1 + 0.1 sin 2 pi x): the energy flux through a face tends to
+    -k dT/dn with k = mu cp / Pr (the cell-full face gradient), at least first order on
+    jittered periodic tets, and without any momentum flux"""
+    from typhon_amd.solver import prim_to_cons
+    gas = _visc_gas(2.0e-3, prandtl=0.72)
+    cp = gas["gamma"] * gas["r_const"] / (gas["gamma"] - 1.0)
+    k = gas["visc_dyn"] * cp / gas["prandtl"]
+    errs = []
+    for nn in (4, 8):
+        m = M.box_tets(nn, 1.0, 0.1, 5, periodic=True)
+        n = m.ncell_int
+        x = m.ccent[:n]
+        T0 = 300.0
+        T = T0 * (1 + 0.1 * np.sin(2 * np.pi * x[:, 0]))
+        u = prim_to_cons(np.ones(n), gas["r_const"] * T, np.zeros((3, n)))
+        sch = S.default_scheme(sch_dis=S.DIS_CELLFULL)
+        fv = _face_flux(m, sch, gas, u) - _face_flux(m, sch, S.default_gas(), u)
+        f = np.arange(m.nface_int)
+        xf = m.fcent[f]
+        dTdn = T0 * 0.1 * 2 * np.pi * np.cos(2 * np.pi * xf[:, 0]) * m.fnorm[f, 0]
+        exact = -k * dTdn
+        scale = k * T0 * 0.1 * 2 * np.pi
+        errs.append(np.abs(fv[1, f] - exact).max() / scale)
+        assert np.abs(fv[2:5, f]).max() < 1e-12 * scale
+    assert errs[1] < 0.1
+    assert errs[0] / errs[1] > 1.8
+
+
+def test_sutherland_viscosity():
+    """Sutherland's law: mu(T_ref) = mu0 and the mu ~ T^(3/2) / (T + S) shape"""
+    g = dict(visc_law=S.VISC_SUTH, visc_dyn=1.68e-5, tref=273.0, tsuth=110.5)
+    R = 287.14
+    assert abs(O.viscosity(1.0, 273.0, **g) - 1.68e-5) < 1e-20
+    T1, T2 = 300.0, 600.0
+    r = O.viscosity(1.0, T2, **g) / O.viscosity(1.0, T1, **g)
+    assert abs(r - (T2 / T1) ** 1.5 * (T1 + 110.5) / (T2 + 110.5)) < 1e-14
+    del R
+
+
+# --- A7 Barth post-limiter ----------------------------------------------------------------------
+
+@pytest.mark.parametrize("post,klim", [(S.POSTLIM_BARTH, 0.5), (S.POSTLIM_SUPERBARTH, 1.0)])
+def test_barth_postlimited_increments_are_bounded(post, klim):
+    """after postlimit_barth every face increment dq = q_f - q_c of rho, p and each velocity
+    component has the sign of the cell-to-neighbour difference D = q_nb - q_c (or is zero) and
+    |dq| <= klim |D| (postlimit_barth.f90:50-223)"""
+    from typhon_amd.solver import prim_to_cons
+    m = M.box_tets(4, 1.0, 0.15, 8, periodic=True)
+    n = m.ncell_int
+    x = m.ccent[:n]
+    rng = np.random.default_rng(4)
+    rho = 1 + 0.3 * np.sin(2 * np.pi * x[:, 0]) + 0.05 * rng.uniform(-1, 1, n)
+    p = 1 + 0.2 * np.cos(2 * np.pi * x[:, 1]) + 0.05 * rng.uniform(-1, 1, n)
+    vel = 0.3 * rng.uniform(-1, 1, (3, n))
+    o = K.make_oracle(m, S.default_scheme(method="M", limiter="A", postlimiter=post), S.default_gas())
+    o.set_cons(prim_to_cons(rho, p, vel))
+    o.calc_rhs(0.01)
+    q = o.prim()
+    qL, qR = o.face_states()
+    f = np.arange(m.nface)
+    per = m.ic2 >= m.ncell_int
+    other = m.ic2.copy()
+    # periodic faces: the ghost holds the partner's interior state
+    for b in m.bocos:
+        if b.ipart is not None:
+            other[b.iface] = m.ic1[b.ipart]
+    for side, qf, c, nb in (("L", qL, m.ic1, other), ("R", qR, other, m.ic1)):
+        sel = f if side == "L" else f[~per]
+        dq = qf[:, sel] - q[:, c[sel]]
+        D = q[:, nb[sel]] - q[:, c[sel]]
+        tol = 1e-13
+        assert np.all(dq * D >= -tol), side
+        assert np.all(np.abs(dq) <= klim * np.abs(D) + tol), side

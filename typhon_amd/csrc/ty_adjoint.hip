@@ -1,0 +1,259 @@
+// Tangent-linear and adjoint of the first-order residual R(U; S, V, q_in) =
+// -dt/V sum_f F(U_L, U_R; n) A (calc_rhs.f90:10, flux_to_res.f90:47-142) at a
+// fixed time step, in FP64.  Inputs of the derivative: the conservative state of
+// the interior cells, the area vector S = A n of every face, the cell volumes and
+// the ghost primitive state of the supersonic inlets.
+//
+// Both kernels keep the primal layout: one work-item per cell walking its face
+// slots, every face flux evaluated in the face's canonical orientation, a
+// deterministic per-cell sum.  The face derivatives come from the flux and
+// boundary templates instantiated with dual numbers (value + one derivative).
+// The adjoint evaluates, per slot, the derivative of the face flux along each
+// of this cell's NV state components and contracts it with the face's adjoint
+// weight m = sigma A (lambda_c dt/V_c - lambda_nb dt/V_nb): (dR/dU)^T lambda
+// gathered without atomics.  Face-area and inlet-state sensitivities are written
+// by the face's owner cell, so every output has exactly one writer.
+#include "ty_kernels_impl.h"
+#include "ty_dual.h"
+#include "ty_adjoint.h"
+
+namespace ty {
+
+template <int DIM, typename R>
+__device__ __forceinline__ void cons_to_prim(const R* u, R g1, R* q)
+{
+    // calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:47-52), same operation order as load_prim
+    const R rho = u[0];
+    R vel[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) vel[d] = u[2 + d] / rho;
+    R sq = vel[0] * vel[0];
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) sq += vel[d] * vel[d];
+    const R ec = R(.5) * rho * sq;
+    q[0] = rho;
+    q[1] = g1 * (u[1] - ec);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) q[2 + d] = vel[d];
+}
+
+// Flux of one face slot in dual arithmetic.  Uc: this cell's conservative state;
+// Uo: the far cell's (ignored on boundary slots); nf: unit normal (owner -> other);
+// dqg: derivative of a supersonic-inlet ghost state (primitive, or null).
+template <int DIM>
+__device__ __forceinline__ void slot_flux(const KArgs<double>& a, const SlotInfo& s, const Dual* Uc, const Dual* Uo,
+                                          const Dual* nf, const double* dqg, int seed_ghost, Dual* F)
+{
+    constexpr int NV = DIM + 2;
+    const Dual g1(a.ph.gamma - 1.0);
+    Dual qc[NV], qL[NV], qR[NV];
+    cons_to_prim<DIM>(Uc, g1, qc);
+    if (s.nb >= 0) {
+        Dual qo[NV];
+        cons_to_prim<DIM>(Uo, g1, qo);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            qL[v] = s.other ? qo[v] : qc[v];
+            qR[v] = s.other ? qc[v] : qo[v];
+        }
+    } else {
+        const BocoDev bc = a.boco[a.g.bboco[-s.nb - 1]];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qL[v] = qc[v];
+        bc_state<DIM>(bc, nf, qL, qR, a.ph);
+        if (bc.kind == BK_INLET_SUP) {
+            if (dqg) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qR[v].d += dqg[v];
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                if (v == seed_ghost) qR[v].d = 1.0;
+        }
+    }
+    num_flux<DIM>(a.ph.sch_hyp, Dual(a.ph.gamma), qL, qR, nf, F);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_res_tangent(AdArgs t)
+{
+    constexpr int NV = DIM + 2;
+    const KArgs<double>& a = t.k;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const int n = a.n;
+    Dual Uc[NV], res[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        Uc[v] = Dual(a.us[(size_t)v * n + c], t.du ? t.du[(size_t)v * n + c] : 0.0);
+        res[v] = Dual(0.0);
+    }
+    for (int k = 0; k < a.ns; ++k) {
+        const int w = a.fw[(size_t)k * n + c];
+        if (w == SLOT_UNUSED) continue;
+        const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
+        double nv[DIM];
+        face_n<DIM>(a.g, s.f, nv);
+        const double A = a.g.fs[s.f];
+        // S = A n: dA = n.dS, dn = (dS - n dA) / A
+        Dual nd[DIM], Ad(A);
+        double dS[DIM], dA = 0.0;
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) {
+            dS[e] = t.ds ? t.ds[(size_t)e * t.nf + s.f] : 0.0;
+            dA += nv[e] * dS[e];
+        }
+        Ad.d = dA;
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e], (dS[e] - nv[e] * dA) / A);
+        Dual Uo[NV];
+        if (s.nb >= 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                Uo[v] = Dual(a.us[(size_t)v * n + s.nb], t.du ? t.du[(size_t)v * n + s.nb] : 0.0);
+        }
+        const double* dqg = nullptr;
+        if (s.nb < 0 && t.dq) dqg = t.dq + (size_t)a.g.bboco[-s.nb - 1] * NVMAX;
+        Dual F[NV];
+        slot_flux<DIM>(a, s, Uc, Uo, nd, dqg, -1, F);
+        // flux_to_res.f90:90-113
+#pragma unroll
+        for (int v = 0; v < NV; ++v) res[v] = s.other ? res[v] + Ad * F[v] : res[v] - Ad * F[v];
+    }
+    const Dual Vd(a.vol[c], t.dvol ? t.dvol[c] : 0.0);
+    const Dual coef = Dual(*a.d_dt) / Vd;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t.out[(size_t)v * n + c] = (coef * res[v]).d;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_res_adjoint(AdArgs t)
+{
+    constexpr int NV = DIM + 2;
+    const KArgs<double>& a = t.k;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const int n = a.n;
+    const double dt = *a.d_dt;
+    const double dtv = dt / a.vol[c];
+    double Uc[NV], lc[NV], gu[NV], res[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        Uc[v] = a.us[(size_t)v * n + c];
+        lc[v] = t.lam[(size_t)v * n + c] * dtv;
+        gu[v] = 0.0;
+        res[v] = 0.0;
+    }
+    for (int k = 0; k < a.ns; ++k) {
+        const int w = a.fw[(size_t)k * n + c];
+        if (w == SLOT_UNUSED) continue;
+        const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
+        double nv[DIM];
+        face_n<DIM>(a.g, s.f, nv);
+        const double A = a.g.fs[s.f];
+        const double sg = s.other ? 1.0 : -1.0;
+        // adjoint weight of this face's flux: d(lambda.R)/dF = sigma A (lambda_c dt/V_c - lambda_nb dt/V_nb)
+        double mv[NV];
+        Dual Uo[NV], Ud[NV], nd[DIM], F[NV];
+        if (s.nb >= 0) {
+            const double dto = dt / a.vol[s.nb];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                Uo[v] = Dual(a.us[(size_t)v * n + s.nb]);
+                mv[v] = sg * A * (lc[v] - t.lam[(size_t)v * n + s.nb] * dto);
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                Uo[v] = Dual(0.0);
+                mv[v] = sg * A * lc[v];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
+        // d/dU_c, one direction per conservative component
+        for (int j = 0; j < NV; ++j) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Ud[v] = Dual(Uc[v], v == j ? 1.0 : 0.0);
+            slot_flux<DIM>(a, s, Ud, Uo, nd, nullptr, -1, F);
+            double acc = 0.0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc += mv[v] * F[v].d;
+            gu[j] += acc;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            res[v] += sg * A * F[v].v;
+            Ud[v] = Dual(Uc[v]);
+        }
+        if (!s.other && t.gs) {
+            // owner writes the face's area-vector sensitivity: through n (flux and boundary
+            // map) and A, then n = S/|S|, A = |S|
+            double Fv[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Fv[v] = F[v].v;
+            double gn[DIM], ndg = 0.0;
+            for (int e = 0; e < DIM; ++e) {
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) nd[i] = Dual(nv[i], i == e ? 1.0 : 0.0);
+                slot_flux<DIM>(a, s, Ud, Uo, nd, nullptr, -1, F);
+                double acc = 0.0;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc += mv[v] * F[v].d;
+                gn[e] = acc;
+                ndg += nv[e] * acc;
+            }
+            double ga = 0.0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ga += (mv[v] / A) * Fv[v];
+#pragma unroll
+            for (int e = 0; e < 3; ++e)
+                t.gs[(size_t)e * t.nf + s.f] = e < DIM ? (gn[e] - nv[e] * ndg) / A + nv[e] * ga : 0.0;
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
+        }
+        if (s.nb < 0 && t.gq) {
+            const int bi = -s.nb - 1;
+            const bool inlet = a.boco[a.g.bboco[bi]].kind == BK_INLET_SUP;
+            for (int j = 0; j < NV; ++j) {
+                double acc = 0.0;
+                if (inlet) {
+                    slot_flux<DIM>(a, s, Ud, Uo, nd, nullptr, j, F);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc += mv[v] * F[v].d;
+                }
+                t.gq[(size_t)j * t.nb + bi] = acc;
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] = gu[v];
+    if (t.gvol) {
+        // d(lambda.R_c)/dV_c = -lambda_c . R_c / V_c with R_c = dt/V_c res_c
+        double acc = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc += lc[v] * res[v];
+        t.gvol[c] = -acc / a.vol[c];
+    }
+}
+
+template <int DIM>
+hipError_t launch_res_tangent(const AdArgs& t, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_res_tangent<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_res_adjoint(const AdArgs& t, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_res_adjoint<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+    return hipGetLastError();
+}
+
+template hipError_t launch_res_tangent<2>(const AdArgs&, hipStream_t);
+template hipError_t launch_res_tangent<3>(const AdArgs&, hipStream_t);
+template hipError_t launch_res_adjoint<2>(const AdArgs&, hipStream_t);
+template hipError_t launch_res_adjoint<3>(const AdArgs&, hipStream_t);
+
+}  // namespace ty

@@ -21,6 +21,7 @@
 #include "../../include/typhon_gpu.h"
 #include "ty_internal.h"
 #include "ty_krylov.h"
+#include "ty_adjoint.h"
 
 using namespace ty;
 
@@ -42,6 +43,8 @@ struct ty_ctx {
     std::vector<int> perm, iperm;  // perm[new] = caller interior index
     std::vector<int> nbr, fw;      // [ns][n]
     std::vector<double> fn, fc, fs;  // [3][nf], [3][nf], [nf]
+    std::vector<int> fsrc;           // [nf] caller face (0-based) whose geometry device face k carries
+    int nface_caller = 0;            // faces of the caller's mesh
     std::vector<double> cc, vol, sumA;  // [3][n], [n], [n]
     std::vector<double> gc;  // [3][nb]
     std::vector<int> bboco;  // [nb]
@@ -404,6 +407,8 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
     c->fn.assign((size_t)3 * ndf, 0.0);
     c->fc.assign((size_t)3 * ndf, 0.0);
     c->fs.assign(ndf, 0.0);
+    c->fsrc.assign(ndf, -1);
+    c->nface_caller = nf;
     c->gc.assign((size_t)3 * c->nb, 0.0);
     c->bboco.assign(c->nb, 0);
     c->ps.assign((size_t)3 * std::max(1, c->nper), 0.0);
@@ -414,6 +419,7 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
             c->fc[(size_t)d * ndf + df] = F(src, 3 + d);
         }
         c->fs[df] = F(src, 6);
+        c->fsrc[df] = src;
     };
     for (int k = 0; k < c->nfi; ++k) {
         const bool per = k >= c->per_base;
@@ -1417,5 +1423,149 @@ int ty_implicit_step(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* 
     HIPCHK(c, e);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (rs) return fail(c, rs, "implicit solve did not reach the requested tolerance");
+    return TY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tangent-linear and adjoint of the first-order residual
+
+namespace {
+
+// device temporaries released when the call returns
+struct DevTmp {
+    ty_ctx* c;
+    std::vector<void*> p;
+    explicit DevTmp(ty_ctx* cc) : c(cc) {}
+    ~DevTmp()
+    {
+        for (void* q : p) {
+            hipFree(q);
+            auto it = std::find(c->owned.begin(), c->owned.end(), q);
+            if (it != c->owned.end()) c->owned.erase(it);
+        }
+    }
+    template <typename T>
+    int alloc(T** out, size_t count)
+    {
+        const int rc = dalloc(c, out, count);
+        if (!rc) p.push_back((void*)*out);
+        return rc;
+    }
+};
+
+int ad_prepare(ty_ctx* c, double dt, AdArgs& t)
+{
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    if (c->ph.method != 'N' || c->ph.viscous)
+        return fail(c, TY_ERR_UNSUPPORTED, "tangent/adjoint residual: first-order inviscid schemes only");
+    hipSetDevice(c->device);
+    c->h_dt = dt;
+    HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    memset(&t, 0, sizeof(t));
+    t.k = make_args<double>(c, c->geo_d);
+    t.k.us = c->d_u0;
+    t.nf = c->nf;
+    t.nb = c->nb;
+    return TY_OK;
+}
+
+}  // namespace
+
+int ty_residual_tangent(ty_ctx* c, double dt, const double* du, const double* ds, const double* dvol,
+                        const double* dq, double* dres)
+{
+    if (!c || !dres) return TY_ERR_ARG;
+    AdArgs t;
+    int rc;
+    if ((rc = ad_prepare(c, dt, t))) return rc;
+    const int n = c->n, nf = c->nf, nv = c->nv;
+    DevTmp tmp(c);
+    double *d_du = nullptr, *d_ds = nullptr, *d_dv = nullptr, *d_dq = nullptr, *d_out = nullptr;
+    std::vector<double> h;
+    if (du) {
+        host_to_dev_vec(c, du, h);
+        if ((rc = tmp.alloc(&d_du, h.size()))) return rc;
+        HIPCHK(c, hipMemcpy(d_du, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (ds) {
+        h.assign((size_t)3 * nf, 0.0);
+        for (int f = 0; f < nf; ++f)
+            for (int e = 0; e < 3; ++e) h[(size_t)e * nf + f] = ds[(size_t)3 * c->fsrc[f] + e];
+        if ((rc = tmp.alloc(&d_ds, h.size()))) return rc;
+        HIPCHK(c, hipMemcpy(d_ds, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (dvol) {
+        h.assign(n, 0.0);
+        for (int i = 0; i < n; ++i) h[i] = dvol[c->perm[i]];
+        if ((rc = tmp.alloc(&d_dv, h.size()))) return rc;
+        HIPCHK(c, hipMemcpy(d_dv, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (dq) {
+        const int nbc = (int)c->bocos.size();
+        h.assign((size_t)std::max(1, nbc) * NVMAX, 0.0);
+        for (int b = 0; b < nbc; ++b)
+            for (int v = 0; v < NVMAX; ++v) h[(size_t)b * NVMAX + v] = dq[(size_t)b * 5 + v];
+        if ((rc = tmp.alloc(&d_dq, h.size()))) return rc;
+        HIPCHK(c, hipMemcpy(d_dq, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if ((rc = tmp.alloc(&d_out, (size_t)nv * n))) return rc;
+    t.du = d_du;
+    t.ds = d_ds;
+    t.dvol = d_dv;
+    t.dq = d_dq;
+    t.out = d_out;
+    HIPCHK(c, c->dim == 2 ? launch_res_tangent<2>(t, c->stream) : launch_res_tangent<3>(t, c->stream));
+    h.assign((size_t)nv * n, 0.0);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_to_host_vec(c, h, dres);
+    return TY_OK;
+}
+
+int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, double* gs, double* gvol, double* gq)
+{
+    if (!c || !lam || !gu) return TY_ERR_ARG;
+    AdArgs t;
+    int rc;
+    if ((rc = ad_prepare(c, dt, t))) return rc;
+    const int n = c->n, nf = c->nf, nb = c->nb, nv = c->nv;
+    DevTmp tmp(c);
+    double *d_lam = nullptr, *d_gu = nullptr, *d_gs = nullptr, *d_gv = nullptr, *d_gq = nullptr;
+    std::vector<double> h;
+    host_to_dev_vec(c, lam, h);
+    if ((rc = tmp.alloc(&d_lam, h.size())) || (rc = tmp.alloc(&d_gu, h.size()))) return rc;
+    HIPCHK(c, hipMemcpy(d_lam, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (gs && (rc = tmp.alloc(&d_gs, (size_t)3 * nf))) return rc;
+    if (gvol && (rc = tmp.alloc(&d_gv, (size_t)n))) return rc;
+    if (gq && (rc = tmp.alloc(&d_gq, (size_t)nv * std::max(1, nb)))) return rc;
+    t.lam = d_lam;
+    t.gu = d_gu;
+    t.gs = d_gs;
+    t.gvol = d_gv;
+    t.gq = d_gq;
+    HIPCHK(c, c->dim == 2 ? launch_res_adjoint<2>(t, c->stream) : launch_res_adjoint<3>(t, c->stream));
+    h.assign((size_t)nv * n, 0.0);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d_gu, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> hs(gs ? (size_t)3 * nf : 0), hv(gvol ? (size_t)n : 0), hq(gq ? (size_t)nv * nb : 0);
+    if (gs) HIPCHK(c, hipMemcpyAsync(hs.data(), d_gs, hs.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (gvol) HIPCHK(c, hipMemcpyAsync(hv.data(), d_gv, hv.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (gq && nb) HIPCHK(c, hipMemcpyAsync(hq.data(), d_gq, hq.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_to_host_vec(c, h, gu);
+    if (gs) {
+        // caller faces [nface][3]; faces whose geometry the device does not use (the far face of
+        // a periodic pair) get zero
+        std::fill(gs, gs + (size_t)3 * c->nface_caller, 0.0);
+        for (int f = 0; f < nf; ++f)
+            for (int e = 0; e < 3; ++e) gs[(size_t)3 * c->fsrc[f] + e] = hs[(size_t)e * nf + f];
+    }
+    if (gvol)
+        for (int i = 0; i < n; ++i) gvol[c->perm[i]] = hv[i];
+    if (gq) {
+        const int nbc = (int)c->bocos.size();
+        std::fill(gq, gq + (size_t)5 * nbc, 0.0);
+        for (int k = 0; k < nb; ++k)  // fixed order: device boundary faces
+            for (int v = 0; v < nv; ++v) gq[(size_t)5 * c->bboco[k] + v] += hq[(size_t)v * nb + k];
+    }
     return TY_OK;
 }

@@ -409,6 +409,54 @@ def box_tets(n=100, L=10.0, jitter=0.05, seed=2, periodic=True, bocos=None):
     return build_ustmesh(3, vtx, cells, "tet", _box_family(n), bocos, wrap)
 
 
+def bump_channel_vertices(nx=63, ny=33, nz=20, lx=3.0, ly=1.0, lz=0.6, jitter=0.05, seed=4, amp=0.05):
+    """Vertices of the bump channel: a (nx+1)(ny+1)(nz+1) grid on [0,lx]x[0,ly]x[0,lz] whose
+    bottom wall follows y_w = amp sin^2(pi (x - 1)) for 1 <= x <= 2 (grid lines stretched
+    between the bump and y = ly), interior vertices jittered by +-jitter h (seeded).
+    Vertex id = (i (ny+1) + j) (nz+1) + k."""
+    hx, hy, hz = lx / nx, ly / ny, lz / nz
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    vid = (i * (ny + 1) + j) * (nz + 1) + k
+    inner = (i > 0) & (i < nx) & (j > 0) & (j < ny) & (k > 0) & (k < nz)
+    x = i * hx + np.where(inner, jitter * hx * uniform_pm1(seed, vid, 0), 0.0)
+    y = j * hy + np.where(inner, jitter * hy * uniform_pm1(seed, vid, 1), 0.0)
+    z = k * hz + np.where(inner, jitter * hz * uniform_pm1(seed, vid, 2), 0.0)
+    yw = np.where((x >= 1.0) & (x <= 2.0), amp * np.sin(np.pi * (x - 1.0)) ** 2, 0.0)
+    y = yw + y * (ly - yw) / ly
+    return np.stack([x, y, z], axis=1)
+
+
+def bump_channel_tets(nx=63, ny=33, nz=20, lx=3.0, ly=1.0, lz=0.6, jitter=0.05, seed=4, amp=0.05, bocos=None,
+                      vtx=None):
+    """Tet channel with a bottom-wall bump (6 Kuhn tets per hex): families IMIN/IMAX (x = 0 / lx),
+    JMIN (bump wall) / JMAX, KMIN/KMAX.  ``vtx`` replaces the generated vertex coordinates (same
+    numbering), e.g. with displaced wall vertices."""
+    if vtx is None:
+        vtx = bump_channel_vertices(nx, ny, nz, lx, ly, lz, jitter, seed, amp)
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+
+    def v(di, dj, dk):
+        return ((ci + di) * (ny + 1) + (cj + dj)) * (nz + 1) + (ck + dk)
+    h = [v(0, 0, 0), v(1, 0, 0), v(1, 1, 0), v(0, 1, 0), v(0, 0, 1), v(1, 0, 1), v(1, 1, 1), v(0, 1, 1)]
+    kuhn = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
+    cells = np.stack([np.stack([h[a], h[b], h[c], h[d]], axis=1) for a, b, c, d in kuhn], axis=1).reshape(-1, 4)
+
+    def fam(fv, fc):
+        vi = fv[:, :3] // ((ny + 1) * (nz + 1))
+        vj = (fv[:, :3] // (nz + 1)) % (ny + 1)
+        vk = fv[:, :3] % (nz + 1)
+        out = np.empty(fv.shape[0], dtype=object)
+        for name, arr, val in (("IMIN", vi, 0), ("IMAX", vi, nx), ("JMIN", vj, 0),
+                               ("JMAX", vj, ny), ("KMIN", vk, 0), ("KMAX", vk, nz)):
+            out[np.all(arr == val, axis=1)] = name
+        return out
+    if bocos is None:
+        bocos = [dict(family=f, type=BC_SYM) for f in ("IMIN", "IMAX", "JMIN", "JMAX", "KMIN", "KMAX")]
+    return build_ustmesh(3, np.asarray(vtx, dtype=np.float64), cells, "tet", fam, bocos, None)
+
+
 def box_hex(n=16, L=2 * np.pi, jitter=0.0, seed=3, periodic=True, bocos=None):
     """n^3 hexes on [0,L]^3."""
     vtx, wrap = _box_vertices(n, L, jitter, seed, periodic)

@@ -156,6 +156,36 @@ class Solver:
         self._check(self.lib.ty_jvp(self.h, dt, _p64(v), _p64(out)), "ty_jvp")
         return out
 
+    def residual_tangent(self, dt, du=None, ds=None, dvol=None, dq=None):
+        """exact directional derivative of the first-order residual at the current state and a
+        fixed dt: du [5, ncell_int], ds [nface, 3] face area vectors S = A n, dvol [ncell_int],
+        dq [nboco, 5] supersonic-inlet ghost state (rho, p, u, v, w); absent = zero"""
+        m = self.mesh
+
+        def arr(x, shape):
+            return None if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(shape)
+        du = arr(du, (5, self.ni))
+        ds = arr(ds, (m.nface, 3))
+        dvol = arr(dvol, (self.ni,))
+        dq = arr(dq, (max(1, len(m.bocos)), 5))
+        out = np.empty((5, self.ni))
+        p = [None if x is None else _p64(x) for x in (du, ds, dvol, dq)]
+        self._check(self.lib.ty_residual_tangent(self.h, dt, *p, _p64(out)), "ty_residual_tangent")
+        return out
+
+    def residual_adjoint(self, dt, lam):
+        """transpose of residual_tangent: dict u [5, ncell_int], s [nface, 3], vol [ncell_int],
+        q [nboco, 5] with the products (dR/d.)^T lam"""
+        m = self.mesh
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(5, self.ni)
+        gu = np.empty((5, self.ni))
+        gs = np.empty((m.nface, 3))
+        gv = np.empty(self.ni)
+        gq = np.empty((max(1, len(m.bocos)), 5))
+        self._check(self.lib.ty_residual_adjoint(self.h, dt, _p64(lam), _p64(gu), _p64(gs), _p64(gv), _p64(gq)),
+                    "ty_residual_adjoint")
+        return dict(u=gu, s=gs, vol=gv, q=gq[:len(m.bocos)])
+
     def linear_solve(self, dt, b, kry=None, allow_unconverged=False):
         """solve (I - J) x = b at the current state; returns (x, stats)"""
         kry = kry or krylov()
