@@ -150,8 +150,8 @@ interface
     import :: c_int, c_int32_t, c_ptr, c_double
     type(c_ptr), value        :: ctx
     integer(c_int32_t), value :: enable
-    real(c_double)            :: ms(3)
-    integer(c_int32_t)        :: count(3)
+    real(c_double)            :: ms(5)
+    integer(c_int32_t)        :: count(5)
   endfunction
 
   integer(c_int) function ty_mesh_info(ctx, ncell, nface_dev, nslot, nbface_dev) bind(C, name="ty_mesh_info")

@@ -190,9 +190,10 @@ int ty_residual_tangent(ty_ctx* ctx, double dt, const double* du, const double* 
 int ty_residual_adjoint(ty_ctx* ctx, double dt, const double* lam, double* gu, double* gs, double* gvol,
                         double* gq);
 /* per-kernel timing for benchmarks (no reference counterpart): when enabled, every
- * gradient, flux and time-step launch is bracketed by HIP events on the context's
- * stream.  Returns the summed milliseconds and launch counts recorded since the last
- * call (ms[3], count[3]: gradient, flux, time step) and sets the enable flag. */
+ * gradient, flux, time-step, tangent and adjoint launch is bracketed by HIP events on the
+ * context's stream.  Returns the summed milliseconds and launch counts recorded since the
+ * last call (ms[5], count[5]: gradient, flux, time step, tangent, adjoint) and sets the
+ * enable flag. */
 int ty_timing(ty_ctx* ctx, int32_t enable, double* ms, int32_t* count);
 
 /* layout facts for the caller / harness: number of slots per cell, merged face count */

@@ -97,7 +97,7 @@ struct ty_ctx {
     double h_dt = 0.0;
     // per-kernel event timing (bench)
     bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_grad, ev_flux, ev_dt;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_grad, ev_flux, ev_dt, ev_tan, ev_adj;
 };
 
 namespace {
@@ -500,83 +500,93 @@ namespace {
 template <typename R>
 int build_geo(ty_ctx* c, Geo<R>& g)
 {
-    const int n = c->n, nf = c->nf, nb = c->nb, dim = c->dim;
-    std::vector<R> h;
-    const int np = std::max(1, c->nper);
-    // layout: fn(3nf) fs(nf) fc(3nf) cc(3n) gc(3nb) lsq(6n) eps2(n) ps(3 nper)
-    size_t o_fn = 0, o_fs = o_fn + 3 * (size_t)nf, o_fc = o_fs + nf, o_cc = o_fc + 3 * (size_t)nf;
-    size_t o_gc = o_cc + 3 * (size_t)n, o_lsq = o_gc + 3 * (size_t)nb, o_eps = o_lsq + 6 * (size_t)n;
-    size_t o_sh = o_eps + n, tot = o_sh + 3 * (size_t)np;
-    h.assign(tot, R(0));
-    for (size_t i = 0; i < 3 * (size_t)nf; ++i) { h[o_fn + i] = (R)c->fn[i]; h[o_fc + i] = (R)c->fc[i]; }
+    const int n = c->n, nf = c->nf, nb = c->nb, dim = c->dim, ns = c->ns;
+    // layout: sn(3 ns n) sa(ns n) sw(3 ns n) sr(3 ns n) sd(3 ns n) fn(3 nf) fs(nf) eps2(n)
+    const size_t S3 = (size_t)3 * ns * n, S1 = (size_t)ns * n;
+    const size_t o_sn = 0, o_sa = o_sn + S3, o_sw = o_sa + S1, o_sr = o_sw + S3, o_sd = o_sr + S3;
+    const size_t o_fn = o_sd + S3, o_fs = o_fn + 3 * (size_t)nf, o_eps = o_fs + nf, tot = o_eps + n;
+    std::vector<R> h(tot, R(0));
+    for (size_t i = 0; i < 3 * (size_t)nf; ++i) h[o_fn + i] = (R)c->fn[i];
     for (int i = 0; i < nf; ++i) h[o_fs + i] = (R)c->fs[i];
-    for (size_t i = 0; i < 3 * (size_t)n; ++i) h[o_cc + i] = (R)c->cc[i];
-    for (size_t i = 0; i < 3 * (size_t)nb; ++i) h[o_gc + i] = (R)c->gc[i];
-    for (size_t i = 0; i < 3 * (size_t)np; ++i) h[o_sh + i] = (R)c->ps[i];
-    // LSQ matrices (precalc_grad_lsq.f90:60-155) and their Cholesky factors, in FP64
-    const bool w = c->sch.gradmeth == 12;
+    auto at3 = [&](size_t base, int k, int e, int i) -> R& { return h[base + ((size_t)k * 3 + e) * n + i]; };
+    // per slot: canonical d = x_R - x_L (owner frame; periodic far sides translated), r = x_f - x_c
+    // in this cell's frame, and the LSQ weights w_k = A^-1 (x_nb - x_c) [/|d|^4 weighted]
+    // (precalc_grad_lsq.f90:60-155: A = sum d d^T [/|d|^4], boundary faces once, 2D A33 += 1)
+    const bool wlsq = c->sch.gradmeth == 12;
     const double K = c->sch.venkat_k;
+    std::vector<double> dl((size_t)MAX_SLOTS * 3);
     for (int i = 0; i < n; ++i) {
         double A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        double x[3] = {c->cc[i], c->cc[(size_t)n + i], c->cc[(size_t)2 * n + i]};
-        for (int k = 0; k < c->ns; ++k) {
-            int wd = c->fw[(size_t)k * n + i];
+        const double x[3] = {c->cc[i], c->cc[(size_t)n + i], c->cc[(size_t)2 * n + i]};
+        for (int k = 0; k < ns; ++k) {
+            const int wd = c->fw[(size_t)k * n + i];
             if (wd == SLOT_UNUSED) continue;
-            int nbv = c->nbr[(size_t)k * n + i];
-            double d[3];
-            if (nbv >= 0) {
-                const bool other = wd < 0;
-                double T[3] = {0.0, 0.0, 0.0};
-                if ((unsigned)wd & PERIODIC_BIT) {
-                    const int j = (int)((unsigned)wd & FACE_MASK) - c->per_base;
-                    for (int e = 0; e < 3; ++e) T[e] = c->ps[(size_t)e * c->nper + j];
-                }
-                for (int e = 0; e < 3; ++e) {
-                    double xo = c->cc[(size_t)e * n + nbv];
+            const int nbv = c->nbr[(size_t)k * n + i];
+            const int f = (int)((unsigned)wd & FACE_MASK);
+            const bool other = wd < 0;
+            double T[3] = {0.0, 0.0, 0.0};
+            if ((unsigned)wd & PERIODIC_BIT) {
+                const int j = f - c->per_base;
+                for (int e = 0; e < 3; ++e) T[e] = c->ps[(size_t)e * c->nper + j];
+            }
+            double d[3], r[3];
+            for (int e = 0; e < 3; ++e) {
+                const double xf = c->fc[(size_t)e * nf + f];
+                if (nbv >= 0) {
+                    const double xo = c->cc[(size_t)e * n + nbv];
                     d[e] = other ? (x[e] + T[e]) - xo : (xo + T[e]) - x[e];
+                    r[e] = other ? (xf - T[e]) - x[e] : xf - x[e];
+                } else {
+                    d[e] = c->gc[(size_t)e * nb + (-nbv - 1)] - x[e];
+                    r[e] = xf - x[e];
                 }
-            } else {
-                int bi = -nbv - 1;
-                for (int e = 0; e < 3; ++e) d[e] = c->gc[(size_t)e * nb + bi] - x[e];
+                at3(o_sn, k, e, i) = (R)c->fn[(size_t)e * nf + f];
+                at3(o_sr, k, e, i) = (R)r[e];
+                at3(o_sd, k, e, i) = (R)d[e];
             }
-            if (w) {
-                double s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                for (int e = 0; e < 3; ++e) d[e] /= s;
-            }
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b) A[a + 3 * b] += d[a] * d[b];
+            h[o_sa + (size_t)k * n + i] = (R)c->fs[f];
+            double m[3];
+            const double s2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            for (int e = 0; e < 3; ++e) m[e] = wlsq ? d[e] / s2 : d[e];
+            for (int a2 = 0; a2 < 3; ++a2)
+                for (int b2 = 0; b2 < 3; ++b2) A[a2 + 3 * b2] += m[a2] * m[b2];
+            // right-hand-side direction x_nb - x_c in this cell's frame
+            for (int e = 0; e < 3; ++e) dl[(size_t)k * 3 + e] = (other ? -d[e] : d[e]) / (wlsq ? s2 * s2 : 1.0);
         }
         if (dim == 2) A[8] += 1.0;
         cholesky3(A);
-        // l11 l21 l31 l22 l32 l33
-        const double L[6] = {A[0], A[1], A[2], A[4], A[5], A[8]};
-        for (int k = 0; k < 6; ++k) h[o_lsq + (size_t)k * n + i] = (R)L[k];
+        const double l11 = A[0], l21 = A[1], l31 = A[2], l22 = A[4], l32 = A[5], l33 = A[8];
+        for (int k = 0; k < ns; ++k) {
+            if (c->fw[(size_t)k * n + i] == SLOT_UNUSED) continue;
+            const double* b = &dl[(size_t)k * 3];
+            // cholesky_solve (MATRIX.f90:79-111)
+            const double y1 = b[0] / l11;
+            const double y2 = (b[1] - l21 * y1) / l22;
+            const double y3 = ((b[2] - l31 * y1) - l32 * y2) / l33;
+            const double x3 = y3 / l33;
+            const double x2 = (y2 - l32 * x3) / l22;
+            const double x1 = ((y1 - l21 * x2) - l31 * x3) / l11;
+            at3(o_sw, k, 0, i) = (R)x1;
+            at3(o_sw, k, 1, i) = (R)x2;
+            at3(o_sw, k, 2, i) = (R)x3;
+        }
         const double h3 = dim == 3 ? c->vol[i] : std::pow(c->vol[i], 1.5);
         h[o_eps + i] = (R)(K * K * K * h3);
     }
-    R* d = nullptr;
-    int rc = dalloc(c, &d, tot);
+    R* dp = nullptr;
+    int rc = dalloc(c, &dp, tot);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpy(d, h.data(), tot * sizeof(R), hipMemcpyHostToDevice));
-    g.fnx = d + o_fn;
-    g.fny = d + o_fn + nf;
-    g.fnz = d + o_fn + 2 * (size_t)nf;
-    g.fs = d + o_fs;
-    g.fcx = d + o_fc;
-    g.fcy = d + o_fc + nf;
-    g.fcz = d + o_fc + 2 * (size_t)nf;
-    g.ccx = d + o_cc;
-    g.ccy = d + o_cc + n;
-    g.ccz = d + o_cc + 2 * (size_t)n;
-    g.gcx = d + o_gc;
-    g.gcy = d + o_gc + nb;
-    g.gcz = d + o_gc + 2 * (size_t)nb;
-    g.lsq = d + o_lsq;
-    g.eps2 = d + o_eps;
-    g.psx = d + o_sh;
-    g.psy = d + o_sh + np;
-    g.psz = d + o_sh + 2 * (size_t)np;
-    g.per_base = c->per_base;
+    HIPCHK(c, hipMemcpy(dp, h.data(), tot * sizeof(R), hipMemcpyHostToDevice));
+    g.sn = dp + o_sn;
+    g.sa = dp + o_sa;
+    g.sw = dp + o_sw;
+    g.sr = dp + o_sr;
+    g.sd = dp + o_sd;
+    g.fnx = dp + o_fn;
+    g.fny = dp + o_fn + nf;
+    g.fnz = dp + o_fn + 2 * (size_t)nf;
+    g.fs = dp + o_fs;
+    g.eps2 = dp + o_eps;
     g.bboco = c->d_bboco;
     return TY_OK;
 }
@@ -754,8 +764,8 @@ int ty_state_set(ty_ctx* c, const double* rho, const double* rhoE, const double*
     }
     HIPCHK(c, hipMemcpyAsync(c->d_u0, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     c->cur = 0;
-    hipError_t e = c->mixed ? launch_cast<float>(c->d_u0, (float*)c->d_us[0], (long)nv * n, c->stream)
-                            : launch_cast<double>(c->d_u0, (double*)c->d_us[0], (long)nv * n, c->stream);
+    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[0], nv, n, c->stream)
+                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], nv, n, c->stream);
     HIPCHK(c, e);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->state_ok = true;
@@ -1016,8 +1026,9 @@ int ty_timing(ty_ctx* c, int32_t enable, double* ms, int32_t* count)
     if (!c) return TY_ERR_ARG;
     hipSetDevice(c->device);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::vector<std::pair<hipEvent_t, hipEvent_t>>* lists[3] = {&c->ev_grad, &c->ev_flux, &c->ev_dt};
-    for (int k = 0; k < 3; ++k) {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>>* lists[5] = {&c->ev_grad, &c->ev_flux, &c->ev_dt, &c->ev_tan,
+                                                                &c->ev_adj};
+    for (int k = 0; k < 5; ++k) {
         double tot = 0.0;
         for (auto& e : *lists[k]) {
             float t = 0.f;
@@ -1144,7 +1155,7 @@ int gmres(ty_ctx* c, const double* rhs, const ty_krylov* k, double unorm, int* i
             return TY_OK;
         }
         const double eps = sq * (1.0 + unorm) / zn;
-        HIPCHK(c, (fill_state<P, P>(pert, c->d_u0, eps, zin, len, c->stream)));
+        HIPCHK(c, (fill_state<P, P>(pert, c->d_u0, eps, zin, c->nv, c->n, c->stream)));
         HIPCHK(c, eval_residual<P>(c, pert, c->k_r1, nullptr, nullptr));
         HIPCHK(c, apply_a<P>(wout, zin, c->k_r1, c->k_r0, 1.0 / eps, len, c->stream));
         return TY_OK;
@@ -1160,7 +1171,7 @@ int gmres(ty_ctx* c, const double* rhs, const ty_krylov* k, double unorm, int* i
             if ((rc = dot_host<double>(c, c->k_dx, c->k_dx, len, &dd))) return rc;
             const double dn = std::sqrt(dd / (double)len);
             const double eps = sq * (1.0 + unorm) / std::max(dn, 1e-300);
-            HIPCHK(c, (fill_state<P, double>(pert, c->d_u0, eps, c->k_dx, len, c->stream)));
+            HIPCHK(c, (fill_state<P, double>(pert, c->d_u0, eps, c->k_dx, c->nv, c->n, c->stream)));
             HIPCHK(c, eval_residual<P>(c, pert, c->k_r1, nullptr, nullptr));
             HIPCHK(c, outer_residual(c->k_r2, rhs, c->k_dx, c->k_r1, c->k_r0, 1.0 / eps, len, c->stream));
             if ((rc = dot_host<double>(c, c->k_r2, c->k_r2, len, &bb))) return rc;
@@ -1249,10 +1260,10 @@ int solve_dev(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* st)
     const double bnorm = std::sqrt(bb);
     // inner-precision residual at U (base of the one-sided JVP) and the preconditioner
     if (prec) {
-        HIPCHK(c, (fill_state<float, double>(c->k_pf, c->d_u0, 0.0, c->d_u0, len, c->stream)));
+        HIPCHK(c, (fill_state<float, double>(c->k_pf, c->d_u0, 0.0, c->d_u0, c->nv, c->n, c->stream)));
         HIPCHK(c, eval_residual<float>(c, c->k_pf, c->k_r0, k->precond ? (float*)c->k_dinv : nullptr, nullptr));
     } else {
-        HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, 0.0, c->d_u0, len, c->stream)));
+        HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, 0.0, c->d_u0, c->nv, c->n, c->stream)));
         HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r0, k->precond ? (double*)c->k_dinv : nullptr, nullptr));
     }
     HIPCHK(c, hipMemsetAsync(c->k_x, 0, len * sizeof(double), c->stream));
@@ -1270,9 +1281,9 @@ int solve_dev(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* st)
             if ((rc = dot_host<double>(c, c->k_x, c->k_x, len, &xx))) return rc;
             const double xn = std::sqrt(xx / (double)len);
             const double eps = cb * (1.0 + unorm) / std::max(xn, 1e-300);
-            HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, eps, c->k_x, len, c->stream)));
+            HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, eps, c->k_x, c->nv, c->n, c->stream)));
             HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r1, nullptr, nullptr));
-            HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, -eps, c->k_x, len, c->stream)));
+            HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, -eps, c->k_x, c->nv, c->n, c->stream)));
             HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r2, nullptr, nullptr));
             HIPCHK(c, outer_residual(c->k_r, c->k_b, c->k_x, c->k_r1, c->k_r2, 0.5 / eps, len, c->stream));
         }
@@ -1325,12 +1336,16 @@ int ty_jacobian_diag(ty_ctx* c, double dt, double* diag)
     const int n = c->n, nv = c->nv;
     double* d_diag = nullptr;
     double* d_inv = nullptr;
+    double* d_st = nullptr;
     int rc;
-    if ((rc = dalloc(c, &d_diag, (size_t)nv * nv * n)) || (rc = dalloc(c, &d_inv, (size_t)nv * nv * n))) return rc;
+    if ((rc = dalloc(c, &d_diag, (size_t)nv * nv * n)) || (rc = dalloc(c, &d_inv, (size_t)nv * nv * n)) ||
+        (rc = dalloc(c, &d_st, (size_t)nv * n)))
+        return rc;
     c->h_dt = dt;
     HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
     if ((rc = ensure_work(c, 0))) return rc;
-    HIPCHK(c, eval_residual<double>(c, c->d_u0, c->d_res, d_inv, d_diag));
+    HIPCHK(c, launch_to_stage<double>(c->d_u0, d_st, nv, n, c->stream));
+    HIPCHK(c, eval_residual<double>(c, d_st, c->d_res, d_inv, d_diag));
     std::vector<double> h((size_t)nv * nv * n);
     HIPCHK(c, hipMemcpyAsync(h.data(), d_diag, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1342,7 +1357,7 @@ int ty_jacobian_diag(ty_ctx* c, double dt, double* diag)
             for (int b = 0; b < nv; ++b) out[a * 5 + b] = h[(size_t)(a * nv + b) * n + i];
     }
     // release the temporaries
-    for (double* p : {d_diag, d_inv}) {
+    for (double* p : {d_diag, d_inv, d_st}) {
         hipFree(p);
         c->owned.erase(std::find(c->owned.begin(), c->owned.end(), (void*)p));
     }
@@ -1367,9 +1382,9 @@ int ty_jvp(ty_ctx* c, double dt, const double* v, double* jv)
         return rc;
     const double unorm = std::sqrt(uu / (double)len), xn = std::sqrt(xx / (double)len);
     const double eps = std::cbrt(2.220446049250313e-16) * (1.0 + unorm) / std::max(xn, 1e-300);
-    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, eps, c->k_x, len, c->stream)));
+    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, eps, c->k_x, c->nv, c->n, c->stream)));
     HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r1, nullptr, nullptr));
-    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, -eps, c->k_x, len, c->stream)));
+    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, -eps, c->k_x, c->nv, c->n, c->stream)));
     HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r2, nullptr, nullptr));
     // Jv = (r1 - r2) / (2 eps): reuse outer_residual with b = 0 and x = 0 -> r = (r1 - r2) s
     HIPCHK(c, hipMemsetAsync(c->k_b, 0, len * sizeof(double), c->stream));
@@ -1412,14 +1427,14 @@ int ty_implicit_step(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* 
     c->h_dt = dt;
     HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
     // b = R(U) in FP64 (calc_rhs.f90:119-145)
-    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, 0.0, c->d_u0, len, c->stream)));
+    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, 0.0, c->d_u0, c->nv, c->n, c->stream)));
     HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_b, nullptr, nullptr));
     const int rs = solve_dev(c, dt, k, st);
     if (rs && rs != TY_ERR_NOT_CONVERGED) return rs;
     // update_field.f90:38: U += dU, then refresh the stage-state copy
     HIPCHK(c, accumulate<double>(c->d_u0, c->k_x, len, c->stream));
-    hipError_t e = c->mixed ? launch_cast<float>(c->d_u0, (float*)c->d_us[c->cur], len, c->stream)
-                            : launch_cast<double>(c->d_u0, (double*)c->d_us[c->cur], len, c->stream);
+    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[c->cur], c->nv, c->n, c->stream)
+                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[c->cur], c->nv, c->n, c->stream);
     HIPCHK(c, e);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (rs) return fail(c, rs, "implicit solve did not reach the requested tolerance");
@@ -1430,6 +1445,28 @@ int ty_implicit_step(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* 
 // tangent-linear and adjoint of the first-order residual
 
 namespace {
+
+// HIP events around a launch when timing is enabled
+struct EventScope {
+    ty_ctx* c;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>>& ev;
+    hipEvent_t e0 = nullptr;
+    EventScope(ty_ctx* cc, std::vector<std::pair<hipEvent_t, hipEvent_t>>& l) : c(cc), ev(l)
+    {
+        if (c->timing) {
+            hipEventCreate(&e0);
+            hipEventRecord(e0, c->stream);
+        }
+    }
+    ~EventScope()
+    {
+        if (!c->timing) return;
+        hipEvent_t e1;
+        hipEventCreate(&e1);
+        hipEventRecord(e1, c->stream);
+        ev.push_back({e0, e1});
+    }
+};
 
 // device temporaries released when the call returns
 struct DevTmp {
@@ -1514,7 +1551,10 @@ int ty_residual_tangent(ty_ctx* c, double dt, const double* du, const double* ds
     t.dvol = d_dv;
     t.dq = d_dq;
     t.out = d_out;
-    HIPCHK(c, c->dim == 2 ? launch_res_tangent<2>(t, c->stream) : launch_res_tangent<3>(t, c->stream));
+    {
+        EventScope ev(c, c->ev_tan);
+        HIPCHK(c, c->dim == 2 ? launch_res_tangent<2>(t, c->stream) : launch_res_tangent<3>(t, c->stream));
+    }
     h.assign((size_t)nv * n, 0.0);
     HIPCHK(c, hipMemcpyAsync(h.data(), d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1543,7 +1583,10 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
     t.gs = d_gs;
     t.gvol = d_gv;
     t.gq = d_gq;
-    HIPCHK(c, c->dim == 2 ? launch_res_adjoint<2>(t, c->stream) : launch_res_adjoint<3>(t, c->stream));
+    {
+        EventScope ev(c, c->ev_adj);
+        HIPCHK(c, c->dim == 2 ? launch_res_adjoint<2>(t, c->stream) : launch_res_adjoint<3>(t, c->stream));
+    }
     h.assign((size_t)nv * n, 0.0);
     HIPCHK(c, hipMemcpyAsync(h.data(), d_gu, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     std::vector<double> hs(gs ? (size_t)3 * nf : 0), hv(gvol ? (size_t)n : 0), hq(gq ? (size_t)nv * nb : 0);

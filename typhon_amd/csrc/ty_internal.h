@@ -1,4 +1,4 @@
-// Internal declarations shared by the HIP kernels (ty_kernels.hip) and the
+// Internal declarations shared by the HIP kernels (ty_kernels_*.hip) and the
 // host side of the C-ABI (ty_host.cpp).  gfx950 only.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -28,22 +28,22 @@ struct BocoDev {
                       // BK_OUTLET_SUB: param[0] = imposed static pressure
 };
 
-// Device geometry in the arithmetic precision R.  Face arrays are indexed by the
-// device face id, cell arrays by the RCM cell index, boundary arrays by the
-// boundary face index.  Normals point from the face's owner (the reference's
-// ic1 side) to the other side.
+// Device geometry in the arithmetic precision R.  The residual kernels read it
+// per face slot, [slot][component][cell], so every load of a wavefront is
+// coalesced and nothing geometric is gathered; the tangent/adjoint kernels read
+// the per-face arrays.  "Canonical" quantities are those of the face's owner
+// (the reference's ic1 side): the normal points owner -> other and
+// d = x_R - x_L is taken in the owner's frame (periodic far sides translated).
 template <typename R>
 struct Geo {
-    const R *fnx, *fny, *fnz, *fs;  // [nf]
-    const R *fcx, *fcy, *fcz;       // [nf] face centre in the owner's frame
-    const R *ccx, *ccy, *ccz;       // [n]
-    const R *gcx, *gcy, *gcz;       // [nb] ghost centroid (init_ustboco_ghost*.f90)
-    const int* bboco;               // [nb] boco index
-    const R* lsq;                   // [6][n] Cholesky factors l11 l21 l31 l22 l32 l33
+    const R* sn;    // [ns][3][n] canonical unit normal of the slot's face
+    const R* sa;    // [ns][n] face area
+    const R* sw;    // [ns][3][n] LSQ weights: grad q_c = sum_k w_k (q_k - q_c)
+    const R* sr;    // [ns][3][n] face centre - cell centre, in this cell's frame
+    const R* sd;    // [ns][3][n] canonical x_R - x_L (the ghost centre on boundary faces)
+    const R *fnx, *fny, *fnz, *fs;  // [nf] per device face: normal, area
+    const int* bboco;               // [nb] boco index of each boundary face
     const R* eps2;                  // [n] Venkatakrishnan (K h)^3
-    const R *psx, *psy, *psz;       // [nper] translation of the far side of periodic face
-                                    // per_base + k (face centre difference of the pair)
-    int per_base;
 };
 
 struct Physics {
@@ -73,14 +73,14 @@ struct KArgs {
     const double* dtfac;  // [n] 2 V / sum(A)
     Physics ph;
     // state
-    const R* us;          // [nv][n] stage state (conservative)
-    R* us_next;           // [nv][n]
+    const R* us;          // [n][nv] stage state (conservative), one record per cell
+    R* us_next;           // [n][nv]
     double* u0;           // [nv][n]
     double* acc;          // [nv][n]
     double* rstore;       // [nstore][nv][n]
-    const R* grad;        // [nv*dim][n] LSQ gradients (viscous flux)
+    const R* grad;        // [n][nv*dim] LSQ gradients (viscous flux), one record per cell
     R* grad_out;          // written by K1 when not null
-    const R* qf;          // [ns][nv][n] own-side face states (second order)
+    const R* qf;          // [ns][n][nv] own-side face states (second order), one record per slot
     R* qf_out;            // written by K1 when not null
     double* res_out;      // [nv][n] residual-only mode
     R* dinv_out;          // [nv*nv][n] inverse diagonal blocks of the implicit operator (residual mode)
@@ -111,8 +111,9 @@ template <int DIM, typename R>
 hipError_t launch_flux(const KArgs<R>& a, hipStream_t s);
 template <int DIM>
 hipError_t launch_timestep(const DtArgs& a, double dtmax, double* d_dt, int* d_err, hipStream_t s);
+// stage state records [n][nv] from a [nv][n] FP64 state
 template <typename R>
-hipError_t launch_cast(const double* src, R* dst, long n, hipStream_t s);
+hipError_t launch_to_stage(const double* src, R* dst, int nv, int n, hipStream_t s);
 
 int grid_blocks(int n);  // XCD-padded block count
 

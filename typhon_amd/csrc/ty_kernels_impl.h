@@ -1,12 +1,14 @@
 // HIP kernels of the residual path for gfx950 (MI355X).
 //
-// Layout: one work-item per interior cell (RCM order).  Each cell walks its
-// face slots (ELL, [slot][cell] so a wavefront's slot loads coalesce), gathers
-// the neighbour's state, and evaluates every face flux in the face's canonical
-// orientation (owner = the reference's ic1 side, geometry in the owner's frame),
-// so the two cells of a face compute bit-identical fluxes.  The residual is a
-// deterministic per-cell sum in slot order: no atomics, no face-flux array.
-// Fluxes and reconstruction follow the reference routines cited per function.
+// Layout: one work-item per interior cell (Hilbert or RCM order).  Each cell walks
+// its face slots (ELL, [slot][cell] so a wavefront's slot loads coalesce) and
+// evaluates every face flux in the face's canonical orientation (owner = the
+// reference's ic1 side, geometry in the owner's frame), so the two cells of a
+// face compute bit-identical fluxes.  The residual is a deterministic per-cell
+// sum in slot order: no atomics, no face-flux array.  Geometry is streamed per
+// slot; the only gathers are whole records (a neighbour's state, the far side's
+// face state), each one or two cache lines.  Fluxes and reconstruction follow the
+// reference routines cited per function.
 #pragma once
 #include "ty_internal.h"
 
@@ -32,14 +34,16 @@ __device__ __forceinline__ int xcd_block()
 }
 
 template <int DIM, typename R>
-__device__ __forceinline__ void load_prim(const R* __restrict__ us, int n, int c, R g1, R* q)
+__device__ __forceinline__ void load_prim(const R* __restrict__ us, int c, R g1, R* q)
 {
-    // calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:47-52)
-    const R rho = us[c];
-    const R rhoE = us[n + c];
+    // calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:47-52) on the cell's record
+    constexpr int NV = DIM + 2;
+    const R* u = us + (size_t)c * NV;
+    const R rho = u[0];
+    const R rhoE = u[1];
     R vel[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) vel[d] = us[(size_t)(2 + d) * n + c] / rho;
+    for (int d = 0; d < DIM; ++d) vel[d] = u[2 + d] / rho;
     R sq = vel[0] * vel[0];
 #pragma unroll
     for (int d = 1; d < DIM; ++d) sq += vel[d] * vel[d];
@@ -347,38 +351,6 @@ __device__ __forceinline__ R muscl_side(char method, char lim, R q, const R* g, 
     return q + dotd<DIM>(gp, cf);
 }
 
-// Face states of one face in canonical orientation for the MUSCL family.
-// side: 0 left only, 1 right only, 2 both.
-template <int DIM, typename R>
-__device__ __forceinline__ void muscl_face(const Physics& ph, const R* qL, const R* qR, const R* gL, const R* gR,
-                                           const R* xL, const R* xR, const R* xf, R* qLf, R* qRf, int side)
-{
-    constexpr int NV = DIM + 2;
-    R u[DIM], LF[DIM], RF[DIM];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        u[d] = xR[d] - xL[d];
-        LF[d] = xf[d] - xL[d];
-        RF[d] = xf[d] - xR[d];
-    }
-    R dlr, kl = R(0), kr = R(0);
-    if (ph.method == 'F') {
-        kl = sqrt(dotd<DIM>(LF, LF));
-        kr = -sqrt(dotd<DIM>(RF, RF));
-        dlr = kl - kr;
-    } else {
-        dlr = sqrt(dotd<DIM>(u, u));
-    }
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) u[d] = u[d] / dlr;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        const R lr = (qR[v] - qL[v]) / dlr;
-        if (side != 1) qLf[v] = muscl_side<DIM>(ph.method, ph.limiter, qL[v], gL + v * DIM, u, lr, LF, kl);
-        if (side != 0) qRf[v] = muscl_side<DIM>(ph.method, ph.limiter, qR[v], gR + v * DIM, u, lr, RF, kr);
-    }
-}
-
 // Venkatakrishnan smooth limiter function (J. Comput. Phys. 118 (1995) 120-130)
 template <typename R>
 __device__ __forceinline__ R venkat_phi(R dmax, R dmin, R d2, R eps2)
@@ -392,7 +364,7 @@ __device__ __forceinline__ R venkat_phi(R dmax, R dmin, R d2, R eps2)
 }
 
 // ---------------------------------------------------------------------------
-// slot decoding and neighbour positions
+// slot decoding and per-slot geometry
 
 struct SlotInfo {
     int nb, f;
@@ -409,27 +381,12 @@ __device__ __forceinline__ SlotInfo decode(int w, int nb)
     return s;
 }
 
-// translation of the far side of a face (zero unless periodic)
+// one 3-vector of slot k of cell c from a [slot][3][cell] array
 template <int DIM, typename R>
-__device__ __forceinline__ void face_shift(const Geo<R>& g, const SlotInfo& s, R* T)
+__device__ __forceinline__ void slot_vec(const R* __restrict__ p, int k, int n, int c, R* x)
 {
-    if (s.per) {
-        const int k = s.f - g.per_base;
-        T[0] = g.psx[k];
-        T[1] = g.psy[k];
-        if (DIM == 3) T[DIM - 1] = g.psz[k];
-    } else {
 #pragma unroll
-        for (int e = 0; e < DIM; ++e) T[e] = R(0);
-    }
-}
-
-template <int DIM, typename R>
-__device__ __forceinline__ void cell_x(const Geo<R>& g, int c, R* x)
-{
-    x[0] = g.ccx[c];
-    x[1] = g.ccy[c];
-    if (DIM == 3) x[DIM - 1] = g.ccz[c];
+    for (int e = 0; e < DIM; ++e) x[e] = p[((size_t)k * 3 + e) * n + c];
 }
 
 template <int DIM, typename R>
@@ -441,88 +398,48 @@ __device__ __forceinline__ void face_n(const Geo<R>& g, int f, R* n)
 }
 
 template <int DIM, typename R>
-__device__ __forceinline__ void face_x(const Geo<R>& g, int f, R* x)
+__device__ __forceinline__ void load_grad(const R* __restrict__ grad, int c, R* gv)
 {
-    x[0] = g.fcx[f];
-    x[1] = g.fcy[f];
-    if (DIM == 3) x[DIM - 1] = g.fcz[f];
+    constexpr int K = (DIM + 2) * DIM;
+    const R* g = grad + (size_t)c * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) gv[k] = g[k];
 }
 
+// The state across slot k: the neighbour's primitive state, or the ghost state of the
+// cell_state boundary map (calcboco_ust.f90:44-105) built from this cell's.
 template <int DIM, typename R>
-__device__ __forceinline__ void load_grad(const R* __restrict__ grad, int n, int c, R* gv)
+__device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int nb, const R* qc, R* qo)
 {
-#pragma unroll
-    for (int k = 0; k < (DIM + 2) * DIM; ++k) gv[k] = grad[(size_t)k * n + c];
-}
-
-// ---------------------------------------------------------------------------
-// Far side of one face slot, in the face's canonical orientation (left = owner,
-// the reference's ic1 side; right = the other cell, or the ghost cell built by
-// the cell_state boundary map; positions in the owner's frame).
-
-template <int DIM, typename R>
-struct Side {
-    R qL[DIM + 2], qR[DIM + 2];
-    R xL[DIM], xR[DIM];
-    bool boundary;
-};
-
-template <int DIM, typename R>
-__device__ __forceinline__ void side_ctx(const KArgs<R>& a, int c, const R* qc, const R* xc, const SlotInfo& s,
-                                         const R* nf, Side<DIM, R>& S)
-{
-    constexpr int NV = DIM + 2;
-    const R g1 = R(a.ph.gamma - 1.0);
-    if (s.nb >= 0) {
-        R qo[NV], xo[DIM], T[DIM];
-        load_prim<DIM>(a.us, a.n, s.nb, g1, qo);
-        cell_x<DIM>(a.g, s.nb, xo);
-        face_shift<DIM>(a.g, s, T);
-        if (!s.other) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) { S.qL[v] = qc[v]; S.qR[v] = qo[v]; }
-#pragma unroll
-            for (int e = 0; e < DIM; ++e) { S.xL[e] = xc[e]; S.xR[e] = xo[e] + T[e]; }
-        } else {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) { S.qL[v] = qo[v]; S.qR[v] = qc[v]; }
-#pragma unroll
-            for (int e = 0; e < DIM; ++e) { S.xL[e] = xo[e]; S.xR[e] = xc[e] + T[e]; }
-        }
-        S.boundary = false;
+    if (nb >= 0) {
+        load_prim<DIM>(a.us, nb, R(a.ph.gamma - 1.0), qo);
     } else {
-        const int bi = -s.nb - 1;
-        const BocoDev bc = a.boco[a.g.bboco[bi]];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) S.qL[v] = qc[v];
-        bc_state<DIM>(bc, nf, qc, S.qR, a.ph);
-#pragma unroll
-        for (int e = 0; e < DIM; ++e) S.xL[e] = xc[e];
-        S.xR[0] = a.g.gcx[bi];
-        S.xR[1] = a.g.gcy[bi];
-        if (DIM == 3) S.xR[DIM - 1] = a.g.gcz[bi];
-        S.boundary = true;
+        R nf[DIM];
+        slot_vec<DIM>(a.g.sn, k, a.n, c, nf);
+        bc_state<DIM>(a.boco[a.g.bboco[-nb - 1]], nf, qc, qo, a.ph);
     }
 }
 
-// One side's MUSCL-family face value from the canonical face data: this cell is
-// the left side when !other (hres_ns_muscl.f90:51-143 and its variants).
+// One side's MUSCL-family face value from the canonical face data (hres_ns_muscl.f90:51-143
+// and its variants): qL/qR are the canonical cell states, d = x_R - x_L, r = x_f - x_c in this
+// cell's frame, g this cell's gradient; this cell is the left side when !other.
 template <int DIM, typename R>
-__device__ __forceinline__ void muscl_own(const Physics& ph, const Side<DIM, R>& S, const R* xf, const R* g,
-                                          bool other, R* qf)
+__device__ __forceinline__ void muscl_own(const Physics& ph, const R* qL, const R* qR, const R* d, const R* r,
+                                          const R* g, bool other, R* qf)
 {
     constexpr int NV = DIM + 2;
-    R u[DIM], CF[DIM];
+    R u[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        u[d] = S.xR[d] - S.xL[d];
-        CF[d] = other ? xf[d] - S.xR[d] : xf[d] - S.xL[d];
-    }
+    for (int e = 0; e < DIM; ++e) u[e] = d[e];
     R dlr, kcf = R(0);
     if (ph.method == 'F') {
+        // x_f - x_L and x_f - x_R from this cell's r and the canonical d
         R LF[DIM], RF[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) { LF[d] = xf[d] - S.xL[d]; RF[d] = xf[d] - S.xR[d]; }
+        for (int e = 0; e < DIM; ++e) {
+            LF[e] = other ? r[e] + d[e] : r[e];
+            RF[e] = other ? r[e] : r[e] - d[e];
+        }
         const R kl = sqrt(dotd<DIM>(LF, LF));
         const R kr = -sqrt(dotd<DIM>(RF, RF));
         dlr = kl - kr;
@@ -531,22 +448,23 @@ __device__ __forceinline__ void muscl_own(const Physics& ph, const Side<DIM, R>&
         dlr = sqrt(dotd<DIM>(u, u));
     }
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) u[d] = u[d] / dlr;
-    const R* q = other ? S.qR : S.qL;
+    for (int e = 0; e < DIM; ++e) u[e] = u[e] / dlr;
+    const R* q = other ? qR : qL;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        const R lr = (S.qR[v] - S.qL[v]) / dlr;
-        qf[v] = muscl_side<DIM>(ph.method, ph.limiter, q[v], g + v * DIM, u, lr, CF, kcf);
+        const R lr = (qR[v] - qL[v]) / dlr;
+        qf[v] = muscl_side<DIM>(ph.method, ph.limiter, q[v], g + v * DIM, u, lr, r, kcf);
     }
 }
 
 // ---------------------------------------------------------------------------
-// K1: LSQ gradients (calc_gradient.f90:75-192 + cholesky_solve MATRIX.f90:79-111)
-// and this cell's own-side face states for every face slot: MUSCL / MUSCL-FAST /
-// MUSCL-UNS with the reference limiters, Barth post-limitation
-// (postlimit_barth.f90:50-223, ratios of rho, p and the velocity vector), or the
-// Venkatakrishnan cell limiter.  Neighbour set = face neighbours incl. ghost
-// cells from the cell_state boundary map.
+// K1: LSQ gradients (calc_gradient.f90:75-192; the normal-equation solve of
+// precalc_grad_lsq.f90:60-155 + cholesky_solve MATRIX.f90:79-111 is folded into
+// per-slot weights w_k = A^-1 d_k computed once on the host) and this cell's
+// own-side face states for every face slot: MUSCL / MUSCL-FAST / MUSCL-UNS with
+// the reference limiters, Barth post-limitation (postlimit_barth.f90:50-223,
+// ratios of rho, p and the velocity vector), or the Venkatakrishnan cell limiter.
+// Neighbour set = face neighbours incl. ghost cells from the cell_state boundary map.
 
 // reconstruction performed by K1 after the gradient
 enum Rec : int { REC_NONE = 0, REC_VENKAT = 1, REC_MUSCL = 2, REC_MUSCL_BARTH = 3 };
@@ -559,78 +477,40 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
     if (c >= a.n) return;
     const int n = a.n;
     const R g1 = R(a.ph.gamma - 1.0);
-    R qc[NV], xc[DIM];
-    load_prim<DIM>(a.us, n, c, g1, qc);
-    cell_x<DIM>(a.g, c, xc);
-    R b[NV][DIM];
+    R qc[NV];
+    load_prim<DIM>(a.us, c, g1, qc);
+    R gr[NV * DIM];
     R qmax[NV], qmin[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         qmax[v] = qc[v];
         qmin[v] = qc[v];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) b[v][d] = R(0);
+        for (int e = 0; e < DIM; ++e) gr[v * DIM + e] = R(0);
     }
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
         if (k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
-        const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
-        R nf[DIM];
-        if (s.nb < 0) face_n<DIM>(a.g, s.f, nf);
-        Side<DIM, R> S;
-        side_ctx<DIM>(a, c, qc, xc, s, nf, S);
-        // owner frame: d = x_R - x_L, dq = q_R - q_L (precalc_grad_lsq.f90:62-73)
-        R d[DIM], dq[NV];
-#pragma unroll
-        for (int e = 0; e < DIM; ++e) d[e] = S.xR[e] - S.xL[e];
+        R qo[NV], wk[DIM];
+        far_state<DIM>(a, k, c, a.nbr[(size_t)k * n + c], qc, qo);
+        slot_vec<DIM>(a.g.sw, k, n, c, wk);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            dq[v] = S.qR[v] - S.qL[v];
-            const R qo = s.other ? S.qL[v] : S.qR[v];
-            qmax[v] = fmax_(qmax[v], qo);
-            qmin[v] = fmin_(qmin[v], qo);
-        }
-        if (a.ph.lsqw) {
-            const R s2 = dotd<DIM>(d, d);
-            const R s4 = s2 * s2;
+            const R dq = qo[v] - qc[v];
 #pragma unroll
-            for (int e = 0; e < DIM; ++e) d[e] = d[e] / s4;
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-#pragma unroll
-            for (int e = 0; e < DIM; ++e) b[v][e] += dq[v] * d[e];
-    }
-    // Cholesky solve with the factors of A (precalc_grad_lsq.f90:60-155)
-    const R l11 = a.g.lsq[c], l21 = a.g.lsq[(size_t)n + c], l22 = a.g.lsq[(size_t)3 * n + c];
-    R gr[NV * DIM];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        if (DIM == 3) {
-            const R l31 = a.g.lsq[(size_t)2 * n + c], l32 = a.g.lsq[(size_t)4 * n + c], l33 = a.g.lsq[(size_t)5 * n + c];
-            const R y1 = b[v][0] / l11;
-            const R y2 = (b[v][1] - l21 * y1) / l22;
-            const R y3 = ((b[v][DIM - 1] - l31 * y1) - l32 * y2) / l33;
-            const R x3 = y3 / l33;
-            const R x2 = (y2 - l32 * x3) / l22;
-            const R x1 = ((y1 - l21 * x2) - l31 * x3) / l11;
-            gr[v * DIM + 0] = x1;
-            gr[v * DIM + 1] = x2;
-            gr[v * DIM + DIM - 1] = x3;
-        } else {
-            const R y1 = b[v][0] / l11;
-            const R y2 = (b[v][1] - l21 * y1) / l22;
-            const R x2 = y2 / l22;
-            const R x1 = (y1 - l21 * x2) / l11;
-            gr[v * DIM + 0] = x1;
-            gr[v * DIM + 1] = x2;
+            for (int e = 0; e < DIM; ++e) gr[v * DIM + e] += wk[e] * dq;
+            if (REC == REC_VENKAT) {
+                qmax[v] = fmax_(qmax[v], qo[v]);
+                qmin[v] = fmin_(qmin[v], qo[v]);
+            }
         }
     }
     if (a.grad_out) {
+        R* go = a.grad_out + (size_t)c * NV * DIM;
 #pragma unroll
-        for (int k = 0; k < NV * DIM; ++k) a.grad_out[(size_t)k * n + c] = gr[k];
+        for (int k = 0; k < NV * DIM; ++k) go[k] = gr[k];
     }
     if (REC == REC_NONE) return;
     if (REC == REC_VENKAT) {
@@ -643,14 +523,9 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 #pragma unroll
             for (int k = 0; k < NSM; ++k) {
                 if (k >= a.ns) break;
-                const int w = a.fw[(size_t)k * n + c];
-                if (w == SLOT_UNUSED) continue;
-                const SlotInfo s = decode(w, 0);
-                R xf[DIM], T[DIM], rk[DIM];
-                face_x<DIM>(a.g, s.f, xf);
-                face_shift<DIM>(a.g, s, T);
-#pragma unroll
-                for (int e = 0; e < DIM; ++e) rk[e] = (s.other ? xf[e] - T[e] : xf[e]) - xc[e];
+                if (a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
+                R rk[DIM];
+                slot_vec<DIM>(a.g.sr, k, n, c, rk);
                 if (pass == 0) {
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
@@ -658,9 +533,9 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
                         phi[v] = fmin_(phi[v], venkat_phi(qmax[v] - qc[v], qmin[v] - qc[v], d2, eps2));
                     }
                 } else {
+                    R* out = a.qf_out + ((size_t)k * n + c) * NV;
 #pragma unroll
-                    for (int v = 0; v < NV; ++v)
-                        a.qf_out[((size_t)k * NV + v) * n + c] = qc[v] + phi[v] * dotd<DIM>(gr + v * DIM, rk);
+                    for (int v = 0; v < NV; ++v) out[v] = qc[v] + phi[v] * dotd<DIM>(gr + v * DIM, rk);
                 }
             }
         }
@@ -676,16 +551,14 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
         if (k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
-        const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
-        R nf[DIM], xf[DIM];
-        face_x<DIM>(a.g, s.f, xf);
-        if (s.nb < 0) face_n<DIM>(a.g, s.f, nf);
-        Side<DIM, R> S;
-        side_ctx<DIM>(a, c, qc, xc, s, nf, S);
+        const bool other = w < 0;
+        R qo[NV], dk[DIM], rk[DIM];
+        far_state<DIM>(a, k, c, a.nbr[(size_t)k * n + c], qc, qo);
+        slot_vec<DIM>(a.g.sd, k, n, c, dk);
+        slot_vec<DIM>(a.g.sr, k, n, c, rk);
         R* q = qfk[barth ? k : 0];
-        muscl_own<DIM>(a.ph, S, xf, gr, s.other, q);
+        muscl_own<DIM>(a.ph, other ? qo : qc, other ? qc : qo, dk, rk, gr, other, q);
         if (barth) {
-            const R* qo = s.other ? S.qL : S.qR;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const R l = (qo[v] - qc[v]) / (q[v] - qc[v]);
@@ -694,8 +567,9 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
                 ratio[gi] = fmin_(ratio[gi], lr);
             }
         } else {
+            R* out = a.qf_out + ((size_t)k * n + c) * NV;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) a.qf_out[((size_t)k * NV + v) * n + c] = q[v];
+            for (int v = 0; v < NV; ++v) out[v] = q[v];
         }
     }
     if (!barth) return;
@@ -705,36 +579,35 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
     for (int k = 0; k < NSM; ++k) {
         if (k >= a.ns) break;
         if (a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
+        R* out = a.qf_out + ((size_t)k * n + c) * NV;
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
-            a.qf_out[((size_t)k * NV + v) * n + c] = qc[v] + ratio[v < 2 ? v : 2] * (qfk[barth ? k : 0][v] - qc[v]);
+        for (int v = 0; v < NV; ++v) out[v] = qc[v] + ratio[v < 2 ? v : 2] * (qfk[barth ? k : 0][v] - qc[v]);
     }
 }
 
-// viscous flux of one face, calc_flux_viscous_cell.f90:75-158 (subtracted from F);
-// S carries the cell states and positions, gL/gR the cell gradients, xf/nv the face.
+// ---------------------------------------------------------------------------
+// viscous flux of one face, calc_flux_viscous_cell.f90:75-158 (subtracted from F):
+// qL/qR the canonical cell states, rL = x_f - x_L, rR = x_f - x_R, vLR = x_R - x_L,
+// gL/gR the cell gradients, nf the canonical normal.
 template <int DIM, typename R>
-__device__ void viscous_face(const Physics& ph, const Side<DIM, R>& F, const R* xf, const R* nf, const R* gL,
-                             const R* gR, R* flux)
+__device__ void viscous_face(const Physics& ph, const R* qL, const R* qR, const R* rL, const R* rR, const R* vLR,
+                             const R* nf, const R* gL, const R* gR, R* flux)
 {
     const R r = R(ph.rgas);
     const R cp = R(ph.gamma) * r / (R(ph.gamma) - R(1));
-    R a[DIM], b[DIM], vLR[DIM];
-#pragma unroll
-    for (int e = 0; e < DIM; ++e) { a[e] = xf[e] - F.xL[e]; b[e] = xf[e] - F.xR[e]; vLR[e] = F.xR[e] - F.xL[e]; }
-    R dHL = sqrt(dotd<DIM>(a, a)), dHR = sqrt(dotd<DIM>(b, b));
+    R dHL = sqrt(dotd<DIM>(rL, rL)), dHR = sqrt(dotd<DIM>(rR, rR));
     const R id = R(1) / (dHL + dHR);
     dHL = id * dHL;
     dHR = id * dHR;
     R velH[DIM];
 #pragma unroll
-    for (int e = 0; e < DIM; ++e) velH[e] = dHR * F.qL[2 + e] + dHL * F.qR[2 + e];
-    const R TL = F.qL[1] / (F.qL[0] * r), TR = F.qR[1] / (F.qR[0] * r);
+    for (int e = 0; e < DIM; ++e) velH[e] = dHR * qL[2 + e] + dHL * qR[2 + e];
+    const R TL = qL[1] / (qL[0] * r), TR = qR[1] / (qR[0] * r);
     const R TH = dHR * TL + dHL * TR;
-    const R rhoH = dHR * F.qL[0] + dHL * F.qR[0];
+    const R rhoH = dHR * qL[0] + dHL * qR[0];
     R gTL[DIM], gTR[DIM];
-    const R kl1 = R(1) / (F.qL[0] * r), kl2 = F.qL[1] / (F.qL[0] * F.qL[0] * r);
-    const R kr1 = R(1) / (F.qR[0] * r), kr2 = F.qR[1] / (F.qR[0] * F.qR[0] * r);
+    const R kl1 = R(1) / (qL[0] * r), kl2 = qL[1] / (qL[0] * qL[0] * r);
+    const R kr1 = R(1) / (qR[0] * r), kr2 = qR[1] / (qR[0] * qR[0] * r);
 #pragma unroll
     for (int e = 0; e < DIM; ++e) {
         gTL[e] = kl1 * gL[DIM + e] - kl2 * gL[e];
@@ -760,7 +633,7 @@ __device__ void viscous_face(const Physics& ph, const Side<DIM, R>& F, const R* 
         for (int e = 0; e < DIM; ++e) kv[e] = (R(1) / dLR2) * vLR[e];
 #pragma unroll
         for (int i = 0; i < DIM; ++i) {
-            const R fa = F.qR[2 + i] - F.qL[2 + i] - dotd<DIM>(G[i], vLR);
+            const R fa = qR[2 + i] - qL[2 + i] - dotd<DIM>(G[i], vLR);
 #pragma unroll
             for (int j = 0; j < DIM; ++j) G[i][j] = G[i][j] + fa * kv[j];
         }
@@ -855,24 +728,22 @@ __device__ __forceinline__ void jac_hll(const Physics& ph, const R* qL, const R*
 
 // viscous diagonal of the face Jacobians (calc_flux_viscous_cell.f90:149-158)
 template <int DIM, typename R>
-__device__ __forceinline__ R visc_jac_diag(const Physics& ph, const Side<DIM, R>& F, const R* xf)
+__device__ __forceinline__ R visc_jac_diag(const Physics& ph, const R* qL, const R* qR, const R* rL, const R* rR,
+                                           const R* d)
 {
     const R r = R(ph.rgas);
-    R a[DIM], b[DIM], d[DIM];
-#pragma unroll
-    for (int e = 0; e < DIM; ++e) { a[e] = xf[e] - F.xL[e]; b[e] = xf[e] - F.xR[e]; d[e] = F.xR[e] - F.xL[e]; }
-    R dHL = sqrt(dotd<DIM>(a, a)), dHR = sqrt(dotd<DIM>(b, b));
+    R dHL = sqrt(dotd<DIM>(rL, rL)), dHR = sqrt(dotd<DIM>(rR, rR));
     const R id0 = R(1) / (dHL + dHR);
     dHL = id0 * dHL;
     dHR = id0 * dHR;
-    const R TL = F.qL[1] / (F.qL[0] * r), TR = F.qR[1] / (F.qR[0] * r);
+    const R TL = qL[1] / (qL[0] * r), TR = qR[1] / (qR[0] * r);
     const R TH = dHR * TL + dHL * TR;
-    const R rhoH = dHR * F.qL[0] + dHL * F.qR[0];
+    const R rhoH = dHR * qL[0] + dHL * qR[0];
     R mu;
     if (ph.visc_law == 11) mu = R(ph.visc_dyn) * dpow(TH / R(ph.tref), R(1.5)) * (R(ph.tref) + R(ph.tsuth)) / (TH + R(ph.tsuth));
     else if (ph.visc_law == 14) mu = R(ph.visc_kin) * rhoH;
     else mu = R(ph.visc_dyn);
-    return mu / (dHR * F.qL[0] + dHL * F.qR[0]) / sqrt(dotd<DIM>(d, d));
+    return mu / (dHR * qL[0] + dHL * qR[0]) / sqrt(dotd<DIM>(d, d));
 }
 
 // in-register Gauss-Jordan inverse with partial pivoting (FP64)
@@ -930,9 +801,8 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
     const int n = a.n;
     const R g1 = R(a.ph.gamma - 1.0);
     const R gam = R(a.ph.gamma);
-    R qc[NV], xc[DIM];
-    if (!FS || VISC) load_prim<DIM>(a.us, n, c, g1, qc);
-    if (VISC || JAC) cell_x<DIM>(a.g, c, xc);
+    R qc[NV];
+    if (!FS || VISC) load_prim<DIM>(a.us, c, g1, qc);
     double D[JAC ? NV * NV : 1];
     const double kdt = JAC ? *a.d_dt / a.vol[c] : 0.0;
     if (JAC) {
@@ -940,7 +810,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
         for (int k = 0; k < NV * NV; ++k) D[k] = (k % (NV + 1) == 0) ? 1.0 : 0.0;
     }
     R gcell[VISC ? NV * DIM : 1];
-    if (VISC) load_grad<DIM>(a.grad, n, c, gcell);
+    if (VISC) load_grad<DIM>(a.grad, c, gcell);
     R res[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) res[v] = R(0);
@@ -949,83 +819,75 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
         if (k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
-        const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
+        const int nb = a.nbr[(size_t)k * n + c];
+        const bool other = w < 0;
         R nf[DIM];
-        face_n<DIM>(a.g, s.f, nf);
+        slot_vec<DIM>(a.g.sn, k, n, c, nf);
+        const R A = a.g.sa[(size_t)k * n + c];
+        // the far cell's state (first order, viscous flux)
+        R qo[NV];
+        if (!FS || VISC) far_state<DIM>(a, k, c, nb, qc, qo);
         R qLf[NV], qRf[NV];
         if (FS) {
-            R own[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) own[v] = a.qf[((size_t)k * NV + v) * n + c];
-            if (s.nb >= 0) {
+            const R* own = a.qf + ((size_t)k * n + c) * NV;
+            if (nb >= 0) {
                 const int mk = ((unsigned)w >> MIRROR_SHIFT) & (MAX_SLOTS - 1);
-                R far[NV];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) far[v] = a.qf[((size_t)mk * NV + v) * n + s.nb];
+                const R* far = a.qf + ((size_t)mk * n + nb) * NV;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    qLf[v] = s.other ? far[v] : own[v];
-                    qRf[v] = s.other ? own[v] : far[v];
+                    const R ov = own[v], fv = far[v];
+                    qLf[v] = other ? fv : ov;
+                    qRf[v] = other ? ov : fv;
                 }
             } else {
-                const BocoDev bc = a.boco[a.g.bboco[-s.nb - 1]];
 #pragma unroll
                 for (int v = 0; v < NV; ++v) qLf[v] = own[v];
-                bc_state<DIM>(bc, nf, qLf, qRf, a.ph);
+                bc_state<DIM>(a.boco[a.g.bboco[-nb - 1]], nf, qLf, qRf, a.ph);
             }
         } else {
-            if (s.nb >= 0) {
-                R qo[NV];
-                load_prim<DIM>(a.us, n, s.nb, g1, qo);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    qLf[v] = s.other ? qo[v] : qc[v];
-                    qRf[v] = s.other ? qc[v] : qo[v];
-                }
-            } else {
-                const BocoDev bc = a.boco[a.g.bboco[-s.nb - 1]];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) qLf[v] = qc[v];
-                bc_state<DIM>(bc, nf, qLf, qRf, a.ph);
+            for (int v = 0; v < NV; ++v) {
+                qLf[v] = other ? qo[v] : qc[v];
+                qRf[v] = other ? qc[v] : qo[v];
             }
         }
         R flux[NV];
         num_flux<DIM>(a.ph.sch_hyp, gam, qLf, qRf, nf, flux);
+        R rL[DIM], rR[DIM], dk[DIM];
         if (VISC) {
-            Side<DIM, R> S;
-            side_ctx<DIM>(a, c, qc, xc, s, nf, S);
-            R xf[DIM], go[NV * DIM];
-            face_x<DIM>(a.g, s.f, xf);
-            if (s.nb >= 0) load_grad<DIM>(a.grad, n, s.nb, go);
-            else bc_grad<DIM>(a.boco[a.g.bboco[-s.nb - 1]], nf, gcell, go);
-            const R* gL = s.other ? go : gcell;
-            const R* gR = s.other ? gcell : go;
-            viscous_face<DIM>(a.ph, S, xf, nf, gL, gR, flux);
+            R rk[DIM];
+            slot_vec<DIM>(a.g.sd, k, n, c, dk);
+            slot_vec<DIM>(a.g.sr, k, n, c, rk);
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) {
+                rL[e] = other ? rk[e] + dk[e] : rk[e];
+                rR[e] = other ? rk[e] : rk[e] - dk[e];
+            }
+            R go[NV * DIM];
+            if (nb >= 0) load_grad<DIM>(a.grad, nb, go);
+            else bc_grad<DIM>(a.boco[a.g.bboco[-nb - 1]], nf, gcell, go);
+            viscous_face<DIM>(a.ph, other ? qo : qc, other ? qc : qo, rL, rR, dk, nf, other ? go : gcell,
+                              other ? gcell : go, flux);
         }
-        const R A = a.g.fs[s.f];
         if (JAC) {
             // build_implicit_bdlu.f90:44-83: diag(ic1) += jacL.A dt/V, diag(ic2) -= jacR.A dt/V
             R jL[NV * NV], jR[NV * NV];
             jac_hll<DIM>(a.ph, qLf, qRf, nf, jL, jR);
             if (VISC) {
-                Side<DIM, R> S;
-                side_ctx<DIM>(a, c, qc, xc, s, nf, S);
-                R xf[DIM];
-                face_x<DIM>(a.g, s.f, xf);
-                const R id = visc_jac_diag<DIM>(a.ph, S, xf);
+                const R id = visc_jac_diag<DIM>(a.ph, other ? qo : qc, other ? qc : qo, rL, rR, dk);
 #pragma unroll
                 for (int i = 0; i < NV; ++i) { jR[i * NV + i] -= id; jL[i * NV + i] += id; }
             }
-            if (s.other) {
+            if (other) {
 #pragma unroll
-                for (int k = 0; k < NV * NV; ++k) D[k] -= (double)(A * jR[k]) * kdt;
+                for (int k2 = 0; k2 < NV * NV; ++k2) D[k2] -= (double)(A * jR[k2]) * kdt;
             } else {
 #pragma unroll
-                for (int k = 0; k < NV * NV; ++k) D[k] += (double)(A * jL[k]) * kdt;
+                for (int k2 = 0; k2 < NV * NV; ++k2) D[k2] += (double)(A * jL[k2]) * kdt;
             }
         }
         // flux_to_res.f90:90-113: R(ic1) -= F.A, R(ic2) += F.A
-        if (s.other) {
+        if (other) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) res[v] = res[v] + A * flux[v];
         } else {
@@ -1068,6 +930,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
 #pragma unroll
         for (int v = 0; v < NV; ++v) rs[(size_t)v * n + c] = Rv[v];
     }
+    R* un = a.us_next + (size_t)c * NV;
     if (a.stage < a.nstage) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) a.acc[(size_t)v * n + c] = acc[v];
@@ -1079,7 +942,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
                 if (sl >= 0) u = u + a.a[j] * a.rstore[((size_t)sl * NV + v) * n + c];
             }
             u = u + a.a[a.stage - 1] * Rv[v];
-            a.us_next[(size_t)v * n + c] = (R)u;
+            un[v] = (R)u;
         }
     } else {
         // update_field.f90:38: U = U0 + residu
@@ -1087,7 +950,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
         for (int v = 0; v < NV; ++v) {
             const double u = a.u0[(size_t)v * n + c] + acc[v];
             a.u0[(size_t)v * n + c] = u;
-            a.us_next[(size_t)v * n + c] = (R)u;
+            un[v] = (R)u;
         }
     }
 }
@@ -1170,11 +1033,13 @@ static __global__ __launch_bounds__(1024) void k_dt_final(const double* part, co
     }
 }
 
+// stage-state records from a [nv][n] FP64 state
 template <typename R>
-__global__ void k_cast(const double* __restrict__ src, R* __restrict__ dst, long n)
+__global__ void k_to_stage(const double* __restrict__ src, R* __restrict__ dst, int nv, int n)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (R)src[i];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int v = 0; v < nv; ++v) dst[(size_t)c * nv + v] = (R)src[(size_t)v * n + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -1234,10 +1099,10 @@ hipError_t launch_timestep(const DtArgs& a, double dtmax, double* d_dt, int* d_e
     return hipGetLastError();
 }
 template <typename R>
-hipError_t launch_cast(const double* src, R* dst, long n, hipStream_t s)
+hipError_t launch_to_stage(const double* src, R* dst, int nv, int n, hipStream_t s)
 {
-    const int nb = (int)((n + 255) / 256);
-    hipLaunchKernelGGL((k_cast<R>), dim3(nb), dim3(256), 0, s, src, dst, n);
+    const int nb = (n + 255) / 256;
+    hipLaunchKernelGGL((k_to_stage<R>), dim3(nb), dim3(256), 0, s, src, dst, nv, n);
     return hipGetLastError();
 }
 

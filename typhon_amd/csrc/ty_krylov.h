@@ -7,8 +7,9 @@ namespace ty {
 constexpr int MAXK = 32;  // Krylov basis size limit (GMRES restart length + 1)
 
 int multidot_blocks(long len);
+// stage-state records dst[c][v] = (P)(u[v][c] + a v[v][c]) for the residual kernels
 template <typename P, typename T>
-hipError_t fill_state(P* dst, const double* u, double a, const T* v, long len, hipStream_t s);
+hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, hipStream_t s);
 template <typename T>
 hipError_t apply_a(T* w, const T* z, const double* r1, const double* r0, double sc, long len, hipStream_t s);
 template <typename T>

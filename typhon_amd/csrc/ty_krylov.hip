@@ -16,13 +16,17 @@ __device__ __forceinline__ double wsum(double v)
     return v;
 }
 
-// dst = (P)(u + a * v)
+// dst[c][k] = (P)(u[k][c] + a * v[k][c]): a perturbed state as stage-state records
 template <typename P, typename T>
 __global__ void k_fill_state(P* __restrict__ dst, const double* __restrict__ u, double a, const T* __restrict__ v,
-                             long len)
+                             int nv, int n)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < len) dst[i] = (P)(u[i] + a * (double)v[i]);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int k = 0; k < nv; ++k) {
+        const size_t i = (size_t)k * n + c;
+        dst[(size_t)c * nv + k] = (P)(u[i] + a * (double)v[i]);
+    }
 }
 
 // w = z - (r1 - r0) * s     (A z = z - J z with J z from a one-sided difference)
@@ -145,9 +149,9 @@ int multidot_blocks(long len)
 }
 
 template <typename P, typename T>
-hipError_t fill_state(P* dst, const double* u, double a, const T* v, long len, hipStream_t s)
+hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_fill_state<P, T>), dim3(nb_of(len)), dim3(KB), 0, s, dst, u, a, v, len);
+    hipLaunchKernelGGL((k_fill_state<P, T>), dim3(nb_of(n)), dim3(KB), 0, s, dst, u, a, v, nv, n);
     return hipGetLastError();
 }
 template <typename T>
@@ -212,10 +216,10 @@ hipError_t outer_residual(double* r, const double* b, const double* x, const dou
     template hipError_t accumulate<T>(double*, const T*, long, hipStream_t);
 TY_KINST(float)
 TY_KINST(double)
-template hipError_t fill_state<float, float>(float*, const double*, double, const float*, long, hipStream_t);
-template hipError_t fill_state<float, double>(float*, const double*, double, const double*, long, hipStream_t);
-template hipError_t fill_state<double, double>(double*, const double*, double, const double*, long, hipStream_t);
-template hipError_t fill_state<double, float>(double*, const double*, double, const float*, long, hipStream_t);
+template hipError_t fill_state<float, float>(float*, const double*, double, const float*, int, int, hipStream_t);
+template hipError_t fill_state<float, double>(float*, const double*, double, const double*, int, int, hipStream_t);
+template hipError_t fill_state<double, double>(double*, const double*, double, const double*, int, int, hipStream_t);
+template hipError_t fill_state<double, float>(double*, const double*, double, const float*, int, int, hipStream_t);
 template hipError_t scale<float, float>(float*, const float*, double, long, hipStream_t);
 template hipError_t scale<float, double>(float*, const double*, double, long, hipStream_t);
 template hipError_t scale<double, double>(double*, const double*, double, long, hipStream_t);

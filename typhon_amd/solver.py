@@ -138,10 +138,11 @@ class Solver:
         self._check(self.lib.ty_sync(self.h), "ty_sync")
 
     def timing(self, enable=True):
-        ms = (C.c_double * 3)()
-        cnt = (C.c_int32 * 3)()
+        ms = (C.c_double * 5)()
+        cnt = (C.c_int32 * 5)()
         self._check(self.lib.ty_timing(self.h, 1 if enable else 0, ms, cnt), "ty_timing")
-        return dict(grad_ms=ms[0], flux_ms=ms[1], dt_ms=ms[2], grad_n=cnt[0], flux_n=cnt[1], dt_n=cnt[2])
+        return dict(grad_ms=ms[0], flux_ms=ms[1], dt_ms=ms[2], tangent_ms=ms[3], adjoint_ms=ms[4],
+                    grad_n=cnt[0], flux_n=cnt[1], dt_n=cnt[2], tangent_n=cnt[3], adjoint_n=cnt[4])
 
     # -- implicit path -----------------------------------------------------
     def jacobian_diag(self, dt):
