@@ -92,11 +92,11 @@ def cpu_baseline(n_cpu, budget_s=20.0):
 
 PROFILE_CMDS = {
     "trace": "rocprofv3 --kernel-trace --stats -f csv -d {d}/trace -o c2 -- python3 bench.py --steps 20 --warmup 2 "
-             "--no-cpu --no-mixed",
+             "--no-cpu --no-mixed --no-adjoint --no-implicit",
     "fetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/fetch -o c2 -- python3 bench.py --steps 2 --warmup 1 --no-cpu "
-             "--no-mixed",
+             "--no-mixed --no-adjoint --no-implicit",
     "write": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/write -o c2 -- python3 bench.py --steps 2 --warmup 1 --no-cpu "
-             "--no-mixed",
+             "--no-mixed --no-adjoint --no-implicit",
     "cfetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/cfetch -o copy -- python3 bench.py --copy-calibration 1024",
     "cwrite": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/cwrite -o copy -- python3 bench.py --copy-calibration 1024",
 }
@@ -185,6 +185,66 @@ def summarize_profiles(d, n=100):
     print("\n".join(lines))
 
 
+def adjoint_ratio(reps=20):
+    """Adjoint/primal cost on the adjoint workload: 3D first-order HLLC residual on the bump channel
+    (63 x 33 x 20 hexes x 6 = 249,480 tets).  Device time of one first-order residual kernel vs one
+    adjoint kernel launch ((dR/dU)^T lambda plus the face-area, volume and inlet sensitivities),
+    HIP events on the context's stream."""
+    import cases as K
+    from typhon_amd.solver import Solver
+    m, s, g, u = K.bump_case()
+    solver = Solver(m, s, g)
+    solver.set_state(u)
+    dt = solver.timestep()
+    rng = np.random.default_rng(0)
+    lam = rng.normal(size=(5, m.ncell_int))
+    solver.calc_rhs(dt)
+    solver.residual_adjoint(dt, lam)
+    solver.timing(True)
+    for _ in range(reps):
+        solver.calc_rhs(dt)
+    tp = solver.timing(True)
+    for _ in range(reps):
+        solver.residual_adjoint(dt, lam)
+    ta = solver.timing(False)
+    solver.close()
+    prim = tp["flux_ms"] / max(1, tp["flux_n"])
+    adj = ta["adjoint_ms"] / max(1, ta["adjoint_n"])
+    return dict(workload=f"3D first-order HLLC residual, bump channel, {m.ncell_int} tets", primal_kernel_ms=round(prim, 4),
+                adjoint_kernel_ms=round(adj, 4), ratio=round(adj / prim, 3))
+
+
+def implicit_bench(n=160, dt_mult=10.0, steps=2):
+    """The implicit workload: compressible NS Taylor-Green field on n^3 periodic hexes (Re 1e4,
+    Sutherland, Pr 0.72), first-order HLLC + cell-full viscous flux; one linearised backward-Euler
+    step = FP32 right-preconditioned GMRES(30) (block-Jacobi 5x5 from the HLL Jacobians, one-sided
+    FD Jacobian-vector products) inside FP64 refinement (central-difference JVP) to 1e-8."""
+    import cases as K
+    from typhon_amd.solver import Solver, krylov
+    t0 = time.perf_counter()
+    m, s, g, u = K.tgv_case(n)
+    tmesh = time.perf_counter() - t0
+    solver = Solver(m, s, g)
+    solver.set_state(u)
+    dt = dt_mult * solver.timestep()
+    kry = krylov(nkrylov=30, max_restart=4, max_outer=8, inner_precision=1, inner_tol=1e-3, outer_tol=1e-8)
+    solver.implicit_step(dt, kry)  # warm-up (allocation of the Krylov basis)
+    solver.sync()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(solver.implicit_step(dt, kry))
+    solver.sync()
+    el = time.perf_counter() - t0
+    solver.close()
+    inner = sum(st["inner_iters"] for st in stats)
+    return dict(workload=f"3D NS Taylor-Green, {n}^3 = {m.ncell_int} periodic hexes, implicit backward Euler, "
+                         f"dt = {dt_mult:g} x explicit", ms_per_implicit_step=round(el / steps * 1e3, 2),
+                outer_iters=[st["outer_iters"] for st in stats], inner_iters=[st["inner_iters"] for st in stats],
+                rel_res=[st["rel_res"] for st in stats], ms_per_arnoldi_step=round(el / max(1, inner) * 1e3, 3),
+                mesh_build_s=round(tmesh, 1))
+
+
 def load_traffic(path):
     try:
         with open(path) as f:
@@ -202,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=20)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-implicit", action="store_true")
     ap.add_argument("--order", default="hilbert", choices=["hilbert", "rcm", "none"],
                     help="device cell order set at mesh upload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -290,6 +352,13 @@ def main():
                         "flux_ms_avg": round(fl, 4),
                         "grad_ms_avg": round(tmm["grad_ms"] / max(1, tmm["grad_n"]), 4),
                         "flux_achieved_GBs": round(bmm["flux"] / (fl * 1e-3) / 1e9, 1)}
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_ratio()
+    if not args.no_implicit:
+        try:
+            out["implicit"] = implicit_bench()
+        except Exception as e:  # reported, not fatal: the headline line must still print
+            out["implicit"] = {"error": str(e)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
     print(json.dumps(out), flush=True)

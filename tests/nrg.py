@@ -23,10 +23,24 @@ def golden(case):
     return np.loadtxt(os.path.join(HERE, "golden", "nrg_rk", f"{case}.tmon"))
 
 
-def setup(case):
-    """quad20x20.tym: uniform unit square, vertices on the 0.05 grid; periodic in x and y"""
+def quad20x20(from_file=True):
+    """NRG/COMMON/quad20x20.tym (the unit square, vertices on the 0.05 grid) read from the copy in
+    tests/golden/meshes, periodic in x and y as the cases' CONNECTION blocks set it; or its
+    synthetic twin (test_tyfmt.py checks that the two agree)"""
+    if not from_file:
+        return M.periodic_quad(20, 20, 1.0, 1.0, 0.0, 1)
+    from typhon_amd import tyfmt
+    bocos = [dict(family="IMIN", type=M.BC_PERIOD, partner="IMAX", period=(1.0, 0.0, 0.0)),
+             dict(family="IMAX", type=M.BC_PERIOD, partner="IMIN", period=(-1.0, 0.0, 0.0)),
+             dict(family="JMIN", type=M.BC_PERIOD, partner="JMAX", period=(0.0, 1.0, 0.0)),
+             dict(family="JMAX", type=M.BC_PERIOD, partner="JMIN", period=(0.0, -1.0, 0.0))]
+    return tyfmt.tym_ustmesh(os.path.join(HERE, "golden", "meshes", "quad20x20.tym"), bocos)
+
+
+def setup(case, from_file=True):
+    """the case's mesh, scheme, gas and initial state (NRG/TIME/RUNGEKUTTA/<case>/main.rpm)"""
     tps, cfl = CASES[case]
-    m = M.periodic_quad(20, 20, 1.0, 1.0, 0.0, 1)
+    m = quad20x20(from_file)
     x = m.ccent[:m.ncell_int]
     rho = 1.0 + (np.sin(2 * np.pi * x[:, 0]) * np.sin(2 * np.pi * x[:, 1])) ** 2 / 2.0
     n = m.ncell_int

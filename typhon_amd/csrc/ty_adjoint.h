@@ -8,6 +8,8 @@ namespace ty {
 struct AdArgs {
     KArgs<double> k;      // mesh, geometry, boundary conditions, physics; k.us = FP64 state [nv][n]
     int nf, nb;           // device faces, device boundary faces
+    const int* bcell;     // [nbcell] cells with at least one boundary face slot (ascending)
+    int nbcell;
     // tangent inputs (each may be null = zero direction) and output
     const double* du;     // [nv][n] state direction
     const double* ds;     // [3][nf] area-vector direction per device face (owner orientation)

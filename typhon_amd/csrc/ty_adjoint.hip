@@ -8,13 +8,16 @@
 // slots, every face flux evaluated in the face's canonical orientation, a
 // deterministic per-cell sum.  The face derivatives come from the flux and
 // boundary templates instantiated with dual numbers (value + one derivative).
-// The adjoint evaluates, per slot, the derivative of the face flux along each
-// of this cell's NV state components and contracts it with the face's adjoint
-// weight m = sigma A (lambda_c dt/V_c - lambda_nb dt/V_nb): (dR/dU)^T lambda
-// gathered without atomics.  Face-area and inlet-state sensitivities are written
+// The adjoint contracts, per slot, the derivative of the face flux with the
+// face's adjoint weight m = sigma A (lambda_c dt/V_c - lambda_nb dt/V_nb):
+// (dR/dU)^T lambda gathered without atomics.  Interior HLLC/HLLCB faces use the
+// hand-written reverse sweep of the flux (one forward + one reverse pass gives
+// the derivative w.r.t. both states and the normal); boundary faces and the
+// other fluxes take one dual-number pass per input direction.  Face-area and inlet-state sensitivities are written
 // by the face's owner cell, so every output has exactly one writer.
 #include "ty_kernels_impl.h"
 #include "ty_dual.h"
+#include "ty_hllc_rev.h"
 #include "ty_adjoint.h"
 
 namespace ty {
@@ -74,6 +77,22 @@ __device__ __forceinline__ void slot_flux(const KArgs<double>& a, const SlotInfo
     num_flux<DIM>(a.ph.sch_hyp, Dual(a.ph.gamma), qL, qR, nf, F);
 }
 
+// A face's area-vector sensitivity from gn = (dF/dn)^T m and the flux (written by the owner):
+// lambda.R depends on S = A n through n = S/|S| and A = |S|.
+template <int DIM>
+__device__ __forceinline__ void face_sens(const AdArgs& t, int f, const double* nv, double A, const double* mv,
+                                          const double* Fv, const double* gn)
+{
+    constexpr int NV = DIM + 2;
+    double ndg = 0.0, ga = 0.0;
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) ndg += nv[e] * gn[e];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ga += (mv[v] / A) * Fv[v];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) t.gs[(size_t)e * t.nf + f] = e < DIM ? (gn[e] - nv[e] * ndg) / A + nv[e] * ga : 0.0;
+}
+
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void k_res_tangent(AdArgs t)
 {
@@ -126,13 +145,164 @@ __global__ __launch_bounds__(BLOCK) void k_res_tangent(AdArgs t)
     for (int v = 0; v < NV; ++v) t.out[(size_t)v * n + c] = (coef * res[v]).d;
 }
 
+// Adjoint contributions of one face slot of cell c by dual-number passes (any flux, any
+// boundary map): gu += (dF/dU_c)^T m, the face's sensitivities if c owns it, and the flux.
 template <int DIM>
+__device__ __forceinline__ void adj_slot_dual(const AdArgs& t, int c, const SlotInfo& s, const double* Uc,
+                                              const double* Uon, const double* nv, double A, const double* mv,
+                                              double* gu, double* Fv)
+{
+    constexpr int NV = DIM + 2;
+    const KArgs<double>& a = t.k;
+    Dual Uo[NV], Ud[NV], nd[DIM], F[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Uo[v] = Dual(Uon[v]);
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
+    for (int j = 0; j < NV; ++j) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Ud[v] = Dual(Uc[v], v == j ? 1.0 : 0.0);
+        slot_flux<DIM>(a, s, Ud, Uo, nd, nullptr, -1, F);
+        double acc = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc += mv[v] * F[v].d;
+        gu[j] += acc;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        Fv[v] = F[v].v;
+        Ud[v] = Dual(Uc[v]);
+    }
+    if (!s.other && t.gs) {
+        double gn[DIM];
+        for (int e = 0; e < DIM; ++e) {
+#pragma unroll
+            for (int i = 0; i < DIM; ++i) nd[i] = Dual(nv[i], i == e ? 1.0 : 0.0);
+            slot_flux<DIM>(a, s, Ud, Uo, nd, nullptr, -1, F);
+            double acc = 0.0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc += mv[v] * F[v].d;
+            gn[e] = acc;
+        }
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
+        face_sens<DIM>(t, s.f, nv, A, mv, Fv, gn);
+    }
+    if (s.nb < 0 && t.gq) {
+        const int bi = -s.nb - 1;
+        const bool inlet = a.boco[a.g.bboco[bi]].kind == BK_INLET_SUP;
+        for (int j = 0; j < NV; ++j) {
+            double acc = 0.0;
+            if (inlet) {
+                slot_flux<DIM>(a, s, Ud, Uo, nd, nullptr, j, F);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc += mv[v] * F[v].d;
+            }
+            t.gq[(size_t)j * t.nb + bi] = acc;
+        }
+    }
+}
+
+// adjoint weight of slot k's flux: d(lambda.R)/dF = sigma A (lambda_c dt/V_c - lambda_nb dt/V_nb)
+template <int DIM>
+__device__ __forceinline__ void adj_weight(const AdArgs& t, const SlotInfo& s, double A, const double* lc, double dt,
+                                           double* Uo, double* mv)
+{
+    constexpr int NV = DIM + 2;
+    const KArgs<double>& a = t.k;
+    const int n = a.n;
+    const double sg = s.other ? 1.0 : -1.0;
+    if (s.nb >= 0) {
+        const double dto = dt / a.vol[s.nb];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            Uo[v] = a.us[(size_t)v * n + s.nb];
+            mv[v] = sg * A * (lc[v] - t.lam[(size_t)v * n + s.nb] * dto);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            Uo[v] = 0.0;
+            mv[v] = sg * A * lc[v];
+        }
+    }
+}
+
+// REV: interior faces of HLLC/HLLCB by the hand-written reverse sweep, boundary slots left to
+// k_res_adjoint_bnd; otherwise every slot by dual-number passes.
+template <int DIM, bool REV>
 __global__ __launch_bounds__(BLOCK) void k_res_adjoint(AdArgs t)
 {
     constexpr int NV = DIM + 2;
     const KArgs<double>& a = t.k;
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n) return;
+    const int n = a.n;
+    const double dt = *a.d_dt;
+    const double dtv = dt / a.vol[c];
+    const double g1d = a.ph.gamma - 1.0;
+    double Uc[NV], lc[NV], gu[NV], res[NV], qcp[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        Uc[v] = a.us[(size_t)v * n + c];
+        lc[v] = t.lam[(size_t)v * n + c] * dtv;
+        gu[v] = 0.0;
+        res[v] = 0.0;
+    }
+    if (REV) cons_to_prim<DIM>(Uc, g1d, qcp);
+    for (int k = 0; k < a.ns; ++k) {
+        const int w = a.fw[(size_t)k * n + c];
+        if (w == SLOT_UNUSED) continue;
+        const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
+        if (REV && s.nb < 0) continue;
+        double nv[DIM];
+        face_n<DIM>(a.g, s.f, nv);
+        const double A = a.g.fs[s.f];
+        double Uo[NV], mv[NV], Fv[NV];
+        adj_weight<DIM>(t, s, A, lc, dt, Uo, mv);
+        if (REV) {
+            // reverse sweep: F, (dF/dq)^T m for both sides, (dF/dn)^T m
+            double qo[NV], qLb[NV], qRb[NV], gn[DIM];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                qLb[v] = 0.0;
+                qRb[v] = 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) gn[e] = 0.0;
+            cons_to_prim<DIM>(Uo, g1d, qo);
+            hllc_rev<DIM>(a.ph.sch_hyp, a.ph.gamma, s.other ? qo : qcp, s.other ? qcp : qo, nv, mv, Fv, qLb, qRb,
+                          gn);
+            prim_rev<DIM>(Uc, g1d, s.other ? qRb : qLb, gu);
+            if (!s.other && t.gs) face_sens<DIM>(t, s.f, nv, A, mv, Fv, gn);
+        } else {
+            adj_slot_dual<DIM>(t, c, s, Uc, Uo, nv, A, mv, gu, Fv);
+        }
+        const double sg = s.other ? 1.0 : -1.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) res[v] += sg * A * Fv[v];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] = gu[v];
+    if (t.gvol) {
+        // d(lambda.R_c)/dV_c = -lambda_c . R_c / V_c with R_c = dt/V_c res_c
+        double acc = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc += lc[v] * res[v];
+        t.gvol[c] = -acc / a.vol[c];
+    }
+}
+
+// boundary slots of the cells in t.bcell (after k_res_adjoint<DIM, true>): one thread per cell,
+// adds to that cell's gu and gvol, writes the boundary faces' sensitivities
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_res_adjoint_bnd(AdArgs t)
+{
+    constexpr int NV = DIM + 2;
+    const KArgs<double>& a = t.k;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= t.nbcell) return;
+    const int c = t.bcell[i];
     const int n = a.n;
     const double dt = *a.d_dt;
     const double dtv = dt / a.vol[c];
@@ -148,92 +318,23 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint(AdArgs t)
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
         const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
+        if (s.nb >= 0) continue;
         double nv[DIM];
         face_n<DIM>(a.g, s.f, nv);
         const double A = a.g.fs[s.f];
-        const double sg = s.other ? 1.0 : -1.0;
-        // adjoint weight of this face's flux: d(lambda.R)/dF = sigma A (lambda_c dt/V_c - lambda_nb dt/V_nb)
-        double mv[NV];
-        Dual Uo[NV], Ud[NV], nd[DIM], F[NV];
-        if (s.nb >= 0) {
-            const double dto = dt / a.vol[s.nb];
+        double Uo[NV], mv[NV], Fv[NV];
+        adj_weight<DIM>(t, s, A, lc, dt, Uo, mv);
+        adj_slot_dual<DIM>(t, c, s, Uc, Uo, nv, A, mv, gu, Fv);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                Uo[v] = Dual(a.us[(size_t)v * n + s.nb]);
-                mv[v] = sg * A * (lc[v] - t.lam[(size_t)v * n + s.nb] * dto);
-            }
-        } else {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                Uo[v] = Dual(0.0);
-                mv[v] = sg * A * lc[v];
-            }
-        }
-#pragma unroll
-        for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
-        // d/dU_c, one direction per conservative component
-        for (int j = 0; j < NV; ++j) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) Ud[v] = Dual(Uc[v], v == j ? 1.0 : 0.0);
-            slot_flux<DIM>(a, s, Ud, Uo, nd, nullptr, -1, F);
-            double acc = 0.0;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) acc += mv[v] * F[v].d;
-            gu[j] += acc;
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            res[v] += sg * A * F[v].v;
-            Ud[v] = Dual(Uc[v]);
-        }
-        if (!s.other && t.gs) {
-            // owner writes the face's area-vector sensitivity: through n (flux and boundary
-            // map) and A, then n = S/|S|, A = |S|
-            double Fv[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) Fv[v] = F[v].v;
-            double gn[DIM], ndg = 0.0;
-            for (int e = 0; e < DIM; ++e) {
-#pragma unroll
-                for (int i = 0; i < DIM; ++i) nd[i] = Dual(nv[i], i == e ? 1.0 : 0.0);
-                slot_flux<DIM>(a, s, Ud, Uo, nd, nullptr, -1, F);
-                double acc = 0.0;
-#pragma unroll
-                for (int v = 0; v < NV; ++v) acc += mv[v] * F[v].d;
-                gn[e] = acc;
-                ndg += nv[e] * acc;
-            }
-            double ga = 0.0;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) ga += (mv[v] / A) * Fv[v];
-#pragma unroll
-            for (int e = 0; e < 3; ++e)
-                t.gs[(size_t)e * t.nf + s.f] = e < DIM ? (gn[e] - nv[e] * ndg) / A + nv[e] * ga : 0.0;
-#pragma unroll
-            for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
-        }
-        if (s.nb < 0 && t.gq) {
-            const int bi = -s.nb - 1;
-            const bool inlet = a.boco[a.g.bboco[bi]].kind == BK_INLET_SUP;
-            for (int j = 0; j < NV; ++j) {
-                double acc = 0.0;
-                if (inlet) {
-                    slot_flux<DIM>(a, s, Ud, Uo, nd, nullptr, j, F);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) acc += mv[v] * F[v].d;
-                }
-                t.gq[(size_t)j * t.nb + bi] = acc;
-            }
-        }
+        for (int v = 0; v < NV; ++v) res[v] -= A * Fv[v];
     }
 #pragma unroll
-    for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] = gu[v];
+    for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] += gu[v];
     if (t.gvol) {
-        // d(lambda.R_c)/dV_c = -lambda_c . R_c / V_c with R_c = dt/V_c res_c
         double acc = 0.0;
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc += lc[v] * res[v];
-        t.gvol[c] = -acc / a.vol[c];
+        t.gvol[c] -= acc / a.vol[c];
     }
 }
 
@@ -247,7 +348,14 @@ hipError_t launch_res_tangent(const AdArgs& t, hipStream_t s)
 template <int DIM>
 hipError_t launch_res_adjoint(const AdArgs& t, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_res_adjoint<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+    const bool rev = t.k.ph.sch_hyp == 25 || t.k.ph.sch_hyp == 26;
+    if (!rev) {
+        hipLaunchKernelGGL((k_res_adjoint<DIM, false>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_res_adjoint<DIM, true>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+    if (t.nbcell > 0)
+        hipLaunchKernelGGL((k_res_adjoint_bnd<DIM>), dim3((t.nbcell + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, t);
     return hipGetLastError();
 }
 

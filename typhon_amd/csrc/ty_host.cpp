@@ -40,6 +40,7 @@ struct ty_ctx {
     std::string err;
     // host mesh (device order)
     int dim = 0, n = 0, ns = 0, nf = 0, nfi = 0, nb = 0;
+    int dense = 0;  // every cell has exactly ns face slots
     std::vector<int> perm, iperm;  // perm[new] = caller interior index
     std::vector<int> nbr, fw;      // [ns][n]
     std::vector<double> fn, fc, fs;  // [3][nf], [3][nf], [nf]
@@ -62,12 +63,15 @@ struct ty_ctx {
     Physics ph{};
     // device buffers
     std::vector<void*> owned;
+    std::vector<void*> mesh_bufs;  // the mesh upload's buffers among owned (kept by ty_set_scheme)
     int* d_nbr = nullptr;
     int* d_fw = nullptr;
     void* d_geo = nullptr;  // one allocation for the geometry of the active precision
     Geo<double> geo_d{};
     Geo<float> geo_f{};
     int* d_bboco = nullptr;
+    int* d_bcell = nullptr;  // cells owning a boundary face (ascending)
+    int nbcell = 0;
     BocoDev* d_boco = nullptr;
     double* d_vol = nullptr;
     double* d_dtfac = nullptr;
@@ -120,10 +124,13 @@ void free_state(ty_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void* p : c->owned) hipFree(p);
     c->owned.clear();
+    c->mesh_bufs.clear();
     c->d_nbr = nullptr;
     c->d_fw = nullptr;
     c->d_geo = nullptr;
     c->d_bboco = nullptr;
+    c->d_bcell = nullptr;
+    c->nbcell = 0;
     c->d_boco = nullptr;
     c->d_vol = c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
     c->d_us[0] = c->d_us[1] = nullptr;
@@ -133,7 +140,7 @@ void free_state(ty_ctx* c)
     c->kry_m = 0;
     c->kry_prec = -1;
     c->d_bad = c->d_err = nullptr;
-    for (auto* v : {&c->ev_grad, &c->ev_flux, &c->ev_dt})
+    for (auto* v : {&c->ev_grad, &c->ev_flux, &c->ev_dt, &c->ev_tan, &c->ev_adj})
         for (auto& e : *v) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
     c->ev_grad.clear();
     c->ev_flux.clear();
@@ -448,6 +455,7 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
     for (auto& sl : slots) ns = std::max<int>(ns, (int)sl.size());
     if (ns > MAX_SLOTS) return fail(c, TY_ERR_UNSUPPORTED, "more than 8 faces per cell");
     c->ns = ns;
+    c->dense = 0;
     c->nbr.assign((size_t)ns * n, 0);
     c->fw.assign((size_t)ns * n, SLOT_UNUSED);
     c->sumA.assign(n, 0.0);
@@ -477,10 +485,17 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         for (int d = 0; d < 3; ++d) c->cc[(size_t)d * n + i] = m->centre[(size_t)3 * o + d];
         c->vol[i] = m->volume[o];
     }
+    std::vector<int> bcell;
+    for (int k = 0; k < c->nb; ++k) bcell.push_back(bfaces[k].a);
+    std::sort(bcell.begin(), bcell.end());
+    bcell.erase(std::unique(bcell.begin(), bcell.end()), bcell.end());
+    c->nbcell = (int)bcell.size();
     int rc;
     if ((rc = upload(c, &c->d_nbr, c->nbr)) || (rc = upload(c, &c->d_fw, c->fw)) ||
-        (rc = upload(c, &c->d_bboco, c->bboco)) || (rc = upload(c, &c->d_vol, c->vol)))
+        (rc = upload(c, &c->d_bboco, c->bboco)) || (rc = upload(c, &c->d_vol, c->vol)) ||
+        (rc = upload(c, &c->d_bcell, bcell)))
         return rc;
+    c->mesh_bufs = c->owned;  // everything allocated so far belongs to the mesh
     c->mesh_ok = true;
     return TY_OK;
 }
@@ -642,7 +657,7 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     {
         std::vector<void*> keep;
         for (void* p : c->owned) {
-            if (p == c->d_nbr || p == c->d_fw || p == c->d_bboco || p == c->d_vol) keep.push_back(p);
+            if (std::find(c->mesh_bufs.begin(), c->mesh_bufs.end(), p) != c->mesh_bufs.end()) keep.push_back(p);
             else hipFree(p);
         }
         c->owned = keep;
@@ -799,6 +814,7 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     memset(&a, 0, sizeof(a));
     a.n = c->n;
     a.ns = c->ns;
+    a.dense = c->dense;
     a.nbr = c->d_nbr;
     a.fw = c->d_fw;
     a.g = g;
@@ -1503,6 +1519,8 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t)
     t.k.us = c->d_u0;
     t.nf = c->nf;
     t.nb = c->nb;
+    t.bcell = c->d_bcell;
+    t.nbcell = c->nbcell;
     return TY_OK;
 }
 

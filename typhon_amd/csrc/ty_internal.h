@@ -64,6 +64,7 @@ struct Physics {
 template <typename R>
 struct KArgs {
     int n, ns, nblk_real;
+    int dense;       // every cell uses all ns slots
     const int* nbr;  // [ns][n] >=0 neighbour cell, <0: -(boundary face + 1)
     const int* fw;   // [ns][n] slot word: face id | periodic << 26 | mirror slot << 27 | other-side << 31;
                      // -1 unused

@@ -41,9 +41,10 @@ __device__ __forceinline__ void load_prim(const R* __restrict__ us, int c, R g1,
     const R* u = us + (size_t)c * NV;
     const R rho = u[0];
     const R rhoE = u[1];
+    const R ir = R(1) / rho;
     R vel[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) vel[d] = u[2 + d] / rho;
+    for (int d = 0; d < DIM; ++d) vel[d] = u[2 + d] * ir;
     R sq = vel[0] * vel[0];
 #pragma unroll
     for (int d = 1; d < DIM; ++d) sq += vel[d] * vel[d];
@@ -159,21 +160,24 @@ __device__ __forceinline__ void bc_grad(const BocoDev& b, const R* n, const R* g
 // ---------------------------------------------------------------------------
 // numerical fluxes per unit area, primitive inputs (rho, p, u..)
 
+// Roe averages (calc_roe_states.f90:35-53) from the primitive states and their
+// reciprocal densities: density, velocity, total enthalpy h and the squared sound
+// speed a2 = (gamma - 1)(h - |v|^2 / 2) (the reference's gamma p / rho of its averaged
+// pressure p = (h - |v|^2/2) rho (gamma - 1) / gamma, without the two divisions).
 template <int DIM, typename R>
-__device__ __forceinline__ void roe_avg(R g, const R* qL, const R* qR, R& rho, R* vel, R& p)
+__device__ __forceinline__ void roe_avg(R g, const R* qL, const R* qR, R irL, R irR, R& rho, R* vel, R& h, R& a2)
 {
-    // calc_roe_states (SOURCE/EQNS/calc_roe_states.f90:35-53)
     const R gig1 = g / (g - R(1));
-    const R t1 = sqrt(qR[0] / qL[0]);
+    const R t1 = sqrt(qR[0] * irL);
     rho = qL[0] * t1;
     const R kl = R(1) / (R(1) + t1);
     const R kr = t1 * kl;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) vel[d] = kl * qL[2 + d] + kr * qR[2 + d];
-    const R hl = gig1 * qL[1] / qL[0] + R(.5) * dotd<DIM>(qL + 2, qL + 2);
-    const R hr = gig1 * qR[1] / qR[0] + R(.5) * dotd<DIM>(qR + 2, qR + 2);
-    const R h = kl * hl + kr * hr;
-    p = (h - R(.5) * dotd<DIM>(vel, vel)) * rho / gig1;
+    const R hl = gig1 * qL[1] * irL + R(.5) * dotd<DIM>(qL + 2, qL + 2);
+    const R hr = gig1 * qR[1] * irR + R(.5) * dotd<DIM>(qR + 2, qR + 2);
+    h = kl * hl + kr * hr;
+    a2 = (g - R(1)) * (h - R(.5) * dotd<DIM>(vel, vel));
 }
 
 template <int DIM, typename R>
@@ -184,8 +188,9 @@ __device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* 
     const R vnr = dotd<DIM>(qR + 2, n);
     const R rel = ig1 * qL[1] + R(.5) * qL[0] * dotd<DIM>(qL + 2, qL + 2);
     const R rer = ig1 * qR[1] + R(.5) * qR[0] * dotd<DIM>(qR + 2, qR + 2);
+    const R irL = R(1) / qL[0], irR = R(1) / qR[0];
+    const R al = sqrt(g * qL[1] * irL), ar = sqrt(g * qR[1] * irR);
     if (sch == 5) {  // Rusanov: calc_flux_rusanov.f90:64-83
-        const R al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
         const R ray = fmax_(fabs(vnl) + al, fabs(vnr) + ar);
         F[0] = R(.5) * ((vnl + ray) * qL[0] + (vnr - ray) * qR[0]);
         F[1] = R(.5) * ((vnl + ray) * rel + (vnr - ray) * rer + vnl * qL[1] + vnr * qR[1]);
@@ -195,13 +200,12 @@ __device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* 
                                 + (qL[1] + qR[1]) * n[d]);
         return;
     }
-    R rrho, rv[DIM], rp;
-    roe_avg<DIM>(g, qL, qR, rrho, rv, rp);
+    R rrho, rv[DIM], H, a2;
+    roe_avg<DIM>(g, qL, qR, irL, irR, rrho, rv, H, a2);
     const R vm = dotd<DIM>(rv, n);
-    const R am = sqrt(g * rp / rrho);
+    const R am = sqrt(a2);
     if (sch == 21 || sch == 22) {  // HLLE(B): calc_flux_hlle.f90:109-143
         const R beta = (sch == 21) ? R(1) : sqrt((g - R(1)) / (R(2) * g));
-        const R al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
         const R sl = fmin_(fmin_(R(0), vnl - beta * al), vm - am);
         const R sr = fmax_(fmax_(R(0), vnr + beta * ar), vm + am);
         const R iks = R(1) / (sr - sl);
@@ -215,7 +219,6 @@ __device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* 
         return;
     }
     if (sch == 10) {  // Roe with Harten entropy fix (the reference parses ROE but stops on it)
-        const R H = (g / (g - R(1))) * rp / rrho + R(.5) * dotd<DIM>(rv, rv);
         const R a = am;
         const R drho = qR[0] - qL[0], dp = qR[1] - qL[1], dvn = vnr - vnl;
         R l1 = fabs(vm - a), l2 = fabs(vm), l3 = fabs(vm + a);
@@ -241,7 +244,6 @@ __device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* 
     }
     // HLLC (Batten), HLLCB: calc_flux_hllc.f90:61-171
     const R beta = (sch == 26) ? sqrt((g - R(1)) / (R(2) * g)) : R(1);
-    const R al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
     const R sl = fmin_(vnl - beta * al, vm - am);
     const R sr = fmax_(vnr + beta * ar, vm + am);
     if (sl >= R(0)) {
@@ -469,7 +471,9 @@ __device__ __forceinline__ void muscl_own(const Physics& ph, const R* qL, const 
 // reconstruction performed by K1 after the gradient
 enum Rec : int { REC_NONE = 0, REC_VENKAT = 1, REC_MUSCL = 2, REC_MUSCL_BARTH = 3 };
 
-template <int DIM, typename R, int NSM, int REC>
+// DENSE: every cell uses all NSM == ns slots (closed tet/hex/quad meshes), so the slot
+// loops carry no guards and every slot's loads can be issued together.
+template <int DIM, typename R, int NSM, int REC, bool DENSE>
 __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 {
     constexpr int NV = DIM + 2;
@@ -490,9 +494,9 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
     }
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
-        if (k >= a.ns) break;
+        if (!DENSE && k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
-        if (w == SLOT_UNUSED) continue;
+        if (!DENSE && w == SLOT_UNUSED) continue;
         R qo[NV], wk[DIM];
         far_state<DIM>(a, k, c, a.nbr[(size_t)k * n + c], qc, qo);
         slot_vec<DIM>(a.g.sw, k, n, c, wk);
@@ -514,30 +518,47 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
     }
     if (REC == REC_NONE) return;
     if (REC == REC_VENKAT) {
-        // Venkatakrishnan: phi_v = min over faces, then q_f = q_c + phi grad(q).(x_f - x_c)
+        // Venkatakrishnan: phi_v = min over faces, then q_f = q_c + phi grad(q).(x_f - x_c).
+        // The face values phi_f = num/den lie in [0, 1] with den > 0 (venkat_phi), so the
+        // minimum is tracked as a fraction by cross-multiplication and divided once.
         const R eps2 = a.g.eps2[c];
-        R phi[NV];
+        R pn[NV], pd[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) phi[v] = R(1);
-        for (int pass = 0; pass < 2; ++pass) {
+        for (int v = 0; v < NV; ++v) {
+            pn[v] = R(1);
+            pd[v] = R(1);
+        }
 #pragma unroll
-            for (int k = 0; k < NSM; ++k) {
-                if (k >= a.ns) break;
-                if (a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
-                R rk[DIM];
-                slot_vec<DIM>(a.g.sr, k, n, c, rk);
-                if (pass == 0) {
+        for (int k = 0; k < NSM; ++k) {
+            if (!DENSE && k >= a.ns) break;
+            if (!DENSE && a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
+            R rk[DIM];
+            slot_vec<DIM>(a.g.sr, k, n, c, rk);
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        const R d2 = dotd<DIM>(gr + v * DIM, rk);
-                        phi[v] = fmin_(phi[v], venkat_phi(qmax[v] - qc[v], qmin[v] - qc[v], d2, eps2));
-                    }
-                } else {
-                    R* out = a.qf_out + ((size_t)k * n + c) * NV;
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) out[v] = qc[v] + phi[v] * dotd<DIM>(gr + v * DIM, rk);
+            for (int v = 0; v < NV; ++v) {
+                const R d2 = dotd<DIM>(gr + v * DIM, rk);
+                if (d2 == R(0)) continue;
+                const R dm = (d2 > R(0)) ? qmax[v] - qc[v] : qmin[v] - qc[v];
+                const R num = (dm * dm + eps2) + R(2) * d2 * dm;
+                const R den = dm * dm + R(2) * d2 * d2 + dm * d2 + eps2;
+                if (num * pd[v] < pn[v] * den) {
+                    pn[v] = num;
+                    pd[v] = den;
                 }
             }
+        }
+        R phi[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) phi[v] = pn[v] / pd[v];
+#pragma unroll
+        for (int k = 0; k < NSM; ++k) {
+            if (!DENSE && k >= a.ns) break;
+            if (!DENSE && a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
+            R rk[DIM];
+            slot_vec<DIM>(a.g.sr, k, n, c, rk);
+            R* out = a.qf_out + ((size_t)k * n + c) * NV;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) out[v] = qc[v] + phi[v] * dotd<DIM>(gr + v * DIM, rk);
         }
         return;
     }
@@ -548,9 +569,9 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
     R qfk[barth ? NSM : 1][NV];
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
-        if (k >= a.ns) break;
+        if (!DENSE && k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
-        if (w == SLOT_UNUSED) continue;
+        if (!DENSE && w == SLOT_UNUSED) continue;
         const bool other = w < 0;
         R qo[NV], dk[DIM], rk[DIM];
         far_state<DIM>(a, k, c, a.nbr[(size_t)k * n + c], qc, qo);
@@ -577,8 +598,8 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
     if (DIM == 2) ratio[2] = fmin_(ratio[2], R(0));
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
-        if (k >= a.ns) break;
-        if (a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
+        if (!DENSE && k >= a.ns) break;
+        if (!DENSE && a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
         R* out = a.qf_out + ((size_t)k * n + c) * NV;
 #pragma unroll
         for (int v = 0; v < NV; ++v) out[v] = qc[v] + ratio[v < 2 ? v : 2] * (qfk[barth ? k : 0][v] - qc[v]);
@@ -708,11 +729,12 @@ __device__ __forceinline__ void jac_hll(const Physics& ph, const R* qL, const R*
     const R g = R(ph.gamma);
     const R beta = (ph.sch_hyp == 26) ? sqrt((g - R(1)) / (R(2) * g)) : R(1);
     const R vnl = dotd<DIM>(qL + 2, n), vnr = dotd<DIM>(qR + 2, n);
-    const R al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
-    R rrho, rv[DIM], rp;
-    roe_avg<DIM>(g, qL, qR, rrho, rv, rp);
+    const R irL = R(1) / qL[0], irR = R(1) / qR[0];
+    const R al = sqrt(g * qL[1] * irL), ar = sqrt(g * qR[1] * irR);
+    R rrho, rv[DIM], H, a2;
+    roe_avg<DIM>(g, qL, qR, irL, irR, rrho, rv, H, a2);
     const R vm = dotd<DIM>(rv, n);
-    const R am = sqrt(g * rp / rrho);
+    const R am = sqrt(a2);
     R sl = fmin_(vnl - beta * al, vm - am);
     R sr = fmax_(vnr + beta * ar, vm + am);
     sl = fmin_(sl, R(0));
@@ -792,7 +814,7 @@ __device__ __forceinline__ void invert_block(double* m, double* inv)
 // at its mirror slot; otherwise first order on cell states.  VISC: add the
 // viscous flux from cell states and cell gradients of both sides.
 
-template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC>
+template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC, bool DENSE>
 __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
 {
     constexpr int NV = DIM + 2;
@@ -816,9 +838,9 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
     for (int v = 0; v < NV; ++v) res[v] = R(0);
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
-        if (k >= a.ns) break;
+        if (!DENSE && k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
-        if (w == SLOT_UNUSED) continue;
+        if (!DENSE && w == SLOT_UNUSED) continue;
         const int nb = a.nbr[(size_t)k * n + c];
         const bool other = w < 0;
         R nf[DIM];
@@ -1045,46 +1067,54 @@ __global__ void k_to_stage(const double* __restrict__ src, R* __restrict__ dst, 
 // ---------------------------------------------------------------------------
 // launchers
 
-template <int DIM, typename R, int NSM>
+template <int DIM, typename R, int NSM, bool DENSE>
 static void gradient_variant(const KArgs<R>& a, hipStream_t s)
 {
     const dim3 grid(grid_blocks(a.n)), block(BLOCK);
-    if (!a.qf_out) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_NONE>), grid, block, 0, s, a);
-    else if (a.ph.limiter == 'W') hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_VENKAT>), grid, block, 0, s, a);
-    else if (a.ph.barth) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL_BARTH>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL>), grid, block, 0, s, a);
+    if (!a.qf_out) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_NONE, DENSE>), grid, block, 0, s, a);
+    else if (a.ph.limiter == 'W') hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_VENKAT, DENSE>), grid, block, 0, s, a);
+    else if (a.ph.barth) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL_BARTH, DENSE>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL, DENSE>), grid, block, 0, s, a);
 }
 
+// slot-count variant: dense 4 (tets, quads) or 6 (hexes) slots per cell, otherwise guarded loops
 template <int DIM, typename R>
 hipError_t launch_gradient(const KArgs<R>& a, hipStream_t s)
 {
-    if (a.ns <= 4) gradient_variant<DIM, R, 4>(a, s);
-    else gradient_variant<DIM, R, MAX_SLOTS>(a, s);
+    if (a.dense && a.ns == 4) gradient_variant<DIM, R, 4, true>(a, s);
+    else if (a.dense && a.ns == 6) gradient_variant<DIM, R, 6, true>(a, s);
+    else if (a.ns <= 4) gradient_variant<DIM, R, 4, false>(a, s);
+    else gradient_variant<DIM, R, MAX_SLOTS, false>(a, s);
     return hipGetLastError();
 }
 
-template <int DIM, typename R, int NSM, bool JAC>
+template <int DIM, typename R, int NSM, bool JAC, bool DENSE>
 static void flux_variant(const KArgs<R>& a, hipStream_t s)
 {
     const dim3 grid(grid_blocks(a.n)), block(BLOCK);
     const bool fs = a.ph.method != 'N';
     const bool visc = a.ph.viscous != 0;
-    if (fs && visc) hipLaunchKernelGGL((k_flux<DIM, R, true, true, NSM, JAC>), grid, block, 0, s, a);
-    else if (fs) hipLaunchKernelGGL((k_flux<DIM, R, true, false, NSM, JAC>), grid, block, 0, s, a);
-    else if (visc) hipLaunchKernelGGL((k_flux<DIM, R, false, true, NSM, JAC>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_flux<DIM, R, false, false, NSM, JAC>), grid, block, 0, s, a);
+    if (fs && visc) hipLaunchKernelGGL((k_flux<DIM, R, true, true, NSM, JAC, DENSE>), grid, block, 0, s, a);
+    else if (fs) hipLaunchKernelGGL((k_flux<DIM, R, true, false, NSM, JAC, DENSE>), grid, block, 0, s, a);
+    else if (visc) hipLaunchKernelGGL((k_flux<DIM, R, false, true, NSM, JAC, DENSE>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_flux<DIM, R, false, false, NSM, JAC, DENSE>), grid, block, 0, s, a);
 }
 
 template <int DIM, typename R>
 hipError_t launch_flux(const KArgs<R>& a, hipStream_t s)
 {
     const bool jac = a.dinv_out != nullptr;
-    if (a.ns <= 4) {
-        if (jac) flux_variant<DIM, R, 4, true>(a, s);
-        else flux_variant<DIM, R, 4, false>(a, s);
+    if (jac) {  // implicit diagonal blocks: guarded loops only
+        if (a.ns <= 4) flux_variant<DIM, R, 4, true, false>(a, s);
+        else flux_variant<DIM, R, MAX_SLOTS, true, false>(a, s);
+    } else if (a.dense && a.ns == 4) {
+        flux_variant<DIM, R, 4, false, true>(a, s);
+    } else if (a.dense && a.ns == 6) {
+        flux_variant<DIM, R, 6, false, true>(a, s);
+    } else if (a.ns <= 4) {
+        flux_variant<DIM, R, 4, false, false>(a, s);
     } else {
-        if (jac) flux_variant<DIM, R, MAX_SLOTS, true>(a, s);
-        else flux_variant<DIM, R, MAX_SLOTS, false>(a, s);
+        flux_variant<DIM, R, MAX_SLOTS, false, false>(a, s);
     }
     return hipGetLastError();
 }

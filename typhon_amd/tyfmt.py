@@ -1,0 +1,236 @@
+"""TYPHON XBIN files (.tym meshes): reader and writer of the section format, and the
+unstructured mesh they hold.
+
+The byte layout follows the reference's XBIN I/O (CFDTOOLS/FileFormat/internal/XBIN_IO.f90,
+XBIN_DATA.f90) and its mesh sections (TYFMT_MESH.f90, TYPHON_FMT.f90):
+
+* header: "XBIN", 2-byte version (high byte first), int32 endianness probe 0x01020304;
+* each section: int16 data type, 32-char name, int16 group index, int16 user type, int16 integer
+  size, int16 real size, int16 nparam, nparam int32 parameters, (version >= 2) int16 string
+  length and the string, then the data and the 3-char "END":
+    - type 1 (no data): nothing;
+    - type 10/20 (ordered int32/real64 array): int32 first index, nelem, dim, then dim*nelem
+      values, dim varying fastest (dim 0 means a 1-D array);
+    - type 15/25 (indirect array): int32 nelem, dim, the nelem int32 indices, then the values.
+* a mesh file: FILE_HEADER, USTMESH (8 parameters: ..., ncell, nface, nvtex, ncellsections,
+  nfacesections, ncellmark, nfacemark), NODES (3 x nvtex reals), CELLS sections (element type
+  as parameter 1, vertex lists), FACES sections (boundary faces), then one mark section per
+  boundary family holding indices into the boundary faces.
+"""
+from __future__ import annotations
+
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# element types of the cell and face sections (CFDTOOLS/Mesh/ELEMVTEX.f90:18-34)
+ELEM_BAR2, ELEM_TRI3, ELEM_QUAD4 = 2, 13, 24
+ELEM_TETRA4, ELEM_PYRA5, ELEM_PENTA6, ELEM_HEXA8 = 101, 121, 131, 141
+# section user types (TYPHON_FMT.f90:28-44)
+TY_FILEDEF, TY_MESHDEF, TY_NODES, TY_CELLS, TY_FACES, TY_MARKS = 1, 10, 11, 12, 15, 20
+DT_NODATA, DT_INT_ORD, DT_INT_IND, DT_REAL_ORD, DT_REAL_IND = 1, 10, 15, 20, 25
+
+
+@dataclass
+class Section:
+    name: str
+    datatype: int
+    usertype: int
+    params: list = field(default_factory=list)
+    string: str = ""
+    first: int = 1
+    index: np.ndarray | None = None   # indirect arrays: element indices (1-based)
+    data: np.ndarray | None = None    # [nelem, dim] (or [nelem] for 1-D arrays)
+
+
+class _Reader:
+    def __init__(self, buf: bytes):
+        self.b = buf
+        self.p = 0
+        self.e = "<"
+
+    def take(self, n):
+        if self.p + n > len(self.b):
+            raise ValueError("XBIN: unexpected end of file")
+        out = self.b[self.p:self.p + n]
+        self.p += n
+        return out
+
+    def i16(self, k=1):
+        v = struct.unpack(f"{self.e}{k}h", self.take(2 * k))
+        return v if k > 1 else v[0]
+
+    def i32(self, k=1):
+        v = struct.unpack(f"{self.e}{k}i", self.take(4 * k))
+        return v if k > 1 else v[0]
+
+
+def read_xbin(path) -> tuple[int, list]:
+    """(version, sections) of an XBIN file"""
+    with open(path, "rb") as f:
+        r = _Reader(f.read())
+    if r.take(4) != b"XBIN":
+        raise ValueError(f"{path}: not an XBIN file")
+    hi, lo = r.take(2)
+    version = hi * 256 + lo
+    probe = r.take(4)
+    if probe == b"\x04\x03\x02\x01":
+        r.e = "<"
+    elif probe == b"\x01\x02\x03\x04":
+        r.e = ">"
+    else:
+        raise ValueError(f"{path}: bad XBIN endianness probe")
+    sections = []
+    while r.p < len(r.b):
+        dtype = r.i16()
+        name = r.take(32).decode("ascii", "replace").strip()
+        _group, usertype, isize, rsize, nparam = (r.i16() for _ in range(5))
+        if isize != 4 or rsize != 8:
+            raise ValueError(f"{path}: section {name}: unsupported integer/real sizes {isize}/{rsize}")
+        params = list(r.i32(nparam)) if nparam > 1 else ([r.i32()] if nparam == 1 else [])
+        string = ""
+        if version >= 2:
+            slen = r.i16()
+            string = r.take(slen).decode("ascii", "replace") if slen > 0 else ""
+        sec = Section(name, dtype, usertype, params, string)
+        if dtype in (DT_INT_ORD, DT_REAL_ORD):
+            sec.first, nelem, dim = r.i32(3)
+        elif dtype in (DT_INT_IND, DT_REAL_IND):
+            nelem, dim = r.i32(2)
+            sec.index = np.frombuffer(r.take(4 * nelem), dtype=f"{r.e}i4").astype(np.int64)
+        elif dtype != DT_NODATA:
+            raise ValueError(f"{path}: section {name}: unknown data type {dtype}")
+        if dtype != DT_NODATA:
+            width = max(dim, 1)
+            if dtype in (DT_INT_ORD, DT_INT_IND):
+                vals = np.frombuffer(r.take(4 * width * nelem), dtype=f"{r.e}i4").astype(np.int64)
+            else:
+                vals = np.frombuffer(r.take(8 * width * nelem), dtype=f"{r.e}f8").astype(np.float64)
+            sec.data = vals.reshape(nelem, width) if dim > 0 else vals
+        if r.take(3) != b"END":
+            raise ValueError(f"{path}: section {name}: missing END")
+        sections.append(sec)
+    return version, sections
+
+
+def write_xbin(path, sections, version=2) -> None:
+    """write sections (as read_xbin returns them) in little-endian XBIN"""
+    out = [b"XBIN", bytes([version // 256, version % 256]), struct.pack("<i", 0x01020304)]
+    for s in sections:
+        out.append(struct.pack("<h", s.datatype))
+        out.append(s.name.encode("ascii").ljust(32)[:32])
+        out.append(struct.pack("<5h", 0, s.usertype, 4, 8, len(s.params)))
+        if s.params:
+            out.append(struct.pack(f"<{len(s.params)}i", *s.params))
+        if version >= 2:
+            sb = s.string.encode("ascii")
+            out.append(struct.pack("<h", len(sb)) + sb)
+        if s.datatype != DT_NODATA:
+            data = np.asarray(s.data)
+            nelem = data.shape[0]
+            dim = data.shape[1] if data.ndim == 2 else 0
+            if s.datatype in (DT_INT_ORD, DT_REAL_ORD):
+                out.append(struct.pack("<3i", s.first, nelem, dim))
+            else:
+                out.append(struct.pack("<2i", nelem, dim))
+                out.append(np.asarray(s.index, dtype="<i4").tobytes())
+            kind = "<i4" if s.datatype in (DT_INT_ORD, DT_INT_IND) else "<f8"
+            out.append(np.ascontiguousarray(data, dtype=kind).tobytes())
+        out.append(b"END")
+    with open(path, "wb") as f:
+        f.write(b"".join(out))
+
+
+@dataclass
+class TyMesh:
+    """raw content of a .tym mesh: vertices, cells per element type, boundary faces, marks"""
+    vtx: np.ndarray                     # [nvtx, 3]
+    cells: dict                         # element type -> [ncell_t, nv] 0-based vertex ids
+    cell_index: dict                    # element type -> [ncell_t] 0-based global cell index
+    bfaces: list                        # list of [nface_t, nv] 0-based vertex ids, in boundary order
+    marks: dict                         # family -> 0-based indices into the concatenated bfaces
+
+
+def read_tym(path) -> TyMesh:
+    _, secs = read_xbin(path)
+    mesh = [s for s in secs if s.name == "USTMESH"]
+    if not mesh:
+        raise ValueError(f"{path}: no USTMESH section")
+    vtx, cells, cidx, bfaces, marks = None, {}, {}, [], {}
+    for s in secs:
+        if s.usertype == TY_NODES:
+            vtx = np.asarray(s.data, dtype=np.float64).reshape(-1, 3)
+        elif s.usertype == TY_CELLS:
+            et = s.params[0]
+            idx = s.index if s.index is not None else np.arange(s.first, s.first + s.data.shape[0])
+            cells[et] = s.data - 1
+            cidx[et] = idx - 1
+        elif s.usertype == TY_FACES:
+            bfaces.append(s.data - 1)
+        elif s.usertype == TY_MARKS:
+            marks[s.name] = np.asarray(s.data).ravel() - 1
+    if vtx is None:
+        raise ValueError(f"{path}: no NODES section")
+    return TyMesh(vtx, cells, cidx, bfaces, marks)
+
+
+_ETYPE = {ELEM_QUAD4: "quad", ELEM_TRI3: "tri", ELEM_TETRA4: "tet", ELEM_HEXA8: "hex"}
+
+
+def tym_ustmesh(path, bocos: list, dim: int | None = None):
+    """Build the solver mesh (typhon_amd.mesh.UstMesh) of a .tym file; ``bocos`` as for
+    mesh.build_ustmesh (families named after the file's marks; periodic bocos name their
+    partner and are paired through the translation ``period`` = (dx, dy, dz) of the spec)."""
+    from . import mesh as M
+    t = read_tym(path)
+    if len(t.cells) != 1:
+        raise NotImplementedError("mixed-element .tym meshes")
+    (et, cv), = t.cells.items()
+    order = np.argsort(t.cell_index[et])
+    cv = cv[order]
+    etype = _ETYPE.get(et)
+    if etype is None:
+        raise NotImplementedError(f"element type {et}")
+    if dim is None:
+        dim = 2 if etype in ("quad", "tri") else 3
+    allb = np.concatenate(t.bfaces, axis=0) if t.bfaces else np.zeros((0, 2), dtype=np.int64)
+    key_fam = {}
+    for fam, ids in t.marks.items():
+        for i in ids:
+            key_fam[tuple(sorted(allb[i].tolist()))] = fam
+    wrap = _periodic_wrap(t, bocos)
+
+    def family(fv, fc):
+        out = np.empty(fv.shape[0], dtype=object)
+        for k in range(fv.shape[0]):
+            key = tuple(sorted(int(v) for v in fv[k] if v >= 0))
+            out[k] = key_fam.get(key)
+        return out
+    return M.build_ustmesh(dim, t.vtx.copy(), cv, etype, family, bocos, wrap)
+
+
+def _periodic_wrap(t: TyMesh, bocos):
+    """vertex ids identified across translation-periodic families: each vertex of a family
+    that a periodic spec names with a 'period' translation maps to the partner-family vertex at
+    x + period (matched on coordinates), so paired faces share wrapped vertex ids"""
+    specs = [b for b in bocos if b.get("period") is not None]
+    if not specs:
+        return None
+    allb = np.concatenate(t.bfaces, axis=0)
+    wrap = np.arange(t.vtx.shape[0])
+    for b in specs:
+        src = np.unique(allb[t.marks[b["family"]]].ravel())
+        dst = np.unique(allb[t.marks[b["partner"]]].ravel())
+        shifted = t.vtx[src] + np.asarray(b["period"], dtype=np.float64)
+        scale = np.abs(t.vtx).max() + 1.0
+        for i, x in zip(src, shifted):
+            d = np.abs(t.vtx[dst] - x).max(axis=1)
+            j = int(np.argmin(d))
+            if d[j] > 1e-9 * scale:
+                raise ValueError(f"no periodic partner for vertex {i} of {b['family']}")
+            wrap[dst[j]] = wrap[i]
+    # resolve chains (corner vertices in two periodic directions)
+    for _ in range(3):
+        wrap = wrap[wrap]
+    return wrap
