@@ -455,7 +455,9 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
     for (auto& sl : slots) ns = std::max<int>(ns, (int)sl.size());
     if (ns > MAX_SLOTS) return fail(c, TY_ERR_UNSUPPORTED, "more than 8 faces per cell");
     c->ns = ns;
-    c->dense = 0;
+    c->dense = 1;
+    for (auto& sl : slots)
+        if ((int)sl.size() != ns) c->dense = 0;
     c->nbr.assign((size_t)ns * n, 0);
     c->fw.assign((size_t)ns * n, SLOT_UNUSED);
     c->sumA.assign(n, 0.0);
