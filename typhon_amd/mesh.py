@@ -120,15 +120,18 @@ def _extract_faces(cells: np.ndarray, etype: str, wrap: np.ndarray | None):
     srt = np.sort(fv[:, :nvf], axis=1)
     nvtx = int(cells.max()) + 1
     bits = int(np.ceil(np.log2(max(nvtx, 2))))
-    if bits * nvf > 63:
-        raise ValueError("mesh too large for the face key")
-    key = np.zeros(srt.shape[0], dtype=np.int64)
-    for j in range(nvf):
-        key = (key << bits) | srt[:, j]
-    order = np.argsort(key, kind="stable")
-    ks = key[order]
-    first = np.ones(ks.shape[0], dtype=bool)
-    first[1:] = ks[1:] != ks[:-1]
+    first = np.ones(srt.shape[0], dtype=bool)
+    if bits * nvf <= 63:
+        key = np.zeros(srt.shape[0], dtype=np.int64)
+        for j in range(nvf):
+            key = (key << bits) | srt[:, j]
+        order = np.argsort(key, kind="stable")
+        ks = key[order]
+        first[1:] = ks[1:] != ks[:-1]
+    else:  # too many vertices to pack the sorted ids into one int64: sort on the columns
+        order = np.lexsort(srt.T[::-1])
+        ss = srt[order]
+        first[1:] = np.any(ss[1:] != ss[:-1], axis=1)
     grp = np.cumsum(first) - 1
     nuniq = int(grp[-1]) + 1
     # first occurrence (lowest local index in cell-major order, stable sort)
@@ -146,10 +149,13 @@ def _extract_faces(cells: np.ndarray, etype: str, wrap: np.ndarray | None):
     c2 = c2[app]
     pkey = None
     if wrap is not None:
+        # one id per set of wrapped vertex ids (boundary faces only are ever compared)
         ws = np.sort(wrap[fvtx[:, :nvf]], axis=1)
-        pkey = np.zeros(ws.shape[0], dtype=np.int64)
-        for j in range(nvf):
-            pkey = (pkey << bits) | ws[:, j]
+        bnd = c2 < 0
+        pkey = np.full(ws.shape[0], -1, dtype=np.int64)
+        if bnd.any():
+            _, inv = np.unique(ws[bnd], axis=0, return_inverse=True)
+            pkey[bnd] = inv.ravel()
     return fvtx, c1, c2, pkey
 
 
