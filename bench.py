@@ -330,6 +330,8 @@ def main():
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "traffic_achieved_GBs": (round(traffic["bytes_per_launch"] / (ms_avg * 1e-3) / 1e9, 1)
+                                              if traffic else None),
                      "algorithmic_bytes_per_launch": int(bm[kern]),
                      "avg_launch_ms": round(ms_avg, 4)},
         "kernels_fp64": {"grad_ms_avg": round(tm64["grad_ms"] / max(1, tm64["grad_n"]), 4),

@@ -43,6 +43,7 @@ struct ty_ctx {
     int dense = 0;  // every cell has exactly ns face slots
     std::vector<int> perm, iperm;  // perm[new] = caller interior index
     std::vector<int> nbr, fw;      // [ns][n]
+    std::vector<int> link;         // [ns][n] link words of the residual kernels
     std::vector<double> fn, fc, fs;  // [3][nf], [3][nf], [nf]
     std::vector<int> fsrc;           // [nf] caller face (0-based) whose geometry device face k carries
     int nface_caller = 0;            // faces of the caller's mesh
@@ -66,6 +67,7 @@ struct ty_ctx {
     std::vector<void*> mesh_bufs;  // the mesh upload's buffers among owned (kept by ty_set_scheme)
     int* d_nbr = nullptr;
     int* d_fw = nullptr;
+    int* d_link = nullptr;
     void* d_geo = nullptr;  // one allocation for the geometry of the active precision
     Geo<double> geo_d{};
     Geo<float> geo_f{};
@@ -127,6 +129,7 @@ void free_state(ty_ctx* c)
     c->mesh_bufs.clear();
     c->d_nbr = nullptr;
     c->d_fw = nullptr;
+    c->d_link = nullptr;
     c->d_geo = nullptr;
     c->d_bboco = nullptr;
     c->d_bcell = nullptr;
@@ -460,6 +463,8 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         if ((int)sl.size() != ns) c->dense = 0;
     c->nbr.assign((size_t)ns * n, 0);
     c->fw.assign((size_t)ns * n, SLOT_UNUSED);
+    c->link.assign((size_t)ns * n, SLOT_UNUSED);
+    if (n >= (1 << 27) || c->nb >= (1 << 27)) return fail(c, TY_ERR_UNSUPPORTED, "too many cells for the slot link word");
     c->sumA.assign(n, 0.0);
     // mirror slots: where the far cell keeps the same face
     std::vector<unsigned char> slot_own(c->nfi, 0), slot_oth(c->nfi, 0);
@@ -478,6 +483,13 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
             }
             c->nbr[k * n + i] = slots[i][k].first;
             c->fw[k * n + i] = (int)w;
+            const int nbv = slots[i][k].first;
+            if (nbv >= 0) {
+                const unsigned mk = (w >> MIRROR_SHIFT) & (MAX_SLOTS - 1);
+                c->link[k * n + i] = (int)(((w & 0x80000000u) ? (1u << 30) : 0u) | (mk << 27) | (unsigned)nbv);
+            } else {
+                c->link[k * n + i] = (int)(0x80000000u | (unsigned)(-nbv - 1));
+            }
             c->sumA[i] += c->fs[w & FACE_MASK];
         }
     c->cc.assign((size_t)3 * n, 0.0);
@@ -495,7 +507,7 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
     int rc;
     if ((rc = upload(c, &c->d_nbr, c->nbr)) || (rc = upload(c, &c->d_fw, c->fw)) ||
         (rc = upload(c, &c->d_bboco, c->bboco)) || (rc = upload(c, &c->d_vol, c->vol)) ||
-        (rc = upload(c, &c->d_bcell, bcell)))
+        (rc = upload(c, &c->d_bcell, bcell)) || (rc = upload(c, &c->d_link, c->link)))
         return rc;
     c->mesh_bufs = c->owned;  // everything allocated so far belongs to the mesh
     c->mesh_ok = true;
@@ -518,9 +530,9 @@ template <typename R>
 int build_geo(ty_ctx* c, Geo<R>& g)
 {
     const int n = c->n, nf = c->nf, nb = c->nb, dim = c->dim, ns = c->ns;
-    // layout: sn(3 ns n) sa(ns n) sw(3 ns n) sr(3 ns n) sd(3 ns n) fn(3 nf) fs(nf) eps2(n)
-    const size_t S3 = (size_t)3 * ns * n, S1 = (size_t)ns * n;
-    const size_t o_sn = 0, o_sa = o_sn + S3, o_sw = o_sa + S1, o_sr = o_sw + S3, o_sd = o_sr + S3;
+    // layout: ss(3 ns n) sw(3 ns n) sr(3 ns n) sd(3 ns n) fn(3 nf) fs(nf) eps2(n)
+    const size_t S3 = (size_t)3 * ns * n;
+    const size_t o_ss = 0, o_sw = o_ss + S3, o_sr = o_sw + S3, o_sd = o_sr + S3;
     const size_t o_fn = o_sd + S3, o_fs = o_fn + 3 * (size_t)nf, o_eps = o_fs + nf, tot = o_eps + n;
     std::vector<R> h(tot, R(0));
     for (size_t i = 0; i < 3 * (size_t)nf; ++i) h[o_fn + i] = (R)c->fn[i];
@@ -557,11 +569,10 @@ int build_geo(ty_ctx* c, Geo<R>& g)
                     d[e] = c->gc[(size_t)e * nb + (-nbv - 1)] - x[e];
                     r[e] = xf - x[e];
                 }
-                at3(o_sn, k, e, i) = (R)c->fn[(size_t)e * nf + f];
+                at3(o_ss, k, e, i) = (R)(c->fn[(size_t)e * nf + f] * c->fs[f]);
                 at3(o_sr, k, e, i) = (R)r[e];
                 at3(o_sd, k, e, i) = (R)d[e];
             }
-            h[o_sa + (size_t)k * n + i] = (R)c->fs[f];
             double m[3];
             const double s2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
             for (int e = 0; e < 3; ++e) m[e] = wlsq ? d[e] / s2 : d[e];
@@ -594,8 +605,7 @@ int build_geo(ty_ctx* c, Geo<R>& g)
     int rc = dalloc(c, &dp, tot);
     if (rc) return rc;
     HIPCHK(c, hipMemcpy(dp, h.data(), tot * sizeof(R), hipMemcpyHostToDevice));
-    g.sn = dp + o_sn;
-    g.sa = dp + o_sa;
+    g.ss = dp + o_ss;
     g.sw = dp + o_sw;
     g.sr = dp + o_sr;
     g.sd = dp + o_sd;
@@ -819,6 +829,7 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     a.dense = c->dense;
     a.nbr = c->d_nbr;
     a.fw = c->d_fw;
+    a.link = c->d_link;
     a.g = g;
     a.boco = c->d_boco;
     a.vol = c->d_vol;

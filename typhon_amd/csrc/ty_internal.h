@@ -36,8 +36,7 @@ struct BocoDev {
 // d = x_R - x_L is taken in the owner's frame (periodic far sides translated).
 template <typename R>
 struct Geo {
-    const R* sn;    // [ns][3][n] canonical unit normal of the slot's face
-    const R* sa;    // [ns][n] face area
+    const R* ss;    // [ns][3][n] canonical area vector S = A n of the slot's face
     const R* sw;    // [ns][3][n] LSQ weights: grad q_c = sum_k w_k (q_k - q_c)
     const R* sr;    // [ns][3][n] face centre - cell centre, in this cell's frame
     const R* sd;    // [ns][3][n] canonical x_R - x_L (the ghost centre on boundary faces)
@@ -67,7 +66,9 @@ struct KArgs {
     int dense;       // every cell uses all ns slots
     const int* nbr;  // [ns][n] >=0 neighbour cell, <0: -(boundary face + 1)
     const int* fw;   // [ns][n] slot word: face id | periodic << 26 | mirror slot << 27 | other-side << 31;
-                     // -1 unused
+                     // -1 unused (tangent/adjoint kernels)
+    const int* link; // [ns][n] link word of the residual kernels: boundary << 31 | other-side << 30 |
+                     // mirror slot << 27 | neighbour cell or boundary face; -1 unused
     Geo<R> g;
     const BocoDev* boco;
     const double* vol;    // [n]

@@ -408,17 +408,39 @@ __device__ __forceinline__ void load_grad(const R* __restrict__ grad, int c, R* 
     for (int k = 0; k < K; ++k) gv[k] = g[k];
 }
 
+// Slot link word (per slot, per cell): bit 31 boundary face, bit 30 this cell is the far
+// side of an interior face, bits 27-29 the slot where the far cell keeps the same face,
+// bits 0-26 the neighbour cell or the boundary face index; SLOT_UNUSED for an empty slot.
+constexpr int LINK_INDEX = (1 << 27) - 1;
+__device__ __forceinline__ bool link_boundary(int L) { return L < 0; }
+__device__ __forceinline__ bool link_other(int L) { return (L >> 30) & 1; }
+__device__ __forceinline__ int link_index(int L) { return L & LINK_INDEX; }
+__device__ __forceinline__ int link_mirror(int L) { return (L >> 27) & (MAX_SLOTS - 1); }
+
+// unit normal and area of slot k's face from its canonical area vector S = A n
+template <int DIM, typename R>
+__device__ __forceinline__ R slot_normal(const Geo<R>& g, int k, int n, int c, R* nf)
+{
+    R S[DIM];
+    slot_vec<DIM>(g.ss, k, n, c, S);
+    const R A = sqrt(dotd<DIM>(S, S));
+    const R ia = R(1) / A;
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) nf[e] = S[e] * ia;
+    return A;
+}
+
 // The state across slot k: the neighbour's primitive state, or the ghost state of the
 // cell_state boundary map (calcboco_ust.f90:44-105) built from this cell's.
 template <int DIM, typename R>
-__device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int nb, const R* qc, R* qo)
+__device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int L, const R* qc, R* qo)
 {
-    if (nb >= 0) {
-        load_prim<DIM>(a.us, nb, R(a.ph.gamma - 1.0), qo);
+    if (!link_boundary(L)) {
+        load_prim<DIM>(a.us, link_index(L), R(a.ph.gamma - 1.0), qo);
     } else {
         R nf[DIM];
-        slot_vec<DIM>(a.g.sn, k, a.n, c, nf);
-        bc_state<DIM>(a.boco[a.g.bboco[-nb - 1]], nf, qc, qo, a.ph);
+        slot_normal<DIM>(a.g, k, a.n, c, nf);
+        bc_state<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, qc, qo, a.ph);
     }
 }
 
@@ -495,10 +517,10 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
         if (!DENSE && k >= a.ns) break;
-        const int w = a.fw[(size_t)k * n + c];
-        if (!DENSE && w == SLOT_UNUSED) continue;
+        const int L = a.link[(size_t)k * n + c];
+        if (!DENSE && L == SLOT_UNUSED) continue;
         R qo[NV], wk[DIM];
-        far_state<DIM>(a, k, c, a.nbr[(size_t)k * n + c], qc, qo);
+        far_state<DIM>(a, k, c, L, qc, qo);
         slot_vec<DIM>(a.g.sw, k, n, c, wk);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -531,7 +553,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 #pragma unroll
         for (int k = 0; k < NSM; ++k) {
             if (!DENSE && k >= a.ns) break;
-            if (!DENSE && a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
+            if (!DENSE && a.link[(size_t)k * n + c] == SLOT_UNUSED) continue;
             R rk[DIM];
             slot_vec<DIM>(a.g.sr, k, n, c, rk);
 #pragma unroll
@@ -553,7 +575,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 #pragma unroll
         for (int k = 0; k < NSM; ++k) {
             if (!DENSE && k >= a.ns) break;
-            if (!DENSE && a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
+            if (!DENSE && a.link[(size_t)k * n + c] == SLOT_UNUSED) continue;
             R rk[DIM];
             slot_vec<DIM>(a.g.sr, k, n, c, rk);
             R* out = a.qf_out + ((size_t)k * n + c) * NV;
@@ -570,11 +592,11 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
         if (!DENSE && k >= a.ns) break;
-        const int w = a.fw[(size_t)k * n + c];
-        if (!DENSE && w == SLOT_UNUSED) continue;
-        const bool other = w < 0;
+        const int L = a.link[(size_t)k * n + c];
+        if (!DENSE && L == SLOT_UNUSED) continue;
+        const bool other = link_other(L);
         R qo[NV], dk[DIM], rk[DIM];
-        far_state<DIM>(a, k, c, a.nbr[(size_t)k * n + c], qc, qo);
+        far_state<DIM>(a, k, c, L, qc, qo);
         slot_vec<DIM>(a.g.sd, k, n, c, dk);
         slot_vec<DIM>(a.g.sr, k, n, c, rk);
         R* q = qfk[barth ? k : 0];
@@ -599,7 +621,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
         if (!DENSE && k >= a.ns) break;
-        if (!DENSE && a.fw[(size_t)k * n + c] == SLOT_UNUSED) continue;
+        if (!DENSE && a.link[(size_t)k * n + c] == SLOT_UNUSED) continue;
         R* out = a.qf_out + ((size_t)k * n + c) * NV;
 #pragma unroll
         for (int v = 0; v < NV; ++v) out[v] = qc[v] + ratio[v < 2 ? v : 2] * (qfk[barth ? k : 0][v] - qc[v]);
@@ -839,22 +861,20 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
         if (!DENSE && k >= a.ns) break;
-        const int w = a.fw[(size_t)k * n + c];
-        if (!DENSE && w == SLOT_UNUSED) continue;
-        const int nb = a.nbr[(size_t)k * n + c];
-        const bool other = w < 0;
+        const int L = a.link[(size_t)k * n + c];
+        if (!DENSE && L == SLOT_UNUSED) continue;
+        const bool bnd = link_boundary(L);
+        const bool other = link_other(L);
         R nf[DIM];
-        slot_vec<DIM>(a.g.sn, k, n, c, nf);
-        const R A = a.g.sa[(size_t)k * n + c];
+        const R A = slot_normal<DIM>(a.g, k, n, c, nf);
         // the far cell's state (first order, viscous flux)
         R qo[NV];
-        if (!FS || VISC) far_state<DIM>(a, k, c, nb, qc, qo);
+        if (!FS || VISC) far_state<DIM>(a, k, c, L, qc, qo);
         R qLf[NV], qRf[NV];
         if (FS) {
             const R* own = a.qf + ((size_t)k * n + c) * NV;
-            if (nb >= 0) {
-                const int mk = ((unsigned)w >> MIRROR_SHIFT) & (MAX_SLOTS - 1);
-                const R* far = a.qf + ((size_t)mk * n + nb) * NV;
+            if (!bnd) {
+                const R* far = a.qf + ((size_t)link_mirror(L) * n + link_index(L)) * NV;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
                     const R ov = own[v], fv = far[v];
@@ -864,7 +884,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
             } else {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) qLf[v] = own[v];
-                bc_state<DIM>(a.boco[a.g.bboco[-nb - 1]], nf, qLf, qRf, a.ph);
+                bc_state<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, qLf, qRf, a.ph);
             }
         } else {
 #pragma unroll
@@ -886,8 +906,8 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
                 rR[e] = other ? rk[e] : rk[e] - dk[e];
             }
             R go[NV * DIM];
-            if (nb >= 0) load_grad<DIM>(a.grad, nb, go);
-            else bc_grad<DIM>(a.boco[a.g.bboco[-nb - 1]], nf, gcell, go);
+            if (!bnd) load_grad<DIM>(a.grad, link_index(L), go);
+            else bc_grad<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, gcell, go);
             viscous_face<DIM>(a.ph, other ? qo : qc, other ? qc : qo, rL, rR, dk, nf, other ? go : gcell,
                               other ? gcell : go, flux);
         }
