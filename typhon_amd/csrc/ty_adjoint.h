@@ -10,6 +10,9 @@ struct AdArgs {
     int nf, nb;           // device faces, device boundary faces
     const int* bcell;     // [nbcell] cells with at least one boundary face slot (ascending)
     int nbcell;
+    const int* fcell;     // [2][nfi] owner and other cell of each two-sided device face
+    int nfi;
+    double* face;         // [nfi][3 nv] work: owner-side and other-side state adjoints, A F
     // tangent inputs (each may be null = zero direction) and output
     const double* du;     // [nv][n] state direction
     const double* ds;     // [3][nf] area-vector direction per device face (owner orientation)

@@ -11,9 +11,10 @@
 // The adjoint contracts, per slot, the derivative of the face flux with the
 // face's adjoint weight m = sigma A (lambda_c dt/V_c - lambda_nb dt/V_nb):
 // (dR/dU)^T lambda gathered without atomics.  Interior HLLC/HLLCB faces use the
-// hand-written reverse sweep of the flux (one forward + one reverse pass gives
-// the derivative w.r.t. both states and the normal); boundary faces and the
-// other fluxes take one dual-number pass per input direction.  Face-area and inlet-state sensitivities are written
+// hand-written reverse sweep of the flux, once per face (one forward + one reverse
+// pass gives the derivative w.r.t. both states and the normal), written per face
+// and gathered per cell in slot order; boundary faces and the other fluxes take
+// one dual-number pass per input direction.  Face-area and inlet-state sensitivities are written
 // by the face's owner cell, so every output has exactly one writer.
 #include "ty_kernels_impl.h"
 #include "ty_dual.h"
@@ -228,9 +229,8 @@ __device__ __forceinline__ void adj_weight(const AdArgs& t, const SlotInfo& s, d
     }
 }
 
-// REV: interior faces of HLLC/HLLCB by the hand-written reverse sweep, boundary slots left to
-// k_res_adjoint_bnd; otherwise every slot by dual-number passes.
-template <int DIM, bool REV>
+// Every slot by dual-number passes (the fluxes without a hand-written reverse sweep).
+template <int DIM>
 __global__ __launch_bounds__(BLOCK) void k_res_adjoint(AdArgs t)
 {
     constexpr int NV = DIM + 2;
@@ -240,8 +240,7 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint(AdArgs t)
     const int n = a.n;
     const double dt = *a.d_dt;
     const double dtv = dt / a.vol[c];
-    const double g1d = a.ph.gamma - 1.0;
-    double Uc[NV], lc[NV], gu[NV], res[NV], qcp[NV];
+    double Uc[NV], lc[NV], gu[NV], res[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         Uc[v] = a.us[(size_t)v * n + c];
@@ -249,35 +248,16 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint(AdArgs t)
         gu[v] = 0.0;
         res[v] = 0.0;
     }
-    if (REV) cons_to_prim<DIM>(Uc, g1d, qcp);
     for (int k = 0; k < a.ns; ++k) {
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
         const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
-        if (REV && s.nb < 0) continue;
         double nv[DIM];
         face_n<DIM>(a.g, s.f, nv);
         const double A = a.g.fs[s.f];
         double Uo[NV], mv[NV], Fv[NV];
         adj_weight<DIM>(t, s, A, lc, dt, Uo, mv);
-        if (REV) {
-            // reverse sweep: F, (dF/dq)^T m for both sides, (dF/dn)^T m
-            double qo[NV], qLb[NV], qRb[NV], gn[DIM];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                qLb[v] = 0.0;
-                qRb[v] = 0.0;
-            }
-#pragma unroll
-            for (int e = 0; e < DIM; ++e) gn[e] = 0.0;
-            cons_to_prim<DIM>(Uo, g1d, qo);
-            hllc_rev<DIM>(a.ph.sch_hyp, a.ph.gamma, s.other ? qo : qcp, s.other ? qcp : qo, nv, mv, Fv, qLb, qRb,
-                          gn);
-            prim_rev<DIM>(Uc, g1d, s.other ? qRb : qLb, gu);
-            if (!s.other && t.gs) face_sens<DIM>(t, s.f, nv, A, mv, Fv, gn);
-        } else {
-            adj_slot_dual<DIM>(t, c, s, Uc, Uo, nv, A, mv, gu, Fv);
-        }
+        adj_slot_dual<DIM>(t, c, s, Uc, Uo, nv, A, mv, gu, Fv);
         const double sg = s.other ? 1.0 : -1.0;
 #pragma unroll
         for (int v = 0; v < NV; ++v) res[v] += sg * A * Fv[v];
@@ -293,7 +273,98 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint(AdArgs t)
     }
 }
 
-// boundary slots of the cells in t.bcell (after k_res_adjoint<DIM, true>): one thread per cell,
+// Interior two-sided faces of HLLC/HLLCB, one work-item per face: the reverse sweep of the face
+// flux once for both cells (the cell-parallel form evaluates every face twice).  Writes, per face,
+// (dF/dU_owner)^T m and (dF/dU_other)^T m, A F, and the face's area-vector sensitivity, with the
+// face weight m = A (lambda_other dt/V_other - lambda_owner dt/V_owner).
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_res_adjoint_face(AdArgs t)
+{
+    constexpr int NV = DIM + 2;
+    const KArgs<double>& a = t.k;
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= t.nfi) return;
+    const int n = a.n;
+    const int cl = t.fcell[f], cr = t.fcell[(size_t)t.nfi + f];
+    const double dt = *a.d_dt;
+    const double g1d = a.ph.gamma - 1.0;
+    double nv[DIM];
+    face_n<DIM>(a.g, f, nv);
+    const double A = a.g.fs[f];
+    const double kl = dt / a.vol[cl], kr = dt / a.vol[cr];
+    double UL[NV], UR[NV], qL[NV], qR[NV], mv[NV], qLb[NV], qRb[NV], gn[DIM], Fv[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        UL[v] = a.us[(size_t)v * n + cl];
+        UR[v] = a.us[(size_t)v * n + cr];
+        mv[v] = A * (t.lam[(size_t)v * n + cr] * kr - t.lam[(size_t)v * n + cl] * kl);
+        qLb[v] = 0.0;
+        qRb[v] = 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) gn[e] = 0.0;
+    cons_to_prim<DIM>(UL, g1d, qL);
+    cons_to_prim<DIM>(UR, g1d, qR);
+    hllc_rev<DIM>(a.ph.sch_hyp, a.ph.gamma, qL, qR, nv, mv, Fv, qLb, qRb, gn);
+    double gL[NV], gR[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        gL[v] = 0.0;
+        gR[v] = 0.0;
+    }
+    prim_rev<DIM>(UL, g1d, qLb, gL);
+    prim_rev<DIM>(UR, g1d, qRb, gR);
+    double* out = t.face + (size_t)f * 3 * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        out[v] = gL[v];
+        out[NV + v] = gR[v];
+        out[2 * NV + v] = A * Fv[v];
+    }
+    if (t.gs) face_sens<DIM>(t, f, nv, A, mv, Fv, gn);
+}
+
+// per cell: the interior faces' contributions gathered in slot order (boundary slots are left to
+// k_res_adjoint_bnd)
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
+{
+    constexpr int NV = DIM + 2;
+    const KArgs<double>& a = t.k;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const int n = a.n;
+    double gu[NV], res[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        gu[v] = 0.0;
+        res[v] = 0.0;
+    }
+    for (int k = 0; k < a.ns; ++k) {
+        const int w = a.fw[(size_t)k * n + c];
+        if (w == SLOT_UNUSED) continue;
+        if (a.nbr[(size_t)k * n + c] < 0) continue;
+        const SlotInfo s = decode(w, 0);
+        const double* in = t.face + (size_t)s.f * 3 * NV;
+        const double sg = s.other ? 1.0 : -1.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            gu[v] += in[(s.other ? NV : 0) + v];
+            res[v] += sg * in[2 * NV + v];
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] = gu[v];
+    if (t.gvol) {
+        const double dtv = *a.d_dt / a.vol[c];
+        double acc = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc += t.lam[(size_t)v * n + c] * dtv * res[v];
+        t.gvol[c] = -acc / a.vol[c];
+    }
+}
+
+// boundary slots of the cells in t.bcell (after k_res_adjoint_gather): one thread per cell,
 // adds to that cell's gu and gvol, writes the boundary faces' sensitivities
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void k_res_adjoint_bnd(AdArgs t)
@@ -350,10 +421,12 @@ hipError_t launch_res_adjoint(const AdArgs& t, hipStream_t s)
 {
     const bool rev = t.k.ph.sch_hyp == 25 || t.k.ph.sch_hyp == 26;
     if (!rev) {
-        hipLaunchKernelGGL((k_res_adjoint<DIM, false>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+        hipLaunchKernelGGL((k_res_adjoint<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL((k_res_adjoint<DIM, true>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+    if (t.nfi > 0)
+        hipLaunchKernelGGL((k_res_adjoint_face<DIM>), dim3((t.nfi + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, t);
+    hipLaunchKernelGGL((k_res_adjoint_gather<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
     if (t.nbcell > 0)
         hipLaunchKernelGGL((k_res_adjoint_bnd<DIM>), dim3((t.nbcell + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, t);
     return hipGetLastError();

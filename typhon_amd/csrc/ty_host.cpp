@@ -44,6 +44,7 @@ struct ty_ctx {
     std::vector<int> perm, iperm;  // perm[new] = caller interior index
     std::vector<int> nbr, fw;      // [ns][n]
     std::vector<int> link;         // [ns][n] link words of the residual kernels
+    std::vector<int> fcell;        // [2][nfi] owner and other cell of each two-sided device face
     std::vector<double> fn, fc, fs;  // [3][nf], [3][nf], [nf]
     std::vector<int> fsrc;           // [nf] caller face (0-based) whose geometry device face k carries
     int nface_caller = 0;            // faces of the caller's mesh
@@ -68,6 +69,7 @@ struct ty_ctx {
     int* d_nbr = nullptr;
     int* d_fw = nullptr;
     int* d_link = nullptr;
+    int* d_fcell = nullptr;
     void* d_geo = nullptr;  // one allocation for the geometry of the active precision
     Geo<double> geo_d{};
     Geo<float> geo_f{};
@@ -130,6 +132,7 @@ void free_state(ty_ctx* c)
     c->d_nbr = nullptr;
     c->d_fw = nullptr;
     c->d_link = nullptr;
+    c->d_fcell = nullptr;
     c->d_geo = nullptr;
     c->d_bboco = nullptr;
     c->d_bcell = nullptr;
@@ -431,10 +434,13 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         c->fs[df] = F(src, 6);
         c->fsrc[df] = src;
     };
+    c->fcell.assign((size_t)2 * std::max(1, c->nfi), 0);
     for (int k = 0; k < c->nfi; ++k) {
         const bool per = k >= c->per_base;
         const DF& d = per ? pfaces[k - c->per_base] : faces[k];
         put_face(k, d.src);
+        c->fcell[k] = d.a;
+        c->fcell[(size_t)c->nfi + k] = d.b;
         unsigned w = (unsigned)k;
         if (per) {
             // far side seen through this face: x_far + (c_face - c_partner) (init_connect_grid.f90:226-231)
@@ -507,7 +513,8 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
     int rc;
     if ((rc = upload(c, &c->d_nbr, c->nbr)) || (rc = upload(c, &c->d_fw, c->fw)) ||
         (rc = upload(c, &c->d_bboco, c->bboco)) || (rc = upload(c, &c->d_vol, c->vol)) ||
-        (rc = upload(c, &c->d_bcell, bcell)) || (rc = upload(c, &c->d_link, c->link)))
+        (rc = upload(c, &c->d_bcell, bcell)) || (rc = upload(c, &c->d_link, c->link)) ||
+        (rc = upload(c, &c->d_fcell, c->fcell)))
         return rc;
     c->mesh_bufs = c->owned;  // everything allocated so far belongs to the mesh
     c->mesh_ok = true;
@@ -1534,6 +1541,8 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t)
     t.nb = c->nb;
     t.bcell = c->d_bcell;
     t.nbcell = c->nbcell;
+    t.fcell = c->d_fcell;
+    t.nfi = c->nfi;
     return TY_OK;
 }
 
@@ -1609,6 +1618,9 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
     if (gs && (rc = tmp.alloc(&d_gs, (size_t)3 * nf))) return rc;
     if (gvol && (rc = tmp.alloc(&d_gv, (size_t)n))) return rc;
     if (gq && (rc = tmp.alloc(&d_gq, (size_t)nv * std::max(1, nb)))) return rc;
+    double* d_face = nullptr;  // per two-sided face: state adjoints of both sides and A F
+    if ((rc = tmp.alloc(&d_face, (size_t)3 * nv * std::max(1, c->nfi)))) return rc;
+    t.face = d_face;
     t.lam = d_lam;
     t.gu = d_gu;
     t.gs = d_gs;
