@@ -8,11 +8,10 @@ namespace ty {
 struct AdArgs {
     KArgs<double> k;      // mesh, geometry, boundary conditions, physics; k.us = FP64 state [nv][n]
     int nf, nb;           // device faces, device boundary faces
-    const int* bcell;     // [nbcell] cells with at least one boundary face slot (ascending)
-    int nbcell;
-    const int* fcell;     // [2][nfi] owner and other cell of each two-sided device face
-    int nfi;
-    double* face;         // [nfi][3 nv] work: owner-side and other-side state adjoints, A F
+    const int* fcell;     // [2][nf] owner and other cell of each device face (other = -1 on boundary faces)
+    int nfi;              // two-sided faces come first: device faces [0, nfi)
+    double* face;         // [nf][2][nv + 1] work: per face side, that cell's state adjoint and the
+                          // side's share of its volume sensitivity
     // tangent inputs (each may be null = zero direction) and output
     const double* du;     // [nv][n] state direction
     const double* ds;     // [3][nf] area-vector direction per device face (owner orientation)

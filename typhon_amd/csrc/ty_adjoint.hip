@@ -4,18 +4,17 @@
 // the interior cells, the area vector S = A n of every face, the cell volumes and
 // the ghost primitive state of the supersonic inlets.
 //
-// Both kernels keep the primal layout: one work-item per cell walking its face
-// slots, every face flux evaluated in the face's canonical orientation, a
-// deterministic per-cell sum.  The face derivatives come from the flux and
-// boundary templates instantiated with dual numbers (value + one derivative).
-// The adjoint contracts, per slot, the derivative of the face flux with the
-// face's adjoint weight m = sigma A (lambda_c dt/V_c - lambda_nb dt/V_nb):
-// (dR/dU)^T lambda gathered without atomics.  Interior HLLC/HLLCB faces use the
-// hand-written reverse sweep of the flux, once per face (one forward + one reverse
-// pass gives the derivative w.r.t. both states and the normal), written per face
-// and gathered per cell in slot order; boundary faces and the other fluxes take
-// one dual-number pass per input direction.  Face-area and inlet-state sensitivities are written
-// by the face's owner cell, so every output has exactly one writer.
+// The tangent keeps the primal layout: one work-item per cell walking its face
+// slots, every face flux evaluated in the face's canonical orientation with dual
+// numbers (value + one derivative), a deterministic per-cell sum.  The adjoint
+// contracts the derivative of each face flux with the face's adjoint weight
+// m = sigma A (lambda_c dt/V_c - lambda_nb dt/V_nb).  For HLLC/HLLCB it runs one
+// work-item per device face through the hand-written reverse sweep of the flux
+// (one forward + one reverse pass gives the derivative w.r.t. both states and the
+// normal; boundary ghost states are chained through their own Jacobian), writes
+// one record per face side, and a per-cell kernel sums the records in slot order:
+// (dR/dU)^T lambda without atomics.  The other fluxes take one dual-number pass
+// per input direction inside a per-cell kernel.  Every output has one writer.
 #include "ty_kernels_impl.h"
 #include "ty_dual.h"
 #include "ty_hllc_rev.h"
@@ -273,140 +272,129 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint(AdArgs t)
     }
 }
 
-// Interior two-sided faces of HLLC/HLLCB, one work-item per face: the reverse sweep of the face
-// flux once for both cells (the cell-parallel form evaluates every face twice).  Writes, per face,
-// (dF/dU_owner)^T m and (dF/dU_other)^T m, A F, and the face's area-vector sensitivity, with the
-// face weight m = A (lambda_other dt/V_other - lambda_owner dt/V_owner).
+// HLLC/HLLCB faces, one work-item per device face: one forward and one reverse sweep of the face
+// flux (the cell-parallel form would evaluate every two-sided face twice).  The face weight is
+// m = A (lambda_other dt/V_other - lambda_owner dt/V_owner), and m = -A lambda_owner dt/V_owner on a
+// boundary face, whose ghost state qR = B(qL, n) is chained through the transpose of B's Jacobian
+// (dual-number passes of the boundary template, cheap next to the flux).  Each face side gets a
+// record [(dF/dU_side)^T m, sigma A lambda_side dt/V_side . F] (sigma = -1 owner, +1 other) for
+// k_res_adjoint_gather; the face's area-vector sensitivity and a boundary face's inlet-state
+// sensitivity are written here.
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void k_res_adjoint_face(AdArgs t)
 {
-    constexpr int NV = DIM + 2;
+    constexpr int NV = DIM + 2, RW = NV + 1;
     const KArgs<double>& a = t.k;
     const int f = blockIdx.x * BLOCK + threadIdx.x;
-    if (f >= t.nfi) return;
+    if (f >= t.nf) return;
     const int n = a.n;
-    const int cl = t.fcell[f], cr = t.fcell[(size_t)t.nfi + f];
+    const bool bnd = f >= t.nfi;
+    const int cl = t.fcell[f], cr = bnd ? cl : t.fcell[(size_t)t.nf + f];
     const double dt = *a.d_dt;
     const double g1d = a.ph.gamma - 1.0;
     double nv[DIM];
     face_n<DIM>(a.g, f, nv);
     const double A = a.g.fs[f];
-    const double kl = dt / a.vol[cl], kr = dt / a.vol[cr];
-    double UL[NV], UR[NV], qL[NV], qR[NV], mv[NV], qLb[NV], qRb[NV], gn[DIM], Fv[NV];
+    const double kl = dt / a.vol[cl], kr = bnd ? 0.0 : dt / a.vol[cr];
+    double UL[NV], qL[NV], qR[NV], mv[NV], qLb[NV], qRb[NV], gn[DIM], Fv[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         UL[v] = a.us[(size_t)v * n + cl];
-        UR[v] = a.us[(size_t)v * n + cr];
-        mv[v] = A * (t.lam[(size_t)v * n + cr] * kr - t.lam[(size_t)v * n + cl] * kl);
+        mv[v] = A * ((bnd ? 0.0 : t.lam[(size_t)v * n + cr] * kr) - t.lam[(size_t)v * n + cl] * kl);
         qLb[v] = 0.0;
         qRb[v] = 0.0;
     }
 #pragma unroll
     for (int e = 0; e < DIM; ++e) gn[e] = 0.0;
     cons_to_prim<DIM>(UL, g1d, qL);
-    cons_to_prim<DIM>(UR, g1d, qR);
+    BocoDev bc;
+    if (bnd) {
+        bc = a.boco[a.g.bboco[f - t.nfi]];
+        bc_state<DIM>(bc, nv, qL, qR, a.ph);
+    } else {
+        double UR[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) UR[v] = a.us[(size_t)v * n + cr];
+        cons_to_prim<DIM>(UR, g1d, qR);
+    }
     hllc_rev<DIM>(a.ph.sch_hyp, a.ph.gamma, qL, qR, nv, mv, Fv, qLb, qRb, gn);
-    double gL[NV], gR[NV];
+    double* out = t.face + (size_t)f * 2 * RW;
+    double sL = 0.0;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        gL[v] = 0.0;
-        gR[v] = 0.0;
+    for (int v = 0; v < NV; ++v) sL += t.lam[(size_t)v * n + cl] * Fv[v];
+    out[NV] = -A * kl * sL;
+    if (bnd) {
+        const int bi = f - t.nfi;
+        const bool inlet = bc.kind == BK_INLET_SUP;
+        if (t.gq) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) t.gq[(size_t)v * t.nb + bi] = inlet ? qRb[v] : 0.0;
+        }
+        if (!inlet) {
+            // qLb += (dB/dqL)^T qRb, gn += (dB/dn)^T qRb
+            for (int e = 0; e < NV + DIM; ++e) {
+                Dual qd[NV], nd[DIM], gd[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qd[v] = Dual(qL[v], v == e ? 1.0 : 0.0);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) nd[d] = Dual(nv[d], NV + d == e ? 1.0 : 0.0);
+                bc_state<DIM>(bc, nd, qd, gd, a.ph);
+                double sacc = 0.0;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sacc += gd[v].d * qRb[v];
+                if (e < NV)
+                    qLb[e] += sacc;
+                else
+                    gn[e - NV] += sacc;
+            }
+        }
+    } else {
+        double UR[NV], gR[NV], sR = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            UR[v] = a.us[(size_t)v * n + cr];
+            gR[v] = 0.0;
+            sR += t.lam[(size_t)v * n + cr] * Fv[v];
+        }
+        prim_rev<DIM>(UR, g1d, qRb, gR);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out[RW + v] = gR[v];
+        out[RW + NV] = A * kr * sR;
     }
+    double gL[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) gL[v] = 0.0;
     prim_rev<DIM>(UL, g1d, qLb, gL);
-    prim_rev<DIM>(UR, g1d, qRb, gR);
-    double* out = t.face + (size_t)f * 3 * NV;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        out[v] = gL[v];
-        out[NV + v] = gR[v];
-        out[2 * NV + v] = A * Fv[v];
-    }
+    for (int v = 0; v < NV; ++v) out[v] = gL[v];
     if (t.gs) face_sens<DIM>(t, f, nv, A, mv, Fv, gn);
 }
 
-// per cell: the interior faces' contributions gathered in slot order (boundary slots are left to
-// k_res_adjoint_bnd)
+// per cell: its face sides' records summed in slot order (one writer per output, no atomics)
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
 {
-    constexpr int NV = DIM + 2;
+    constexpr int NV = DIM + 2, RW = NV + 1;
     const KArgs<double>& a = t.k;
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n) return;
     const int n = a.n;
-    double gu[NV], res[NV];
+    double gu[NV], sv = 0.0;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        gu[v] = 0.0;
-        res[v] = 0.0;
-    }
+    for (int v = 0; v < NV; ++v) gu[v] = 0.0;
     for (int k = 0; k < a.ns; ++k) {
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
-        if (a.nbr[(size_t)k * n + c] < 0) continue;
         const SlotInfo s = decode(w, 0);
-        const double* in = t.face + (size_t)s.f * 3 * NV;
-        const double sg = s.other ? 1.0 : -1.0;
+        const double* in = t.face + ((size_t)s.f * 2 + (s.other ? 1 : 0)) * RW;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            gu[v] += in[(s.other ? NV : 0) + v];
-            res[v] += sg * in[2 * NV + v];
-        }
+        for (int v = 0; v < NV; ++v) gu[v] += in[v];
+        sv += in[NV];
     }
 #pragma unroll
     for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] = gu[v];
-    if (t.gvol) {
-        const double dtv = *a.d_dt / a.vol[c];
-        double acc = 0.0;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc += t.lam[(size_t)v * n + c] * dtv * res[v];
-        t.gvol[c] = -acc / a.vol[c];
-    }
-}
-
-// boundary slots of the cells in t.bcell (after k_res_adjoint_gather): one thread per cell,
-// adds to that cell's gu and gvol, writes the boundary faces' sensitivities
-template <int DIM>
-__global__ __launch_bounds__(BLOCK) void k_res_adjoint_bnd(AdArgs t)
-{
-    constexpr int NV = DIM + 2;
-    const KArgs<double>& a = t.k;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= t.nbcell) return;
-    const int c = t.bcell[i];
-    const int n = a.n;
-    const double dt = *a.d_dt;
-    const double dtv = dt / a.vol[c];
-    double Uc[NV], lc[NV], gu[NV], res[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        Uc[v] = a.us[(size_t)v * n + c];
-        lc[v] = t.lam[(size_t)v * n + c] * dtv;
-        gu[v] = 0.0;
-        res[v] = 0.0;
-    }
-    for (int k = 0; k < a.ns; ++k) {
-        const int w = a.fw[(size_t)k * n + c];
-        if (w == SLOT_UNUSED) continue;
-        const SlotInfo s = decode(w, a.nbr[(size_t)k * n + c]);
-        if (s.nb >= 0) continue;
-        double nv[DIM];
-        face_n<DIM>(a.g, s.f, nv);
-        const double A = a.g.fs[s.f];
-        double Uo[NV], mv[NV], Fv[NV];
-        adj_weight<DIM>(t, s, A, lc, dt, Uo, mv);
-        adj_slot_dual<DIM>(t, c, s, Uc, Uo, nv, A, mv, gu, Fv);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) res[v] -= A * Fv[v];
-    }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] += gu[v];
-    if (t.gvol) {
-        double acc = 0.0;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc += lc[v] * res[v];
-        t.gvol[c] -= acc / a.vol[c];
-    }
+    // d(lambda.R_c)/dV_c = -lambda_c . R_c / V_c
+    if (t.gvol) t.gvol[c] = -sv / a.vol[c];
 }
 
 template <int DIM>
@@ -424,11 +412,8 @@ hipError_t launch_res_adjoint(const AdArgs& t, hipStream_t s)
         hipLaunchKernelGGL((k_res_adjoint<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
         return hipGetLastError();
     }
-    if (t.nfi > 0)
-        hipLaunchKernelGGL((k_res_adjoint_face<DIM>), dim3((t.nfi + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, t);
+    hipLaunchKernelGGL((k_res_adjoint_face<DIM>), dim3((t.nf + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, t);
     hipLaunchKernelGGL((k_res_adjoint_gather<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
-    if (t.nbcell > 0)
-        hipLaunchKernelGGL((k_res_adjoint_bnd<DIM>), dim3((t.nbcell + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, t);
     return hipGetLastError();
 }
 

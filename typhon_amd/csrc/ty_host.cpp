@@ -44,7 +44,7 @@ struct ty_ctx {
     std::vector<int> perm, iperm;  // perm[new] = caller interior index
     std::vector<int> nbr, fw;      // [ns][n]
     std::vector<int> link;         // [ns][n] link words of the residual kernels
-    std::vector<int> fcell;        // [2][nfi] owner and other cell of each two-sided device face
+    std::vector<int> fcell;        // [2][nf] owner and other cell of each device face (-1: boundary)
     std::vector<double> fn, fc, fs;  // [3][nf], [3][nf], [nf]
     std::vector<int> fsrc;           // [nf] caller face (0-based) whose geometry device face k carries
     int nface_caller = 0;            // faces of the caller's mesh
@@ -74,8 +74,6 @@ struct ty_ctx {
     Geo<double> geo_d{};
     Geo<float> geo_f{};
     int* d_bboco = nullptr;
-    int* d_bcell = nullptr;  // cells owning a boundary face (ascending)
-    int nbcell = 0;
     BocoDev* d_boco = nullptr;
     double* d_vol = nullptr;
     double* d_dtfac = nullptr;
@@ -135,8 +133,6 @@ void free_state(ty_ctx* c)
     c->d_fcell = nullptr;
     c->d_geo = nullptr;
     c->d_bboco = nullptr;
-    c->d_bcell = nullptr;
-    c->nbcell = 0;
     c->d_boco = nullptr;
     c->d_vol = c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
     c->d_us[0] = c->d_us[1] = nullptr;
@@ -434,13 +430,13 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         c->fs[df] = F(src, 6);
         c->fsrc[df] = src;
     };
-    c->fcell.assign((size_t)2 * std::max(1, c->nfi), 0);
+    c->fcell.assign((size_t)2 * ndf, -1);
     for (int k = 0; k < c->nfi; ++k) {
         const bool per = k >= c->per_base;
         const DF& d = per ? pfaces[k - c->per_base] : faces[k];
         put_face(k, d.src);
         c->fcell[k] = d.a;
-        c->fcell[(size_t)c->nfi + k] = d.b;
+        c->fcell[(size_t)ndf + k] = d.b;
         unsigned w = (unsigned)k;
         if (per) {
             // far side seen through this face: x_far + (c_face - c_partner) (init_connect_grid.f90:226-231)
@@ -455,6 +451,7 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         const DF& d = bfaces[k];
         int df = c->nfi + k;
         put_face(df, d.src);
+        c->fcell[df] = d.a;
         int g = ic2[d.src];
         for (int e = 0; e < 3; ++e) c->gc[(size_t)e * c->nb + k] = m->centre[(size_t)3 * g + e];
         c->bboco[k] = d.bidx;
@@ -505,15 +502,10 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         for (int d = 0; d < 3; ++d) c->cc[(size_t)d * n + i] = m->centre[(size_t)3 * o + d];
         c->vol[i] = m->volume[o];
     }
-    std::vector<int> bcell;
-    for (int k = 0; k < c->nb; ++k) bcell.push_back(bfaces[k].a);
-    std::sort(bcell.begin(), bcell.end());
-    bcell.erase(std::unique(bcell.begin(), bcell.end()), bcell.end());
-    c->nbcell = (int)bcell.size();
     int rc;
     if ((rc = upload(c, &c->d_nbr, c->nbr)) || (rc = upload(c, &c->d_fw, c->fw)) ||
         (rc = upload(c, &c->d_bboco, c->bboco)) || (rc = upload(c, &c->d_vol, c->vol)) ||
-        (rc = upload(c, &c->d_bcell, bcell)) || (rc = upload(c, &c->d_link, c->link)) ||
+        (rc = upload(c, &c->d_link, c->link)) ||
         (rc = upload(c, &c->d_fcell, c->fcell)))
         return rc;
     c->mesh_bufs = c->owned;  // everything allocated so far belongs to the mesh
@@ -1539,8 +1531,6 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t)
     t.k.us = c->d_u0;
     t.nf = c->nf;
     t.nb = c->nb;
-    t.bcell = c->d_bcell;
-    t.nbcell = c->nbcell;
     t.fcell = c->d_fcell;
     t.nfi = c->nfi;
     return TY_OK;
@@ -1618,8 +1608,8 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
     if (gs && (rc = tmp.alloc(&d_gs, (size_t)3 * nf))) return rc;
     if (gvol && (rc = tmp.alloc(&d_gv, (size_t)n))) return rc;
     if (gq && (rc = tmp.alloc(&d_gq, (size_t)nv * std::max(1, nb)))) return rc;
-    double* d_face = nullptr;  // per two-sided face: state adjoints of both sides and A F
-    if ((rc = tmp.alloc(&d_face, (size_t)3 * nv * std::max(1, c->nfi)))) return rc;
+    double* d_face = nullptr;  // per face side: state adjoint and volume-sensitivity share
+    if ((rc = tmp.alloc(&d_face, (size_t)2 * (nv + 1) * nf))) return rc;
     t.face = d_face;
     t.lam = d_lam;
     t.gu = d_gu;
