@@ -92,11 +92,11 @@ def cpu_baseline(n_cpu, budget_s=20.0):
 
 PROFILE_CMDS = {
     "trace": "rocprofv3 --kernel-trace --stats -f csv -d {d}/trace -o c2 -- python3 bench.py --steps 20 --warmup 2 "
-             "--no-cpu --no-mixed --no-adjoint --no-implicit",
+             "--no-cpu --no-mixed --no-adjoint --no-implicit --no-ns",
     "fetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/fetch -o c2 -- python3 bench.py --steps 2 --warmup 1 --no-cpu "
-             "--no-mixed --no-adjoint --no-implicit",
+             "--no-mixed --no-adjoint --no-implicit --no-ns",
     "write": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/write -o c2 -- python3 bench.py --steps 2 --warmup 1 --no-cpu "
-             "--no-mixed --no-adjoint --no-implicit",
+             "--no-mixed --no-adjoint --no-implicit --no-ns",
     "cfetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/cfetch -o copy -- python3 bench.py --copy-calibration 1024",
     "cwrite": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/cwrite -o copy -- python3 bench.py --copy-calibration 1024",
 }
@@ -214,6 +214,100 @@ def adjoint_ratio(reps=20):
                 adjoint_kernel_ms=round(adj, 4), ratio=round(adj / prim, 3))
 
 
+def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
+    """The 16M-cell unsteady NS workload: compressible Taylor-Green field on n^3 hexes (Re 1e4,
+    Sutherland, Pr 0.72, 1% seeded noise), periodic in x and z with slip walls at y = 0 and
+    y = 2 pi L (the field's v vanishes there), first-order HLLC + cell-full viscous flux with LSQ
+    gradients, RK4 at CFL 0.5, one uploaded mesh for everything below.
+
+    - throughput: FP64 and mixed (FP32 flux/gradient/stage state, FP64 U0 and RK accumulator)
+      advance the same initial field by ``steps`` steps; density error of mixed vs FP64;
+    - adjoint: J = pressure force normal to the y = 0 wall after ``adj_steps`` steps (mixed
+      forward), dJ/dU0 by the device-resident checkpointed sweep (``snaps`` snapshots, FP64
+      reverse), checked against a central difference of two FP64 forward runs along a seeded
+      direction."""
+    import torch
+    import cases as K
+    from typhon_amd import adjoint as A
+    from typhon_amd import scheme as S
+    from typhon_amd.solver import Solver
+    t0 = time.perf_counter()
+    m, s, g, u = K.tgv_case(n, walls=True)
+    tmesh = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    solver = Solver(m, s, g)
+    tup = time.perf_counter() - t0
+    nc = m.ncell_int
+    out = dict(workload=f"3D NS Taylor-Green, {n}^3 = {nc} hexes (periodic x/z, slip walls y), first-order HLLC + "
+                        f"viscous (cell-full), LSQ gradients, RK4, CFL 0.5", cells=nc, steps=steps,
+               host_setup_s=dict(mesh_build=round(tmesh, 1), upload=round(tup, 1)))
+    states = {}
+    dt = None
+    for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
+        sc = dict(s)
+        sc["precision"] = prec
+        solver.set_scheme(sc, g)
+        solver.set_state(u)
+        if dt is None:
+            dt = solver.timestep()
+        for _ in range(warmup):
+            solver.rk_step(dt)
+        solver.set_state(u)
+        solver.sync()
+        solver.timing(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            solver.rk_step(dt)
+        solver.sync()
+        el = time.perf_counter() - t0
+        tm = solver.timing(False)
+        states[name] = solver.get_state()[0]
+        out[name] = dict(value=round(nc * 4 * steps / el / 1e6, 1), unit="Mcell-updates/s",
+                         ms_per_step=round(el / steps * 1e3, 3),
+                         grad_ms_avg=round(tm["grad_ms"] / max(1, tm["grad_n"]), 4),
+                         flux_ms_avg=round(tm["flux_ms"] / max(1, tm["flux_n"]), 4))
+        print(f"# ns {name}: {out[name]}", file=sys.stderr, flush=True)
+    r64, rmx = states.pop("fp64"), states.pop("mixed")
+    out["speedup_mixed_vs_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
+    out["density_rel_err_l2"] = float(np.sqrt(np.mean((rmx - r64) ** 2)) / np.sqrt(np.mean(r64 ** 2)))
+    out["density_rel_err_linf"] = float(np.abs(rmx - r64).max() / np.abs(r64).max())
+    out["dt"] = dt
+    del r64, rmx
+    if adj_steps > 0:
+        dts = np.full(adj_steps, dt)
+        obj = lambda U, Sx: A.wall_force(m, U, Sx, "JMIN", axis=1)  # noqa: E731
+        solver.set_scheme(dict(s, precision=S.PREC_MIXED), g)
+        t0 = time.perf_counter()
+        J, gU0, _, st = A.unsteady_gradient_device(solver, m, u, dts, obj, snaps=snaps)
+        el = time.perf_counter() - t0
+        # central difference of J along a seeded direction, FP64 forward runs
+        solver.set_scheme(dict(s, precision=S.PREC_FP64), g)
+        rng = np.random.default_rng(12)
+        d = rng.normal(size=u.shape) * np.abs(u).max(axis=1, keepdims=True)
+        h = 1e-6
+        Sx = torch.as_tensor(m.fnorm * m.fsurf[:, None])
+        Jpm = []
+        t1 = time.perf_counter()
+        for sgn in (1.0, -1.0):
+            solver.set_state(u + sgn * h * d)
+            for k in range(adj_steps):
+                solver.rk_step(dt)
+            Jpm.append(float(obj(torch.as_tensor(solver.get_state()), Sx)))
+        tfd = time.perf_counter() - t1
+        fd = (Jpm[0] - Jpm[1]) / (2 * h)
+        ad = float((gU0 * d).sum())
+        out["adjoint"] = dict(objective="pressure force normal to the y = 0 wall", steps=adj_steps, snapshots=snaps,
+                              forward_precision="mixed", reverse_precision="fp64", J=J,
+                              total_s=round(el, 2), device_sweep_s=round(st["ms"] / 1e3, 2),
+                              forward_steps_in_sweep=st["forward_steps"], max_snapshots=st["max_snapshots"],
+                              snapshot_gb=round(snaps * 5 * nc * 8 / 1e9, 1),
+                              fd_check=dict(direction="seeded normal x field scale", h=h, dJ_adjoint=ad, dJ_fd=fd,
+                                            rel_diff=abs(ad - fd) / max(abs(fd), 1e-300), fd_s=round(tfd, 1)))
+        print(f"# ns adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
+    solver.close()
+    return out
+
+
 def implicit_bench(n=160, dt_mult=10.0, steps=2):
     """The implicit workload: compressible NS Taylor-Green field on n^3 periodic hexes (Re 1e4,
     Sutherland, Pr 0.72), first-order HLLC + cell-full viscous flux; one linearised backward-Euler
@@ -264,6 +358,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-implicit", action="store_true")
+    ap.add_argument("--no-ns", action="store_true", help="skip the 16M-cell NS FP64/mixed section")
+    ap.add_argument("--ns-n", type=int, default=256, help="hexes per edge of the NS section's box")
+    ap.add_argument("--ns-steps", type=int, default=1000)
+    ap.add_argument("--ns-adj-steps", type=int, default=200, help="RK steps of the NS adjoint (0: skip)")
     ap.add_argument("--order", default="hilbert", choices=["hilbert", "rcm", "none"],
                     help="device cell order set at mesh upload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -361,6 +459,11 @@ def main():
             out["implicit"] = implicit_bench()
         except Exception as e:  # reported, not fatal: the headline line must still print
             out["implicit"] = {"error": str(e)}
+    if not args.no_ns:
+        try:
+            out["ns_16m"] = ns_bench(args.ns_n, args.ns_steps, adj_steps=args.ns_adj_steps)
+        except Exception as e:  # reported, not fatal
+            out["ns_16m"] = {"error": str(e)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
     print(json.dumps(out), flush=True)

@@ -70,6 +70,11 @@ type, bind(C) :: ty_krylov_stats
   real(c_double)     :: rel_res, rhs_norm
 endtype ty_krylov_stats
 
+type, bind(C) :: ty_unsteady_stats
+  integer(c_int32_t) :: forward_steps, adjoint_steps, max_snapshots
+  real(c_double)     :: ms
+endtype ty_unsteady_stats
+
 interface
 
   integer(c_int) function ty_create(ctx, device) bind(C, name="ty_create")
@@ -206,6 +211,16 @@ interface
     real(c_double), value :: dt
     real(c_double)        :: lam(*), gu(*)
     type(c_ptr), value    :: gs, gvol, gq
+  endfunction
+
+  integer(c_int) function ty_unsteady_adjoint(ctx, nsteps, dts, snaps, lamN, lam0, gq, stats) &
+      bind(C, name="ty_unsteady_adjoint")
+    import :: c_int, c_int32_t, c_ptr, c_double, ty_unsteady_stats
+    type(c_ptr), value          :: ctx
+    integer(c_int32_t), value   :: nsteps, snaps
+    real(c_double)              :: dts(*), lamN(*), lam0(*)
+    type(c_ptr), value          :: gq
+    type(ty_unsteady_stats)     :: stats
   endfunction
 
 endinterface

@@ -180,15 +180,38 @@ int ty_implicit_step(ty_ctx* ctx, double dt, const ty_krylov* kry, ty_krylov_sta
  * ds: [nface][3] direction of each face's area vector S = surface * normale (caller faces);
  * dvol: [ncell_int] cell volumes; dq: [nboco][5] ghost primitive state (rho, p, u, v, w) of
  * supersonic inlets (other bocos ignored).  Any input may be NULL (zero direction).
- * TY_ERR_UNSUPPORTED unless the scheme is first order and inviscid. */
+ * Viscous (Navier-Stokes) residuals are covered for du, dvol and dq; their flux also reads
+ * cell and face centres, so ds must be NULL for them.
+ * TY_ERR_UNSUPPORTED unless the scheme is first order. */
 int ty_residual_tangent(ty_ctx* ctx, double dt, const double* du, const double* ds, const double* dvol,
                         const double* dq, double* dres);
 /* Adjoint residual (no reference counterpart): the transpose of ty_residual_tangent,
  * gu = (dR/dU)^T lam, gs = (dR/dS)^T lam, gvol = (dR/dV)^T lam, gq = (dR/dq)^T lam, same
  * layouts; gs, gvol and gq may be NULL.  Far faces of merged periodic pairs get zero in gs
- * (the device uses the geometry of the pair's first face). */
+ * (the device uses the geometry of the pair's first face).  Viscous residuals: HLLC/HLLCB
+ * fluxes, gs must be NULL. */
 int ty_residual_adjoint(ty_ctx* ctx, double dt, const double* lam, double* gu, double* gs, double* gvol,
                         double* gq);
+
+typedef struct ty_unsteady_stats {
+    int32_t forward_steps;  /* RK steps advanced for checkpoints and recomputation */
+    int32_t adjoint_steps;  /* RK steps taken in reverse (= nsteps) */
+    int32_t max_snapshots;  /* checkpoint states held at once (besides the initial state) */
+    double ms;              /* wall time of the call */
+} ty_unsteady_stats;
+
+/* Unsteady adjoint (no reference counterpart): the gradient of an objective J(U_N) of the state
+ * after nsteps RK steps of the current scheme (tableau of ty_set_scheme, treelevel_rungekutta.f90:
+ * 50-113) from the current state U_0 with the time steps dts[nsteps], given lamN = dJ/dU_N
+ * ([5][ncell_int], caller order).  Returns lam0 = dJ/dU_0 (same layout) and, when gq is not
+ * NULL, dJ/dq ([nboco][5], supersonic-inlet ghost states (rho, p, u, v, w), summed over all
+ * stages and steps).  Checkpoints live in device memory: at most `snaps` states besides U_0,
+ * placed by binomial checkpointing (Griewank's treeverse), so the forward work grows as
+ * nsteps * r with C(snaps + r, snaps) >= nsteps.  Forward steps run in the context's precision;
+ * each reversed step recomputes its stage states and residual adjoints in FP64.  The state is
+ * U_0 again on return.  Same scheme limits as ty_residual_adjoint. */
+int ty_unsteady_adjoint(ty_ctx* ctx, int32_t nsteps, const double* dts, int32_t snaps, const double* lamN,
+                        double* lam0, double* gq, ty_unsteady_stats* stats);
 /* per-kernel timing for benchmarks (no reference counterpart): when enabled, every
  * gradient, flux, time-step, tangent and adjoint launch is bracketed by HIP events on the
  * context's stream.  Returns the summed milliseconds and launch counts recorded since the

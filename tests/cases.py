@@ -86,9 +86,11 @@ def periodic_quad_case(nx=20, ny=20, jitter=0.0, seed=1, **scheme):
     return m, S.default_scheme(**sch), S.default_gas(), cons
 
 
-def tgv_case(n=16, seed=3, mach=0.3, re=1e4, noise=0.01, jitter=0.0, **scheme):
+def tgv_case(n=16, seed=3, mach=0.3, re=1e4, noise=0.01, jitter=0.0, walls=False, **scheme):
     """3D compressible Navier-Stokes Taylor-Green field on a periodic (2 pi L)^3 hex box, air with
-    Sutherland's law (T0 = 300 K, p0 = 1e5 Pa), Re = rho0 U0 L / mu(T0), Pr = 0.72, 1% seeded noise."""
+    Sutherland's law (T0 = 300 K, p0 = 1e5 Pa), Re = rho0 U0 L / mu(T0), Pr = 0.72, 1% seeded noise.
+    walls: slip walls at y = 0 and y = 2 pi L instead of periodicity in y (the field's v vanishes
+    there)."""
     gamma, r = GAMMA, R_AIR
     T0, p0 = 300.0, 1e5
     rho0 = p0 / (r * T0)
@@ -97,7 +99,7 @@ def tgv_case(n=16, seed=3, mach=0.3, re=1e4, noise=0.01, jitter=0.0, **scheme):
     mu0, tref, ts = 1.68e-5, 273.0, 110.5
     mu = mu0 * (T0 / tref) ** 1.5 * (tref + ts) / (T0 + ts)
     Lr = re * mu / (rho0 * U0)
-    m = M.box_hex(n, 2 * np.pi * Lr, jitter, seed, periodic=True)
+    m = M.box_hex(n, 2 * np.pi * Lr, jitter, seed, periodic=(True, not walls, True))
     x = m.ccent[:m.ncell_int] / Lr
     ni = m.ncell_int
     ids = np.arange(ni)

@@ -89,6 +89,8 @@ def test_torch_geometry_derivatives_match_differences():
 class LinearSolver:
     """stand-in for Solver with R(U) = dt * (M U + b(q)) + geometry terms, for the plumbing"""
 
+    viscous = False
+
     def __init__(self, n, rng):
         self.n = n
         self.M = rng.normal(size=(5 * n, 5 * n)) * 0.1
@@ -100,9 +102,9 @@ class LinearSolver:
     def calc_rhs(self, dt):
         return dt * (self.M @ self.state.ravel()).reshape(5, self.n)
 
-    def residual_adjoint(self, dt, lam):
+    def residual_adjoint(self, dt, lam, area_vectors=True):
         gu = dt * (self.M.T @ lam.ravel()).reshape(5, self.n)
-        return dict(u=gu, s=np.zeros((2, 3)), vol=np.zeros(self.n), q=np.zeros((1, 5)))
+        return dict(u=gu, s=np.zeros((2, 3)) if area_vectors else None, vol=np.zeros(self.n), q=np.zeros((1, 5)))
 
 
 @pytest.mark.parametrize("method", [S.TPS_RK2, S.TPS_RK4, S.TPS_LSRK25BB])

@@ -1,0 +1,191 @@
+"""Tangent-linear and adjoint of the viscous (Navier-Stokes) first-order residual on the GPU, FP64.
+
+The viscous flux reads both cells' LSQ gradients of the primitive state, so the derivative of a
+cell's residual reaches its neighbours' neighbours, and across boundary faces through the ghost
+states and ghost gradients.  Checks:
+
+- dot-product test <lam, T d> = <A lam, d> over state, volume and inlet-state directions, 1e-12;
+- the tangent against central differences of the oracle's viscous residual (state and inlet
+  directions), 1e-6;
+- the unsteady gradient of the wall pressure force after N RK2 steps of the viscous residual
+  (checkpointed adjoint through torch.autograd) w.r.t. the inflow state and an initial-state
+  direction, against central differences of GPU runs, 1e-5.
+"""
+import numpy as np
+import pytest
+import torch
+
+import cases as K
+from typhon_amd import adjoint as A
+from typhon_amd import scheme as S
+from typhon_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def _ns(build, mu=0.05, dis=S.DIS_CELLFULL, grad=S.GRAD_LSQ, **kw):
+    """the case made viscous: laminar NS with constant dynamic viscosity mu (GAS=AIR constants)"""
+    m, s, g, u = build(sch_dis=dis, gradmeth=grad, **kw)
+    g = dict(g, typ_fluid=S.EQ_NSLAM, typ_visc=S.VISC_DYNCST, visc_dyn=mu)
+    return m, s, g, u
+
+
+CASES = {
+    "channel_full": lambda: _ns(lambda **k: K.channel_case(8, 4, **k)),
+    "channel_avg": lambda: _ns(lambda **k: K.channel_case(8, 4, **k), dis=S.DIS_CELLAVG2),
+    "channel_lsqw_hllcb": lambda: _ns(lambda **k: K.channel_case(8, 4, **k), grad=S.GRAD_LSQW, sch_hyp=S.SCH_HLLCB),
+    "bump_full": lambda: _ns(lambda **k: K.bump_case(6, 4, 3, **k), mu=0.02),
+    "tgv_sutherland": lambda: K.tgv_case(3, noise=0.2),
+}
+
+
+def _noisy(m, u, seed=7):
+    nv = m.dim + 2
+    rng = np.random.default_rng(seed)
+    u = u.copy()
+    u[:nv] += 1e-2 * np.abs(u[:nv]).max(axis=1, keepdims=True) * rng.uniform(-1, 1, size=u[:nv].shape)
+    return u
+
+
+def _setup(name):
+    m, s, g, u = CASES[name]()
+    u = _noisy(m, u)
+    gpu = Solver(m, s, g)
+    assert gpu.viscous
+    gpu.set_state(u)
+    dt = gpu.timestep()
+    return m, s, g, u, gpu, dt
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_viscous_adjoint_is_transpose_of_tangent(name):
+    m, s, g, u, gpu, dt = _setup(name)
+    nv = m.dim + 2
+    rng = np.random.default_rng(11)
+    du = np.zeros((5, m.ncell_int))
+    du[:nv] = rng.normal(size=(nv, m.ncell_int))
+    dv = rng.normal(size=m.ncell_int)
+    dq = rng.normal(size=(max(1, len(m.bocos)), 5))
+    lam = np.zeros((5, m.ncell_int))
+    lam[:nv] = rng.normal(size=(nv, m.ncell_int))
+    T = gpu.residual_tangent(dt, du=du, dvol=dv, dq=dq)
+    G = gpu.residual_adjoint(dt, lam, area_vectors=False)
+    assert G["s"] is None
+    terms = [(lam[:nv] * T[:nv]).sum(), (G["u"][:nv] * du[:nv]).sum(), (G["vol"] * dv).sum(),
+             (G["q"] * dq[:len(m.bocos)]).sum()]
+    lhs, rhs = terms[0], sum(terms[1:])
+    scale = sum(abs(t) for t in terms)
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs, terms)
+    for k, kw in enumerate([dict(du=du), dict(dvol=dv), dict(dq=dq)]):
+        Tk = gpu.residual_tangent(dt, **kw)
+        lk = (lam[:nv] * Tk[:nv]).sum()
+        assert abs(lk - terms[k + 1]) <= 1e-12 * (abs(lk) + abs(terms[k + 1]) + 1e-300) + 1e-14 * scale
+
+
+def test_viscous_area_vector_derivatives_are_refused():
+    m, s, g, u, gpu, dt = _setup("channel_full")
+    with pytest.raises(Exception, match="area-vector"):
+        gpu.residual_adjoint(dt, np.zeros((5, m.ncell_int)))
+    with pytest.raises(Exception, match="area-vector"):
+        gpu.residual_tangent(dt, ds=np.zeros((m.nface, 3)))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_viscous_state_tangent_matches_oracle_differences(name):
+    m, s, g, u, gpu, dt = _setup(name)
+    nv = m.dim + 2
+    rng = np.random.default_rng(5)
+    du = np.zeros_like(u)
+    du[:nv] = rng.normal(size=(nv, m.ncell_int)) * np.abs(u[:nv]).max(axis=1, keepdims=True)
+    h = 1e-7
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u + h * du)
+    rp = o.calc_rhs(dt)
+    o.set_cons(u - h * du)
+    rm = o.calc_rhs(dt)
+    ref = (rp - rm) / (2 * h)
+    got = gpu.residual_tangent(dt, du=du)
+    assert K.rel_err(got[:nv], ref[:nv]) <= 1e-6
+
+
+@pytest.mark.parametrize("var", [0, 1, 2])
+def test_viscous_inlet_tangent_matches_oracle_differences(var):
+    inflow = np.array([1.0, 0.5 * np.sqrt(1.4), 1.0])  # rho, u, p
+    build = lambda inflow_, p_out=None: _ns(  # noqa: E731
+        lambda **k: K.bump_case(6, 4, 3, inflow=tuple(inflow_), p_out=p_out, **k), mu=0.02)
+    m, s, g, u = build(inflow)
+    u = _noisy(m, u)
+    gpu = Solver(m, s, g)
+    gpu.set_state(u)
+    dt = gpu.timestep()
+    k = {0: 0, 1: 2, 2: 1}[var]  # ghost order (rho, p, u, v, w); inflow order (rho, u, p)
+    h = 1e-6 * inflow[k]
+    res = []
+    for sgn in (1, -1):
+        f = inflow.copy()
+        f[k] += sgn * h
+        mm, ss, gg, _ = build(f, p_out=inflow[2])
+        o = K.make_oracle(mm, ss, gg)
+        o.set_cons(u)
+        res.append(o.calc_rhs(dt))
+    ref = (res[0] - res[1]) / (2 * h)
+    dq = np.zeros((len(m.bocos), 5))
+    dq[0, var] = 1.0  # boco 0 is the inlet
+    got = gpu.residual_tangent(dt, dq=dq)
+    assert K.rel_err(got, ref) <= 1e-6
+
+
+def _run_J(inflow, U0, dts, nxyz, p_out=1.0):
+    m, s, g, _ = _ns(lambda **k: K.bump_case(*nxyz, inflow=tuple(inflow), p_out=p_out, **k), mu=0.02)
+    gpu = Solver(m, s, g)
+    coef = S.rk_tableau(s["tps_meth"])
+    U = torch.as_tensor(U0)
+    Sx = torch.as_tensor(m.fnorm * m.fsurf[:, None])
+    V = torch.as_tensor(m.cvol[:m.ncell_int])
+    q = torch.zeros(len(m.bocos), 5, dtype=torch.float64)
+    with torch.no_grad():
+        for dt in dts:
+            U = A.rk_step(U, Sx, V, q, gpu, float(dt), coef)
+    J = float(A.wall_force_x(m, U, Sx, "JMIN"))
+    gpu.close()
+    return J
+
+
+def test_viscous_unsteady_gradient_matches_differences():
+    """dJ/d(inflow rho, u, p) and dJ/dU0 along a seeded direction, J = pressure force on the bump
+    wall after N RK2 steps of the viscous residual, checkpointed adjoint (3 snapshots)"""
+    nxyz = (8, 4, 3)
+    inflow = np.array([1.0, 0.5 * np.sqrt(1.4), 1.0])
+    m, s, g, U0 = _ns(lambda **k: K.bump_case(*nxyz, inflow=tuple(inflow), **k), mu=0.02)
+    gpu = Solver(m, s, g)
+    gpu.set_state(U0)
+    N = 12
+    dts = []
+    U = U0
+    for _ in range(N):
+        gpu.set_state(U)
+        dt = gpu.timestep()
+        dts.append(dt)
+        gpu.rk_step(dt)
+        U = gpu.get_state()
+    coef = S.rk_tableau(s["tps_meth"])
+    J, gU0, gvtx, gq = A.unsteady_gradient(gpu, m, U0, dts, coef, lambda U, Sx: A.wall_force_x(m, U, Sx, "JMIN"),
+                                           snaps=3)
+    assert gvtx is None
+    J_ref = _run_J(inflow, U0, dts, nxyz)
+    assert abs(J - J_ref) <= 1e-12 * abs(J_ref)
+    ad = {"rho": gq[0, 0], "u": gq[0, 2], "p": gq[0, 1]}
+    for k, name in enumerate(("rho", "u", "p")):
+        h = 1e-6 * inflow[k]
+        fp, fm = inflow.copy(), inflow.copy()
+        fp[k] += h
+        fm[k] -= h
+        fd = (_run_J(fp, U0, dts, nxyz) - _run_J(fm, U0, dts, nxyz)) / (2 * h)
+        assert abs(ad[name] - fd) <= 1e-5 * abs(fd) + 1e-10, (name, ad[name], fd)
+    rng = np.random.default_rng(3)
+    d = np.zeros_like(U0)
+    d[:5] = rng.normal(size=(5, m.ncell_int)) * np.abs(U0[:5]).max(axis=1, keepdims=True) * 1e-2
+    h = 1e-5
+    fd = (_run_J(inflow, U0 + h * d, dts, nxyz) - _run_J(inflow, U0 - h * d, dts, nxyz)) / (2 * h)
+    adU = float((gU0 * d).sum())
+    assert abs(adU - fd) <= 1e-5 * abs(fd) + 1e-10, (adU, fd)

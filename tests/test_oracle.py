@@ -195,6 +195,30 @@ def test_geometry_closed_forms():
     assert (np.einsum("ij,ij->i", d, mt.fnorm) > 0).all()
 
 
+def test_partly_periodic_box():
+    """hex box periodic in x and z with slip walls at y = 0 and y = L: n^2 faces per wall, periodic
+    pairs one period apart and volumes summing to L^3 (unjittered: the jittered hexes' faces are
+    not planar, and a quad's centroid depends on its diagonal split), closed cells"""
+    n, L = 4, 2.0
+    m = M.box_hex(n, L, 0.1, 5, periodic=(True, False, True))
+    kinds = {b.family: b.type for b in m.bocos}
+    assert kinds["JMIN"] == M.BC_SYM and kinds["JMAX"] == M.BC_SYM
+    assert kinds["IMIN"] == M.BC_PERIOD and kinds["KMAX"] == M.BC_PERIOD
+    assert m.boco("JMIN").iface.size == n * n and m.boco("JMAX").iface.size == n * n
+    m0 = M.box_hex(n, L, 0.0, 5, periodic=(True, False, True))
+    assert abs(m0.cvol[:m0.ncell_int].sum() - L ** 3) < 1e-12
+    assert np.allclose(m.fcent[m.boco("JMIN").iface, 1], 0.0) and np.allclose(m.fcent[m.boco("JMAX").iface, 1], L)
+    for fam, axis in (("IMIN", 0), ("KMIN", 2)):
+        b = m0.boco(fam)
+        d = m0.fcent[b.ipart] - m0.fcent[b.iface]
+        assert np.allclose(np.abs(d[:, axis]), L) and np.allclose(np.delete(d, axis, axis=1), 0.0, atol=1e-12)
+    S_ = m.fnorm * m.fsurf[:, None]
+    acc = np.zeros((m.ncell, 3))
+    np.add.at(acc, m.ic1, S_)
+    np.add.at(acc, m.ic2, -S_)
+    assert np.abs(acc[:m.ncell_int]).max() < 1e-14
+
+
 def test_meshgeom_c_restatement_matches_builder():
     for m in (M.channel_quad(6, 5, 1.0, 0.5, 0.2, 3), M.box_tets(3, 1.0, 0.1, 5, periodic=False),
               M.box_hex(3, 1.0, 0.1, 6, periodic=False)):

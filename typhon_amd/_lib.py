@@ -16,7 +16,7 @@ F64P = C.POINTER(C.c_double)
 EXPORTS = ["ty_create", "ty_destroy", "ty_last_error", "ty_mesh_upload", "ty_set_scheme", "ty_state_set",
            "ty_state_get", "ty_timestep", "ty_residual", "ty_rk_step", "ty_advance", "ty_sync", "ty_timing",
            "ty_mesh_info", "ty_jacobian_diag", "ty_jvp", "ty_linear_solve", "ty_implicit_step",
-           "ty_residual_tangent", "ty_residual_adjoint"]
+           "ty_residual_tangent", "ty_residual_adjoint", "ty_unsteady_adjoint"]
 
 
 class TyBocoDesc(C.Structure):
@@ -52,6 +52,11 @@ class TyKrylov(C.Structure):
 class TyKrylovStats(C.Structure):
     _fields_ = [("outer_iters", C.c_int32), ("inner_iters", C.c_int32), ("rel_res", C.c_double),
                 ("rhs_norm", C.c_double)]
+
+
+class TyUnsteadyStats(C.Structure):
+    _fields_ = [("forward_steps", C.c_int32), ("adjoint_steps", C.c_int32), ("max_snapshots", C.c_int32),
+                ("ms", C.c_double)]
 
 
 class TyError(RuntimeError):
@@ -93,6 +98,8 @@ def load():
     lib.ty_implicit_step.argtypes = [vp, C.c_double, C.POINTER(TyKrylov), C.POINTER(TyKrylovStats)]
     lib.ty_residual_tangent.argtypes = [vp, C.c_double, F64P, F64P, F64P, F64P, F64P]
     lib.ty_residual_adjoint.argtypes = [vp, C.c_double, F64P, F64P, F64P, F64P, F64P]
+    lib.ty_unsteady_adjoint.argtypes = [vp, C.c_int32, F64P, C.c_int32, F64P, F64P, F64P,
+                                        C.POINTER(TyUnsteadyStats)]
     for name in EXPORTS:
         if name != "ty_last_error":
             getattr(lib, name).restype = C.c_int

@@ -29,7 +29,8 @@ import torch
 class Residual(torch.autograd.Function):
     """R = residual(U; S, V, q) on the solver's mesh.  S, V and q must hold the geometry
     and inlet states the solver was built with: they are inputs so that autograd routes
-    their gradients, the forward value does not read them."""
+    their gradients, the forward value does not read them.  The gradient w.r.t. S is
+    computed only when S requires it (not provided for viscous residuals)."""
 
     @staticmethod
     def forward(ctx, U, S, V, q, solver, dt):
@@ -45,8 +46,8 @@ class Residual(torch.autograd.Function):
         (U,) = ctx.saved_tensors
         s = ctx.solver
         s.set_state(U.detach().cpu().numpy())
-        g = s.residual_adjoint(ctx.dt, gR.detach().cpu().numpy())
-        t = [torch.from_numpy(np.ascontiguousarray(g[k])) for k in ("u", "s", "vol", "q")]
+        g = s.residual_adjoint(ctx.dt, gR.detach().cpu().numpy(), area_vectors=ctx.needs_input_grad[1])
+        t = [None if g[k] is None else torch.from_numpy(np.ascontiguousarray(g[k])) for k in ("u", "s", "vol", "q")]
         return t[0], t[1], t[2], t[3], None, None
 
 
@@ -118,15 +119,20 @@ def cell_volumes(mesh, vtx, S=None):
     return V
 
 
-def wall_force_x(mesh, U, S, family, gamma=1.4):
-    """J = sum over the family's faces of p(U of the wall cell) S_x (pressure force along x
-    on the wall, S pointing out of the domain)"""
+def wall_force(mesh, U, S, family, axis=0, gamma=1.4):
+    """J = sum over the family's faces of p(U of the wall cell) S_axis (pressure force along the
+    axis on the wall, S pointing out of the domain)"""
     b = mesh.boco(family)
     f = torch.as_tensor(b.iface, dtype=torch.long)
     c = torch.as_tensor(mesh.ic1[b.iface], dtype=torch.long)
     rho, rhoE, mom = U[0, c], U[1, c], U[2:2 + mesh.dim, c]
     p = (gamma - 1.0) * (rhoE - 0.5 * (mom * mom).sum(0) / rho)
-    return (p * S[f, 0]).sum()
+    return (p * S[f, axis]).sum()
+
+
+def wall_force_x(mesh, U, S, family, gamma=1.4):
+    """wall_force along x"""
+    return wall_force(mesh, U, S, family, 0, gamma)
 
 
 # ---------------------------------------------------------------------------
@@ -178,14 +184,17 @@ def unsteady_gradient(solver, mesh, U0, dts, coef, objective, vtx=None, q=None, 
 
     objective(U_N, S) -> scalar torch tensor.  vtx: torch [nvtx, 3] vertex coordinates (the
     mesh's), q: torch [nboco, 5] inlet ghost states (the scheme's).  Returns
-    (J, dJ/dU0 [5, ncell_int], dJ/dvtx [nvtx, 3], dJ/dq [nboco, 5])."""
-    vtx = torch.as_tensor(mesh.vtx if vtx is None else vtx, dtype=torch.float64).clone().requires_grad_(True)
+    (J, dJ/dU0 [5, ncell_int], dJ/dvtx [nvtx, 3], dJ/dq [nboco, 5]); dJ/dvtx is None for a
+    viscous residual (its flux also depends on the cell and face centres, which the area
+    vectors and volumes do not carry)."""
+    geometry = not solver.viscous
+    vtx = torch.as_tensor(mesh.vtx if vtx is None else vtx, dtype=torch.float64).clone().requires_grad_(geometry)
     nb = max(1, len(mesh.bocos))
     q = torch.zeros(nb, 5, dtype=torch.float64) if q is None else torch.as_tensor(q, dtype=torch.float64)
     q = q.clone().requires_grad_(True)
     S = face_area_vectors(mesh, vtx)
     V = cell_volumes(mesh, vtx, S)
-    Sd, Vd = S.detach().requires_grad_(True), V.detach().requires_grad_(True)
+    Sd, Vd = S.detach().requires_grad_(geometry), V.detach().requires_grad_(geometry)
     gS = torch.zeros_like(Sd)
     gV = torch.zeros_like(Vd)
     U0 = torch.as_tensor(U0, dtype=torch.float64)
@@ -202,8 +211,9 @@ def unsteady_gradient(solver, mesh, U0, dts, coef, objective, vtx=None, q=None, 
             t.grad = None
         out = rk_step(Ui, Sd, Vd, q, solver, float(dts[k]), coef)
         torch.autograd.backward(out, lam)
-        gS = gS + Sd.grad
-        gV = gV + Vd.grad
+        if geometry:
+            gS = gS + Sd.grad
+            gV = gV + Vd.grad
         return Ui.grad.detach()
 
     # forward to U_N for the objective (no checkpoint kept: treeverse recomputes)
@@ -225,8 +235,28 @@ def unsteady_gradient(solver, mesh, U0, dts, coef, objective, vtx=None, q=None, 
         return r
 
     lam0 = treeverse(0, N, U0, lamN, snaps, advance, step_adjoint_q) if N > 0 else lamN
+    if not geometry:
+        return float(J), lam0.numpy(), None, qgrad.detach().numpy()
     if gS_obj is not None:
         gS = gS + gS_obj
     # chain S(vtx), V(vtx) -> vtx
     gvtx = torch.autograd.grad((S, V), vtx, (gS, gV), allow_unused=True)[0]
     return float(J), lam0.numpy(), gvtx.numpy(), qgrad.detach().numpy()
+
+
+def unsteady_gradient_device(solver, mesh, U0, dts, objective, snaps=20):
+    """As unsteady_gradient for dJ/dU0 and dJ/dq, with the reverse sweep on the device
+    (ty_unsteady_adjoint: checkpoints and stage states in device memory).  One forward sweep gives
+    U_N; the objective and lam_N = dJ/dU_N are evaluated with torch on the host.  Returns
+    (J, dJ/dU0 [5, ncell_int], dJ/dq [nboco, 5], stats); the solver's state is U0 on return."""
+    solver.set_state(U0)
+    for dt in dts:
+        solver.rk_step(float(dt))
+    UN = torch.as_tensor(solver.get_state()).requires_grad_(True)
+    S = torch.as_tensor(mesh.fnorm * mesh.fsurf[:, None])
+    J = objective(UN, S)
+    (lamN,) = torch.autograd.grad(J, (UN,), allow_unused=True)
+    lamN = np.zeros(UN.shape) if lamN is None else lamN.numpy()
+    solver.set_state(U0)
+    out = solver.unsteady_adjoint(np.asarray(dts, dtype=np.float64), lamN, snaps)
+    return float(J), out["u"], out["q"], out["stats"]

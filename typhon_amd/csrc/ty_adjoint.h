@@ -10,8 +10,14 @@ struct AdArgs {
     int nf, nb;           // device faces, device boundary faces
     const int* fcell;     // [2][nf] owner and other cell of each device face (other = -1 on boundary faces)
     int nfi;              // two-sided faces come first: device faces [0, nfi)
-    double* face;         // [nf][2][nv + 1] work: per face side, that cell's state adjoint and the
-                          // side's share of its volume sensitivity
+    double* face;         // [nf][2][RW] work: per face side, that cell's state adjoint, (viscous) the
+                          // adjoint of its primitive-variable gradient, and the side's share of its
+                          // volume sensitivity; RW = nv (+ nv dim) + 1
+    // viscous residuals (k.grad holds the cell gradients of the state)
+    const double* fgeo;   // [6][nf] work: x_f - x_owner and the canonical centre difference d per face
+    double* slot;         // [ns][n][nv] work: per slot, the far cell's primitive adjoint through this
+                          // cell's gradient
+    double* dgrad;        // [n][nv dim] work: tangent of the cell gradients
     // tangent inputs (each may be null = zero direction) and output
     const double* du;     // [nv][n] state direction
     const double* ds;     // [3][nf] area-vector direction per device face (owner orientation)
@@ -25,6 +31,16 @@ struct AdArgs {
     double* gvol;         // [n] (dR/dV)^T lambda
     double* gq;           // [nv][nb] per boundary face: (dR/dq_ghost)^T lambda (inlet faces, else 0)
 };
+
+// out = sum_i c[i] v[i] over m vectors of len doubles, summed in index order (RK stage states and
+// the reverse RK sweep); out may alias one of the v[i]
+constexpr int COMB_MAX = MAX_STAGES + 1;
+struct Comb {
+    const double* v[COMB_MAX];
+    double c[COMB_MAX];
+    int m;
+};
+hipError_t launch_comb(double* out, const Comb& cb, long len, hipStream_t s);
 
 template <int DIM>
 hipError_t launch_res_tangent(const AdArgs& t, hipStream_t s);

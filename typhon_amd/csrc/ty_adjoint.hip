@@ -18,6 +18,7 @@
 #include "ty_kernels_impl.h"
 #include "ty_dual.h"
 #include "ty_hllc_rev.h"
+#include "ty_visc_rev.h"
 #include "ty_adjoint.h"
 
 namespace ty {
@@ -272,18 +273,174 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint(AdArgs t)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Viscous first-order residual: the flux also reads both cells' gradients of the primitive
+// state (LSQ, g_c = sum_k w_k (q_k - q_c), ghost states across boundary slots), so the
+// derivative reaches the neighbours' neighbours through the gradients.
+
+// tangent of the cell gradients (value in k.grad): dg_c = sum_k w_k (dq_k - dq_c)
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_grad_tangent(AdArgs t)
+{
+    constexpr int NV = DIM + 2;
+    const KArgs<double>& a = t.k;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const int n = a.n;
+    const Dual g1(a.ph.gamma - 1.0);
+    Dual Uc[NV], qc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Uc[v] = Dual(a.us[(size_t)v * n + c], t.du ? t.du[(size_t)v * n + c] : 0.0);
+    cons_to_prim<DIM>(Uc, g1, qc);
+    double dg[NV * DIM];
+#pragma unroll
+    for (int k = 0; k < NV * DIM; ++k) dg[k] = 0.0;
+    for (int k = 0; k < a.ns; ++k) {
+        const int L = a.link[(size_t)k * n + c];
+        if (L == SLOT_UNUSED) continue;
+        Dual qo[NV];
+        if (!link_boundary(L)) {
+            const int nb = link_index(L);
+            Dual Uo[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Uo[v] = Dual(a.us[(size_t)v * n + nb], t.du ? t.du[(size_t)v * n + nb] : 0.0);
+            cons_to_prim<DIM>(Uo, g1, qo);
+        } else {
+            const int bi = link_index(L);
+            double nv[DIM];
+            slot_normal<DIM>(a.g, k, n, c, nv);
+            Dual nd[DIM];
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
+            const BocoDev bc = a.boco[a.g.bboco[bi]];
+            bc_state<DIM>(bc, nd, qc, qo, a.ph);
+            if (bc.kind == BK_INLET_SUP && t.dq) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qo[v].d += t.dq[(size_t)a.g.bboco[bi] * NVMAX + v];
+            }
+        }
+        double wk[DIM];
+        slot_vec<DIM>(a.g.sw, k, n, c, wk);
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) dg[v * DIM + e] += wk[e] * (qo[v].d - qc[v].d);
+    }
+    double* out = t.dgrad + (size_t)c * NV * DIM;
+#pragma unroll
+    for (int k = 0; k < NV * DIM; ++k) out[k] = dg[k];
+}
+
+// tangent of the viscous first-order residual (state, volume and inlet directions)
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_res_tangent_visc(AdArgs t)
+{
+    constexpr int NV = DIM + 2, NG = NV * DIM;
+    const KArgs<double>& a = t.k;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const int n = a.n;
+    const Dual g1(a.ph.gamma - 1.0);
+    Dual Uc[NV], qc[NV], gc[NG], res[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        Uc[v] = Dual(a.us[(size_t)v * n + c], t.du ? t.du[(size_t)v * n + c] : 0.0);
+        res[v] = Dual(0.0);
+    }
+    cons_to_prim<DIM>(Uc, g1, qc);
+#pragma unroll
+    for (int k = 0; k < NG; ++k) gc[k] = Dual(a.grad[(size_t)c * NG + k], t.dgrad[(size_t)c * NG + k]);
+    for (int k = 0; k < a.ns; ++k) {
+        const int L = a.link[(size_t)k * n + c];
+        if (L == SLOT_UNUSED) continue;
+        const bool other = link_other(L);
+        double nv[DIM];
+        const double A = slot_normal<DIM>(a.g, k, n, c, nv);
+        Dual nd[DIM];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
+        Dual qo[NV], go[NG];
+        if (!link_boundary(L)) {
+            const int nb = link_index(L);
+            Dual Uo[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Uo[v] = Dual(a.us[(size_t)v * n + nb], t.du ? t.du[(size_t)v * n + nb] : 0.0);
+            cons_to_prim<DIM>(Uo, g1, qo);
+#pragma unroll
+            for (int q = 0; q < NG; ++q) go[q] = Dual(a.grad[(size_t)nb * NG + q], t.dgrad[(size_t)nb * NG + q]);
+        } else {
+            const int bi = link_index(L);
+            const BocoDev bc = a.boco[a.g.bboco[bi]];
+            bc_state<DIM>(bc, nd, qc, qo, a.ph);
+            if (bc.kind == BK_INLET_SUP && t.dq) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qo[v].d += t.dq[(size_t)a.g.bboco[bi] * NVMAX + v];
+            }
+            bc_grad<DIM>(bc, nd, gc, go);
+        }
+        const Dual* qL = other ? qo : qc;
+        const Dual* qR = other ? qc : qo;
+        Dual F[NV];
+        num_flux<DIM>(a.ph.sch_hyp, Dual(a.ph.gamma), qL, qR, nd, F);
+        double rk[DIM], dk[DIM];
+        slot_vec<DIM>(a.g.sr, k, n, c, rk);
+        slot_vec<DIM>(a.g.sd, k, n, c, dk);
+        Dual rL[DIM], rR[DIM], dd[DIM];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) {
+            rL[e] = Dual(other ? rk[e] + dk[e] : rk[e]);
+            rR[e] = Dual(other ? rk[e] : rk[e] - dk[e]);
+            dd[e] = Dual(dk[e]);
+        }
+        viscous_face<DIM>(a.ph, qL, qR, rL, rR, dd, nd, other ? go : gc, other ? gc : go, F);
+        const Dual Ad(A);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) res[v] = other ? res[v] + Ad * F[v] : res[v] - Ad * F[v];
+    }
+    const Dual Vd(a.vol[c], t.dvol ? t.dvol[c] : 0.0);
+    const Dual coef = Dual(*a.d_dt) / Vd;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t.out[(size_t)v * n + c] = (coef * res[v]).d;
+}
+
+// per device face: x_f - x_owner and the canonical centre difference d = x_other - x_owner
+// (the ghost centre on boundary faces), from the owner's slot geometry
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_adj_face_geom(AdArgs t)
+{
+    const KArgs<double>& a = t.k;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const int n = a.n;
+    for (int k = 0; k < a.ns; ++k) {
+        const int w = a.fw[(size_t)k * n + c];
+        if (w == SLOT_UNUSED || w < 0) continue;  // unused, or this cell is the far side
+        const int f = (int)((unsigned)w & FACE_MASK);
+        double rk[DIM], dk[DIM];
+        slot_vec<DIM>(a.g.sr, k, n, c, rk);
+        slot_vec<DIM>(a.g.sd, k, n, c, dk);
+        double* g = const_cast<double*>(t.fgeo);
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) {
+            g[(size_t)e * t.nf + f] = rk[e];
+            g[(size_t)(3 + e) * t.nf + f] = dk[e];
+        }
+    }
+}
+
 // HLLC/HLLCB faces, one work-item per device face: one forward and one reverse sweep of the face
 // flux (the cell-parallel form would evaluate every two-sided face twice).  The face weight is
 // m = A (lambda_other dt/V_other - lambda_owner dt/V_owner), and m = -A lambda_owner dt/V_owner on a
-// boundary face, whose ghost state qR = B(qL, n) is chained through the transpose of B's Jacobian
-// (dual-number passes of the boundary template, cheap next to the flux).  Each face side gets a
-// record [(dF/dU_side)^T m, sigma A lambda_side dt/V_side . F] (sigma = -1 owner, +1 other) for
-// k_res_adjoint_gather; the face's area-vector sensitivity and a boundary face's inlet-state
-// sensitivity are written here.
-template <int DIM>
+// boundary face, whose ghost state qR = B(qL, n) (and, viscous, ghost gradient gR = B_g(gL, n)) is
+// chained through the transpose of B's Jacobian (dual-number passes of the boundary template,
+// cheap next to the flux).  Each face side gets a record [(dF/dU_side)^T m, (viscous) (dF/dg_side)^T m,
+// sigma A lambda_side dt/V_side . F] (sigma = -1 owner, +1 other) for k_res_adjoint_gather; the
+// face's area-vector sensitivity (inviscid) and a boundary face's inlet-state sensitivity are
+// written here.
+template <int DIM, bool VISC>
 __global__ __launch_bounds__(BLOCK) void k_res_adjoint_face(AdArgs t)
 {
-    constexpr int NV = DIM + 2, RW = NV + 1;
+    constexpr int NV = DIM + 2, NG = VISC ? NV * DIM : 0, RW = NV + NG + 1;
     const KArgs<double>& a = t.k;
     const int f = blockIdx.x * BLOCK + threadIdx.x;
     if (f >= t.nf) return;
@@ -319,10 +476,41 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_face(AdArgs t)
     }
     hllc_rev<DIM>(a.ph.sch_hyp, a.ph.gamma, qL, qR, nv, mv, Fv, qLb, qRb, gn);
     double* out = t.face + (size_t)f * 2 * RW;
+    double gLb[VISC ? NG : 1], gRb[VISC ? NG : 1];
+    if (VISC) {
+        double gL[VISC ? NG : 1], gR[VISC ? NG : 1], rL[DIM], rR[DIM], dk[DIM];
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            gL[q] = a.grad[(size_t)cl * NG + q];
+            gLb[q] = 0.0;
+            gRb[q] = 0.0;
+        }
+        if (bnd) {
+            bc_grad<DIM>(bc, nv, gL, gR);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NG; ++q) gR[q] = a.grad[(size_t)cr * NG + q];
+        }
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) {
+            rL[e] = t.fgeo[(size_t)e * t.nf + f];
+            dk[e] = t.fgeo[(size_t)(3 + e) * t.nf + f];
+            rR[e] = rL[e] - dk[e];
+        }
+        viscous_rev<DIM>(a.ph, qL, qR, rL, rR, dk, nv, gL, gR, mv, Fv, qLb, qRb, gLb, gRb);
+        if (bnd) {
+            // gLb += (dB_g/dgL)^T gRb: B_g copies the gradient, or reflects the scalar gradients
+            // by S = I - 2 n n^T and the velocity gradient as S G S; both maps are self-adjoint
+            double tb[VISC ? NG : 1];
+            bc_grad<DIM>(bc, nv, gRb, tb);
+#pragma unroll
+            for (int q = 0; q < NG; ++q) gLb[q] += tb[q];
+        }
+    }
     double sL = 0.0;
 #pragma unroll
     for (int v = 0; v < NV; ++v) sL += t.lam[(size_t)v * n + cl] * Fv[v];
-    out[NV] = -A * kl * sL;
+    out[RW - 1] = -A * kl * sL;
     if (bnd) {
         const int bi = f - t.nfi;
         const bool inlet = bc.kind == BK_INLET_SUP;
@@ -359,7 +547,11 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_face(AdArgs t)
         prim_rev<DIM>(UR, g1d, qRb, gR);
 #pragma unroll
         for (int v = 0; v < NV; ++v) out[RW + v] = gR[v];
-        out[RW + NV] = A * kr * sR;
+        if (VISC) {
+#pragma unroll
+            for (int q = 0; q < NG; ++q) out[RW + NV + q] = gRb[q];
+        }
+        out[2 * RW - 1] = A * kr * sR;
     }
     double gL[NV];
 #pragma unroll
@@ -367,21 +559,33 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_face(AdArgs t)
     prim_rev<DIM>(UL, g1d, qLb, gL);
 #pragma unroll
     for (int v = 0; v < NV; ++v) out[v] = gL[v];
-    if (t.gs) face_sens<DIM>(t, f, nv, A, mv, Fv, gn);
+    if (VISC) {
+#pragma unroll
+        for (int q = 0; q < NG; ++q) out[NV + q] = gLb[q];
+    }
+    if (!VISC && t.gs) face_sens<DIM>(t, f, nv, A, mv, Fv, gn);
 }
 
-// per cell: its face sides' records summed in slot order (one writer per output, no atomics)
-template <int DIM>
+// per cell: its face sides' records summed in slot order (one writer per output, no atomics).
+// Viscous: the gradient adjoint gb of this cell is pushed back through g_c = sum_k w_k (q_k - q_c):
+// this cell's own primitive adjoint gets -sum_k w_k . gb (and the ghost states' Jacobians on
+// boundary slots, or the inlet-state sensitivity), and the far cell of slot k gets t_k = w_k . gb,
+// left in t.slot for k_res_adjoint_tail.
+template <int DIM, bool VISC>
 __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
 {
-    constexpr int NV = DIM + 2, RW = NV + 1;
+    constexpr int NV = DIM + 2, NG = VISC ? NV * DIM : 0, RW = NV + NG + 1;
     const KArgs<double>& a = t.k;
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n) return;
     const int n = a.n;
-    double gu[NV], sv = 0.0;
+    double gu[NV], sv = 0.0, gb[VISC ? NG : 1];
 #pragma unroll
     for (int v = 0; v < NV; ++v) gu[v] = 0.0;
+    if (VISC) {
+#pragma unroll
+        for (int q = 0; q < NG; ++q) gb[q] = 0.0;
+    }
     for (int k = 0; k < a.ns; ++k) {
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
@@ -389,7 +593,66 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
         const double* in = t.face + ((size_t)s.f * 2 + (s.other ? 1 : 0)) * RW;
 #pragma unroll
         for (int v = 0; v < NV; ++v) gu[v] += in[v];
-        sv += in[NV];
+        if (VISC) {
+#pragma unroll
+            for (int q = 0; q < NG; ++q) gb[q] += in[NV + q];
+        }
+        sv += in[RW - 1];
+    }
+    if (VISC) {
+        const double g1d = a.ph.gamma - 1.0;
+        double Uc[NV], qc[NV], qb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            Uc[v] = a.us[(size_t)v * n + c];
+            qb[v] = 0.0;
+        }
+        cons_to_prim<DIM>(Uc, g1d, qc);
+        for (int k = 0; k < a.ns; ++k) {
+            const int L = a.link[(size_t)k * n + c];
+            if (L == SLOT_UNUSED) continue;
+            double wk[DIM], tk[NV];
+            slot_vec<DIM>(a.g.sw, k, n, c, wk);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                double x = 0.0;
+#pragma unroll
+                for (int e = 0; e < DIM; ++e) x += wk[e] * gb[v * DIM + e];
+                tk[v] = x;
+                qb[v] -= x;
+            }
+            if (!link_boundary(L)) {
+                double* o = t.slot + ((size_t)k * n + c) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) o[v] = tk[v];
+            } else {
+                const int bi = link_index(L);
+                const BocoDev bc = a.boco[a.g.bboco[bi]];
+                if (bc.kind == BK_INLET_SUP) {
+                    if (t.gq) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) t.gq[(size_t)v * t.nb + bi] += tk[v];
+                    }
+                } else {
+                    double nv[DIM];
+                    slot_normal<DIM>(a.g, k, n, c, nv);
+                    Dual nd[DIM];
+#pragma unroll
+                    for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
+                    for (int j = 0; j < NV; ++j) {
+                        Dual qd[NV], gd[NV];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) qd[v] = Dual(qc[v], v == j ? 1.0 : 0.0);
+                        bc_state<DIM>(bc, nd, qd, gd, a.ph);
+                        double sacc = 0.0;
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) sacc += gd[v].d * tk[v];
+                        qb[j] += sacc;
+                    }
+                }
+            }
+        }
+        prim_rev<DIM>(Uc, g1d, qb, gu);
     }
 #pragma unroll
     for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] = gu[v];
@@ -397,10 +660,45 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
     if (t.gvol) t.gvol[c] = -sv / a.vol[c];
 }
 
+// viscous: each cell collects what its neighbours' gradients pushed back to it (t.slot at the
+// mirror slot of the shared face), in slot order
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_res_adjoint_tail(AdArgs t)
+{
+    constexpr int NV = DIM + 2;
+    const KArgs<double>& a = t.k;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const int n = a.n;
+    double qb[NV], Uc[NV], gu[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        qb[v] = 0.0;
+        Uc[v] = a.us[(size_t)v * n + c];
+        gu[v] = 0.0;
+    }
+    for (int k = 0; k < a.ns; ++k) {
+        const int L = a.link[(size_t)k * n + c];
+        if (L == SLOT_UNUSED || link_boundary(L)) continue;
+        const double* in = t.slot + ((size_t)link_mirror(L) * n + link_index(L)) * NV;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qb[v] += in[v];
+    }
+    prim_rev<DIM>(Uc, a.ph.gamma - 1.0, qb, gu);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] += gu[v];
+}
+
 template <int DIM>
 hipError_t launch_res_tangent(const AdArgs& t, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_res_tangent<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+    const dim3 grid(grid_blocks(t.k.n));
+    if (t.k.ph.viscous) {
+        hipLaunchKernelGGL((k_grad_tangent<DIM>), grid, dim3(BLOCK), 0, s, t);
+        hipLaunchKernelGGL((k_res_tangent_visc<DIM>), grid, dim3(BLOCK), 0, s, t);
+    } else {
+        hipLaunchKernelGGL((k_res_tangent<DIM>), grid, dim3(BLOCK), 0, s, t);
+    }
     return hipGetLastError();
 }
 
@@ -408,12 +706,37 @@ template <int DIM>
 hipError_t launch_res_adjoint(const AdArgs& t, hipStream_t s)
 {
     const bool rev = t.k.ph.sch_hyp == 25 || t.k.ph.sch_hyp == 26;
-    if (!rev) {
-        hipLaunchKernelGGL((k_res_adjoint<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+    const dim3 grid(grid_blocks(t.k.n)), fgrid((t.nf + BLOCK - 1) / BLOCK);
+    if (t.k.ph.viscous) {
+        if (!rev) return hipErrorInvalidValue;  // the host rejects this case first
+        hipLaunchKernelGGL((k_adj_face_geom<DIM>), grid, dim3(BLOCK), 0, s, t);
+        hipLaunchKernelGGL((k_res_adjoint_face<DIM, true>), fgrid, dim3(BLOCK), 0, s, t);
+        hipLaunchKernelGGL((k_res_adjoint_gather<DIM, true>), grid, dim3(BLOCK), 0, s, t);
+        hipLaunchKernelGGL((k_res_adjoint_tail<DIM>), grid, dim3(BLOCK), 0, s, t);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL((k_res_adjoint_face<DIM>), dim3((t.nf + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, t);
-    hipLaunchKernelGGL((k_res_adjoint_gather<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+    if (!rev) {
+        hipLaunchKernelGGL((k_res_adjoint<DIM>), grid, dim3(BLOCK), 0, s, t);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_res_adjoint_face<DIM, false>), fgrid, dim3(BLOCK), 0, s, t);
+    hipLaunchKernelGGL((k_res_adjoint_gather<DIM, false>), grid, dim3(BLOCK), 0, s, t);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(BLOCK) void k_comb(double* out, Comb cb, long len)
+{
+    const long e = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= len) return;
+    double s = 0.0;
+    for (int i = 0; i < cb.m; ++i) s += cb.c[i] * cb.v[i][e];
+    out[e] = s;
+}
+
+hipError_t launch_comb(double* out, const Comb& cb, long len, hipStream_t s)
+{
+    if (cb.m < 1 || cb.m > COMB_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_comb, dim3((unsigned)((len + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, out, cb, len);
     return hipGetLastError();
 }
 

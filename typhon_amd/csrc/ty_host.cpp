@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -1518,11 +1519,11 @@ struct DevTmp {
     }
 };
 
-int ad_prepare(ty_ctx* c, double dt, AdArgs& t)
+int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
 {
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    if (c->ph.method != 'N' || c->ph.viscous)
-        return fail(c, TY_ERR_UNSUPPORTED, "tangent/adjoint residual: first-order inviscid schemes only");
+    if (c->ph.method != 'N')
+        return fail(c, TY_ERR_UNSUPPORTED, "tangent/adjoint residual: first-order schemes only");
     hipSetDevice(c->device);
     c->h_dt = dt;
     HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1533,6 +1534,19 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t)
     t.nb = c->nb;
     t.fcell = c->d_fcell;
     t.nfi = c->nfi;
+    if (c->ph.viscous) {
+        // the cell gradients of the current state, as the residual kernels compute them
+        int rc;
+        double* d_st = nullptr;
+        if ((rc = ensure_work(c, 0)) || (rc = tmp.alloc(&d_st, (size_t)c->nv * c->n))) return rc;
+        HIPCHK(c, launch_to_stage<double>(c->d_u0, d_st, c->nv, c->n, c->stream));
+        KArgs<double> a = make_args<double>(c, c->geo_d);
+        a.us = d_st;
+        a.qf_out = nullptr;
+        const hipError_t e = c->dim == 2 ? launch_gradient<2, double>(a, c->stream) : launch_gradient<3, double>(a, c->stream);
+        HIPCHK(c, e);
+        t.k.grad = (const double*)c->work_grad[0];
+    }
     return TY_OK;
 }
 
@@ -1544,9 +1558,11 @@ int ty_residual_tangent(ty_ctx* c, double dt, const double* du, const double* ds
     if (!c || !dres) return TY_ERR_ARG;
     AdArgs t;
     int rc;
-    if ((rc = ad_prepare(c, dt, t))) return rc;
-    const int n = c->n, nf = c->nf, nv = c->nv;
     DevTmp tmp(c);
+    if (c->ph.viscous && ds)
+        return fail(c, TY_ERR_UNSUPPORTED, "tangent of the viscous residual: area-vector directions not provided");
+    if ((rc = ad_prepare(c, dt, t, tmp))) return rc;
+    const int n = c->n, nf = c->nf, nv = c->nv;
     double *d_du = nullptr, *d_ds = nullptr, *d_dv = nullptr, *d_dq = nullptr, *d_out = nullptr;
     std::vector<double> h;
     if (du) {
@@ -1581,6 +1597,11 @@ int ty_residual_tangent(ty_ctx* c, double dt, const double* du, const double* ds
     t.dvol = d_dv;
     t.dq = d_dq;
     t.out = d_out;
+    if (c->ph.viscous) {
+        double* d_dg = nullptr;
+        if ((rc = tmp.alloc(&d_dg, (size_t)nv * c->dim * n))) return rc;
+        t.dgrad = d_dg;
+    }
     {
         EventScope ev(c, c->ev_tan);
         HIPCHK(c, c->dim == 2 ? launch_res_tangent<2>(t, c->stream) : launch_res_tangent<3>(t, c->stream));
@@ -1597,9 +1618,13 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
     if (!c || !lam || !gu) return TY_ERR_ARG;
     AdArgs t;
     int rc;
-    if ((rc = ad_prepare(c, dt, t))) return rc;
-    const int n = c->n, nf = c->nf, nb = c->nb, nv = c->nv;
     DevTmp tmp(c);
+    if (c->ph.viscous && gs)
+        return fail(c, TY_ERR_UNSUPPORTED, "adjoint of the viscous residual: area-vector sensitivities not provided");
+    if (c->ph.viscous && c->ph.sch_hyp != 25 && c->ph.sch_hyp != 26)
+        return fail(c, TY_ERR_UNSUPPORTED, "adjoint of the viscous residual: HLLC/HLLCB fluxes only");
+    if ((rc = ad_prepare(c, dt, t, tmp))) return rc;
+    const int n = c->n, nf = c->nf, nb = c->nb, nv = c->nv;
     double *d_lam = nullptr, *d_gu = nullptr, *d_gs = nullptr, *d_gv = nullptr, *d_gq = nullptr;
     std::vector<double> h;
     host_to_dev_vec(c, lam, h);
@@ -1608,9 +1633,16 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
     if (gs && (rc = tmp.alloc(&d_gs, (size_t)3 * nf))) return rc;
     if (gvol && (rc = tmp.alloc(&d_gv, (size_t)n))) return rc;
     if (gq && (rc = tmp.alloc(&d_gq, (size_t)nv * std::max(1, nb)))) return rc;
-    double* d_face = nullptr;  // per face side: state adjoint and volume-sensitivity share
-    if ((rc = tmp.alloc(&d_face, (size_t)2 * (nv + 1) * nf))) return rc;
+    const int rw = nv + (c->ph.viscous ? nv * c->dim : 0) + 1;
+    double* d_face = nullptr;  // per face side: state (and gradient) adjoint, volume-sensitivity share
+    if ((rc = tmp.alloc(&d_face, (size_t)2 * rw * nf))) return rc;
     t.face = d_face;
+    if (c->ph.viscous) {
+        double *d_fg = nullptr, *d_sl = nullptr;
+        if ((rc = tmp.alloc(&d_fg, (size_t)6 * nf)) || (rc = tmp.alloc(&d_sl, (size_t)c->ns * n * nv))) return rc;
+        t.fgeo = d_fg;
+        t.slot = d_sl;
+    }
     t.lam = d_lam;
     t.gu = d_gu;
     t.gs = d_gs;
@@ -1642,6 +1674,259 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
         std::fill(gq, gq + (size_t)5 * nbc, 0.0);
         for (int k = 0; k < nb; ++k)  // fixed order: device boundary faces
             for (int v = 0; v < nv; ++v) gq[(size_t)5 * c->bboco[k] + v] += hq[(size_t)v * nb + k];
+    }
+    return TY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Unsteady adjoint: reverse sweep over RK steps with device-resident checkpoints.
+
+namespace {
+
+// Binomial checkpointing (Griewank, Optim. Methods Softw. 1 (1992)): with `snaps` free
+// snapshots over `steps` steps the repetition number r is the smallest with
+// C(snaps + r, snaps) >= steps; the right part of the range keeps C(snaps - 1 + r, snaps - 1) steps.
+double binom(int n, int k)
+{
+    if (k < 0 || k > n) return 0.0;
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+int tv_split(int steps, int snaps)
+{
+    if (steps < 2) return 1;
+    int r = 0;
+    while (binom(snaps + r, snaps) < steps) ++r;
+    const double right = snaps >= 1 ? binom(snaps - 1 + r, snaps - 1) : 1.0;
+    const double left = steps - right;
+    return (int)std::min<double>(std::max<double>(1.0, left), steps - 1);
+}
+
+struct Unsteady {
+    ty_ctx* c = nullptr;
+    long len = 0;                // nv n
+    int ns = 0;                  // RK stages
+    const double* dts = nullptr;  // [nsteps] device
+    std::vector<double*> pool;   // snapshot buffers
+    double* work = nullptr;      // recomputation state for ranges without snapshots
+    double* stU = nullptr;       // [ns - 1][len] stage states 2..ns
+    double* stR = nullptr;       // [ns][len] stage residuals
+    double* nu = nullptr;        // [ns][len] stage-state adjoints
+    double* mu = nullptr;        // [len] stage-residual adjoint
+    double* rec = nullptr;       // [len] stage state as residual-kernel records
+    double* gq_tmp = nullptr;    // [nv][nb] inlet sensitivities of one residual adjoint
+    double* gq_acc = nullptr;    // [nv][nb] summed
+    AdArgs t;
+    int live = 0, fwd = 0, adj = 0, maxlive = 0;
+
+    double coef(int i, int j) const { return c->rk[(size_t)(i - 1) * ns + (j - 1)]; }
+
+    int set_dt(int k)
+    {
+        HIPCHK(c, hipMemcpyAsync(c->d_dt, dts + k, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        return TY_OK;
+    }
+
+    // U <- state after step k, by the context's own RK path
+    int advance(double* U, int k)
+    {
+        int rc;
+        if ((rc = set_dt(k))) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->d_u0, U, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        c->cur = 0;
+        hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[0], c->nv, c->n, c->stream)
+                                : launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], c->nv, c->n, c->stream);
+        HIPCHK(c, e);
+        if ((rc = run_step(c))) return rc;
+        HIPCHK(c, hipMemcpyAsync(U, c->d_u0, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        ++fwd;
+        return TY_OK;
+    }
+
+    // out = (dR/dU at U)^T m, FP64; inlet sensitivities added to gq_acc
+    int adjoint_apply(const double* U, const double* m, double* out)
+    {
+        AdArgs a = t;
+        a.k.us = U;
+        if (c->ph.viscous) {
+            HIPCHK(c, launch_to_stage<double>(U, rec, c->nv, c->n, c->stream));
+            KArgs<double> g = make_args<double>(c, c->geo_d);
+            g.us = rec;
+            g.qf_out = nullptr;
+            const hipError_t e = c->dim == 2 ? launch_gradient<2, double>(g, c->stream)
+                                             : launch_gradient<3, double>(g, c->stream);
+            HIPCHK(c, e);
+            a.k.grad = (const double*)c->work_grad[0];
+        }
+        a.lam = m;
+        a.gu = out;
+        a.gq = gq_tmp;
+        HIPCHK(c, c->dim == 2 ? launch_res_adjoint<2>(a, c->stream) : launch_res_adjoint<3>(a, c->stream));
+        if (gq_acc) HIPCHK(c, accumulate<double>(gq_acc, gq_tmp, (long)c->nv * c->nb, c->stream));
+        return TY_OK;
+    }
+
+    // lam <- (dU_{k+1}/dU_k)^T lam, step k starting from Uk (treelevel_rungekutta.f90:60-105:
+    // U_s = U_k + sum_{j<s} coef(s-1, j) R_j, U_{k+1} = U_k + sum_j coef(ns, j) R_j)
+    int step_adjoint(int k, const double* Uk, double* lam)
+    {
+        int rc;
+        if ((rc = set_dt(k))) return rc;
+        const double* Us[MAX_STAGES];
+        Us[0] = Uk;
+        for (int s = 1; s <= ns; ++s) {
+            HIPCHK(c, launch_to_stage<double>(Us[s - 1], rec, c->nv, c->n, c->stream));
+            HIPCHK(c, eval_residual<double>(c, rec, stR + (size_t)(s - 1) * len, nullptr, nullptr));
+            if (s == ns) break;
+            Comb cb;
+            memset(&cb, 0, sizeof(cb));
+            cb.v[0] = Uk;
+            cb.c[0] = 1.0;
+            cb.m = 1;
+            for (int j = 1; j <= s; ++j) {
+                cb.v[cb.m] = stR + (size_t)(j - 1) * len;
+                cb.c[cb.m] = coef(s, j);
+                ++cb.m;
+            }
+            double* out = stU + (size_t)(s - 1) * len;
+            HIPCHK(c, launch_comb(out, cb, len, c->stream));
+            Us[s] = out;
+        }
+        // reverse: mu_s = coef(ns, s) lam + sum_{m>s} coef(m-1, s) nu_m, nu_s = (dR/dU at U_s)^T mu_s
+        for (int s = ns; s >= 1; --s) {
+            Comb cb;
+            memset(&cb, 0, sizeof(cb));
+            cb.v[0] = lam;
+            cb.c[0] = coef(ns, s);
+            cb.m = 1;
+            for (int m = s + 1; m <= ns; ++m) {
+                cb.v[cb.m] = nu + (size_t)(m - 1) * len;
+                cb.c[cb.m] = coef(m - 1, s);
+                ++cb.m;
+            }
+            HIPCHK(c, launch_comb(mu, cb, len, c->stream));
+            if ((rc = adjoint_apply(Us[s - 1], mu, nu + (size_t)(s - 1) * len))) return rc;
+        }
+        // lam_k = lam + sum_s nu_s
+        Comb cb;
+        memset(&cb, 0, sizeof(cb));
+        cb.v[0] = lam;
+        cb.c[0] = 1.0;
+        cb.m = 1;
+        for (int s = 1; s <= ns; ++s) {
+            cb.v[cb.m] = nu + (size_t)(s - 1) * len;
+            cb.c[cb.m] = 1.0;
+            ++cb.m;
+        }
+        HIPCHK(c, launch_comb(lam, cb, len, c->stream));
+        ++adj;
+        return TY_OK;
+    }
+
+    // lam: adjoint at step b on entry, at step a on return; Ua: state at step a (kept)
+    int treeverse(int a, int b, const double* Ua, double* lam, int snaps)
+    {
+        int rc;
+        if (b - a == 1) return step_adjoint(a, Ua, lam);
+        if (snaps == 0) {
+            for (int k = b - 1; k >= a; --k) {
+                HIPCHK(c, hipMemcpyAsync(work, Ua, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+                for (int j = a; j < k; ++j)
+                    if ((rc = advance(work, j))) return rc;
+                if ((rc = step_adjoint(k, work, lam))) return rc;
+            }
+            return TY_OK;
+        }
+        const int m = a + tv_split(b - a, snaps);
+        double* Um = pool[live++];
+        maxlive = std::max(maxlive, live);
+        HIPCHK(c, hipMemcpyAsync(Um, Ua, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        for (int j = a; j < m; ++j)
+            if ((rc = advance(Um, j))) return rc;
+        if ((rc = treeverse(m, b, Um, lam, snaps - 1))) return rc;
+        --live;
+        return treeverse(a, m, Ua, lam, snaps);
+    }
+};
+
+}  // namespace
+
+int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t snaps, const double* lamN,
+                        double* lam0, double* gq, ty_unsteady_stats* stats)
+{
+    if (!c || nsteps < 0 || snaps < 0 || (nsteps > 0 && !dts) || !lamN || !lam0) return TY_ERR_ARG;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (c->ph.viscous && c->ph.sch_hyp != 25 && c->ph.sch_hyp != 26)
+        return fail(c, TY_ERR_UNSUPPORTED, "adjoint of the viscous residual: HLLC/HLLCB fluxes only");
+    DevTmp tmp(c);
+    Unsteady u;
+    int rc;
+    if ((rc = ad_prepare(c, nsteps > 0 ? dts[0] : 0.0, u.t, tmp))) return rc;
+    if ((rc = ensure_work(c, c->mixed ? 1 : 0))) return rc;
+    u.c = c;
+    u.ns = c->sch.nstage;
+    const int n = c->n, nv = c->nv, nb = c->nb, nf = c->nf;
+    u.len = (long)nv * n;
+    const size_t len = (size_t)u.len;
+    double *d_dts = nullptr, *U0 = nullptr, *lam = nullptr;
+    if ((rc = tmp.alloc(&d_dts, (size_t)std::max(1, nsteps))) || (rc = tmp.alloc(&U0, len)) ||
+        (rc = tmp.alloc(&lam, len)) || (rc = tmp.alloc(&u.work, len)) ||
+        (rc = tmp.alloc(&u.stU, (size_t)std::max(1, u.ns - 1) * len)) || (rc = tmp.alloc(&u.stR, (size_t)u.ns * len)) ||
+        (rc = tmp.alloc(&u.nu, (size_t)u.ns * len)) || (rc = tmp.alloc(&u.mu, len)) || (rc = tmp.alloc(&u.rec, len)) ||
+        (rc = tmp.alloc(&u.gq_tmp, (size_t)nv * std::max(1, nb))))
+        return rc;
+    u.pool.assign(snaps, nullptr);
+    for (int k = 0; k < snaps; ++k)
+        if ((rc = tmp.alloc(&u.pool[k], len))) return rc;
+    if (gq) {
+        if ((rc = tmp.alloc(&u.gq_acc, (size_t)nv * std::max(1, nb)))) return rc;
+        HIPCHK(c, hipMemsetAsync(u.gq_acc, 0, (size_t)nv * std::max(1, nb) * sizeof(double), c->stream));
+    }
+    // residual-adjoint work buffers (as ty_residual_adjoint)
+    const int rw = nv + (c->ph.viscous ? nv * c->dim : 0) + 1;
+    double* d_face = nullptr;
+    if ((rc = tmp.alloc(&d_face, (size_t)2 * rw * nf))) return rc;
+    u.t.face = d_face;
+    if (c->ph.viscous) {
+        double *d_fg = nullptr, *d_sl = nullptr;
+        if ((rc = tmp.alloc(&d_fg, (size_t)6 * nf)) || (rc = tmp.alloc(&d_sl, (size_t)c->ns * n * nv))) return rc;
+        u.t.fgeo = d_fg;
+        u.t.slot = d_sl;
+    }
+    u.t.gs = nullptr;
+    u.t.gvol = nullptr;
+    if (nsteps > 0)
+        HIPCHK(c, hipMemcpy(d_dts, dts, (size_t)nsteps * sizeof(double), hipMemcpyHostToDevice));
+    u.dts = d_dts;
+    std::vector<double> h;
+    host_to_dev_vec(c, lamN, h);
+    HIPCHK(c, hipMemcpy(lam, h.data(), len * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpyAsync(U0, c->d_u0, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (nsteps > 0 && (rc = u.treeverse(0, nsteps, U0, lam, snaps))) return rc;
+    // the state is U_0 again
+    HIPCHK(c, hipMemcpyAsync(c->d_u0, U0, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    c->cur = 0;
+    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[0], nv, n, c->stream)
+                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], nv, n, c->stream);
+    HIPCHK(c, e);
+    HIPCHK(c, hipMemcpyAsync(h.data(), lam, len * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> hq(gq ? (size_t)nv * nb : 0);
+    if (gq && nb) HIPCHK(c, hipMemcpyAsync(hq.data(), u.gq_acc, hq.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_to_host_vec(c, h, lam0);
+    if (gq) {
+        const int nbc = (int)c->bocos.size();
+        std::fill(gq, gq + (size_t)5 * nbc, 0.0);
+        for (int k = 0; k < nb; ++k)
+            for (int v = 0; v < nv; ++v) gq[(size_t)5 * c->bboco[k] + v] += hq[(size_t)v * nb + k];
+    }
+    if (stats) {
+        stats->forward_steps = u.fwd;
+        stats->adjoint_steps = u.adj;
+        stats->max_snapshots = u.maxlive;
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return TY_OK;
 }

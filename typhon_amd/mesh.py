@@ -349,21 +349,40 @@ def channel_quad(nx=256, ny=128, lx=1.0, ly=0.5, jitter=0.1, seed=1, bocos=None)
     return build_ustmesh(2, vtx, cells, "quad", fam, bocos, wrap)
 
 
+def _axes(periodic):
+    """per-axis periodic flags from a bool or a 3-sequence"""
+    if isinstance(periodic, (bool, np.bool_)):
+        return (bool(periodic),) * 3
+    return tuple(bool(p) for p in periodic)
+
+
 def _box_vertices(n, L, jitter, seed, periodic):
-    if periodic and n < 3:
+    axes = _axes(periodic)
+    if any(axes) and n < 3:
         # partner faces are found through vertex ids taken modulo n: at n = 2 two different
         # faces of one side carry the same set of wrapped ids
         raise ValueError("a periodic box needs n >= 3")
     h = L / n
     i, j, k = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
     i, j, k = i.ravel(), j.ravel(), k.ravel()
-    if periodic:
+    if all(axes):
         wi, wj, wk = i % n, j % n, k % n
         rid = (wi * n + wj) * n + wk
         dx = jitter * h * uniform_pm1(seed, rid, 0)
         dy = jitter * h * uniform_pm1(seed, rid, 1)
         dz = jitter * h * uniform_pm1(seed, rid, 2)
         wrap = ((wi * (n + 1) + wj) * (n + 1) + wk)
+    elif any(axes):
+        # periodic along some axes: ids wrap there; walls (the other axes' end planes) stay flat
+        wi, wj, wk = (idx % n if ax else idx for idx, ax in zip((i, j, k), axes))
+        wrap = (wi * (n + 1) + wj) * (n + 1) + wk
+        inner = np.ones(i.shape, dtype=bool)
+        for idx, ax in zip((i, j, k), axes):
+            if not ax:
+                inner &= (idx > 0) & (idx < n)
+        dx = np.where(inner, jitter * h * uniform_pm1(seed, wrap, 0), 0.0)
+        dy = np.where(inner, jitter * h * uniform_pm1(seed, wrap, 1), 0.0)
+        dz = np.where(inner, jitter * h * uniform_pm1(seed, wrap, 2), 0.0)
     else:
         vid = (i * (n + 1) + j) * (n + 1) + k
         inner = (i > 0) & (i < n) & (j > 0) & (j < n) & (k > 0) & (k < n)
@@ -373,6 +392,17 @@ def _box_vertices(n, L, jitter, seed, periodic):
         wrap = None
     vtx = np.stack([i * h + dx, j * h + dy, k * h + dz], axis=1)
     return vtx, wrap
+
+
+def _box_bocos(periodic):
+    """periodic pairs along the periodic axes, slip walls (SYMMETRY) on the others"""
+    out = []
+    for ax, (lo, hi) in zip(_axes(periodic), (("IMIN", "IMAX"), ("JMIN", "JMAX"), ("KMIN", "KMAX"))):
+        if ax:
+            out += [dict(family=lo, type=BC_PERIOD, partner=hi), dict(family=hi, type=BC_PERIOD, partner=lo)]
+        else:
+            out += [dict(family=lo, type=BC_SYM), dict(family=hi, type=BC_SYM)]
+    return out
 
 
 def _box_family(n):
@@ -388,15 +418,6 @@ def _box_family(n):
     return fam
 
 
-def _periodic_box_bocos():
-    return [dict(family="IMIN", type=BC_PERIOD, partner="IMAX"),
-            dict(family="IMAX", type=BC_PERIOD, partner="IMIN"),
-            dict(family="JMIN", type=BC_PERIOD, partner="JMAX"),
-            dict(family="JMAX", type=BC_PERIOD, partner="JMIN"),
-            dict(family="KMIN", type=BC_PERIOD, partner="KMAX"),
-            dict(family="KMAX", type=BC_PERIOD, partner="KMIN")]
-
-
 def box_tets(n=100, L=10.0, jitter=0.05, seed=2, periodic=True, bocos=None):
     """n^3 hexes on [0,L]^3, each split into 6 tets about the main diagonal
     (Kuhn/Freudenthal split, conforming between neighbours)."""
@@ -410,8 +431,7 @@ def box_tets(n=100, L=10.0, jitter=0.05, seed=2, periodic=True, bocos=None):
     kuhn = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
     cells = np.stack([np.stack([h[a], h[b], h[c], h[d]], axis=1) for a, b, c, d in kuhn], axis=1).reshape(-1, 4)
     if bocos is None:
-        bocos = _periodic_box_bocos() if periodic else [dict(family=f, type=BC_SYM) for f in
-                                                         ("IMIN", "IMAX", "JMIN", "JMAX", "KMIN", "KMAX")]
+        bocos = _box_bocos(periodic)
     return build_ustmesh(3, vtx, cells, "tet", _box_family(n), bocos, wrap)
 
 
@@ -464,7 +484,7 @@ def bump_channel_tets(nx=63, ny=33, nz=20, lx=3.0, ly=1.0, lz=0.6, jitter=0.05, 
 
 
 def box_hex(n=16, L=2 * np.pi, jitter=0.0, seed=3, periodic=True, bocos=None):
-    """n^3 hexes on [0,L]^3."""
+    """n^3 hexes on [0,L]^3; periodic: bool or per-axis (x, y, z) flags (slip walls elsewhere)."""
     vtx, wrap = _box_vertices(n, L, jitter, seed, periodic)
     ci, cj, ck = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
     ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
@@ -474,8 +494,7 @@ def box_hex(n=16, L=2 * np.pi, jitter=0.0, seed=3, periodic=True, bocos=None):
     cells = np.stack([v(0, 0, 0), v(1, 0, 0), v(1, 1, 0), v(0, 1, 0),
                       v(0, 0, 1), v(1, 0, 1), v(1, 1, 1), v(0, 1, 1)], axis=1)
     if bocos is None:
-        bocos = _periodic_box_bocos() if periodic else [dict(family=f, type=BC_SYM) for f in
-                                                         ("IMIN", "IMAX", "JMIN", "JMAX", "KMIN", "KMAX")]
+        bocos = _box_bocos(periodic)
     return build_ustmesh(3, vtx, cells, "hex", _box_family(n), bocos, wrap)
 
 

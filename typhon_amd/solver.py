@@ -20,7 +20,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .scheme import default_gas, default_scheme, rk_tableau
+from .scheme import EQ_NSLAM, default_gas, default_scheme, rk_tableau
 
 
 def _p64(a):
@@ -100,6 +100,11 @@ class Solver:
         self._check(self.lib.ty_set_scheme(self.h, C.byref(gd), C.byref(sd)), "ty_set_scheme")
         self.scheme, self.gas = s, g
 
+    @property
+    def viscous(self) -> bool:
+        """Navier-Stokes residual (the viscous flux reads cell gradients)"""
+        return self.gas["typ_fluid"] == EQ_NSLAM
+
     # -- reference interface ----------------------------------------------
     def set_state(self, cons):
         cons = np.asarray(cons, dtype=np.float64).reshape(5, self.ni)
@@ -160,7 +165,8 @@ class Solver:
     def residual_tangent(self, dt, du=None, ds=None, dvol=None, dq=None):
         """exact directional derivative of the first-order residual at the current state and a
         fixed dt: du [5, ncell_int], ds [nface, 3] face area vectors S = A n, dvol [ncell_int],
-        dq [nboco, 5] supersonic-inlet ghost state (rho, p, u, v, w); absent = zero"""
+        dq [nboco, 5] supersonic-inlet ghost state (rho, p, u, v, w); absent = zero.  Viscous
+        residuals take no ds (their flux also reads cell and face centres)."""
         m = self.mesh
 
         def arr(x, shape):
@@ -174,18 +180,35 @@ class Solver:
         self._check(self.lib.ty_residual_tangent(self.h, dt, *p, _p64(out)), "ty_residual_tangent")
         return out
 
-    def residual_adjoint(self, dt, lam):
-        """transpose of residual_tangent: dict u [5, ncell_int], s [nface, 3], vol [ncell_int],
-        q [nboco, 5] with the products (dR/d.)^T lam"""
+    def residual_adjoint(self, dt, lam, area_vectors=True):
+        """transpose of residual_tangent: dict u [5, ncell_int], s [nface, 3] (None unless
+        area_vectors; not provided for viscous residuals), vol [ncell_int], q [nboco, 5] with the
+        products (dR/d.)^T lam"""
         m = self.mesh
         lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(5, self.ni)
         gu = np.empty((5, self.ni))
-        gs = np.empty((m.nface, 3))
+        gs = np.empty((m.nface, 3)) if area_vectors else None
         gv = np.empty(self.ni)
         gq = np.empty((max(1, len(m.bocos)), 5))
-        self._check(self.lib.ty_residual_adjoint(self.h, dt, _p64(lam), _p64(gu), _p64(gs), _p64(gv), _p64(gq)),
-                    "ty_residual_adjoint")
+        self._check(self.lib.ty_residual_adjoint(self.h, dt, _p64(lam), _p64(gu), None if gs is None else _p64(gs),
+                                                 _p64(gv), _p64(gq)), "ty_residual_adjoint")
         return dict(u=gu, s=gs, vol=gv, q=gq[:len(m.bocos)])
+
+    def unsteady_adjoint(self, dts, lamN, snaps=20):
+        """dJ/dU_0 and dJ/dq (inlet ghost states) of an objective of the state after len(dts) RK
+        steps from the current state, given lamN = dJ/dU_N [5, ncell_int]; checkpoints in device
+        memory (at most ``snaps`` besides U_0).  Returns dict(u, q, stats); the state is U_0 again."""
+        m = self.mesh
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        lamN = np.ascontiguousarray(lamN, dtype=np.float64).reshape(5, self.ni)
+        lam0 = np.empty((5, self.ni))
+        gq = np.empty((max(1, len(m.bocos)), 5))
+        st = _lib.TyUnsteadyStats()
+        self._check(self.lib.ty_unsteady_adjoint(self.h, dts.shape[0], _p64(dts), snaps, _p64(lamN), _p64(lam0),
+                                                 _p64(gq), C.byref(st)), "ty_unsteady_adjoint")
+        stats = dict(forward_steps=st.forward_steps, adjoint_steps=st.adjoint_steps,
+                     max_snapshots=st.max_snapshots, ms=st.ms)
+        return dict(u=lam0, q=gq[:len(m.bocos)], stats=stats)
 
     def linear_solve(self, dt, b, kry=None, allow_unconverged=False):
         """solve (I - J) x = b at the current state; returns (x, stats)"""
