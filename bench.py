@@ -92,11 +92,13 @@ def cpu_baseline(n_cpu, budget_s=20.0):
 
 PROFILE_CMDS = {
     "trace": "rocprofv3 --kernel-trace --stats -f csv -d {d}/trace -o c2 -- python3 bench.py --steps 20 --warmup 2 "
-             "--no-cpu --no-mixed --no-adjoint --no-implicit --no-ns",
+             "--no-cpu --no-implicit --no-ns",
     "fetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/fetch -o c2 -- python3 bench.py --steps 2 --warmup 1 --no-cpu "
-             "--no-mixed --no-adjoint --no-implicit --no-ns",
+             "--no-adjoint --no-implicit --no-ns",
     "write": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/write -o c2 -- python3 bench.py --steps 2 --warmup 1 --no-cpu "
-             "--no-mixed --no-adjoint --no-implicit --no-ns",
+             "--no-adjoint --no-implicit --no-ns",
+    "ns": "rocprofv3 --kernel-trace --stats -f csv -d {d}/ns -o ns -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
+          "--no-mixed --no-adjoint --no-implicit --ns-steps 20 --ns-adj-steps 4",
     "cfetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/cfetch -o copy -- python3 bench.py --copy-calibration 1024",
     "cwrite": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/cwrite -o copy -- python3 bench.py --copy-calibration 1024",
 }
@@ -125,6 +127,7 @@ def summarize_profiles(d, n=100):
     nc = 6 * n ** 3
     nf = 2 * nc  # periodic Kuhn-split box: every face is shared by two tets
     bm = byte_model(nc, nf, 0, mixed=False)
+    bmm = byte_model(nc, nf, 0, mixed=True)
     stats = _csv_rows(os.path.join(d, "trace", "c2_kernel_stats.csv"))
 
     def pmc(sub, prefix, counter):
@@ -147,18 +150,19 @@ def summarize_profiles(d, n=100):
     fe, wr = pmc("fetch", "c2", "FETCH_SIZE"), pmc("write", "c2", "WRITE_SIZE")
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     lines = ["# Residual kernels, 3D Euler MUSCL/LSQ/Venkatakrishnan + HLLC + RK4, 6,000,000 periodic tets (FP64 and FP32)",
-             "", "Commands (from the repository root, output directory D):", ""]
+             "", "The trace run also holds the adjoint workload's kernels (first-order HLLC residual and its adjoint on "
+             "the 249,480-tet bump channel, 20 launches each).", "",
+             "Commands (from the repository root, output directory D):", ""]
     lines += ["    " + c.format(d="D") for c in PROFILE_CMDS.values()]
     lines += ["    python3 bench.py --summarize D", "",
               "## Kernel durations (rocprofv3 --kernel-trace --stats)", "",
               "| kernel | calls | avg us | min us | max us | share % |", "|---|---|---|---|---|---|"]
     for r in stats:
-        tag, prec = _kernel_key(r["Name"])
-        if tag is None:
+        if "ty::" not in r["Name"]:
             continue
         lines.append(f"| {r['Name'].replace('|', '/')} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                      f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
-    lines += ["", "## HBM traffic per launch (FP64 kernels)", "",
+    lines += ["", "## HBM traffic per launch (FP64 and FP32 residual kernels of the 6M-tet workload)", "",
               f"Copy calibration (1 GiB device copy): FETCH_SIZE factor {f_fac:.3f}, WRITE_SIZE factor {w_fac:.3f} "
               "(bytes moved / counter bytes).", "",
               "| kernel | FETCH_SIZE GB (raw) | read GB (scaled) | WRITE_SIZE GB (raw) | written GB (scaled) | "
@@ -166,21 +170,40 @@ def summarize_profiles(d, n=100):
     traffic = {}
     for name, vals in fe.items():
         tag, prec = _kernel_key(name)
-        if tag not in ("flux", "grad") or prec != "double":
+        if tag not in ("flux", "grad") or prec not in ("double", "float"):
             continue
         f_raw = sum(vals) / len(vals)
         wv = wr.get(name, [])
         w_raw = sum(wv) / len(wv) if wv else float("nan")
         tot = f_raw * f_fac + w_raw * w_fac
-        alg = bm[tag]
-        traffic[tag] = dict(kernel=name, fetch_raw=f_raw, write_raw=w_raw, fetch_factor=f_fac, write_factor=w_fac,
+        alg = (bm if prec == "double" else bmm)[tag]
+        key = tag if prec == "double" else f"{tag}_fp32"
+        traffic[key] = dict(kernel=name, fetch_raw=f_raw, write_raw=w_raw, fetch_factor=f_fac, write_factor=w_fac,
                             bytes_per_launch=int(tot), algorithmic_bytes=int(alg), launches=len(vals))
         lines.append(f"| {name} | {f_raw / 1e9:.3f} | {f_raw * f_fac / 1e9:.3f} | {w_raw / 1e9:.3f} | "
                      f"{w_raw * w_fac / 1e9:.3f} | {tot / 1e9:.3f} | {alg / 1e9:.3f} | {tot / alg:.2f} |")
     with open(os.path.join(ROOT, "profiles", "residual_kernels_c2.md"), "w") as f:
         f.write("\n".join(lines) + "\n")
-    for tag, t in traffic.items():
-        with open(os.path.join(ROOT, "profiles", f"traffic_{tag}_c2_fp64.json"), "w") as f:
+    ns_stats = os.path.join(d, "ns", "ns_kernel_stats.csv")
+    if os.path.exists(ns_stats):
+        nl = ["# NS kernels, 256^3 = 16,777,216 hexes (periodic x/z, slip walls y), first-order HLLC + cell-full "
+              "viscous flux, LSQ gradients, RK4", "",
+              "FP64 and mixed residual kernels (20 RK4 steps each) and the FP64 adjoint kernels of a 4-step "
+              "checkpointed unsteady adjoint (viscous face reverse sweep, gather with the LSQ-gradient transpose, "
+              "tail gather, FP64 stage residuals).  The run also holds one FP64 step of the 6M-tet workload.", "",
+              "Command (from the repository root, output directory D):", "",
+              "    " + PROFILE_CMDS["ns"].format(d="D"), "    python3 bench.py --summarize D", "",
+              "| kernel | calls | avg us | min us | max us | share % |", "|---|---|---|---|---|---|"]
+        for r in _csv_rows(ns_stats):
+            if "ty::" not in r["Name"]:
+                continue
+            nl.append(f"| {r['Name'].replace('|', '/')} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                      f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+        with open(os.path.join(ROOT, "profiles", "ns_kernels_16m.md"), "w") as f:
+            f.write("\n".join(nl) + "\n")
+    for key, t in traffic.items():
+        name = f"traffic_{key[:-5]}_c2_fp32.json" if key.endswith("_fp32") else f"traffic_{key}_c2_fp64.json"
+        with open(os.path.join(ROOT, "profiles", name), "w") as f:
             json.dump(t, f, indent=1)
     print("\n".join(lines))
 
