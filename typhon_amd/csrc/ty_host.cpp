@@ -700,6 +700,12 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     ph.lsqw = s->gradmeth == 12;
     ph.barth = s->method != 'N' && s->limiter != 'W' && s->postlimiter != 0;
     ph.klim = s->postlimiter == 21 ? 1.0 : 0.5;
+    ph.ig1 = 1.0 / (gas->gamma - 1.0);
+    ph.gig1 = gas->gamma / (gas->gamma - 1.0);
+    ph.irgas = 1.0 / gas->r_const;
+    ph.cp_pr = gas->gamma * gas->r_const / (gas->gamma - 1.0) / gas->prandtl;
+    ph.itref = gas->tref != 0.0 ? 1.0 / gas->tref : 0.0;
+    ph.tsum = gas->tref + gas->tsuth;
     // boundary table: supersonic inlet state from pi_ti_mach_dir2nspri (EQNS.f90:139-160)
     std::vector<BocoDev> tab(std::max<size_t>(1, c->bocos.size()));
     for (size_t b = 0; b < c->bocos.size(); ++b) {

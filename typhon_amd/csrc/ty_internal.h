@@ -57,6 +57,9 @@ struct Physics {
     int lsqw;      // weighted LSQ
     int barth;     // post-limiter on
     double klim;
+    // launch constants derived on the host: 1/(gamma-1), gamma/(gamma-1), 1/r,
+    // conductivity factor cp/Pr, 1/Tref and Tref + S of Sutherland's law
+    double ig1, gig1, irgas, cp_pr, itref, tsum;
 };
 
 // Everything one launch of the residual kernels needs, passed by value.

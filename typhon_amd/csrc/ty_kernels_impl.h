@@ -20,6 +20,22 @@ __device__ __forceinline__ R fmin_(R a, R b) { return a < b ? a : b; }
 template <typename R>
 __device__ __forceinline__ R fmax_(R a, R b) { return a > b ? a : b; }
 
+// Reciprocal, quotient and square root in the arithmetic precision of the residual kernels:
+// IEEE-rounded in FP64 (the reference's arithmetic); in the FP32 form of the mixed mode the
+// hardware v_rcp_f32 / v_sqrt_f32 (about 1 ulp) instead of the correctly rounded sequences.
+template <typename R>
+__device__ __forceinline__ R arcp(R x) { return R(1) / x; }
+template <typename R>
+__device__ __forceinline__ R adiv(R a, R b) { return a / b; }
+template <typename R>
+__device__ __forceinline__ R asqrt(R x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float arcp(float x) { return __builtin_amdgcn_rcpf(x); }
+template <>
+__device__ __forceinline__ float adiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <>
+__device__ __forceinline__ float asqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ __forceinline__ double dpow(double a, double b) { return pow(a, b); }
 __device__ __forceinline__ float dpow(float a, float b) { return powf(a, b); }
 
@@ -33,15 +49,15 @@ __device__ __forceinline__ int xcd_block()
     return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
+// calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:47-52) on the cell's record; ir = 1/rho
 template <int DIM, typename R>
-__device__ __forceinline__ void load_prim(const R* __restrict__ us, int c, R g1, R* q)
+__device__ __forceinline__ void load_prim(const R* __restrict__ us, int c, R g1, R* q, R& ir)
 {
-    // calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:47-52) on the cell's record
     constexpr int NV = DIM + 2;
     const R* u = us + (size_t)c * NV;
     const R rho = u[0];
     const R rhoE = u[1];
-    const R ir = R(1) / rho;
+    ir = arcp(rho);
     R vel[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) vel[d] = u[2 + d] * ir;
@@ -53,6 +69,13 @@ __device__ __forceinline__ void load_prim(const R* __restrict__ us, int c, R g1,
     q[1] = g1 * (rhoE - ec);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) q[2 + d] = vel[d];
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void load_prim(const R* __restrict__ us, int c, R g1, R* q)
+{
+    R ir;
+    load_prim<DIM>(us, c, g1, q, ir);
 }
 
 template <int DIM, typename R>
@@ -165,12 +188,12 @@ __device__ __forceinline__ void bc_grad(const BocoDev& b, const R* n, const R* g
 // speed a2 = (gamma - 1)(h - |v|^2 / 2) (the reference's gamma p / rho of its averaged
 // pressure p = (h - |v|^2/2) rho (gamma - 1) / gamma, without the two divisions).
 template <int DIM, typename R>
-__device__ __forceinline__ void roe_avg(R g, const R* qL, const R* qR, R irL, R irR, R& rho, R* vel, R& h, R& a2)
+__device__ __forceinline__ void roe_avg(R g, R gig1, const R* qL, const R* qR, R irL, R irR, R& rho, R* vel, R& h,
+                                        R& a2)
 {
-    const R gig1 = g / (g - R(1));
-    const R t1 = sqrt(qR[0] * irL);
+    const R t1 = asqrt(qR[0] * irL);
     rho = qL[0] * t1;
-    const R kl = R(1) / (R(1) + t1);
+    const R kl = arcp(R(1) + t1);
     const R kr = t1 * kl;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) vel[d] = kl * qL[2 + d] + kr * qR[2 + d];
@@ -180,16 +203,16 @@ __device__ __forceinline__ void roe_avg(R g, const R* qL, const R* qR, R irL, R 
     a2 = (g - R(1)) * (h - R(.5) * dotd<DIM>(vel, vel));
 }
 
+// ig1 = 1/(gamma - 1), gig1 = gamma/(gamma - 1), irL/irR the reciprocal densities
 template <int DIM, typename R>
-__device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* F)
+__device__ __forceinline__ void num_flux_pre(int sch, R g, R ig1, R gig1, const R* qL, const R* qR, R irL, R irR,
+                                             const R* n, R* F)
 {
-    const R ig1 = R(1) / (g - R(1));
     const R vnl = dotd<DIM>(qL + 2, n);
     const R vnr = dotd<DIM>(qR + 2, n);
     const R rel = ig1 * qL[1] + R(.5) * qL[0] * dotd<DIM>(qL + 2, qL + 2);
     const R rer = ig1 * qR[1] + R(.5) * qR[0] * dotd<DIM>(qR + 2, qR + 2);
-    const R irL = R(1) / qL[0], irR = R(1) / qR[0];
-    const R al = sqrt(g * qL[1] * irL), ar = sqrt(g * qR[1] * irR);
+    const R al = asqrt(g * qL[1] * irL), ar = asqrt(g * qR[1] * irR);
     if (sch == 5) {  // Rusanov: calc_flux_rusanov.f90:64-83
         const R ray = fmax_(fabs(vnl) + al, fabs(vnr) + ar);
         F[0] = R(.5) * ((vnl + ray) * qL[0] + (vnr - ray) * qR[0]);
@@ -201,14 +224,14 @@ __device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* 
         return;
     }
     R rrho, rv[DIM], H, a2;
-    roe_avg<DIM>(g, qL, qR, irL, irR, rrho, rv, H, a2);
+    roe_avg<DIM>(g, gig1, qL, qR, irL, irR, rrho, rv, H, a2);
     const R vm = dotd<DIM>(rv, n);
-    const R am = sqrt(a2);
+    const R am = asqrt(a2);
     if (sch == 21 || sch == 22) {  // HLLE(B): calc_flux_hlle.f90:109-143
         const R beta = (sch == 21) ? R(1) : sqrt((g - R(1)) / (R(2) * g));
         const R sl = fmin_(fmin_(R(0), vnl - beta * al), vm - am);
         const R sr = fmax_(fmax_(R(0), vnr + beta * ar), vm + am);
-        const R iks = R(1) / (sr - sl);
+        const R iks = arcp(sr - sl);
         const R kl = sr * iks, kr = -sl * iks, ku = sl * sr * iks;
         F[0] = (kl * vnl - ku) * qL[0] + (kr * vnr + ku) * qR[0];
         F[1] = (kl * vnl - ku) * rel + (kr * vnr + ku) * rer + (kl * vnl * qL[1] + kr * vnr * qR[1]);
@@ -258,7 +281,7 @@ __device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* 
         for (int d = 0; d < DIM; ++d) F[2 + d] = (vnr * qR[0]) * qR[2 + d] + qR[1] * n[d];
     } else {
         const R rqL = qL[0] * (sl - vnl), rqR = qR[0] * (sr - vnr);
-        const R sst = (rqR * vnr - rqL * vnl - qR[1] + qL[1]) / (rqR - rqL);
+        const R sst = adiv(rqR * vnr - rqL * vnl - qR[1] + qL[1], rqR - rqL);
         // star state of the side of S* (calc_flux_hllc.f90:149-161)
         const bool left = sst >= R(0);
         const R* q = left ? qL : qR;
@@ -266,7 +289,7 @@ __device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* 
         const R vn = left ? vnl : vnr;
         const R s = left ? sl : sr;
         const R re = left ? rel : rer;
-        const R iks = R(1) / (s - sst);
+        const R iks = arcp(s - sst);
         const R rst = rq * iks;
         const R pst = q[1] - rq * (vn - sst);
         const R rest = iks * (re * (s - vn) - q[1] * vn + pst * sst);
@@ -278,6 +301,14 @@ __device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* 
             F[2 + d] = (sst * rvst) + (pst * n[d]);
         }
     }
+}
+
+template <int DIM, typename R>
+__device__ void num_flux(int sch, R g, const R* qL, const R* qR, const R* n, R* F)
+{
+    const R ig1 = R(1) / (g - R(1));
+    const R gig1 = g / (g - R(1));
+    num_flux_pre<DIM>(sch, g, ig1, gig1, qL, qR, R(1) / qL[0], R(1) / qR[0], n, F);
 }
 
 // ---------------------------------------------------------------------------
@@ -423,8 +454,8 @@ __device__ __forceinline__ R slot_normal(const Geo<R>& g, int k, int n, int c, R
 {
     R S[DIM];
     slot_vec<DIM>(g.ss, k, n, c, S);
-    const R A = sqrt(dotd<DIM>(S, S));
-    const R ia = R(1) / A;
+    const R A = asqrt(dotd<DIM>(S, S));
+    const R ia = arcp(A);
 #pragma unroll
     for (int e = 0; e < DIM; ++e) nf[e] = S[e] * ia;
     return A;
@@ -432,6 +463,19 @@ __device__ __forceinline__ R slot_normal(const Geo<R>& g, int k, int n, int c, R
 
 // The state across slot k: the neighbour's primitive state, or the ghost state of the
 // cell_state boundary map (calcboco_ust.f90:44-105) built from this cell's.
+template <int DIM, typename R>
+__device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int L, const R* qc, R* qo, R& ir)
+{
+    if (!link_boundary(L)) {
+        load_prim<DIM>(a.us, link_index(L), R(a.ph.gamma - 1.0), qo, ir);
+    } else {
+        R nf[DIM];
+        slot_normal<DIM>(a.g, k, a.n, c, nf);
+        bc_state<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, qc, qo, a.ph);
+        ir = arcp(qo[0]);
+    }
+}
+
 template <int DIM, typename R>
 __device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int L, const R* qc, R* qo)
 {
@@ -571,7 +615,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
         }
         R phi[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) phi[v] = pn[v] / pd[v];
+        for (int v = 0; v < NV; ++v) phi[v] = adiv(pn[v], pd[v]);
 #pragma unroll
         for (int k = 0; k < NSM; ++k) {
             if (!DENSE && k >= a.ns) break;
@@ -713,6 +757,97 @@ __device__ void viscous_face(const Physics& ph, const R* qL, const R* qR, const 
     flux[1] = flux[1] - conduct * gTH;
 }
 
+// One side's temperature and temperature gradient for the viscous flux
+// (calc_flux_viscous_cell.f90:96-109): T = p/(rho r), grad T = grad p/(rho r) - p grad rho/(rho^2 r),
+// from the primitive state q, its reciprocal density ir and its gradient record g (rho, p, u..).
+template <int DIM, typename R>
+__device__ __forceinline__ void side_temp(const Physics& ph, const R* q, R ir, const R* g, R& T, R* gT)
+{
+    const R k1 = ir * R(ph.irgas);
+    T = q[1] * k1;
+    const R k2 = T * ir;
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) gT[e] = k1 * g[DIM + e] - k2 * g[e];
+}
+
+// viscous_face for the residual kernel: the same flux from per-side temperatures and
+// temperature gradients computed once per cell (side_temp), velocity gradients gUL/gUR
+// (row-major d u_i / d x_j), host-derived constants, and Sutherland's (T/Tref)^1.5 as
+// t sqrt(t) (calc_viscosity, EQNS.f90:243-247).
+template <int DIM, typename R>
+__device__ __forceinline__ void viscous_face_pre(const Physics& ph, const R* qL, const R* qR, R TL, R TR,
+                                                 const R* gTL, const R* gTR, const R* gUL, const R* gUR,
+                                                 const R* rL, const R* rR, const R* vLR, const R* nf, R* flux)
+{
+    R dHL = asqrt(dotd<DIM>(rL, rL)), dHR = asqrt(dotd<DIM>(rR, rR));
+    const R id = arcp(dHL + dHR);
+    dHL = id * dHL;
+    dHR = id * dHR;
+    R velH[DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) velH[e] = dHR * qL[2 + e] + dHL * qR[2 + e];
+    const R TH = dHR * TL + dHL * TR;
+    R mu;
+    if (ph.visc_law == 11) {
+        const R t = TH * R(ph.itref);
+        mu = adiv(R(ph.visc_dyn) * (t * asqrt(t)) * R(ph.tsum), TH + R(ph.tsuth));
+    } else if (ph.visc_law == 14) {
+        mu = R(ph.visc_kin) * (dHR * qL[0] + dHL * qR[0]);
+    } else {
+        mu = R(ph.visc_dyn);
+    }
+    R G[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) G[i][j] = dHL * gUR[i * DIM + j] + dHR * gUL[i * DIM + j];
+    R gavg[DIM];
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) gavg[e] = dHL * gTR[e] + dHR * gTL[e];
+    R gTH;
+    if (ph.sch_dis == 10) {
+        const R idLR2 = arcp(dotd<DIM>(vLR, vLR));
+        R kv[DIM];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) kv[e] = idLR2 * vLR[e];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i) {
+            const R fa = qR[2 + i] - qL[2 + i] - dotd<DIM>(G[i], vLR);
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) G[i][j] = G[i][j] + fa * kv[j];
+        }
+        const R pscal = dotd<DIM>(vLR, nf) * idLR2;
+        const R fcomp = pscal * (TR - TL);
+        R vi[DIM];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) vi[e] = nf[e] - pscal * vLR[e];
+        gTH = fcomp + dotd<DIM>(gavg, vi);
+    } else {
+        gTH = dotd<DIM>(gavg, nf);
+    }
+    R tr = G[0][0];
+#pragma unroll
+    for (int i = 1; i < DIM; ++i) tr += G[i][i];
+    const R addv = -(R(2) / R(3)) * tr;
+    R sn[DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+        R s = R(0);
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) {
+            R sij = G[i][j] + G[j][i];
+            if (i == j) sij += addv;
+            s += (mu * sij) * nf[j];
+        }
+        sn[i] = s;
+    }
+    const R svn = dotd<DIM>(sn, velH);
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) flux[2 + e] = flux[2 + e] - sn[e];
+    flux[1] = flux[1] - svn;
+    flux[1] = flux[1] - (mu * R(ph.cp_pr)) * gTH;
+}
+
 // ---------------------------------------------------------------------------
 // Face Jacobians for the implicit operator: calc_jacflux (calc_jacflux.f90:49-77), the
 // Euler flux Jacobian A(n) in (rho, rhoE, rho u..) at a primitive state, weighted by
@@ -754,7 +889,7 @@ __device__ __forceinline__ void jac_hll(const Physics& ph, const R* qL, const R*
     const R irL = R(1) / qL[0], irR = R(1) / qR[0];
     const R al = sqrt(g * qL[1] * irL), ar = sqrt(g * qR[1] * irR);
     R rrho, rv[DIM], H, a2;
-    roe_avg<DIM>(g, qL, qR, irL, irR, rrho, rv, H, a2);
+    roe_avg<DIM>(g, R(ph.gig1), qL, qR, irL, irR, rrho, rv, H, a2);
     const R vm = dotd<DIM>(rv, n);
     const R am = sqrt(a2);
     R sl = fmin_(vnl - beta * al, vm - am);
@@ -845,8 +980,8 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
     const int n = a.n;
     const R g1 = R(a.ph.gamma - 1.0);
     const R gam = R(a.ph.gamma);
-    R qc[NV];
-    if (!FS || VISC) load_prim<DIM>(a.us, c, g1, qc);
+    R qc[NV], irc = R(0);
+    if (!FS || VISC) load_prim<DIM>(a.us, c, g1, qc, irc);
     double D[JAC ? NV * NV : 1];
     const double kdt = JAC ? *a.d_dt / a.vol[c] : 0.0;
     if (JAC) {
@@ -854,7 +989,11 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
         for (int k = 0; k < NV * NV; ++k) D[k] = (k % (NV + 1) == 0) ? 1.0 : 0.0;
     }
     R gcell[VISC ? NV * DIM : 1];
-    if (VISC) load_grad<DIM>(a.grad, c, gcell);
+    R Tc = R(0), gTc[DIM];
+    if (VISC) {
+        load_grad<DIM>(a.grad, c, gcell);
+        side_temp<DIM>(a.ph, qc, irc, gcell, Tc, gTc);
+    }
     R res[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) res[v] = R(0);
@@ -868,9 +1007,9 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
         R nf[DIM];
         const R A = slot_normal<DIM>(a.g, k, n, c, nf);
         // the far cell's state (first order, viscous flux)
-        R qo[NV];
-        if (!FS || VISC) far_state<DIM>(a, k, c, L, qc, qo);
-        R qLf[NV], qRf[NV];
+        R qo[NV], iro = R(0);
+        if (!FS || VISC) far_state<DIM>(a, k, c, L, qc, qo, iro);
+        R qLf[NV], qRf[NV], irL, irR;
         if (FS) {
             const R* own = a.qf + ((size_t)k * n + c) * NV;
             if (!bnd) {
@@ -886,15 +1025,19 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
                 for (int v = 0; v < NV; ++v) qLf[v] = own[v];
                 bc_state<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, qLf, qRf, a.ph);
             }
+            irL = arcp(qLf[0]);
+            irR = arcp(qRf[0]);
         } else {
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 qLf[v] = other ? qo[v] : qc[v];
                 qRf[v] = other ? qc[v] : qo[v];
             }
+            irL = other ? iro : irc;
+            irR = other ? irc : iro;
         }
         R flux[NV];
-        num_flux<DIM>(a.ph.sch_hyp, gam, qLf, qRf, nf, flux);
+        num_flux_pre<DIM>(a.ph.sch_hyp, gam, R(a.ph.ig1), R(a.ph.gig1), qLf, qRf, irL, irR, nf, flux);
         R rL[DIM], rR[DIM], dk[DIM];
         if (VISC) {
             R rk[DIM];
@@ -908,8 +1051,13 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
             R go[NV * DIM];
             if (!bnd) load_grad<DIM>(a.grad, link_index(L), go);
             else bc_grad<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, gcell, go);
-            viscous_face<DIM>(a.ph, other ? qo : qc, other ? qc : qo, rL, rR, dk, nf, other ? go : gcell,
-                              other ? gcell : go, flux);
+            R To, gTo[DIM];
+            side_temp<DIM>(a.ph, qo, iro, go, To, gTo);
+            const R* guc = gcell + 2 * DIM;
+            const R* guo = go + 2 * DIM;
+            viscous_face_pre<DIM>(a.ph, other ? qo : qc, other ? qc : qo, other ? To : Tc, other ? Tc : To,
+                                  other ? gTo : gTc, other ? gTc : gTo, other ? guo : guc, other ? guc : guo,
+                                  rL, rR, dk, nf, flux);
         }
         if (JAC) {
             // build_implicit_bdlu.f90:44-83: diag(ic1) += jacL.A dt/V, diag(ic2) -= jacR.A dt/V
