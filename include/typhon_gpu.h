@@ -35,12 +35,20 @@ typedef struct ty_ctx ty_ctx;
  * defsolver%boco(idef)%typ_boco (SOURCE/MODCOM/VARCOM.f90:75-93) and its
  * resolved uniform parameters (setboco_ns_*.f90). */
 typedef struct {
-    int32_t type;            /* 10 symmetry, 11 periodic, 31 supersonic inlet, 35 subsonic outlet,
-                                36 supersonic outlet */
+    int32_t type;            /* 10 symmetry, 11 periodic, 20 adiabatic wall, 21 isothermal wall,
+                                22 heat-flux wall, 30 subsonic inlet, 31 supersonic inlet,
+                                35 subsonic outlet, 36 supersonic outlet */
     int32_t nface;
     const int32_t* iface;    /* umesh%boco(ib)%iface(1:nface), 1-based */
     const int32_t* ipart;    /* periodic only: boco%gridcon%i_param(1:nface), 1-based partner faces */
-    double param[8];         /* 31: ptot, ttot, mach, dir_x, dir_y, dir_z; 35: pstat */
+    double param[8];         /* 31: ptot, ttot, mach, dir_x, dir_y, dir_z; 35: pstat;
+                                30: ptot, ttot, unused, dir_x, dir_y, dir_z (setboco_ns_inlet_sub.f90,
+                                    uniform values);
+                                21: temp_wall, -, -, -, wall velocity x, y, z (setboco_ns_isoth.f90);
+                                22: boco_ns%flux (the user's WALL_FLUX negated, def_boco_ns.f90:73-74),
+                                    -, -, -, wall velocity x, y, z (setboco_ns_flux.f90, ns_bocoflux.f90);
+                                20: -, -, -, -, wall velocity x, y, z (a 22 with zero flux,
+                                    def_boco_ns.f90:44-49).  20 and 22 need the laminar NS model. */
 } ty_boco_desc;
 
 /* The unstructured mesh as Typhon holds it after init_world (SOURCE/MAIN/main.f90:95):

@@ -165,12 +165,77 @@ static void bc_sym(const double* n, const double* qi, double* q)
     q[2] = qi[2] - k * n[0]; q[3] = qi[3] - k * n[1]; q[4] = qi[4] - k * n[2];
 }
 
-static void bc_state(const or_ctx* o, const or_boco* b, int f, const double* qi, double* q)
+/* setboco_ns_inlet_sub.f90:75-134 with uniform PI, TI and DIRECTION (no FCT dependence):
+   ps from the interior cell (cell_state) or the left face state (face_state), dir/|dir|
+   (:107), then pi_ti_ps_dir2nspri (EQNS.f90:166-191) */
+static void bc_inlet_sub(const or_config* c, const double* prm, const double* qi, double* q)
+{
+    double g = c->gamma, g1 = g - 1.0;
+    double dx = prm[3], dy = prm[4], dz = prm[5];
+    double dn = sqrt(dx * dx + dy * dy + dz * dz);
+    dx /= dn; dy /= dn; dz /= dn;
+    double pi = prm[0], ti = prm[1], ps = qi[1];
+    double fm = pow(pi / ps, g1 / g);
+    if (fm < 1.0) fm = 1.0;
+    double rho = ps / (c->rgas * ti) * fm;
+    double vm = sqrt((fm - 1.0) * 2.0 / g1) * sqrt(g * ps / rho);
+    q[0] = rho; q[1] = ps; q[2] = vm * dx; q[3] = vm * dy; q[4] = vm * dz;
+}
+
+/* setboco_ns_isoth.f90:66-84, both state modes: T_g = 2 T_w - p/(rho r), p_g = p,
+   rho_g = p_g/(r T_g), u_g = 2 u_w - u (calc_wallvelocity without ALE/MRF: u_w as given) */
+static void bc_wall_isoth(const or_config* c, const double* prm, const double* qi, double* q)
+{
+    double r = c->rgas;
+    double temp = 2.0 * prm[0] - qi[1] / (qi[0] * r);
+    q[1] = qi[1];
+    q[0] = q[1] / (r * temp);
+    for (int d = 0; d < 3; ++d) q[2 + d] = 2.0 * prm[4 + d] - qi[2 + d];
+}
+
+/* setboco_ns_flux.f90 (adiabatic wall = flux wall with zero flux, def_boco_ns.f90:44-49;
+   prm[0] is the flux in the algorithm's convention, the user's WALL_FLUX negated, :73-74).
+   cell_state (:79-117): d = |x_f - x_c|^2 / |(x_f - x_c).n|, conductivity at the interior
+   temperature, T_g = p/(rho r) - flux d / conduct, p_g = p, rho_g = p/(r T_g), u_g = 2 u_w - u.
+   face_state (:120-141): rho_R = p_L (the reference's weak condition), p_R = p_L,
+   u_R = 2 u_w - u_L. */
+static void bc_wall_flux(const or_ctx* o, const or_boco* b, int f, int ic, int face, const double* qi, double* q)
+{
+    const or_config* c = &o->c;
+    double r = c->rgas;
+    if (face) {
+        q[0] = qi[1];
+        q[1] = qi[1];
+    } else {
+        const double* xf = c->fcent + 3 * (size_t)f;
+        const double* xc = c->ccent + 3 * (size_t)ic;
+        const double* n = c->fnorm + 3 * (size_t)f;
+        double dv[3] = {xf[0] - xc[0], xf[1] - xc[1], xf[2] - xc[2]};
+        double d = dot3(dv, dv) / fabs(dot3(dv, n));
+        double gsgmu = c->gamma / (c->gamma - 1.0);
+        double cp = gsgmu * r;
+        double TH = qi[1] / (r * qi[0]);
+        double mu = or_viscosity(c, qi[0], TH);
+        double conduct = mu * cp / c->prandtl;
+        double temp = qi[1] / (qi[0] * r) - b->param[0] * d / conduct;
+        q[1] = qi[1];
+        q[0] = q[1] / (r * temp);
+    }
+    for (int dd = 0; dd < 3; ++dd) q[2 + dd] = 2.0 * b->param[4 + dd] - qi[2 + dd];
+}
+
+/* boundary state of face f from the interior/left state qi; ic is the interior cell,
+   face != 0 in face_state mode */
+static void bc_state(const or_ctx* o, const or_boco* b, int f, int ic, int face, const double* qi, double* q)
 {
     switch (b->type) {
     case OR_BC_SYM: bc_sym(o->c.fnorm + 3 * (size_t)f, qi, q); break;
     case OR_BC_INLET_SUP: bc_inlet_sup(&o->c, b->param, q); break;
+    case OR_BC_INLET_SUB: bc_inlet_sub(&o->c, b->param, qi, q); break;
     case OR_BC_OUTLET_SUB: bc_outlet_sub(&o->c, b->param, qi, q); break;
+    case OR_BC_WALL_ISOTH: bc_wall_isoth(&o->c, b->param, qi, q); break;
+    case OR_BC_WALL_ADIAB:
+    case OR_BC_WALL_FLUX: bc_wall_flux(o, b, f, ic, face, qi, q); break;
     case OR_BC_OUTLET_SUP:
     default:
         for (int v = 0; v < NV; ++v) q[v] = qi[v];
@@ -191,7 +256,7 @@ static void boco_cell_state(or_ctx* o)
             int is = (b->type == OR_BC_PERIOD) ? o->c.ic1[b->ipart[k]] : o->c.ic1[f];
             for (int v = 0; v < NV; ++v) qi[v] = PRIM(o, v, is);
             if (b->type == OR_BC_PERIOD) { for (int v = 0; v < NV; ++v) q[v] = qi[v]; }
-            else bc_state(o, b, f, qi, q);
+            else bc_state(o, b, f, is, 0, qi, q);
             for (int v = 0; v < NV; ++v) PRIM(o, v, ig) = q[v];
         }
     }
@@ -208,7 +273,7 @@ static void boco_face_state(or_ctx* o)
             int is = (b->type == OR_BC_PERIOD) ? b->ipart[k] : f;
             for (int v = 0; v < NV; ++v) qi[v] = QL(o, v, is);
             if (b->type == OR_BC_PERIOD) { for (int v = 0; v < NV; ++v) q[v] = qi[v]; }
-            else bc_state(o, b, f, qi, q);
+            else bc_state(o, b, f, o->c.ic1[f], 1, qi, q);
             for (int v = 0; v < NV; ++v) QR(o, v, f) = q[v];
         }
     }
@@ -216,7 +281,15 @@ static void boco_face_state(or_ctx* o)
 
 /* bccon_cell_grad: ghost gradients.  Symmetry mirrors (calcboco_ust_sym.f90:65-77:
    vectors v-2(v.n)n, tensor S.T.S with S=I-2nn^T, VEC3D.f90:679-694); the NS
-   inlet/outlet routines copy (setboco_ns_inlet_sup.f90:140-149); periodic copies. */
+   inlet/outlet routines copy (setboco_ns_inlet_sup.f90:140-149); periodic copies.
+   Isothermal wall (setboco_ns_isoth.f90:92-105): grad rho copied, grad p negated then
+   mirrored (v3d_sym, VEC3D.f90:630-637), grad u copied.  Flux/adiabatic walls
+   (setboco_ns_flux.f90:150-162) and the subsonic inlet (setboco_ns_inlet_sub.f90:141-154) loop
+   `do ifb = ista(ib), iend(ifb)` with ifb not yet set, a loop that runs no iteration in the
+   reference's runs: their ghost gradients keep the zero they start with.  The NRG viscous
+   goldens (NRG/NS/NUM/FLUX/VISCOUS/CELLAVG, moving adiabatic walls, cell-average face
+   gradients) are reproduced with zero ghost gradients and not with the negate/copy rule the
+   loop body states. */
 static void boco_cell_grad(or_ctx* o)
 {
     for (int ib = 0; ib < o->c.nboco; ++ib) {
@@ -246,6 +319,15 @@ static void boco_cell_grad(or_ctx* o)
                     GRAD(o, 2 + i, j, ig) = s; }
             } else {
                 for (int v = 0; v < 15; ++v) o->grad[(size_t)v * o->nc + ig] = o->grad[(size_t)v * o->nc + is];
+                if (b->type == OR_BC_WALL_ISOTH) {
+                    const double* n = o->c.fnorm + 3 * (size_t)f;
+                    double g[3] = {-GRAD(o, 1, 0, is), -GRAD(o, 1, 1, is), -GRAD(o, 1, 2, is)};
+                    double k2 = 2.0 * (g[0] * n[0] + g[1] * n[1] + g[2] * n[2]);
+                    for (int d = 0; d < 3; ++d) GRAD(o, 1, d, ig) = g[d] - k2 * n[d];
+                } else if (b->type == OR_BC_WALL_FLUX || b->type == OR_BC_WALL_ADIAB ||
+                           b->type == OR_BC_INLET_SUB) {
+                    for (int v = 0; v < 15; ++v) o->grad[(size_t)v * o->nc + ig] = 0.0;
+                }
             }
         }
     }
@@ -768,6 +850,56 @@ static void viscous_fluxes(or_ctx* o)
     }
 }
 
+/* ns_bocoflux (SOURCE/EQNS/ns_bocoflux.f90:47-111) on flux and adiabatic walls: the
+   conductive flux is recomputed from the cell states and gradients of the face's two cells,
+   as the viscous flux computed it, and added back together with the imposed flux (the
+   boundary field holds the flux set by the last state-mode call, setboco_ns_flux.f90:104,129). */
+static void ns_bocoflux(or_ctx* o)
+{
+    const or_config* c = &o->c;
+    double r = c->rgas;
+    double cp = c->gamma * r / (c->gamma - 1);
+    for (int ib = 0; ib < c->nboco; ++ib) {
+        const or_boco* b = &c->boco[ib];
+        if (b->type != OR_BC_WALL_FLUX && b->type != OR_BC_WALL_ADIAB) continue;
+        for (int k = 0; k < b->nface; ++k) {
+            int f = b->iface[k];
+            int cl = c->ic1[f], cr = c->ic2[f];
+            const double* xl = c->ccent + 3 * (size_t)cl;
+            const double* xr = c->ccent + 3 * (size_t)cr;
+            const double* xf = c->fcent + 3 * (size_t)f;
+            const double* n = c->fnorm + 3 * (size_t)f;
+            double a[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+            double bb[3] = {xf[0] - xr[0], xf[1] - xr[1], xf[2] - xr[2]};
+            double dHL = sqrt(dot3(a, a)), dHR = sqrt(dot3(bb, bb));
+            double id = 1.0 / (dHL + dHR);
+            dHL = id * dHL; dHR = id * dHR;
+            double vLR[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+            double rhoL = PRIM(o, 0, cl), rhoR = PRIM(o, 0, cr);
+            double TL = PRIM(o, 1, cl) / (rhoL * r), TR = PRIM(o, 1, cr) / (rhoR * r);
+            double TH = dHR * TL + dHL * TR, rhoH = dHR * rhoL + dHL * rhoR;
+            double gTL[3], gTR[3];
+            for (int d = 0; d < 3; ++d) {
+                gTL[d] = 1.0 / (rhoL * r) * GRAD(o, 1, d, cl) - PRIM(o, 1, cl) / (rhoL * rhoL * r) * GRAD(o, 0, d, cl);
+                gTR[d] = 1.0 / (rhoR * r) * GRAD(o, 1, d, cr) - PRIM(o, 1, cr) / (rhoR * rhoR * r) * GRAD(o, 0, d, cr);
+            }
+            double mu = or_viscosity(c, rhoH, TH);
+            double gavg[3] = {dHL * gTR[0] + dHR * gTL[0], dHL * gTR[1] + dHR * gTL[1], dHL * gTR[2] + dHR * gTL[2]};
+            double gTH;
+            if (c->sch_dis == OR_DIS_CELLFULL) {
+                double dLR2 = dot3(vLR, vLR);
+                double pscal = dot3(vLR, n) / dLR2;
+                double vi[3] = {n[0] - pscal * vLR[0], n[1] - pscal * vLR[1], n[2] - pscal * vLR[2]};
+                gTH = 1.0 * (pscal * (TR - TL)) + dot3(gavg, vi);
+            } else {
+                gTH = dot3(gavg, n);
+            }
+            double conduct = mu * cp / c->prandtl;
+            FLX(o, 1, f) = FLX(o, 1, f) + conduct * gTH + b->param[0];
+        }
+    }
+}
+
 /* ---------------------------------------------------------------------------
  * calc_rhs (SOURCE/MAIN/calc_rhs.f90:48-145) for one grid with global dt. */
 static void calc_rhs_internal(or_ctx* o, double dt)
@@ -796,7 +928,10 @@ static void calc_rhs_internal(or_ctx* o, double dt)
         or_flux_face(c, ql, qr, c->fnorm + 3 * (size_t)f, F);
         for (int v = 0; v < NV; ++v) FLX(o, v, f) = F[v];
     }
-    if (c->viscous) viscous_fluxes(o);
+    if (c->viscous) {
+        viscous_fluxes(o);
+        ns_bocoflux(o);
+    }
     /* flux_to_res (SOURCE/MAIN/flux_to_res.f90:47-142) */
     for (int v = 0; v < NV; ++v)
         for (int f = 0; f < o->nf; ++f) FLX(o, v, f) = c->fsurf[f] * FLX(o, v, f);

@@ -76,3 +76,74 @@ def replay(step_fn, dt_fn, get_rho, m, ncycle=100, duration=0.1):
         curtps = curtps + dtbase
         out.append((icycle, curtps, rhoerr(m, get_rho(), curtps)))
     return np.array(out)
+
+
+# ---------------------------------------------------------------------------
+# NRG/NS/NUM/FLUX/VISCOUS/{CELLFULL,CELLAVG}: the Stokes layer between two moving adiabatic
+# walls (main.rpm of both cases): mxlay2x20quad-h50.tym (2 x 20 quads, y in [-50, 50] clustered
+# at y = 0), periodic in x (translation 1), ADIABATIC_WALL at JMIN / JMAX moving at -1 / +1,
+# laminar air with constant kinematic viscosity 1, HLLC + MUSCL-FAST + van Albada, LSQ
+# gradients, RK25BB at CFL 0.5, rho = 1, p = 100/1.4, u = erf(y/2) at t = 0 (the layer's age
+# is 1), 10 cycles over a duration of 10.  Probes: VXERR = VOL_AVERAGE (u - erf(y/2/sqrt(t+1)))^2
+# and DELTA = VOL_AVERAGE 2*50*(1 - u^2).
+
+VISC_CASES = {"CELLFULL": S.DIS_CELLFULL, "CELLAVG": S.DIS_CELLAVG2}
+
+
+def golden_visc(case, probe):
+    return np.loadtxt(os.path.join(HERE, "golden", "nrg_visc", case, f"{probe}.tmon"))
+
+
+def mxlay():
+    from typhon_amd import tyfmt
+    bocos = [dict(family="IMIN", type=M.BC_PERIOD, partner="IMAX", period=(1.0, 0.0, 0.0)),
+             dict(family="IMAX", type=M.BC_PERIOD, partner="IMIN", period=(-1.0, 0.0, 0.0)),
+             dict(family="JMIN", type=M.BC_WALL_ADIAB, param=[0.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0.0]),
+             dict(family="JMAX", type=M.BC_WALL_ADIAB, param=[0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0])]
+    return tyfmt.tym_ustmesh(os.path.join(HERE, "golden", "meshes", "mxlay2x20quad-h50.tym"), bocos)
+
+
+def setup_visc(case):
+    from math import erf
+    m = mxlay()
+    n = m.ncell_int
+    y = m.ccent[:n, 1]
+    vel = np.zeros((3, n))
+    vel[0] = [erf(v / 2.0) for v in y]
+    cons = prim_to_cons(np.ones(n), np.full(n, 100 / 1.4), vel)
+    sch = S.default_scheme(sch_hyp=S.SCH_HLLC, method="F", limiter="A", gradmeth=S.GRAD_LSQ,
+                           tps_meth=S.TPS_LSRK25BB, cfl=0.5, sch_dis=VISC_CASES[case])
+    gas = S.default_gas(typ_fluid=S.EQ_NSLAM, typ_visc=S.VISC_KINCST, visc_kin=1.0)
+    return m, sch, gas, cons
+
+
+def visc_probes(m, cons, t):
+    from math import erf, sqrt
+    n = m.ncell_int
+    y = m.ccent[:n, 1]
+    v = m.cvol[:n]
+    vx = cons[2] / cons[0]
+    exact = np.array([erf(a / 2.0 / sqrt(t + 1.0)) for a in y])
+    vxerr = ((vx - exact) ** 2 * v).sum() / v.sum()
+    delta = (2 * 50 * (1 - vx ** 2) * v).sum() / v.sum()
+    return vxerr, delta
+
+
+def replay_probes(step_fn, dt_fn, probe_fn, ncycle, duration):
+    """the cycle loop of replay() with a probe function of the cycle end time"""
+    dtbase = duration / ncycle
+    curtps = 0.0
+    out = []
+    for icycle in range(1, ncycle + 1):
+        cycle_time = 0.0
+        end = False
+        while not end:
+            dt = dt_fn(dtbase - cycle_time)
+            if dt >= dtbase - cycle_time:
+                end = True
+                dt = dtbase - cycle_time
+            step_fn(dt)
+            cycle_time = cycle_time + dt
+        curtps = curtps + dtbase
+        out.append((icycle, curtps, *probe_fn(curtps)))
+    return np.array(out)

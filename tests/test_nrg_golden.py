@@ -31,3 +31,29 @@ def test_gpu_matches_nrg_golden(case):
     res = nrg.replay(gpu.rk_step, lambda rem: gpu.timestep(rem), lambda: gpu.get_state()[0], m)
     gold = nrg.golden(case)
     np.testing.assert_allclose(res[:, 2], gold[:, 2], rtol=RTOL)
+
+
+@pytest.mark.parametrize("case", list(nrg.VISC_CASES))
+def test_oracle_matches_nrg_visc_golden(case):
+    """NRG/NS/NUM/FLUX/VISCOUS/{CELLFULL,CELLAVG}: Stokes layer between moving adiabatic walls,
+    VXERR and DELTA histories over 10 cycles (8 significant digits)."""
+    m, s, g, u = nrg.setup_visc(case)
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u)
+    res = nrg.replay_probes(o.rk_step, lambda rem: o.timestep(s["cfl"], rem),
+                            lambda t: nrg.visc_probes(m, o.get_cons(), t), 10, 10.0)
+    np.testing.assert_allclose(res[:, 2], nrg.golden_visc(case, "VXERR")[:, 2], rtol=RTOL)
+    np.testing.assert_allclose(res[:, 3], nrg.golden_visc(case, "DELTA")[:, 2], rtol=RTOL)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", list(nrg.VISC_CASES))
+def test_gpu_matches_nrg_visc_golden(case):
+    from typhon_amd.solver import Solver
+    m, s, g, u = nrg.setup_visc(case)
+    gpu = Solver(m, s, g)
+    gpu.set_state(u)
+    res = nrg.replay_probes(gpu.rk_step, lambda rem: gpu.timestep(rem),
+                            lambda t: nrg.visc_probes(m, gpu.get_state(), t), 10, 10.0)
+    np.testing.assert_allclose(res[:, 2], nrg.golden_visc(case, "VXERR")[:, 2], rtol=RTOL)
+    np.testing.assert_allclose(res[:, 3], nrg.golden_visc(case, "DELTA")[:, 2], rtol=RTOL)

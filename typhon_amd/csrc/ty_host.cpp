@@ -101,6 +101,7 @@ struct ty_ctx {
     double* d_dt = nullptr;
     int* d_err = nullptr;
     bool state_ok = false;
+    bool ext_bc = false;  // wall or subsonic-inlet boundaries (no tangent/adjoint for them)
     double h_dt = 0.0;
     // per-kernel event timing (bench)
     bool timing = false;
@@ -350,7 +351,8 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
                 int p = d.ipart[k] - 1;
                 if (p < nfi || p >= nf) return fail(c, TY_ERR_ARG, "bad periodic partner face");
                 fpart[f] = p;
-            } else if (d.type != 10 && d.type != 31 && d.type != 35 && d.type != 36) {
+            } else if (d.type != 10 && d.type != 20 && d.type != 21 && d.type != 22 && d.type != 30 &&
+                       d.type != 31 && d.type != 35 && d.type != 36) {
                 return fail(c, TY_ERR_UNSUPPORTED, "boundary condition type " + std::to_string(d.type) +
                                                        " is not provided by this library");
             }
@@ -736,11 +738,33 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
             if (c->dim == 3) t.param[4] = ma * dz;
             break;
         }
+        case 20:  // adiabatic wall: a flux wall with zero flux (def_boco_ns.f90:44-49)
+        case 22:
+        case 21:
+            if (d.type != 21 && !viscous)
+                return fail(c, TY_ERR_UNSUPPORTED, "flux and adiabatic walls need the laminar NS model "
+                                                   "(their ghost state and ns_bocoflux use the conductivity)");
+            t.kind = d.type == 21 ? BK_WALL_ISOTH : BK_WALL_FLUX;
+            t.param[0] = d.type == 20 ? 0.0 : d.param[0];
+            for (int k = 4; k < 7; ++k) t.param[k] = d.param[k];
+            break;
+        case 30: {
+            // uniform PI, TI, DIRECTION; dir/|dir| as setboco_ns_inlet_sub.f90:107
+            t.kind = BK_INLET_SUB;
+            t.param[0] = d.param[0];
+            t.param[1] = d.param[1];
+            const double dn = std::sqrt(d.param[3] * d.param[3] + d.param[4] * d.param[4] + d.param[5] * d.param[5]);
+            for (int k = 3; k < 6; ++k) t.param[k] = d.param[k] / dn;
+            break;
+        }
         default: t.kind = BK_OUTLET_SUP;
         }
     }
     int rc;
     if ((rc = upload(c, &c->d_boco, tab))) return rc;
+    c->ext_bc = false;
+    for (const BocoDev& t : tab)
+        if (t.kind == BK_WALL_ISOTH || t.kind == BK_WALL_FLUX || t.kind == BK_INLET_SUB) c->ext_bc = true;
     // both precisions: the FP32 geometry also serves the FP32 Krylov products
     if ((rc = build_geo<float>(c, c->geo_f)) || (rc = build_geo<double>(c, c->geo_d))) return rc;
     std::vector<double> dtfac(c->n);
@@ -1530,6 +1554,8 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
     if (c->ph.method != 'N')
         return fail(c, TY_ERR_UNSUPPORTED, "tangent/adjoint residual: first-order schemes only");
+    if (c->ext_bc)
+        return fail(c, TY_ERR_UNSUPPORTED, "tangent/adjoint residual: wall and subsonic-inlet boundaries not provided");
     hipSetDevice(c->device);
     c->h_dt = dt;
     HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1866,6 +1892,8 @@ int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t sn
     const auto t0 = std::chrono::steady_clock::now();
     if (c->ph.viscous && c->ph.sch_hyp != 25 && c->ph.sch_hyp != 26)
         return fail(c, TY_ERR_UNSUPPORTED, "adjoint of the viscous residual: HLLC/HLLCB fluxes only");
+    if (c->ext_bc)
+        return fail(c, TY_ERR_UNSUPPORTED, "unsteady adjoint: wall and subsonic-inlet boundaries not provided");
     DevTmp tmp(c);
     Unsteady u;
     int rc;

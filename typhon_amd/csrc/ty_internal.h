@@ -19,13 +19,20 @@ constexpr int BLOCK = 256;       // threads per block (4 wavefronts of 64)
 constexpr int MAX_BOCO = 64;
 
 // boundary kinds handled on the device (the reference's bc_* codes map onto these)
-enum BcKind : int { BK_SYM = 0, BK_INLET_SUP = 1, BK_OUTLET_SUB = 2, BK_OUTLET_SUP = 3 };
+enum BcKind : int {
+    BK_SYM = 0, BK_INLET_SUP = 1, BK_OUTLET_SUB = 2, BK_OUTLET_SUP = 3,
+    BK_WALL_ISOTH = 4, BK_WALL_FLUX = 5, BK_INLET_SUB = 6
+};
 
 struct BocoDev {
     int kind;
     int pad;
     double param[8];  // BK_INLET_SUP: precomputed ghost primitive state (rho, p, u, v, w);
-                      // BK_OUTLET_SUB: param[0] = imposed static pressure
+                      // BK_OUTLET_SUB: param[0] = imposed static pressure;
+                      // BK_WALL_ISOTH: param[0] = wall temperature, param[4..6] = wall velocity;
+                      // BK_WALL_FLUX: param[0] = wall heat flux (outward), param[4..6] = wall velocity;
+                      // BK_INLET_SUB: param[0] = total pressure, param[1] = total temperature,
+                      //               param[3..5] = unit flow direction
 };
 
 // Device geometry in the arithmetic precision R.  The residual kernels read it

@@ -87,16 +87,73 @@ __device__ __forceinline__ R dotd(const R* a, const R* b)
     return s;
 }
 
-// Boundary states (cell_state and face_state modes use the same map):
+// dynamic viscosity (calc_viscosity, EQNS.f90:225-265) in the reference's form
+template <typename R>
+__device__ __forceinline__ R visc_mu(const Physics& ph, R rho, R T)
+{
+    if (ph.visc_law == 11)
+        return R(ph.visc_dyn) * dpow(T / R(ph.tref), R(1.5)) * (R(ph.tref) + R(ph.tsuth)) / (T + R(ph.tsuth));
+    if (ph.visc_law == 14) return R(ph.visc_kin) * rho;
+    return R(ph.visc_dyn);
+}
+
+// Boundary states from the interior cell's (cell_state) or the left face state
+// (face_state, face = true); n is the outward unit normal, rfc = x_f - x_c of the
+// interior cell (read by the flux wall in cell_state mode only):
 // symmetry calcboco_ust_sym.f90:46-79, supersonic inlet setboco_ns_inlet_sup.f90
 // (state precomputed on the host from pi_ti_mach_dir2nspri, EQNS.f90:139-160),
-// subsonic outlet setboco_ns_outlet_sub.f90:57-95 (EQNS.f90:166-219),
-// supersonic outlet setboco_ns_outlet_sup.f90 (copy).
+// subsonic inlet setboco_ns_inlet_sub.f90:75-134 (uniform PI/TI/DIRECTION,
+// pi_ti_ps_dir2nspri EQNS.f90:166-191), subsonic outlet setboco_ns_outlet_sub.f90:57-95
+// (EQNS.f90:166-219), supersonic outlet setboco_ns_outlet_sup.f90 (copy), isothermal wall
+// setboco_ns_isoth.f90:66-84, flux and adiabatic walls setboco_ns_flux.f90:79-141.
 template <int DIM, typename R>
-__device__ __forceinline__ void bc_state(const BocoDev& b, const R* n, const R* qi, R* qo, const Physics& ph)
+__device__ __forceinline__ void bc_state(const BocoDev& b, const R* n, const R* qi, R* qo, const Physics& ph,
+                                         bool face = false, const R* rfc = nullptr)
 {
     constexpr int NV = DIM + 2;
     switch (b.kind) {
+    case BK_WALL_ISOTH: {
+        // T_g = 2 T_w - p/(rho r), p_g = p, rho_g = p_g/(r T_g), u_g = 2 u_w - u
+        const R r = R(ph.rgas);
+        const R temp = R(2) * R(b.param[0]) - qi[1] / (qi[0] * r);
+        qo[1] = qi[1];
+        qo[0] = qo[1] / (r * temp);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) qo[2 + d] = R(2) * R(b.param[4 + d]) - qi[2 + d];
+        break;
+    }
+    case BK_WALL_FLUX: {
+        const R r = R(ph.rgas);
+        if (face) {
+            // face_state: the reference's weak condition rho_R = p_L (setboco_ns_flux.f90:136)
+            qo[0] = qi[1];
+            qo[1] = qi[1];
+        } else {
+            // cell_state: T_g = T - flux d / conduct, d = |x_f - x_c|^2 / |(x_f - x_c).n|
+            const R dd = dotd<DIM>(rfc, rfc) / fabs(dotd<DIM>(rfc, n));
+            const R TH = qi[1] / (r * qi[0]);
+            const R conduct = visc_mu(ph, qi[0], TH) * R(ph.cp_pr);
+            const R temp = qi[1] / (qi[0] * r) - R(b.param[0]) * dd / conduct;
+            qo[1] = qi[1];
+            qo[0] = qo[1] / (r * temp);
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) qo[2 + d] = R(2) * R(b.param[4 + d]) - qi[2 + d];
+        break;
+    }
+    case BK_INLET_SUB: {
+        const R g = R(ph.gamma), g1 = g - R(1);
+        const R ps = qi[1];
+        R fm = dpow(R(b.param[0]) / ps, g1 / g);
+        if (fm < R(1)) fm = R(1);
+        const R rho = ps / (R(ph.rgas) * R(b.param[1])) * fm;
+        const R vm = sqrt((fm - R(1)) * R(2) / g1) * sqrt(g * ps / rho);
+        qo[0] = rho;
+        qo[1] = ps;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) qo[2 + d] = vm * R(b.param[3 + d]);
+        break;
+    }
     case BK_SYM: {
         const R vn = dotd<DIM>(qi + 2, n);
         const R k = R(2) * vn;
@@ -139,7 +196,10 @@ __device__ __forceinline__ void bc_state(const BocoDev& b, const R* n, const R* 
 }
 
 // ghost-cell gradients for the viscous flux (cell_grad mode): symmetry mirrors
-// (calcboco_ust_sym.f90:65-77 with calc_symn, VEC3D.f90:679-694), others copy.
+// (calcboco_ust_sym.f90:65-77 with calc_symn, VEC3D.f90:679-694); isothermal wall: grad p
+// negated and mirrored, the rest copied (setboco_ns_isoth.f90:92-105); flux/adiabatic walls and
+// the subsonic inlet: zero, as their loops (setboco_ns_flux.f90:156, setboco_ns_inlet_sub.f90:146,
+// bound iend(ifb) with ifb unset) run no iteration in the reference; other inlets and outlets copy.
 template <int DIM, typename R>
 __device__ __forceinline__ void bc_grad(const BocoDev& b, const R* n, const R* gi, R* go)
 {
@@ -147,6 +207,17 @@ __device__ __forceinline__ void bc_grad(const BocoDev& b, const R* n, const R* g
     if (b.kind != BK_SYM) {
 #pragma unroll
         for (int k = 0; k < NV * DIM; ++k) go[k] = gi[k];
+        if (b.kind == BK_WALL_ISOTH) {
+            R gp[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) gp[d] = -gi[DIM + d];
+            const R k2 = R(2) * dotd<DIM>(gp, n);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) go[DIM + d] = gp[d] - k2 * n[d];
+        } else if (b.kind == BK_WALL_FLUX || b.kind == BK_INLET_SUB) {
+#pragma unroll
+            for (int k = 0; k < NV * DIM; ++k) go[k] = R(0);
+        }
         return;
     }
 #pragma unroll
@@ -464,14 +535,27 @@ __device__ __forceinline__ R slot_normal(const Geo<R>& g, int k, int n, int c, R
 // The state across slot k: the neighbour's primitive state, or the ghost state of the
 // cell_state boundary map (calcboco_ust.f90:44-105) built from this cell's.
 template <int DIM, typename R>
+__device__ __forceinline__ void ghost_state(const KArgs<R>& a, int k, int c, int L, const R* qc, R* qo)
+{
+    R nf[DIM];
+    slot_normal<DIM>(a.g, k, a.n, c, nf);
+    const BocoDev& b = a.boco[a.g.bboco[link_index(L)]];
+    if (b.kind == BK_WALL_FLUX) {
+        R rk[DIM];
+        slot_vec<DIM>(a.g.sr, k, a.n, c, rk);
+        bc_state<DIM>(b, nf, qc, qo, a.ph, false, rk);
+    } else {
+        bc_state<DIM>(b, nf, qc, qo, a.ph);
+    }
+}
+
+template <int DIM, typename R>
 __device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int L, const R* qc, R* qo, R& ir)
 {
     if (!link_boundary(L)) {
         load_prim<DIM>(a.us, link_index(L), R(a.ph.gamma - 1.0), qo, ir);
     } else {
-        R nf[DIM];
-        slot_normal<DIM>(a.g, k, a.n, c, nf);
-        bc_state<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, qc, qo, a.ph);
+        ghost_state<DIM>(a, k, c, L, qc, qo);
         ir = arcp(qo[0]);
     }
 }
@@ -482,9 +566,7 @@ __device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int L
     if (!link_boundary(L)) {
         load_prim<DIM>(a.us, link_index(L), R(a.ph.gamma - 1.0), qo);
     } else {
-        R nf[DIM];
-        slot_normal<DIM>(a.g, k, a.n, c, nf);
-        bc_state<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, qc, qo, a.ph);
+        ghost_state<DIM>(a, k, c, L, qc, qo);
     }
 }
 
@@ -773,11 +855,13 @@ __device__ __forceinline__ void side_temp(const Physics& ph, const R* q, R ir, c
 // viscous_face for the residual kernel: the same flux from per-side temperatures and
 // temperature gradients computed once per cell (side_temp), velocity gradients gUL/gUR
 // (row-major d u_i / d x_j), host-derived constants, and Sutherland's (T/Tref)^1.5 as
-// t sqrt(t) (calc_viscosity, EQNS.f90:243-247).
+// t sqrt(t) (calc_viscosity, EQNS.f90:243-247).  heat = false leaves out the conductive
+// term (flux walls, where the imposed flux replaces it).
 template <int DIM, typename R>
 __device__ __forceinline__ void viscous_face_pre(const Physics& ph, const R* qL, const R* qR, R TL, R TR,
                                                  const R* gTL, const R* gTR, const R* gUL, const R* gUR,
-                                                 const R* rL, const R* rR, const R* vLR, const R* nf, R* flux)
+                                                 const R* rL, const R* rR, const R* vLR, const R* nf, bool heat,
+                                                 R* flux)
 {
     R dHL = asqrt(dotd<DIM>(rL, rL)), dHR = asqrt(dotd<DIM>(rR, rR));
     const R id = arcp(dHL + dHR);
@@ -845,7 +929,7 @@ __device__ __forceinline__ void viscous_face_pre(const Physics& ph, const R* qL,
 #pragma unroll
     for (int e = 0; e < DIM; ++e) flux[2 + e] = flux[2 + e] - sn[e];
     flux[1] = flux[1] - svn;
-    flux[1] = flux[1] - (mu * R(ph.cp_pr)) * gTH;
+    if (heat) flux[1] = flux[1] - (mu * R(ph.cp_pr)) * gTH;
 }
 
 // ---------------------------------------------------------------------------
@@ -1023,7 +1107,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
             } else {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) qLf[v] = own[v];
-                bc_state<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, qLf, qRf, a.ph);
+                bc_state<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, qLf, qRf, a.ph, true);
             }
             irL = arcp(qLf[0]);
             irR = arcp(qRf[0]);
@@ -1035,6 +1119,12 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
             }
             irL = other ? iro : irc;
             irR = other ? irc : iro;
+            if (bnd && a.boco[a.g.bboco[link_index(L)]].kind == BK_WALL_FLUX) {
+                // first order still takes the right face state from the face_state map
+                // (calc_rhs.f90:103-109), which differs from the ghost cell's on flux walls
+                bc_state<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, qc, qRf, a.ph, true);
+                irR = arcp(qRf[0]);
+            }
         }
         R flux[NV];
         num_flux_pre<DIM>(a.ph.sch_hyp, gam, R(a.ph.ig1), R(a.ph.gig1), qLf, qRf, irL, irR, nf, flux);
@@ -1055,9 +1145,13 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
             side_temp<DIM>(a.ph, qo, iro, go, To, gTo);
             const R* guc = gcell + 2 * DIM;
             const R* guo = go + 2 * DIM;
+            // ns_bocoflux (ns_bocoflux.f90:47-111): on flux and adiabatic walls the imposed heat
+            // flux replaces the conductive one
+            const bool fwall = bnd && a.boco[a.g.bboco[link_index(L)]].kind == BK_WALL_FLUX;
             viscous_face_pre<DIM>(a.ph, other ? qo : qc, other ? qc : qo, other ? To : Tc, other ? Tc : To,
                                   other ? gTo : gTc, other ? gTc : gTo, other ? guo : guc, other ? guc : guo,
-                                  rL, rR, dk, nf, flux);
+                                  rL, rR, dk, nf, !fwall, flux);
+            if (fwall) flux[1] = flux[1] + R(a.boco[a.g.bboco[link_index(L)]].param[0]);
         }
         if (JAC) {
             // build_implicit_bdlu.f90:44-83: diag(ic1) += jacL.A dt/V, diag(ic2) -= jacR.A dt/V
