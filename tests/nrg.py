@@ -147,3 +147,53 @@ def replay_probes(step_fn, dt_fn, probe_fn, ncycle, duration):
         curtps = curtps + dtbase
         out.append((icycle, curtps, *probe_fn(curtps)))
     return np.array(out)
+
+
+# ---------------------------------------------------------------------------
+# NRG/MESH/ELEM/TYM-QUAD (quad20x10.tym, every boundary SYMMETRY, Euler air, first-order HLLC,
+# RK3-SSP at CFL 1; P = 1e5, T = 300, MACH = 0.2 along x; PAVG = VOL_AVERAGE of the static
+# pressure, 10 cycles over 5e-3) and NRG/MESH/CON/PER2D-TRANS (quad20x20.tym periodic in x and y,
+# Rusanov, RK3-SSP at CFL 1; rho = 1 + 9 x(x-1) y(y-1), p = 1e5/14, u = 10; RHODEV = VOL_AVERAGE
+# (rho - 1.25)^2, 100 cycles over 0.1).
+
+MESH_CASES = {"TYM-QUAD": ("PAVG", 10, 0.005), "PER2D-TRANS": ("RHODEV", 100, 0.1)}
+
+
+def golden_mesh(case):
+    return np.loadtxt(os.path.join(HERE, "golden", "nrg_mesh", case, f"{MESH_CASES[case][0]}.tmon"))
+
+
+def setup_mesh_case(case):
+    from typhon_amd import tyfmt
+    if case == "TYM-QUAD":
+        bocos = [dict(family=f, type=M.BC_SYM) for f in ("IMIN", "IMAX", "JMIN", "JMAX")]
+        m = tyfmt.tym_ustmesh(os.path.join(HERE, "golden", "meshes", "quad20x10.tym"), bocos)
+        n = m.ncell_int
+        g = S.default_gas()
+        T, p, mach = 300.0, 1e5, 0.2
+        rho = p / (g["r_const"] * T)
+        vel = np.zeros((3, n))
+        vel[0] = mach * np.sqrt(g["gamma"] * g["r_const"] * T)
+        cons = prim_to_cons(np.full(n, rho), np.full(n, p), vel)
+        sch = S.default_scheme(sch_hyp=S.SCH_HLLC, tps_meth=S.TPS_RK3SSP, cfl=1.0)
+        return m, sch, g, cons
+    m = quad20x20()
+    n = m.ncell_int
+    x = m.ccent[:n]
+    rho = 1.0 + 9 * x[:, 0] * (x[:, 0] - 1) * x[:, 1] * (x[:, 1] - 1)
+    vel = np.zeros((3, n))
+    vel[0] = 10.0
+    cons = prim_to_cons(rho, np.full(n, 100000 / 14.), vel)
+    sch = S.default_scheme(sch_hyp=S.SCH_RUSANOV, tps_meth=S.TPS_RK3SSP, cfl=1.0)
+    return m, sch, S.default_gas(), cons
+
+
+def mesh_case_probe(case, m, cons):
+    n = m.ncell_int
+    v = m.cvol[:n]
+    if case == "TYM-QUAD":
+        g = S.default_gas()["gamma"]
+        q = (g - 1) * (cons[1] - 0.5 * (cons[2] ** 2 + cons[3] ** 2 + cons[4] ** 2) / cons[0])
+    else:
+        q = (cons[0] - 1.25) ** 2
+    return ((q * v).sum() / v.sum(),)

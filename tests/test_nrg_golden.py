@@ -1,6 +1,7 @@
-"""The reference's golden RK histories (NRG/TIME/RUNGEKUTTA/*/RHOERR.tmon) replayed by the oracle
+"""The reference's golden probe histories (NRG/TIME/RUNGEKUTTA/*/RHOERR.tmon, NRG/NS/NUM/FLUX/VISCOUS,
+NRG/MESH/ELEM/TYM-QUAD, NRG/MESH/CON/PER2D-TRANS) replayed by the oracle
 (CPU) and by the HIP path (GPU).  The files hold 8 significant digits; the check is 2e-7 relative
-per cycle over all 100 cycles."""
+per cycle over every cycle."""
 import numpy as np
 import pytest
 
@@ -57,3 +58,29 @@ def test_gpu_matches_nrg_visc_golden(case):
                             lambda t: nrg.visc_probes(m, gpu.get_state(), t), 10, 10.0)
     np.testing.assert_allclose(res[:, 2], nrg.golden_visc(case, "VXERR")[:, 2], rtol=RTOL)
     np.testing.assert_allclose(res[:, 3], nrg.golden_visc(case, "DELTA")[:, 2], rtol=RTOL)
+
+
+@pytest.mark.parametrize("case", list(nrg.MESH_CASES))
+def test_oracle_matches_nrg_mesh_golden(case):
+    """NRG/MESH/ELEM/TYM-QUAD (first-order HLLC, slip walls, PAVG) and NRG/MESH/CON/PER2D-TRANS
+    (Rusanov, periodic, RHODEV) on the reference's own .tym meshes."""
+    probe, ncyc, dur = nrg.MESH_CASES[case]
+    m, s, g, u = nrg.setup_mesh_case(case)
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u)
+    res = nrg.replay_probes(o.rk_step, lambda rem: o.timestep(s["cfl"], rem),
+                            lambda t: nrg.mesh_case_probe(case, m, o.get_cons()), ncyc, dur)
+    np.testing.assert_allclose(res[:, 2], nrg.golden_mesh(case)[:, 2], rtol=RTOL)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", list(nrg.MESH_CASES))
+def test_gpu_matches_nrg_mesh_golden(case):
+    from typhon_amd.solver import Solver
+    probe, ncyc, dur = nrg.MESH_CASES[case]
+    m, s, g, u = nrg.setup_mesh_case(case)
+    gpu = Solver(m, s, g)
+    gpu.set_state(u)
+    res = nrg.replay_probes(gpu.rk_step, lambda rem: gpu.timestep(rem),
+                            lambda t: nrg.mesh_case_probe(case, m, gpu.get_state()), ncyc, dur)
+    np.testing.assert_allclose(res[:, 2], nrg.golden_mesh(case)[:, 2], rtol=RTOL)
