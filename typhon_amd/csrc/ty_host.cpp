@@ -15,8 +15,10 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/typhon_gpu.h"
@@ -77,6 +79,8 @@ struct ty_ctx {
     int* d_bboco = nullptr;
     BocoDev* d_boco = nullptr;
     double* d_vol = nullptr;
+    // FP64 mesh arrays the device geometry build reads (kept with the mesh)
+    double *d_cc = nullptr, *d_fc = nullptr, *d_fn = nullptr, *d_fs = nullptr, *d_gc = nullptr, *d_ps = nullptr;
     double* d_dtfac = nullptr;
     double* d_u0 = nullptr;
     void* d_us[2] = {nullptr, nullptr};
@@ -116,6 +120,19 @@ int fail(ty_ctx* c, int code, const std::string& msg)
     return code;
 }
 
+// Setup phase durations on stderr when TY_SETUP_TIMING is set in the environment.
+struct PhaseTimer {
+    bool on = getenv("TY_SETUP_TIMING") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void operator()(const char* what)
+    {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        fprintf(stderr, "# setup %-28s %8.3f s\n", what, std::chrono::duration<double>(now - t).count());
+        t = now;
+    }
+};
+
 #define HIPCHK(ctx, expr)                                                                       \
     do {                                                                                        \
         hipError_t e_ = (expr);                                                                 \
@@ -137,6 +154,7 @@ void free_state(ty_ctx* c)
     c->d_bboco = nullptr;
     c->d_boco = nullptr;
     c->d_vol = c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
+    c->d_cc = c->d_fc = c->d_fn = c->d_fs = c->d_gc = c->d_ps = nullptr;
     c->d_us[0] = c->d_us[1] = nullptr;
     c->d_grad = c->d_qf = nullptr;
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
@@ -170,6 +188,27 @@ int upload(ty_ctx* c, T** p, const std::vector<T>& h)
     if (rc) return rc;
     if (!h.empty()) HIPCHK(c, hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     return TY_OK;
+}
+
+// fn(i) for i in [0, n) over up to 16 host threads (contiguous chunks); serial for small n
+template <typename Fn>
+void parallel_for(int n, Fn fn)
+{
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < (1 << 16) || nt == 1) {
+        for (int i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const int lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo >= hi) break;
+        th.emplace_back([&fn, lo, hi] {
+            for (int i = lo; i < hi; ++i) fn(i);
+        });
+    }
+    for (auto& x : th) x.join();
 }
 
 // reverse Cuthill-McKee on the interior-cell graph
@@ -228,7 +267,7 @@ std::vector<int> rcm(int n, const std::vector<int>& xadj, const std::vector<int>
 
 // Hilbert-curve order of the cell centres (J. Skilling, "Programming the Hilbert
 // curve", AIP Conf. Proc. 707 (2004) 381: axes -> transposed index, then bit interleave)
-std::vector<int> hilbert_order(int n, const double* centre, int dim)
+int hilbert_order(ty_ctx* c, int n, const double* centre, int dim, std::vector<int>& order)
 {
     const int bits = dim == 3 ? 21 : 31;
     double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
@@ -239,8 +278,8 @@ std::vector<int> hilbert_order(int n, const double* centre, int dim)
         }
     const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], dim == 3 ? hi[2] - lo[2] : 0.0, 1e-300});
     const double scale = (double)((1u << (bits - 1)) - 1) * 2.0 / span;
-    std::vector<std::pair<uint64_t, int>> key(n);
-    for (int i = 0; i < n; ++i) {
+    std::vector<uint64_t> key(n);
+    parallel_for(n, [&](int i) {
         uint32_t X[3] = {0, 0, 0};
         for (int d = 0; d < dim; ++d) X[d] = (uint32_t)((centre[(size_t)3 * i + d] - lo[d]) * scale);
         const uint32_t M = 1u << (bits - 1);
@@ -259,24 +298,12 @@ std::vector<int> hilbert_order(int n, const double* centre, int dim)
         uint64_t h = 0;
         for (int b = bits - 1; b >= 0; --b)
             for (int d = 0; d < dim; ++d) h = (h << 1) | ((X[d] >> b) & 1u);
-        key[i] = {h, i};
-    }
-    std::stable_sort(key.begin(), key.end());
-    std::vector<int> order(n);
-    for (int i = 0; i < n; ++i) order[i] = key[i].second;
-    return order;
-}
-
-// cholesky_decomp (SOURCE/MATH/MATRIX.f90:42-70), a(i,j) column-major 3x3
-void cholesky3(double* a)
-{
-    for (int i = 0; i < 3; ++i)
-        for (int j = i; j < 3; ++j) {
-            double sum = a[i + 3 * j];
-            for (int k = i - 1; k >= 0; --k) sum = sum - a[i + 3 * k] * a[j + 3 * k];
-            if (i == j) a[i + 3 * i] = std::sqrt(sum);
-            else a[j + 3 * i] = sum / a[i + 3 * i];
-        }
+        key[i] = h;
+    });
+    // stable in the cell index: the order std::stable_sort gives on (key, cell)
+    order.resize(n);
+    HIPCHK(c, sort_keys_device(key.data(), n, 64, order.data(), c->stream));
+    return TY_OK;
 }
 
 }  // namespace
@@ -318,6 +345,7 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
     const int ni = m->ncell_int, nc = m->ncell, nf = m->nface, nfi = m->nface_int;
     if (ni <= 0 || nc < ni || nf <= 0 || nfi < 0 || nfi > nf) return fail(c, TY_ERR_ARG, "bad mesh sizes");
     hipSetDevice(c->device);
+    PhaseTimer phase;
     free_state(c);
     c->mesh_ok = c->scheme_ok = c->state_ok = false;
     c->dim = m->dim;
@@ -376,10 +404,12 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
             bfaces.push_back({ic1[f], -1, f, -1, fboco[f]});
         }
     }
+    phase("mesh: checks, face lists");
     // cell order: RCM over the face graph, a Hilbert curve through the cell centres, or the caller's
     std::vector<int> order;
     if (m->cell_order == 1) {
-        order = hilbert_order(ni, m->centre, m->dim);
+        int rc0 = hilbert_order(c, ni, m->centre, m->dim, order);
+        if (rc0) return rc0;
     } else if (m->cell_order == 2) {
         order.resize(ni);
         for (int i = 0; i < ni; ++i) order[i] = i;
@@ -394,20 +424,36 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
             for (auto& d : *L) { adj[pos[d.a]++] = d.b; adj[pos[d.b]++] = d.a; }
         order = rcm(ni, xadj, adj);
     }
+    phase("mesh: cell order");
     c->perm = order;
     c->iperm.assign(ni, 0);
     for (int i = 0; i < ni; ++i) c->iperm[c->perm[i]] = i;
-    auto lowfirst = [](const DF& x, const DF& y) {
-        int xl = std::min(x.a, x.b), yl = std::min(y.a, y.b);
-        if (xl != yl) return xl < yl;
-        return std::max(x.a, x.b) < std::max(y.a, y.b);
-    };
     for (auto* L : {&faces, &pfaces})
-        for (auto& d : *L) { d.a = c->iperm[d.a]; d.b = c->iperm[d.b]; }
-    for (auto& d : bfaces) d.a = c->iperm[d.a];
-    std::stable_sort(faces.begin(), faces.end(), lowfirst);
-    std::stable_sort(pfaces.begin(), pfaces.end(), lowfirst);
-    std::stable_sort(bfaces.begin(), bfaces.end(), [](const DF& x, const DF& y) { return x.a < y.a; });
+        parallel_for((int)L->size(), [&](int k) { DF& d = (*L)[k]; d.a = c->iperm[d.a]; d.b = c->iperm[d.b]; });
+    parallel_for((int)bfaces.size(), [&](int k) { bfaces[k].a = c->iperm[bfaces[k].a]; });
+    // faces ordered by (lower cell, higher cell), boundary faces by cell, ties in list order:
+    // a stable device radix sort of the packed keys
+    auto sort_faces = [&](std::vector<DF>& L, bool two) -> int {
+        const int mf = (int)L.size();
+        std::vector<uint64_t> key(mf);
+        parallel_for(mf, [&](int k) {
+            const DF& d = L[k];
+            key[k] = two ? ((uint64_t)(uint32_t)std::min(d.a, d.b) << 32) | (uint32_t)std::max(d.a, d.b)
+                         : (uint64_t)(uint32_t)d.a;
+        });
+        std::vector<int> idx(mf);
+        HIPCHK(c, sort_keys_device(key.data(), mf, two ? 64 : 32, idx.data(), c->stream));
+        std::vector<DF> out(mf);
+        parallel_for(mf, [&](int k) { out[k] = L[idx[k]]; });
+        L.swap(out);
+        return TY_OK;
+    };
+    {
+        int rc0;
+        if ((rc0 = sort_faces(faces, true)) || (rc0 = sort_faces(pfaces, true)) || (rc0 = sort_faces(bfaces, false)))
+            return rc0;
+    }
+    phase("mesh: face renumbering");
     // device face ranges: internal | periodic pairs | physical boundary
     c->per_base = (int)faces.size();
     c->nper = (int)pfaces.size();
@@ -424,7 +470,6 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
     c->gc.assign((size_t)3 * c->nb, 0.0);
     c->bboco.assign(c->nb, 0);
     c->ps.assign((size_t)3 * std::max(1, c->nper), 0.0);
-    std::vector<std::vector<std::pair<int, int>>> slots(n);  // (nbr, word)
     auto put_face = [&](int df, int src) {
         for (int d = 0; d < 3; ++d) {
             c->fn[(size_t)d * ndf + df] = F(src, d);
@@ -434,83 +479,105 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         c->fsrc[df] = src;
     };
     c->fcell.assign((size_t)2 * ndf, -1);
-    for (int k = 0; k < c->nfi; ++k) {
-        const bool per = k >= c->per_base;
-        const DF& d = per ? pfaces[k - c->per_base] : faces[k];
+    auto dface = [&](int k) -> const DF& { return k >= c->per_base ? pfaces[k - c->per_base] : faces[k]; };
+    parallel_for(c->nfi, [&](int k) {
+        const DF& d = dface(k);
         put_face(k, d.src);
         c->fcell[k] = d.a;
         c->fcell[(size_t)ndf + k] = d.b;
-        unsigned w = (unsigned)k;
-        if (per) {
+        if (k >= c->per_base) {
             // far side seen through this face: x_far + (c_face - c_partner) (init_connect_grid.f90:226-231)
             const int j = k - c->per_base;
             for (int e = 0; e < 3; ++e) c->ps[(size_t)e * c->nper + j] = F(d.src, 3 + e) - F(d.part, 3 + e);
-            w |= PERIODIC_BIT;
         }
-        slots[d.a].push_back({d.b, (int)w});
-        slots[d.b].push_back({d.a, (int)(w | 0x80000000u)});
-    }
-    for (int k = 0; k < c->nb; ++k) {
+    });
+    parallel_for(c->nb, [&](int k) {
         const DF& d = bfaces[k];
-        int df = c->nfi + k;
+        const int df = c->nfi + k;
         put_face(df, d.src);
         c->fcell[df] = d.a;
-        int g = ic2[d.src];
+        const int g = ic2[d.src];
         for (int e = 0; e < 3; ++e) c->gc[(size_t)e * c->nb + k] = m->centre[(size_t)3 * g + e];
         c->bboco[k] = d.bidx;
-        slots[d.a].push_back({-(k + 1), df});
-    }
+    });
+    phase("mesh: faces, slot lists");
+    // slot lists (CSR): each cell's faces in ascending device face order, as (neighbour or
+    // -(boundary face + 1), slot word = face | periodic bit | other-side bit)
+    std::vector<int> sbeg(n + 1, 0);
+    for (int k = 0; k < c->nfi; ++k) { const DF& d = dface(k); ++sbeg[d.a + 1]; ++sbeg[d.b + 1]; }
+    for (int k = 0; k < c->nb; ++k) ++sbeg[bfaces[k].a + 1];
     int ns = 0;
-    for (auto& sl : slots) ns = std::max<int>(ns, (int)sl.size());
+    c->dense = 1;
+    for (int i = 0; i < n; ++i) ns = std::max(ns, sbeg[i + 1]);
+    for (int i = 0; i < n; ++i)
+        if (sbeg[i + 1] != ns) c->dense = 0;
+    for (int i = 0; i < n; ++i) sbeg[i + 1] += sbeg[i];
     if (ns > MAX_SLOTS) return fail(c, TY_ERR_UNSUPPORTED, "more than 8 faces per cell");
     c->ns = ns;
-    c->dense = 1;
-    for (auto& sl : slots)
-        if ((int)sl.size() != ns) c->dense = 0;
+    std::vector<int> snb(sbeg[n]), sword(sbeg[n]), spos(sbeg.begin(), sbeg.end() - 1);
+    for (int k = 0; k < c->nfi; ++k) {
+        const DF& d = dface(k);
+        const unsigned w = (unsigned)k | (k >= c->per_base ? PERIODIC_BIT : 0u);
+        snb[spos[d.a]] = d.b;
+        sword[spos[d.a]++] = (int)w;
+        snb[spos[d.b]] = d.a;
+        sword[spos[d.b]++] = (int)(w | 0x80000000u);
+    }
+    for (int k = 0; k < c->nb; ++k) {
+        const int a = bfaces[k].a;
+        snb[spos[a]] = -(k + 1);
+        sword[spos[a]++] = c->nfi + k;
+    }
     c->nbr.assign((size_t)ns * n, 0);
     c->fw.assign((size_t)ns * n, SLOT_UNUSED);
     c->link.assign((size_t)ns * n, SLOT_UNUSED);
     if (n >= (1 << 27) || c->nb >= (1 << 27)) return fail(c, TY_ERR_UNSUPPORTED, "too many cells for the slot link word");
     c->sumA.assign(n, 0.0);
-    // mirror slots: where the far cell keeps the same face
+    // mirror slots: where the far cell keeps the same face (each face side is written once)
     std::vector<unsigned char> slot_own(c->nfi, 0), slot_oth(c->nfi, 0);
-    for (int i = 0; i < n; ++i)
-        for (size_t k = 0; k < slots[i].size(); ++k) {
-            if (slots[i][k].first < 0) continue;
-            const unsigned w = (unsigned)slots[i][k].second;
+    parallel_for(n, [&](int i) {
+        for (int k = 0; k < sbeg[i + 1] - sbeg[i]; ++k) {
+            if (snb[sbeg[i] + k] < 0) continue;
+            const unsigned w = (unsigned)sword[sbeg[i] + k];
             (w & 0x80000000u ? slot_oth : slot_own)[w & FACE_MASK] = (unsigned char)k;
         }
-    for (int i = 0; i < n; ++i)
-        for (size_t k = 0; k < slots[i].size(); ++k) {
-            unsigned w = (unsigned)slots[i][k].second;
-            if (slots[i][k].first >= 0) {
+    });
+    parallel_for(n, [&](int i) {
+        for (int k = 0; k < sbeg[i + 1] - sbeg[i]; ++k) {
+            const int nbv = snb[sbeg[i] + k];
+            unsigned w = (unsigned)sword[sbeg[i] + k];
+            if (nbv >= 0) {
                 const unsigned mk = (w & 0x80000000u) ? slot_own[w & FACE_MASK] : slot_oth[w & FACE_MASK];
                 w |= mk << MIRROR_SHIFT;
             }
-            c->nbr[k * n + i] = slots[i][k].first;
-            c->fw[k * n + i] = (int)w;
-            const int nbv = slots[i][k].first;
+            c->nbr[(size_t)k * n + i] = nbv;
+            c->fw[(size_t)k * n + i] = (int)w;
             if (nbv >= 0) {
                 const unsigned mk = (w >> MIRROR_SHIFT) & (MAX_SLOTS - 1);
-                c->link[k * n + i] = (int)(((w & 0x80000000u) ? (1u << 30) : 0u) | (mk << 27) | (unsigned)nbv);
+                c->link[(size_t)k * n + i] = (int)(((w & 0x80000000u) ? (1u << 30) : 0u) | (mk << 27) | (unsigned)nbv);
             } else {
-                c->link[k * n + i] = (int)(0x80000000u | (unsigned)(-nbv - 1));
+                c->link[(size_t)k * n + i] = (int)(0x80000000u | (unsigned)(-nbv - 1));
             }
             c->sumA[i] += c->fs[w & FACE_MASK];
         }
+    });
     c->cc.assign((size_t)3 * n, 0.0);
     c->vol.assign(n, 0.0);
-    for (int i = 0; i < n; ++i) {
-        int o = c->perm[i];
+    parallel_for(n, [&](int i) {
+        const int o = c->perm[i];
         for (int d = 0; d < 3; ++d) c->cc[(size_t)d * n + i] = m->centre[(size_t)3 * o + d];
         c->vol[i] = m->volume[o];
-    }
+    });
+    phase("mesh: slot words, cell data");
     int rc;
     if ((rc = upload(c, &c->d_nbr, c->nbr)) || (rc = upload(c, &c->d_fw, c->fw)) ||
         (rc = upload(c, &c->d_bboco, c->bboco)) || (rc = upload(c, &c->d_vol, c->vol)) ||
         (rc = upload(c, &c->d_link, c->link)) ||
-        (rc = upload(c, &c->d_fcell, c->fcell)))
+        (rc = upload(c, &c->d_fcell, c->fcell)) || (rc = upload(c, &c->d_cc, c->cc)) ||
+        (rc = upload(c, &c->d_fc, c->fc)) || (rc = upload(c, &c->d_fn, c->fn)) || (rc = upload(c, &c->d_fs, c->fs)) ||
+        (rc = upload(c, &c->d_gc, c->gc)) || (rc = upload(c, &c->d_ps, c->ps)))
         return rc;
+    phase("mesh: uploads");
     c->mesh_bufs = c->owned;  // everything allocated so far belongs to the mesh
     c->mesh_ok = true;
     return TY_OK;
@@ -531,82 +598,35 @@ namespace {
 template <typename R>
 int build_geo(ty_ctx* c, Geo<R>& g)
 {
-    const int n = c->n, nf = c->nf, nb = c->nb, dim = c->dim, ns = c->ns;
-    // layout: ss(3 ns n) sw(3 ns n) sr(3 ns n) sd(3 ns n) fn(3 nf) fs(nf) eps2(n)
+    // per-slot geometry, built on the device from the mesh arrays (ty_setup.hip):
+    // ss | sw | sr | sd ([ns][3][n] each) | fn ([3][nf]) | fs ([nf]) | eps2 ([n])
+    const int n = c->n, nf = c->nf, ns = c->ns;
     const size_t S3 = (size_t)3 * ns * n;
     const size_t o_ss = 0, o_sw = o_ss + S3, o_sr = o_sw + S3, o_sd = o_sr + S3;
     const size_t o_fn = o_sd + S3, o_fs = o_fn + 3 * (size_t)nf, o_eps = o_fs + nf, tot = o_eps + n;
-    std::vector<R> h(tot, R(0));
-    for (size_t i = 0; i < 3 * (size_t)nf; ++i) h[o_fn + i] = (R)c->fn[i];
-    for (int i = 0; i < nf; ++i) h[o_fs + i] = (R)c->fs[i];
-    auto at3 = [&](size_t base, int k, int e, int i) -> R& { return h[base + ((size_t)k * 3 + e) * n + i]; };
-    // per slot: canonical d = x_R - x_L (owner frame; periodic far sides translated), r = x_f - x_c
-    // in this cell's frame, and the LSQ weights w_k = A^-1 (x_nb - x_c) [/|d|^4 weighted]
-    // (precalc_grad_lsq.f90:60-155: A = sum d d^T [/|d|^4], boundary faces once, 2D A33 += 1)
-    const bool wlsq = c->sch.gradmeth == 12;
-    const double K = c->sch.venkat_k;
-    std::vector<double> dl((size_t)MAX_SLOTS * 3);
-    for (int i = 0; i < n; ++i) {
-        double A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        const double x[3] = {c->cc[i], c->cc[(size_t)n + i], c->cc[(size_t)2 * n + i]};
-        for (int k = 0; k < ns; ++k) {
-            const int wd = c->fw[(size_t)k * n + i];
-            if (wd == SLOT_UNUSED) continue;
-            const int nbv = c->nbr[(size_t)k * n + i];
-            const int f = (int)((unsigned)wd & FACE_MASK);
-            const bool other = wd < 0;
-            double T[3] = {0.0, 0.0, 0.0};
-            if ((unsigned)wd & PERIODIC_BIT) {
-                const int j = f - c->per_base;
-                for (int e = 0; e < 3; ++e) T[e] = c->ps[(size_t)e * c->nper + j];
-            }
-            double d[3], r[3];
-            for (int e = 0; e < 3; ++e) {
-                const double xf = c->fc[(size_t)e * nf + f];
-                if (nbv >= 0) {
-                    const double xo = c->cc[(size_t)e * n + nbv];
-                    d[e] = other ? (x[e] + T[e]) - xo : (xo + T[e]) - x[e];
-                    r[e] = other ? (xf - T[e]) - x[e] : xf - x[e];
-                } else {
-                    d[e] = c->gc[(size_t)e * nb + (-nbv - 1)] - x[e];
-                    r[e] = xf - x[e];
-                }
-                at3(o_ss, k, e, i) = (R)(c->fn[(size_t)e * nf + f] * c->fs[f]);
-                at3(o_sr, k, e, i) = (R)r[e];
-                at3(o_sd, k, e, i) = (R)d[e];
-            }
-            double m[3];
-            const double s2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            for (int e = 0; e < 3; ++e) m[e] = wlsq ? d[e] / s2 : d[e];
-            for (int a2 = 0; a2 < 3; ++a2)
-                for (int b2 = 0; b2 < 3; ++b2) A[a2 + 3 * b2] += m[a2] * m[b2];
-            // right-hand-side direction x_nb - x_c in this cell's frame
-            for (int e = 0; e < 3; ++e) dl[(size_t)k * 3 + e] = (other ? -d[e] : d[e]) / (wlsq ? s2 * s2 : 1.0);
-        }
-        if (dim == 2) A[8] += 1.0;
-        cholesky3(A);
-        const double l11 = A[0], l21 = A[1], l31 = A[2], l22 = A[4], l32 = A[5], l33 = A[8];
-        for (int k = 0; k < ns; ++k) {
-            if (c->fw[(size_t)k * n + i] == SLOT_UNUSED) continue;
-            const double* b = &dl[(size_t)k * 3];
-            // cholesky_solve (MATRIX.f90:79-111)
-            const double y1 = b[0] / l11;
-            const double y2 = (b[1] - l21 * y1) / l22;
-            const double y3 = ((b[2] - l31 * y1) - l32 * y2) / l33;
-            const double x3 = y3 / l33;
-            const double x2 = (y2 - l32 * x3) / l22;
-            const double x1 = ((y1 - l21 * x2) - l31 * x3) / l11;
-            at3(o_sw, k, 0, i) = (R)x1;
-            at3(o_sw, k, 1, i) = (R)x2;
-            at3(o_sw, k, 2, i) = (R)x3;
-        }
-        const double h3 = dim == 3 ? c->vol[i] : std::pow(c->vol[i], 1.5);
-        h[o_eps + i] = (R)(K * K * K * h3);
-    }
     R* dp = nullptr;
     int rc = dalloc(c, &dp, tot);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpy(dp, h.data(), tot * sizeof(R), hipMemcpyHostToDevice));
+    GeoSrc src;
+    src.n = n;
+    src.nf = nf;
+    src.nb = c->nb;
+    src.nper = std::max(1, c->nper);
+    src.per_base = c->per_base;
+    src.ns = ns;
+    src.dim = c->dim;
+    src.wlsq = c->sch.gradmeth == 12;
+    src.K = c->sch.venkat_k;
+    src.cc = c->d_cc;
+    src.fc = c->d_fc;
+    src.fn = c->d_fn;
+    src.fs = c->d_fs;
+    src.gc = c->d_gc;
+    src.ps = c->d_ps;
+    src.vol = c->d_vol;
+    src.nbr = c->d_nbr;
+    src.fw = c->d_fw;
+    HIPCHK(c, launch_geo<R>(src, dp, c->stream));
     g.ss = dp + o_ss;
     g.sw = dp + o_sw;
     g.sr = dp + o_sr;
@@ -766,7 +786,10 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     for (const BocoDev& t : tab)
         if (t.kind == BK_WALL_ISOTH || t.kind == BK_WALL_FLUX || t.kind == BK_INLET_SUB) c->ext_bc = true;
     // both precisions: the FP32 geometry also serves the FP32 Krylov products
+    PhaseTimer phase;
     if ((rc = build_geo<float>(c, c->geo_f)) || (rc = build_geo<double>(c, c->geo_d))) return rc;
+    if (phase.on) hipStreamSynchronize(c->stream);
+    phase("scheme: slot geometry (device)");
     std::vector<double> dtfac(c->n);
     for (int i = 0; i < c->n; ++i) dtfac[i] = s->cfl * 2.0 * c->vol[i] / c->sumA[i];
     if ((rc = upload(c, &c->d_dtfac, dtfac))) return rc;

@@ -129,4 +129,20 @@ hipError_t launch_to_stage(const double* src, R* dst, int nv, int n, hipStream_t
 
 int grid_blocks(int n);  // XCD-padded block count
 
+// Device inputs of the per-slot geometry build (ty_setup.hip): mesh arrays in FP64 as the
+// host renumbered them ([3][n] cell centres, [3][nf] face centres and normals, [nf] areas,
+// [3][nb] ghost centres, [3][nper] periodic translations, [n] volumes) and the slot lists.
+struct GeoSrc {
+    int n, nf, nb, nper, per_base, ns, dim, wlsq;
+    double K;
+    const double *cc, *fc, *fn, *fs, *gc, *ps, *vol;
+    const int *nbr, *fw;
+};
+// writes ss | sw | sr | sd ([ns][3][n] each) | fn ([3][nf]) | fs ([nf]) | eps2 ([n]) in precision R
+template <typename R>
+hipError_t launch_geo(const GeoSrc& s, R* out, hipStream_t st);
+// stable ascending device sort of n host keys over bits [0, end_bit): idx_out (host) gets the
+// input position of each key in sorted order (ty_setup.hip)
+hipError_t sort_keys_device(const uint64_t* keys, int n, int end_bit, int* idx_out, hipStream_t st);
+
 }  // namespace ty
