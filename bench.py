@@ -290,6 +290,27 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
                          grad_ms_avg=round(tm["grad_ms"] / max(1, tm["grad_n"]), 4),
                          flux_ms_avg=round(tm["flux_ms"] / max(1, tm["flux_n"]), 4))
         print(f"# ns {name}: {out[name]}", file=sys.stderr, flush=True)
+    # adjoint/primal cost of one NS residual: device time of the FP64 primal kernels (LSQ gradient +
+    # flux) vs the adjoint kernel chain (face reverse sweeps, gather with the gradient transpose, tail)
+    solver.set_scheme(dict(s, precision=S.PREC_FP64), g)
+    solver.set_state(u)
+    lam = np.random.default_rng(5).normal(size=(5, nc))
+    solver.calc_rhs(dt)
+    solver.residual_adjoint(dt, lam, area_vectors=False)
+    solver.sync()
+    solver.timing(True)
+    for _ in range(5):
+        solver.calc_rhs(dt)
+    tp = solver.timing(True)
+    for _ in range(5):
+        solver.residual_adjoint(dt, lam, area_vectors=False)
+    ta = solver.timing(False)
+    prim = tp["grad_ms"] / max(1, tp["grad_n"]) + tp["flux_ms"] / max(1, tp["flux_n"])
+    adj = ta["adjoint_ms"] / max(1, ta["adjoint_n"])
+    out["adjoint_ratio"] = dict(primal_kernels_ms=round(prim, 4), adjoint_kernels_ms=round(adj, 4),
+                                ratio=round(adj / prim, 3))
+    print(f"# ns adjoint/primal: {out['adjoint_ratio']}", file=sys.stderr, flush=True)
+    del lam
     r64, rmx = states.pop("fp64"), states.pop("mixed")
     out["speedup_mixed_vs_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
     out["density_rel_err_l2"] = float(np.sqrt(np.mean((rmx - r64) ** 2)) / np.sqrt(np.mean(r64 ** 2)))

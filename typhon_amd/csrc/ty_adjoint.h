@@ -30,6 +30,8 @@ struct AdArgs {
     double* gs;           // [3][nf] (dR/dS)^T lambda, written by each face's owner cell
     double* gvol;         // [n] (dR/dV)^T lambda
     double* gq;           // [nv][nb] per boundary face: (dR/dq_ghost)^T lambda (inlet faces, else 0)
+    const int* bcell;     // [nbcell] cells with a boundary slot, ascending
+    int nbcell;
 };
 
 // out = sum_i c[i] v[i] over m vectors of len doubles, summed in index order (RK stage states and
@@ -46,5 +48,7 @@ template <int DIM>
 hipError_t launch_res_tangent(const AdArgs& t, hipStream_t s);
 template <int DIM>
 hipError_t launch_res_adjoint(const AdArgs& t, hipStream_t s);
+template <int DIM>
+hipError_t launch_adj_face_geom(const AdArgs& t, hipStream_t s);  // fills t.fgeo (viscous adjoint)
 
 }  // namespace ty

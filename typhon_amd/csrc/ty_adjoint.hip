@@ -437,15 +437,16 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face_geom(AdArgs t)
 // sigma A lambda_side dt/V_side . F] (sigma = -1 owner, +1 other) for k_res_adjoint_gather; the
 // face's area-vector sensitivity (inviscid) and a boundary face's inlet-state sensitivity are
 // written here.
-template <int DIM, bool VISC>
-__global__ __launch_bounds__(BLOCK) void k_res_adjoint_face(AdArgs t)
+template <int DIM, bool VISC, bool BND>
+__global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs t)
 {
     constexpr int NV = DIM + 2, NG = VISC ? NV * DIM : 0, RW = NV + NG + 1;
     const KArgs<double>& a = t.k;
-    const int f = blockIdx.x * BLOCK + threadIdx.x;
-    if (f >= t.nf) return;
+    // two launches: two-sided faces [0, nfi), boundary faces [nfi, nf) (BND)
+    const int f = (BND ? t.nfi : 0) + (int)(blockIdx.x * BLOCK + threadIdx.x);
+    if (f >= (BND ? t.nf : t.nfi)) return;
     const int n = a.n;
-    const bool bnd = f >= t.nfi;
+    constexpr bool bnd = BND;
     const int cl = t.fcell[f], cr = bnd ? cl : t.fcell[(size_t)t.nf + f];
     const double dt = *a.d_dt;
     const double g1d = a.ph.gamma - 1.0;
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_face(AdArgs t)
 // this cell's own primitive adjoint gets -sum_k w_k . gb (and the ghost states' Jacobians on
 // boundary slots, or the inlet-state sensitivity), and the far cell of slot k gets t_k = w_k . gb,
 // left in t.slot for k_res_adjoint_tail.
-template <int DIM, bool VISC>
+template <int DIM, bool VISC, int NSM>
 __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
 {
     constexpr int NV = DIM + 2, NG = VISC ? NV * DIM : 0, RW = NV + NG + 1;
@@ -586,7 +587,10 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
 #pragma unroll
         for (int q = 0; q < NG; ++q) gb[q] = 0.0;
     }
-    for (int k = 0; k < a.ns; ++k) {
+    // slots unrolled to NSM >= ns: every slot's record loads can be in flight together
+#pragma unroll
+    for (int k = 0; k < NSM; ++k) {
+        if (k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
         const SlotInfo s = decode(w, 0);
@@ -601,13 +605,12 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
     }
     if (VISC) {
         const double g1d = a.ph.gamma - 1.0;
-        double Uc[NV], qc[NV], qb[NV];
+        double Uc[NV], qb[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             Uc[v] = a.us[(size_t)v * n + c];
             qb[v] = 0.0;
         }
-        cons_to_prim<DIM>(Uc, g1d, qc);
         for (int k = 0; k < a.ns; ++k) {
             const int L = a.link[(size_t)k * n + c];
             if (L == SLOT_UNUSED) continue;
@@ -627,28 +630,16 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
                 for (int v = 0; v < NV; ++v) o[v] = tk[v];
             } else {
                 const int bi = link_index(L);
-                const BocoDev bc = a.boco[a.g.bboco[bi]];
-                if (bc.kind == BK_INLET_SUP) {
+                if (a.boco[a.g.bboco[bi]].kind == BK_INLET_SUP) {
                     if (t.gq) {
 #pragma unroll
                         for (int v = 0; v < NV; ++v) t.gq[(size_t)v * t.nb + bi] += tk[v];
                     }
                 } else {
-                    double nv[DIM];
-                    slot_normal<DIM>(a.g, k, n, c, nv);
-                    Dual nd[DIM];
+                    // the ghost state's Jacobian is applied by k_res_adjoint_bnd_cells
+                    double* o = t.slot + ((size_t)k * n + c) * NV;
 #pragma unroll
-                    for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
-                    for (int j = 0; j < NV; ++j) {
-                        Dual qd[NV], gd[NV];
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) qd[v] = Dual(qc[v], v == j ? 1.0 : 0.0);
-                        bc_state<DIM>(bc, nd, qd, gd, a.ph);
-                        double sacc = 0.0;
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) sacc += gd[v].d * tk[v];
-                        qb[j] += sacc;
-                    }
+                    for (int v = 0; v < NV; ++v) o[v] = tk[v];
                 }
             }
         }
@@ -658,6 +649,56 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
     for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] = gu[v];
     // d(lambda.R_c)/dV_c = -lambda_c . R_c / V_c
     if (t.gvol) t.gvol[c] = -sv / a.vol[c];
+}
+
+// viscous, cells with boundary slots (t.bcell): the ghost cells' share of this cell's gradient
+// adjoint, t.slot at the boundary slot, chained through the transpose of the ghost-state map's
+// Jacobian (dual-number passes of the boundary template) and the primitive-variable reverse
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_res_adjoint_bnd_cells(AdArgs t)
+{
+    constexpr int NV = DIM + 2;
+    const KArgs<double>& a = t.k;
+    const int j0 = blockIdx.x * BLOCK + threadIdx.x;
+    if (j0 >= t.nbcell) return;
+    const int c = t.bcell[j0];
+    const int n = a.n;
+    const double g1d = a.ph.gamma - 1.0;
+    double Uc[NV], qc[NV], qb[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        Uc[v] = a.us[(size_t)v * n + c];
+        qb[v] = 0.0;
+    }
+    cons_to_prim<DIM>(Uc, g1d, qc);
+    for (int k = 0; k < a.ns; ++k) {
+        const int L = a.link[(size_t)k * n + c];
+        if (L == SLOT_UNUSED || !link_boundary(L)) continue;
+        const BocoDev bc = a.boco[a.g.bboco[link_index(L)]];
+        if (bc.kind == BK_INLET_SUP) continue;
+        const double* tk = t.slot + ((size_t)k * n + c) * NV;
+        double nv[DIM];
+        slot_normal<DIM>(a.g, k, n, c, nv);
+        Dual nd[DIM];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) nd[e] = Dual(nv[e]);
+        for (int j = 0; j < NV; ++j) {
+            Dual qd[NV], gd[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qd[v] = Dual(qc[v], v == j ? 1.0 : 0.0);
+            bc_state<DIM>(bc, nd, qd, gd, a.ph);
+            double sacc = 0.0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sacc += gd[v].d * tk[v];
+            qb[j] += sacc;
+        }
+    }
+    double gu[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) gu[v] = 0.0;
+    prim_rev<DIM>(Uc, g1d, qb, gu);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] += gu[v];
 }
 
 // viscous: each cell collects what its neighbours' gradients pushed back to it (t.slot at the
@@ -702,16 +743,29 @@ hipError_t launch_res_tangent(const AdArgs& t, hipStream_t s)
     return hipGetLastError();
 }
 
+// t.fgeo from the slot geometry: once per work buffer, before the viscous adjoint launches
+template <int DIM>
+hipError_t launch_adj_face_geom(const AdArgs& t, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_adj_face_geom<DIM>), dim3(grid_blocks(t.k.n)), dim3(BLOCK), 0, s, t);
+    return hipGetLastError();
+}
+
 template <int DIM>
 hipError_t launch_res_adjoint(const AdArgs& t, hipStream_t s)
 {
     const bool rev = t.k.ph.sch_hyp == 25 || t.k.ph.sch_hyp == 26;
-    const dim3 grid(grid_blocks(t.k.n)), fgrid((t.nf + BLOCK - 1) / BLOCK);
+    const dim3 grid(grid_blocks(t.k.n)), fgrid((t.nfi + BLOCK - 1) / BLOCK), bgrid((t.nb + BLOCK - 1) / BLOCK);
     if (t.k.ph.viscous) {
         if (!rev) return hipErrorInvalidValue;  // the host rejects this case first
-        hipLaunchKernelGGL((k_adj_face_geom<DIM>), grid, dim3(BLOCK), 0, s, t);
-        hipLaunchKernelGGL((k_res_adjoint_face<DIM, true>), fgrid, dim3(BLOCK), 0, s, t);
-        hipLaunchKernelGGL((k_res_adjoint_gather<DIM, true>), grid, dim3(BLOCK), 0, s, t);
+        if (t.nfi > 0) hipLaunchKernelGGL((k_res_adjoint_face<DIM, true, false>), fgrid, dim3(BLOCK), 0, s, t);
+        if (t.nb > 0) hipLaunchKernelGGL((k_res_adjoint_face<DIM, true, true>), bgrid, dim3(BLOCK), 0, s, t);
+        if (t.k.ns <= 4) hipLaunchKernelGGL((k_res_adjoint_gather<DIM, true, 4>), grid, dim3(BLOCK), 0, s, t);
+        else if (t.k.ns <= 6) hipLaunchKernelGGL((k_res_adjoint_gather<DIM, true, 6>), grid, dim3(BLOCK), 0, s, t);
+        else hipLaunchKernelGGL((k_res_adjoint_gather<DIM, true, MAX_SLOTS>), grid, dim3(BLOCK), 0, s, t);
+        if (t.nbcell > 0)
+            hipLaunchKernelGGL((k_res_adjoint_bnd_cells<DIM>), dim3((t.nbcell + BLOCK - 1) / BLOCK), dim3(BLOCK), 0,
+                               s, t);
         hipLaunchKernelGGL((k_res_adjoint_tail<DIM>), grid, dim3(BLOCK), 0, s, t);
         return hipGetLastError();
     }
@@ -719,8 +773,11 @@ hipError_t launch_res_adjoint(const AdArgs& t, hipStream_t s)
         hipLaunchKernelGGL((k_res_adjoint<DIM>), grid, dim3(BLOCK), 0, s, t);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL((k_res_adjoint_face<DIM, false>), fgrid, dim3(BLOCK), 0, s, t);
-    hipLaunchKernelGGL((k_res_adjoint_gather<DIM, false>), grid, dim3(BLOCK), 0, s, t);
+    if (t.nfi > 0) hipLaunchKernelGGL((k_res_adjoint_face<DIM, false, false>), fgrid, dim3(BLOCK), 0, s, t);
+    if (t.nb > 0) hipLaunchKernelGGL((k_res_adjoint_face<DIM, false, true>), bgrid, dim3(BLOCK), 0, s, t);
+    if (t.k.ns <= 4) hipLaunchKernelGGL((k_res_adjoint_gather<DIM, false, 4>), grid, dim3(BLOCK), 0, s, t);
+    else if (t.k.ns <= 6) hipLaunchKernelGGL((k_res_adjoint_gather<DIM, false, 6>), grid, dim3(BLOCK), 0, s, t);
+    else hipLaunchKernelGGL((k_res_adjoint_gather<DIM, false, MAX_SLOTS>), grid, dim3(BLOCK), 0, s, t);
     return hipGetLastError();
 }
 
@@ -742,6 +799,8 @@ hipError_t launch_comb(double* out, const Comb& cb, long len, hipStream_t s)
 
 template hipError_t launch_res_tangent<2>(const AdArgs&, hipStream_t);
 template hipError_t launch_res_tangent<3>(const AdArgs&, hipStream_t);
+template hipError_t launch_adj_face_geom<2>(const AdArgs&, hipStream_t);
+template hipError_t launch_adj_face_geom<3>(const AdArgs&, hipStream_t);
 template hipError_t launch_res_adjoint<2>(const AdArgs&, hipStream_t);
 template hipError_t launch_res_adjoint<3>(const AdArgs&, hipStream_t);
 

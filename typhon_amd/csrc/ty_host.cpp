@@ -81,6 +81,8 @@ struct ty_ctx {
     double* d_vol = nullptr;
     // FP64 mesh arrays the device geometry build reads (kept with the mesh)
     double *d_cc = nullptr, *d_fc = nullptr, *d_fn = nullptr, *d_fs = nullptr, *d_gc = nullptr, *d_ps = nullptr;
+    std::vector<int> bcell;     // cells with a physical boundary slot, ascending
+    int* d_bcell = nullptr;
     double* d_dtfac = nullptr;
     double* d_u0 = nullptr;
     void* d_us[2] = {nullptr, nullptr};
@@ -155,6 +157,7 @@ void free_state(ty_ctx* c)
     c->d_boco = nullptr;
     c->d_vol = c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
     c->d_cc = c->d_fc = c->d_fn = c->d_fs = c->d_gc = c->d_ps = nullptr;
+    c->d_bcell = nullptr;
     c->d_us[0] = c->d_us[1] = nullptr;
     c->d_grad = c->d_qf = nullptr;
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
@@ -568,6 +571,9 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         for (int d = 0; d < 3; ++d) c->cc[(size_t)d * n + i] = m->centre[(size_t)3 * o + d];
         c->vol[i] = m->volume[o];
     });
+    c->bcell.clear();
+    for (int k = 0; k < c->nb; ++k) c->bcell.push_back(bfaces[k].a);  // bfaces are sorted by cell
+    c->bcell.erase(std::unique(c->bcell.begin(), c->bcell.end()), c->bcell.end());
     phase("mesh: slot words, cell data");
     int rc;
     if ((rc = upload(c, &c->d_nbr, c->nbr)) || (rc = upload(c, &c->d_fw, c->fw)) ||
@@ -575,7 +581,8 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         (rc = upload(c, &c->d_link, c->link)) ||
         (rc = upload(c, &c->d_fcell, c->fcell)) || (rc = upload(c, &c->d_cc, c->cc)) ||
         (rc = upload(c, &c->d_fc, c->fc)) || (rc = upload(c, &c->d_fn, c->fn)) || (rc = upload(c, &c->d_fs, c->fs)) ||
-        (rc = upload(c, &c->d_gc, c->gc)) || (rc = upload(c, &c->d_ps, c->ps)))
+        (rc = upload(c, &c->d_gc, c->gc)) || (rc = upload(c, &c->d_ps, c->ps)) ||
+        (rc = upload(c, &c->d_bcell, c->bcell)))
         return rc;
     phase("mesh: uploads");
     c->mesh_bufs = c->owned;  // everything allocated so far belongs to the mesh
@@ -1584,6 +1591,8 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
     HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
     memset(&t, 0, sizeof(t));
     t.k = make_args<double>(c, c->geo_d);
+    t.bcell = c->d_bcell;
+    t.nbcell = (int)c->bcell.size();
     t.k.us = c->d_u0;
     t.nf = c->nf;
     t.nb = c->nb;
@@ -1697,6 +1706,7 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
         if ((rc = tmp.alloc(&d_fg, (size_t)6 * nf)) || (rc = tmp.alloc(&d_sl, (size_t)c->ns * n * nv))) return rc;
         t.fgeo = d_fg;
         t.slot = d_sl;
+        HIPCHK(c, c->dim == 2 ? launch_adj_face_geom<2>(t, c->stream) : launch_adj_face_geom<3>(t, c->stream));
     }
     t.lam = d_lam;
     t.gu = d_gu;
@@ -1951,6 +1961,7 @@ int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t sn
         if ((rc = tmp.alloc(&d_fg, (size_t)6 * nf)) || (rc = tmp.alloc(&d_sl, (size_t)c->ns * n * nv))) return rc;
         u.t.fgeo = d_fg;
         u.t.slot = d_sl;
+        HIPCHK(c, c->dim == 2 ? launch_adj_face_geom<2>(u.t, c->stream) : launch_adj_face_geom<3>(u.t, c->stream));
     }
     u.t.gs = nullptr;
     u.t.gvol = nullptr;
