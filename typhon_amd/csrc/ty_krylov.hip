@@ -1,8 +1,8 @@
 // Vector kernels of the matrix-free implicit solve (restarted GMRES with
 // classical Gram-Schmidt applied twice, block-Jacobi preconditioning, FD
 // Jacobian-vector products).  Vectors are [nv][n] arrays of length len = nv*n.
-// Reductions are two-pass (per-block partials in a fixed order, then one block),
-// so every dot product is bitwise repeatable.
+// Reductions are two-pass (per-block partials in a fixed order, then one block per
+// output), so every dot product is bitwise repeatable.
 #include "ty_krylov.h"
 
 namespace ty {
@@ -57,39 +57,60 @@ __global__ void k_block_apply(const T* __restrict__ dinv, const T* __restrict__ 
     }
 }
 
-// partial[b][i] = sum over block b of w . V_i, i < m (m <= MAXK)
+// partial[b][i] = sum over block b of w . V_i, i < m (m <= MAXK): vectors in groups of MG held
+// in registers (compile-time indexed), each group one grid-stride sweep that re-reads w
+constexpr int MG = 8;
 template <typename T>
 __global__ __launch_bounds__(KB) void k_multidot(const T* __restrict__ w, const T* __restrict__ V, long stride,
                                                  int m, long len, double* __restrict__ partial)
 {
-    __shared__ double red[KB / 64][MAXK];
-    double acc[MAXK];
-    for (int i = 0; i < m; ++i) acc[i] = 0.0;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long)gridDim.x * blockDim.x) {
-        const double we = (double)w[e];
-        for (int i = 0; i < m; ++i) acc[i] += we * (double)V[(long)i * stride + e];
-    }
+    __shared__ double red[KB / 64][MG];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int i = 0; i < m; ++i) {
-        const double s = wsum(acc[i]);
-        if (lane == 0) red[wid][i] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < m) {
-        double s = 0.0;
-        for (int k = 0; k < KB / 64; ++k) s += red[k][threadIdx.x];
-        partial[(long)blockIdx.x * MAXK + threadIdx.x] = s;
+    for (int g0 = 0; g0 < m; g0 += MG) {
+        const int mg = m - g0 < MG ? m - g0 : MG;
+        const T* Vg = V + (long)g0 * stride;
+        double acc[MG];
+#pragma unroll
+        for (int i = 0; i < MG; ++i) acc[i] = 0.0;
+        for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long)gridDim.x * blockDim.x) {
+            const double we = (double)w[e];
+#pragma unroll
+            for (int i = 0; i < MG; ++i)
+                if (i < mg) acc[i] += we * (double)Vg[(long)i * stride + e];
+        }
+#pragma unroll
+        for (int i = 0; i < MG; ++i) {
+            const double sv = wsum(acc[i]);
+            if (lane == 0) red[wid][i] = sv;
+        }
+        __syncthreads();
+        if (threadIdx.x < mg) {
+            double sv = 0.0;
+            for (int k = 0; k < KB / 64; ++k) sv += red[k][threadIdx.x];
+            partial[(long)blockIdx.x * MAXK + g0 + threadIdx.x] = sv;
+        }
+        __syncthreads();
     }
 }
 
-__global__ void k_reduce_partials(const double* __restrict__ partial, int nblk, int m, double* __restrict__ out)
+// out[i] = sum over blocks of partial[b][i]: one block per output, strided per-thread sums, a
+// butterfly within each wave and the four wave sums in order (fixed summation order)
+__global__ __launch_bounds__(KB) void k_reduce_partials(const double* __restrict__ partial, int nblk, int m,
+                                                        double* __restrict__ out)
 {
-    // one thread per output, fixed summation order
-    const int i = threadIdx.x;
+    __shared__ double red[KB / 64];
+    const int i = blockIdx.x;
     if (i >= m) return;
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += partial[(long)b * MAXK + i];
-    out[i] = s;
+    double sv = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += KB) sv += partial[(long)b * MAXK + i];
+    sv = wsum(sv);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sv;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int k = 1; k < KB / 64; ++k) t += red[k];
+        out[i] = t;
+    }
 }
 
 // w -= sum_i h_i V_i
@@ -173,7 +194,7 @@ hipError_t multidot(const T* w, const T* V, long stride, int m, long len, double
 {
     const int nb = multidot_blocks(len);
     hipLaunchKernelGGL((k_multidot<T>), dim3(nb), dim3(KB), 0, s, w, V, stride, m, len, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, (const double*)partial, nb, m, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(m), dim3(KB), 0, s, (const double*)partial, nb, m, out);
     return hipGetLastError();
 }
 template <typename T>
