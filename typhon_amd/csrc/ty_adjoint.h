@@ -10,9 +10,11 @@ struct AdArgs {
     int nf, nb;           // device faces, device boundary faces
     const int* fcell;     // [2][nf] owner and other cell of each device face (other = -1 on boundary faces)
     int nfi;              // two-sided faces come first: device faces [0, nfi)
-    double* face;         // [nf][2][RW] work: per face side, that cell's state adjoint, (viscous) the
-                          // adjoint of its primitive-variable gradient, and the side's share of its
-                          // volume sensitivity; RW = nv (+ nv dim) + 1
+    double* face;         // [ns][n][RW] work: per face side, at the slot where that side's cell keeps
+                          // the face, the cell's state adjoint, (viscous) the adjoint of its
+                          // primitive-variable gradient, and the side's share of its volume
+                          // sensitivity; RW = nv (+ nv dim) + 1
+    const unsigned char* fslot;  // [2][nf] slot of each device face in its owner's / other cell's list
     // viscous residuals (k.grad holds the cell gradients of the state)
     const double* fgeo;   // [6][nf] work: x_f - x_owner and the canonical centre difference d per face
     double* slot;         // [ns][n][nv] work: per slot, the far cell's primitive adjoint through this

@@ -476,7 +476,9 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
         cons_to_prim<DIM>(UR, g1d, qR);
     }
     hllc_rev<DIM>(a.ph.sch_hyp, a.ph.gamma, qL, qR, nv, mv, Fv, qLb, qRb, gn);
-    double* out = t.face + (size_t)f * 2 * RW;
+    // side records at the slot each side keeps the face in: [slot][cell][RW]
+    double* out = t.face + ((size_t)t.fslot[f] * n + cl) * RW;
+    double* oto = bnd ? out : t.face + ((size_t)t.fslot[(size_t)t.nf + f] * n + cr) * RW;
     double gLb[VISC ? NG : 1], gRb[VISC ? NG : 1];
     if (VISC) {
         double gL[VISC ? NG : 1], gR[VISC ? NG : 1], rL[DIM], rR[DIM], dk[DIM];
@@ -547,12 +549,12 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
         }
         prim_rev<DIM>(UR, g1d, qRb, gR);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) out[RW + v] = gR[v];
+        for (int v = 0; v < NV; ++v) oto[v] = gR[v];
         if (VISC) {
 #pragma unroll
-            for (int q = 0; q < NG; ++q) out[RW + NV + q] = gRb[q];
+            for (int q = 0; q < NG; ++q) oto[NV + q] = gRb[q];
         }
-        out[2 * RW - 1] = A * kr * sR;
+        oto[RW - 1] = A * kr * sR;
     }
     double gL[NV];
 #pragma unroll
@@ -593,8 +595,7 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
         if (k >= a.ns) break;
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED) continue;
-        const SlotInfo s = decode(w, 0);
-        const double* in = t.face + ((size_t)s.f * 2 + (s.other ? 1 : 0)) * RW;
+        const double* in = t.face + ((size_t)k * n + c) * RW;
 #pragma unroll
         for (int v = 0; v < NV; ++v) gu[v] += in[v];
         if (VISC) {

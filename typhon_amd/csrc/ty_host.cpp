@@ -83,6 +83,8 @@ struct ty_ctx {
     double *d_cc = nullptr, *d_fc = nullptr, *d_fn = nullptr, *d_fs = nullptr, *d_gc = nullptr, *d_ps = nullptr;
     std::vector<int> bcell;     // cells with a physical boundary slot, ascending
     int* d_bcell = nullptr;
+    std::vector<unsigned char> fslot;  // [2][nf] slot of each device face in its owner's / other cell's list
+    unsigned char* d_fslot = nullptr;
     double* d_dtfac = nullptr;
     double* d_u0 = nullptr;
     void* d_us[2] = {nullptr, nullptr};
@@ -158,6 +160,7 @@ void free_state(ty_ctx* c)
     c->d_vol = c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
     c->d_cc = c->d_fc = c->d_fn = c->d_fs = c->d_gc = c->d_ps = nullptr;
     c->d_bcell = nullptr;
+    c->d_fslot = nullptr;
     c->d_us[0] = c->d_us[1] = nullptr;
     c->d_grad = c->d_qf = nullptr;
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
@@ -536,11 +539,13 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
     c->link.assign((size_t)ns * n, SLOT_UNUSED);
     if (n >= (1 << 27) || c->nb >= (1 << 27)) return fail(c, TY_ERR_UNSUPPORTED, "too many cells for the slot link word");
     c->sumA.assign(n, 0.0);
-    // mirror slots: where the far cell keeps the same face (each face side is written once)
-    std::vector<unsigned char> slot_own(c->nfi, 0), slot_oth(c->nfi, 0);
+    // the slot where each side keeps each device face (each face side is written once): the
+    // mirror slots of the slot words and the adjoint's per-side record positions
+    c->fslot.assign((size_t)2 * ndf, 0);
+    unsigned char* slot_own = c->fslot.data();
+    unsigned char* slot_oth = c->fslot.data() + ndf;
     parallel_for(n, [&](int i) {
         for (int k = 0; k < sbeg[i + 1] - sbeg[i]; ++k) {
-            if (snb[sbeg[i] + k] < 0) continue;
             const unsigned w = (unsigned)sword[sbeg[i] + k];
             (w & 0x80000000u ? slot_oth : slot_own)[w & FACE_MASK] = (unsigned char)k;
         }
@@ -582,7 +587,7 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         (rc = upload(c, &c->d_fcell, c->fcell)) || (rc = upload(c, &c->d_cc, c->cc)) ||
         (rc = upload(c, &c->d_fc, c->fc)) || (rc = upload(c, &c->d_fn, c->fn)) || (rc = upload(c, &c->d_fs, c->fs)) ||
         (rc = upload(c, &c->d_gc, c->gc)) || (rc = upload(c, &c->d_ps, c->ps)) ||
-        (rc = upload(c, &c->d_bcell, c->bcell)))
+        (rc = upload(c, &c->d_bcell, c->bcell)) || (rc = upload(c, &c->d_fslot, c->fslot)))
         return rc;
     phase("mesh: uploads");
     c->mesh_bufs = c->owned;  // everything allocated so far belongs to the mesh
@@ -1593,6 +1598,7 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
     t.k = make_args<double>(c, c->geo_d);
     t.bcell = c->d_bcell;
     t.nbcell = (int)c->bcell.size();
+    t.fslot = c->d_fslot;
     t.k.us = c->d_u0;
     t.nf = c->nf;
     t.nb = c->nb;
@@ -1699,7 +1705,7 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
     if (gq && (rc = tmp.alloc(&d_gq, (size_t)nv * std::max(1, nb)))) return rc;
     const int rw = nv + (c->ph.viscous ? nv * c->dim : 0) + 1;
     double* d_face = nullptr;  // per face side: state (and gradient) adjoint, volume-sensitivity share
-    if ((rc = tmp.alloc(&d_face, (size_t)2 * rw * nf))) return rc;
+    if ((rc = tmp.alloc(&d_face, (size_t)c->ns * n * rw))) return rc;
     t.face = d_face;
     if (c->ph.viscous) {
         double *d_fg = nullptr, *d_sl = nullptr;
@@ -1954,7 +1960,7 @@ int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t sn
     // residual-adjoint work buffers (as ty_residual_adjoint)
     const int rw = nv + (c->ph.viscous ? nv * c->dim : 0) + 1;
     double* d_face = nullptr;
-    if ((rc = tmp.alloc(&d_face, (size_t)2 * rw * nf))) return rc;
+    if ((rc = tmp.alloc(&d_face, (size_t)c->ns * n * rw))) return rc;
     u.t.face = d_face;
     if (c->ph.viscous) {
         double *d_fg = nullptr, *d_sl = nullptr;
