@@ -196,6 +196,19 @@ interface
     type(ty_krylov_stats) :: stats
   endfunction
 
+  ! the reference's assembled implicit step (build_implicit_bdlu + bdlu_gmres), deftime%implicite
+  ! nkrylov / maxres / max_it; info = GMRES iterations - 1, or -1 when it did not converge
+  integer(c_int) function ty_implicit_bdlu_step(ctx, dt, nkrylov, maxres, max_it, info) &
+                 bind(C, name="ty_implicit_bdlu_step")
+    import :: c_int, c_int32_t, c_ptr, c_double
+    type(c_ptr), value        :: ctx
+    real(c_double), value     :: dt
+    integer(c_int32_t), value :: nkrylov
+    real(c_double), value     :: maxres
+    integer(c_int32_t), value :: max_it
+    integer(c_int32_t)        :: info
+  endfunction
+
   ! tangent-linear / adjoint of the first-order residual; pass c_null_ptr for absent arrays
   integer(c_int) function ty_residual_tangent(ctx, dt, du, ds, dvol, dq, dres) bind(C, name="ty_residual_tangent")
     import :: c_int, c_ptr, c_double

@@ -179,6 +179,18 @@ int ty_linear_solve(ty_ctx* ctx, double dt, const ty_krylov* kry, const double* 
  * tstep_implicit convention) */
 int ty_implicit_step(ty_ctx* ctx, double dt, const ty_krylov* kry, ty_krylov_stats* stats);
 
+/* One implicit step of the reference's assembled path (METHOD=IMPLICIT with INVERSION=GMRES):
+ * calc_rhs with HLL face Jacobians (calc_jac_hll.f90:46-57, viscous diagonal
+ * calc_flux_viscous_cell.f90:149-158, ns_bocojacobian.f90:44-92), the block matrix of
+ * build_implicit_bdlu.f90:44-83 over the interior cells and one ghost unknown per boundary face
+ * side (ghost rows: identity, right-hand side = the boundary-flux sum flux_to_res leaves there),
+ * bdlu_gmres.f90:55-212 (restarted GMRES(nkrylov), modified Gram-Schmidt, no preconditioner,
+ * x0 = rhs/diag, restarts while ||r||/||rhs|| >= maxres ||x0||_1 and nit <= 2 max_it; maxres =
+ * INV_RES, max_it = IMP_MAX_IT, def_time.f90:204-206), then U += solution on interior cells
+ * (tstep_implicit.f90 -> implicit_solve.f90:39-84, update_field.f90:38).  FP64 throughout.
+ * info: the GMRES info (iterations - 1, or -1 when it did not converge). */
+int ty_implicit_bdlu_step(ty_ctx* ctx, double dt, int32_t nkrylov, double maxres, int32_t max_it, int32_t* info);
+
 /* Tangent-linear residual (no reference counterpart: the reference has no derivative code).
  * Exact forward-mode derivative of the first-order residual R = -dt/V sum_f F.A
  * (calc_rhs.f90:10, flux_to_res.f90:47-142, first-order face states distrib_field.f90) at the

@@ -67,6 +67,13 @@ def _load():
     lib.or_viscosity.argtypes = [C.POINTER(OrConfig), C.c_double, C.c_double]
     lib.or_jac_hll.argtypes = [C.POINTER(OrConfig), F64P, F64P, F64P, F64P, F64P]
     lib.or_jac_diag.argtypes = [C.c_void_p, C.c_double, F64P]
+    lib.or_bdlu_assemble.argtypes = [C.c_void_p, C.c_double, F64P, F64P, F64P]
+    lib.or_bdlu_gmres.restype = C.c_int
+    lib.or_bdlu_gmres.argtypes = [C.c_void_p, F64P, F64P, F64P, F64P, C.c_int, C.c_double, C.c_int]
+    lib.or_implicit_bdlu_step.restype = C.c_int
+    lib.or_implicit_bdlu_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int]
+    lib.or_res_all.restype = F64P
+    lib.or_res_all.argtypes = [C.c_void_p]
     lib.or_meshgeom.restype = C.c_int
     lib.or_meshgeom.argtypes = [C.c_int, C.c_int, F64P, C.c_int, I32P, I32P, I32P, C.c_int,
                                 F64P, F64P, F64P, F64P, F64P]
@@ -207,6 +214,30 @@ class Oracle:
         out = np.empty((self.mesh.ncell_int, 5, 5))
         self.lib.or_jac_diag(self.h, dt, _p64(out))
         return out
+
+    # A15: the reference's assembled implicit path
+    def bdlu_assemble(self, dt):
+        """diag [ncell, 25], upper/lower [nface, 25] (row-major 5x5 blocks) after calc_rhs(dt)"""
+        m = self.mesh
+        d = np.empty((m.ncell, 25))
+        up = np.empty((m.nface, 25))
+        lo = np.empty((m.nface, 25))
+        self.lib.or_bdlu_assemble(self.h, dt, _p64(d), _p64(up), _p64(lo))
+        return d, up, lo
+
+    def bdlu_gmres(self, diag, upper, lower, rhs, nkrylov=50, maxres=1e-4, max_it=50):
+        """bdlu_gmres on rhs [ncell, 5] (block-major); returns (solution [ncell, 5], info)"""
+        sol = np.ascontiguousarray(rhs, dtype=np.float64).copy()
+        info = self.lib.or_bdlu_gmres(self.h, _p64(np.ascontiguousarray(diag)), _p64(np.ascontiguousarray(upper)),
+                                      _p64(np.ascontiguousarray(lower)), _p64(sol), nkrylov, maxres, max_it)
+        return sol, info
+
+    def implicit_bdlu_step(self, dt, nkrylov=50, maxres=1e-4, max_it=50):
+        return self.lib.or_implicit_bdlu_step(self.h, dt, nkrylov, maxres, max_it)
+
+    def res_all(self):
+        """residual of every cell after calc_rhs, ghost rows included [5, ncell]"""
+        return self.view("or_res_all", 5, self.mesh.ncell)
 
 
 def flux_face(qL, qR, n, sch_hyp=25, gamma=1.4):

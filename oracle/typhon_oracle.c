@@ -14,6 +14,7 @@
  * ratios of postlimit_barth.
  */
 #include "typhon_oracle.h"
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -97,6 +98,7 @@ const double* or_grad(const or_ctx* o) { return o->grad; }
 const double* or_face_left(const or_ctx* o) { return o->qL; }
 const double* or_face_right(const or_ctx* o) { return o->qR; }
 const double* or_flux(const or_ctx* o) { return o->flux; }
+const double* or_res_all(const or_ctx* o) { return o->res; }
 
 /* ---------------------------------------------------------------------------
  * A2: calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:37-57), all cells */
@@ -1195,4 +1197,220 @@ void or_jac_diag(or_ctx* o, double dt, double* diag)
             for (int k = 0; k < 25; ++k) diag[(size_t)25 * c2 + k] -= (s * jR[k]) * k2;
         }
     }
+}
+
+/* ---------------------------------------------------------------------------
+ * A15: the assembled implicit path.  Face Jacobians at the face states (calc_jac_hll.f90:46-57,
+ * calc_jacflux.f90:49-77), the viscous diagonal (calc_flux_viscous_cell.f90:149-158), the boundary
+ * Jacobians of ns_bocojacobian.f90:44-92 (SYMMETRY: jL += jR dQR/dQL with the mirror on the
+ * momentum block, jR = 0; SUPERSONIC_OUTLET: jL += jR, jR = 0; others unchanged), both times the
+ * face area (flux_to_res.f90:75-81), assembled as build_implicit_bdlu.f90:44-83 over every cell
+ * including ghosts (init_implicit_bdlu.f90:37-49: interior diagonals start at I; a ghost row is I). */
+void or_bdlu_assemble(or_ctx* o, double dt, double* diag, double* upper, double* lower)
+{
+    const or_config* c = &o->c;
+    for (int i = 0; i < o->ni; ++i)
+        for (int k = 0; k < 25; ++k) diag[(size_t)25 * i + k] = (k % 6 == 0) ? 1.0 : 0.0;
+    for (int f = 0; f < o->nf; ++f) {
+        double ql[NV], qr[NV], jL[25], jR[25];
+        for (int v = 0; v < NV; ++v) { ql[v] = QL(o, v, f); qr[v] = QR(o, v, f); }
+        const double* n = c->fnorm + 3 * (size_t)f;
+        or_jac_hll(c, ql, qr, n, jL, jR);
+        int cl = c->ic1[f], cr = c->ic2[f];
+        if (c->viscous) {
+            const double* xl = c->ccent + 3 * (size_t)cl;
+            const double* xr = c->ccent + 3 * (size_t)cr;
+            const double* xf = c->fcent + 3 * (size_t)f;
+            double a[3] = {xf[0] - xl[0], xf[1] - xl[1], xf[2] - xl[2]};
+            double b[3] = {xf[0] - xr[0], xf[1] - xr[1], xf[2] - xr[2]};
+            double dHL = sqrt(dot3(a, a)), dHR = sqrt(dot3(b, b));
+            double id0 = 1.0 / (dHL + dHR);
+            dHL = id0 * dHL; dHR = id0 * dHR;
+            double rL = PRIM(o, 0, cl), rR = PRIM(o, 0, cr);
+            double TL = PRIM(o, 1, cl) / (rL * c->rgas), TR = PRIM(o, 1, cr) / (rR * c->rgas);
+            double TH = dHR * TL + dHL * TR, rhoH = dHR * rL + dHL * rR;
+            double mu = or_viscosity(c, rhoH, TH);
+            double d[3] = {xr[0] - xl[0], xr[1] - xl[1], xr[2] - xl[2]};
+            double id = mu / (dHR * rL + dHL * rR) / sqrt(dot3(d, d));
+            for (int i = 0; i < 5; ++i) { jR[i * 5 + i] = jR[i * 5 + i] - id; jL[i * 5 + i] = jL[i * 5 + i] + id; }
+        }
+        if (f >= o->nfi) {
+            int bt = -1;
+            for (int ib = 0; ib < c->nboco && bt < 0; ++ib)
+                for (int k = 0; k < c->boco[ib].nface; ++k)
+                    if (c->boco[ib].iface[k] == f) { bt = c->boco[ib].type; break; }
+            if (bt == OR_BC_SYM || bt == OR_BC_OUTLET_SUP) {
+                double S[25];
+                for (int k = 0; k < 25; ++k) S[k] = (k % 6 == 0) ? 1.0 : 0.0;
+                if (bt == OR_BC_SYM)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) S[(2 + i) * 5 + 2 + j] = S[(2 + i) * 5 + 2 + j] - 2.0 * (n[i] * n[j]);
+                for (int i = 0; i < 5; ++i)
+                    for (int j = 0; j < 5; ++j) {
+                        double s = 0.0;
+                        for (int k = 0; k < 5; ++k) s += jR[i * 5 + k] * S[k * 5 + j];
+                        jL[i * 5 + j] = jL[i * 5 + j] + s;
+                    }
+                for (int k = 0; k < 25; ++k) jR[k] = 0.0;
+            }
+        }
+        double s = c->fsurf[f];
+        for (int k = 0; k < 25; ++k) { jL[k] = s * jL[k]; jR[k] = s * jR[k]; }
+        for (int k = 0; k < 25; ++k) {
+            diag[(size_t)25 * cl + k] = diag[(size_t)25 * cl + k] + jL[k] * dt / c->cvol[cl];
+            upper[(size_t)25 * f + k] = jR[k] * dt / c->cvol[cl];
+        }
+        if (f < o->nfi) {
+            for (int k = 0; k < 25; ++k) {
+                diag[(size_t)25 * cr + k] = diag[(size_t)25 * cr + k] - jR[k] * dt / c->cvol[cr];
+                lower[(size_t)25 * f + k] = -jL[k] * dt / c->cvol[cr];
+            }
+        } else {
+            for (int k = 0; k < 25; ++k) {
+                diag[(size_t)25 * cr + k] = (k % 6 == 0) ? 1.0 : 0.0;
+                lower[(size_t)25 * f + k] = 0.0;
+            }
+        }
+    }
+}
+
+/* bdlu_yeqax (SPMAT_BDLU.f90:139-163): y = A x, block-major vectors [ncell][5] */
+static void bdlu_yeqax(const or_ctx* o, const double* diag, const double* upper, const double* lower,
+                       const double* x, double* y)
+{
+    for (int i = 0; i < o->nc; ++i)
+        for (int r = 0; r < 5; ++r) {
+            double s = 0.0;
+            for (int k = 0; k < 5; ++k) s += diag[(size_t)25 * i + r * 5 + k] * x[(size_t)5 * i + k];
+            y[(size_t)5 * i + r] = s;
+        }
+    for (int f = 0; f < o->nf; ++f) {
+        int i1 = o->c.ic1[f], i2 = o->c.ic2[f];
+        for (int r = 0; r < 5; ++r) {
+            double s = 0.0;
+            for (int k = 0; k < 5; ++k) s += lower[(size_t)25 * f + r * 5 + k] * x[(size_t)5 * i1 + k];
+            y[(size_t)5 * i2 + r] = y[(size_t)5 * i2 + r] + s;
+        }
+        for (int r = 0; r < 5; ++r) {
+            double s = 0.0;
+            for (int k = 0; k < 5; ++k) s += upper[(size_t)25 * f + r * 5 + k] * x[(size_t)5 * i2 + k];
+            y[(size_t)5 * i1 + r] = y[(size_t)5 * i1 + r] + s;
+        }
+    }
+}
+
+static double vdot(const double* a, const double* b, size_t n)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+/* bdlu_gmres (SOURCE/MATH/bdlu_gmres.f90:55-212): restarted GMRES(nkrylov), modified Gram-Schmidt,
+   Givens rotations, no preconditioner; x0 = rhs / diag(ib,ib); restarts while
+   ||rhs - A x|| / ||rhs|| >= ||x0||_1 maxres (the reference's mixed scales) and nit <= 2 max_it.
+   sol holds the right-hand side on entry and the solution on exit; returns info (nit - 1, or -1). */
+int or_bdlu_gmres(const or_ctx* o, const double* diag, const double* upper, const double* lower, double* sol,
+                  int nkrylov, double maxres, int max_it)
+{
+    const size_t dim = (size_t)5 * o->nc;
+    const int m = nkrylov;
+    double* p1 = (double*)malloc(sizeof(double) * dim);
+    double* r1 = (double*)malloc(sizeof(double) * dim);
+    double* v1 = (double*)malloc(sizeof(double) * dim * (size_t)m);
+    double* w1 = (double*)malloc(sizeof(double) * dim);
+    double* hh = (double*)calloc((size_t)(m + 1) * m, sizeof(double)); /* hh(i,j) = hh[(j-1)*(m+1) + i-1] */
+    double* ss = (double*)calloc((size_t)m + 1, sizeof(double));
+    double* qc = (double*)calloc((size_t)m, sizeof(double));
+    double* qs = (double*)calloc((size_t)m, sizeof(double));
+    double* yy = (double*)calloc((size_t)m, sizeof(double));
+#define HH(i, j) hh[(size_t)((j) - 1) * (m + 1) + (i) - 1]
+    memcpy(p1, sol, sizeof(double) * dim);
+    for (int i = 0; i < o->nc; ++i)
+        for (int ib = 0; ib < 5; ++ib) sol[(size_t)5 * i + ib] = p1[(size_t)5 * i + ib] / diag[(size_t)25 * i + ib * 6];
+    double ref = 0.0;
+    for (size_t i = 0; i < dim; ++i) ref += fabs(sol[i]);
+    const double normp1 = sqrt(vdot(p1, p1, dim));
+    double errgmres = DBL_MAX;
+    int nit = 0;
+    while (errgmres >= ref * maxres && nit <= max_it * 2) {
+        bdlu_yeqax(o, diag, upper, lower, sol, r1);
+        for (size_t i = 0; i < dim; ++i) r1[i] = p1[i] - r1[i];
+        const double beta = sqrt(vdot(r1, r1, dim));
+        double fact = 1.0 / beta;
+        for (size_t i = 0; i < dim; ++i) v1[i] = fact * r1[i];
+        for (int i = 0; i <= m; ++i) ss[i] = 0.0;
+        ss[0] = beta;
+        for (size_t i = 0; i < (size_t)(m + 1) * m; ++i) hh[i] = 0.0;
+        int jk;
+        for (jk = 1; jk <= m; ++jk) {
+            bdlu_yeqax(o, diag, upper, lower, v1 + (size_t)(jk - 1) * dim, w1);
+            for (int ik = 1; ik <= jk; ++ik) {
+                const double* vi = v1 + (size_t)(ik - 1) * dim;
+                HH(ik, jk) = vdot(w1, vi, dim);
+                for (size_t i = 0; i < dim; ++i) w1[i] = w1[i] - HH(ik, jk) * vi[i];
+            }
+            HH(jk + 1, jk) = sqrt(vdot(w1, w1, dim));
+            for (int ik = 1; ik <= jk - 1; ++ik) {
+                double tmp = HH(ik, jk);
+                HH(ik, jk) = qc[ik - 1] * tmp + qs[ik - 1] * HH(ik + 1, jk);
+                HH(ik + 1, jk) = -qs[ik - 1] * tmp + qc[ik - 1] * HH(ik + 1, jk);
+            }
+            if (HH(jk + 1, jk) > DBL_EPSILON * 10 * normp1) {
+                if (jk < m) {
+                    fact = 1.0 / HH(jk + 1, jk);
+                    double* vn = v1 + (size_t)jk * dim;
+                    for (size_t i = 0; i < dim; ++i) vn[i] = fact * w1[i];
+                }
+                if (HH(jk, jk) == 0) break;
+                double tmp = sqrt(HH(jk, jk) * HH(jk, jk) + HH(jk + 1, jk) * HH(jk + 1, jk));
+                qc[jk - 1] = HH(jk, jk) / tmp;
+                qs[jk - 1] = HH(jk + 1, jk) / tmp;
+                HH(jk, jk) = tmp;
+                HH(jk + 1, jk) = 0.0;
+                ss[jk] = -qs[jk - 1] * ss[jk - 1];
+                ss[jk - 1] = qc[jk - 1] * ss[jk - 1];
+                if (fabs(ss[jk]) < maxres * fabs(ss[0])) break;
+            } else {
+                break;
+            }
+        }
+        const int nk = jk < m ? jk : m;
+        for (int j = nk; j >= 1; --j) {
+            double tmp = ss[j - 1];
+            for (int ik = j + 1; ik <= nk; ++ik) tmp = tmp - HH(j, ik) * yy[ik - 1];
+            yy[j - 1] = tmp / HH(j, j);
+            const double* vj = v1 + (size_t)(j - 1) * dim;
+            for (size_t i = 0; i < dim; ++i) sol[i] = sol[i] + yy[j - 1] * vj[i];
+        }
+        bdlu_yeqax(o, diag, upper, lower, sol, r1);
+        for (size_t i = 0; i < dim; ++i) r1[i] = p1[i] - r1[i];
+        ss[nk] = sqrt(vdot(r1, r1, dim));
+        errgmres = ss[nk] / normp1;
+        nit = nit + 1;
+    }
+#undef HH
+    free(p1); free(r1); free(v1); free(w1); free(hh); free(ss); free(qc); free(qs); free(yy);
+    return nit <= max_it ? nit - 1 : -1;
+}
+
+/* one implicit time step of the reference (calc_rhs with Jacobians, tstep_implicit.f90 ->
+   implicit_solve.f90:39-84 with the GMRES method, update_field.f90:38): the residual of every
+   cell including the ghost rows (their boundary-flux sums, flux_to_res.f90:90-113), the assembled
+   matrix, bdlu_gmres, U += solution on interior cells.  Returns the GMRES info. */
+int or_implicit_bdlu_step(or_ctx* o, double dt, int nkrylov, double maxres, int max_it)
+{
+    calc_rhs_internal(o, dt);
+    double* diag = (double*)malloc(sizeof(double) * 25 * (size_t)o->nc);
+    double* upper = (double*)malloc(sizeof(double) * 25 * (size_t)o->nf);
+    double* lower = (double*)malloc(sizeof(double) * 25 * (size_t)o->nf);
+    double* sol = (double*)malloc(sizeof(double) * 5 * (size_t)o->nc);
+    or_bdlu_assemble(o, dt, diag, upper, lower);
+    for (int i = 0; i < o->nc; ++i)
+        for (int v = 0; v < NV; ++v) sol[(size_t)5 * i + v] = RES(o, v, i);
+    int info = or_bdlu_gmres(o, diag, upper, lower, sol, nkrylov, maxres, max_it);
+    for (int i = 0; i < o->ni; ++i)
+        for (int v = 0; v < NV; ++v) CONS(o, v, i) = CONS(o, v, i) + sol[(size_t)5 * i + v];
+    free(diag); free(upper); free(lower); free(sol);
+    return info;
 }

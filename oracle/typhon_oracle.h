@@ -142,6 +142,17 @@ double or_viscosity(const or_config* cfg, double rho, double T); /* EQNS.f90:225
 void or_jac_hll(const or_config* cfg, const double* qL, const double* qR, const double* n, double* jL, double* jR);
 void or_jac_diag(or_ctx* o, double dt, double* diag);
 
+/* A15, the reference's assembled implicit path (after or_calc_rhs at the same state and dt):
+   block matrix over every cell incl. ghosts, diag [ncell][25], upper/lower [nface][25] row-major;
+   its restarted GMRES (rhs in / solution out, block-major [ncell][5]; returns info, -1 if not
+   converged); and one implicit step (U += solution on interior cells; returns the GMRES info). */
+void or_bdlu_assemble(or_ctx* o, double dt, double* diag, double* upper, double* lower);
+int or_bdlu_gmres(const or_ctx* o, const double* diag, const double* upper, const double* lower, double* sol,
+                  int nkrylov, double maxres, int max_it);
+int or_implicit_bdlu_step(or_ctx* o, double dt, int nkrylov, double maxres, int max_it);
+/* residual of every cell after or_calc_rhs, ghost rows included ([5][ncell]) */
+const double* or_res_all(const or_ctx* o);
+
 /* MESHGEOM restatement (A1): geometry of faces given by vertex lists.
    fvtx [nface][4] 0-based vertex ids (-1 padded), ic1/ic2 in/out (swapped so ic1<ic2 when ic2>=0,
    ic2 = -1 on boundary faces), outputs face normal/centre/surface and cell centre/volume

@@ -16,7 +16,7 @@ F64P = C.POINTER(C.c_double)
 EXPORTS = ["ty_create", "ty_destroy", "ty_last_error", "ty_mesh_upload", "ty_set_scheme", "ty_state_set",
            "ty_state_get", "ty_timestep", "ty_residual", "ty_rk_step", "ty_advance", "ty_sync", "ty_timing",
            "ty_mesh_info", "ty_jacobian_diag", "ty_jvp", "ty_linear_solve", "ty_implicit_step",
-           "ty_residual_tangent", "ty_residual_adjoint", "ty_unsteady_adjoint"]
+           "ty_implicit_bdlu_step", "ty_residual_tangent", "ty_residual_adjoint", "ty_unsteady_adjoint"]
 
 
 class TyBocoDesc(C.Structure):
@@ -96,6 +96,7 @@ def load():
     lib.ty_jvp.argtypes = [vp, C.c_double, F64P, F64P]
     lib.ty_linear_solve.argtypes = [vp, C.c_double, C.POINTER(TyKrylov), F64P, F64P, C.POINTER(TyKrylovStats)]
     lib.ty_implicit_step.argtypes = [vp, C.c_double, C.POINTER(TyKrylov), C.POINTER(TyKrylovStats)]
+    lib.ty_implicit_bdlu_step.argtypes = [vp, C.c_double, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_int32)]
     lib.ty_residual_tangent.argtypes = [vp, C.c_double, F64P, F64P, F64P, F64P, F64P]
     lib.ty_residual_adjoint.argtypes = [vp, C.c_double, F64P, F64P, F64P, F64P, F64P]
     lib.ty_unsteady_adjoint.argtypes = [vp, C.c_int32, F64P, C.c_int32, F64P, F64P, F64P,

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <thread>
 #include <vector>
@@ -2002,5 +2003,137 @@ int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t sn
         stats->max_snapshots = u.maxlive;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
+    return TY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the reference's assembled implicit step: calc_rhs with face Jacobians, build_implicit_bdlu,
+// bdlu_gmres, update_field (tstep_implicit.f90, implicit_solve.f90:39-84)
+
+int ty_implicit_bdlu_step(ty_ctx* c, double dt, int32_t nkrylov, double maxres, int32_t max_it, int32_t* info)
+{
+    if (!c || nkrylov < 1 || max_it < 0 || !info) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    hipSetDevice(c->device);
+    int rc;
+    if ((rc = ensure_work(c, 0))) return rc;
+    DevTmp tmp(c);
+    const int n = c->n, nv = c->nv, ns = c->ns;
+    const long ng = (long)c->nb + 2L * c->nper;
+    const long N = n + ng, L = (long)nv * N;
+    const int m = nkrylov;
+    double *stage, *res, *D, *B, *p1, *sol, *r1, *w1, *V, *part, *hd;
+    int* tgt;
+    if ((rc = tmp.alloc(&stage, (size_t)nv * n)) || (rc = tmp.alloc(&res, (size_t)nv * n)) ||
+        (rc = tmp.alloc(&D, (size_t)nv * nv * n)) || (rc = tmp.alloc(&B, (size_t)ns * nv * nv * n)) ||
+        (rc = tmp.alloc(&tgt, (size_t)ns * n)) || (rc = tmp.alloc(&p1, (size_t)L)) || (rc = tmp.alloc(&sol, (size_t)L)) ||
+        (rc = tmp.alloc(&r1, (size_t)L)) || (rc = tmp.alloc(&w1, (size_t)L)) || (rc = tmp.alloc(&V, (size_t)m * L)) ||
+        (rc = tmp.alloc(&part, (size_t)1024 * MAXK)) || (rc = tmp.alloc(&hd, (size_t)MAXK)))
+        return rc;
+    c->h_dt = dt;
+    HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(tgt, 0xff, (size_t)ns * n * sizeof(int), c->stream));
+    HIPCHK(c, hipMemsetAsync(p1, 0, (size_t)L * sizeof(double), c->stream));
+    // residual of every unknown (interior rows -dt/V sum F.A; ghost rows their boundary-flux sums),
+    // diagonal blocks with ns_bocojacobian, per-slot coupling blocks
+    HIPCHK(c, (fill_state<double, double>(stage, c->d_u0, 0.0, c->d_u0, nv, n, c->stream)));
+    KArgs<double> a = make_args<double>(c, c->geo_d);
+    a.us = stage;
+    a.mode = 1;
+    a.res_out = res;
+    a.diag_out = D;
+    a.dinv_out = nullptr;
+    a.bdlu_B = B;
+    a.bdlu_tgt = tgt;
+    a.bdlu_grhs = p1 + n;
+    a.bdlu_ld = N;
+    a.nb_dev = c->nb;
+    a.per_base = c->per_base;
+    HIPCHK(c, (c->dim == 2 ? eval_kernels<2, double>(c, a) : eval_kernels<3, double>(c, a)));
+    for (int v = 0; v < nv; ++v)
+        HIPCHK(c, hipMemcpyAsync(p1 + (size_t)v * N, res + (size_t)v * n, (size_t)n * sizeof(double),
+                                 hipMemcpyDeviceToDevice, c->stream));
+    auto Ax = [&](const double* x, double* y) { return bdlu_apply(nv, D, B, tgt, ns, n, N, x, y, c->stream); };
+    auto dot = [&](const double* x, const double* y, double* out) -> int {
+        HIPCHK(c, multidot<double>(x, y, 0, 1, L, part, hd, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out, hd, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return TY_OK;
+    };
+    // bdlu_gmres.f90:55-212
+    std::vector<double> hh((size_t)(m + 1) * m), ss(m + 1), qc(m), qs(m), yy(m);
+    auto HH = [&](int i, int j) -> double& { return hh[(size_t)(j - 1) * (m + 1) + (i - 1)]; };
+    HIPCHK(c, bdlu_x0(D, p1, sol, nv, n, N, c->stream));
+    double ref = 0.0, pp = 0.0;
+    HIPCHK(c, asum(sol, L, part, hd, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&ref, hd, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = dot(p1, p1, &pp))) return rc;
+    const double normp1 = std::sqrt(pp);
+    double errgmres = std::numeric_limits<double>::max();
+    int nit = 0;
+    while (errgmres >= ref * maxres && nit <= max_it * 2) {
+        HIPCHK(c, Ax(sol, r1));
+        HIPCHK(c, pmr(r1, p1, L, c->stream));
+        double rr;
+        if ((rc = dot(r1, r1, &rr))) return rc;
+        const double beta = std::sqrt(rr);
+        HIPCHK(c, (scale<double, double>(V, r1, 1.0 / beta, L, c->stream)));
+        std::fill(ss.begin(), ss.end(), 0.0);
+        ss[0] = beta;
+        std::fill(hh.begin(), hh.end(), 0.0);
+        int jk;
+        for (jk = 1; jk <= m; ++jk) {
+            HIPCHK(c, Ax(V + (size_t)(jk - 1) * L, w1));
+            for (int ik = 1; ik <= jk; ++ik) {
+                const double* vi = V + (size_t)(ik - 1) * L;
+                if ((rc = dot(w1, vi, &HH(ik, jk)))) return rc;
+                HIPCHK(c, axpy(w1, vi, -HH(ik, jk), L, c->stream));
+            }
+            double ww;
+            if ((rc = dot(w1, w1, &ww))) return rc;
+            HH(jk + 1, jk) = std::sqrt(ww);
+            for (int ik = 1; ik <= jk - 1; ++ik) {
+                const double t = HH(ik, jk);
+                HH(ik, jk) = qc[ik - 1] * t + qs[ik - 1] * HH(ik + 1, jk);
+                HH(ik + 1, jk) = -qs[ik - 1] * t + qc[ik - 1] * HH(ik + 1, jk);
+            }
+            if (HH(jk + 1, jk) > std::numeric_limits<double>::epsilon() * 10 * normp1) {
+                if (jk < m) HIPCHK(c, (scale<double, double>(V + (size_t)jk * L, w1, 1.0 / HH(jk + 1, jk), L, c->stream)));
+                if (HH(jk, jk) == 0) break;
+                const double t = std::sqrt(HH(jk, jk) * HH(jk, jk) + HH(jk + 1, jk) * HH(jk + 1, jk));
+                qc[jk - 1] = HH(jk, jk) / t;
+                qs[jk - 1] = HH(jk + 1, jk) / t;
+                HH(jk, jk) = t;
+                HH(jk + 1, jk) = 0.0;
+                ss[jk] = -qs[jk - 1] * ss[jk - 1];
+                ss[jk - 1] = qc[jk - 1] * ss[jk - 1];
+                if (std::fabs(ss[jk]) < maxres * std::fabs(ss[0])) break;
+            } else {
+                break;
+            }
+        }
+        const int nk = std::min(jk, m);
+        for (int j = nk; j >= 1; --j) {
+            double t = ss[j - 1];
+            for (int ik = j + 1; ik <= nk; ++ik) t -= HH(j, ik) * yy[ik - 1];
+            yy[j - 1] = t / HH(j, j);
+            HIPCHK(c, axpy(sol, V + (size_t)(j - 1) * L, yy[j - 1], L, c->stream));
+        }
+        HIPCHK(c, Ax(sol, r1));
+        HIPCHK(c, pmr(r1, p1, L, c->stream));
+        if ((rc = dot(r1, r1, &rr))) return rc;
+        ss[nk] = std::sqrt(rr);
+        errgmres = ss[nk] / normp1;
+        ++nit;
+    }
+    *info = nit <= max_it ? nit - 1 : -1;
+    // update_field.f90:38 on the interior unknowns, then the stage-state copy
+    for (int v = 0; v < nv; ++v)
+        HIPCHK(c, accumulate<double>(c->d_u0 + (size_t)v * n, sol + (size_t)v * N, n, c->stream));
+    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[c->cur], nv, n, c->stream)
+                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[c->cur], nv, n, c->stream);
+    HIPCHK(c, e);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return TY_OK;
 }

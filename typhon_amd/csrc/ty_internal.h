@@ -98,6 +98,15 @@ struct KArgs {
     R* dinv_out;          // [nv*nv][n] inverse diagonal blocks of the implicit operator (residual mode)
     double* diag_out;     // [nv*nv][n] the diagonal blocks themselves (optional)
     const double* d_dt;   // device time step
+    // assembled implicit matrix (residual mode with Jacobians; null otherwise): per slot the block
+    // coupling this cell's row to the slot's unknown [slot][NV*NV][n], that unknown's index (a cell,
+    // or n + ghost: boundary face b -> b, periodic face j -> nb + 2 j (+1 on its far side)), and the
+    // ghost rows' right-hand side (their boundary-flux sums) at bdlu_grhs[v * bdlu_ld + ghost]
+    double* bdlu_B;
+    int* bdlu_tgt;
+    double* bdlu_grhs;
+    long bdlu_ld;
+    int nb_dev, per_base;
     // RK bookkeeping for this stage
     int mode;             // 0: stage update, 1: residual only
     int stage, nstage;

@@ -1162,12 +1162,73 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
 #pragma unroll
                 for (int i = 0; i < NV; ++i) { jR[i * NV + i] -= id; jL[i * NV + i] += id; }
             }
+            if (a.bdlu_B && bnd) {
+                // ns_bocojacobian.f90:44-92: slip wall jL += jR dQR/dQL (mirror of the momentum
+                // block), supersonic outlet jL += jR; then jR = 0
+                const int kind = a.boco[a.g.bboco[link_index(L)]].kind;
+                if (kind == BK_SYM || kind == BK_OUTLET_SUP) {
+                    R Sm[NV * NV];
+#pragma unroll
+                    for (int k2 = 0; k2 < NV * NV; ++k2) Sm[k2] = (k2 % (NV + 1) == 0) ? R(1) : R(0);
+                    if (kind == BK_SYM) {
+#pragma unroll
+                        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                            for (int j = 0; j < DIM; ++j) Sm[(2 + i) * NV + 2 + j] -= R(2) * (nf[i] * nf[j]);
+                    }
+#pragma unroll
+                    for (int i = 0; i < NV; ++i)
+#pragma unroll
+                        for (int j = 0; j < NV; ++j) {
+                            R sacc = R(0);
+#pragma unroll
+                            for (int l = 0; l < NV; ++l) sacc += jR[i * NV + l] * Sm[l * NV + j];
+                            jL[i * NV + j] += sacc;
+                        }
+#pragma unroll
+                    for (int k2 = 0; k2 < NV * NV; ++k2) jR[k2] = R(0);
+                }
+            }
             if (other) {
 #pragma unroll
                 for (int k2 = 0; k2 < NV * NV; ++k2) D[k2] -= (double)(A * jR[k2]) * kdt;
             } else {
 #pragma unroll
                 for (int k2 = 0; k2 < NV * NV; ++k2) D[k2] += (double)(A * jL[k2]) * kdt;
+            }
+            if (a.bdlu_B) {
+                // off-diagonal block of this row: upper(f) = jacR.A dt/V_owner on the owner's row,
+                // lower(f) = -jacL.A dt/V_other on the other cell's row; a periodic face is two
+                // boundary faces in the reference, each coupling its cell to its own ghost
+                const unsigned fwk = (unsigned)a.fw[(size_t)k * n + c];
+                const int pj = (int)(fwk & FACE_MASK) - a.per_base;
+                const bool per = (fwk & PERIODIC_BIT) != 0u;
+                double* Bk = a.bdlu_B + (size_t)k * NV * NV * n + c;
+                int tgt;
+                if (other) {
+#pragma unroll
+                    for (int k2 = 0; k2 < NV * NV; ++k2) Bk[(size_t)k2 * n] = -((double)(A * jL[k2]) * kdt);
+                    tgt = per ? n + a.nb_dev + 2 * pj + 1 : link_index(L);
+                } else {
+#pragma unroll
+                    for (int k2 = 0; k2 < NV * NV; ++k2) Bk[(size_t)k2 * n] = (double)(A * jR[k2]) * kdt;
+                    tgt = bnd ? n + link_index(L) : (per ? n + a.nb_dev + 2 * pj : link_index(L));
+                }
+                a.bdlu_tgt[(size_t)k * n + c] = tgt;
+            }
+        }
+        if (JAC && a.bdlu_grhs) {
+            // ghost rows of the residual: the boundary-flux sums flux_to_res leaves there
+            // (flux_to_res.f90:90-113, ghost = ic2 of its boundary face)
+            const unsigned fwk = (unsigned)a.fw[(size_t)k * n + c];
+            const bool per = (fwk & PERIODIC_BIT) != 0u;
+            if (bnd || per) {
+                const int g = bnd ? link_index(L) : a.nb_dev + 2 * ((int)(fwk & FACE_MASK) - a.per_base) + (other ? 1 : 0);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const double fa = (double)(A * flux[v]);
+                    a.bdlu_grhs[(size_t)v * a.bdlu_ld + g] = other ? -fa : fa;
+                }
             }
         }
         // flux_to_res.f90:90-113: R(ic1) -= F.A, R(ic2) += F.A
@@ -1193,10 +1254,12 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
 #pragma unroll
                 for (int k = 0; k < NV * NV; ++k) a.diag_out[(size_t)k * n + c] = D[k];
             }
-            double inv[NV * NV];
-            invert_block<NV>(D, inv);
+            if (a.dinv_out) {
+                double inv[NV * NV];
+                invert_block<NV>(D, inv);
 #pragma unroll
-            for (int k = 0; k < NV * NV; ++k) a.dinv_out[(size_t)k * n + c] = (R)inv[k];
+                for (int k = 0; k < NV * NV; ++k) a.dinv_out[(size_t)k * n + c] = (R)inv[k];
+            }
         }
         return;
     }
@@ -1365,7 +1428,7 @@ static void flux_variant(const KArgs<R>& a, hipStream_t s)
 template <int DIM, typename R>
 hipError_t launch_flux(const KArgs<R>& a, hipStream_t s)
 {
-    const bool jac = a.dinv_out != nullptr;
+    const bool jac = a.dinv_out != nullptr || a.diag_out != nullptr;
     if (jac) {  // implicit diagonal blocks: guarded loops only
         if (a.ns <= 4) flux_variant<DIM, R, 4, true, false>(a, s);
         else flux_variant<DIM, R, MAX_SLOTS, true, false>(a, s);

@@ -26,5 +26,13 @@ template <typename U>
 hipError_t accumulate(double* dst, const U* src, long len, hipStream_t s);
 hipError_t outer_residual(double* r, const double* b, const double* x, const double* r1, const double* r0, double sc,
                           long len, hipStream_t s);
+// the reference's assembled matrix: y = A x over [nv][N] vectors (N = n interior + ghost unknowns),
+// diagonal blocks D [nv*nv][n], per-slot blocks B [ns][nv*nv][n] and their unknowns tgt [ns][n]
+hipError_t bdlu_apply(int nv, const double* D, const double* B, const int* tgt, int ns, int n, long N, const double* x,
+                      double* y, hipStream_t s);
+hipError_t bdlu_x0(const double* D, const double* p, double* x, int nv, int n, long N, hipStream_t s);  // x = p/diag
+hipError_t axpy(double* y, const double* x, double a, long len, hipStream_t s);                          // y += a x
+hipError_t pmr(double* r, const double* p, long len, hipStream_t s);                                     // r = p - r
+hipError_t asum(const double* x, long len, double* partial, double* out, hipStream_t s);                 // sum |x|
 
 }  // namespace ty

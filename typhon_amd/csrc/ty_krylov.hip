@@ -228,6 +228,123 @@ hipError_t outer_residual(double* r, const double* b, const double* x, const dou
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// the reference's assembled implicit matrix (bdlu_yeqax, SPMAT_BDLU.f90:139-163) and the vector
+// operations of its GMRES (bdlu_gmres.f90); vectors [nv][N], N = n interior + ghost unknowns
+
+// y = A x: interior rows D_c x_c + sum over slots B_k x_tgt(k); ghost rows are the identity
+template <int NV>
+__global__ __launch_bounds__(KB) void k_bdlu_apply(const double* __restrict__ D, const double* __restrict__ B,
+                                                   const int* __restrict__ tgt, int ns, int n, long N,
+                                                   const double* __restrict__ x, double* __restrict__ y)
+{
+    const long c = (long)blockIdx.x * KB + threadIdx.x;
+    if (c >= N) return;
+    if (c >= n) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) y[(size_t)v * N + c] = x[(size_t)v * N + c];
+        return;
+    }
+    double xc[NV], acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) xc[j] = x[(size_t)j * N + c];
+#pragma unroll
+    for (int r = 0; r < NV; ++r) {
+        double sv = 0.0;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) sv += D[(size_t)(r * NV + j) * n + c] * xc[j];
+        acc[r] = sv;
+    }
+    for (int k = 0; k < ns; ++k) {
+        const int t = tgt[(size_t)k * n + c];
+        if (t < 0) continue;
+        double xt[NV];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) xt[j] = x[(size_t)j * N + t];
+        const double* Bk = B + (size_t)k * NV * NV * n + c;
+#pragma unroll
+        for (int r = 0; r < NV; ++r) {
+            double sv = 0.0;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) sv += Bk[(size_t)(r * NV + j) * n] * xt[j];
+            acc[r] += sv;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) y[(size_t)v * N + c] = acc[v];
+}
+
+// x0 = rhs / diag(ib, ib) (bdlu_gmres.f90:75-81); ghost rows have a unit diagonal
+__global__ void k_bdlu_x0(const double* __restrict__ D, const double* __restrict__ p, double* __restrict__ x,
+                          int nv, int n, long N)
+{
+    const long i = (long)blockIdx.x * KB + threadIdx.x;
+    if (i >= (long)nv * N) return;
+    const int v = (int)(i / N);
+    const long c = i - (long)v * N;
+    x[i] = c < n ? p[i] / D[(size_t)(v * nv + v) * n + c] : p[i];
+}
+
+__global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, double a, long len)
+{
+    const long i = (long)blockIdx.x * KB + threadIdx.x;
+    if (i < len) y[i] = y[i] + a * x[i];
+}
+
+// r = p - r
+__global__ void k_pmr(double* __restrict__ r, const double* __restrict__ p, long len)
+{
+    const long i = (long)blockIdx.x * KB + threadIdx.x;
+    if (i < len) r[i] = p[i] - r[i];
+}
+
+// per-block partial sums of |x| (fixed order), reduced by k_reduce_partials
+__global__ __launch_bounds__(KB) void k_asum(const double* __restrict__ x, long len, double* __restrict__ partial)
+{
+    __shared__ double red[KB / 64];
+    double sv = 0.0;
+    for (long e = (long)blockIdx.x * KB + threadIdx.x; e < len; e += (long)gridDim.x * KB) sv += fabs(x[e]);
+    sv = wsum(sv);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sv;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int k = 1; k < KB / 64; ++k) t += red[k];
+        partial[(long)blockIdx.x * MAXK] = t;
+    }
+}
+
+hipError_t bdlu_apply(int nv, const double* D, const double* B, const int* tgt, int ns, int n, long N, const double* x,
+                      double* y, hipStream_t s)
+{
+    const dim3 grid((unsigned)((N + KB - 1) / KB));
+    if (nv == 4) hipLaunchKernelGGL((k_bdlu_apply<4>), grid, dim3(KB), 0, s, D, B, tgt, ns, n, N, x, y);
+    else hipLaunchKernelGGL((k_bdlu_apply<5>), grid, dim3(KB), 0, s, D, B, tgt, ns, n, N, x, y);
+    return hipGetLastError();
+}
+hipError_t bdlu_x0(const double* D, const double* p, double* x, int nv, int n, long N, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bdlu_x0, dim3(nb_of((long)nv * N)), dim3(KB), 0, s, D, p, x, nv, n, N);
+    return hipGetLastError();
+}
+hipError_t axpy(double* y, const double* x, double a, long len, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_axpy, dim3(nb_of(len)), dim3(KB), 0, s, y, x, a, len);
+    return hipGetLastError();
+}
+hipError_t pmr(double* r, const double* p, long len, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pmr, dim3(nb_of(len)), dim3(KB), 0, s, r, p, len);
+    return hipGetLastError();
+}
+hipError_t asum(const double* x, long len, double* partial, double* out, hipStream_t s)
+{
+    const int nb = multidot_blocks(len);
+    hipLaunchKernelGGL(k_asum, dim3(nb), dim3(KB), 0, s, x, len, partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(KB), 0, s, (const double*)partial, nb, 1, out);
+    return hipGetLastError();
+}
+
 #define TY_KINST(T)                                                                                       \
     template hipError_t apply_a<T>(T*, const T*, const double*, const double*, double, long, hipStream_t); \
     template hipError_t block_apply<T>(int, const T*, const T*, T*, int, hipStream_t);                     \

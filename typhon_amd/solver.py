@@ -227,6 +227,14 @@ class Solver:
         self._check(self.lib.ty_implicit_step(self.h, dt, C.byref(kry), C.byref(st)), "ty_implicit_step")
         return _stats(st)
 
+    def implicit_bdlu_step(self, dt, nkrylov=50, maxres=1e-4, max_it=50):
+        """one implicit step of the reference's assembled path (build_implicit_bdlu + bdlu_gmres,
+        INV_RES = maxres, NKRYLOV, IMP_MAX_IT = max_it); returns the GMRES info"""
+        info = C.c_int32()
+        self._check(self.lib.ty_implicit_bdlu_step(self.h, dt, nkrylov, maxres, max_it, C.byref(info)),
+                    "ty_implicit_bdlu_step")
+        return info.value
+
     def mesh_info(self):
         v = [C.c_int32() for _ in range(4)]
         self._check(self.lib.ty_mesh_info(self.h, *[C.byref(x) for x in v]), "ty_mesh_info")
