@@ -99,6 +99,8 @@ PROFILE_CMDS = {
              "--no-adjoint --no-implicit --no-ns",
     "ns": "rocprofv3 --kernel-trace --stats -f csv -d {d}/ns -o ns -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
           "--no-mixed --no-adjoint --no-implicit --ns-steps 20 --ns-adj-steps 4",
+    "imp": "rocprofv3 --kernel-trace --stats -f csv -d {d}/imp -o imp -- python3 bench.py --steps 1 --warmup 1 "
+           "--no-cpu --no-mixed --no-adjoint --no-ns",
     "cfetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/cfetch -o copy -- python3 bench.py --copy-calibration 1024",
     "cwrite": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/cwrite -o copy -- python3 bench.py --copy-calibration 1024",
 }
@@ -201,6 +203,23 @@ def summarize_profiles(d, n=100):
                       f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
         with open(os.path.join(ROOT, "profiles", "ns_kernels_16m.md"), "w") as f:
             f.write("\n".join(nl) + "\n")
+    imp_stats = os.path.join(d, "imp", "imp_kernel_stats.csv")
+    if os.path.exists(imp_stats):
+        il = ["# Implicit NS kernels, 160^3 = 4,096,000 periodic hexes (Taylor-Green, dt = 10 x explicit)", "",
+              "Two backward-Euler steps of the matrix-free path: FP32 GMRES(30) with block-Jacobi inside FP64 "
+              "refinement (FD Jacobian-vector products = FP32 residual kernels, multi-dot / CGS2 update / "
+              "block-Jacobi apply), FP64 outer residuals and the block-Jacobi diagonal (JAC kernel).  The run "
+              "also holds one FP64 step of the 6M-tet workload and the device mesh set-up kernels.", "",
+              "Command (from the repository root, output directory D):", "",
+              "    " + PROFILE_CMDS["imp"].format(d="D"), "    python3 bench.py --summarize D", "",
+              "| kernel | calls | avg us | min us | max us | share % |", "|---|---|---|---|---|---|"]
+        for r in _csv_rows(imp_stats):
+            if "ty::" not in r["Name"] and "rocprim" not in r["Name"]:
+                continue
+            il.append(f"| {r['Name'].replace('|', '/')[:120]} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                      f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+        with open(os.path.join(ROOT, "profiles", "implicit_kernels_c3.md"), "w") as f:
+            f.write("\n".join(il) + "\n")
     for key, t in traffic.items():
         name = f"traffic_{key[:-5]}_c2_fp32.json" if key.endswith("_fp32") else f"traffic_{key}_c2_fp64.json"
         with open(os.path.join(ROOT, "profiles", name), "w") as f:
