@@ -48,7 +48,10 @@ typedef struct {
                                 22: boco_ns%flux (the user's WALL_FLUX negated, def_boco_ns.f90:73-74),
                                     -, -, -, wall velocity x, y, z (setboco_ns_flux.f90, ns_bocoflux.f90);
                                 20: -, -, -, -, wall velocity x, y, z (a 22 with zero flux,
-                                    def_boco_ns.f90:44-49).  20 and 22 need the laminar NS model. */
+                                    def_boco_ns.f90:44-49).  20 and 22 need the laminar NS model.
+                                23 CONVECTION_WALL and 24 GEN_CONV_WALL are refused (TY_ERR_UNSUPPORTED):
+                                the reference refuses both for NS (MENU_NS.f90:95-113 has no case
+                                for them; setboco_ns_hconv.f90:78 stops). */
 } ty_boco_desc;
 
 /* The unstructured mesh as Typhon holds it after init_world (SOURCE/MAIN/main.f90:95):

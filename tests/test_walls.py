@@ -249,3 +249,16 @@ def test_gpu_adjoint_rejects_walls():
     gpu.set_state(u)
     with pytest.raises(TyError):
         gpu.residual_adjoint(1e-3, np.ones((5, m.ncell_int)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("btype", [M.BC_WALL_HCONV, M.BC_WALL_HGEN])
+def test_gpu_convection_walls_refused(btype):
+    """CONVECTION_WALL / GEN_CONV_WALL: the reference has no NS ghost rule for them
+    (MENU_NS.f90:95-113; setboco_ns_hconv.f90:78 stops), so the upload refuses them"""
+    from typhon_amd.solver import Solver
+    from typhon_amd._lib import TyError
+    m, s, g, u = couette("isoth")
+    m.bocos[-1].type = btype
+    with pytest.raises(TyError, match="convection wall"):
+        Solver(m, s, g)

@@ -386,6 +386,12 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
                 int p = d.ipart[k] - 1;
                 if (p < nfi || p >= nf) return fail(c, TY_ERR_ARG, "bad periodic partner face");
                 fpart[f] = p;
+            } else if (d.type == 23 || d.type == 24) {
+                // convection walls: the reference refuses them for NS at set-up (bctype_of_nsboco,
+                // MENU_NS.f90:95-113, has no case for them) and setboco_ns_hconv.f90:78 stops anyway
+                return fail(c, TY_ERR_UNSUPPORTED, "boundary condition type " + std::to_string(d.type) +
+                                                       " (convection wall): the reference stops in "
+                                                       "setboco_ns_hconv (not implemented there)");
             } else if (d.type != 10 && d.type != 20 && d.type != 21 && d.type != 22 && d.type != 30 &&
                        d.type != 31 && d.type != 35 && d.type != 36) {
                 return fail(c, TY_ERR_UNSUPPORTED, "boundary condition type " + std::to_string(d.type) +

@@ -23,6 +23,8 @@ BC_PERIOD = 11
 BC_WALL_ADIAB = 20
 BC_WALL_ISOTH = 21
 BC_WALL_FLUX = 22
+BC_WALL_HCONV = 23  # CONVECTION_WALL and GEN_CONV_WALL: refused for NS (MENU_NS.f90:95-113)
+BC_WALL_HGEN = 24
 BC_INLET_SUB = 30
 BC_INLET_SUP = 31
 BC_OUTLET_SUB = 35
