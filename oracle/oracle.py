@@ -46,6 +46,9 @@ def _load():
     if not os.path.exists(_LIB):
         build()
     lib = C.CDLL(_LIB)
+    lib.or_threads.restype = C.c_int
+    lib.or_threads.argtypes = []
+    lib.or_set_threads.argtypes = [C.c_int]
     lib.or_create.restype = C.c_void_p
     lib.or_create.argtypes = [C.POINTER(OrConfig)]
     lib.or_destroy.argtypes = [C.c_void_p]

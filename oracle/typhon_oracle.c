@@ -1,7 +1,14 @@
 /*
- * typhon_oracle.c -- CPU restatement (FP64, single thread) of Typhon's
- * Euler/NS residual + explicit RK path.  TEST INFRASTRUCTURE ONLY: the parity
- * checker of the HIP path, never linked into the product.
+ * typhon_oracle.c -- CPU restatement (FP64) of Typhon's Euler/NS residual +
+ * explicit RK path.  TEST INFRASTRUCTURE ONLY: the parity checker of the HIP
+ * path, never linked into the product.
+ *
+ * OpenMP (as the reference's own !$OMP loops): loops whose iterations write
+ * disjoint entries (per face or per cell) run in parallel above OR_PAR_MIN
+ * iterations.  A face->cell scatter is run as a per-cell gather over the
+ * cell's faces in increasing face index (a CSR built once), which adds the
+ * same terms to each cell in the same order as the serial face loop: results
+ * are bitwise those of the serial restatement, whatever the thread count.
  *
  * Each function cites the reference routine it restates (paths relative to the
  * reference root).  Arithmetic follows the reference's operation order where
@@ -19,6 +26,11 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_PAR_MIN 4096
 #define FMAX(a, b) ((a) > (b) ? (a) : (b))
 #define FMIN(a, b) ((a) < (b) ? (a) : (b))
 #define NV 5
@@ -37,6 +49,8 @@ struct or_ctx {
     int lsq_done;
     double* dtloc; /* [ncell] */
     double* ratio; /* [ncell] */
+    int* cf_ptr;   /* [ncell+1] faces of each cell (interior and ghost), increasing face index */
+    int* cf_idx;   /* [2 nface] */
 };
 
 #define CONS(o, v, i) ((o)->cons[(size_t)(v) * (o)->nc + (i)])
@@ -48,6 +62,45 @@ struct or_ctx {
 #define RES(o, v, i) ((o)->res[(size_t)(v) * (o)->nc + (i)])
 
 static double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* the faces of every cell in increasing face index (counting sort over the face loop) */
+static void cell_faces(or_ctx* o)
+{
+    if (o->cf_ptr) return;
+    int nc = o->nc, nf = o->nf;
+    o->cf_ptr = (int*)calloc((size_t)nc + 1, sizeof(int));
+    o->cf_idx = (int*)malloc(sizeof(int) * 2 * (size_t)(nf > 0 ? nf : 1));
+    for (int f = 0; f < nf; ++f) {
+        o->cf_ptr[o->c.ic1[f] + 1]++;
+        o->cf_ptr[o->c.ic2[f] + 1]++;
+    }
+    for (int i = 0; i < nc; ++i) o->cf_ptr[i + 1] += o->cf_ptr[i];
+    int* fill = (int*)malloc(sizeof(int) * (size_t)(nc > 0 ? nc : 1));
+    memcpy(fill, o->cf_ptr, sizeof(int) * (size_t)nc);
+    for (int f = 0; f < nf; ++f) {
+        o->cf_idx[fill[o->c.ic1[f]]++] = f;
+        o->cf_idx[fill[o->c.ic2[f]]++] = f;
+    }
+    free(fill);
+}
+
+int or_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
 
 or_ctx* or_create(const or_config* cfg)
 {
@@ -78,6 +131,7 @@ void or_destroy(or_ctx* o)
     if (!o) return;
     free(o->cons); free(o->prim); free(o->grad); free(o->qL); free(o->qR);
     free(o->flux); free(o->res); free(o->lsq); free(o->dtloc); free(o->ratio);
+    free(o->cf_ptr); free(o->cf_idx);
     free(o);
 }
 
@@ -105,6 +159,7 @@ const double* or_res_all(const or_ctx* o) { return o->res; }
 static void varprim(or_ctx* o)
 {
     double g1 = o->c.gamma - 1.0;
+#pragma omp parallel for if (o->nc > OR_PAR_MIN) schedule(static)
     for (int i = 0; i < o->nc; ++i) {
         double rho = CONS(o, 0, i);
         double u = CONS(o, 2, i) / rho, v = CONS(o, 3, i) / rho, w = CONS(o, 4, i) / rho;
@@ -391,25 +446,26 @@ static void precalc_lsq(or_ctx* o)
 static void calc_gradient(or_ctx* o)
 {
     if (!o->lsq_done) precalc_lsq(o);
+    cell_faces(o);
     int ni = o->ni;
-    double* rhs = (double*)calloc((size_t)15 * ni, sizeof(double)); /* [cell][var][3] */
-    for (int f = 0; f < o->nf; ++f) {
-        double d[3]; face_dcg(o, f, d, 1);
-        int c1 = o->c.ic1[f], c2 = o->c.ic2[f];
-        for (int v = 0; v < NV; ++v) {
-            double ds = PRIM(o, v, c2) - PRIM(o, v, c1);
-            double* r1 = rhs + ((size_t)c1 * NV + v) * 3;
-            r1[0] += ds * d[0]; r1[1] += ds * d[1]; r1[2] += ds * d[2];
-            if (f < o->nfi) {
-                double* r2 = rhs + ((size_t)c2 * NV + v) * 3;
-                r2[0] += ds * d[0]; r2[1] += ds * d[1]; r2[2] += ds * d[2];
+    /* right-hand side sum_f (q_2 - q_1) d_f over the cell's faces (an interior cell is ic2 only
+       of two-sided faces), then the Cholesky solve */
+#pragma omp parallel for if (ni > OR_PAR_MIN) schedule(static)
+    for (int i = 0; i < ni; ++i) {
+        double rhs[NV * 3] = {0.0};
+        for (int p = o->cf_ptr[i]; p < o->cf_ptr[i + 1]; ++p) {
+            int f = o->cf_idx[p];
+            double d[3]; face_dcg(o, f, d, 1);
+            int c1 = o->c.ic1[f], c2 = o->c.ic2[f];
+            for (int v = 0; v < NV; ++v) {
+                double ds = PRIM(o, v, c2) - PRIM(o, v, c1);
+                double* r = rhs + v * 3;
+                r[0] += ds * d[0]; r[1] += ds * d[1]; r[2] += ds * d[2];
             }
         }
-    }
-    for (int i = 0; i < ni; ++i) {
         const double* a = o->lsq + 9 * (size_t)i;
         for (int v = 0; v < NV; ++v) {
-            double* b = rhs + ((size_t)i * NV + v) * 3;
+            double* b = rhs + v * 3;
             for (int r = 0; r < 3; ++r) {
                 for (int j = 0; j < r; ++j) b[r] = b[r] - a[r + 3 * j] * b[j];
                 b[r] = b[r] / a[r + 3 * r];
@@ -421,7 +477,6 @@ static void calc_gradient(or_ctx* o)
             for (int d = 0; d < 3; ++d) GRAD(o, v, d, i) = b[d];
         }
     }
-    free(rhs);
 }
 
 /* ---------------------------------------------------------------------------
@@ -493,6 +548,7 @@ static double fast_slope(char lim, double s, double lr)
 static void hres_states(or_ctx* o)
 {
     const or_config* c = &o->c;
+#pragma omp parallel for if (o->nf > OR_PAR_MIN) schedule(static)
     for (int f = 0; f < o->nf; ++f) {
         int cl = c->ic1[f], cr = c->ic2[f];
         if (c->method == 'N') {
@@ -597,40 +653,48 @@ static void venkat_states(or_ctx* o)
     double* qmax = (double*)malloc(sizeof(double) * NV * (size_t)nc);
     double* qmin = (double*)malloc(sizeof(double) * NV * (size_t)nc);
     double* phi = (double*)malloc(sizeof(double) * NV * (size_t)nc);
-    for (int v = 0; v < NV; ++v)
-        for (int i = 0; i < nc; ++i) {
+    cell_faces(o);
+    double k3 = c->venkat_k * c->venkat_k * c->venkat_k;
+    /* per cell, over its faces in increasing face index: the neighbour extrema, then the minimum
+       of the face limiter values.  Ghost cells keep qmax = qmin = q and phi = 1 (they are the
+       second cell of one-sided faces only, which update nothing). */
+#pragma omp parallel for if (nc > OR_PAR_MIN) schedule(static)
+    for (int i = 0; i < nc; ++i) {
+        for (int v = 0; v < NV; ++v) {
             qmax[(size_t)v * nc + i] = PRIM(o, v, i);
             qmin[(size_t)v * nc + i] = PRIM(o, v, i);
             phi[(size_t)v * nc + i] = 1.0;
         }
-    for (int f = 0; f < o->nf; ++f) {
-        int cl = c->ic1[f], cr = c->ic2[f];
-        for (int v = 0; v < NV; ++v) {
-            double a = PRIM(o, v, cl), b = PRIM(o, v, cr);
-            size_t il = (size_t)v * nc + cl, ir = (size_t)v * nc + cr;
-            qmax[il] = FMAX(qmax[il], b); qmin[il] = FMIN(qmin[il], b);
-            if (f < o->nfi) { qmax[ir] = FMAX(qmax[ir], a); qmin[ir] = FMIN(qmin[ir], a); }
+        if (i >= o->ni) continue;
+        for (int p = o->cf_ptr[i]; p < o->cf_ptr[i + 1]; ++p) {
+            int f = o->cf_idx[p];
+            int cl = c->ic1[f], cr = c->ic2[f];
+            if (cl != i && f >= o->nfi) continue;
+            int other = (cl == i) ? cr : cl;
+            for (int v = 0; v < NV; ++v) {
+                size_t ii = (size_t)v * nc + i;
+                double b = PRIM(o, v, other);
+                qmax[ii] = FMAX(qmax[ii], b); qmin[ii] = FMIN(qmin[ii], b);
+            }
         }
-    }
-    double k3 = c->venkat_k * c->venkat_k * c->venkat_k;
-    for (int f = 0; f < o->nf; ++f) {
-        for (int side = 0; side < 2; ++side) {
-            if (side == 1 && f >= o->nfi) continue;
-            int ci = side ? c->ic2[f] : c->ic1[f];
-            const double* xc = c->ccent + 3 * (size_t)ci;
+        const double* xc = c->ccent + 3 * (size_t)i;
+        double h = (c->dim == 2) ? sqrt(c->cvol[i]) : cbrt(c->cvol[i]);
+        double eps2 = k3 * h * h * h;
+        for (int p = o->cf_ptr[i]; p < o->cf_ptr[i + 1]; ++p) {
+            int f = o->cf_idx[p];
+            if (c->ic1[f] != i && f >= o->nfi) continue;
             const double* xf = c->fcent + 3 * (size_t)f;
             double r[3] = {xf[0] - xc[0], xf[1] - xc[1], xf[2] - xc[2]};
-            double h = (c->dim == 2) ? sqrt(c->cvol[ci]) : cbrt(c->cvol[ci]);
-            double eps2 = k3 * h * h * h;
             for (int v = 0; v < NV; ++v) {
-                double g[3] = {GRAD(o, v, 0, ci), GRAD(o, v, 1, ci), GRAD(o, v, 2, ci)};
+                double g[3] = {GRAD(o, v, 0, i), GRAD(o, v, 1, i), GRAD(o, v, 2, i)};
                 double d2 = dot3(g, r);
-                size_t ii = (size_t)v * nc + ci;
-                double p = or_venkat_phi(qmax[ii] - PRIM(o, v, ci), qmin[ii] - PRIM(o, v, ci), d2, eps2);
-                phi[ii] = FMIN(phi[ii], p);
+                size_t ii = (size_t)v * nc + i;
+                double pv = or_venkat_phi(qmax[ii] - PRIM(o, v, i), qmin[ii] - PRIM(o, v, i), d2, eps2);
+                phi[ii] = FMIN(phi[ii], pv);
             }
         }
     }
+#pragma omp parallel for if (o->nf > OR_PAR_MIN) schedule(static)
     for (int f = 0; f < o->nf; ++f) {
         int cl = c->ic1[f], cr = c->ic2[f];
         for (int side = 0; side < 2; ++side) {
@@ -924,6 +988,7 @@ static void calc_rhs_internal(or_ctx* o, double dt)
     }
     boco_face_state(o);
     /* integration_ns_ust (EQNS/integration_ns_ust.f90:87-192): inviscid fluxes */
+#pragma omp parallel for if (o->nf > OR_PAR_MIN) schedule(static)
     for (int f = 0; f < o->nf; ++f) {
         double ql[NV], qr[NV], F[NV];
         for (int v = 0; v < NV; ++v) { ql[v] = QL(o, v, f); qr[v] = QR(o, v, f); }
@@ -935,16 +1000,24 @@ static void calc_rhs_internal(or_ctx* o, double dt)
         ns_bocoflux(o);
     }
     /* flux_to_res (SOURCE/MAIN/flux_to_res.f90:47-142) */
-    for (int v = 0; v < NV; ++v)
-        for (int f = 0; f < o->nf; ++f) FLX(o, v, f) = c->fsurf[f] * FLX(o, v, f);
-    memset(o->res, 0, sizeof(double) * NV * (size_t)o->nc);
-    for (int f = 0; f < o->nf; ++f) {
-        int c1 = c->ic1[f], c2 = c->ic2[f];
-        for (int v = 0; v < NV; ++v) {
-            RES(o, v, c1) = RES(o, v, c1) - FLX(o, v, f);
-            RES(o, v, c2) = RES(o, v, c2) + FLX(o, v, f);
+#pragma omp parallel for if (o->nf > OR_PAR_MIN) schedule(static)
+    for (int f = 0; f < o->nf; ++f)
+        for (int v = 0; v < NV; ++v) FLX(o, v, f) = c->fsurf[f] * FLX(o, v, f);
+    /* R_c = sum over the cell's faces in face order of -F (c = ic1) or +F (c = ic2) */
+    cell_faces(o);
+#pragma omp parallel for if (o->nc > OR_PAR_MIN) schedule(static)
+    for (int i = 0; i < o->nc; ++i) {
+        double r[NV] = {0.0};
+        for (int p = o->cf_ptr[i]; p < o->cf_ptr[i + 1]; ++p) {
+            int f = o->cf_idx[p];
+            if (c->ic1[f] == i)
+                for (int v = 0; v < NV; ++v) r[v] = r[v] - FLX(o, v, f);
+            else
+                for (int v = 0; v < NV; ++v) r[v] = r[v] + FLX(o, v, f);
         }
+        for (int v = 0; v < NV; ++v) RES(o, v, i) = r[v];
     }
+#pragma omp parallel for if (o->ni > OR_PAR_MIN) schedule(static)
     for (int i = 0; i < o->ni; ++i) {
         double coef = dt / c->cvol[i];
         for (int v = 0; v < NV; ++v) RES(o, v, i) = coef * RES(o, v, i);
@@ -1004,6 +1077,7 @@ void or_rk_step(or_ctx* o, double dt)
             for (int j = 1; j <= s - 1; ++j) {
                 double a = c->coef[(s - 2) * 8 + (j - 1)];
                 const double* r = rhs + (size_t)(j - 1) * sz;
+#pragma omp parallel for if (sz > OR_PAR_MIN) schedule(static)
                 for (size_t k = 0; k < sz; ++k) o->cons[k] = o->cons[k] + a * r[k];
             }
         }
@@ -1012,12 +1086,15 @@ void or_rk_step(or_ctx* o, double dt)
     }
     /* residu = coef(n,1) R1 + sum coef(n,j) Rj ; U = U0 + residu */
     double b1 = c->coef[(ns - 1) * 8 + 0];
+#pragma omp parallel for if (sz > OR_PAR_MIN) schedule(static)
     for (size_t k = 0; k < sz; ++k) o->res[k] = rhs[k] * b1;
     for (int j = 2; j <= ns; ++j) {
         double b = c->coef[(ns - 1) * 8 + (j - 1)];
         const double* r = rhs + (size_t)(j - 1) * sz;
+#pragma omp parallel for if (sz > OR_PAR_MIN) schedule(static)
         for (size_t k = 0; k < sz; ++k) o->res[k] = o->res[k] + b * r[k];
     }
+#pragma omp parallel for if (sz > OR_PAR_MIN) schedule(static)
     for (size_t k = 0; k < sz; ++k) o->cons[k] = u0[k] + o->res[k];
     free(u0); free(rhs);
 }

@@ -107,6 +107,9 @@ typedef struct {
 typedef struct or_ctx or_ctx;
 
 or_ctx* or_create(const or_config* cfg);
+/* OpenMP threads the parallel loops use (1 when built without OpenMP) */
+int or_threads(void);
+void or_set_threads(int n);
 void    or_destroy(or_ctx* o);
 
 /* state in/out: conservative interior cells [5][ncell_int] (rho, rhoE, mx, my, mz) */

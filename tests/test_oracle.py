@@ -572,3 +572,31 @@ def test_barth_postlimited_increments_are_bounded(post, klim):
         tol = 1e-13
         assert np.all(dq * D >= -tol), side
         assert np.all(np.abs(dq) <= klim * np.abs(D) + tol), side
+
+
+@pytest.mark.parametrize("case", ["vortex_venkat", "tgv_ns"])
+def test_threads_do_not_change_results(case):
+    """The oracle's OpenMP loops write disjoint entries and its face->cell sums run per cell in
+    face order: residuals and RK steps are bitwise the same for 1 and several threads (meshes
+    above the parallel threshold)."""
+    if case == "vortex_venkat":
+        m, s, g, u = K.vortex_case(10, L=10.0)
+    else:
+        m, s, g, u = K.tgv_case(18, jitter=0.05)
+    L = O.lib()
+    n0 = L.or_threads()
+    res = []
+    try:
+        for nt in (1, max(2, n0)):
+            L.or_set_threads(nt)
+            o = K.make_oracle(m, s, g)
+            o.set_cons(u)
+            dt = o.timestep(s["cfl"])
+            o.calc_rhs(dt)
+            r = o.res_all().copy()
+            o.rk_step(dt)
+            res.append((r, o.get_cons()))
+    finally:
+        L.or_set_threads(n0)
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    np.testing.assert_array_equal(res[0][1], res[1][1])
