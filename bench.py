@@ -89,6 +89,7 @@ def cpu_baseline(n_cpu, budget_s=20.0):
     el = time.perf_counter() - t0
     rate = m.ncell_int * 4 * steps / el / 1e6
     return dict(value=rate, unit="Mcell-updates/s", cores=threads, kind="port",
+                us_per_cell_it_x_ncpu=round(threads * el / steps / m.ncell_int * 1e6, 4),
                 sample=f"oracle C restatement, {threads} OpenMP threads, same scheme on {n_cpu}^3 hexes x 6 = "
                        f"{m.ncell_int} periodic tets, {steps} RK4 steps ({el:.1f} s)")
 
@@ -481,6 +482,9 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(t64 / args.steps * 1e3, 4),
+        # the reference's normalized walltime (integration_cyclezone.f90:174-175): processes x
+        # threads x walltime / iterations / cells, in microseconds; one GPU counts as one
+        "us_per_cell_it": round(t64 / args.steps / nc * 1e6, 6),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
