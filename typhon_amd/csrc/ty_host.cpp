@@ -147,8 +147,9 @@ struct PhaseTimer {
 
 void free_state(ty_ctx* c)
 {
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (void* p : c->owned) hipFree(p);
+    // cleanup is best effort: a failed free or a sticky error from an earlier launch must not stop it
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->owned) (void)hipFree(p);
     c->owned.clear();
     c->mesh_bufs.clear();
     c->d_nbr = nullptr;
@@ -170,7 +171,7 @@ void free_state(ty_ctx* c)
     c->kry_prec = -1;
     c->d_bad = c->d_err = nullptr;
     for (auto* v : {&c->ev_grad, &c->ev_flux, &c->ev_dt, &c->ev_tan, &c->ev_adj})
-        for (auto& e : *v) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+        for (auto& e : *v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     c->ev_grad.clear();
     c->ev_flux.clear();
     c->ev_dt.clear();
@@ -336,9 +337,9 @@ int ty_create(ty_ctx** ctx, int device)
 int ty_destroy(ty_ctx* c)
 {
     if (!c) return TY_OK;
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);  // best effort: destruction frees what it can
     free_state(c);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return TY_OK;
 }
@@ -351,7 +352,7 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
     if (m->dim != 2 && m->dim != 3) return fail(c, TY_ERR_ARG, "dim must be 2 or 3");
     const int ni = m->ncell_int, nc = m->ncell, nf = m->nface, nfi = m->nface_int;
     if (ni <= 0 || nc < ni || nf <= 0 || nfi < 0 || nfi > nf) return fail(c, TY_ERR_ARG, "bad mesh sizes");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     PhaseTimer phase;
     free_state(c);
     c->mesh_ok = c->scheme_ok = c->state_ok = false;
@@ -704,14 +705,14 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     if ((s->method != 'N' || viscous) && s->gradmeth != 10 && s->gradmeth != 12)
         return fail(c, TY_ERR_UNSUPPORTED, "gradient method (LSQ or W-LSQ required)");
     if (s->precision != TY_PREC_FP64 && s->precision != TY_PREC_MIXED) return fail(c, TY_ERR_ARG, "precision");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     // keep the mesh buffers, drop the rest
-    hipStreamSynchronize(c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     {
         std::vector<void*> keep;
         for (void* p : c->owned) {
             if (std::find(c->mesh_bufs.begin(), c->mesh_bufs.end(), p) != c->mesh_bufs.end()) keep.push_back(p);
-            else hipFree(p);
+            else (void)hipFree(p);  // an earlier scheme's buffers: best effort
         }
         c->owned = keep;
     }
@@ -807,7 +808,7 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     // both precisions: the FP32 geometry also serves the FP32 Krylov products
     PhaseTimer phase;
     if ((rc = build_geo<float>(c, c->geo_f)) || (rc = build_geo<double>(c, c->geo_d))) return rc;
-    if (phase.on) hipStreamSynchronize(c->stream);
+    if (phase.on) HIPCHK(c, hipStreamSynchronize(c->stream));
     phase("scheme: slot geometry (device)");
     std::vector<double> dtfac(c->n);
     for (int i = 0; i < c->n; ++i) dtfac[i] = s->cfl * 2.0 * c->vol[i] / c->sumA[i];
@@ -852,7 +853,7 @@ int ty_state_set(ty_ctx* c, const double* rho, const double* rhoE, const double*
 {
     if (!c || !rho || !rhoE || !mom) return TY_ERR_ARG;
     if (!c->scheme_ok) return fail(c, TY_ERR_ARG, "ty_state_set before ty_set_scheme");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     const int n = c->n, nv = c->nv;
     std::vector<double> h((size_t)nv * n);
     for (int i = 0; i < n; ++i) {
@@ -875,7 +876,7 @@ int ty_state_get(ty_ctx* c, double* rho, double* rhoE, double* mom)
 {
     if (!c || !rho || !rhoE || !mom) return TY_ERR_ARG;
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     const int n = c->n, nv = c->nv;
     std::vector<double> h((size_t)nv * n);
     HIPCHK(c, hipMemcpyAsync(h.data(), c->d_u0, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -924,19 +925,40 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     return a;
 }
 
+// a timing event pair: both created and the first recorded on the context stream, or neither kept
+hipError_t ev_begin(ty_ctx* c, hipEvent_t* e0, hipEvent_t* e1)
+{
+    *e0 = *e1 = nullptr;
+    hipError_t e = hipEventCreate(e0);
+    if (e == hipSuccess) e = hipEventCreate(e1);
+    if (e == hipSuccess) e = hipEventRecord(*e0, c->stream);
+    if (e != hipSuccess) {
+        if (*e0) (void)hipEventDestroy(*e0);
+        if (*e1) (void)hipEventDestroy(*e1);
+        *e0 = *e1 = nullptr;
+    }
+    return e;
+}
+
+// records the second event and hands the pair to the list (ty_timing or free_state destroys it)
+hipError_t ev_end(ty_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>& ev, hipEvent_t e0, hipEvent_t e1)
+{
+    hipError_t e = hipEventRecord(e1, c->stream);
+    ev.push_back({e0, e1});
+    return e;
+}
+
 template <typename R>
 hipError_t timed(ty_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>& ev, hipError_t (*fn)(const KArgs<R>&, hipStream_t),
                  const KArgs<R>& a)
 {
     if (!c->timing) return fn(a, c->stream);
     hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, c->stream);
-    hipError_t e = fn(a, c->stream);
-    hipEventRecord(e1, c->stream);
-    ev.push_back({e0, e1});
-    return e;
+    hipError_t e = ev_begin(c, &e0, &e1);
+    if (e != hipSuccess) return e;
+    e = fn(a, c->stream);
+    const hipError_t er = ev_end(c, ev, e0, e1);
+    return e != hipSuccess ? e : er;
 }
 
 // residual kernels of one evaluation on the current stage state
@@ -1003,16 +1025,12 @@ int launch_dt(ty_ctx* c, double dtmax)
     d.part = c->d_part;
     d.bad = c->d_bad;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->timing) {
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
-        hipEventRecord(e0, c->stream);
-    }
+    if (c->timing) HIPCHK(c, ev_begin(c, &e0, &e1));
     hipError_t e = c->dim == 2 ? launch_timestep<2>(d, dtmax, c->d_dt, c->d_err, c->stream)
                                : launch_timestep<3>(d, dtmax, c->d_dt, c->d_err, c->stream);
     if (c->timing) {
-        hipEventRecord(e1, c->stream);
-        c->ev_dt.push_back({e0, e1});
+        const hipError_t er = ev_end(c, c->ev_dt, e0, e1);
+        if (e == hipSuccess) e = er;
     }
     if (e != hipSuccess) return fail(c, TY_ERR_HIP, std::string("time step launch: ") + hipGetErrorString(e));
     return TY_OK;
@@ -1024,8 +1042,8 @@ int check_err(ty_ctx* c)
     HIPCHK(c, hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (h) {
-        hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream);
-        hipStreamSynchronize(c->stream);
+        HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         return fail(c, TY_ERR_STATE, "negative or NaN internal energy (calc_ns_timestep)");
     }
     return TY_OK;
@@ -1037,7 +1055,7 @@ int ty_timestep(ty_ctx* c, double dtmax, double* dt)
 {
     if (!c || !dt) return TY_ERR_ARG;
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     int rc = launch_dt(c, dtmax);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(&c->h_dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1051,7 +1069,7 @@ int ty_residual(ty_ctx* c, double dt, double* res_rho, double* res_rhoE, double*
 {
     if (!c || !res_rho || !res_rhoE || !res_mom) return TY_ERR_ARG;
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     c->h_dt = dt;
     HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
     hipError_t e;
@@ -1082,7 +1100,7 @@ int ty_rk_step(ty_ctx* c, double dt)
 {
     if (!c) return TY_ERR_ARG;
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     c->h_dt = dt;
     HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
     int rc = run_step(c);
@@ -1095,7 +1113,7 @@ int ty_advance(ty_ctx* c, int32_t nsteps, double dtmax, double* dt_last)
 {
     if (!c || nsteps < 0) return TY_ERR_ARG;
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     for (int s = 0; s < nsteps; ++s) {
         int rc = launch_dt(c, dtmax);
         if (rc) return rc;
@@ -1114,7 +1132,7 @@ int ty_advance(ty_ctx* c, int32_t nsteps, double dtmax, double* dt_last)
 int ty_sync(ty_ctx* c)
 {
     if (!c) return TY_ERR_ARG;
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->d_err) return check_err(c);
     return TY_OK;
@@ -1125,18 +1143,24 @@ int ty_sync(ty_ctx* c)
 int ty_timing(ty_ctx* c, int32_t enable, double* ms, int32_t* count)
 {
     if (!c) return TY_ERR_ARG;
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<std::pair<hipEvent_t, hipEvent_t>>* lists[5] = {&c->ev_grad, &c->ev_flux, &c->ev_dt, &c->ev_tan,
                                                                 &c->ev_adj};
     for (int k = 0; k < 5; ++k) {
         double tot = 0.0;
+        hipError_t err = hipSuccess;
         for (auto& e : *lists[k]) {
             float t = 0.f;
-            hipEventElapsedTime(&t, e.first, e.second);
-            tot += t;
-            hipEventDestroy(e.first);
-            hipEventDestroy(e.second);
+            const hipError_t et = hipEventElapsedTime(&t, e.first, e.second);
+            if (et == hipSuccess) tot += t;
+            else if (err == hipSuccess) err = et;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        if (err != hipSuccess) {
+            lists[k]->clear();
+            return fail(c, TY_ERR_HIP, std::string("hipEventElapsedTime: ") + hipGetErrorString(err));
         }
         if (ms) ms[k] = tot;
         if (count) count[k] = (int32_t)lists[k]->size();
@@ -1433,7 +1457,7 @@ int ty_jacobian_diag(ty_ctx* c, double dt, double* diag)
 {
     if (!c || !diag) return TY_ERR_ARG;
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     const int n = c->n, nv = c->nv;
     double* d_diag = nullptr;
     double* d_inv = nullptr;
@@ -1459,7 +1483,7 @@ int ty_jacobian_diag(ty_ctx* c, double dt, double* diag)
     }
     // release the temporaries
     for (double* p : {d_diag, d_inv, d_st}) {
-        hipFree(p);
+        (void)hipFree(p);
         c->owned.erase(std::find(c->owned.begin(), c->owned.end(), (void*)p));
     }
     return TY_OK;
@@ -1469,7 +1493,7 @@ int ty_jvp(ty_ctx* c, double dt, const double* v, double* jv)
 {
     if (!c || !v || !jv) return TY_ERR_ARG;
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     int rc;
     if ((rc = alloc_krylov(c, std::max(1, c->kry_m), c->kry_prec < 0 ? 0 : c->kry_prec))) return rc;
     const long len = (long)c->nv * c->n;
@@ -1501,7 +1525,7 @@ int ty_linear_solve(ty_ctx* c, double dt, const ty_krylov* k, const double* b, d
 {
     if (!c || !k || !b || !x || !st) return TY_ERR_ARG;
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     int rc;
     if ((rc = alloc_krylov(c, k->nkrylov, k->inner_precision == TY_PREC_MIXED ? 1 : 0))) return rc;
     const long len = (long)c->nv * c->n;
@@ -1521,7 +1545,7 @@ int ty_implicit_step(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* 
 {
     if (!c || !k || !st) return TY_ERR_ARG;
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     int rc;
     if ((rc = alloc_krylov(c, k->nkrylov, k->inner_precision == TY_PREC_MIXED ? 1 : 0))) return rc;
     const long len = (long)c->nv * c->n;
@@ -1552,20 +1576,15 @@ struct EventScope {
     ty_ctx* c;
     std::vector<std::pair<hipEvent_t, hipEvent_t>>& ev;
     hipEvent_t e0 = nullptr;
+    hipEvent_t e1 = nullptr;
+    // timing only: a pair that cannot be created or recorded is dropped (the call itself goes on)
     EventScope(ty_ctx* cc, std::vector<std::pair<hipEvent_t, hipEvent_t>>& l) : c(cc), ev(l)
     {
-        if (c->timing) {
-            hipEventCreate(&e0);
-            hipEventRecord(e0, c->stream);
-        }
+        if (c->timing) (void)ev_begin(c, &e0, &e1);
     }
     ~EventScope()
     {
-        if (!c->timing) return;
-        hipEvent_t e1;
-        hipEventCreate(&e1);
-        hipEventRecord(e1, c->stream);
-        ev.push_back({e0, e1});
+        if (c->timing && e0) (void)ev_end(c, ev, e0, e1);
     }
 };
 
@@ -1577,7 +1596,7 @@ struct DevTmp {
     ~DevTmp()
     {
         for (void* q : p) {
-            hipFree(q);
+            (void)hipFree(q);
             auto it = std::find(c->owned.begin(), c->owned.end(), q);
             if (it != c->owned.end()) c->owned.erase(it);
         }
@@ -1598,7 +1617,7 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
         return fail(c, TY_ERR_UNSUPPORTED, "tangent/adjoint residual: first-order schemes only");
     if (c->ext_bc)
         return fail(c, TY_ERR_UNSUPPORTED, "tangent/adjoint residual: wall and subsonic-inlet boundaries not provided");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     c->h_dt = dt;
     HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
     memset(&t, 0, sizeof(t));
@@ -2020,7 +2039,7 @@ int ty_implicit_bdlu_step(ty_ctx* c, double dt, int32_t nkrylov, double maxres, 
 {
     if (!c || nkrylov < 1 || max_it < 0 || !info) return TY_ERR_ARG;
     if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
-    hipSetDevice(c->device);
+    HIPCHK(c, hipSetDevice(c->device));
     int rc;
     if ((rc = ensure_work(c, 0))) return rc;
     DevTmp tmp(c);

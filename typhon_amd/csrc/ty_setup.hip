@@ -148,7 +148,12 @@ hipError_t sort_keys_device(const uint64_t* keys, int n, int end_bit, int* idx_o
     size_t tb = 0;
     hipError_t e = hipSuccess;
     auto done = [&](hipError_t r) {
-        hipFree(k0); hipFree(k1); hipFree(v0); hipFree(v1); hipFree(tmp);
+        // hipFree waits for the device; a failing free must not hide the call's own result
+        (void)hipFree(k0);
+        (void)hipFree(k1);
+        (void)hipFree(v0);
+        (void)hipFree(v1);
+        (void)hipFree(tmp);
         return r;
     };
     if ((e = hipMalloc(&k0, (size_t)n * 8)) != hipSuccess || (e = hipMalloc(&k1, (size_t)n * 8)) != hipSuccess ||
