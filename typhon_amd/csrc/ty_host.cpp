@@ -1210,8 +1210,18 @@ int alloc_krylov(ty_ctx* c, int m, int prec)
         if ((rc = dalloc(c, &c->k_b, len)) || (rc = dalloc(c, &c->k_x, len)) || (rc = dalloc(c, &c->k_r, len)) ||
             (rc = dalloc(c, &c->k_r0, len)) || (rc = dalloc(c, &c->k_r1, len)) || (rc = dalloc(c, &c->k_r2, len)) ||
             (rc = dalloc(c, &c->k_pd, len)) || (rc = dalloc(c, &c->k_dx, len)) || (rc = dalloc(c, &c->k_pf, len)) ||
-            (rc = dalloc(c, &c->k_part, (size_t)multidot_blocks(len) * MAXK)) || (rc = dalloc(c, &c->k_h, MAXK)))
+            (rc = dalloc(c, &c->k_part, (size_t)multidot_blocks(len) * MAXK)) || (rc = dalloc(c, &c->k_h, 3 * MAXK)))
             return rc;
+    }
+    if (c->kry_ok) {
+        // a larger basis or another inner precision: release the previous basis and work vectors
+        for (void** q : {&c->k_V, &c->k_z, &c->k_w, &c->k_t, &c->k_dinv}) {
+            auto it = std::find(c->owned.begin(), c->owned.end(), *q);
+            if (it != c->owned.end()) c->owned.erase(it);
+            HIPCHK(c, hipFree(*q));
+            *q = nullptr;
+        }
+        c->kry_ok = false;
     }
     void* p = nullptr;
     HIPCHK(c, hipMalloc(&p, (size_t)(m + 1) * len * pb));
@@ -1267,7 +1277,7 @@ int gmres(ty_ctx* c, const double* rhs, const ty_krylov* k, double unorm, int* i
     *rel = 0.0;
     if (beta0 == 0.0) return TY_OK;
     const double target = k->inner_tol * beta0;
-    std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m), hh(MAXK), h2(MAXK);
+    std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m), hh(MAXK), h2(MAXK), hbuf(3 * MAXK);
     double res = beta0;
     HIPCHK(c, hipMemsetAsync(c->k_dx, 0, len * sizeof(double), c->stream));
     auto A_apply = [&](const P* zin, P* wout) -> int {  // wout = zin - J zin
@@ -1315,16 +1325,23 @@ int gmres(ty_ctx* c, const double* rhs, const ty_krylov* k, double unorm, int* i
                 zj = z;
             }
             if ((rc = A_apply(zj, w))) return rc;
-            // classical Gram-Schmidt, twice
-            for (int pass = 0; pass < 2; ++pass) {
-                double* hp = pass ? h2.data() : hh.data();
-                HIPCHK(c, multidot<P>(w, V, len, j + 1, len, c->k_part, c->k_h, c->stream));
-                HIPCHK(c, hipMemcpyAsync(hp, c->k_h, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(c, cgs_update<P>(w, V, len, c->k_h, j + 1, len, c->stream));
-                HIPCHK(c, hipStreamSynchronize(c->stream));
+            // classical Gram-Schmidt, twice, in three sweeps over the basis: h1 = V^T w; w -= V h1
+            // with h2 = V^T w in the same sweep; w -= V h2 with ||w||^2 in the same sweep (the sums
+            // are those of separate multi-dot passes, bit for bit); one host round trip per step
+            double* dh1 = c->k_h;
+            double* dh2 = c->k_h + MAXK;
+            double* dnn = c->k_h + 2 * MAXK;
+            HIPCHK(c, multidot<P>(w, V, len, j + 1, len, c->k_part, dh1, c->stream));
+            HIPCHK(c, cgs_pass<P>(w, V, len, dh1, j + 1, true, len, c->k_part, dh2, c->stream));
+            HIPCHK(c, cgs_pass<P>(w, V, len, dh2, j + 1, false, len, c->k_part, dnn, c->stream));
+            HIPCHK(c, hipMemcpyAsync(hbuf.data(), c->k_h, hbuf.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (int i = 0; i <= j; ++i) {
+                hh[i] = hbuf[i];
+                h2[i] = hbuf[MAXK + i];
             }
-            double ww;
-            if ((rc = dot_host<P>(c, w, w, len, &ww))) return rc;
+            const double ww = hbuf[2 * MAXK];
             const double hn = std::sqrt(ww);
             double* col = &H[(size_t)j * (m + 1)];
             for (int i = 0; i <= j; ++i) col[i] = hh[i] + h2[i];

@@ -16,8 +16,10 @@ template <typename T>
 hipError_t block_apply(int nv, const T* dinv, const T* v, T* z, int n, hipStream_t s);
 template <typename T>
 hipError_t multidot(const T* w, const T* V, long stride, int m, long len, double* partial, double* out, hipStream_t s);
+// w -= V h, then (dots) out[i] = V_i . w for i < m, or out[0] = w . w: one sweep over the basis
 template <typename T>
-hipError_t cgs_update(T* w, const T* V, long stride, const double* h, int m, long len, hipStream_t s);
+hipError_t cgs_pass(T* w, const T* V, long stride, const double* h, int m, bool dots, long len, double* partial,
+                    double* out, hipStream_t s);
 template <typename T>
 hipError_t lincomb(T* out, const T* V, long stride, const double* y, int m, long len, hipStream_t s);
 template <typename T, typename U>

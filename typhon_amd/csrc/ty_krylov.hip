@@ -113,16 +113,59 @@ __global__ __launch_bounds__(KB) void k_reduce_partials(const double* __restrict
     }
 }
 
-// w -= sum_i h_i V_i
-template <typename T>
-__global__ void k_cgs_update(T* __restrict__ w, const T* __restrict__ V, long stride, const double* __restrict__ h,
-                             int m, long len)
+// One fused classical Gram-Schmidt pass after the first multi-dot: w -= sum_i h_i V_i (as
+// a CGS update), then, from the updated w as stored, either the partial dots V_i . w for the
+// second pass (DOTS) or the partial ||w||^2.  Each element is owned by the thread k_multidot gives
+// it (same grid, same grid-stride order), so the partials equal what k_multidot would return on
+// the stored w, bit for bit, and the basis is read once instead of twice.
+template <typename T, bool DOTS>
+__global__ __launch_bounds__(KB) void k_cgs_pass(T* __restrict__ w, const T* __restrict__ V, long stride,
+                                                 const double* __restrict__ h, int m, long len,
+                                                 double* __restrict__ partial)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= len) return;
-    double s = (double)w[e];
-    for (int i = 0; i < m; ++i) s -= h[i] * (double)V[(long)i * stride + e];
-    w[e] = (T)s;
+    constexpr int NA = DOTS ? MAXK : 1;
+    __shared__ double red[KB / 64][NA];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double hr[MAXK], acc[NA];
+#pragma unroll
+    for (int i = 0; i < MAXK; ++i) hr[i] = i < m ? h[i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) acc[i] = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long)gridDim.x * blockDim.x) {
+        T vi[MAXK];
+        double s = (double)w[e];
+#pragma unroll
+        for (int i = 0; i < MAXK; ++i) {
+            if (i < m) {
+                vi[i] = V[(long)i * stride + e];
+                s -= hr[i] * (double)vi[i];
+            }
+        }
+        const T ws = (T)s;
+        w[e] = ws;
+        const double wd = (double)ws;
+        if (DOTS) {
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+                if (i < m) acc[i] += wd * (double)vi[i];
+        } else {
+            acc[0] += wd * wd;
+        }
+    }
+    const int no = DOTS ? m : 1;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        if (i < no) {
+            const double sv = wsum(acc[i]);
+            if (lane == 0) red[wid][i] = sv;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < no) {
+        double sv = 0.0;
+        for (int k = 0; k < KB / 64; ++k) sv += red[k][threadIdx.x];
+        partial[(long)blockIdx.x * MAXK + threadIdx.x] = sv;
+    }
 }
 
 // out = sum_i y_i V_i
@@ -198,9 +241,14 @@ hipError_t multidot(const T* w, const T* V, long stride, int m, long len, double
     return hipGetLastError();
 }
 template <typename T>
-hipError_t cgs_update(T* w, const T* V, long stride, const double* h, int m, long len, hipStream_t s)
+hipError_t cgs_pass(T* w, const T* V, long stride, const double* h, int m, bool dots, long len, double* partial,
+                    double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cgs_update<T>), dim3(nb_of(len)), dim3(KB), 0, s, w, V, stride, h, m, len);
+    const int nb = multidot_blocks(len);
+    if (dots) hipLaunchKernelGGL((k_cgs_pass<T, true>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
+    else hipLaunchKernelGGL((k_cgs_pass<T, false>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(dots ? m : 1), dim3(KB), 0, s, (const double*)partial, nb,
+                       dots ? m : 1, out);
     return hipGetLastError();
 }
 template <typename T>
@@ -349,7 +397,8 @@ hipError_t asum(const double* x, long len, double* partial, double* out, hipStre
     template hipError_t apply_a<T>(T*, const T*, const double*, const double*, double, long, hipStream_t); \
     template hipError_t block_apply<T>(int, const T*, const T*, T*, int, hipStream_t);                     \
     template hipError_t multidot<T>(const T*, const T*, long, int, long, double*, double*, hipStream_t);   \
-    template hipError_t cgs_update<T>(T*, const T*, long, const double*, int, long, hipStream_t);          \
+    template hipError_t cgs_pass<T>(T*, const T*, long, const double*, int, bool, long, double*, double*,   \
+                                    hipStream_t);                                                          \
     template hipError_t lincomb<T>(T*, const T*, long, const double*, int, long, hipStream_t);             \
     template hipError_t accumulate<T>(double*, const T*, long, hipStream_t);
 TY_KINST(float)
