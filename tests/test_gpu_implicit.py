@@ -150,3 +150,20 @@ def test_fp32_inner_refines_to_fp64_answer():
     assert st32["rel_res"] <= 1e-8 and st64["rel_res"] <= 1e-8
     assert st32["outer_iters"] >= 2  # FP32 inner alone cannot reach 1e-8: refinement did the rest
     assert K.rel_err(x32[:5], x64[:5]) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_krylov_buffers_regrow_between_solves():
+    """a solve with a larger basis, then the other inner precision, then the first again on one
+    context (the basis and work vectors are released and re-allocated): every solve converges and
+    repeats the first solve's answer bit for bit when its settings come back"""
+    m, s, g, u, gpu, orc, dt = _setup(SMOOTH[0])
+    b = _oracle_residual(orc, u, dt)
+    x1, st1 = gpu.linear_solve(dt, b, krylov(nkrylov=8, inner_precision=S.PREC_FP64, outer_tol=1e-9, max_outer=20))
+    x2, st2 = gpu.linear_solve(dt, b, krylov(nkrylov=24, inner_precision=S.PREC_FP64, outer_tol=1e-9, max_outer=20))
+    x3, st3 = gpu.linear_solve(dt, b, krylov(nkrylov=24, inner_precision=S.PREC_MIXED, outer_tol=1e-9, max_outer=20))
+    x4, st4 = gpu.linear_solve(dt, b, krylov(nkrylov=8, inner_precision=S.PREC_FP64, outer_tol=1e-9, max_outer=20))
+    scale = np.abs(x1).max()
+    for x in (x2, x3):
+        assert np.abs(x - x1).max() <= 1e-7 * scale
+    np.testing.assert_array_equal(x4, x1)

@@ -1210,7 +1210,7 @@ int alloc_krylov(ty_ctx* c, int m, int prec)
         if ((rc = dalloc(c, &c->k_b, len)) || (rc = dalloc(c, &c->k_x, len)) || (rc = dalloc(c, &c->k_r, len)) ||
             (rc = dalloc(c, &c->k_r0, len)) || (rc = dalloc(c, &c->k_r1, len)) || (rc = dalloc(c, &c->k_r2, len)) ||
             (rc = dalloc(c, &c->k_pd, len)) || (rc = dalloc(c, &c->k_dx, len)) || (rc = dalloc(c, &c->k_pf, len)) ||
-            (rc = dalloc(c, &c->k_part, (size_t)multidot_blocks(len) * MAXK)) || (rc = dalloc(c, &c->k_h, 3 * MAXK)))
+            (rc = dalloc(c, &c->k_part, (size_t)krylov_partials(len))) || (rc = dalloc(c, &c->k_h, 3 * MAXK)))
             return rc;
     }
     if (c->kry_ok) {
@@ -1326,8 +1326,8 @@ int gmres(ty_ctx* c, const double* rhs, const ty_krylov* k, double unorm, int* i
             }
             if ((rc = A_apply(zj, w))) return rc;
             // classical Gram-Schmidt, twice, in three sweeps over the basis: h1 = V^T w; w -= V h1
-            // with h2 = V^T w in the same sweep; w -= V h2 with ||w||^2 in the same sweep (the sums
-            // are those of separate multi-dot passes, bit for bit); one host round trip per step
+            // with h2 = V^T w in the same sweep (the dots a separate multi-dot would give, bit for
+            // bit); w -= V h2 with ||w||^2 in the same sweep; one host round trip per step
             double* dh1 = c->k_h;
             double* dh2 = c->k_h + MAXK;
             double* dnn = c->k_h + 2 * MAXK;

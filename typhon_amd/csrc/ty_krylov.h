@@ -7,6 +7,7 @@ namespace ty {
 constexpr int MAXK = 32;  // Krylov basis size limit (GMRES restart length + 1)
 
 int multidot_blocks(long len);
+long krylov_partials(long len);  // doubles of partial-sum space the reductions below need
 // stage-state records dst[c][v] = (P)(u[v][c] + a v[v][c]) for the residual kernels
 template <typename P, typename T>
 hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, hipStream_t s);

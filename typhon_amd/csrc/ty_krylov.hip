@@ -115,9 +115,10 @@ __global__ __launch_bounds__(KB) void k_reduce_partials(const double* __restrict
 
 // One fused classical Gram-Schmidt pass after the first multi-dot: w -= sum_i h_i V_i (as
 // a CGS update), then, from the updated w as stored, either the partial dots V_i . w for the
-// second pass (DOTS) or the partial ||w||^2.  Each element is owned by the thread k_multidot gives
-// it (same grid, same grid-stride order), so the partials equal what k_multidot would return on
-// the stored w, bit for bit, and the basis is read once instead of twice.
+// second pass (DOTS) or the partial ||w||^2.  The DOTS form runs on k_multidot's grid, so each
+// element is owned by the thread k_multidot gives it and the partials equal what k_multidot
+// would return on the stored w, bit for bit; the basis is read once instead of twice.  The norm
+// form runs on a larger grid (fixed, so its sum is repeatable too).
 template <typename T, bool DOTS>
 __global__ __launch_bounds__(KB) void k_cgs_pass(T* __restrict__ w, const T* __restrict__ V, long stride,
                                                  const double* __restrict__ h, int m, long len,
@@ -212,6 +213,19 @@ int multidot_blocks(long len)
     return (int)(nb < 1024 ? nb : 1024);
 }
 
+// the update + norm sweep is light on registers: a grid large enough for full occupancy
+static int norm_pass_blocks(long len)
+{
+    long nb = (len + KB - 1) / KB;
+    return (int)(nb < 8192 ? nb : 8192);
+}
+
+long krylov_partials(long len)
+{
+    const long a = multidot_blocks(len), b = norm_pass_blocks(len);
+    return (a > b ? a : b) * MAXK;
+}
+
 template <typename P, typename T>
 hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, hipStream_t s)
 {
@@ -244,7 +258,7 @@ template <typename T>
 hipError_t cgs_pass(T* w, const T* V, long stride, const double* h, int m, bool dots, long len, double* partial,
                     double* out, hipStream_t s)
 {
-    const int nb = multidot_blocks(len);
+    const int nb = dots ? multidot_blocks(len) : norm_pass_blocks(len);
     if (dots) hipLaunchKernelGGL((k_cgs_pass<T, true>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
     else hipLaunchKernelGGL((k_cgs_pass<T, false>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
     hipLaunchKernelGGL(k_reduce_partials, dim3(dots ? m : 1), dim3(KB), 0, s, (const double*)partial, nb,
