@@ -99,6 +99,7 @@ struct ty_ctx {
     void* work_qf[2] = {nullptr, nullptr};
     // matrix-free implicit solve (allocated on first use)
     bool kry_ok = false;
+    bool kry_base = false;  // the solver's fixed-size vectors (k_b .. k_h) are allocated
     int kry_m = 0, kry_prec = -1;
     double *k_b = nullptr, *k_x = nullptr, *k_r = nullptr, *k_r0 = nullptr, *k_r1 = nullptr, *k_r2 = nullptr;
     double *k_pd = nullptr, *k_part = nullptr, *k_h = nullptr, *k_dx = nullptr;
@@ -167,6 +168,7 @@ void free_state(ty_ctx* c)
     c->d_grad = c->d_qf = nullptr;
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
     c->kry_ok = false;
+    c->kry_base = false;
     c->kry_m = 0;
     c->kry_prec = -1;
     c->d_bad = c->d_err = nullptr;
@@ -833,6 +835,7 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     }
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
     c->kry_ok = false;
+    c->kry_base = false;
     c->kry_m = 0;
     c->kry_prec = -1;
     if ((rc = ensure_work(c, c->mixed ? 1 : 0))) return rc;
@@ -1206,12 +1209,13 @@ int alloc_krylov(ty_ctx* c, int m, int prec)
     if ((rc = ensure_work(c, prec)) || (rc = ensure_work(c, 0))) return rc;
     const long len = (long)c->nv * c->n;
     const size_t pb = prec ? sizeof(float) : sizeof(double);
-    if (!c->kry_ok) {
+    if (!c->kry_base) {
         if ((rc = dalloc(c, &c->k_b, len)) || (rc = dalloc(c, &c->k_x, len)) || (rc = dalloc(c, &c->k_r, len)) ||
             (rc = dalloc(c, &c->k_r0, len)) || (rc = dalloc(c, &c->k_r1, len)) || (rc = dalloc(c, &c->k_r2, len)) ||
             (rc = dalloc(c, &c->k_pd, len)) || (rc = dalloc(c, &c->k_dx, len)) || (rc = dalloc(c, &c->k_pf, len)) ||
             (rc = dalloc(c, &c->k_part, (size_t)krylov_partials(len))) || (rc = dalloc(c, &c->k_h, 3 * MAXK)))
             return rc;
+        c->kry_base = true;
     }
     if (c->kry_ok) {
         // a larger basis or another inner precision: release the previous basis and work vectors
