@@ -1,0 +1,296 @@
+"""Pins of the oracle's reconstruction and HLL fluxes against independent forms (CPU).
+
+- Venkatakrishnan's limiter function: closed-form values at chosen slope ratios, and the
+  published form of the function (Venkatakrishnan, J. Comput. Phys. 118 (1995) 120-130).
+- The limited reconstruction on uniform periodic meshes whose field varies along x only.
+  There the LSQ gradient of a cell is the central difference of its two x neighbours (the
+  normal matrix of a uniform stencil is diagonal), so each face value has a hand-derivable
+  1-D form:
+  * MUSCL (tvdgradstr_*.f90:35-58) is the textbook q_i + h/2 lim(D-/h, D+/h);
+  * MUSCL-UNS (tvdgraduns_*.f90:35-57) limits the centred slope against the face slope;
+  * Venkatakrishnan: q_f = q_c + phi g.(x_f - x_c) with eps^2 = (K h)^3, h = V^(1/dim).
+- HLLE/HLLEB (calc_flux_hlle.f90:109-143) against Toro's HLL form (Riemann Solvers 3rd ed.,
+  eq. 10.21) with the reference's wave speeds.
+Each check fails under a plausible slip in the restatement: a wrong coefficient in the
+Venkatakrishnan numerator, eps^2 = K h^3 instead of (K h)^3, the sign of the HLL
+dissipation, or the MUSCL slope limited as lim(s, D) instead of lim(2s - D, D).
+"""
+import numpy as np
+import pytest
+
+import cases as K
+from oracle import oracle as O
+from typhon_amd import mesh as M
+from typhon_amd import scheme as S
+from typhon_amd.solver import prim_to_cons
+
+G = 1.4
+
+
+# --- Venkatakrishnan's function ---------------------------------------------------------------
+
+def test_venkat_closed_form_values():
+    V = O.venkat_phi
+    # eps = 0: phi(y) at y = Delta+ / Delta- (hand-computed fractions)
+    assert abs(V(1.0, -9.0, 1.0, 0.0) - 3.0 / 4.0) < 1e-15      # y = 1
+    assert abs(V(0.5, -9.0, 1.0, 0.0) - 5.0 / 11.0) < 1e-15     # y = 1/2: 1.25 / 2.75
+    assert abs(V(1.5, -9.0, 1.0, 0.0) - 21.0 / 23.0) < 1e-15    # y = 3/2: 5.25 / 5.75
+    assert abs(V(0.25, -9.0, 1.0, 0.0) - 9.0 / 37.0) < 1e-15    # y = 1/4: 0.5625 / 2.3125
+    assert V(2.0, -9.0, 1.0, 0.0) == 1.0                        # y = 2: 8 / 8
+    assert V(4.0, -9.0, 1.0, 0.0) == 1.0                        # y = 4: 24 / 22, clipped
+    # negative extrapolation: Delta- = dmin
+    assert abs(V(9.0, -0.5, -1.0, 0.0) - 5.0 / 11.0) < 1e-15
+    assert abs(V(9.0, -2.0, -2.0, 0.0) - 3.0 / 4.0) < 1e-15
+    # eps > 0: (1 + 1 + 2) / (1 + 2 + 1 + 1) and (.25 + .25 + 1) / (.25 + 2 + .5 + .25)
+    assert abs(V(1.0, -9.0, 1.0, 1.0) - 4.0 / 5.0) < 1e-15
+    assert abs(V(0.5, -9.0, 1.0, 0.25) - 0.5) < 1e-15
+    assert abs(V(9.0, -1.0, -1.0, 2.0) - 5.0 / 6.0) < 1e-15     # (1 + 2 + 2) / (1 + 2 + 1 + 2)
+
+
+def venkat_published(dp, dm, eps2):
+    """Venkatakrishnan (1995): phi = 1/D- [(D+^2 + eps^2) D- + 2 D-^2 D+] / [D+^2 + 2 D-^2 + D+ D- + eps^2],
+    D+ the bound on the side of D-, phi = 1 for D- = 0, clipped to [0, 1]."""
+    if dm == 0.0:
+        return 1.0
+    phi = ((dp * dp + eps2) * dm + 2.0 * dm * dm * dp) / (dp * dp + 2.0 * dm * dm + dp * dm + eps2) / dm
+    return min(1.0, max(0.0, phi))
+
+
+def test_venkat_matches_published_form():
+    rng = np.random.default_rng(21)
+    for _ in range(400):
+        dmax, dmin = abs(rng.normal()), -abs(rng.normal())
+        d2 = rng.normal()
+        eps2 = abs(rng.normal()) * rng.choice([0.0, 1e-2, 1.0])
+        dp = dmax if d2 > 0 else dmin
+        assert abs(O.venkat_phi(dmax, dmin, d2, eps2) - venkat_published(dp, d2, eps2)) < 1e-14
+
+
+# --- 1-D rows on uniform periodic meshes --------------------------------------------------------
+
+def _row_mesh(dim):
+    if dim == 2:
+        m = M.periodic_quad(8, 4, 1.0, 0.5, jitter=0.0)
+        return m, 0.125, 8
+    m = M.box_hex(6, 1.5, 0.0, 1, periodic=True)
+    return m, 0.25, 6
+
+
+def _row_oracle(m, h, ncol, scheme, rho_col, ux_col, uy_col):
+    """oracle on a field that depends on the cell's x column only; returns (oracle, column of
+    every interior cell)"""
+    n = m.ncell_int
+    col = np.floor(m.ccent[:n, 0] / h + 1e-9).astype(int)
+    assert col.min() == 0 and col.max() == ncol - 1
+    vel = np.vstack([ux_col[col], uy_col[col], np.zeros(n)])
+    o = K.make_oracle(m, scheme, S.default_gas())
+    o.set_cons(prim_to_cons(rho_col[col], np.ones(n), vel))
+    o.calc_rhs(1e-3)
+    return o, col
+
+
+def _x_faces(m, col, ncol):
+    """interior faces normal to x: (face, ic1 column, ic2 column, column behind ic1, column ahead of ic2)"""
+    out = []
+    for f in range(m.nface_int):
+        if abs(abs(m.fnorm[f, 0]) - 1.0) > 1e-12:
+            continue
+        a, b = col[m.ic1[f]], col[m.ic2[f]]
+        e = b - a
+        assert abs(e) == 1
+        out.append((f, a, b, (a - e) % ncol, (b + e) % ncol))
+    return out
+
+
+# columns 5-7 hold differences below lim03's smoothness threshold (|slopes| < 0.01)
+COLS = dict(rho=np.array([1.0, 1.12, 0.97, 0.70, 1.60, 1.25, 1.2502, 1.2501]),
+            ux=np.array([0.30, 0.10, -0.20, -0.15, 0.40, 0.42, 0.05, 0.0]),
+            uy=np.array([-0.1, 0.2, 0.25, 0.1, -0.3, 0.0, 0.00004, 0.00001]))
+
+
+def _cols(ncol):
+    return {k: v[:ncol] for k, v in COLS.items()}
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lsq_gradient_is_central_difference_on_uniform_rows(dim):
+    m, h, nc = _row_mesh(dim)
+    c = _cols(nc)
+    o, col = _row_oracle(m, h, nc, S.default_scheme(method="M", limiter="X"), c["rho"], c["ux"], c["uy"])
+    g = o.grad()[:, :m.ncell_int]
+    r = c["rho"]
+    exp = (r[(col + 1) % nc] - r[(col - 1) % nc]) / (2 * h)
+    np.testing.assert_allclose(g[0], exp, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(g[1:3], 0.0, atol=1e-12)
+
+
+def _muscl_expected(method, lim, qa, qb, qbehind, qahead, h):
+    """1-D face values of the MUSCL family at the face between a (= ic1) and b on a uniform row"""
+    L = O.limiter
+    dm, dp, dpp = (qa - qbehind) / h, (qb - qa) / h, (qahead - qb) / h
+    if method in "MF":
+        # tvdgradstr: lim(2 s - D, D) with s the central slope -> lim(D-, D+) (kim3 swaps its arguments)
+        sl = L(lim, dp, dm) if lim == "K" else L(lim, dm, dp)
+        sr = L(lim, dp, dpp) if lim == "K" else L(lim, dpp, dp)
+    else:
+        # tvdgraduns: lim(s, D) with s the central slope of the side's own cell
+        cl, cr = 0.5 * (dm + dp), 0.5 * (dp + dpp)
+        sl = L(lim, dp, cl) if lim == "K" else L(lim, cl, dp)
+        sr = L(lim, dp, cr) if lim == "K" else L(lim, cr, dp)
+    return qa + 0.5 * h * sl, qb - 0.5 * h * sr
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("method", ["M", "U", "F"])
+@pytest.mark.parametrize("lim", list("MAVSKL"))
+def test_limited_muscl_on_uniform_rows(dim, method, lim):
+    m, h, nc = _row_mesh(dim)
+    c = _cols(nc)
+    o, col = _row_oracle(m, h, nc, S.default_scheme(method=method, limiter=lim), c["rho"], c["ux"], c["uy"])
+    qL, qR = o.face_states()
+    faces = _x_faces(m, col, nc)
+    assert len(faces) >= nc
+    for v, key in ((0, "rho"), (2, "ux"), (3, "uy")):
+        q = c[key]
+        for f, a, b, be, ah in faces:
+            el, er = _muscl_expected(method, lim, q[a], q[b], q[be], q[ah], h)
+            assert abs(qL[v, f] - el) < 1e-13, (key, f, qL[v, f], el)
+            assert abs(qR[v, f] - er) < 1e-13, (key, f, qR[v, f], er)
+    # pressure is uniform: both face values are the cell value on every face
+    np.testing.assert_allclose(qL[1, :m.nface_int], 1.0, atol=1e-14)
+
+
+def test_minmod_muscl_row_by_hand():
+    """one face worked by hand: columns (0.97, 0.70, 1.60, 1.25) around the face between
+    columns 3 and 4, h = 1/8, minmod: q_L = 0.70 + 1/2 minmod(-0.27, 0.90) = 0.70 (opposite
+    signs), q_R = 1.60 - 1/2 minmod(-0.35, 0.90) = 1.60"""
+    m, h, nc = _row_mesh(2)
+    c = _cols(nc)
+    o, col = _row_oracle(m, h, nc, S.default_scheme(method="M", limiter="M"), c["rho"], c["ux"], c["uy"])
+    qL, qR = o.face_states()
+    hits = 0
+    for f, a, b, _, _ in _x_faces(m, col, nc):
+        if (a, b) == (3, 4):
+            assert abs(qL[0, f] - 0.70) < 1e-14 and abs(qR[0, f] - 1.60) < 1e-14
+            hits += 1
+        if (a, b) == (1, 2):
+            # columns (1.0, 1.12, 0.97, 0.70): D- = 0.12, D+ = -0.15, D++ = -0.27
+            # q_L = 1.12 + 1/2 minmod(0.12, -0.15) = 1.12; q_R = 0.97 - 1/2 minmod(-0.27, -0.15) = 1.045
+            assert abs(qL[0, f] - 1.12) < 1e-14 and abs(qR[0, f] - 1.045) < 1e-13
+            hits += 1
+    assert hits == 2 * 4  # each pair of columns is joined by ny = 4 faces
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kv", [5.0, 0.3])
+def test_venkat_reconstruction_on_uniform_rows(dim, kv):
+    m, h, nc = _row_mesh(dim)
+    c = _cols(nc)
+    o, col = _row_oracle(m, h, nc, S.default_scheme(method="M", limiter="W", venkat_k=kv), c["rho"], c["ux"],
+                         c["uy"])
+    qL, qR = o.face_states()
+    vol = h ** dim
+    hv = vol ** (1.0 / dim)
+    eps2 = (kv * hv) ** 3
+    phi = {}
+    for key in ("rho", "ux", "uy"):
+        q = c[key]
+        ph = np.empty(nc)
+        for i in range(nc):
+            qm, qp = q[(i - 1) % nc], q[(i + 1) % nc]
+            g = (qp - qm) / (2 * h)
+            dmax, dmin = max(qm, qp, q[i]) - q[i], min(qm, qp, q[i]) - q[i]
+            # faces at +x and -x extrapolate +-g h/2; faces normal to y and z do not extrapolate
+            ph[i] = min(venkat_published(dmax if s * g > 0 else dmin, s * g * h / 2, eps2) for s in (1, -1))
+        phi[key] = ph
+    for v, key in ((0, "rho"), (2, "ux"), (3, "uy")):
+        q = c[key]
+        for f, a, b, be, ah in _x_faces(m, col, nc):
+            ga, gb = (q[b] - q[be]) / (2 * h), (q[ah] - q[a]) / (2 * h)
+            e = b - a
+            assert abs(qL[v, f] - (q[a] + phi[key][a] * ga * e * h / 2)) < 1e-13
+            assert abs(qR[v, f] - (q[b] - phi[key][b] * gb * e * h / 2)) < 1e-13
+
+
+def test_venkat_cell_by_hand():
+    """the face between columns 3 and 4 of the 2D row, h = 1/8, V = 1/64, K = 5, so
+    eps^2 = (K V^(1/2))^3 = (5/8)^3 = 0.244140625.
+    Column 3 (0.97 | 0.70 | 1.60) is the row's minimum: g h = (1.60 - 0.97)/2 = 0.315, its +x face
+    extrapolates d2 = 0.1575 against Delta+ = 0.90 (y = 5.7: phi = 1), its -x face -0.1575
+    against Delta- = 0: phi = eps^2 / (2 d2^2 + eps^2) = 0.244140625 / 0.293753125.
+    Column 4 (0.70 | 1.60 | 1.25) is the row's maximum: g h = 0.275, its -x face gives y = 6.5
+    (phi = 1), its +x face d2 = 0.1375 against Delta+ = 0: phi = 0.244140625 / 0.281953125."""
+    m, h, nc = _row_mesh(2)
+    c = _cols(nc)
+    o, col = _row_oracle(m, h, nc, S.default_scheme(method="M", limiter="W"), c["rho"], c["ux"], c["uy"])
+    qL, qR = o.face_states()
+    phi3 = 0.244140625 / 0.293753125
+    phi4 = 0.244140625 / 0.281953125
+    hits = 0
+    for f, a, b, _, _ in _x_faces(m, col, nc):
+        if (a, b) == (3, 4):
+            assert abs(qL[0, f] - (0.70 + phi3 * 0.1575)) < 1e-14
+            assert abs(qR[0, f] - (1.60 - phi4 * 0.1375)) < 1e-14
+            hits += 1
+    assert hits == 4  # ny = 4 faces join the two columns
+
+
+# --- HLLE / HLLEB against Toro's HLL form ---------------------------------------------------------
+
+def _phys_flux(q, n, g=G):
+    rho, p, u = q[0], q[1], np.asarray(q[2:5])
+    vn = u @ n
+    E = p / (g - 1) + .5 * rho * u @ u
+    return np.concatenate([[rho * vn, vn * (E + p)], rho * vn * u + p * n])
+
+
+def _cons(q, g=G):
+    rho, p, u = q[0], q[1], np.asarray(q[2:5])
+    return np.concatenate([[rho, p / (g - 1) + .5 * rho * u @ u], rho * u])
+
+
+def toro_hll(qL, qR, n, beta, g=G):
+    """F = (S_R F_L - S_L F_R + S_L S_R (U_R - U_L)) / (S_R - S_L) (Toro, eq. 10.21) with
+    S_L = min(0, u_L - beta a_L, u~ - a~), S_R = max(0, u_R + beta a_R, u~ + a~) from Roe averages
+    (Einfeldt's speeds; the zero bounds make the supersonic cases one-sided)"""
+    rl, rr = qL[0], qR[0]
+    sq = np.sqrt(rr / rl)
+    ut = (qL[2:5] + sq * qR[2:5]) / (1 + sq)
+    hl = g / (g - 1) * qL[1] / rl + .5 * qL[2:5] @ qL[2:5]
+    hr = g / (g - 1) * qR[1] / rr + .5 * qR[2:5] @ qR[2:5]
+    ht = (hl + sq * hr) / (1 + sq)
+    at = np.sqrt((g - 1) * (ht - .5 * ut @ ut))
+    ul, ur = qL[2:5] @ n, qR[2:5] @ n
+    al, ar = np.sqrt(g * qL[1] / rl), np.sqrt(g * qR[1] / rr)
+    sl = min(0.0, ul - beta * al, ut @ n - at)
+    sr = max(0.0, ur + beta * ar, ut @ n + at)
+    return (sr * _phys_flux(qL, n, g) - sl * _phys_flux(qR, n, g) + sl * sr * (_cons(qR, g) - _cons(qL, g))) / (sr - sl)
+
+
+@pytest.mark.parametrize("sch,beta", [(S.SCH_HLLE, 1.0), (S.SCH_HLLEB, np.sqrt(0.4 / 2.8))])
+def test_hlle_matches_toro_hll_form(sch, beta):
+    rng = np.random.default_rng(22)
+    for k in range(300):
+        vmax = 3.0 if k % 3 == 0 else 1.0  # some supersonic pairs
+        qL = np.concatenate([[rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0)], rng.uniform(-vmax, vmax, 3)])
+        qR = np.concatenate([[rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0)], rng.uniform(-vmax, vmax, 3)])
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        np.testing.assert_allclose(O.flux_face(qL, qR, n, sch), toro_hll(qL, qR, n, beta), rtol=1e-12, atol=1e-12)
+
+
+def test_hll_dissipation_of_a_density_jump():
+    """Stationary density jump (u = 0, p equal): the physical fluxes are pure pressure, so the mass
+    flux is all dissipation, S_L S_R / (S_R - S_L) (rho_R - rho_L) with S_L = -a~ < 0 < S_R = a~:
+    mass flux = -a~/2 (rho_R - rho_L), flowing from the dense side"""
+    qL = np.array([2.0, 1.0, 0.0, 0.0, 0.0])
+    qR = np.array([1.0, 1.0, 0.0, 0.0, 0.0])
+    n = np.array([1.0, 0.0, 0.0])
+    f = O.flux_face(qL, qR, n, S.SCH_HLLE)
+    sq = np.sqrt(0.5)
+    ht = (3.5 * 0.5 + sq * 3.5) / (1 + sq)
+    at = np.sqrt(0.4 * ht)
+    # Einfeldt bound vs the sides' own sound speeds: a_L = sqrt(0.7) < a~ < a_R = sqrt(1.4)
+    sl, sr = -max(at, np.sqrt(0.7)), max(at, np.sqrt(1.4))
+    assert abs(f[0] - sl * sr / (sr - sl) * (qR[0] - qL[0])) < 1e-14
+    assert f[0] > 0.0
