@@ -268,9 +268,8 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
 
     - throughput: FP64 and mixed (FP32 flux/gradient/stage state, FP64 U0 and RK accumulator)
       advance the same initial field by ``steps`` steps; density error of mixed vs FP64;
-    - adjoint: J = pressure force normal to the y = 0 wall after ``adj_steps`` steps (mixed
-      forward), dJ/dU0 by the device-resident checkpointed sweep (``snaps`` snapshots, FP64
-      reverse), checked against a central difference of two FP64 forward runs along a seeded
+    - adjoint: J = pressure force normal to the y = 0 wall after ``adj_steps`` FP64 steps,
+      dJ/dU0 by the device-resident checkpointed sweep (``snaps`` snapshots, FP64), checked against a central difference of two FP64 forward runs along a seeded
       direction."""
     import torch
     import cases as K
@@ -343,7 +342,7 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
     if adj_steps > 0:
         dts = np.full(adj_steps, dt)
         obj = lambda U, Sx: A.wall_force(m, U, Sx, "JMIN", axis=1)  # noqa: E731
-        solver.set_scheme(dict(s, precision=S.PREC_MIXED), g)
+        solver.set_scheme(dict(s, precision=S.PREC_FP64), g)
         t0 = time.perf_counter()
         J, gU0, _, st = A.unsteady_gradient_device(solver, m, u, dts, obj, snaps=snaps)
         el = time.perf_counter() - t0
@@ -364,9 +363,10 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
         fd = (Jpm[0] - Jpm[1]) / (2 * h)
         ad = float((gU0 * d).sum())
         out["adjoint"] = dict(objective="pressure force normal to the y = 0 wall", steps=adj_steps, snapshots=snaps,
-                              forward_precision="mixed", reverse_precision="fp64", J=J,
+                              forward_precision="fp64", reverse_precision="fp64", J=J,
                               total_s=round(el, 2), device_sweep_s=round(st["ms"] / 1e3, 2),
                               forward_steps_in_sweep=st["forward_steps"], max_snapshots=st["max_snapshots"],
+                              snapshots_allocated=st["snapshots_allocated"],
                               snapshot_gb=round(snaps * 5 * nc * 8 / 1e9, 1),
                               fd_check=dict(direction="seeded normal x field scale", h=h, dJ_adjoint=ad, dJ_fd=fd,
                                             rel_diff=abs(ad - fd) / max(abs(fd), 1e-300), fd_s=round(tfd, 1)))

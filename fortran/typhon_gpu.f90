@@ -71,7 +71,7 @@ type, bind(C) :: ty_krylov_stats
 endtype ty_krylov_stats
 
 type, bind(C) :: ty_unsteady_stats
-  integer(c_int32_t) :: forward_steps, adjoint_steps, max_snapshots
+  integer(c_int32_t) :: forward_steps, adjoint_steps, max_snapshots, snapshots_allocated
   real(c_double)     :: ms
 endtype ty_unsteady_stats
 

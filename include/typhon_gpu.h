@@ -205,14 +205,15 @@ int ty_implicit_bdlu_step(ty_ctx* ctx, double dt, int32_t nkrylov, double maxres
  * supersonic inlets (other bocos ignored).  Any input may be NULL (zero direction).
  * Viscous (Navier-Stokes) residuals are covered for du, dvol and dq; their flux also reads
  * cell and face centres, so ds must be NULL for them.
- * TY_ERR_UNSUPPORTED unless the scheme is first order. */
+ * TY_ERR_UNSUPPORTED unless the scheme is first order and FP64 (TY_PREC_FP64: a mixed-precision
+ * context does not evaluate the FP64 residual this differentiates). */
 int ty_residual_tangent(ty_ctx* ctx, double dt, const double* du, const double* ds, const double* dvol,
                         const double* dq, double* dres);
 /* Adjoint residual (no reference counterpart): the transpose of ty_residual_tangent,
  * gu = (dR/dU)^T lam, gs = (dR/dS)^T lam, gvol = (dR/dV)^T lam, gq = (dR/dq)^T lam, same
  * layouts; gs, gvol and gq may be NULL.  Far faces of merged periodic pairs get zero in gs
  * (the device uses the geometry of the pair's first face).  Viscous residuals: HLLC/HLLCB
- * fluxes, gs must be NULL. */
+ * fluxes, gs must be NULL.  FP64 schemes only, as ty_residual_tangent. */
 int ty_residual_adjoint(ty_ctx* ctx, double dt, const double* lam, double* gu, double* gs, double* gvol,
                         double* gq);
 
@@ -220,6 +221,7 @@ typedef struct ty_unsteady_stats {
     int32_t forward_steps;  /* RK steps advanced for checkpoints and recomputation */
     int32_t adjoint_steps;  /* RK steps taken in reverse (= nsteps) */
     int32_t max_snapshots;  /* checkpoint states held at once (besides the initial state) */
+    int32_t snapshots_allocated; /* checkpoint buffers allocated: min(snaps, nsteps - 1) */
     double ms;              /* wall time of the call */
 } ty_unsteady_stats;
 
@@ -230,9 +232,10 @@ typedef struct ty_unsteady_stats {
  * NULL, dJ/dq ([nboco][5], supersonic-inlet ghost states (rho, p, u, v, w), summed over all
  * stages and steps).  Checkpoints live in device memory: at most `snaps` states besides U_0,
  * placed by binomial checkpointing (Griewank's treeverse), so the forward work grows as
- * nsteps * r with C(snaps + r, snaps) >= nsteps.  Forward steps run in the context's precision;
- * each reversed step recomputes its stage states and residual adjoints in FP64.  The state is
- * U_0 again on return.  Same scheme limits as ty_residual_adjoint. */
+ * nsteps * r with C(snaps + r, snaps) >= nsteps.  The gradient is that of the FP64 discrete map:
+ * the forward recomputation and every reversed step run in FP64, also in a mixed-precision
+ * context (which then holds two extra FP64 stage-state buffers for the call).  The state is U_0
+ * again on return.  Same scheme and boundary limits as ty_residual_adjoint, any precision. */
 int ty_unsteady_adjoint(ty_ctx* ctx, int32_t nsteps, const double* dts, int32_t snaps, const double* lamN,
                         double* lam0, double* gq, ty_unsteady_stats* stats);
 /* per-kernel timing for benchmarks (no reference counterpart): when enabled, every

@@ -99,9 +99,9 @@ def test_snapshot_budget_changes_work_not_result():
             assert _rel(out["q"], ref["q"]) <= 1e-13
 
 
-def test_mixed_precision_forward_close_to_fp64():
-    """forward steps in FP32 (+FP64 accumulation), reverse in FP64: the gradient moves by the
-    state difference of the two trajectories only"""
+def test_mixed_context_gives_the_fp64_gradient():
+    """a mixed-precision context recomputes the trajectory in FP64, so its gradient is the FP64
+    map's: the same as an FP64 context's, with min(snaps, N - 1) checkpoint buffers"""
     m, s, g, U0 = _viscous(*K.bump_case(8, 4, 3))
     N = 9
     res = {}
@@ -110,7 +110,22 @@ def test_mixed_precision_forward_close_to_fp64():
         dts = _schedule(gpu, U0, N) if prec == S.PREC_FP64 else res[S.PREC_FP64][2]
         J, gU, gq, st = A.unsteady_gradient_device(gpu, m, U0, dts, lambda U, Sx: A.wall_force_x(m, U, Sx, "JMIN"),
                                                    snaps=3)
+        assert st["snapshots_allocated"] == 3 and st["max_snapshots"] <= 3
+        assert gpu.scheme["precision"] == prec
         res[prec] = (J, gU, dts, gq)
         gpu.close()
-    assert abs(res[S.PREC_MIXED][0] - res[S.PREC_FP64][0]) <= 1e-5 * abs(res[S.PREC_FP64][0])
-    assert _rel(res[S.PREC_MIXED][1], res[S.PREC_FP64][1]) <= 1e-4
+    assert abs(res[S.PREC_MIXED][0] - res[S.PREC_FP64][0]) <= 1e-14 * abs(res[S.PREC_FP64][0])
+    assert _rel(res[S.PREC_MIXED][1], res[S.PREC_FP64][1]) <= 1e-13
+    assert _rel(res[S.PREC_MIXED][3], res[S.PREC_FP64][3]) <= 1e-13
+
+
+def test_snapshot_buffers_bounded_by_steps():
+    m, s, g, U0 = K.bump_case(8, 4, 3)
+    gpu = Solver(m, s, g)
+    dts = _schedule(gpu, U0, 3)
+    lamN = np.random.default_rng(3).normal(size=(5, m.ncell_int))
+    for N, snaps, alloc in ((3, 20, 2), (1, 20, 0), (3, 1, 1)):
+        gpu.set_state(U0)
+        st = gpu.unsteady_adjoint(dts[:N], lamN, snaps)["stats"]
+        assert st["snapshots_allocated"] == alloc, (N, snaps, st)
+        assert st["max_snapshots"] <= alloc

@@ -262,3 +262,42 @@ def test_gpu_convection_walls_refused(btype):
     m.bocos[-1].type = btype
     with pytest.raises(TyError, match="convection wall"):
         Solver(m, s, g)
+
+
+@pytest.mark.gpu
+def test_gpu_failed_rescheme_keeps_previous_scheme():
+    """set_scheme validates everything before it releases anything: a refused scheme (adiabatic
+    walls under the Euler model) leaves the previous scheme and state as they were"""
+    from typhon_amd.solver import Solver
+    from typhon_amd._lib import TyError
+    m, s, g, u = couette("adiab", noise=0.01)
+    gpu = Solver(m, s, g)
+    gpu.set_state(u)
+    before = gpu.get_state()
+    with pytest.raises(TyError, match="laminar NS") as e:
+        gpu.set_scheme(s, dict(g, typ_fluid=S.EQ_EULER))
+    assert e.value.code == 4
+    assert np.array_equal(gpu.get_state(), before)
+    o = K.make_oracle(m, s, g)
+    o.set_cons(before)
+    assert K.rel_err(gpu.calc_rhs(1e-3), o.calc_rhs(1e-3)) <= TOL64
+    gpu.rk_step(1e-3)
+    o.rk_step(1e-3)
+    assert K.rel_err(gpu.get_state(), o.get_cons()) <= TOL64
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_context_refuses_residual_derivatives():
+    """the residual tangent and adjoint differentiate the FP64 residual, which a mixed-precision
+    context does not evaluate"""
+    from typhon_amd.solver import Solver
+    from typhon_amd._lib import TyError
+    m, s, g, u = K.bump_case(6, 3, 3, precision=S.PREC_MIXED)
+    gpu = Solver(m, s, g)
+    gpu.set_state(u)
+    lam = np.ones((5, m.ncell_int))
+    with pytest.raises(TyError, match="FP64 schemes only") as e:
+        gpu.residual_adjoint(1e-3, lam)
+    assert e.value.code == 4
+    with pytest.raises(TyError, match="FP64 schemes only"):
+        gpu.residual_tangent(1e-3, du=lam)

@@ -56,7 +56,7 @@ class TyKrylovStats(C.Structure):
 
 class TyUnsteadyStats(C.Structure):
     _fields_ = [("forward_steps", C.c_int32), ("adjoint_steps", C.c_int32), ("max_snapshots", C.c_int32),
-                ("ms", C.c_double)]
+                ("snapshots_allocated", C.c_int32), ("ms", C.c_double)]
 
 
 class TyError(RuntimeError):

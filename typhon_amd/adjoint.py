@@ -15,8 +15,13 @@ The reference has no derivative code; this module differentiates the path it doe
 * ``unsteady_gradient`` runs N steps forward and the adjoint backward with binomial
   checkpointing (``treeverse``: at most ``snaps`` stored states, Griewank's schedule).
 
-Scope: first-order inviscid schemes in FP64 (the library refuses others).  The time step
-of every step is an input (held fixed under differentiation), as a fixed-dt schedule.
+Scope: first-order schemes in FP64 contexts, inviscid (every flux; state, area-vector, volume
+and inlet-state gradients) and laminar Navier-Stokes (HLLC/HLLCB; state, volume and inlet-state
+gradients), with symmetry, periodic, supersonic inlet and outlet and subsonic outlet boundaries;
+the library refuses the rest.  The time step of every step is an input (held fixed under
+differentiation), as a fixed-dt schedule.  ``unsteady_gradient_device`` accepts a mixed-precision
+solver: the library then recomputes the trajectory in FP64, so the gradient is that of the FP64
+map, and the forward sweep for the objective here runs in FP64 too.
 """
 from __future__ import annotations
 
@@ -236,27 +241,25 @@ def unsteady_gradient(solver, mesh, U0, dts, coef, objective, vtx=None, q=None, 
 
     lam0 = treeverse(0, N, U0, lamN, snaps, advance, step_adjoint_q) if N > 0 else lamN
     if not geometry:
-        return float(J), lam0.numpy(), None, qgrad.detach().numpy()
+        return float(J.detach()), lam0.numpy(), None, qgrad.detach().numpy()
     if gS_obj is not None:
         gS = gS + gS_obj
     # chain S(vtx), V(vtx) -> vtx
     gvtx = torch.autograd.grad((S, V), vtx, (gS, gV), allow_unused=True)[0]
-    return float(J), lam0.numpy(), gvtx.numpy(), qgrad.detach().numpy()
+    return float(J.detach()), lam0.numpy(), gvtx.numpy(), qgrad.detach().numpy()
 
 
 def unsteady_gradient_device(solver, mesh, U0, dts, objective, snaps=20):
     """As unsteady_gradient for dJ/dU0 and dJ/dq, with the reverse sweep on the device
     (ty_unsteady_adjoint: checkpoints and stage states in device memory).  One forward sweep gives
     U_N; the objective and lam_N = dJ/dU_N are evaluated with torch on the host.  Returns
-    (J, dJ/dU0 [5, ncell_int], dJ/dq [nboco, 5], stats); the solver's state is U0 on return."""
-    solver.set_state(U0)
-    for dt in dts:
-        solver.rk_step(float(dt))
-    UN = torch.as_tensor(solver.get_state()).requires_grad_(True)
+    (J, dJ/dU0 [5, ncell_int], dJ/dq [nboco, 5], stats); the solver's state is U0 on return.
+    The objective is evaluated on the FP64 trajectory, the map the library's sweep linearises."""
+    UN = torch.as_tensor(solver.advance_fp64(U0, dts)).requires_grad_(True)
     S = torch.as_tensor(mesh.fnorm * mesh.fsurf[:, None])
     J = objective(UN, S)
     (lamN,) = torch.autograd.grad(J, (UN,), allow_unused=True)
     lamN = np.zeros(UN.shape) if lamN is None else lamN.numpy()
     solver.set_state(U0)
     out = solver.unsteady_adjoint(np.asarray(dts, dtype=np.float64), lamN, snaps)
-    return float(J), out["u"], out["q"], out["stats"]
+    return float(J.detach()), out["u"], out["q"], out["stats"]

@@ -172,11 +172,10 @@ void free_state(ty_ctx* c)
     c->kry_m = 0;
     c->kry_prec = -1;
     c->d_bad = c->d_err = nullptr;
-    for (auto* v : {&c->ev_grad, &c->ev_flux, &c->ev_dt, &c->ev_tan, &c->ev_adj})
+    for (auto* v : {&c->ev_grad, &c->ev_flux, &c->ev_dt, &c->ev_tan, &c->ev_adj}) {
         for (auto& e : *v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
-    c->ev_grad.clear();
-    c->ev_flux.clear();
-    c->ev_dt.clear();
+        v->clear();
+    }
 }
 
 template <typename T>
@@ -707,9 +706,28 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     if ((s->method != 'N' || viscous) && s->gradmeth != 10 && s->gradmeth != 12)
         return fail(c, TY_ERR_UNSUPPORTED, "gradient method (LSQ or W-LSQ required)");
     if (s->precision != TY_PREC_FP64 && s->precision != TY_PREC_MIXED) return fail(c, TY_ERR_ARG, "precision");
+    for (const ty_boco_desc& d : c->bocos)
+        if ((d.type == 20 || d.type == 22) && !viscous)
+            return fail(c, TY_ERR_UNSUPPORTED, "flux and adiabatic walls need the laminar NS model "
+                                               "(their ghost state and ns_bocoflux use the conductivity)");
     HIPCHK(c, hipSetDevice(c->device));
-    // keep the mesh buffers, drop the rest
+    // From here on the context holds no scheme and no state until the rebuild below completes:
+    // a failure part-way leaves scheme_ok / state_ok false and no pointer to a freed buffer.
+    c->scheme_ok = c->state_ok = false;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->d_boco = nullptr;
+    c->d_geo = nullptr;
+    c->geo_d = Geo<double>{};
+    c->geo_f = Geo<float>{};
+    c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
+    c->d_us[0] = c->d_us[1] = nullptr;
+    c->d_grad = c->d_qf = nullptr;
+    c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
+    c->kry_ok = c->kry_base = false;
+    c->kry_m = 0;
+    c->kry_prec = -1;
+    c->d_bad = c->d_err = nullptr;
+    // keep the mesh buffers, drop the rest
     {
         std::vector<void*> keep;
         for (void* p : c->owned) {
@@ -783,9 +801,6 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
         case 20:  // adiabatic wall: a flux wall with zero flux (def_boco_ns.f90:44-49)
         case 22:
         case 21:
-            if (d.type != 21 && !viscous)
-                return fail(c, TY_ERR_UNSUPPORTED, "flux and adiabatic walls need the laminar NS model "
-                                                   "(their ghost state and ns_bocoflux use the conductivity)");
             t.kind = d.type == 21 ? BK_WALL_ISOTH : BK_WALL_FLUX;
             t.param[0] = d.type == 20 ? 0.0 : d.param[0];
             for (int k = 4; k < 7; ++k) t.param[k] = d.param[k];
@@ -833,11 +848,6 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
         c->owned.push_back(p);
         c->d_us[k] = p;
     }
-    c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
-    c->kry_ok = false;
-    c->kry_base = false;
-    c->kry_m = 0;
-    c->kry_prec = -1;
     if ((rc = ensure_work(c, c->mixed ? 1 : 0))) return rc;
     c->d_grad = c->work_grad[c->mixed ? 1 : 0];
     c->d_qf = c->work_qf[c->mixed ? 1 : 0];
@@ -1678,6 +1688,9 @@ int ty_residual_tangent(ty_ctx* c, double dt, const double* du, const double* ds
     DevTmp tmp(c);
     if (c->ph.viscous && ds)
         return fail(c, TY_ERR_UNSUPPORTED, "tangent of the viscous residual: area-vector directions not provided");
+    if (c->mixed)
+        return fail(c, TY_ERR_UNSUPPORTED, "tangent residual: FP64 schemes only (it differentiates the FP64 residual, "
+                                           "which a mixed-precision context does not evaluate)");
     if ((rc = ad_prepare(c, dt, t, tmp))) return rc;
     const int n = c->n, nf = c->nf, nv = c->nv;
     double *d_du = nullptr, *d_ds = nullptr, *d_dv = nullptr, *d_dq = nullptr, *d_out = nullptr;
@@ -1740,6 +1753,9 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
         return fail(c, TY_ERR_UNSUPPORTED, "adjoint of the viscous residual: area-vector sensitivities not provided");
     if (c->ph.viscous && c->ph.sch_hyp != 25 && c->ph.sch_hyp != 26)
         return fail(c, TY_ERR_UNSUPPORTED, "adjoint of the viscous residual: HLLC/HLLCB fluxes only");
+    if (c->mixed)
+        return fail(c, TY_ERR_UNSUPPORTED, "adjoint residual: FP64 schemes only (it transposes the FP64 residual's "
+                                           "derivative, which a mixed-precision context does not evaluate)");
     if ((rc = ad_prepare(c, dt, t, tmp))) return rc;
     const int n = c->n, nf = c->nf, nb = c->nb, nv = c->nv;
     double *d_lam = nullptr, *d_gu = nullptr, *d_gs = nullptr, *d_gv = nullptr, *d_gq = nullptr;
@@ -1836,6 +1852,7 @@ struct Unsteady {
     double* rec = nullptr;       // [len] stage state as residual-kernel records
     double* gq_tmp = nullptr;    // [nv][nb] inlet sensitivities of one residual adjoint
     double* gq_acc = nullptr;    // [nv][nb] summed
+    void* us64[2] = {nullptr, nullptr};  // FP64 stage-state records of a mixed-precision context
     AdArgs t;
     int live = 0, fwd = 0, adj = 0, maxlive = 0;
 
@@ -1847,17 +1864,24 @@ struct Unsteady {
         return TY_OK;
     }
 
-    // U <- state after step k, by the context's own RK path
+    // U <- state after step k by the FP64 RK path (the map the reverse sweep linearises; a
+    // mixed-precision context runs it on FP64 stage records of its own)
     int advance(double* U, int k)
     {
         int rc;
         if ((rc = set_dt(k))) return rc;
         HIPCHK(c, hipMemcpyAsync(c->d_u0, U, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        void* keep[2] = {c->d_us[0], c->d_us[1]};
+        if (c->mixed) {
+            c->d_us[0] = us64[0];
+            c->d_us[1] = us64[1];
+        }
         c->cur = 0;
-        hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[0], c->nv, c->n, c->stream)
-                                : launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], c->nv, c->n, c->stream);
+        hipError_t e = launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], c->nv, c->n, c->stream);
+        if (e == hipSuccess) e = c->dim == 2 ? rk_stages<2, double>(c, c->geo_d) : rk_stages<3, double>(c, c->geo_d);
+        c->d_us[0] = keep[0];
+        c->d_us[1] = keep[1];
         HIPCHK(c, e);
-        if ((rc = run_step(c))) return rc;
         HIPCHK(c, hipMemcpyAsync(U, c->d_u0, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         ++fwd;
         return TY_OK;
@@ -1958,6 +1982,9 @@ struct Unsteady {
             return TY_OK;
         }
         const int m = a + tv_split(b - a, snaps);
+        // live <= min(snaps of the call, nsteps - 1) = pool.size(): each level holds one more
+        // snapshot with one snapshot less and at least one step less to cover
+        if (live >= (int)pool.size()) return fail(c, TY_ERR_ARG, "unsteady adjoint: snapshot pool exhausted");
         double* Um = pool[live++];
         maxlive = std::max(maxlive, live);
         HIPCHK(c, hipMemcpyAsync(Um, Ua, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -1984,7 +2011,7 @@ int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t sn
     Unsteady u;
     int rc;
     if ((rc = ad_prepare(c, nsteps > 0 ? dts[0] : 0.0, u.t, tmp))) return rc;
-    if ((rc = ensure_work(c, c->mixed ? 1 : 0))) return rc;
+    if ((rc = ensure_work(c, 0))) return rc;
     u.c = c;
     u.ns = c->sch.nstage;
     const int n = c->n, nv = c->nv, nb = c->nb, nf = c->nf;
@@ -1997,9 +2024,14 @@ int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t sn
         (rc = tmp.alloc(&u.nu, (size_t)u.ns * len)) || (rc = tmp.alloc(&u.mu, len)) || (rc = tmp.alloc(&u.rec, len)) ||
         (rc = tmp.alloc(&u.gq_tmp, (size_t)nv * std::max(1, nb))))
         return rc;
-    u.pool.assign(snaps, nullptr);
-    for (int k = 0; k < snaps; ++k)
+    // treeverse holds at most min(snaps, nsteps - 1) snapshots at once
+    const int nsnap = std::min(snaps, std::max(0, nsteps - 1));
+    u.pool.assign(nsnap, nullptr);
+    for (int k = 0; k < nsnap; ++k)
         if ((rc = tmp.alloc(&u.pool[k], len))) return rc;
+    if (c->mixed)
+        for (int k = 0; k < 2; ++k)
+            if ((rc = tmp.alloc((double**)&u.us64[k], len))) return rc;
     if (gq) {
         if ((rc = tmp.alloc(&u.gq_acc, (size_t)nv * std::max(1, nb)))) return rc;
         HIPCHK(c, hipMemsetAsync(u.gq_acc, 0, (size_t)nv * std::max(1, nb) * sizeof(double), c->stream));
@@ -2047,6 +2079,7 @@ int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t sn
         stats->forward_steps = u.fwd;
         stats->adjoint_steps = u.adj;
         stats->max_snapshots = u.maxlive;
+        stats->snapshots_allocated = nsnap;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return TY_OK;

@@ -20,7 +20,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .scheme import EQ_NSLAM, default_gas, default_scheme, rk_tableau
+from .scheme import EQ_NSLAM, PREC_FP64, default_gas, default_scheme, rk_tableau
 
 
 def _p64(a):
@@ -131,6 +131,22 @@ class Solver:
     def rk_step(self, dt):
         self._check(self.lib.ty_rk_step(self.h, dt), "ty_rk_step")
 
+    def advance_fp64(self, U0, dts):
+        """state [5, ncell_int] after len(dts) FP64 RK steps from U0 with the given time steps; a
+        mixed-precision solver takes its FP64 form for the sweep and returns to its own form.  The
+        solver's state is U0 on return."""
+        form = self.scheme["precision"]
+        if form != PREC_FP64:
+            self.set_scheme(dict(self.scheme, precision=PREC_FP64), self.gas)
+        self.set_state(U0)
+        for dt in dts:
+            self.rk_step(float(dt))
+        UN = self.get_state()
+        if form != PREC_FP64:
+            self.set_scheme(dict(self.scheme, precision=form), self.gas)
+        self.set_state(U0)
+        return UN
+
     def advance(self, nsteps, dtmax=np.inf, want_dt=False):
         if want_dt:
             dt = C.c_double()
@@ -207,7 +223,7 @@ class Solver:
         self._check(self.lib.ty_unsteady_adjoint(self.h, dts.shape[0], _p64(dts), snaps, _p64(lamN), _p64(lam0),
                                                  _p64(gq), C.byref(st)), "ty_unsteady_adjoint")
         stats = dict(forward_steps=st.forward_steps, adjoint_steps=st.adjoint_steps,
-                     max_snapshots=st.max_snapshots, ms=st.ms)
+                     max_snapshots=st.max_snapshots, snapshots_allocated=st.snapshots_allocated, ms=st.ms)
         return dict(u=lam0, q=gq[:len(m.bocos)], stats=stats)
 
     def linear_solve(self, dt, b, kry=None, allow_unconverged=False):
