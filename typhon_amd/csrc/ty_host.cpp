@@ -877,8 +877,8 @@ int ty_state_set(ty_ctx* c, const double* rho, const double* rhoE, const double*
     }
     HIPCHK(c, hipMemcpyAsync(c->d_u0, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     c->cur = 0;
-    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[0], nv, n, c->stream)
-                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], nv, n, c->stream);
+    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[0], nv, n, c->gas.gamma, c->stream)
+                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], nv, n, c->gas.gamma, c->stream);
     HIPCHK(c, e);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->state_ok = true;
@@ -1304,7 +1304,7 @@ int gmres(ty_ctx* c, const double* rhs, const ty_krylov* k, double unorm, int* i
             return TY_OK;
         }
         const double eps = sq * (1.0 + unorm) / zn;
-        HIPCHK(c, (fill_state<P, P>(pert, c->d_u0, eps, zin, c->nv, c->n, c->stream)));
+        HIPCHK(c, (fill_state<P, P>(pert, c->d_u0, eps, zin, c->nv, c->n, c->gas.gamma, c->stream)));
         HIPCHK(c, eval_residual<P>(c, pert, c->k_r1, nullptr, nullptr));
         HIPCHK(c, apply_a<P>(wout, zin, c->k_r1, c->k_r0, 1.0 / eps, len, c->stream));
         return TY_OK;
@@ -1320,7 +1320,7 @@ int gmres(ty_ctx* c, const double* rhs, const ty_krylov* k, double unorm, int* i
             if ((rc = dot_host<double>(c, c->k_dx, c->k_dx, len, &dd))) return rc;
             const double dn = std::sqrt(dd / (double)len);
             const double eps = sq * (1.0 + unorm) / std::max(dn, 1e-300);
-            HIPCHK(c, (fill_state<P, double>(pert, c->d_u0, eps, c->k_dx, c->nv, c->n, c->stream)));
+            HIPCHK(c, (fill_state<P, double>(pert, c->d_u0, eps, c->k_dx, c->nv, c->n, c->gas.gamma, c->stream)));
             HIPCHK(c, eval_residual<P>(c, pert, c->k_r1, nullptr, nullptr));
             HIPCHK(c, outer_residual(c->k_r2, rhs, c->k_dx, c->k_r1, c->k_r0, 1.0 / eps, len, c->stream));
             if ((rc = dot_host<double>(c, c->k_r2, c->k_r2, len, &bb))) return rc;
@@ -1416,10 +1416,10 @@ int solve_dev(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* st)
     const double bnorm = std::sqrt(bb);
     // inner-precision residual at U (base of the one-sided JVP) and the preconditioner
     if (prec) {
-        HIPCHK(c, (fill_state<float, double>(c->k_pf, c->d_u0, 0.0, c->d_u0, c->nv, c->n, c->stream)));
+        HIPCHK(c, (fill_state<float, double>(c->k_pf, c->d_u0, 0.0, c->d_u0, c->nv, c->n, c->gas.gamma, c->stream)));
         HIPCHK(c, eval_residual<float>(c, c->k_pf, c->k_r0, k->precond ? (float*)c->k_dinv : nullptr, nullptr));
     } else {
-        HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, 0.0, c->d_u0, c->nv, c->n, c->stream)));
+        HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, 0.0, c->d_u0, c->nv, c->n, c->gas.gamma, c->stream)));
         HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r0, k->precond ? (double*)c->k_dinv : nullptr, nullptr));
     }
     HIPCHK(c, hipMemsetAsync(c->k_x, 0, len * sizeof(double), c->stream));
@@ -1437,9 +1437,9 @@ int solve_dev(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* st)
             if ((rc = dot_host<double>(c, c->k_x, c->k_x, len, &xx))) return rc;
             const double xn = std::sqrt(xx / (double)len);
             const double eps = cb * (1.0 + unorm) / std::max(xn, 1e-300);
-            HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, eps, c->k_x, c->nv, c->n, c->stream)));
+            HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, eps, c->k_x, c->nv, c->n, c->gas.gamma, c->stream)));
             HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r1, nullptr, nullptr));
-            HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, -eps, c->k_x, c->nv, c->n, c->stream)));
+            HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, -eps, c->k_x, c->nv, c->n, c->gas.gamma, c->stream)));
             HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r2, nullptr, nullptr));
             HIPCHK(c, outer_residual(c->k_r, c->k_b, c->k_x, c->k_r1, c->k_r2, 0.5 / eps, len, c->stream));
         }
@@ -1500,7 +1500,7 @@ int ty_jacobian_diag(ty_ctx* c, double dt, double* diag)
     c->h_dt = dt;
     HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
     if ((rc = ensure_work(c, 0))) return rc;
-    HIPCHK(c, launch_to_stage<double>(c->d_u0, d_st, nv, n, c->stream));
+    HIPCHK(c, launch_to_stage<double>(c->d_u0, d_st, nv, n, c->gas.gamma, c->stream));
     HIPCHK(c, eval_residual<double>(c, d_st, c->d_res, d_inv, d_diag));
     std::vector<double> h((size_t)nv * nv * n);
     HIPCHK(c, hipMemcpyAsync(h.data(), d_diag, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1538,9 +1538,9 @@ int ty_jvp(ty_ctx* c, double dt, const double* v, double* jv)
         return rc;
     const double unorm = std::sqrt(uu / (double)len), xn = std::sqrt(xx / (double)len);
     const double eps = std::cbrt(2.220446049250313e-16) * (1.0 + unorm) / std::max(xn, 1e-300);
-    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, eps, c->k_x, c->nv, c->n, c->stream)));
+    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, eps, c->k_x, c->nv, c->n, c->gas.gamma, c->stream)));
     HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r1, nullptr, nullptr));
-    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, -eps, c->k_x, c->nv, c->n, c->stream)));
+    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, -eps, c->k_x, c->nv, c->n, c->gas.gamma, c->stream)));
     HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_r2, nullptr, nullptr));
     // Jv = (r1 - r2) / (2 eps): reuse outer_residual with b = 0 and x = 0 -> r = (r1 - r2) s
     HIPCHK(c, hipMemsetAsync(c->k_b, 0, len * sizeof(double), c->stream));
@@ -1583,14 +1583,14 @@ int ty_implicit_step(ty_ctx* c, double dt, const ty_krylov* k, ty_krylov_stats* 
     c->h_dt = dt;
     HIPCHK(c, hipMemcpyAsync(c->d_dt, &c->h_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
     // b = R(U) in FP64 (calc_rhs.f90:119-145)
-    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, 0.0, c->d_u0, c->nv, c->n, c->stream)));
+    HIPCHK(c, (fill_state<double, double>(c->k_pd, c->d_u0, 0.0, c->d_u0, c->nv, c->n, c->gas.gamma, c->stream)));
     HIPCHK(c, eval_residual<double>(c, c->k_pd, c->k_b, nullptr, nullptr));
     const int rs = solve_dev(c, dt, k, st);
     if (rs && rs != TY_ERR_NOT_CONVERGED) return rs;
     // update_field.f90:38: U += dU, then refresh the stage-state copy
     HIPCHK(c, accumulate<double>(c->d_u0, c->k_x, len, c->stream));
-    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[c->cur], c->nv, c->n, c->stream)
-                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[c->cur], c->nv, c->n, c->stream);
+    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[c->cur], c->nv, c->n, c->gas.gamma, c->stream)
+                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[c->cur], c->nv, c->n, c->gas.gamma, c->stream);
     HIPCHK(c, e);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (rs) return fail(c, rs, "implicit solve did not reach the requested tolerance");
@@ -1666,7 +1666,7 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
         int rc;
         double* d_st = nullptr;
         if ((rc = ensure_work(c, 0)) || (rc = tmp.alloc(&d_st, (size_t)c->nv * c->n))) return rc;
-        HIPCHK(c, launch_to_stage<double>(c->d_u0, d_st, c->nv, c->n, c->stream));
+        HIPCHK(c, launch_to_stage<double>(c->d_u0, d_st, c->nv, c->n, c->gas.gamma, c->stream));
         KArgs<double> a = make_args<double>(c, c->geo_d);
         a.us = d_st;
         a.qf_out = nullptr;
@@ -1877,7 +1877,7 @@ struct Unsteady {
             c->d_us[1] = us64[1];
         }
         c->cur = 0;
-        hipError_t e = launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], c->nv, c->n, c->stream);
+        hipError_t e = launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], c->nv, c->n, c->gas.gamma, c->stream);
         if (e == hipSuccess) e = c->dim == 2 ? rk_stages<2, double>(c, c->geo_d) : rk_stages<3, double>(c, c->geo_d);
         c->d_us[0] = keep[0];
         c->d_us[1] = keep[1];
@@ -1893,7 +1893,7 @@ struct Unsteady {
         AdArgs a = t;
         a.k.us = U;
         if (c->ph.viscous) {
-            HIPCHK(c, launch_to_stage<double>(U, rec, c->nv, c->n, c->stream));
+            HIPCHK(c, launch_to_stage<double>(U, rec, c->nv, c->n, c->gas.gamma, c->stream));
             KArgs<double> g = make_args<double>(c, c->geo_d);
             g.us = rec;
             g.qf_out = nullptr;
@@ -1919,7 +1919,7 @@ struct Unsteady {
         const double* Us[MAX_STAGES];
         Us[0] = Uk;
         for (int s = 1; s <= ns; ++s) {
-            HIPCHK(c, launch_to_stage<double>(Us[s - 1], rec, c->nv, c->n, c->stream));
+            HIPCHK(c, launch_to_stage<double>(Us[s - 1], rec, c->nv, c->n, c->gas.gamma, c->stream));
             HIPCHK(c, eval_residual<double>(c, rec, stR + (size_t)(s - 1) * len, nullptr, nullptr));
             if (s == ns) break;
             Comb cb;
@@ -2061,8 +2061,8 @@ int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t sn
     // the state is U_0 again
     HIPCHK(c, hipMemcpyAsync(c->d_u0, U0, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     c->cur = 0;
-    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[0], nv, n, c->stream)
-                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], nv, n, c->stream);
+    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[0], nv, n, c->gas.gamma, c->stream)
+                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], nv, n, c->gas.gamma, c->stream);
     HIPCHK(c, e);
     HIPCHK(c, hipMemcpyAsync(h.data(), lam, len * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     std::vector<double> hq(gq ? (size_t)nv * nb : 0);
@@ -2115,7 +2115,7 @@ int ty_implicit_bdlu_step(ty_ctx* c, double dt, int32_t nkrylov, double maxres, 
     HIPCHK(c, hipMemsetAsync(p1, 0, (size_t)L * sizeof(double), c->stream));
     // residual of every unknown (interior rows -dt/V sum F.A; ghost rows their boundary-flux sums),
     // diagonal blocks with ns_bocojacobian, per-slot coupling blocks
-    HIPCHK(c, (fill_state<double, double>(stage, c->d_u0, 0.0, c->d_u0, nv, n, c->stream)));
+    HIPCHK(c, (fill_state<double, double>(stage, c->d_u0, 0.0, c->d_u0, nv, n, c->gas.gamma, c->stream)));
     KArgs<double> a = make_args<double>(c, c->geo_d);
     a.us = stage;
     a.mode = 1;
@@ -2210,8 +2210,8 @@ int ty_implicit_bdlu_step(ty_ctx* c, double dt, int32_t nkrylov, double maxres, 
     // update_field.f90:38 on the interior unknowns, then the stage-state copy
     for (int v = 0; v < nv; ++v)
         HIPCHK(c, accumulate<double>(c->d_u0 + (size_t)v * n, sol + (size_t)v * N, n, c->stream));
-    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[c->cur], nv, n, c->stream)
-                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[c->cur], nv, n, c->stream);
+    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[c->cur], nv, n, c->gas.gamma, c->stream)
+                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[c->cur], nv, n, c->gas.gamma, c->stream);
     HIPCHK(c, e);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return TY_OK;

@@ -85,7 +85,8 @@ struct KArgs {
     const double* dtfac;  // [n] 2 V / sum(A)
     Physics ph;
     // state
-    const R* us;          // [n][nv] stage state (conservative), one record per cell
+    const R* us;          // [n][nv] stage state as primitive records (rho, p, u..), one per cell,
+                          // recovered from the FP64 conservative state by its producer
     R* us_next;           // [n][nv]
     double* u0;           // [nv][n]
     double* acc;          // [nv][n]
@@ -116,6 +117,25 @@ struct KArgs {
     int read_slot[MAX_STAGES];// rstore slot of R_j, j < stage (-1: not stored / zero coefficient)
 };
 
+// calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:47-52) in FP64 on one conservative state
+// u = (rho, rhoE, rho u..): the stage-state record (rho, p, u..) in the arithmetic precision R.
+// Every producer of stage records (the RK stage update, the state upload, the perturbed states
+// of the Krylov products) recovers pressure from total energy here, in FP64, once per cell.
+template <typename R>
+__device__ __forceinline__ void store_prim(const double* u, int nv, double g1, R* out)
+{
+    const double rho = u[0];
+    const double ir = 1.0 / rho;
+    double sq = 0.0;
+    for (int d = 0; d < nv - 2; ++d) {
+        const double vd = u[2 + d] * ir;
+        sq = d == 0 ? vd * vd : sq + vd * vd;
+        out[2 + d] = (R)vd;
+    }
+    out[0] = (R)rho;
+    out[1] = (R)(g1 * (u[1] - .5 * rho * sq));
+}
+
 struct DtArgs {
     int n, nblk_real;
     const double* u0;
@@ -132,9 +152,9 @@ template <int DIM, typename R>
 hipError_t launch_flux(const KArgs<R>& a, hipStream_t s);
 template <int DIM>
 hipError_t launch_timestep(const DtArgs& a, double dtmax, double* d_dt, int* d_err, hipStream_t s);
-// stage state records [n][nv] from a [nv][n] FP64 state
+// stage-state primitive records [n][nv] from a [nv][n] FP64 conservative state
 template <typename R>
-hipError_t launch_to_stage(const double* src, R* dst, int nv, int n, hipStream_t s);
+hipError_t launch_to_stage(const double* src, R* dst, int nv, int n, double gamma, hipStream_t s);
 
 int grid_blocks(int n);  // XCD-padded block count
 

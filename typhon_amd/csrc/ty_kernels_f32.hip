@@ -8,6 +8,6 @@ namespace ty {
     template hipError_t launch_flux<DIM, R>(const KArgs<R>&, hipStream_t);
 TY_INST(2, float)
 TY_INST(3, float)
-template hipError_t launch_to_stage<float>(const double*, float*, int, int, hipStream_t);
+template hipError_t launch_to_stage<float>(const double*, float*, int, int, double, hipStream_t);
 
 }  // namespace ty

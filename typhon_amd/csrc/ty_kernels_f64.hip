@@ -17,6 +17,6 @@ TY_INST(2, double)
 TY_INST(3, double)
 template hipError_t launch_timestep<2>(const DtArgs&, double, double*, int*, hipStream_t);
 template hipError_t launch_timestep<3>(const DtArgs&, double, double*, int*, hipStream_t);
-template hipError_t launch_to_stage<double>(const double*, double*, int, int, hipStream_t);
+template hipError_t launch_to_stage<double>(const double*, double*, int, int, double, hipStream_t);
 
 }  // namespace ty

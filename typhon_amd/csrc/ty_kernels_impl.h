@@ -49,33 +49,46 @@ __device__ __forceinline__ int xcd_block()
     return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
-// calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:47-52) on the cell's record; ir = 1/rho
+// store_prim (ty_internal.h) with the state's dimension known at compile time
 template <int DIM, typename R>
-__device__ __forceinline__ void load_prim(const R* __restrict__ us, int c, R g1, R* q, R& ir)
+__device__ __forceinline__ void store_prim_n(const double* u, double g1, R* out)
+{
+    constexpr int NV = DIM + 2;
+    const double rho = u[0];
+    const double ir = 1.0 / rho;
+    double vel[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) vel[d] = u[2 + d] * ir;
+    double sq = vel[0] * vel[0];
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) sq += vel[d] * vel[d];
+    R q[NV];
+    q[0] = (R)rho;
+    q[1] = (R)(g1 * (u[1] - .5 * rho * sq));
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) q[2 + d] = (R)vel[d];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[v] = q[v];
+}
+
+// the cell's primitive record (rho, p, u..) and ir = 1/rho
+template <int DIM, typename R>
+__device__ __forceinline__ void load_prim(const R* __restrict__ us, int c, R* q, R& ir)
 {
     constexpr int NV = DIM + 2;
     const R* u = us + (size_t)c * NV;
-    const R rho = u[0];
-    const R rhoE = u[1];
-    ir = arcp(rho);
-    R vel[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) vel[d] = u[2 + d] * ir;
-    R sq = vel[0] * vel[0];
-#pragma unroll
-    for (int d = 1; d < DIM; ++d) sq += vel[d] * vel[d];
-    const R ec = R(.5) * rho * sq;
-    q[0] = rho;
-    q[1] = g1 * (rhoE - ec);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) q[2 + d] = vel[d];
+    for (int v = 0; v < NV; ++v) q[v] = u[v];
+    ir = arcp(q[0]);
 }
 
 template <int DIM, typename R>
-__device__ __forceinline__ void load_prim(const R* __restrict__ us, int c, R g1, R* q)
+__device__ __forceinline__ void load_prim(const R* __restrict__ us, int c, R* q)
 {
-    R ir;
-    load_prim<DIM>(us, c, g1, q, ir);
+    constexpr int NV = DIM + 2;
+    const R* u = us + (size_t)c * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) q[v] = u[v];
 }
 
 template <int DIM, typename R>
@@ -553,7 +566,7 @@ template <int DIM, typename R>
 __device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int L, const R* qc, R* qo, R& ir)
 {
     if (!link_boundary(L)) {
-        load_prim<DIM>(a.us, link_index(L), R(a.ph.gamma - 1.0), qo, ir);
+        load_prim<DIM>(a.us, link_index(L), qo, ir);
     } else {
         ghost_state<DIM>(a, k, c, L, qc, qo);
         ir = arcp(qo[0]);
@@ -564,7 +577,7 @@ template <int DIM, typename R>
 __device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int L, const R* qc, R* qo)
 {
     if (!link_boundary(L)) {
-        load_prim<DIM>(a.us, link_index(L), R(a.ph.gamma - 1.0), qo);
+        load_prim<DIM>(a.us, link_index(L), qo);
     } else {
         ghost_state<DIM>(a, k, c, L, qc, qo);
     }
@@ -628,9 +641,8 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n) return;
     const int n = a.n;
-    const R g1 = R(a.ph.gamma - 1.0);
     R qc[NV];
-    load_prim<DIM>(a.us, c, g1, qc);
+    load_prim<DIM>(a.us, c, qc);
     R gr[NV * DIM];
     R qmax[NV], qmin[NV];
 #pragma unroll
@@ -1062,10 +1074,9 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n) return;
     const int n = a.n;
-    const R g1 = R(a.ph.gamma - 1.0);
     const R gam = R(a.ph.gamma);
     R qc[NV], irc = R(0);
-    if (!FS || VISC) load_prim<DIM>(a.us, c, g1, qc, irc);
+    if (!FS || VISC) load_prim<DIM>(a.us, c, qc, irc);
     double D[JAC ? NV * NV : 1];
     const double kdt = JAC ? *a.d_dt / a.vol[c] : 0.0;
     if (JAC) {
@@ -1277,7 +1288,8 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
 #pragma unroll
         for (int v = 0; v < NV; ++v) rs[(size_t)v * n + c] = Rv[v];
     }
-    R* un = a.us_next + (size_t)c * NV;
+    // the next stage state in FP64, stored as its primitive record (pressure recovered in FP64)
+    double un[NV];
     if (a.stage < a.nstage) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) a.acc[(size_t)v * n + c] = acc[v];
@@ -1288,18 +1300,17 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
                 const int sl = a.read_slot[j];
                 if (sl >= 0) u = u + a.a[j] * a.rstore[((size_t)sl * NV + v) * n + c];
             }
-            u = u + a.a[a.stage - 1] * Rv[v];
-            un[v] = (R)u;
+            un[v] = u + a.a[a.stage - 1] * Rv[v];
         }
     } else {
         // update_field.f90:38: U = U0 + residu
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const double u = a.u0[(size_t)v * n + c] + acc[v];
-            a.u0[(size_t)v * n + c] = u;
-            un[v] = (R)u;
+            un[v] = a.u0[(size_t)v * n + c] + acc[v];
+            a.u0[(size_t)v * n + c] = un[v];
         }
     }
+    store_prim_n<DIM>(un, a.ph.gamma - 1.0, a.us_next + (size_t)c * NV);
 }
 
 // ---------------------------------------------------------------------------
@@ -1380,13 +1391,15 @@ static __global__ __launch_bounds__(1024) void k_dt_final(const double* part, co
     }
 }
 
-// stage-state records from a [nv][n] FP64 state
+// stage-state (primitive) records from a [nv][n] FP64 conservative state
 template <typename R>
-__global__ void k_to_stage(const double* __restrict__ src, R* __restrict__ dst, int nv, int n)
+__global__ void k_to_stage(const double* __restrict__ src, R* __restrict__ dst, int nv, int n, double g1)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    for (int v = 0; v < nv; ++v) dst[(size_t)c * nv + v] = (R)src[(size_t)v * n + c];
+    double u[NVMAX];
+    for (int v = 0; v < nv; ++v) u[v] = src[(size_t)v * n + c];
+    store_prim<R>(u, nv, g1, dst + (size_t)c * nv);
 }
 
 // ---------------------------------------------------------------------------
@@ -1454,10 +1467,10 @@ hipError_t launch_timestep(const DtArgs& a, double dtmax, double* d_dt, int* d_e
     return hipGetLastError();
 }
 template <typename R>
-hipError_t launch_to_stage(const double* src, R* dst, int nv, int n, hipStream_t s)
+hipError_t launch_to_stage(const double* src, R* dst, int nv, int n, double gamma, hipStream_t s)
 {
     const int nb = (n + 255) / 256;
-    hipLaunchKernelGGL((k_to_stage<R>), dim3(nb), dim3(256), 0, s, src, dst, nv, n);
+    hipLaunchKernelGGL((k_to_stage<R>), dim3(nb), dim3(256), 0, s, src, dst, nv, n, gamma - 1.0);
     return hipGetLastError();
 }
 

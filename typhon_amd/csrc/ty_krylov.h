@@ -10,7 +10,7 @@ int multidot_blocks(long len);
 long krylov_partials(long len);  // doubles of partial-sum space the reductions below need
 // stage-state records dst[c][v] = (P)(u[v][c] + a v[v][c]) for the residual kernels
 template <typename P, typename T>
-hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, hipStream_t s);
+hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, double gamma, hipStream_t s);
 template <typename T>
 hipError_t apply_a(T* w, const T* z, const double* r1, const double* r0, double sc, long len, hipStream_t s);
 template <typename T>

@@ -3,6 +3,7 @@
 // Jacobian-vector products).  Vectors are [nv][n] arrays of length len = nv*n.
 // Reductions are two-pass (per-block partials in a fixed order, then one block per
 // output), so every dot product is bitwise repeatable.
+#include "ty_internal.h"
 #include "ty_krylov.h"
 
 namespace ty {
@@ -16,17 +17,20 @@ __device__ __forceinline__ double wsum(double v)
     return v;
 }
 
-// dst[c][k] = (P)(u[k][c] + a * v[k][c]): a perturbed state as stage-state records
+// dst[c] = the stage-state (primitive) record of u[.][c] + a * v[.][c]: a perturbed state,
+// pressure recovered in FP64 (g1 = gamma - 1)
 template <typename P, typename T>
 __global__ void k_fill_state(P* __restrict__ dst, const double* __restrict__ u, double a, const T* __restrict__ v,
-                             int nv, int n)
+                             int nv, int n, double g1)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
+    double w[NVMAX];
     for (int k = 0; k < nv; ++k) {
         const size_t i = (size_t)k * n + c;
-        dst[(size_t)c * nv + k] = (P)(u[i] + a * (double)v[i]);
+        w[k] = u[i] + a * (double)v[i];
     }
+    store_prim<P>(w, nv, g1, dst + (size_t)c * nv);
 }
 
 // w = z - (r1 - r0) * s     (A z = z - J z with J z from a one-sided difference)
@@ -227,9 +231,9 @@ long krylov_partials(long len)
 }
 
 template <typename P, typename T>
-hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, hipStream_t s)
+hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, double gamma, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_fill_state<P, T>), dim3(nb_of(n)), dim3(KB), 0, s, dst, u, a, v, nv, n);
+    hipLaunchKernelGGL((k_fill_state<P, T>), dim3(nb_of(n)), dim3(KB), 0, s, dst, u, a, v, nv, n, gamma - 1.0);
     return hipGetLastError();
 }
 template <typename T>
@@ -417,10 +421,10 @@ hipError_t asum(const double* x, long len, double* partial, double* out, hipStre
     template hipError_t accumulate<T>(double*, const T*, long, hipStream_t);
 TY_KINST(float)
 TY_KINST(double)
-template hipError_t fill_state<float, float>(float*, const double*, double, const float*, int, int, hipStream_t);
-template hipError_t fill_state<float, double>(float*, const double*, double, const double*, int, int, hipStream_t);
-template hipError_t fill_state<double, double>(double*, const double*, double, const double*, int, int, hipStream_t);
-template hipError_t fill_state<double, float>(double*, const double*, double, const float*, int, int, hipStream_t);
+template hipError_t fill_state<float, float>(float*, const double*, double, const float*, int, int, double, hipStream_t);
+template hipError_t fill_state<float, double>(float*, const double*, double, const double*, int, int, double, hipStream_t);
+template hipError_t fill_state<double, double>(double*, const double*, double, const double*, int, int, double, hipStream_t);
+template hipError_t fill_state<double, float>(double*, const double*, double, const float*, int, int, double, hipStream_t);
 template hipError_t scale<float, float>(float*, const float*, double, long, hipStream_t);
 template hipError_t scale<float, double>(float*, const double*, double, long, hipStream_t);
 template hipError_t scale<double, double>(double*, const double*, double, long, hipStream_t);
