@@ -97,6 +97,7 @@ struct ty_ctx {
     void* d_qf = nullptr;
     void* work_grad[2] = {nullptr, nullptr};  // per precision: [0] FP64, [1] FP32
     void* work_qf[2] = {nullptr, nullptr};
+    void* work_gbnd[2] = {nullptr, nullptr};  // ghost-side viscous records of the boundary faces
     // matrix-free implicit solve (allocated on first use)
     bool kry_ok = false;
     bool kry_base = false;  // the solver's fixed-size vectors (k_b .. k_h) are allocated
@@ -167,6 +168,7 @@ void free_state(ty_ctx* c)
     c->d_us[0] = c->d_us[1] = nullptr;
     c->d_grad = c->d_qf = nullptr;
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
+    c->work_gbnd[0] = c->work_gbnd[1] = nullptr;
     c->kry_ok = false;
     c->kry_base = false;
     c->kry_m = 0;
@@ -620,11 +622,12 @@ template <typename R>
 int build_geo(ty_ctx* c, Geo<R>& g)
 {
     // per-slot geometry, built on the device from the mesh arrays (ty_setup.hip):
-    // ss | sw | sr | sd ([ns][3][n] each) | fn ([3][nf]) | fs ([nf]) | eps2 ([n])
+    // ss | sw | sr | sd ([ns][3][n] each) | fn ([3][nf]) | fs ([nf]) | eps2 ([n]) | sh ([ns][n])
     const int n = c->n, nf = c->nf, ns = c->ns;
     const size_t S3 = (size_t)3 * ns * n;
     const size_t o_ss = 0, o_sw = o_ss + S3, o_sr = o_sw + S3, o_sd = o_sr + S3;
-    const size_t o_fn = o_sd + S3, o_fs = o_fn + 3 * (size_t)nf, o_eps = o_fs + nf, tot = o_eps + n;
+    const size_t o_fn = o_sd + S3, o_fs = o_fn + 3 * (size_t)nf, o_eps = o_fs + nf, o_sh = o_eps + n;
+    const size_t tot = o_sh + (size_t)ns * n;
     R* dp = nullptr;
     int rc = dalloc(c, &dp, tot);
     if (rc) return rc;
@@ -657,6 +660,7 @@ int build_geo(ty_ctx* c, Geo<R>& g)
     g.fnz = dp + o_fn + 2 * (size_t)nf;
     g.fs = dp + o_fs;
     g.eps2 = dp + o_eps;
+    g.sh = dp + o_sh;
     g.bboco = c->d_bboco;
     return TY_OK;
 }
@@ -674,6 +678,11 @@ int ensure_work(ty_ctx* c, int prec)
         HIPCHK(c, hipMalloc(&p, (size_t)c->nv * c->dim * c->n * rb));
         c->owned.push_back(p);
         c->work_grad[prec] = p;
+    }
+    if (c->ph.viscous && !c->work_gbnd[prec]) {  // their ghost sides on boundary faces
+        HIPCHK(c, hipMalloc(&p, (size_t)(c->dim * c->dim + c->dim) * std::max(1, c->nb) * rb));
+        c->owned.push_back(p);
+        c->work_gbnd[prec] = p;
     }
     if (c->ph.method != 'N' && !c->work_qf[prec]) {  // own-side face states of every slot
         HIPCHK(c, hipMalloc(&p, (size_t)c->ns * c->nv * c->n * rb));
@@ -723,6 +732,7 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     c->d_us[0] = c->d_us[1] = nullptr;
     c->d_grad = c->d_qf = nullptr;
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
+    c->work_gbnd[0] = c->work_gbnd[1] = nullptr;
     c->kry_ok = c->kry_base = false;
     c->kry_m = 0;
     c->kry_prec = -1;
@@ -929,6 +939,10 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     const int prec = sizeof(R) == sizeof(float) ? 1 : 0;
     a.grad = (const R*)c->work_grad[prec];
     a.grad_out = (R*)c->work_grad[prec];
+    // residual evaluations of a viscous scheme pass the viscous records from K1 to K2
+    a.vrec = c->ph.viscous ? 1 : 0;
+    a.gbnd = (const R*)c->work_gbnd[prec];
+    a.gbnd_out = (R*)c->work_gbnd[prec];
     a.qf = (const R*)c->work_qf[prec];
     a.qf_out = (R*)c->work_qf[prec];  // null for first-order schemes
     a.res_out = c->d_res;
@@ -1670,6 +1684,7 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
         KArgs<double> a = make_args<double>(c, c->geo_d);
         a.us = d_st;
         a.qf_out = nullptr;
+        a.vrec = 0;  // the derivative kernels read the full gradients of (rho, p, u..)
         const hipError_t e = c->dim == 2 ? launch_gradient<2, double>(a, c->stream) : launch_gradient<3, double>(a, c->stream);
         HIPCHK(c, e);
         t.k.grad = (const double*)c->work_grad[0];
@@ -1897,6 +1912,7 @@ struct Unsteady {
             KArgs<double> g = make_args<double>(c, c->geo_d);
             g.us = rec;
             g.qf_out = nullptr;
+            g.vrec = 0;  // full gradients for the adjoint kernels
             const hipError_t e = c->dim == 2 ? launch_gradient<2, double>(g, c->stream)
                                              : launch_gradient<3, double>(g, c->stream);
             HIPCHK(c, e);

@@ -18,6 +18,22 @@ constexpr int SLOT_UNUSED = -1;  // slot word of an empty slot
 constexpr int BLOCK = 256;       // threads per block (4 wavefronts of 64)
 constexpr int MAX_BOCO = 64;
 
+// Reciprocal, quotient and square root in the arithmetic precision of the residual kernels:
+// IEEE-rounded in FP64 (the reference's arithmetic); in the FP32 form of the mixed mode the
+// hardware v_rcp_f32 / v_sqrt_f32 (about 1 ulp) instead of the correctly rounded sequences.
+template <typename R>
+__device__ __forceinline__ R arcp(R x) { return R(1) / x; }
+template <typename R>
+__device__ __forceinline__ R adiv(R a, R b) { return a / b; }
+template <typename R>
+__device__ __forceinline__ R asqrt(R x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float arcp(float x) { return __builtin_amdgcn_rcpf(x); }
+template <>
+__device__ __forceinline__ float adiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <>
+__device__ __forceinline__ float asqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // boundary kinds handled on the device (the reference's bc_* codes map onto these)
 enum BcKind : int {
     BK_SYM = 0, BK_INLET_SUP = 1, BK_OUTLET_SUB = 2, BK_OUTLET_SUP = 3,
@@ -50,6 +66,8 @@ struct Geo {
     const R *fnx, *fny, *fnz, *fs;  // [nf] per device face: normal, area
     const int* bboco;               // [nb] boco index of each boundary face
     const R* eps2;                  // [n] Venkatakrishnan (K h)^3
+    const R* sh;                    // [ns][n] viscous face weight dHL = |x_f - x_L| / (|x_f - x_L| + |x_f - x_R|)
+                                    // (canonical sides; the other side's weight is 1 - dHL)
 };
 
 struct Physics {
@@ -91,8 +109,13 @@ struct KArgs {
     double* u0;           // [nv][n]
     double* acc;          // [nv][n]
     double* rstore;       // [nstore][nv][n]
-    const R* grad;        // [n][nv*dim] LSQ gradients (viscous flux), one record per cell
+    const R* grad;        // [n][nv*dim] LSQ gradients of (rho, p, u..), one record per cell; with vrec
+                          // set, the viscous records [n][dim*dim + dim] instead: the velocity
+                          // gradient (row-major d u_i / d x_j) and the temperature gradient
     R* grad_out;          // written by K1 when not null
+    int vrec;             // K1 writes (and K2 reads) viscous records
+    const R* gbnd;        // [nb][dim*dim + dim] viscous records of the ghost side of each boundary face
+    R* gbnd_out;          // written by K1 (vrec)
     const R* qf;          // [ns][n][nv] own-side face states (second order), one record per slot
     R* qf_out;            // written by K1 when not null
     double* res_out;      // [nv][n] residual-only mode
@@ -167,7 +190,7 @@ struct GeoSrc {
     const double *cc, *fc, *fn, *fs, *gc, *ps, *vol;
     const int *nbr, *fw;
 };
-// writes ss | sw | sr | sd ([ns][3][n] each) | fn ([3][nf]) | fs ([nf]) | eps2 ([n]) in precision R
+// writes ss | sw | sr | sd ([ns][3][n] each) | fn ([3][nf]) | fs ([nf]) | eps2 ([n]) | sh ([ns][n]) in precision R
 template <typename R>
 hipError_t launch_geo(const GeoSrc& s, R* out, hipStream_t st);
 // stable ascending device sort of n host keys over bits [0, end_bit): idx_out (host) gets the

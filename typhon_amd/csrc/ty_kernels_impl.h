@@ -20,22 +20,6 @@ __device__ __forceinline__ R fmin_(R a, R b) { return a < b ? a : b; }
 template <typename R>
 __device__ __forceinline__ R fmax_(R a, R b) { return a > b ? a : b; }
 
-// Reciprocal, quotient and square root in the arithmetic precision of the residual kernels:
-// IEEE-rounded in FP64 (the reference's arithmetic); in the FP32 form of the mixed mode the
-// hardware v_rcp_f32 / v_sqrt_f32 (about 1 ulp) instead of the correctly rounded sequences.
-template <typename R>
-__device__ __forceinline__ R arcp(R x) { return R(1) / x; }
-template <typename R>
-__device__ __forceinline__ R adiv(R a, R b) { return a / b; }
-template <typename R>
-__device__ __forceinline__ R asqrt(R x) { return sqrt(x); }
-template <>
-__device__ __forceinline__ float arcp(float x) { return __builtin_amdgcn_rcpf(x); }
-template <>
-__device__ __forceinline__ float adiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-template <>
-__device__ __forceinline__ float asqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-
 __device__ __forceinline__ double dpow(double a, double b) { return pow(a, b); }
 __device__ __forceinline__ float dpow(float a, float b) { return powf(a, b); }
 
@@ -514,6 +498,14 @@ __device__ __forceinline__ void face_n(const Geo<R>& g, int f, R* n)
     if (DIM == 3) n[DIM - 1] = g.fnz[f];
 }
 
+template <int K, typename R>
+__device__ __forceinline__ void load_rec(const R* __restrict__ p, int i, R* out)
+{
+    const R* r = p + (size_t)i * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = r[k];
+}
+
 template <int DIM, typename R>
 __device__ __forceinline__ void load_grad(const R* __restrict__ grad, int c, R* gv)
 {
@@ -629,6 +621,42 @@ __device__ __forceinline__ void muscl_own(const Physics& ph, const R* qL, const 
 // ratios of rho, p and the velocity vector), or the Venkatakrishnan cell limiter.
 // Neighbour set = face neighbours incl. ghost cells from the cell_state boundary map.
 
+// side_temp (below) ahead of its definition
+template <int DIM, typename R>
+__device__ __forceinline__ void side_temp(const Physics& ph, const R* q, R ir, const R* g, R& T, R* gT);
+
+// The viscous flux reads a side's gradients only as the velocity gradient and the temperature
+// gradient (calc_flux_viscous_cell.f90:96-109): K1 stores those per cell (dim^2 + dim values
+// instead of nv dim) and, for the ghost side of each boundary face, the gradients of the
+// cell_grad boundary map (bc_grad) with the ghost's temperature gradient.
+template <int DIM, typename R>
+__device__ __forceinline__ void write_visc_records(const KArgs<R>& a, int c, const R* qc, const R* gr)
+{
+    constexpr int VR = DIM * DIM + DIM;
+    R* go = a.grad_out + (size_t)c * VR;
+#pragma unroll
+    for (int k = 0; k < DIM * DIM; ++k) go[k] = gr[2 * DIM + k];
+    R T, gT[DIM];
+    side_temp<DIM>(a.ph, qc, arcp(qc[0]), gr, T, gT);
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) go[DIM * DIM + e] = gT[e];
+    for (int k = 0; k < a.ns; ++k) {
+        const int L = a.link[(size_t)k * a.n + c];
+        if (L == SLOT_UNUSED || !link_boundary(L)) continue;
+        R nf[DIM], qo[DIM + 2], gb[(DIM + 2) * DIM];
+        slot_normal<DIM>(a.g, k, a.n, c, nf);
+        ghost_state<DIM>(a, k, c, L, qc, qo);
+        bc_grad<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, gr, gb);
+        R To, gTo[DIM];
+        side_temp<DIM>(a.ph, qo, arcp(qo[0]), gb, To, gTo);
+        R* ob = a.gbnd_out + (size_t)link_index(L) * VR;
+#pragma unroll
+        for (int q = 0; q < DIM * DIM; ++q) ob[q] = gb[2 * DIM + q];
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) ob[DIM * DIM + e] = gTo[e];
+    }
+}
+
 // reconstruction performed by K1 after the gradient
 enum Rec : int { REC_NONE = 0, REC_VENKAT = 1, REC_MUSCL = 2, REC_MUSCL_BARTH = 3 };
 
@@ -672,9 +700,12 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
         }
     }
     if (a.grad_out) {
-        R* go = a.grad_out + (size_t)c * NV * DIM;
+        if (a.vrec) write_visc_records<DIM>(a, c, qc, gr);
+        else {
+            R* go = a.grad_out + (size_t)c * NV * DIM;
 #pragma unroll
-        for (int k = 0; k < NV * DIM; ++k) go[k] = gr[k];
+            for (int k = 0; k < NV * DIM; ++k) go[k] = gr[k];
+        }
     }
     if (REC == REC_NONE) return;
     if (REC == REC_VENKAT) {
@@ -865,20 +896,17 @@ __device__ __forceinline__ void side_temp(const Physics& ph, const R* q, R ir, c
 }
 
 // viscous_face for the residual kernel: the same flux from per-side temperatures and
-// temperature gradients computed once per cell (side_temp), velocity gradients gUL/gUR
-// (row-major d u_i / d x_j), host-derived constants, and Sutherland's (T/Tref)^1.5 as
-// t sqrt(t) (calc_viscosity, EQNS.f90:243-247).  heat = false leaves out the conductive
-// term (flux walls, where the imposed flux replaces it).
+// temperature gradients computed once per cell (side_temp, in K1), velocity gradients gUL/gUR
+// (row-major d u_i / d x_j), the face's distance weight dHL (dHR = 1 - dHL) precomputed per
+// slot, host-derived constants, and Sutherland's (T/Tref)^1.5 as t sqrt(t) (calc_viscosity,
+// EQNS.f90:243-247).  heat = false leaves out the conductive term (flux walls, where the
+// imposed flux replaces it).
 template <int DIM, typename R>
 __device__ __forceinline__ void viscous_face_pre(const Physics& ph, const R* qL, const R* qR, R TL, R TR,
                                                  const R* gTL, const R* gTR, const R* gUL, const R* gUR,
-                                                 const R* rL, const R* rR, const R* vLR, const R* nf, bool heat,
-                                                 R* flux)
+                                                 R dHL, const R* vLR, const R* nf, bool heat, R* flux)
 {
-    R dHL = asqrt(dotd<DIM>(rL, rL)), dHR = asqrt(dotd<DIM>(rR, rR));
-    const R id = arcp(dHL + dHR);
-    dHL = id * dHL;
-    dHR = id * dHR;
+    const R dHR = R(1) - dHL;
     R velH[DIM];
 #pragma unroll
     for (int e = 0; e < DIM; ++e) velH[e] = dHR * qL[2 + e] + dHL * qR[2 + e];
@@ -1003,14 +1031,10 @@ __device__ __forceinline__ void jac_hll(const Physics& ph, const R* qL, const R*
 
 // viscous diagonal of the face Jacobians (calc_flux_viscous_cell.f90:149-158)
 template <int DIM, typename R>
-__device__ __forceinline__ R visc_jac_diag(const Physics& ph, const R* qL, const R* qR, const R* rL, const R* rR,
-                                           const R* d)
+__device__ __forceinline__ R visc_jac_diag(const Physics& ph, const R* qL, const R* qR, R dHL, const R* d)
 {
     const R r = R(ph.rgas);
-    R dHL = sqrt(dotd<DIM>(rL, rL)), dHR = sqrt(dotd<DIM>(rR, rR));
-    const R id0 = R(1) / (dHL + dHR);
-    dHL = id0 * dHL;
-    dHR = id0 * dHR;
+    const R dHR = R(1) - dHL;
     const R TL = qL[1] / (qL[0] * r), TR = qR[1] / (qR[0] * r);
     const R TH = dHR * TL + dHL * TR;
     const R rhoH = dHR * qL[0] + dHL * qR[0];
@@ -1083,11 +1107,12 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
 #pragma unroll
         for (int k = 0; k < NV * NV; ++k) D[k] = (k % (NV + 1) == 0) ? 1.0 : 0.0;
     }
-    R gcell[VISC ? NV * DIM : 1];
-    R Tc = R(0), gTc[DIM];
+    constexpr int VR = DIM * DIM + DIM;
+    R gcell[VISC ? VR : 1];
+    R Tc = R(0);
     if (VISC) {
-        load_grad<DIM>(a.grad, c, gcell);
-        side_temp<DIM>(a.ph, qc, irc, gcell, Tc, gTc);
+        load_rec<VR>(a.grad, c, gcell);
+        Tc = qc[1] * (irc * R(a.ph.irgas));
     }
     R res[NV];
 #pragma unroll
@@ -1139,29 +1164,19 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
         }
         R flux[NV];
         num_flux_pre<DIM>(a.ph.sch_hyp, gam, R(a.ph.ig1), R(a.ph.gig1), qLf, qRf, irL, irR, nf, flux);
-        R rL[DIM], rR[DIM], dk[DIM];
+        R dk[DIM], dHL = R(0);
         if (VISC) {
-            R rk[DIM];
             slot_vec<DIM>(a.g.sd, k, n, c, dk);
-            slot_vec<DIM>(a.g.sr, k, n, c, rk);
-#pragma unroll
-            for (int e = 0; e < DIM; ++e) {
-                rL[e] = other ? rk[e] + dk[e] : rk[e];
-                rR[e] = other ? rk[e] : rk[e] - dk[e];
-            }
-            R go[NV * DIM];
-            if (!bnd) load_grad<DIM>(a.grad, link_index(L), go);
-            else bc_grad<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, gcell, go);
-            R To, gTo[DIM];
-            side_temp<DIM>(a.ph, qo, iro, go, To, gTo);
-            const R* guc = gcell + 2 * DIM;
-            const R* guo = go + 2 * DIM;
+            dHL = a.g.sh[(size_t)k * n + c];
+            R go[VR];
+            load_rec<VR>(bnd ? a.gbnd : a.grad, link_index(L), go);
+            const R To = qo[1] * (iro * R(a.ph.irgas));
             // ns_bocoflux (ns_bocoflux.f90:47-111): on flux and adiabatic walls the imposed heat
             // flux replaces the conductive one
             const bool fwall = bnd && a.boco[a.g.bboco[link_index(L)]].kind == BK_WALL_FLUX;
             viscous_face_pre<DIM>(a.ph, other ? qo : qc, other ? qc : qo, other ? To : Tc, other ? Tc : To,
-                                  other ? gTo : gTc, other ? gTc : gTo, other ? guo : guc, other ? guc : guo,
-                                  rL, rR, dk, nf, !fwall, flux);
+                                  (other ? go : gcell) + DIM * DIM, (other ? gcell : go) + DIM * DIM,
+                                  other ? go : gcell, other ? gcell : go, dHL, dk, nf, !fwall, flux);
             if (fwall) flux[1] = flux[1] + R(a.boco[a.g.bboco[link_index(L)]].param[0]);
         }
         if (JAC) {
@@ -1169,7 +1184,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(KArgs<R> a)
             R jL[NV * NV], jR[NV * NV];
             jac_hll<DIM>(a.ph, qLf, qRf, nf, jL, jR);
             if (VISC) {
-                const R id = visc_jac_diag<DIM>(a.ph, other ? qo : qc, other ? qc : qo, rL, rR, dk);
+                const R id = visc_jac_diag<DIM>(a.ph, other ? qo : qc, other ? qc : qo, dHL, dk);
 #pragma unroll
                 for (int i = 0; i < NV; ++i) { jR[i * NV + i] -= id; jL[i * NV + i] += id; }
             }

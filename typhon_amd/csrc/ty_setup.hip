@@ -22,6 +22,7 @@ __global__ __launch_bounds__(BLOCK) void k_geo_slots(GeoSrc s, R* __restrict__ o
     R* sr = out + 2 * S3;
     R* sd = out + 3 * S3;
     R* eps = out + 4 * S3 + (size_t)4 * s.nf;
+    R* sh = eps + n;
     auto at = [&](R* base, int k, int e) -> R& { return base[((size_t)k * 3 + e) * n + i]; };
     const double x[3] = {s.cc[i], s.cc[(size_t)n + i], s.cc[(size_t)2 * n + i]};
     double A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -53,6 +54,30 @@ __global__ __launch_bounds__(BLOCK) void k_geo_slots(GeoSrc s, R* __restrict__ o
             at(ss, k, e) = (R)(s.fn[(size_t)e * s.nf + f] * s.fs[f]);
             at(sr, k, e) = (R)r[e];
             at(sd, k, e) = (R)d[e];
+        }
+        {
+            // viscous distance weight from the face's canonical sides, the owner's frame
+            // (calc_flux_viscous_cell.f90:85-95): both cells of a face store the same value
+            double rl2 = 0.0, rr2 = 0.0;
+            for (int e = 0; e < 3; ++e) {
+                const double xf = s.fc[(size_t)e * s.nf + f];
+                double xl, xr;
+                if (nbv < 0) {
+                    xl = x[e];
+                    xr = s.gc[(size_t)e * s.nb + (-nbv - 1)];
+                } else if (other) {
+                    xl = s.cc[(size_t)e * n + nbv];
+                    xr = x[e] + T[e];
+                } else {
+                    xl = x[e];
+                    xr = s.cc[(size_t)e * n + nbv] + T[e];
+                }
+                const double rl = xf - xl, rr = xf - xr;
+                rl2 += rl * rl;
+                rr2 += rr * rr;
+            }
+            const double hl = sqrt(rl2), hr = sqrt(rr2);
+            sh[(size_t)k * n + i] = (R)(hl / (hl + hr));
         }
         const double s2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         double m[3];
@@ -111,6 +136,8 @@ hipError_t launch_geo(const GeoSrc& s, R* out, hipStream_t st)
     // a slot-array layout padded to whole slots: zero the unused slots first
     const size_t S3 = (size_t)3 * s.ns * s.n;
     hipError_t e = hipMemsetAsync(out, 0, 4 * S3 * sizeof(R), st);
+    if (e != hipSuccess) return e;
+    e = hipMemsetAsync(out + 4 * S3 + (size_t)4 * s.nf + s.n, 0, (size_t)s.ns * s.n * sizeof(R), st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_geo_slots<R>), dim3((s.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, s, out);
     if ((e = hipGetLastError()) != hipSuccess) return e;
