@@ -11,12 +11,17 @@ struct AdArgs {
     const int* fcell;     // [2][nf] owner and other cell of each device face (other = -1 on boundary faces)
     int nfi;              // two-sided faces come first: device faces [0, nfi)
     double* face;         // [ns][n][RW] work: per face side, at the slot where that side's cell keeps
-                          // the face, the cell's state adjoint, (viscous) the adjoint of its
-                          // primitive-variable gradient, and the side's share of its volume
-                          // sensitivity; RW = nv (+ nv dim) + 1
+                          // the face, the cell's state adjoint, (viscous) the adjoint of its velocity
+                          // gradient and of its temperature gradient, and the side's share of its
+                          // volume sensitivity; RW = nv (+ dim^2 + dim) + 1
     const unsigned char* fslot;  // [2][nf] slot of each device face in its owner's / other cell's list
+    const double* qrec;   // [n][nv] primitive records (rho, p, u..) of k.us (the HLLC face kernels)
+    double* lrec;         // [n][nv] work: lambda dt / V per cell, one record per cell
     // viscous residuals (k.grad holds the cell gradients of the state)
-    const double* fgeo;   // [6][nf] work: x_f - x_owner and the canonical centre difference d per face
+    const double* fgeo;   // [4][nf] work: the viscous distance weight dHL and the canonical centre
+                          // difference d per face
+    double* gbx;          // [nb][2 dim] work: a boundary face's adjoint of its interior cell's density
+                          // and pressure gradients (the ghost side's gradient map included)
     double* slot;         // [ns][n][nv] work: per slot, the far cell's primitive adjoint through this
                           // cell's gradient
     double* dgrad;        // [n][nv dim] work: tangent of the cell gradients

@@ -23,6 +23,32 @@
 
 namespace ty {
 
+// (dq/dU)^T qb added to Ub, from the primitive state q (rho, p, u..): prim_rev of ty_hllc_rev.h with
+// the velocity taken from the record
+template <int DIM>
+__device__ __forceinline__ void prim_rev_q(const double* q, double g1, const double* qb, double* Ub)
+{
+    const double ir = 1.0 / q[0];
+    double uu = 0.0, ub_u = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        uu += q[2 + d] * q[2 + d];
+        ub_u += qb[2 + d] * q[2 + d];
+    }
+    Ub[0] += qb[0] + qb[1] * g1 * 0.5 * uu - ub_u * ir;
+    Ub[1] += qb[1] * g1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) Ub[2 + d] += -g1 * q[2 + d] * qb[1] + qb[2 + d] * ir;
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_q(const AdArgs& t, int c, double* q)
+{
+    const double* r = t.qrec + (size_t)c * (DIM + 2);
+#pragma unroll
+    for (int v = 0; v < DIM + 2; ++v) q[v] = r[v];
+}
+
 template <int DIM, typename R>
 __device__ __forceinline__ void cons_to_prim(const R* u, R g1, R* q)
 {
@@ -382,17 +408,12 @@ __global__ __launch_bounds__(BLOCK) void k_res_tangent_visc(AdArgs t)
         const Dual* qR = other ? qc : qo;
         Dual F[NV];
         num_flux<DIM>(a.ph.sch_hyp, Dual(a.ph.gamma), qL, qR, nd, F);
-        double rk[DIM], dk[DIM];
-        slot_vec<DIM>(a.g.sr, k, n, c, rk);
+        double dk[DIM];
         slot_vec<DIM>(a.g.sd, k, n, c, dk);
-        Dual rL[DIM], rR[DIM], dd[DIM];
+        Dual dd[DIM];
 #pragma unroll
-        for (int e = 0; e < DIM; ++e) {
-            rL[e] = Dual(other ? rk[e] + dk[e] : rk[e]);
-            rR[e] = Dual(other ? rk[e] : rk[e] - dk[e]);
-            dd[e] = Dual(dk[e]);
-        }
-        viscous_face<DIM>(a.ph, qL, qR, rL, rR, dd, nd, other ? go : gc, other ? gc : go, F);
+        for (int e = 0; e < DIM; ++e) dd[e] = Dual(dk[e]);
+        viscous_face<DIM>(a.ph, qL, qR, a.g.sh[(size_t)k * n + c], dd, nd, other ? go : gc, other ? gc : go, F);
         const Dual Ad(A);
 #pragma unroll
         for (int v = 0; v < NV; ++v) res[v] = other ? res[v] + Ad * F[v] : res[v] - Ad * F[v];
@@ -403,8 +424,8 @@ __global__ __launch_bounds__(BLOCK) void k_res_tangent_visc(AdArgs t)
     for (int v = 0; v < NV; ++v) t.out[(size_t)v * n + c] = (coef * res[v]).d;
 }
 
-// per device face: x_f - x_owner and the canonical centre difference d = x_other - x_owner
-// (the ghost centre on boundary faces), from the owner's slot geometry
+// per device face: the viscous distance weight dHL and the canonical centre difference
+// d = x_other - x_owner (the ghost centre on boundary faces), from the owner's slot geometry
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void k_adj_face_geom(AdArgs t)
 {
@@ -416,16 +437,25 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face_geom(AdArgs t)
         const int w = a.fw[(size_t)k * n + c];
         if (w == SLOT_UNUSED || w < 0) continue;  // unused, or this cell is the far side
         const int f = (int)((unsigned)w & FACE_MASK);
-        double rk[DIM], dk[DIM];
-        slot_vec<DIM>(a.g.sr, k, n, c, rk);
+        double dk[DIM];
         slot_vec<DIM>(a.g.sd, k, n, c, dk);
         double* g = const_cast<double*>(t.fgeo);
+        g[f] = a.g.sh[(size_t)k * n + c];
 #pragma unroll
-        for (int e = 0; e < DIM; ++e) {
-            g[(size_t)e * t.nf + f] = rk[e];
-            g[(size_t)(3 + e) * t.nf + f] = dk[e];
-        }
+        for (int e = 0; e < DIM; ++e) g[(size_t)(1 + e) * t.nf + f] = dk[e];
     }
+}
+
+// lrec[c] = lambda_c dt / V_c as one record per cell
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_adj_lrec(AdArgs t)
+{
+    const KArgs<double>& a = t.k;
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= a.n) return;
+    const double k = *a.d_dt / a.vol[c];
+#pragma unroll
+    for (int v = 0; v < DIM + 2; ++v) t.lrec[(size_t)c * (DIM + 2) + v] = t.lam[(size_t)v * a.n + c] * k;
 }
 
 // HLLC/HLLCB faces, one work-item per device face: one forward and one reverse sweep of the face
@@ -433,14 +463,17 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face_geom(AdArgs t)
 // m = A (lambda_other dt/V_other - lambda_owner dt/V_owner), and m = -A lambda_owner dt/V_owner on a
 // boundary face, whose ghost state qR = B(qL, n) (and, viscous, ghost gradient gR = B_g(gL, n)) is
 // chained through the transpose of B's Jacobian (dual-number passes of the boundary template,
-// cheap next to the flux).  Each face side gets a record [(dF/dU_side)^T m, (viscous) (dF/dg_side)^T m,
-// sigma A lambda_side dt/V_side . F] (sigma = -1 owner, +1 other) for k_res_adjoint_gather; the
-// face's area-vector sensitivity (inviscid) and a boundary face's inlet-state sensitivity are
-// written here.
+// cheap next to the flux).  Cell data come as one record per cell (primitive state, lambda dt/V).
+// Each face side gets a record [(dF/dU_side)^T m, (viscous) the adjoints of the side's velocity
+// gradient and temperature gradient, sigma A lambda_side dt/V_side . F] (sigma = -1 owner, +1
+// other) for k_res_adjoint_gather.  A boundary face's viscous sweep goes through the ghost
+// gradient map on the full gradients: its record's temperature-gradient adjoint is zero and the
+// interior cell's density and pressure gradient adjoints go to t.gbx.  The face's area-vector
+// sensitivity (inviscid) and a boundary face's inlet-state sensitivity are written here.
 template <int DIM, bool VISC, bool BND>
 __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs t)
 {
-    constexpr int NV = DIM + 2, NG = VISC ? NV * DIM : 0, RW = NV + NG + 1;
+    constexpr int NV = DIM + 2, NG = NV * DIM, NC = DIM * DIM + DIM, RW = NV + (VISC ? NC : 0) + 1;
     const KArgs<double>& a = t.k;
     // two launches: two-sided faces [0, nfi), boundary faces [nfi, nf) (BND)
     const int f = (BND ? t.nfi : 0) + (int)(blockIdx.x * BLOCK + threadIdx.x);
@@ -448,43 +481,41 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
     const int n = a.n;
     constexpr bool bnd = BND;
     const int cl = t.fcell[f], cr = bnd ? cl : t.fcell[(size_t)t.nf + f];
-    const double dt = *a.d_dt;
     const double g1d = a.ph.gamma - 1.0;
     double nv[DIM];
     face_n<DIM>(a.g, f, nv);
     const double A = a.g.fs[f];
-    const double kl = dt / a.vol[cl], kr = bnd ? 0.0 : dt / a.vol[cr];
-    double UL[NV], qL[NV], qR[NV], mv[NV], qLb[NV], qRb[NV], gn[DIM], Fv[NV];
+    double qL[NV], qR[NV], lL[NV], lR[NV], mv[NV], qLb[NV], qRb[NV], gn[DIM], Fv[NV];
+    load_q<DIM>(t, cl, qL);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-        UL[v] = a.us[(size_t)v * n + cl];
-        mv[v] = A * ((bnd ? 0.0 : t.lam[(size_t)v * n + cr] * kr) - t.lam[(size_t)v * n + cl] * kl);
+        lL[v] = t.lrec[(size_t)cl * NV + v];
+        lR[v] = bnd ? 0.0 : t.lrec[(size_t)cr * NV + v];
+        mv[v] = A * (lR[v] - lL[v]);
         qLb[v] = 0.0;
         qRb[v] = 0.0;
     }
 #pragma unroll
     for (int e = 0; e < DIM; ++e) gn[e] = 0.0;
-    cons_to_prim<DIM>(UL, g1d, qL);
     BocoDev bc;
     if (bnd) {
         bc = a.boco[a.g.bboco[f - t.nfi]];
         bc_state<DIM>(bc, nv, qL, qR, a.ph);
     } else {
-        double UR[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) UR[v] = a.us[(size_t)v * n + cr];
-        cons_to_prim<DIM>(UR, g1d, qR);
+        load_q<DIM>(t, cr, qR);
     }
     hllc_rev<DIM>(a.ph.sch_hyp, a.ph.gamma, qL, qR, nv, mv, Fv, qLb, qRb, gn);
     // side records at the slot each side keeps the face in: [slot][cell][RW]
     double* out = t.face + ((size_t)t.fslot[f] * n + cl) * RW;
     double* oto = bnd ? out : t.face + ((size_t)t.fslot[(size_t)t.nf + f] * n + cr) * RW;
-    double gLb[VISC ? NG : 1], gRb[VISC ? NG : 1];
+    constexpr int NB = VISC ? (BND ? NG : NC) : 1;
+    double gLb[NB], gRb[NB];
     if (VISC) {
-        double gL[VISC ? NG : 1], gR[VISC ? NG : 1], rL[DIM], rR[DIM], dk[DIM];
+        double gL[NG], gR[NG], dk[DIM];
 #pragma unroll
-        for (int q = 0; q < NG; ++q) {
-            gL[q] = a.grad[(size_t)cl * NG + q];
+        for (int q = 0; q < NG; ++q) gL[q] = a.grad[(size_t)cl * NG + q];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
             gLb[q] = 0.0;
             gRb[q] = 0.0;
         }
@@ -494,26 +525,23 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
 #pragma unroll
             for (int q = 0; q < NG; ++q) gR[q] = a.grad[(size_t)cr * NG + q];
         }
+        const double dHL = t.fgeo[f];
 #pragma unroll
-        for (int e = 0; e < DIM; ++e) {
-            rL[e] = t.fgeo[(size_t)e * t.nf + f];
-            dk[e] = t.fgeo[(size_t)(3 + e) * t.nf + f];
-            rR[e] = rL[e] - dk[e];
-        }
-        viscous_rev<DIM>(a.ph, qL, qR, rL, rR, dk, nv, gL, gR, mv, Fv, qLb, qRb, gLb, gRb);
+        for (int e = 0; e < DIM; ++e) dk[e] = t.fgeo[(size_t)(1 + e) * t.nf + f];
+        viscous_rev<DIM, !BND>(a.ph, qL, qR, dHL, dk, nv, gL, gR, mv, Fv, qLb, qRb, gLb, gRb);
         if (bnd) {
             // gLb += (dB_g/dgL)^T gRb: B_g copies the gradient, or reflects the scalar gradients
             // by S = I - 2 n n^T and the velocity gradient as S G S; both maps are self-adjoint
-            double tb[VISC ? NG : 1];
+            double tb[NB];
             bc_grad<DIM>(bc, nv, gRb, tb);
 #pragma unroll
-            for (int q = 0; q < NG; ++q) gLb[q] += tb[q];
+            for (int q = 0; q < NB; ++q) gLb[q] += tb[q];
         }
     }
     double sL = 0.0;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) sL += t.lam[(size_t)v * n + cl] * Fv[v];
-    out[RW - 1] = -A * kl * sL;
+    for (int v = 0; v < NV; ++v) sL += lL[v] * Fv[v];
+    out[RW - 1] = -A * sL;
     if (bnd) {
         const int bi = f - t.nfi;
         const bool inlet = bc.kind == BK_INLET_SUP;
@@ -539,55 +567,70 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
                     gn[e - NV] += sacc;
             }
         }
-    } else {
-        double UR[NV], gR[NV], sR = 0.0;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            UR[v] = a.us[(size_t)v * n + cr];
-            gR[v] = 0.0;
-            sR += t.lam[(size_t)v * n + cr] * Fv[v];
-        }
-        prim_rev<DIM>(UR, g1d, qRb, gR);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) oto[v] = gR[v];
         if (VISC) {
 #pragma unroll
-            for (int q = 0; q < NG; ++q) oto[NV + q] = gRb[q];
+            for (int q = 0; q < 2 * DIM; ++q) t.gbx[(size_t)bi * 2 * DIM + q] = gLb[q];
         }
-        oto[RW - 1] = A * kr * sR;
+    } else {
+        double gRc[NV], sR = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            gRc[v] = 0.0;
+            sR += lR[v] * Fv[v];
+        }
+        prim_rev_q<DIM>(qR, g1d, qRb, gRc);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) oto[v] = gRc[v];
+        if (VISC) {
+#pragma unroll
+            for (int q = 0; q < NB; ++q) oto[NV + q] = gRb[q];
+        }
+        oto[RW - 1] = A * sR;
     }
-    double gL[NV];
+    double gLc[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) gL[v] = 0.0;
-    prim_rev<DIM>(UL, g1d, qLb, gL);
+    for (int v = 0; v < NV; ++v) gLc[v] = 0.0;
+    prim_rev_q<DIM>(qL, g1d, qLb, gLc);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) out[v] = gL[v];
+    for (int v = 0; v < NV; ++v) out[v] = gLc[v];
     if (VISC) {
+        if (BND) {
 #pragma unroll
-        for (int q = 0; q < NG; ++q) out[NV + q] = gLb[q];
+            for (int q = 0; q < DIM * DIM; ++q) out[NV + q] = gLb[2 * DIM + q];
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) out[NV + DIM * DIM + e] = 0.0;
+        } else {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) out[NV + q] = gLb[q];
+        }
     }
     if (!VISC && t.gs) face_sens<DIM>(t, f, nv, A, mv, Fv, gn);
 }
 
 // per cell: its face sides' records summed in slot order (one writer per output, no atomics).
-// Viscous: the gradient adjoint gb of this cell is pushed back through g_c = sum_k w_k (q_k - q_c):
-// this cell's own primitive adjoint gets -sum_k w_k . gb (and the ghost states' Jacobians on
-// boundary slots, or the inlet-state sensitivity), and the far cell of slot k gets t_k = w_k . gb,
-// left in t.slot for k_res_adjoint_tail.
+// Viscous: the summed temperature-gradient adjoint is chained through grad T = k1 grad p -
+// k2 grad rho (k1 = 1/(rho r), k2 = p/(rho^2 r); linear in the adjoint, so once per cell), the
+// boundary faces' density and pressure gradient adjoints are added, and the gradient adjoint gb of
+// this cell is pushed back through g_c = sum_k w_k (q_k - q_c): this cell's own primitive adjoint
+// gets -sum_k w_k . gb (and the ghost states' Jacobians on boundary slots, or the inlet-state
+// sensitivity), and the far cell of slot k gets t_k = w_k . gb, left in t.slot for
+// k_res_adjoint_tail.
 template <int DIM, bool VISC, int NSM>
 __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
 {
-    constexpr int NV = DIM + 2, NG = VISC ? NV * DIM : 0, RW = NV + NG + 1;
+    constexpr int NV = DIM + 2, NG = NV * DIM, NC = DIM * DIM + DIM, RW = NV + (VISC ? NC : 0) + 1;
     const KArgs<double>& a = t.k;
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n) return;
     const int n = a.n;
-    double gu[NV], sv = 0.0, gb[VISC ? NG : 1];
+    double gu[NV], sv = 0.0, gc[VISC ? NC : 1], gx[VISC ? 2 * DIM : 1];
 #pragma unroll
     for (int v = 0; v < NV; ++v) gu[v] = 0.0;
     if (VISC) {
 #pragma unroll
-        for (int q = 0; q < NG; ++q) gb[q] = 0.0;
+        for (int q = 0; q < NC; ++q) gc[q] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 2 * DIM; ++q) gx[q] = 0.0;
     }
     // slots unrolled to NSM >= ns: every slot's record loads can be in flight together
 #pragma unroll
@@ -600,18 +643,38 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
         for (int v = 0; v < NV; ++v) gu[v] += in[v];
         if (VISC) {
 #pragma unroll
-            for (int q = 0; q < NG; ++q) gb[q] += in[NV + q];
+            for (int q = 0; q < NC; ++q) gc[q] += in[NV + q];
+            const int L = a.link[(size_t)k * n + c];
+            if (link_boundary(L)) {
+                const double* bx = t.gbx + (size_t)link_index(L) * 2 * DIM;
+#pragma unroll
+                for (int q = 0; q < 2 * DIM; ++q) gx[q] += bx[q];
+            }
         }
         sv += in[RW - 1];
     }
     if (VISC) {
-        const double g1d = a.ph.gamma - 1.0;
-        double Uc[NV], qb[NV];
+        const double g1d = a.ph.gamma - 1.0, r = a.ph.rgas;
+        double qc[NV], qb[NV], gb[NG];
+        load_q<DIM>(t, c, qc);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            Uc[v] = a.us[(size_t)v * n + c];
-            qb[v] = 0.0;
+        for (int v = 0; v < NV; ++v) qb[v] = 0.0;
+        // grad T = k1 grad p - k2 grad rho (calc_flux_viscous_cell.f90:96-109)
+        const double k1 = 1.0 / (qc[0] * r), k2 = qc[1] / (qc[0] * qc[0] * r);
+        const double* gcell = a.grad + (size_t)c * NG;
+        double k1_b = 0.0, k2_b = 0.0;
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) {
+            const double tb = gc[DIM * DIM + e];
+            gb[e] = gx[e] - k2 * tb;
+            gb[DIM + e] = gx[DIM + e] + k1 * tb;
+            k1_b += tb * gcell[DIM + e];
+            k2_b -= tb * gcell[e];
         }
+#pragma unroll
+        for (int q = 0; q < DIM * DIM; ++q) gb[2 * DIM + q] = gc[q];
+        qb[1] += k2_b * k1 / qc[0];
+        qb[0] += -k1_b * k1 / qc[0] - 2.0 * k2_b * k2 / qc[0];
         for (int k = 0; k < a.ns; ++k) {
             const int L = a.link[(size_t)k * n + c];
             if (L == SLOT_UNUSED) continue;
@@ -644,7 +707,7 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
                 }
             }
         }
-        prim_rev<DIM>(Uc, g1d, qb, gu);
+        prim_rev_q<DIM>(qc, g1d, qb, gu);
     }
 #pragma unroll
     for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] = gu[v];
@@ -665,13 +728,10 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_bnd_cells(AdArgs t)
     const int c = t.bcell[j0];
     const int n = a.n;
     const double g1d = a.ph.gamma - 1.0;
-    double Uc[NV], qc[NV], qb[NV];
+    double qc[NV], qb[NV];
+    load_q<DIM>(t, c, qc);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        Uc[v] = a.us[(size_t)v * n + c];
-        qb[v] = 0.0;
-    }
-    cons_to_prim<DIM>(Uc, g1d, qc);
+    for (int v = 0; v < NV; ++v) qb[v] = 0.0;
     for (int k = 0; k < a.ns; ++k) {
         const int L = a.link[(size_t)k * n + c];
         if (L == SLOT_UNUSED || !link_boundary(L)) continue;
@@ -697,7 +757,7 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_bnd_cells(AdArgs t)
     double gu[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) gu[v] = 0.0;
-    prim_rev<DIM>(Uc, g1d, qb, gu);
+    prim_rev_q<DIM>(qc, g1d, qb, gu);
 #pragma unroll
     for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] += gu[v];
 }
@@ -712,11 +772,11 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_tail(AdArgs t)
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n) return;
     const int n = a.n;
-    double qb[NV], Uc[NV], gu[NV];
+    double qb[NV], qc[NV], gu[NV];
+    load_q<DIM>(t, c, qc);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         qb[v] = 0.0;
-        Uc[v] = a.us[(size_t)v * n + c];
         gu[v] = 0.0;
     }
     for (int k = 0; k < a.ns; ++k) {
@@ -726,7 +786,7 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_tail(AdArgs t)
 #pragma unroll
         for (int v = 0; v < NV; ++v) qb[v] += in[v];
     }
-    prim_rev<DIM>(Uc, a.ph.gamma - 1.0, qb, gu);
+    prim_rev_q<DIM>(qc, a.ph.gamma - 1.0, qb, gu);
 #pragma unroll
     for (int v = 0; v < NV; ++v) t.gu[(size_t)v * n + c] += gu[v];
 }
@@ -757,6 +817,8 @@ hipError_t launch_res_adjoint(const AdArgs& t, hipStream_t s)
 {
     const bool rev = t.k.ph.sch_hyp == 25 || t.k.ph.sch_hyp == 26;
     const dim3 grid(grid_blocks(t.k.n)), fgrid((t.nfi + BLOCK - 1) / BLOCK), bgrid((t.nb + BLOCK - 1) / BLOCK);
+    if (rev || t.k.ph.viscous)
+        hipLaunchKernelGGL((k_adj_lrec<DIM>), dim3((t.k.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, t);
     if (t.k.ph.viscous) {
         if (!rev) return hipErrorInvalidValue;  // the host rejects this case first
         if (t.nfi > 0) hipLaunchKernelGGL((k_res_adjoint_face<DIM, true, false>), fgrid, dim3(BLOCK), 0, s, t);

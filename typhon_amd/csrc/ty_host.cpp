@@ -1675,12 +1675,15 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
     t.nb = c->nb;
     t.fcell = c->d_fcell;
     t.nfi = c->nfi;
+    // primitive records of the state (the HLLC face kernels and the gathers read them)
+    int rc;
+    double* d_st = nullptr;
+    if ((rc = tmp.alloc(&d_st, (size_t)c->nv * c->n))) return rc;
+    HIPCHK(c, launch_to_stage<double>(c->d_u0, d_st, c->nv, c->n, c->gas.gamma, c->stream));
+    t.qrec = d_st;
     if (c->ph.viscous) {
         // the cell gradients of the current state, as the residual kernels compute them
-        int rc;
-        double* d_st = nullptr;
-        if ((rc = ensure_work(c, 0)) || (rc = tmp.alloc(&d_st, (size_t)c->nv * c->n))) return rc;
-        HIPCHK(c, launch_to_stage<double>(c->d_u0, d_st, c->nv, c->n, c->gas.gamma, c->stream));
+        if ((rc = ensure_work(c, 0))) return rc;
         KArgs<double> a = make_args<double>(c, c->geo_d);
         a.us = d_st;
         a.qf_out = nullptr;
@@ -1688,6 +1691,31 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
         const hipError_t e = c->dim == 2 ? launch_gradient<2, double>(a, c->stream) : launch_gradient<3, double>(a, c->stream);
         HIPCHK(c, e);
         t.k.grad = (const double*)c->work_grad[0];
+    }
+    return TY_OK;
+}
+
+// work buffers of the residual adjoint (ty_residual_adjoint and each reversed stage of
+// ty_unsteady_adjoint): per face side records, lambda dt/V records, (viscous) per face geometry,
+// boundary-face gradient adjoints and the per-slot pushback
+int alloc_adjoint_work(ty_ctx* c, AdArgs& t, DevTmp& tmp)
+{
+    const int n = c->n, nv = c->nv, dim = c->dim;
+    const int rw = nv + (c->ph.viscous ? dim * dim + dim : 0) + 1;
+    int rc;
+    double *d_face = nullptr, *d_lrec = nullptr;
+    if ((rc = tmp.alloc(&d_face, (size_t)c->ns * n * rw)) || (rc = tmp.alloc(&d_lrec, (size_t)nv * n))) return rc;
+    t.face = d_face;
+    t.lrec = d_lrec;
+    if (c->ph.viscous) {
+        double *d_fg = nullptr, *d_sl = nullptr, *d_gx = nullptr;
+        if ((rc = tmp.alloc(&d_fg, (size_t)4 * c->nf)) || (rc = tmp.alloc(&d_sl, (size_t)c->ns * n * nv)) ||
+            (rc = tmp.alloc(&d_gx, (size_t)2 * dim * std::max(1, c->nb))))
+            return rc;
+        t.fgeo = d_fg;
+        t.slot = d_sl;
+        t.gbx = d_gx;
+        HIPCHK(c, dim == 2 ? launch_adj_face_geom<2>(t, c->stream) : launch_adj_face_geom<3>(t, c->stream));
     }
     return TY_OK;
 }
@@ -1781,17 +1809,7 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
     if (gs && (rc = tmp.alloc(&d_gs, (size_t)3 * nf))) return rc;
     if (gvol && (rc = tmp.alloc(&d_gv, (size_t)n))) return rc;
     if (gq && (rc = tmp.alloc(&d_gq, (size_t)nv * std::max(1, nb)))) return rc;
-    const int rw = nv + (c->ph.viscous ? nv * c->dim : 0) + 1;
-    double* d_face = nullptr;  // per face side: state (and gradient) adjoint, volume-sensitivity share
-    if ((rc = tmp.alloc(&d_face, (size_t)c->ns * n * rw))) return rc;
-    t.face = d_face;
-    if (c->ph.viscous) {
-        double *d_fg = nullptr, *d_sl = nullptr;
-        if ((rc = tmp.alloc(&d_fg, (size_t)6 * nf)) || (rc = tmp.alloc(&d_sl, (size_t)c->ns * n * nv))) return rc;
-        t.fgeo = d_fg;
-        t.slot = d_sl;
-        HIPCHK(c, c->dim == 2 ? launch_adj_face_geom<2>(t, c->stream) : launch_adj_face_geom<3>(t, c->stream));
-    }
+    if ((rc = alloc_adjoint_work(c, t, tmp))) return rc;
     t.lam = d_lam;
     t.gu = d_gu;
     t.gs = d_gs;
@@ -1907,8 +1925,9 @@ struct Unsteady {
     {
         AdArgs a = t;
         a.k.us = U;
+        HIPCHK(c, launch_to_stage<double>(U, rec, c->nv, c->n, c->gas.gamma, c->stream));
+        a.qrec = rec;
         if (c->ph.viscous) {
-            HIPCHK(c, launch_to_stage<double>(U, rec, c->nv, c->n, c->gas.gamma, c->stream));
             KArgs<double> g = make_args<double>(c, c->geo_d);
             g.us = rec;
             g.qf_out = nullptr;
@@ -2030,7 +2049,7 @@ int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t sn
     if ((rc = ensure_work(c, 0))) return rc;
     u.c = c;
     u.ns = c->sch.nstage;
-    const int n = c->n, nv = c->nv, nb = c->nb, nf = c->nf;
+    const int n = c->n, nv = c->nv, nb = c->nb;
     u.len = (long)nv * n;
     const size_t len = (size_t)u.len;
     double *d_dts = nullptr, *U0 = nullptr, *lam = nullptr;
@@ -2052,18 +2071,7 @@ int ty_unsteady_adjoint(ty_ctx* c, int32_t nsteps, const double* dts, int32_t sn
         if ((rc = tmp.alloc(&u.gq_acc, (size_t)nv * std::max(1, nb)))) return rc;
         HIPCHK(c, hipMemsetAsync(u.gq_acc, 0, (size_t)nv * std::max(1, nb) * sizeof(double), c->stream));
     }
-    // residual-adjoint work buffers (as ty_residual_adjoint)
-    const int rw = nv + (c->ph.viscous ? nv * c->dim : 0) + 1;
-    double* d_face = nullptr;
-    if ((rc = tmp.alloc(&d_face, (size_t)c->ns * n * rw))) return rc;
-    u.t.face = d_face;
-    if (c->ph.viscous) {
-        double *d_fg = nullptr, *d_sl = nullptr;
-        if ((rc = tmp.alloc(&d_fg, (size_t)6 * nf)) || (rc = tmp.alloc(&d_sl, (size_t)c->ns * n * nv))) return rc;
-        u.t.fgeo = d_fg;
-        u.t.slot = d_sl;
-        HIPCHK(c, c->dim == 2 ? launch_adj_face_geom<2>(u.t, c->stream) : launch_adj_face_geom<3>(u.t, c->stream));
-    }
+    if ((rc = alloc_adjoint_work(c, u.t, tmp))) return rc;
     u.t.gs = nullptr;
     u.t.gvol = nullptr;
     if (nsteps > 0)

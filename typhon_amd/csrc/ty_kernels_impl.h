@@ -799,18 +799,15 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 
 // ---------------------------------------------------------------------------
 // viscous flux of one face, calc_flux_viscous_cell.f90:75-158 (subtracted from F):
-// qL/qR the canonical cell states, rL = x_f - x_L, rR = x_f - x_R, vLR = x_R - x_L,
-// gL/gR the cell gradients, nf the canonical normal.
-template <int DIM, typename R>
-__device__ void viscous_face(const Physics& ph, const R* qL, const R* qR, const R* rL, const R* rR, const R* vLR,
+// qL/qR the canonical cell states, dHL the distance weight |x_f - x_L| / (|x_f - x_L| + |x_f - x_R|)
+// (dHR = 1 - dHL), vLR = x_R - x_L, gL/gR the cell gradients, nf the canonical normal.
+template <int DIM, typename R, typename W>
+__device__ void viscous_face(const Physics& ph, const R* qL, const R* qR, W dHLw, const R* vLR,
                              const R* nf, const R* gL, const R* gR, R* flux)
 {
     const R r = R(ph.rgas);
     const R cp = R(ph.gamma) * r / (R(ph.gamma) - R(1));
-    R dHL = sqrt(dotd<DIM>(rL, rL)), dHR = sqrt(dotd<DIM>(rR, rR));
-    const R id = R(1) / (dHL + dHR);
-    dHL = id * dHL;
-    dHR = id * dHR;
+    const R dHL = R(dHLw), dHR = R(1.0 - dHLw);
     R velH[DIM];
 #pragma unroll
     for (int e = 0; e < DIM; ++e) velH[e] = dHR * qL[2 + e] + dHL * qR[2 + e];

@@ -3,26 +3,27 @@
 // EQNS.f90:225-265).  Given the weight Fb = dJ/dF it adds to F the viscous part of the
 // flux (the forward value, same operations as viscous_face) and accumulates
 // qLb += (dF/dqL)^T Fb, qRb += (dF/dqR)^T Fb, gLb += (dF/dgL)^T Fb, gRb += (dF/dgR)^T Fb.
-// The face geometry (rL, rR, vLR, n) is held fixed: this is the state adjoint.
-// Gradient records: [rho (DIM), p (DIM), d u_i / d x_j at (2 + i) DIM + j].
+// The face geometry (distance weight dHL, dHR = 1 - dHL, vLR, n) is held fixed: this is the
+// state adjoint.  Gradient records: [rho (DIM), p (DIM), d u_i / d x_j at (2 + i) DIM + j].
+// COMPACT: the sweep stops at the temperature gradient grad T = k1 grad p - k2 grad rho of each
+// side (k1 = 1/(rho r), k2 = p/(rho^2 r)): gLb / gRb are [d u_i / d x_j (DIM^2), grad T (DIM)]
+// and qLb / qRb leave out grad T's dependence on (rho, p), which the per-cell gather adds once
+// per cell from the summed grad T adjoint (it is linear in it).
 #pragma once
 #include "ty_kernels_impl.h"
 
 namespace ty {
 
-template <int DIM>
-__device__ __forceinline__ void viscous_rev(const Physics& ph, const double* qL, const double* qR, const double* rL,
-                                            const double* rR, const double* vLR, const double* nf, const double* gL,
+template <int DIM, bool COMPACT = false>
+__device__ __forceinline__ void viscous_rev(const Physics& ph, const double* qL, const double* qR, double dHL,
+                                            const double* vLR, const double* nf, const double* gL,
                                             const double* gR, const double* Fb, double* F, double* qLb, double* qRb,
                                             double* gLb, double* gRb)
 {
     // ---- forward
     const double r = ph.rgas;
     const double cp = ph.gamma * r / (ph.gamma - 1.0);
-    double dHL = ::sqrt(dotd<DIM>(rL, rL)), dHR = ::sqrt(dotd<DIM>(rR, rR));
-    const double id = 1.0 / (dHL + dHR);
-    dHL = id * dHL;
-    dHR = id * dHR;
+    const double dHR = 1.0 - dHL;
     double velH[DIM];
 #pragma unroll
     for (int e = 0; e < DIM; ++e) velH[e] = dHR * qL[2 + e] + dHL * qR[2 + e];
@@ -152,32 +153,38 @@ __device__ __forceinline__ void viscous_rev(const Physics& ph, const double* qL,
         }
     }
     // G0 = dHL gR_u + dHR gL_u
+    constexpr int GU = COMPACT ? 0 : 2 * DIM;  // velocity-gradient offset in gLb / gRb
 #pragma unroll
     for (int i = 0; i < DIM; ++i)
 #pragma unroll
         for (int j = 0; j < DIM; ++j) {
-            gRb[(2 + i) * DIM + j] += dHL * Gb[i][j];
-            gLb[(2 + i) * DIM + j] += dHR * Gb[i][j];
+            gRb[GU + i * DIM + j] += dHL * Gb[i][j];
+            gLb[GU + i * DIM + j] += dHR * Gb[i][j];
         }
     // gavg = dHL gTR + dHR gTL; gT = k1 grad p - k2 grad rho
     double kl1_b = 0.0, kl2_b = 0.0, kr1_b = 0.0, kr2_b = 0.0;
 #pragma unroll
     for (int e = 0; e < DIM; ++e) {
         const double tLb = dHR * gavg_b[e], tRb = dHL * gavg_b[e];
-        gLb[DIM + e] += kl1 * tLb;
-        gLb[e] -= kl2 * tLb;
-        gRb[DIM + e] += kr1 * tRb;
-        gRb[e] -= kr2 * tRb;
-        kl1_b += tLb * gL[DIM + e];
-        kl2_b -= tLb * gL[e];
-        kr1_b += tRb * gR[DIM + e];
-        kr2_b -= tRb * gR[e];
+        if (COMPACT) {
+            gLb[DIM * DIM + e] += tLb;
+            gRb[DIM * DIM + e] += tRb;
+        } else {
+            gLb[DIM + e] += kl1 * tLb;
+            gLb[e] -= kl2 * tLb;
+            gRb[DIM + e] += kr1 * tRb;
+            gRb[e] -= kr2 * tRb;
+            kl1_b += tLb * gL[DIM + e];
+            kl2_b -= tLb * gL[e];
+            kr1_b += tRb * gR[DIM + e];
+            kr2_b -= tRb * gR[e];
+        }
     }
     const double TH_b = mu_b * dmu_dTH;
     const double rhoH_b = mu_b * dmu_drhoH;
     TL_b += dHR * TH_b;
     TR_b += dHL * TH_b;
-    // kl1 = 1/(rho r), kl2 = p/(rho^2 r), T = p/(rho r)
+    // kl1 = 1/(rho r), kl2 = p/(rho^2 r), T = p/(rho r) (kl1_b = kl2_b = 0 when COMPACT)
     {
         const double iL = 1.0 / (qL[0] * r), iR = 1.0 / (qR[0] * r);
         qLb[1] += TL_b * iL + kl2_b * iL / qL[0];
