@@ -318,14 +318,14 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
     solver.set_state(u)
     lam = np.random.default_rng(5).normal(size=(5, nc))
     solver.calc_rhs(dt)
-    solver.residual_adjoint(dt, lam, area_vectors=False)
+    solver.residual_adjoint(dt, lam, area_vectors=False, volumes=False)
     solver.sync()
     solver.timing(True)
     for _ in range(5):
         solver.calc_rhs(dt)
     tp = solver.timing(True)
     for _ in range(5):
-        solver.residual_adjoint(dt, lam, area_vectors=False)
+        solver.residual_adjoint(dt, lam, area_vectors=False, volumes=False)
     ta = solver.timing(False)
     prim = tp["grad_ms"] / max(1, tp["grad_n"]) + tp["flux_ms"] / max(1, tp["flux_n"])
     adj = ta["adjoint_ms"] / max(1, ta["adjoint_n"])

@@ -102,9 +102,10 @@ class LinearSolver:
     def calc_rhs(self, dt):
         return dt * (self.M @ self.state.ravel()).reshape(5, self.n)
 
-    def residual_adjoint(self, dt, lam, area_vectors=True):
+    def residual_adjoint(self, dt, lam, area_vectors=True, volumes=True):
         gu = dt * (self.M.T @ lam.ravel()).reshape(5, self.n)
-        return dict(u=gu, s=np.zeros((2, 3)) if area_vectors else None, vol=np.zeros(self.n), q=np.zeros((1, 5)))
+        return dict(u=gu, s=np.zeros((2, 3)) if area_vectors else None, vol=np.zeros(self.n) if volumes else None,
+                    q=np.zeros((1, 5)))
 
 
 @pytest.mark.parametrize("method", [S.TPS_RK2, S.TPS_RK4, S.TPS_LSRK25BB])

@@ -82,6 +82,18 @@ def test_viscous_adjoint_is_transpose_of_tangent(name):
         assert abs(lk - terms[k + 1]) <= 1e-12 * (abs(lk) + abs(terms[k + 1]) + 1e-300) + 1e-14 * scale
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_adjoint_without_volume_sensitivities(name):
+    """the records drop the volume share when no volume sensitivity is asked for: the state and
+    inlet adjoints are unchanged, bit for bit"""
+    m, s, g, u, gpu, dt = _setup(name)
+    lam = np.random.default_rng(12).normal(size=(5, m.ncell_int))
+    full = gpu.residual_adjoint(dt, lam, area_vectors=False)
+    part = gpu.residual_adjoint(dt, lam, area_vectors=False, volumes=False)
+    assert part["vol"] is None
+    assert np.array_equal(full["u"], part["u"]) and np.array_equal(full["q"], part["q"])
+
+
 def test_viscous_area_vector_derivatives_are_refused():
     m, s, g, u, gpu, dt = _setup("channel_full")
     with pytest.raises(Exception, match="area-vector"):

@@ -51,7 +51,8 @@ class Residual(torch.autograd.Function):
         (U,) = ctx.saved_tensors
         s = ctx.solver
         s.set_state(U.detach().cpu().numpy())
-        g = s.residual_adjoint(ctx.dt, gR.detach().cpu().numpy(), area_vectors=ctx.needs_input_grad[1])
+        g = s.residual_adjoint(ctx.dt, gR.detach().cpu().numpy(), area_vectors=ctx.needs_input_grad[1],
+                               volumes=ctx.needs_input_grad[2])
         t = [None if g[k] is None else torch.from_numpy(np.ascontiguousarray(g[k])) for k in ("u", "s", "vol", "q")]
         return t[0], t[1], t[2], t[3], None, None
 

@@ -16,8 +16,11 @@ struct AdArgs {
                           // volume sensitivity; RW = nv (+ dim^2 + dim) + 1
     const unsigned char* fslot;  // [2][nf] slot of each device face in its owner's / other cell's list
     const double* qrec;   // [n][nv] primitive records (rho, p, u..) of k.us (the HLLC face kernels)
-    double* lrec;         // [n][nv] work: lambda dt / V per cell, one record per cell
-    // viscous residuals (k.grad holds the cell gradients of the state)
+    double* lrec;         // [n][nv] lambda dt / V per cell, one record per cell (HLLC face kernels),
+                          // filled by the caller
+    // viscous residuals (k.grad holds the cell gradients of the state, vgrad their viscous records
+    // [n][dim^2 + dim]: velocity gradient and temperature gradient)
+    const double* vgrad;
     const double* fgeo;   // [4][nf] work: the viscous distance weight dHL and the canonical centre
                           // difference d per face
     double* gbx;          // [nb][2 dim] work: a boundary face's adjoint of its interior cell's density
@@ -50,6 +53,9 @@ struct Comb {
     int m;
 };
 hipError_t launch_comb(double* out, const Comb& cb, long len, hipStream_t s);
+// lrec[c][v] = (sum_i c[i] v[i][v][c]) dt / vol[c]: the combination as lambda dt / V records
+hipError_t launch_comb_lrec(double* lrec, const Comb& cb, const double* vol, const double* d_dt, int nv, int n,
+                            hipStream_t s);
 
 template <int DIM>
 hipError_t launch_res_tangent(const AdArgs& t, hipStream_t s);

@@ -446,17 +446,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face_geom(AdArgs t)
     }
 }
 
-// lrec[c] = lambda_c dt / V_c as one record per cell
-template <int DIM>
-__global__ __launch_bounds__(BLOCK) void k_adj_lrec(AdArgs t)
-{
-    const KArgs<double>& a = t.k;
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= a.n) return;
-    const double k = *a.d_dt / a.vol[c];
-#pragma unroll
-    for (int v = 0; v < DIM + 2; ++v) t.lrec[(size_t)c * (DIM + 2) + v] = t.lam[(size_t)v * a.n + c] * k;
-}
+
 
 // HLLC/HLLCB faces, one work-item per device face: one forward and one reverse sweep of the face
 // flux (the cell-parallel form would evaluate every two-sided face twice).  The face weight is
@@ -505,17 +495,22 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
         load_q<DIM>(t, cr, qR);
     }
     hllc_rev<DIM>(a.ph.sch_hyp, a.ph.gamma, qL, qR, nv, mv, Fv, qLb, qRb, gn);
-    // side records at the slot each side keeps the face in: [slot][cell][RW]
-    double* out = t.face + ((size_t)t.fslot[f] * n + cl) * RW;
-    double* oto = bnd ? out : t.face + ((size_t)t.fslot[(size_t)t.nf + f] * n + cr) * RW;
+    // side records at the slot each side keeps the face in: [slot][cell][rw]; the volume share
+    // (last entry) only when volume sensitivities are asked for
+    const bool vol = t.gvol != nullptr;
+    const int rw = vol ? RW : RW - 1;
+    double* out = t.face + ((size_t)t.fslot[f] * n + cl) * rw;
+    double* oto = bnd ? out : t.face + ((size_t)t.fslot[(size_t)t.nf + f] * n + cr) * rw;
+    // gradients: full records on boundary faces (the ghost gradient map acts on them), viscous
+    // records on two-sided faces
     constexpr int NB = VISC ? (BND ? NG : NC) : 1;
     double gLb[NB], gRb[NB];
     if (VISC) {
-        double gL[NG], gR[NG], dk[DIM];
-#pragma unroll
-        for (int q = 0; q < NG; ++q) gL[q] = a.grad[(size_t)cl * NG + q];
+        double gL[NB], gR[NB], dk[DIM];
+        const double* gsrc = BND ? a.grad : t.vgrad;
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
+            gL[q] = gsrc[(size_t)cl * NB + q];
             gLb[q] = 0.0;
             gRb[q] = 0.0;
         }
@@ -523,7 +518,7 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
             bc_grad<DIM>(bc, nv, gL, gR);
         } else {
 #pragma unroll
-            for (int q = 0; q < NG; ++q) gR[q] = a.grad[(size_t)cr * NG + q];
+            for (int q = 0; q < NB; ++q) gR[q] = gsrc[(size_t)cr * NB + q];
         }
         const double dHL = t.fgeo[f];
 #pragma unroll
@@ -538,10 +533,12 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
             for (int q = 0; q < NB; ++q) gLb[q] += tb[q];
         }
     }
-    double sL = 0.0;
+    if (vol) {
+        double sL = 0.0;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) sL += lL[v] * Fv[v];
-    out[RW - 1] = -A * sL;
+        for (int v = 0; v < NV; ++v) sL += lL[v] * Fv[v];
+        out[RW - 1] = -A * sL;
+    }
     if (bnd) {
         const int bi = f - t.nfi;
         const bool inlet = bc.kind == BK_INLET_SUP;
@@ -572,12 +569,9 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
             for (int q = 0; q < 2 * DIM; ++q) t.gbx[(size_t)bi * 2 * DIM + q] = gLb[q];
         }
     } else {
-        double gRc[NV], sR = 0.0;
+        double gRc[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            gRc[v] = 0.0;
-            sR += lR[v] * Fv[v];
-        }
+        for (int v = 0; v < NV; ++v) gRc[v] = 0.0;
         prim_rev_q<DIM>(qR, g1d, qRb, gRc);
 #pragma unroll
         for (int v = 0; v < NV; ++v) oto[v] = gRc[v];
@@ -585,7 +579,12 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
 #pragma unroll
             for (int q = 0; q < NB; ++q) oto[NV + q] = gRb[q];
         }
-        oto[RW - 1] = A * sR;
+        if (vol) {
+            double sR = 0.0;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sR += lR[v] * Fv[v];
+            oto[RW - 1] = A * sR;
+        }
     }
     double gLc[NV];
 #pragma unroll
@@ -623,6 +622,8 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n) return;
     const int n = a.n;
+    const bool vol = t.gvol != nullptr;
+    const int rw = vol ? RW : RW - 1;
     double gu[NV], sv = 0.0, gc[VISC ? NC : 1], gx[VISC ? 2 * DIM : 1];
 #pragma unroll
     for (int v = 0; v < NV; ++v) gu[v] = 0.0;
@@ -636,22 +637,21 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
         if (k >= a.ns) break;
-        const int w = a.fw[(size_t)k * n + c];
-        if (w == SLOT_UNUSED) continue;
-        const double* in = t.face + ((size_t)k * n + c) * RW;
+        const int L = a.link[(size_t)k * n + c];
+        if (L == SLOT_UNUSED) continue;
+        const double* in = t.face + ((size_t)k * n + c) * rw;
 #pragma unroll
         for (int v = 0; v < NV; ++v) gu[v] += in[v];
         if (VISC) {
 #pragma unroll
             for (int q = 0; q < NC; ++q) gc[q] += in[NV + q];
-            const int L = a.link[(size_t)k * n + c];
             if (link_boundary(L)) {
                 const double* bx = t.gbx + (size_t)link_index(L) * 2 * DIM;
 #pragma unroll
                 for (int q = 0; q < 2 * DIM; ++q) gx[q] += bx[q];
             }
         }
-        sv += in[RW - 1];
+        if (vol) sv += in[RW - 1];
     }
     if (VISC) {
         const double g1d = a.ph.gamma - 1.0, r = a.ph.rgas;
@@ -817,8 +817,6 @@ hipError_t launch_res_adjoint(const AdArgs& t, hipStream_t s)
 {
     const bool rev = t.k.ph.sch_hyp == 25 || t.k.ph.sch_hyp == 26;
     const dim3 grid(grid_blocks(t.k.n)), fgrid((t.nfi + BLOCK - 1) / BLOCK), bgrid((t.nb + BLOCK - 1) / BLOCK);
-    if (rev || t.k.ph.viscous)
-        hipLaunchKernelGGL((k_adj_lrec<DIM>), dim3((t.k.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, t);
     if (t.k.ph.viscous) {
         if (!rev) return hipErrorInvalidValue;  // the host rejects this case first
         if (t.nfi > 0) hipLaunchKernelGGL((k_res_adjoint_face<DIM, true, false>), fgrid, dim3(BLOCK), 0, s, t);
@@ -857,6 +855,35 @@ hipError_t launch_comb(double* out, const Comb& cb, long len, hipStream_t s)
 {
     if (cb.m < 1 || cb.m > COMB_MAX) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_comb, dim3((unsigned)((len + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, out, cb, len);
+    return hipGetLastError();
+}
+
+// the same sums over [nv][n] vectors, written as one record per cell scaled by dt / V_c (the
+// lambda dt / V records of the HLLC adjoint kernels)
+__global__ __launch_bounds__(BLOCK) void k_comb_lrec(double* out, Comb cb, const double* vol, const double* d_dt,
+                                                     int nv, int n)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= n) return;
+    const double k = *d_dt / vol[c];
+    double s[NVMAX];
+#pragma unroll
+    for (int v = 0; v < NVMAX; ++v) s[v] = 0.0;
+    for (int i = 0; i < cb.m; ++i)
+#pragma unroll
+        for (int v = 0; v < NVMAX; ++v)
+            if (v < nv) s[v] += cb.c[i] * cb.v[i][(size_t)v * n + c];
+#pragma unroll
+    for (int v = 0; v < NVMAX; ++v)
+        if (v < nv) out[(size_t)c * nv + v] = s[v] * k;
+}
+
+hipError_t launch_comb_lrec(double* lrec, const Comb& cb, const double* vol, const double* d_dt, int nv, int n,
+                            hipStream_t s)
+{
+    if (cb.m < 1 || cb.m > COMB_MAX || nv > NVMAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_comb_lrec, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, lrec, cb, vol, d_dt,
+                       nv, n);
     return hipGetLastError();
 }
 

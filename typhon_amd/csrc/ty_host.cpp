@@ -1684,13 +1684,17 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
     if (c->ph.viscous) {
         // the cell gradients of the current state, as the residual kernels compute them
         if ((rc = ensure_work(c, 0))) return rc;
+        double* d_vg = nullptr;
+        if ((rc = tmp.alloc(&d_vg, (size_t)(c->dim * c->dim + c->dim) * c->n))) return rc;
         KArgs<double> a = make_args<double>(c, c->geo_d);
         a.us = d_st;
         a.qf_out = nullptr;
         a.vrec = 0;  // the derivative kernels read the full gradients of (rho, p, u..)
+        a.vrec_out = d_vg;  // and the viscous records
         const hipError_t e = c->dim == 2 ? launch_gradient<2, double>(a, c->stream) : launch_gradient<3, double>(a, c->stream);
         HIPCHK(c, e);
         t.k.grad = (const double*)c->work_grad[0];
+        t.vgrad = d_vg;
     }
     return TY_OK;
 }
@@ -1806,10 +1810,19 @@ int ty_residual_adjoint(ty_ctx* c, double dt, const double* lam, double* gu, dou
     host_to_dev_vec(c, lam, h);
     if ((rc = tmp.alloc(&d_lam, h.size())) || (rc = tmp.alloc(&d_gu, h.size()))) return rc;
     HIPCHK(c, hipMemcpy(d_lam, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = alloc_adjoint_work(c, t, tmp))) return rc;
+    {
+        // lambda dt / V as one record per cell (the HLLC face kernels), marshalled with the upload
+        std::vector<double> lr((size_t)nv * n);
+        parallel_for(n, [&](int i) {
+            const double k = dt / c->vol[i];
+            for (int v = 0; v < nv; ++v) lr[(size_t)i * nv + v] = h[(size_t)v * n + i] * k;
+        });
+        HIPCHK(c, hipMemcpy(t.lrec, lr.data(), lr.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     if (gs && (rc = tmp.alloc(&d_gs, (size_t)3 * nf))) return rc;
     if (gvol && (rc = tmp.alloc(&d_gv, (size_t)n))) return rc;
     if (gq && (rc = tmp.alloc(&d_gq, (size_t)nv * std::max(1, nb)))) return rc;
-    if ((rc = alloc_adjoint_work(c, t, tmp))) return rc;
     t.lam = d_lam;
     t.gu = d_gu;
     t.gs = d_gs;
@@ -1920,7 +1933,8 @@ struct Unsteady {
         return TY_OK;
     }
 
-    // out = (dR/dU at U)^T m, FP64; inlet sensitivities added to gq_acc
+    // out = (dR/dU at U)^T m, FP64; inlet sensitivities added to gq_acc.  The HLLC kernels read m as
+    // the lambda dt / V records in t.lrec, which the caller has filled from the same combination.
     int adjoint_apply(const double* U, const double* m, double* out)
     {
         AdArgs a = t;
@@ -1932,6 +1946,7 @@ struct Unsteady {
             g.us = rec;
             g.qf_out = nullptr;
             g.vrec = 0;  // full gradients for the adjoint kernels
+            g.vrec_out = const_cast<double*>(t.vgrad);  // and their viscous records
             const hipError_t e = c->dim == 2 ? launch_gradient<2, double>(g, c->stream)
                                              : launch_gradient<3, double>(g, c->stream);
             HIPCHK(c, e);
@@ -1983,7 +1998,11 @@ struct Unsteady {
                 cb.c[cb.m] = coef(m - 1, s);
                 ++cb.m;
             }
-            HIPCHK(c, launch_comb(mu, cb, len, c->stream));
+            // the HLLC kernels take mu as lambda dt / V records, the dual-number kernel as a vector
+            if (c->ph.sch_hyp == 25 || c->ph.sch_hyp == 26)
+                HIPCHK(c, launch_comb_lrec(t.lrec, cb, c->d_vol, c->d_dt, c->nv, c->n, c->stream));
+            else
+                HIPCHK(c, launch_comb(mu, cb, len, c->stream));
             if ((rc = adjoint_apply(Us[s - 1], mu, nu + (size_t)(s - 1) * len))) return rc;
         }
         // lam_k = lam + sum_s nu_s

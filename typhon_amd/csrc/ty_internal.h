@@ -114,6 +114,7 @@ struct KArgs {
                           // gradient (row-major d u_i / d x_j) and the temperature gradient
     R* grad_out;          // written by K1 when not null
     int vrec;             // K1 writes (and K2 reads) viscous records
+    R* vrec_out;          // with vrec unset: K1 also writes each cell's viscous record here (adjoint)
     const R* gbnd;        // [nb][dim*dim + dim] viscous records of the ghost side of each boundary face
     R* gbnd_out;          // written by K1 (vrec)
     const R* qf;          // [ns][n][nv] own-side face states (second order), one record per slot

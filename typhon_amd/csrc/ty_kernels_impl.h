@@ -630,16 +630,18 @@ __device__ __forceinline__ void side_temp(const Physics& ph, const R* q, R ir, c
 // instead of nv dim) and, for the ghost side of each boundary face, the gradients of the
 // cell_grad boundary map (bc_grad) with the ghost's temperature gradient.
 template <int DIM, typename R>
-__device__ __forceinline__ void write_visc_records(const KArgs<R>& a, int c, const R* qc, const R* gr)
+__device__ __forceinline__ void write_visc_records(const KArgs<R>& a, int c, const R* qc, const R* gr, R* dst,
+                                                   bool ghosts)
 {
     constexpr int VR = DIM * DIM + DIM;
-    R* go = a.grad_out + (size_t)c * VR;
+    R* go = dst + (size_t)c * VR;
 #pragma unroll
     for (int k = 0; k < DIM * DIM; ++k) go[k] = gr[2 * DIM + k];
     R T, gT[DIM];
     side_temp<DIM>(a.ph, qc, arcp(qc[0]), gr, T, gT);
 #pragma unroll
     for (int e = 0; e < DIM; ++e) go[DIM * DIM + e] = gT[e];
+    if (!ghosts) return;
     for (int k = 0; k < a.ns; ++k) {
         const int L = a.link[(size_t)k * a.n + c];
         if (L == SLOT_UNUSED || !link_boundary(L)) continue;
@@ -700,11 +702,12 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
         }
     }
     if (a.grad_out) {
-        if (a.vrec) write_visc_records<DIM>(a, c, qc, gr);
+        if (a.vrec) write_visc_records<DIM>(a, c, qc, gr, a.grad_out, true);
         else {
             R* go = a.grad_out + (size_t)c * NV * DIM;
 #pragma unroll
             for (int k = 0; k < NV * DIM; ++k) go[k] = gr[k];
+            if (a.vrec_out) write_visc_records<DIM>(a, c, qc, gr, a.vrec_out, false);
         }
     }
     if (REC == REC_NONE) return;

@@ -5,10 +5,11 @@
 // qLb += (dF/dqL)^T Fb, qRb += (dF/dqR)^T Fb, gLb += (dF/dgL)^T Fb, gRb += (dF/dgR)^T Fb.
 // The face geometry (distance weight dHL, dHR = 1 - dHL, vLR, n) is held fixed: this is the
 // state adjoint.  Gradient records: [rho (DIM), p (DIM), d u_i / d x_j at (2 + i) DIM + j].
-// COMPACT: the sweep stops at the temperature gradient grad T = k1 grad p - k2 grad rho of each
-// side (k1 = 1/(rho r), k2 = p/(rho^2 r)): gLb / gRb are [d u_i / d x_j (DIM^2), grad T (DIM)]
-// and qLb / qRb leave out grad T's dependence on (rho, p), which the per-cell gather adds once
-// per cell from the summed grad T adjoint (it is linear in it).
+// COMPACT: each side's gradients come as its viscous record [d u_i / d x_j (DIM^2), grad T (DIM)]
+// (grad T = k1 grad p - k2 grad rho, k1 = 1/(rho r), k2 = p/(rho^2 r), formed by the gradient
+// kernel) and gLb / gRb are the adjoints of those records; qLb / qRb then leave out grad T's
+// dependence on (rho, p), which the per-cell gather adds once per cell from the summed grad T
+// adjoint (it is linear in it).
 #pragma once
 #include "ty_kernels_impl.h"
 
@@ -32,11 +33,12 @@ __device__ __forceinline__ void viscous_rev(const Physics& ph, const double* qL,
     const double rhoH = dHR * qL[0] + dHL * qR[0];
     const double kl1 = 1.0 / (qL[0] * r), kl2 = qL[1] / (qL[0] * qL[0] * r);
     const double kr1 = 1.0 / (qR[0] * r), kr2 = qR[1] / (qR[0] * qR[0] * r);
+    constexpr int GU = COMPACT ? 0 : 2 * DIM;  // velocity-gradient offset in the gradient records
     double gTL[DIM], gTR[DIM];
 #pragma unroll
     for (int e = 0; e < DIM; ++e) {
-        gTL[e] = kl1 * gL[DIM + e] - kl2 * gL[e];
-        gTR[e] = kr1 * gR[DIM + e] - kr2 * gR[e];
+        gTL[e] = COMPACT ? gL[DIM * DIM + e] : kl1 * gL[DIM + e] - kl2 * gL[e];
+        gTR[e] = COMPACT ? gR[DIM * DIM + e] : kr1 * gR[DIM + e] - kr2 * gR[e];
     }
     double mu, dmu_dTH = 0.0, dmu_drhoH = 0.0;
     if (ph.visc_law == 11) {
@@ -52,7 +54,7 @@ __device__ __forceinline__ void viscous_rev(const Physics& ph, const double* qL,
 #pragma unroll
     for (int i = 0; i < DIM; ++i)
 #pragma unroll
-        for (int j = 0; j < DIM; ++j) G0[i][j] = dHL * gR[(2 + i) * DIM + j] + dHR * gL[(2 + i) * DIM + j];
+        for (int j = 0; j < DIM; ++j) G0[i][j] = dHL * gR[GU + i * DIM + j] + dHR * gL[GU + i * DIM + j];
     const double dLR2 = dotd<DIM>(vLR, vLR);
     double gavg[DIM];
 #pragma unroll
@@ -153,7 +155,6 @@ __device__ __forceinline__ void viscous_rev(const Physics& ph, const double* qL,
         }
     }
     // G0 = dHL gR_u + dHR gL_u
-    constexpr int GU = COMPACT ? 0 : 2 * DIM;  // velocity-gradient offset in gLb / gRb
 #pragma unroll
     for (int i = 0; i < DIM; ++i)
 #pragma unroll

@@ -196,18 +196,18 @@ class Solver:
         self._check(self.lib.ty_residual_tangent(self.h, dt, *p, _p64(out)), "ty_residual_tangent")
         return out
 
-    def residual_adjoint(self, dt, lam, area_vectors=True):
+    def residual_adjoint(self, dt, lam, area_vectors=True, volumes=True):
         """transpose of residual_tangent: dict u [5, ncell_int], s [nface, 3] (None unless
-        area_vectors; not provided for viscous residuals), vol [ncell_int], q [nboco, 5] with the
-        products (dR/d.)^T lam"""
+        area_vectors; not provided for viscous residuals), vol [ncell_int] (None unless volumes),
+        q [nboco, 5] with the products (dR/d.)^T lam"""
         m = self.mesh
         lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(5, self.ni)
         gu = np.empty((5, self.ni))
         gs = np.empty((m.nface, 3)) if area_vectors else None
-        gv = np.empty(self.ni)
+        gv = np.empty(self.ni) if volumes else None
         gq = np.empty((max(1, len(m.bocos)), 5))
         self._check(self.lib.ty_residual_adjoint(self.h, dt, _p64(lam), _p64(gu), None if gs is None else _p64(gs),
-                                                 _p64(gv), _p64(gq)), "ty_residual_adjoint")
+                                                 None if gv is None else _p64(gv), _p64(gq)), "ty_residual_adjoint")
         return dict(u=gu, s=gs, vol=gv, q=gq[:len(m.bocos)])
 
     def unsteady_adjoint(self, dts, lamN, snaps=20):
