@@ -75,6 +75,18 @@ type, bind(C) :: ty_unsteady_stats
   real(c_double)     :: ms
 endtype ty_unsteady_stats
 
+! steady cycle settings and statistics (integration_cyclezone for time_steady)
+type, bind(C) :: ty_steady
+  integer(c_int32_t) :: local_dt, maxit, check_every
+  real(c_double)     :: cfl, cfl_max, maxres, residu_ref, wlog
+  real(c_double)     :: refsca(2), refvec
+endtype ty_steady
+
+type, bind(C) :: ty_steady_stats
+  integer(c_int32_t) :: iterations, converged
+  real(c_double)     :: cur_res, residu_ref, cfl
+endtype ty_steady_stats
+
 interface
 
   integer(c_int) function ty_create(ctx, device) bind(C, name="ty_create")
@@ -234,6 +246,26 @@ interface
     real(c_double)              :: dts(*), lamN(*), lam0(*)
     type(c_ptr), value          :: gq
     type(ty_unsteady_stats)     :: stats
+  endfunction
+
+  integer(c_int) function ty_refcons(ctx, out) bind(C, name="ty_refcons")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value    :: ctx
+    real(c_double)        :: out(3)
+  endfunction
+
+  integer(c_int) function ty_local_timestep(ctx, dt_cells, dt_min) bind(C, name="ty_local_timestep")
+    import :: c_int, c_ptr, c_double
+    type(c_ptr), value    :: ctx
+    real(c_double)        :: dt_cells(*), dt_min
+  endfunction
+
+  integer(c_int) function ty_steady_cycle(ctx, steady, res_hist, stats) bind(C, name="ty_steady_cycle")
+    import :: c_int, c_ptr, c_double, ty_steady, ty_steady_stats
+    type(c_ptr), value    :: ctx
+    type(ty_steady)       :: steady
+    real(c_double)        :: res_hist(*)
+    type(ty_steady_stats) :: stats
   endfunction
 
 endinterface

@@ -238,6 +238,51 @@ typedef struct ty_unsteady_stats {
  * again on return.  Same scheme and boundary limits as ty_residual_adjoint, any precision. */
 int ty_unsteady_adjoint(ty_ctx* ctx, int32_t nsteps, const double* dts, int32_t snaps, const double* lamN,
                         double* lam0, double* gq, ty_unsteady_stats* stats);
+/* Steady mode (time_steady).  Reference values of the conserved variables: calc_refcons
+ * (SOURCE/ZONE/calc_refcons.f90:20-117) with define_refcons (SOURCE/PARAM/MENU_SOLVER.f90:262-276),
+ * out[3] = volume averages of rho and rhoE over the interior cells of the current state, and the
+ * momentum reference sqrt(out[0] out[1]) (defsolver%refsca(1:2), defsolver%refvec(1)). */
+int ty_refcons(ty_ctx* ctx, double* out);
+
+/* per-cell time steps of loc_stab_cond at the scheme's CFL number (calc_zonetimestep.f90:64-66 with
+ * calc_ns_timestep.f90:52-102, not reset to the minimum): dt_cells[ncell_int] in the caller's cell
+ * order, dt_min (may be NULL) their minimum.  TY_ERR_STATE on a bad state. */
+int ty_local_timestep(ty_ctx* ctx, double* dt_cells, double* dt_min);
+
+/* One steady cycle of integration_cyclezone (SOURCE/MAIN/integration_cyclezone.f90:80-167). */
+typedef struct {
+    int32_t local_dt;        /* 1: loc_stab_cond (each cell's own time step), 0: stab_cond (global minimum) */
+    int32_t maxit;           /* iterations of the cycle at most (zinfo%maxit, integration_cycle.f90:92) */
+    int32_t check_every;     /* iterations between the host's looks at the device's stop flag (>= 1) */
+    double cfl;              /* deftime%stabnb */
+    double cfl_max;          /* deftime%stabnb_max (CFL_MAX, def_time.f90:73) */
+    double maxres;           /* zone convergence cur_res / residu_ref <= maxres (check_end_cycle.f90:33) */
+    double residu_ref;       /* the zone's reference residual at the cycle start (integration_cycle.f90:94) */
+    double wlog;             /* log10(wcur_res / wres_ref) of the world (calc_zonetimestep.f90:70-72; 0 for one
+                                zone in its first cycle) */
+    double refsca[2];        /* defsolver%refsca (rho, rhoE), from ty_refcons at the initial state */
+    double refvec;           /* defsolver%refvec */
+} ty_steady;
+
+typedef struct {
+    int32_t iterations;      /* iterations performed */
+    int32_t converged;       /* cur_res / residu_ref <= maxres at the end */
+    double cur_res;          /* update_field's residual of the last iteration */
+    double residu_ref;       /* the zone's reference residual at the end */
+    double cfl;              /* CFL number the next iteration would use */
+} ty_steady_stats;
+
+/* Per iteration, all on the device: the time step with the steady CFL ramp
+ * cfl = min(stabnb max(1, 1 - log10(cur_res / residu_ref) - wlog), stabnb_max) (calc_zonetimestep.f90:
+ * 64-73), the scheme's RK step with the local or global time step (treelevel_rungekutta.f90:50-113),
+ * update_field's residual (update_field.f90:42-60: sum over interior cells of |residu|/refsca for rho
+ * and rhoE and |residu momentum|/refvec, residu = the step's increment), check_end_cycle
+ * (check_end_cycle.f90:32-33: residu_ref = max(residu_ref, cur_res), stop at cur_res/residu_ref <= maxres
+ * or maxit iterations).  res_hist[maxit] (may be NULL) receives cur_res of every iteration (the monitor
+ * of write_monitors_iteration.f90:32-35).  The host synchronises only every check_every iterations;
+ * launches after the stop do nothing, so the iteration count is the reference's. */
+int ty_steady_cycle(ty_ctx* ctx, const ty_steady* steady, double* res_hist, ty_steady_stats* stats);
+
 /* per-kernel timing for benchmarks (no reference counterpart): when enabled, every
  * gradient, flux, time-step, tangent and adjoint launch is bracketed by HIP events on the
  * context's stream.  Returns the summed milliseconds and launch counts recorded since the

@@ -58,6 +58,10 @@ def _load():
     lib.or_timestep.argtypes = [C.c_void_p, C.c_double, C.c_double]
     lib.or_calc_rhs.argtypes = [C.c_void_p, C.c_double, F64P]
     lib.or_rk_step.argtypes = [C.c_void_p, C.c_double]
+    lib.or_rk_step_local.argtypes = [C.c_void_p]
+    lib.or_residual_norm.restype = C.c_double
+    lib.or_residual_norm.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+    lib.or_refcons.argtypes = [C.c_void_p, F64P]
     for name in ("or_prim", "or_grad", "or_face_left", "or_face_right", "or_flux"):
         getattr(lib, name).restype = F64P
         getattr(lib, name).argtypes = [C.c_void_p]
@@ -77,6 +81,8 @@ def _load():
     lib.or_implicit_bdlu_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int]
     lib.or_res_all.restype = F64P
     lib.or_res_all.argtypes = [C.c_void_p]
+    lib.or_dtloc.restype = F64P
+    lib.or_dtloc.argtypes = [C.c_void_p]
     lib.or_meshgeom.restype = C.c_int
     lib.or_meshgeom.argtypes = [C.c_int, C.c_int, F64P, C.c_int, I32P, I32P, I32P, C.c_int,
                                 F64P, F64P, F64P, F64P, F64P]
@@ -195,6 +201,22 @@ class Oracle:
 
     def rk_step(self, dt):
         self.lib.or_rk_step(self.h, dt)
+
+    def rk_step_local(self):
+        """RK step with the per-cell time steps of the last timestep() call (loc_stab_cond)"""
+        self.lib.or_rk_step_local(self.h)
+
+    def local_dt(self):
+        """per-cell time steps of the last timestep() call [ncell_int]"""
+        return np.ctypeslib.as_array(self.lib.or_dtloc(self.h), shape=(self.mesh.ncell_int,)).copy()
+
+    def residual_norm(self, refsca0, refsca1, refvec):
+        return self.lib.or_residual_norm(self.h, refsca0, refsca1, refvec)
+
+    def refcons(self):
+        out = np.empty(3)
+        self.lib.or_refcons(self.h, _p64(out))
+        return out
 
     def view(self, name, nvar, n):
         p = getattr(self.lib, name)(self.h)

@@ -153,6 +153,7 @@ const double* or_face_left(const or_ctx* o) { return o->qL; }
 const double* or_face_right(const or_ctx* o) { return o->qR; }
 const double* or_flux(const or_ctx* o) { return o->flux; }
 const double* or_res_all(const or_ctx* o) { return o->res; }
+const double* or_dtloc(const or_ctx* o) { return o->dtloc; }
 
 /* ---------------------------------------------------------------------------
  * A2: calc_varprim_ns (SOURCE/EQNS/calc_varprim_ns.f90:37-57), all cells */
@@ -968,7 +969,11 @@ static void ns_bocoflux(or_ctx* o)
 
 /* ---------------------------------------------------------------------------
  * calc_rhs (SOURCE/MAIN/calc_rhs.f90:48-145) for one grid with global dt. */
-static void calc_rhs_internal(or_ctx* o, double dt)
+static void calc_rhs_dt(or_ctx* o, double dt, const double* dtl);
+static void calc_rhs_internal(or_ctx* o, double dt) { calc_rhs_dt(o, dt, NULL); }
+
+/* dtl: per-cell time steps (loc_stab_cond) or NULL (the global dt) */
+static void calc_rhs_dt(or_ctx* o, double dt, const double* dtl)
 {
     const or_config* c = &o->c;
     varprim(o);
@@ -1019,7 +1024,7 @@ static void calc_rhs_internal(or_ctx* o, double dt)
     }
 #pragma omp parallel for if (o->ni > OR_PAR_MIN) schedule(static)
     for (int i = 0; i < o->ni; ++i) {
-        double coef = dt / c->cvol[i];
+        double coef = (dtl ? dtl[i] : dt) / c->cvol[i];
         for (int v = 0; v < NV; ++v) RES(o, v, i) = coef * RES(o, v, i);
     }
 }
@@ -1061,9 +1066,52 @@ double or_timestep(or_ctx* o, double cfl, double dtmax)
     return dt;
 }
 
+static void rk_step_dt(or_ctx* o, double dt, const double* dtl);
+
 /* A14: treelevel_rungekutta (SOURCE/MAIN/treelevel_rungekutta.f90:50-113) and
    update_field (SOURCE/ZONE/update_field.f90:38) with global dt. */
-void or_rk_step(or_ctx* o, double dt)
+void or_rk_step(or_ctx* o, double dt) { rk_step_dt(o, dt, NULL); }
+
+/* the same with the per-cell time steps of the last or_timestep (loc_stab_cond,
+   calc_zonetimestep.f90:64-66: dtloc is not reset to the minimum) */
+void or_rk_step_local(or_ctx* o) { rk_step_dt(o, 0.0, o->dtloc); }
+
+/* A14: update_field's steady residual (SOURCE/ZONE/update_field.f90:42-60) of the last RK step:
+   sum over interior cells of |residu| / refsca for rho and rhoE, plus |residu momentum| (the
+   vector's norm) / refvec, with residu the step's increment. */
+double or_residual_norm(const or_ctx* o, double refsca0, double refsca1, double refvec)
+{
+    int ni = o->ni;
+    double r = 0.0, s0 = 0.0, s1 = 0.0;
+    for (int i = 0; i < ni; ++i) s0 += fabs(RES(o, 0, i));
+    r = r + s0 / refsca0;
+    for (int i = 0; i < ni; ++i) s1 += fabs(RES(o, 1, i));
+    r = r + s1 / refsca1;
+    for (int i = 0; i < ni; ++i) {
+        double m = sqrt(RES(o, 2, i) * RES(o, 2, i) + RES(o, 3, i) * RES(o, 3, i) + RES(o, 4, i) * RES(o, 4, i));
+        r = r + m / refvec;
+    }
+    return r;
+}
+
+/* calc_refcons (SOURCE/ZONE/calc_refcons.f90:20-117) with define_refcons
+   (SOURCE/PARAM/MENU_SOLVER.f90:262-276) for the NS solver: volume averages of rho and rhoE over
+   the interior cells, and the momentum reference sqrt(rho_ref rhoE_ref).  out[3]. */
+void or_refcons(const or_ctx* o, double* out)
+{
+    const or_config* c = &o->c;
+    double v = 0.0, s0 = 0.0, s1 = 0.0;
+    for (int i = 0; i < o->ni; ++i) {
+        v += c->cvol[i];
+        s0 += CONS(o, 0, i) * c->cvol[i];
+        s1 += CONS(o, 1, i) * c->cvol[i];
+    }
+    out[0] = s0 / v;
+    out[1] = s1 / v;
+    out[2] = sqrt(out[0] * out[1]);
+}
+
+static void rk_step_dt(or_ctx* o, double dt, const double* dtl)
 {
     const or_config* c = &o->c;
     int ns = c->nstage, nc = o->nc;
@@ -1081,7 +1129,7 @@ void or_rk_step(or_ctx* o, double dt)
                 for (size_t k = 0; k < sz; ++k) o->cons[k] = o->cons[k] + a * r[k];
             }
         }
-        calc_rhs_internal(o, dt);
+        calc_rhs_dt(o, dt, dtl);
         memcpy(rhs + (size_t)(s - 1) * sz, o->res, sizeof(double) * sz);
     }
     /* residu = coef(n,1) R1 + sum coef(n,j) Rj ; U = U0 + residu */

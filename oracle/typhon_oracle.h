@@ -125,6 +125,12 @@ void or_calc_rhs(or_ctx* o, double dt, double* res);
 
 /* one explicit RK step with global dt (treelevel_rungekutta + update_field) */
 void or_rk_step(or_ctx* o, double dt);
+/* RK step with the per-cell time steps of the last or_timestep (loc_stab_cond) */
+void or_rk_step_local(or_ctx* o);
+/* update_field's steady residual of the last RK step (update_field.f90:42-60) */
+double or_residual_norm(const or_ctx* o, double refsca0, double refsca1, double refvec);
+/* calc_refcons + define_refcons: rho_ref, rhoE_ref, sqrt(rho_ref rhoE_ref) of the current state */
+void or_refcons(const or_ctx* o, double* out);
 
 /* debug views of the last calc_rhs: prim [5][ncell], grad [15][ncell], face states [5][nface], flux [5][nface] */
 const double* or_prim(const or_ctx* o);
@@ -155,6 +161,8 @@ int or_bdlu_gmres(const or_ctx* o, const double* diag, const double* upper, cons
 int or_implicit_bdlu_step(or_ctx* o, double dt, int nkrylov, double maxres, int max_it);
 /* residual of every cell after or_calc_rhs, ghost rows included ([5][ncell]) */
 const double* or_res_all(const or_ctx* o);
+/* per-cell time steps of the last or_timestep ([ncell_int]) */
+const double* or_dtloc(const or_ctx* o);
 
 /* MESHGEOM restatement (A1): geometry of faces given by vertex lists.
    fvtx [nface][4] 0-based vertex ids (-1 padded), ic1/ic2 in/out (swapped so ic1<ic2 when ic2>=0,

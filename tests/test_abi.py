@@ -28,6 +28,11 @@ STRUCTS = {
     "ty_krylov": (_lib.TyKrylov, ["nkrylov", "max_restart", "max_outer", "inner_precision", "precond", "inner_tol",
                                   "outer_tol"]),
     "ty_krylov_stats": (_lib.TyKrylovStats, ["outer_iters", "inner_iters", "rel_res", "rhs_norm"]),
+    "ty_unsteady_stats": (_lib.TyUnsteadyStats, ["forward_steps", "adjoint_steps", "max_snapshots",
+                                                 "snapshots_allocated", "ms"]),
+    "ty_steady": (_lib.TySteady, ["local_dt", "maxit", "check_every", "cfl", "cfl_max", "maxres", "residu_ref",
+                                  "wlog", "refsca", "refvec"]),
+    "ty_steady_stats": (_lib.TySteadyStats, ["iterations", "converged", "cur_res", "residu_ref", "cfl"]),
 }
 
 

@@ -23,7 +23,7 @@ def test_oracle_matches_nrg_golden(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["RK2", "RK4", "RK25BB"])
+@pytest.mark.parametrize("case", list(nrg.CASES))
 def test_gpu_matches_nrg_golden(case):
     from typhon_amd.solver import Solver
     m, s, g, u = nrg.setup(case)

@@ -1,0 +1,197 @@
+"""Steady mode (time_steady): reference values (calc_refcons), local time steps (loc_stab_cond),
+update_field's residual, the CFL ramp and check_end_cycle.
+
+CPU: the oracle's pieces against closed forms and observable quantities --
+  * refcons: volume averages on a uniform mesh with a two-valued state (exact), momentum
+    reference sqrt(rho_ref rhoE_ref) (define_refcons);
+  * local time steps: cfl 2 V / sum(A) / (|u| + a) per cell with sum(A) taken from the mesh's
+    face lists;
+  * the residual of a step is the state increment measured from the states before and after,
+    with the momentum vector's norm (not its components);
+  * a steady cycle of a channel with a density bump converges to the uniform freestream.
+GPU: ty_local_timestep, ty_refcons and ty_steady_cycle against the oracle (the residual history
+iteration by iteration, the iteration count, the converged state), on synthetic meshes and on the
+reference's quad20x10.tym.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cases as K
+import steady
+from typhon_amd import mesh as M
+from typhon_amd import scheme as S
+from typhon_amd.solver import prim_to_cons
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_refcons_closed_form():
+    m = M.channel_quad(8, 4, 1.0, 0.5, 0.0, 1)
+    n = m.ncell_int
+    x = m.ccent[:n, 0]
+    rho = np.where(x < 0.5, 1.0, 2.0)
+    p = np.where(x < 0.5, 1.0, 3.0)
+    vel = np.vstack([np.full(n, 0.5), np.zeros(n), np.zeros(n)])
+    o = K.make_oracle(m, S.default_scheme(), S.default_gas())
+    o.set_cons(prim_to_cons(rho, p, vel))
+    r = o.refcons()
+    # half the volume at each value: rho 1.5; rhoE = p/0.4 + rho/8: (2.5 + 0.125 + 7.5 + 0.25) / 2
+    assert abs(r[0] - 1.5) < 1e-15
+    assert abs(r[1] - (2.625 + 7.75) / 2) < 1e-14
+    assert abs(r[2] - np.sqrt(r[0] * r[1])) < 1e-15
+
+
+def test_local_timestep_closed_form():
+    m, s, g, u = K.channel_case(12, 6, seed=3)
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u)
+    dtmin = o.timestep(0.7)
+    dl = o.local_dt()
+    n = m.ncell_int
+    suma = np.zeros(m.ncell)
+    np.add.at(suma, m.ic1, m.fsurf)
+    np.add.at(suma, m.ic2, m.fsurf)
+    rho, rhoe, mom = u[0], u[1], u[2:5]
+    vmag = np.linalg.norm(mom, axis=0) / rho
+    a = np.sqrt(1.4 * 0.4 * (rhoe - 0.5 * (mom ** 2).sum(0) / rho) / rho)
+    exp = 0.7 * 2.0 * m.cvol[:n] / suma[:n] / (vmag + a)
+    np.testing.assert_allclose(dl, exp, rtol=1e-13)
+    assert dtmin == dl.min()
+
+
+def test_residual_is_the_state_increment():
+    m, s, g, u = K.bump_case(6, 3, 3, noise=0.05)
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u)
+    o.timestep(0.8)
+    U0 = o.get_cons()
+    o.rk_step_local()
+    dU = o.get_cons() - U0
+    ref = (1.3, 2.1, 0.7)
+    exp = np.abs(dU[0]).sum() / ref[0] + np.abs(dU[1]).sum() / ref[1] + np.linalg.norm(dU[2:5], axis=0).sum() / ref[2]
+    assert abs(o.residual_norm(*ref) - exp) <= 1e-12 * exp
+    # the momentum enters through the vector's norm: a componentwise sum would differ here
+    assert np.abs(dU[3]).max() > 1e-3 * np.abs(dU[2]).max()
+
+
+@pytest.mark.parametrize("local", [True, False])
+def test_steady_cycle_converges_to_freestream(local):
+    m, s, g, u = K.channel_case(24, 12, tps_meth=S.TPS_RK4)
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u)
+    hist, rref = steady.oracle_cycle(o, 2000, 1.0, 2.0, 1e-6, local)
+    assert len(hist) < 2000 and hist[-1] / rref <= 1e-6
+    U = o.get_cons()
+    # the steady solution of the straight channel is the inflow state (rho 1, u 0.5, v 0)
+    assert np.abs(U[0] - 1.0).max() < 1e-6
+    assert np.abs(U[2] / U[0] - 0.5).max() < 1e-6 and np.abs(U[3]).max() < 1e-6
+
+
+def tym_channel():
+    """the reference's quad20x10.tym (NRG/MESH/ELEM/TYM-QUAD) as a channel: supersonic-inlet
+    Dirichlet state at x = 0, static pressure at x = 1, slip walls, a seeded density bump"""
+    from typhon_amd import tyfmt
+    rho, p, mach = 1.0, 1.0 / 1.4, 0.5
+    bocos = [dict(family="IMIN", type=M.BC_INLET_SUP, param=K.freestream_inlet(rho, p, mach)),
+             dict(family="IMAX", type=M.BC_OUTLET_SUB, param=[p]),
+             dict(family="JMIN", type=M.BC_SYM), dict(family="JMAX", type=M.BC_SYM)]
+    m = tyfmt.tym_ustmesh(os.path.join(HERE, "golden", "meshes", "quad20x10.tym"), bocos)
+    n = m.ncell_int
+    x = m.ccent[:n]
+    c = x.mean(axis=0)
+    r2 = ((x[:, :2] - c[:2]) ** 2).sum(1) / (0.1 * (x[:, 0].max() - x[:, 0].min())) ** 2
+    vel = np.zeros((3, n))
+    vel[0] = mach * np.sqrt(1.4 * p / rho)
+    u = prim_to_cons(rho * (1 + 0.1 * np.exp(-r2)), np.full(n, p), vel)
+    return m, S.default_scheme(sch_hyp=S.SCH_HLLC, tps_meth=S.TPS_RK4, cfl=1.0), S.default_gas(), u
+
+
+STEADY_CASES = {
+    "channel_local_rk4": (lambda: K.channel_case(24, 12, tps_meth=S.TPS_RK4), True, 1.0, 2.0),
+    "channel_global_rk4": (lambda: K.channel_case(24, 12, tps_meth=S.TPS_RK4), False, 1.0, 2.0),
+    "channel_muscl_rk2": (lambda: K.channel_case(16, 8, method="M", limiter="A", tps_meth=S.TPS_RK2), True, 0.5, 0.9),
+    "tym_quad20x10": (tym_channel, True, 1.0, 2.0),
+    "bump_tets": (lambda: K.bump_case(8, 4, 3, tps_meth=S.TPS_RK4), True, 1.0, 1.5),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(STEADY_CASES))
+def test_gpu_local_timestep_and_refcons(name):
+    from typhon_amd.solver import Solver
+    mk, *_ = STEADY_CASES[name]
+    m, s, g, u = mk()
+    gpu = Solver(m, s, g)
+    gpu.set_state(u)
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u)
+    dmin_o = o.timestep(s["cfl"])
+    dl, dmin = gpu.local_timestep()
+    np.testing.assert_allclose(dl, o.local_dt(), rtol=1e-13)
+    assert abs(dmin - dmin_o) <= 1e-13 * dmin_o
+    np.testing.assert_allclose(gpu.refcons(), o.refcons(), rtol=1e-14)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(STEADY_CASES))
+def test_gpu_steady_cycle_matches_oracle(name):
+    from typhon_amd.solver import Solver
+    mk, local, cfl, cmax = STEADY_CASES[name]
+    m, s, g, u = mk()
+    maxit, maxres = 1500, 1e-6
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u)
+    ref = o.refcons()
+    h_o, rr_o = steady.oracle_cycle(o, maxit, cfl, cmax, maxres, local, ref)
+    gpu = Solver(m, s, g)
+    gpu.set_state(u)
+    h_g, st = gpu.steady_cycle(maxit, cfl=cfl, cfl_max=cmax, maxres=maxres, local_dt=local, ref=ref)
+    print(name, "iterations", len(h_o), "final residual ratio", h_o[-1] / rr_o)
+    assert st["iterations"] == len(h_o) and st["converged"] == (h_o[-1] / rr_o <= maxres)
+    # the residual history iteration by iteration: 1e-8 of each entry plus 1e-12 of the cycle's
+    # reference residual (the residual is an increment; near convergence it is 1e-6 of the
+    # reference, where states agreeing to round-off differ in it by ~1e-8 of itself)
+    err = np.abs(h_g - h_o) - (1e-8 * h_o + 1e-12 * rr_o)
+    assert err.max() <= 0.0, (np.abs(h_g - h_o).max() / rr_o, int(err.argmax()))
+    assert abs(st["residu_ref"] - rr_o) <= 1e-12 * rr_o
+    assert K.rel_err(gpu.get_state(), o.get_cons()) <= 1e-11
+
+
+@pytest.mark.gpu
+def test_gpu_steady_host_checks_do_not_change_the_cycle():
+    """the host's looks at the device stop flag only decide when queuing ends: the same
+    iterations and bitwise the same state for every check interval, also when the cycle stops
+    between two looks (maxit or convergence)"""
+    from typhon_amd.solver import Solver
+    m, s, g, u = K.channel_case(16, 8, tps_meth=S.TPS_RK4)
+    out = []
+    for every in (1, 7, 64):
+        gpu = Solver(m, s, g)
+        gpu.set_state(u)
+        h, st = gpu.steady_cycle(1000, cfl=1.0, cfl_max=2.0, maxres=1e-5, check_every=every)
+        h2, st2 = gpu.steady_cycle(40, cfl=1.0, cfl_max=2.0, maxres=1e-12, check_every=every,
+                                   residu_ref=st["residu_ref"])
+        out.append((h, st, h2, st2, gpu.get_state()))
+        gpu.close()
+    for h, st, h2, st2, U in out[1:]:
+        assert np.array_equal(h, out[0][0]) and st == out[0][1]
+        assert np.array_equal(h2, out[0][2]) and st2["iterations"] == 40
+        assert np.array_equal(U, out[0][4])
+
+
+@pytest.mark.gpu
+def test_gpu_steady_cycle_mixed_precision():
+    """FP32 fluxes with FP64 accumulation and residual: the history follows the FP64 one while the
+    residual is above the FP32 round-off of the fluxes"""
+    from typhon_amd.solver import Solver
+    m, s, g, u = K.channel_case(24, 12, tps_meth=S.TPS_RK4)
+    res = {}
+    for prec in (S.PREC_FP64, S.PREC_MIXED):
+        gpu = Solver(m, dict(s, precision=prec), g)
+        gpu.set_state(u)
+        res[prec] = gpu.steady_cycle(60, cfl=1.0, cfl_max=2.0, maxres=1e-12)[0]
+    h64, h32 = res[S.PREC_FP64], res[S.PREC_MIXED]
+    assert len(h64) == len(h32) == 60
+    assert (np.abs(h32 - h64) / h64).max() <= 1e-3

@@ -16,7 +16,8 @@ F64P = C.POINTER(C.c_double)
 EXPORTS = ["ty_create", "ty_destroy", "ty_last_error", "ty_mesh_upload", "ty_set_scheme", "ty_state_set",
            "ty_state_get", "ty_timestep", "ty_residual", "ty_rk_step", "ty_advance", "ty_sync", "ty_timing",
            "ty_mesh_info", "ty_jacobian_diag", "ty_jvp", "ty_linear_solve", "ty_implicit_step",
-           "ty_implicit_bdlu_step", "ty_residual_tangent", "ty_residual_adjoint", "ty_unsteady_adjoint"]
+           "ty_implicit_bdlu_step", "ty_residual_tangent", "ty_residual_adjoint", "ty_unsteady_adjoint",
+           "ty_refcons", "ty_local_timestep", "ty_steady_cycle"]
 
 
 class TyBocoDesc(C.Structure):
@@ -57,6 +58,17 @@ class TyKrylovStats(C.Structure):
 class TyUnsteadyStats(C.Structure):
     _fields_ = [("forward_steps", C.c_int32), ("adjoint_steps", C.c_int32), ("max_snapshots", C.c_int32),
                 ("snapshots_allocated", C.c_int32), ("ms", C.c_double)]
+
+
+class TySteady(C.Structure):
+    _fields_ = [("local_dt", C.c_int32), ("maxit", C.c_int32), ("check_every", C.c_int32), ("cfl", C.c_double),
+                ("cfl_max", C.c_double), ("maxres", C.c_double), ("residu_ref", C.c_double), ("wlog", C.c_double),
+                ("refsca", C.c_double * 2), ("refvec", C.c_double)]
+
+
+class TySteadyStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("cur_res", C.c_double),
+                ("residu_ref", C.c_double), ("cfl", C.c_double)]
 
 
 class TyError(RuntimeError):
@@ -101,6 +113,9 @@ def load():
     lib.ty_residual_adjoint.argtypes = [vp, C.c_double, F64P, F64P, F64P, F64P, F64P]
     lib.ty_unsteady_adjoint.argtypes = [vp, C.c_int32, F64P, C.c_int32, F64P, F64P, F64P,
                                         C.POINTER(TyUnsteadyStats)]
+    lib.ty_refcons.argtypes = [vp, F64P]
+    lib.ty_local_timestep.argtypes = [vp, F64P, F64P]
+    lib.ty_steady_cycle.argtypes = [vp, C.POINTER(TySteady), F64P, C.POINTER(TySteadyStats)]
     for name in EXPORTS:
         if name != "ty_last_error":
             getattr(lib, name).restype = C.c_int

@@ -86,7 +86,15 @@ struct ty_ctx {
     int* d_bcell = nullptr;
     std::vector<unsigned char> fslot;  // [2][nf] slot of each device face in its owner's / other cell's list
     unsigned char* d_fslot = nullptr;
-    double* d_dtfac = nullptr;
+    double* d_sumA = nullptr;  // [n] sum of each cell's face areas (mesh)
+    double* d_cfl = nullptr;   // the scheme's CFL number
+    // steady cycle (ty_steady_cycle): per-cell time steps and residual terms, the cycle's device
+    // state; the kernels read them through make_args while a cycle runs
+    double *d_dtloc = nullptr, *d_rnorm = nullptr, *d_spart = nullptr, *d_hist = nullptr;
+    SteadyDev* d_steady = nullptr;
+    int hist_len = 0;
+    bool steady_on = false, steady_local = false;
+    double steady_ref[3] = {1.0, 1.0, 1.0};
     double* d_u0 = nullptr;
     void* d_us[2] = {nullptr, nullptr};
     int cur = 0;
@@ -161,7 +169,10 @@ void free_state(ty_ctx* c)
     c->d_geo = nullptr;
     c->d_bboco = nullptr;
     c->d_boco = nullptr;
-    c->d_vol = c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
+    c->d_vol = c->d_sumA = c->d_cfl = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
+    c->d_dtloc = c->d_rnorm = c->d_spart = c->d_hist = nullptr;
+    c->d_steady = nullptr;
+    c->hist_len = 0;
     c->d_cc = c->d_fc = c->d_fn = c->d_fs = c->d_gc = c->d_ps = nullptr;
     c->d_bcell = nullptr;
     c->d_fslot = nullptr;
@@ -598,7 +609,8 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         (rc = upload(c, &c->d_fcell, c->fcell)) || (rc = upload(c, &c->d_cc, c->cc)) ||
         (rc = upload(c, &c->d_fc, c->fc)) || (rc = upload(c, &c->d_fn, c->fn)) || (rc = upload(c, &c->d_fs, c->fs)) ||
         (rc = upload(c, &c->d_gc, c->gc)) || (rc = upload(c, &c->d_ps, c->ps)) ||
-        (rc = upload(c, &c->d_bcell, c->bcell)) || (rc = upload(c, &c->d_fslot, c->fslot)))
+        (rc = upload(c, &c->d_bcell, c->bcell)) || (rc = upload(c, &c->d_fslot, c->fslot)) ||
+        (rc = upload(c, &c->d_sumA, c->sumA)))
         return rc;
     phase("mesh: uploads");
     c->mesh_bufs = c->owned;  // everything allocated so far belongs to the mesh
@@ -728,7 +740,10 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     c->d_geo = nullptr;
     c->geo_d = Geo<double>{};
     c->geo_f = Geo<float>{};
-    c->d_dtfac = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
+    c->d_cfl = c->d_u0 = c->d_acc = c->d_rstore = c->d_res = c->d_part = c->d_dt = nullptr;
+    c->d_dtloc = c->d_rnorm = c->d_spart = c->d_hist = nullptr;
+    c->d_steady = nullptr;
+    c->hist_len = 0;
     c->d_us[0] = c->d_us[1] = nullptr;
     c->d_grad = c->d_qf = nullptr;
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
@@ -837,9 +852,7 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     if ((rc = build_geo<float>(c, c->geo_f)) || (rc = build_geo<double>(c, c->geo_d))) return rc;
     if (phase.on) HIPCHK(c, hipStreamSynchronize(c->stream));
     phase("scheme: slot geometry (device)");
-    std::vector<double> dtfac(c->n);
-    for (int i = 0; i < c->n; ++i) dtfac[i] = s->cfl * 2.0 * c->vol[i] / c->sumA[i];
-    if ((rc = upload(c, &c->d_dtfac, dtfac))) return rc;
+    if ((rc = upload(c, &c->d_cfl, std::vector<double>{s->cfl}))) return rc;
     // RK storage plan: R_s is kept if a later non-final stage state uses it off the diagonal
     c->nstore = 0;
     const size_t cells = (size_t)c->nv * c->n;
@@ -929,7 +942,6 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     a.g = g;
     a.boco = c->d_boco;
     a.vol = c->d_vol;
-    a.dtfac = c->d_dtfac;
     a.ph = c->ph;
     a.us = (const R*)c->d_us[c->cur];
     a.us_next = (R*)c->d_us[c->cur ^ 1];
@@ -947,6 +959,12 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     a.qf_out = (R*)c->work_qf[prec];  // null for first-order schemes
     a.res_out = c->d_res;
     a.d_dt = c->d_dt;
+    if (c->steady_on) {
+        a.dtloc = c->steady_local ? c->d_dtloc : nullptr;
+        a.rnorm = c->d_rnorm;
+        a.stop = &c->d_steady->stop;
+        for (int k = 0; k < 3; ++k) a.rref[k] = c->steady_ref[k];
+    }
     a.store_slot = -1;
     for (int j = 0; j < MAX_STAGES; ++j) a.read_slot[j] = -1;
     return a;
@@ -1044,13 +1062,20 @@ int run_step(ty_ctx* c)
 int launch_dt(ty_ctx* c, double dtmax)
 {
     DtArgs d;
+    memset(&d, 0, sizeof(d));
     d.n = c->n;
     d.u0 = c->d_u0;
-    d.dtfac = c->d_dtfac;
+    d.vol = c->d_vol;
+    d.sumA = c->d_sumA;
+    d.cfl = c->d_cfl;
     d.gamma = c->gas.gamma;
-    d.cfl = c->sch.cfl;
     d.part = c->d_part;
     d.bad = c->d_bad;
+    if (c->steady_on) {
+        d.cfl = &c->d_steady->cfl;
+        d.dtloc = c->steady_local ? c->d_dtloc : nullptr;
+        d.stop = &c->d_steady->stop;
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) HIPCHK(c, ev_begin(c, &e0, &e1));
     hipError_t e = c->dim == 2 ? launch_timestep<2>(d, dtmax, c->d_dt, c->d_err, c->stream)
@@ -1162,6 +1187,158 @@ int ty_sync(ty_ctx* c)
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->d_err) return check_err(c);
+    return TY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// steady mode: reference values, local time steps, the device-resident steady cycle
+
+int ty_refcons(ty_ctx* c, double* out)
+{
+    if (!c || !out) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int n = c->n;
+    std::vector<double> h((size_t)c->nv * n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_u0, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // calc_refcons.f90:46-105 in the caller's cell order, define_refcons (MENU_SOLVER.f90:262-276)
+    double v = 0.0, s0 = 0.0, s1 = 0.0;
+    for (int o = 0; o < n; ++o) {
+        const int i = c->iperm[o];
+        v += c->vol[i];
+        s0 += h[i] * c->vol[i];
+        s1 += h[(size_t)n + i] * c->vol[i];
+    }
+    out[0] = s0 / v;
+    out[1] = s1 / v;
+    out[2] = std::sqrt(out[0] * out[1]);
+    return TY_OK;
+}
+
+namespace {
+
+int ensure_steady(ty_ctx* c, int maxit)
+{
+    int rc;
+    if (!c->d_dtloc) {
+        if ((rc = dalloc(c, &c->d_dtloc, c->n)) || (rc = dalloc(c, &c->d_rnorm, c->n)) ||
+            (rc = dalloc(c, &c->d_spart, (c->n + BLOCK - 1) / BLOCK)) || (rc = dalloc(c, &c->d_steady, 1)))
+            return rc;
+    }
+    if (c->hist_len < maxit) {
+        if ((rc = dalloc(c, &c->d_hist, maxit))) return rc;
+        c->hist_len = maxit;
+    }
+    return TY_OK;
+}
+
+}  // namespace
+
+int ty_local_timestep(ty_ctx* c, double* dt_cells, double* dt_min)
+{
+    if (!c || !dt_cells) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = ensure_steady(c, 1))) return rc;
+    // the time-step kernels with per-cell output (loc_stab_cond) at the scheme's CFL number
+    DtArgs d;
+    memset(&d, 0, sizeof(d));
+    d.n = c->n;
+    d.u0 = c->d_u0;
+    d.vol = c->d_vol;
+    d.sumA = c->d_sumA;
+    d.cfl = c->d_cfl;
+    d.gamma = c->gas.gamma;
+    d.part = c->d_part;
+    d.bad = c->d_bad;
+    d.dtloc = c->d_dtloc;
+    const hipError_t e = c->dim == 2 ? launch_timestep<2>(d, HUGE_VAL, c->d_dt, c->d_err, c->stream)
+                                     : launch_timestep<3>(d, HUGE_VAL, c->d_dt, c->d_err, c->stream);
+    HIPCHK(c, e);
+    std::vector<double> h(c->n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_dtloc, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&c->h_dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = check_err(c))) return rc;
+    for (int i = 0; i < c->n; ++i) dt_cells[c->perm[i]] = h[i];
+    if (dt_min) *dt_min = c->h_dt;
+    return TY_OK;
+}
+
+int ty_steady_cycle(ty_ctx* c, const ty_steady* st, double* res_hist, ty_steady_stats* stats)
+{
+    if (!c || !st) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    if (st->maxit < 1 || st->check_every < 1 || !(st->cfl > 0.0) || !(st->cfl_max > 0.0) || !(st->residu_ref > 0.0) ||
+        !(st->refsca[0] > 0.0) || !(st->refsca[1] > 0.0) || !(st->refvec > 0.0))
+        return fail(c, TY_ERR_ARG, "steady cycle: bad parameters");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = ensure_steady(c, st->maxit))) return rc;
+    // integration_cyclezone.f90:50: the cycle starts with cur_res = residu_ref, so the first
+    // iteration's CFL ramp term log10(cur_res / residu_ref) is zero
+    SteadyDev h0;
+    h0.cur_res = h0.residu_ref = st->residu_ref;
+    h0.cfl = std::min(st->cfl * std::max(1.0, 1.0 - st->wlog), st->cfl_max);
+    h0.iter = 0;
+    h0.stop = 0;
+    HIPCHK(c, hipMemcpyAsync(c->d_steady, &h0, sizeof(h0), hipMemcpyHostToDevice, c->stream));
+    SteadyArgs sa;
+    sa.stabnb = st->cfl;
+    sa.cfl_max = st->cfl_max;
+    sa.maxres = st->maxres;
+    sa.wlog = st->wlog;
+    sa.maxit = st->maxit;
+    sa.nblk = (c->n + BLOCK - 1) / BLOCK;
+    sa.st = c->d_steady;
+    sa.part = c->d_spart;
+    sa.hist = c->d_hist;
+    c->steady_on = true;
+    c->steady_local = st->local_dt != 0;
+    c->steady_ref[0] = st->refsca[0];
+    c->steady_ref[1] = st->refsca[1];
+    c->steady_ref[2] = st->refvec;
+    // iterations are queued without host synchronisation; once the device sets the stop flag
+    // (convergence or maxit), the later launches do nothing; the host looks every check_every
+    for (int it = 0; it < st->maxit && rc == TY_OK; ++it) {
+        if ((rc = launch_dt(c, HUGE_VAL)) || (rc = run_step(c))) break;
+        const hipError_t e = launch_steady_update(c->d_rnorm, c->n, sa, c->d_spart, c->stream);
+        if (e != hipSuccess) {
+            rc = fail(c, TY_ERR_HIP, std::string("steady update launch: ") + hipGetErrorString(e));
+            break;
+        }
+        if ((it + 1) % st->check_every == 0 && it + 1 < st->maxit) {
+            int stop = 0;
+            const hipError_t e2 = hipMemcpyAsync(&stop, &c->d_steady->stop, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+            if (e2 != hipSuccess) {
+                rc = fail(c, TY_ERR_HIP, std::string("steady stop flag: ") + hipGetErrorString(e2));
+                break;
+            }
+            if ((rc = check_err(c))) break;  // synchronises; a bad state stops the cycle
+            if (stop) break;
+        }
+    }
+    c->steady_on = false;
+    c->steady_local = false;
+    if (rc) return rc;
+    // stages after the stop did nothing: the stage records follow the FP64 state again
+    c->cur = 0;
+    hipError_t e = c->mixed ? launch_to_stage<float>(c->d_u0, (float*)c->d_us[0], c->nv, c->n, c->gas.gamma, c->stream)
+                            : launch_to_stage<double>(c->d_u0, (double*)c->d_us[0], c->nv, c->n, c->gas.gamma, c->stream);
+    HIPCHK(c, e);
+    SteadyDev h1;
+    HIPCHK(c, hipMemcpyAsync(&h1, c->d_steady, sizeof(h1), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = check_err(c))) return rc;
+    if (res_hist && h1.iter > 0)
+        HIPCHK(c, hipMemcpy(res_hist, c->d_hist, (size_t)h1.iter * sizeof(double), hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->iterations = h1.iter;
+        stats->converged = h1.cur_res / h1.residu_ref <= st->maxres ? 1 : 0;
+        stats->cur_res = h1.cur_res;
+        stats->residu_ref = h1.residu_ref;
+        stats->cfl = h1.cfl;
+    }
     return TY_OK;
 }
 

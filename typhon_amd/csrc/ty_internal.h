@@ -100,7 +100,6 @@ struct KArgs {
     Geo<R> g;
     const BocoDev* boco;
     const double* vol;    // [n]
-    const double* dtfac;  // [n] 2 V / sum(A)
     Physics ph;
     // state
     const R* us;          // [n][nv] stage state as primitive records (rho, p, u..), one per cell,
@@ -123,6 +122,10 @@ struct KArgs {
     R* dinv_out;          // [nv*nv][n] inverse diagonal blocks of the implicit operator (residual mode)
     double* diag_out;     // [nv*nv][n] the diagonal blocks themselves (optional)
     const double* d_dt;   // device time step
+    const double* dtloc;  // [n] per-cell time steps (loc_stab_cond) in place of *d_dt, or null
+    double* rnorm;        // [n] last stage: update_field's steady residual term of each cell, or null
+    double rref[3];       // refsca(rho), refsca(rhoE), refvec: the norm's reference values
+    const int* stop;      // steady cycle: nothing to do once *stop is set (null: never)
     // assembled implicit matrix (residual mode with Jacobians; null otherwise): per slot the block
     // coupling this cell's row to the slot's unknown [slot][NV*NV][n], that unknown's index (a cell,
     // or n + ghost: boundary face b -> b, periodic face j -> nb + 2 j (+1 on its far side)), and the
@@ -163,11 +166,31 @@ __device__ __forceinline__ void store_prim(const double* u, int nv, double g1, R
 struct DtArgs {
     int n, nblk_real;
     const double* u0;
-    const double* dtfac;
-    double gamma, cfl;
+    const double* vol;    // [n]
+    const double* sumA;   // [n] sum of the cell's face areas
+    const double* cfl;    // device CFL number (fixed, or the steady ramp's)
+    double gamma;
     double* part;  // [nblocks]
     int* bad;      // [nblocks]
+    double* dtloc; // [n] per-cell time steps are written here when not null (loc_stab_cond)
+    const int* stop;  // steady cycle: the kernels do nothing once *stop is set (null: never)
 };
+
+// Device state of a steady cycle (integration_cyclezone.f90:80-167 for time_steady): the residual of
+// the last iteration, the zone's reference residual, the CFL of the next iteration, counters.
+struct SteadyDev {
+    double cur_res, residu_ref, cfl;
+    int iter, stop;
+};
+struct SteadyArgs {
+    double stabnb, cfl_max, maxres, wlog;
+    int maxit, nblk;
+    SteadyDev* st;
+    const double* part;  // [nblk] block sums of the per-cell residual terms
+    double* hist;        // [maxit] cur_res of every iteration
+};
+// per-block sums of the per-cell steady residual terms, then the cycle bookkeeping (one block)
+hipError_t launch_steady_update(const double* rnorm, int n, const SteadyArgs& a, double* part, hipStream_t s);
 
 // launchers (ty_kernels.hip)
 template <int DIM, typename R>

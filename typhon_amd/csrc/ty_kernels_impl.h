@@ -669,7 +669,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 {
     constexpr int NV = DIM + 2;
     const int c = xcd_block() * BLOCK + threadIdx.x;
-    if (c >= a.n) return;
+    if (c >= a.n || (a.stop && *a.stop)) return;
     const int n = a.n;
     R qc[NV];
     load_prim<DIM>(a.us, c, qc);
@@ -1103,7 +1103,7 @@ __global__ __launch_bounds__(BLOCK, (flux_min_waves<R, FS, VISC, JAC>())) void k
 {
     constexpr int NV = DIM + 2;
     const int c = xcd_block() * BLOCK + threadIdx.x;
-    if (c >= a.n) return;
+    if (c >= a.n || (a.stop && *a.stop)) return;
     const int n = a.n;
     const R gam = R(a.ph.gamma);
     R qc[NV], irc = R(0);
@@ -1273,8 +1273,8 @@ __global__ __launch_bounds__(BLOCK, (flux_min_waves<R, FS, VISC, JAC>())) void k
             for (int v = 0; v < NV; ++v) res[v] = res[v] - A * flux[v];
         }
     }
-    // flux_to_res.f90:126-142: R = dt/V * sum
-    const double dt = *a.d_dt;
+    // flux_to_res.f90:126-142: R = dt/V * sum (the cell's own time step with loc_stab_cond)
+    const double dt = a.dtloc ? a.dtloc[c] : *a.d_dt;
     const double coef = dt / a.vol[c];
     double Rv[NV];
 #pragma unroll
@@ -1331,6 +1331,14 @@ __global__ __launch_bounds__(BLOCK, (flux_min_waves<R, FS, VISC, JAC>())) void k
             un[v] = a.u0[(size_t)v * n + c] + acc[v];
             a.u0[(size_t)v * n + c] = un[v];
         }
+        if (a.rnorm) {
+            // update_field.f90:42-60: |residu| / refsca for rho and rhoE, the momentum vector's norm
+            // / refvec
+            double m2 = acc[2] * acc[2];
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) m2 += acc[2 + d] * acc[2 + d];
+            a.rnorm[c] = fabs(acc[0]) / a.rref[0] + fabs(acc[1]) / a.rref[1] + sqrt(m2) / a.rref[2];
+        }
     }
     store_prim_n<DIM>(un, a.ph.gamma - 1.0, a.us_next + (size_t)c * NV);
 }
@@ -1359,6 +1367,7 @@ __global__ __launch_bounds__(BLOCK) void k_dt_local(DtArgs a)
     __shared__ int sbad[BLOCK / 64];
     const int lb = xcd_block();
     const int c = lb * BLOCK + threadIdx.x;
+    if (a.stop && *a.stop) return;  // uniform over the grid
     double dt = HUGE_VAL;
     int bad = 0;
     if (c < a.n) {
@@ -1371,9 +1380,10 @@ __global__ __launch_bounds__(BLOCK) void k_dt_local(DtArgs a)
         for (int d = 1; d < DIM; ++d) rv2 += a.u0[(size_t)(2 + d) * n + c] * a.u0[(size_t)(2 + d) * n + c];
         const double a2 = (a.u0[(size_t)n + c] - .5 * rv2 * irho) * gg1 * irho;
         if (!(a2 > 0.0)) bad = 1;
-        dt = a.dtfac[c];  // cfl * 2 V / sum(A) premultiplied by cfl on the host
+        dt = *a.cfl * 2.0 * a.vol[c] / a.sumA[c];
         dt = dt / (sqrt(rv2) * irho + sqrt(a2));
         if (!(dt == dt)) bad = 1;
+        if (a.dtloc) a.dtloc[c] = dt;
         (void)NV;
     }
     dt = wave_min(dt);

@@ -155,6 +155,37 @@ class Solver:
         self._check(self.lib.ty_advance(self.h, nsteps, dtmax, None), "ty_advance")
         return None
 
+    # -- steady mode (time_steady) ----------------------------------------
+    def refcons(self):
+        """calc_refcons + define_refcons at the current state: (rho_ref, rhoE_ref, momentum ref)"""
+        out = np.empty(3)
+        self._check(self.lib.ty_refcons(self.h, _p64(out)), "ty_refcons")
+        return out
+
+    def local_timestep(self):
+        """per-cell time steps of loc_stab_cond [ncell_int] and their minimum"""
+        dt = np.empty(self.ni)
+        dmin = C.c_double()
+        self._check(self.lib.ty_local_timestep(self.h, _p64(dt), C.byref(dmin)), "ty_local_timestep")
+        return dt, dmin.value
+
+    def steady_cycle(self, maxit, cfl=None, cfl_max=None, maxres=1e-6, local_dt=True, residu_ref=None,
+                     wlog=0.0, ref=None, check_every=16):
+        """one steady cycle (integration_cyclezone for time_steady) on the device: returns (history of
+        cur_res per iteration, stats dict).  residu_ref defaults to the first cycle's value (tiny), ref
+        to refcons() at the current state."""
+        ref = self.refcons() if ref is None else np.asarray(ref, dtype=np.float64)
+        cfl = self.scheme["cfl"] if cfl is None else cfl
+        st = _lib.TySteady(1 if local_dt else 0, maxit, check_every, cfl, cfl if cfl_max is None else cfl_max, maxres,
+                           np.finfo(float).tiny if residu_ref is None else residu_ref, wlog,
+                           (C.c_double * 2)(ref[0], ref[1]), ref[2])
+        hist = np.zeros(maxit)
+        out = _lib.TySteadyStats()
+        self._check(self.lib.ty_steady_cycle(self.h, C.byref(st), _p64(hist), C.byref(out)), "ty_steady_cycle")
+        stats = dict(iterations=out.iterations, converged=bool(out.converged), cur_res=out.cur_res,
+                     residu_ref=out.residu_ref, cfl=out.cfl)
+        return hist[:out.iterations], stats
+
     def sync(self):
         self._check(self.lib.ty_sync(self.h), "ty_sync")
 
