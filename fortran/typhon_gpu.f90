@@ -268,6 +268,24 @@ interface
     type(ty_steady_stats) :: stats
   endfunction
 
+  ! volume probes: quantity is a C string (trim(q)//c_null_char); type 21 VOL_AVERAGE, 22 VOL_MIN, 23 VOL_MAX
+  integer(c_int) function ty_probe_define(ctx, quantity, type, id) bind(C, name="ty_probe_define")
+    import :: c_int, c_ptr, c_char, c_int32_t
+    type(c_ptr), value          :: ctx
+    character(kind=c_char)      :: quantity(*)
+    integer(c_int32_t), value   :: type
+    integer(c_int32_t)          :: id
+  endfunction
+
+  integer(c_int) function ty_probe_eval(ctx, t, nprobe, ids, results) bind(C, name="ty_probe_eval")
+    import :: c_int, c_ptr, c_double, c_int32_t
+    type(c_ptr), value          :: ctx
+    real(c_double), value       :: t
+    integer(c_int32_t), value   :: nprobe
+    integer(c_int32_t)          :: ids(*)
+    real(c_double)              :: results(*)
+  endfunction
+
 endinterface
 
 endmodule TYPHON_GPU

@@ -283,6 +283,27 @@ typedef struct {
  * launches after the stop do nothing, so the iteration count is the reference's. */
 int ty_steady_cycle(ty_ctx* ctx, const ty_steady* steady, double* res_hist, ty_steady_stats* stats);
 
+/* Volume probes (BLOCK:PROBE, def_probe.f90:34-80; the reference's codes, DEFPROBE.f90:26-28). */
+#define TY_PROBE_VOL_AVERAGE 21 /* sum of f V / sum of V over the interior cells */
+#define TY_PROBE_VOL_MIN 22
+#define TY_PROBE_VOL_MAX 23
+
+/* Compile the probe QUANTITY expression (FCT syntax, FCT_PARSER.f90:113-340: + - * / ^, brackets,
+ * real constants, _pi, _e, the functions inv sqr sqrt exp ln log sin cos tan sinh cosh tanh asin acos
+ * atan asinh acosh atanh erf erfc abs sign step ramp, and the variables RHO, PS, V (velocity norm),
+ * V_X, V_Y, V_Z, X, Y, Z, T of prb_vol_calc, PROBECALC.f90:141-233; case and spaces ignored).
+ * `sign` evaluates as `inv`, as FCT_EVAL.f90:297-298 does.  Comparison and logical operators,
+ * which FCT_EVAL.f90 does not evaluate, and unknown names are refused with TY_ERR_ARG and the
+ * reason in ty_last_error.  *id receives the probe's index (0, 1, ... in definition order).
+ * Probes belong to the context and survive ty_set_scheme and ty_set_state. */
+int ty_probe_define(ty_ctx* ctx, const char* quantity, int32_t type, int32_t* id);
+
+/* Evaluate probes ids[nprobe] on the current state at time t (prb_zone_vol.f90:34-51 after the
+ * calc_varprim of write_monitors_cycle.f90:43-47): primitive fields from the FP64 conservative state,
+ * the expression per interior cell on the device, reductions in a fixed order (the same bits on
+ * every call).  results[nprobe] on return. */
+int ty_probe_eval(ty_ctx* ctx, double t, int32_t nprobe, const int32_t* ids, double* results);
+
 /* per-kernel timing for benchmarks (no reference counterpart): when enabled, every
  * gradient, flux, time-step, tangent and adjoint launch is bracketed by HIP events on the
  * context's stream.  Returns the summed milliseconds and launch counts recorded since the

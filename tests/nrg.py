@@ -51,6 +51,12 @@ def setup(case, from_file=True):
     return m, sch, S.default_gas(), cons
 
 
+# the cases' BLOCK:PROBE QUANTITY strings (main.rpm), evaluated on the device by ty_probe_eval
+RK_PROBE = "sqr(RHO-(1+sqr(sin(2*_pi*(x-50*t))*sin(2*_pi*y))/2))"
+VISC_PROBES = ("sqr(V_X-(erf(y/2/sqrt(t+1.))))", "2*50*(1-sqr(V_X))")
+MESH_PROBES = {"TYM-QUAD": "PS", "PER2D-TRANS": "(RHO-1.25)^2"}
+
+
 def rhoerr(m, rho, t):
     x = m.ccent[:m.ncell_int]
     q = (rho - (1 + (np.sin(2 * np.pi * (x[:, 0] - 50 * t)) * np.sin(2 * np.pi * x[:, 1])) ** 2 / 2)) ** 2

@@ -1,7 +1,8 @@
 """The reference's golden probe histories (NRG/TIME/RUNGEKUTTA/*/RHOERR.tmon, NRG/NS/NUM/FLUX/VISCOUS,
 NRG/MESH/ELEM/TYM-QUAD, NRG/MESH/CON/PER2D-TRANS) replayed by the oracle
 (CPU) and by the HIP path (GPU).  The files hold 8 significant digits; the check is 2e-7 relative
-per cycle over every cycle."""
+per cycle over every cycle.  On the GPU the probes are the cases' own QUANTITY strings evaluated
+by the device probes (ty_probe_eval)."""
 import numpy as np
 import pytest
 
@@ -29,7 +30,8 @@ def test_gpu_matches_nrg_golden(case):
     m, s, g, u = nrg.setup(case)
     gpu = Solver(m, s, g)
     gpu.set_state(u)
-    res = nrg.replay(gpu.rk_step, lambda rem: gpu.timestep(rem), lambda: gpu.get_state()[0], m)
+    pid = gpu.define_probe(nrg.RK_PROBE, "VOL_AVERAGE")
+    res = nrg.replay_probes(gpu.rk_step, lambda rem: gpu.timestep(rem), lambda t: (gpu.probe(pid, t),), 100, 0.1)
     gold = nrg.golden(case)
     np.testing.assert_allclose(res[:, 2], gold[:, 2], rtol=RTOL)
 
@@ -54,8 +56,8 @@ def test_gpu_matches_nrg_visc_golden(case):
     m, s, g, u = nrg.setup_visc(case)
     gpu = Solver(m, s, g)
     gpu.set_state(u)
-    res = nrg.replay_probes(gpu.rk_step, lambda rem: gpu.timestep(rem),
-                            lambda t: nrg.visc_probes(m, gpu.get_state(), t), 10, 10.0)
+    ids = [gpu.define_probe(q) for q in nrg.VISC_PROBES]
+    res = nrg.replay_probes(gpu.rk_step, lambda rem: gpu.timestep(rem), lambda t: gpu.probe(ids, t), 10, 10.0)
     np.testing.assert_allclose(res[:, 2], nrg.golden_visc(case, "VXERR")[:, 2], rtol=RTOL)
     np.testing.assert_allclose(res[:, 3], nrg.golden_visc(case, "DELTA")[:, 2], rtol=RTOL)
 
@@ -81,6 +83,6 @@ def test_gpu_matches_nrg_mesh_golden(case):
     m, s, g, u = nrg.setup_mesh_case(case)
     gpu = Solver(m, s, g)
     gpu.set_state(u)
-    res = nrg.replay_probes(gpu.rk_step, lambda rem: gpu.timestep(rem),
-                            lambda t: nrg.mesh_case_probe(case, m, gpu.get_state()), ncyc, dur)
+    pid = gpu.define_probe(nrg.MESH_PROBES[case], "VOL_AVERAGE")
+    res = nrg.replay_probes(gpu.rk_step, lambda rem: gpu.timestep(rem), lambda t: (gpu.probe(pid, t),), ncyc, dur)
     np.testing.assert_allclose(res[:, 2], nrg.golden_mesh(case)[:, 2], rtol=RTOL)

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtyphon_gpu.so")
-SOURCES = ["ty_kernels_f64.hip", "ty_kernels_f32.hip", "ty_krylov.hip", "ty_adjoint.hip", "ty_setup.hip",
+SOURCES = ["ty_kernels_f64.hip", "ty_kernels_f32.hip", "ty_krylov.hip", "ty_adjoint.hip", "ty_setup.hip", "ty_probe.hip",
            "ty_host.cpp"]
 ARCH = os.environ.get("TY_OFFLOAD_ARCH", "gfx950")
 

@@ -26,6 +26,7 @@
 #include "ty_internal.h"
 #include "ty_krylov.h"
 #include "ty_adjoint.h"
+#include "ty_probe.h"
 
 using namespace ty;
 
@@ -93,6 +94,11 @@ struct ty_ctx {
     double *d_dtloc = nullptr, *d_rnorm = nullptr, *d_spart = nullptr, *d_hist = nullptr;
     SteadyDev* d_steady = nullptr;
     int hist_len = 0;
+    // volume probes: compiled programs (kept for the context's life) and the device partials +
+    // results of ty_probe_eval
+    std::vector<ProbeProg> probes;
+    double* d_probe = nullptr;
+    int probe_cap = 0;
     bool steady_on = false, steady_local = false;
     double steady_ref[3] = {1.0, 1.0, 1.0};
     double* d_u0 = nullptr;
@@ -173,6 +179,8 @@ void free_state(ty_ctx* c)
     c->d_dtloc = c->d_rnorm = c->d_spart = c->d_hist = nullptr;
     c->d_steady = nullptr;
     c->hist_len = 0;
+    c->d_probe = nullptr;
+    c->probe_cap = 0;
     c->d_cc = c->d_fc = c->d_fn = c->d_fs = c->d_gc = c->d_ps = nullptr;
     c->d_bcell = nullptr;
     c->d_fslot = nullptr;
@@ -744,6 +752,8 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     c->d_dtloc = c->d_rnorm = c->d_spart = c->d_hist = nullptr;
     c->d_steady = nullptr;
     c->hist_len = 0;
+    c->d_probe = nullptr;
+    c->probe_cap = 0;
     c->d_us[0] = c->d_us[1] = nullptr;
     c->d_grad = c->d_qf = nullptr;
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
@@ -1339,6 +1349,51 @@ int ty_steady_cycle(ty_ctx* c, const ty_steady* st, double* res_hist, ty_steady_
         stats->residu_ref = h1.residu_ref;
         stats->cfl = h1.cfl;
     }
+    return TY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// volume probes (def_probe.f90:34-80, prb_zone_vol.f90:34-51)
+
+int ty_probe_define(ty_ctx* c, const char* quantity, int32_t type, int32_t* id)
+{
+    if (!c || !quantity || !id) return TY_ERR_ARG;
+    if (type != PROBE_AVG && type != PROBE_MIN && type != PROBE_MAX)
+        return fail(c, TY_ERR_ARG, "probe type must be VOL_AVERAGE (21), VOL_MIN (22) or VOL_MAX (23)");
+    ProbeProg p;
+    std::string err;
+    if (probe_compile(quantity, type, &p, &err)) return fail(c, TY_ERR_ARG, "probe \"" + std::string(quantity) + "\": " + err);
+    c->probes.push_back(p);
+    *id = (int32_t)c->probes.size() - 1;
+    return TY_OK;
+}
+
+int ty_probe_eval(ty_ctx* c, double t, int32_t nprobe, const int32_t* ids, double* results)
+{
+    if (!c || nprobe < 0 || (nprobe > 0 && (!ids || !results))) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    for (int i = 0; i < nprobe; ++i)
+        if (ids[i] < 0 || ids[i] >= (int)c->probes.size()) return fail(c, TY_ERR_ARG, "unknown probe id");
+    if (nprobe == 0) return TY_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nb = probe_blocks(c->n);
+    if (c->probe_cap < nprobe) {
+        int rc;
+        if ((rc = dalloc(c, &c->d_probe, (size_t)2 * nb + nprobe))) return rc;
+        c->probe_cap = nprobe;
+    }
+    ProbeData d;
+    d.n = c->n;
+    d.dim = c->dim;
+    d.u0 = c->d_u0;
+    d.cc = c->d_cc;
+    d.vol = c->d_vol;
+    d.g1 = c->gas.gamma - 1.0;
+    d.t = t;
+    double* out = c->d_probe + (size_t)2 * nb;
+    for (int i = 0; i < nprobe; ++i) HIPCHK(c, launch_probe(c->probes[ids[i]], d, c->d_probe, out + i, c->stream));
+    HIPCHK(c, hipMemcpyAsync(results, out, (size_t)nprobe * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return TY_OK;
 }
 

@@ -186,6 +186,26 @@ class Solver:
                      residu_ref=out.residu_ref, cfl=out.cfl)
         return hist[:out.iterations], stats
 
+    # -- volume probes (BLOCK:PROBE) ------------------------------------------
+    PROBE_TYPES = {"VOL_AVERAGE": 21, "VOL_MIN": 22, "VOL_MAX": 23}
+
+    def define_probe(self, quantity, type="VOL_AVERAGE"):
+        """compile a probe QUANTITY (FCT expression of RHO, PS, V, V_X, V_Y, V_Z, X, Y, Z, T) of the
+        given TYPE (def_probe.f90:34-80); returns its id"""
+        code = self.PROBE_TYPES[type.upper()] if isinstance(type, str) else int(type)
+        pid = C.c_int32()
+        self._check(self.lib.ty_probe_define(self.h, quantity.encode(), code, C.byref(pid)), "ty_probe_define")
+        return pid.value
+
+    def probe(self, ids, t=0.0):
+        """the probes ids (an int or a sequence) on the current state at time t (prb_zone_vol)"""
+        scalar = np.isscalar(ids)
+        idv = np.ascontiguousarray(np.atleast_1d(ids), dtype=np.int32)
+        out = np.empty(idv.size)
+        self._check(self.lib.ty_probe_eval(self.h, float(t), idv.size, _p32(idv), _p64(out)),
+                    "ty_probe_eval")
+        return float(out[0]) if scalar else out
+
     def sync(self):
         self._check(self.lib.ty_sync(self.h), "ty_sync")
 
