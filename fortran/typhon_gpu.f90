@@ -286,6 +286,16 @@ interface
     real(c_double)              :: results(*)
   endfunction
 
+  ! ty_probe_eval with its partials in a fixed per-GPU workspace (no device allocation)
+  integer(c_int) function ty_probe_eval_noalloc(ctx, t, nprobe, ids, results) bind(C, name="ty_probe_eval_noalloc")
+    import :: c_int, c_ptr, c_double, c_int32_t
+    type(c_ptr), value          :: ctx
+    real(c_double), value       :: t
+    integer(c_int32_t), value   :: nprobe
+    integer(c_int32_t)          :: ids(*)
+    real(c_double)              :: results(*)
+  endfunction
+
 endinterface
 
 endmodule TYPHON_GPU

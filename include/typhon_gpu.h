@@ -304,6 +304,17 @@ int ty_probe_define(ty_ctx* ctx, const char* quantity, int32_t type, int32_t* id
  * every call).  results[nprobe] on return. */
 int ty_probe_eval(ty_ctx* ctx, double t, int32_t nprobe, const int32_t* ids, double* results);
 
+/* ty_probe_eval without device allocation (no counterpart in the reference): for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc).  The same kernels and
+ * the same results, bit for bit, with the block partials and results in a fixed 2 MiB workspace that
+ * each GPU holds for the life of the process (looked up by ty_create).  Limits: at most
+ * TY_PROBE_NOALLOC_MAX_CELLS interior cells and TY_PROBE_NOALLOC_MAX_PROBES probes per call, else
+ * TY_ERR_ARG.  Every context on a GPU shares that GPU's workspace: calls that overlap in time on one
+ * GPU (contexts driven from several threads) race and must be serialised by the caller. */
+#define TY_PROBE_NOALLOC_MAX_CELLS (1 << 25)
+#define TY_PROBE_NOALLOC_MAX_PROBES 64
+int ty_probe_eval_noalloc(ty_ctx* ctx, double t, int32_t nprobe, const int32_t* ids, double* results);
+
 /* per-kernel timing for benchmarks (no reference counterpart): when enabled, every
  * gradient, flux, time-step, tangent and adjoint launch is bracketed by HIP events on the
  * context's stream.  Returns the summed milliseconds and launch counts recorded since the

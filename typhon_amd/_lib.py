@@ -17,7 +17,7 @@ EXPORTS = ["ty_create", "ty_destroy", "ty_last_error", "ty_mesh_upload", "ty_set
            "ty_state_get", "ty_timestep", "ty_residual", "ty_rk_step", "ty_advance", "ty_sync", "ty_timing",
            "ty_mesh_info", "ty_jacobian_diag", "ty_jvp", "ty_linear_solve", "ty_implicit_step",
            "ty_implicit_bdlu_step", "ty_residual_tangent", "ty_residual_adjoint", "ty_unsteady_adjoint",
-           "ty_refcons", "ty_local_timestep", "ty_steady_cycle", "ty_probe_define", "ty_probe_eval"]
+           "ty_refcons", "ty_local_timestep", "ty_steady_cycle", "ty_probe_define", "ty_probe_eval", "ty_probe_eval_noalloc"]
 
 
 class TyBocoDesc(C.Structure):
@@ -118,6 +118,7 @@ def load():
     lib.ty_steady_cycle.argtypes = [vp, C.POINTER(TySteady), F64P, C.POINTER(TySteadyStats)]
     lib.ty_probe_define.argtypes = [vp, C.c_char_p, C.c_int32, I32P]
     lib.ty_probe_eval.argtypes = [vp, C.c_double, C.c_int32, I32P, F64P]
+    lib.ty_probe_eval_noalloc.argtypes = [vp, C.c_double, C.c_int32, I32P, F64P]
     for name in EXPORTS:
         if name != "ty_last_error":
             getattr(lib, name).restype = C.c_int

@@ -99,6 +99,7 @@ struct ty_ctx {
     std::vector<ProbeProg> probes;
     double* d_probe = nullptr;
     int probe_cap = 0;
+    double* d_scratch = nullptr;  // this device's fixed probe workspace (ty_probe_eval_noalloc)
     bool steady_on = false, steady_local = false;
     double steady_ref[3] = {1.0, 1.0, 1.0};
     double* d_u0 = nullptr;
@@ -348,6 +349,7 @@ int ty_create(ty_ctx** ctx, int device)
     c->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = probe_scratch_setup(device, &c->d_scratch);
     if (e != hipSuccess) {
         delete c;
         return TY_ERR_HIP;
@@ -1392,6 +1394,37 @@ int ty_probe_eval(ty_ctx* c, double t, int32_t nprobe, const int32_t* ids, doubl
     d.t = t;
     double* out = c->d_probe + (size_t)2 * nb;
     for (int i = 0; i < nprobe; ++i) HIPCHK(c, launch_probe(c->probes[ids[i]], d, c->d_probe, out + i, c->stream));
+    HIPCHK(c, hipMemcpyAsync(results, out, (size_t)nprobe * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TY_OK;
+}
+
+int ty_probe_eval_noalloc(ty_ctx* c, double t, int32_t nprobe, const int32_t* ids, double* results)
+{
+    if (!c || nprobe < 0 || (nprobe > 0 && (!ids || !results))) return TY_ERR_ARG;
+    if (!c->state_ok) return fail(c, TY_ERR_ARG, "no state");
+    if (c->n > PROBE_SCRATCH_MAX_CELLS)
+        return fail(c, TY_ERR_ARG, "ty_probe_eval_noalloc: " + std::to_string(c->n) + " cells exceed its workspace limit of " +
+                                       std::to_string(PROBE_SCRATCH_MAX_CELLS));
+    if (nprobe > PROBE_SCRATCH_MAX_PROBES)
+        return fail(c, TY_ERR_ARG, "ty_probe_eval_noalloc: " + std::to_string(nprobe) + " probes exceed its limit of " +
+                                       std::to_string(PROBE_SCRATCH_MAX_PROBES) + " per call");
+    for (int i = 0; i < nprobe; ++i)
+        if (ids[i] < 0 || ids[i] >= (int)c->probes.size()) return fail(c, TY_ERR_ARG, "unknown probe id");
+    if (nprobe == 0) return TY_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    ProbeData d;
+    d.n = c->n;
+    d.dim = c->dim;
+    d.u0 = c->d_u0;
+    d.cc = c->d_cc;
+    d.vol = c->d_vol;
+    d.g1 = c->gas.gamma - 1.0;
+    d.t = t;
+    // the same kernels as ty_probe_eval with the partials and results in the fixed workspace
+    double* part = c->d_scratch;
+    double* out = c->d_scratch + (size_t)2 * (PROBE_SCRATCH_MAX_CELLS / 256);
+    for (int i = 0; i < nprobe; ++i) HIPCHK(c, launch_probe(c->probes[ids[i]], d, part, out + i, c->stream));
     HIPCHK(c, hipMemcpyAsync(results, out, (size_t)nprobe * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return TY_OK;

@@ -5,6 +5,8 @@
 
 #include <string>
 
+#include "../../include/typhon_gpu.h"
+
 namespace ty {
 
 // probe types, the reference's codes (CFDTOOLS/Mesh/DEFPROBE.f90:26-28)
@@ -42,5 +44,13 @@ int probe_compile(const char* expr, int type, ProbeProg* prog, std::string* err)
 int probe_blocks(int n);
 // *out (device) = the probe's value
 hipError_t launch_probe(const ProbeProg& p, const ProbeData& d, double* part, double* out, hipStream_t s);
+
+// limits of the allocation-free evaluation's fixed device workspace (2 MiB per GPU)
+constexpr int PROBE_SCRATCH_MAX_CELLS = TY_PROBE_NOALLOC_MAX_CELLS;
+constexpr int PROBE_SCRATCH_MAX_PROBES = TY_PROBE_NOALLOC_MAX_PROBES;
+constexpr size_t PROBE_SCRATCH_DOUBLES = 2 * (PROBE_SCRATCH_MAX_CELLS / 256) + PROBE_SCRATCH_MAX_PROBES;
+// device address of the current device's workspace (looked up once per device; the device must
+// be current): partials at [0, 2 * probe_blocks(n)), results after 2 * (PROBE_SCRATCH_MAX_CELLS / 256)
+hipError_t probe_scratch_setup(int device, double** ptr);
 
 }  // namespace ty

@@ -21,9 +21,14 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "ty_probe.h"
+
+// fixed workspace of the allocation-free probe evaluation (one copy per GPU, host code reaches it
+// only through the device address probe_scratch_setup looks up)
+static __device__ __align__(256) double scratch[ty::PROBE_SCRATCH_DOUBLES];
 
 namespace ty {
 
@@ -475,6 +480,41 @@ hipError_t launch_probe(const ProbeProg& p, const ProbeData& d, double* part, do
     hipLaunchKernelGGL(k_probe, dim3(nb), dim3(PB), 0, s, p, d, part);
     hipLaunchKernelGGL(k_probe_final, dim3(1), dim3(1024), 0, s, p.type, (const double*)part, nb, out);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// fixed workspace of the allocation-free evaluation: the block partials and results of
+// ty_probe_eval_noalloc.  Every GPU holds one copy for the life of the process; each call
+// overwrites every partial it reads, so nothing in it has to start at zero.
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[MAX_DEVICES] = {};
+
+}  // namespace
+
+static_assert(PROBE_SCRATCH_DOUBLES >= 2 * ((PROBE_SCRATCH_MAX_CELLS + PB - 1) / PB) + PROBE_SCRATCH_MAX_PROBES,
+              "probe workspace too small for its limits");
+
+hipError_t probe_scratch_setup(int device, double** ptr)
+{
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        // the device address of this GPU's copy of the array, looked up with the device current
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<double*>(p);
+    }
+    *ptr = g_scratch[device];
+    return hipSuccess;
 }
 
 }  // namespace ty

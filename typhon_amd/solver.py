@@ -197,13 +197,15 @@ class Solver:
         self._check(self.lib.ty_probe_define(self.h, quantity.encode(), code, C.byref(pid)), "ty_probe_define")
         return pid.value
 
-    def probe(self, ids, t=0.0):
-        """the probes ids (an int or a sequence) on the current state at time t (prb_zone_vol)"""
+    def probe(self, ids, t=0.0, noalloc=False):
+        """the probes ids (an int or a sequence) on the current state at time t (prb_zone_vol);
+        noalloc: through ty_probe_eval_noalloc (no device allocation, fixed per-GPU workspace)"""
         scalar = np.isscalar(ids)
         idv = np.ascontiguousarray(np.atleast_1d(ids), dtype=np.int32)
         out = np.empty(idv.size)
-        self._check(self.lib.ty_probe_eval(self.h, float(t), idv.size, _p32(idv), _p64(out)),
-                    "ty_probe_eval")
+        fn, name = ((self.lib.ty_probe_eval_noalloc, "ty_probe_eval_noalloc") if noalloc
+                    else (self.lib.ty_probe_eval, "ty_probe_eval"))
+        self._check(fn(self.h, float(t), idv.size, _p32(idv), _p64(out)), name)
         return float(out[0]) if scalar else out
 
     def sync(self):
