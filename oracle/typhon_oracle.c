@@ -730,6 +730,39 @@ static void roe_state(double g, const double* qL, const double* qR, double* rho,
     *p = (h - .5 * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2])) * (*rho) / gig1;
 }
 
+#define OR_PI 3.141592653589793238462643 /* PIcst, CFDTOOLS/Math/MATH.f90:15 */
+
+/* Wave speeds of the HLL family (calc_flux_hlle.f90:64-123, calc_flux_hllc.f90:64-125): Einfeldt's
+ * bounds with beta = 1 (HLLE, HLLC) or sqrt((g-1)/2g) (the B variants), HLLE's bounded by zero; the
+ * kinetic variants (HLLEK/HLLEKB/HLLCK/HLLCKB) replace the side bounds by the mean speeds of the
+ * side's half Maxwellian, c h(M) with c = a / sqrt(g/2), M = vn/c -+ sqrt(g/2) beta. */
+static void hll_speeds(int sch, double g, const double* qL, const double* qR, const double* n, double vnl, double vnr,
+                       double* sl, double* sr)
+{
+    int kin = (sch == OR_SCH_HLLEK || sch == OR_SCH_HLLEKB || sch == OR_SCH_HLLCK || sch == OR_SCH_HLLCKB);
+    double beta = (sch == OR_SCH_HLLEB || sch == OR_SCH_HLLEKB || sch == OR_SCH_HLLCB || sch == OR_SCH_HLLCKB)
+                  ? sqrt((g - 1.0) / (2 * g)) : 1.0;
+    double al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
+    double rrho, rv[3], rp; roe_state(g, qL, qR, &rrho, rv, &rp);
+    double vm = rv[0] * n[0] + rv[1] * n[1] + rv[2] * n[2];
+    double am = sqrt(g * rp / rrho);
+    if (kin) {
+        double f1 = sqrt(0.5 * g), f2 = 0.5 / sqrt(OR_PI);
+        double cl = al / f1, cr = ar / f1;
+        double ml = vnl / cl - f1 * beta, mr = vnr / cr + f1 * beta;
+        double kinwl = 0.5 * (1.0 - erf(ml)), kinwr = 0.5 * (1.0 + erf(mr));
+        double kindl = -f2 * exp(-ml * ml), kindr = +f2 * exp(-mr * mr);
+        *sl = FMIN(cl * (ml * kinwl + kindl), vm - am);
+        *sr = FMAX(cr * (mr * kinwr + kindr), vm + am);
+    } else if (sch == OR_SCH_HLLE || sch == OR_SCH_HLLEB) {
+        *sl = FMIN(FMIN(0.0, vnl - beta * al), vm - am);
+        *sr = FMAX(FMAX(0.0, vnr + beta * ar), vm + am);
+    } else {
+        *sl = FMIN(vnl - beta * al, vm - am);
+        *sr = FMAX(vnr + beta * ar, vm + am);
+    }
+}
+
 void or_flux_face(const or_config* c, const double* qL, const double* qR, const double* n, double* F)
 {
     double g = c->gamma, ig1 = 1.0 / (g - 1.0);
@@ -747,14 +780,9 @@ void or_flux_face(const or_config* c, const double* qL, const double* qR, const 
             F[2 + d] = .5 * (((vnl + ray) * qL[0]) * qL[2 + d] + ((vnr - ray) * qR[0]) * qR[2 + d]
                              + (qL[1] + qR[1]) * n[d]);
         return; }
-    case OR_SCH_HLLE: case OR_SCH_HLLEB: { /* calc_flux_hlle.f90:109-143 */
-        double beta = (c->sch_hyp == OR_SCH_HLLE) ? 1.0 : sqrt((g - 1.0) / (2 * g));
-        double al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
-        double rrho, rv[3], rp; roe_state(g, qL, qR, &rrho, rv, &rp);
-        double vm = rv[0] * n[0] + rv[1] * n[1] + rv[2] * n[2];
-        double am = sqrt(g * rp / rrho);
-        double sl = FMIN(FMIN(0.0, vnl - beta * al), vm - am);
-        double sr = FMAX(FMAX(0.0, vnr + beta * ar), vm + am);
+    case OR_SCH_HLLE: case OR_SCH_HLLEB: case OR_SCH_HLLEK: case OR_SCH_HLLEKB: { /* calc_flux_hlle.f90:109-143 */
+        double sl, sr;
+        hll_speeds(c->sch_hyp, g, qL, qR, n, vnl, vnr, &sl, &sr);
         double iks = 1.0 / (sr - sl);
         double kl = sr * iks, kr = -sl * iks, ku = sl * sr * iks;
         F[0] = (kl * vnl - ku) * qL[0] + (kr * vnr + ku) * qR[0];
@@ -793,14 +821,9 @@ void or_flux_face(const or_config* c, const double* qL, const double* qR, const 
             D[2 + d] = w1 * (rv[d] - a * n[d]) + w2 * rv[d] + w3 * (rv[d] + a * n[d]) + w4 * dvt[d];
         for (int k = 0; k < 5; ++k) F[k] = .5 * (FL[k] + FR[k] - D[k]);
         return; }
-    case OR_SCH_HLLC: case OR_SCH_HLLCB: default: { /* calc_flux_hllc.f90:61-171 */
-        double beta = (c->sch_hyp == OR_SCH_HLLCB) ? sqrt((g - 1.0) / (2 * g)) : 1.0;
-        double al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
-        double rrho, rv[3], rp; roe_state(g, qL, qR, &rrho, rv, &rp);
-        double vm = rv[0] * n[0] + rv[1] * n[1] + rv[2] * n[2];
-        double am = sqrt(g * rp / rrho);
-        double sl = FMIN(vnl - beta * al, vm - am);
-        double sr = FMAX(vnr + beta * ar, vm + am);
+    case OR_SCH_HLLC: case OR_SCH_HLLCB: case OR_SCH_HLLCK: case OR_SCH_HLLCKB: default: { /* calc_flux_hllc.f90:61-171 */
+        double sl, sr;
+        hll_speeds(c->sch_hyp, g, qL, qR, n, vnl, vnr, &sl, &sr);
         if (sl >= 0.0) {
             F[0] = vnl * qL[0]; F[1] = vnl * (rel + qL[1]);
             for (int d = 0; d < 3; ++d) F[2 + d] = (vnl * qL[0]) * qL[2 + d] + qL[1] * n[d];
@@ -1237,8 +1260,8 @@ int or_meshgeom(int dim, int nvtx, const double* vtx, int nface, const int32_t* 
  * A15: flux Jacobians and the diagonal blocks of the implicit operator.
  * calc_jacflux (SOURCE/EQNS/calc_jacflux.f90:49-77): Euler flux Jacobian A(n) in
  * (rho, rhoE, rho u, rho v, rho w) at a primitive state; calc_jac_hll
- * (calc_jac_hll.f90:46-57) with the HLLC wave speeds clipped at zero
- * (calc_flux_hllc.f90:185-189).  Blocks are row-major m[i*5+j] = dF_i/dU_j. */
+ * (calc_jac_hll.f90:46-57) with the flux's wave speeds clipped at zero
+ * (calc_flux_hllc.f90:185-189, calc_flux_hlle.f90:158-160).  Blocks are row-major m[i*5+j] = dF_i/dU_j. */
 static void jacflux(double g, const double* q, const double* n, double vn, double* m)
 {
     double g1 = g - 1.0, ig1 = 1.0 / g1;
@@ -1262,15 +1285,13 @@ static void jacflux(double g, const double* q, const double* n, double vn, doubl
 void or_jac_hll(const or_config* c, const double* qL, const double* qR, const double* n, double* jL, double* jR)
 {
     double g = c->gamma;
-    double beta = (c->sch_hyp == OR_SCH_HLLCB) ? sqrt((g - 1.0) / (2 * g)) : 1.0;
     double vnl = qL[2] * n[0] + qL[3] * n[1] + qL[4] * n[2];
     double vnr = qR[2] * n[0] + qR[3] * n[1] + qR[4] * n[2];
-    double al = sqrt(g * qL[1] / qL[0]), ar = sqrt(g * qR[1] / qR[0]);
-    double rrho, rv[3], rp; roe_state(g, qL, qR, &rrho, rv, &rp);
-    double vm = rv[0] * n[0] + rv[1] * n[1] + rv[2] * n[2];
-    double am = sqrt(g * rp / rrho);
-    double sl = FMIN(vnl - beta * al, vm - am);
-    double sr = FMAX(vnr + beta * ar, vm + am);
+    /* the flux's own wave speeds (HLLC's for the fluxes outside the HLL family, which the
+       reference does not pair with HLL Jacobians) */
+    int sch = (c->sch_hyp >= OR_SCH_HLLE && c->sch_hyp <= OR_SCH_HLLCKB) ? c->sch_hyp : OR_SCH_HLLC;
+    double sl, sr;
+    hll_speeds(sch, g, qL, qR, n, vnl, vnr, &sl, &sr);
     sl = FMIN(sl, 0.0);
     sr = FMAX(sr, 0.0);
     jacflux(g, qL, n, vnl, jL);

@@ -40,8 +40,12 @@ extern "C" {
 #define OR_SCH_ROE     10
 #define OR_SCH_HLLE    21
 #define OR_SCH_HLLEB   22
+#define OR_SCH_HLLEK   23
+#define OR_SCH_HLLEKB  24
 #define OR_SCH_HLLC    25
 #define OR_SCH_HLLCB   26
+#define OR_SCH_HLLCK   27
+#define OR_SCH_HLLCKB  28
 
 /* gradient methods: MENU_NUM.f90:128-131 */
 #define OR_GRAD_NONE 5

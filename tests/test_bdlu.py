@@ -149,11 +149,14 @@ def _gpu_cases():
         return K.bump_case(8, 5, 4)
     d = _cases()
     d["bump"] = bump
+    # HLL Jacobians with the kinetic fluxes' own wave speeds
+    d["channel_hllekb"] = lambda: K.channel_case(12, 6, jitter=0.1, sch_hyp=S.SCH_HLLEKB)
+    d["channel_hllck"] = lambda: K.channel_case(12, 6, jitter=0.1, sch_hyp=S.SCH_HLLCK)
     return d
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["channel", "periodic", "tgv", "bump"])
+@pytest.mark.parametrize("name", ["channel", "periodic", "tgv", "bump", "channel_hllekb", "channel_hllck"])
 def test_gpu_bdlu_step_matches_oracle(name):
     from typhon_amd.solver import Solver
     m, s, g, u = _gpu_cases()[name]()

@@ -25,6 +25,8 @@ CASES = {
     "channel_rusanov": lambda: K.channel_case(8, 4, sch_hyp=S.SCH_RUSANOV),
     "channel_hllcb": lambda: K.channel_case(8, 4, sch_hyp=S.SCH_HLLCB),
     "channel_roe": lambda: K.channel_case(8, 4, sch_hyp=S.SCH_ROE),
+    "channel_hllck": lambda: K.channel_case(8, 4, sch_hyp=S.SCH_HLLCK),
+    "channel_hllekb": lambda: K.channel_case(8, 4, sch_hyp=S.SCH_HLLEKB),
     "bump_hllc": lambda: K.bump_case(6, 4, 3),
     "vortex_periodic": lambda: K.vortex_case(3, L=3.0, method="N", limiter="X", gradmeth=S.GRAD_NONE),
 }

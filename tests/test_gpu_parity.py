@@ -63,9 +63,34 @@ def test_channel_first_order_full_size_steps():
     _step_parity(lambda: K.channel_case(256, 128), nsteps=5)
 
 
-@pytest.mark.parametrize("sch", [S.SCH_HLLC, S.SCH_HLLCB, S.SCH_HLLE, S.SCH_HLLEB, S.SCH_RUSANOV, S.SCH_ROE])
+ALL_FLUXES = [S.SCH_HLLC, S.SCH_HLLCB, S.SCH_HLLCK, S.SCH_HLLCKB, S.SCH_HLLE, S.SCH_HLLEB, S.SCH_HLLEK,
+              S.SCH_HLLEKB, S.SCH_RUSANOV, S.SCH_ROE]
+
+
+@pytest.mark.parametrize("sch", ALL_FLUXES)
 def test_fluxes_residual(sch):
     _residual_parity(lambda: K.channel_case(24, 12, sch_hyp=sch))
+
+
+@pytest.mark.parametrize("sch", [S.SCH_HLLCK, S.SCH_HLLCKB, S.SCH_HLLEK, S.SCH_HLLEKB])
+def test_kinetic_fluxes_second_order_steps(sch):
+    """the kinetic wave speeds (erf/exp of the half-Maxwellian Mach numbers) on 3D tets with MUSCL"""
+    _step_parity(lambda: K.bump_case(6, 3, 3, noise=0.05, sch_hyp=sch, method="M", limiter="A",
+                                     gradmeth=S.GRAD_LSQ, tps_meth=S.TPS_RK2), nsteps=4)
+
+
+@pytest.mark.parametrize("sch", [S.SCH_HLLCK, S.SCH_HLLEKB])
+def test_kinetic_fluxes_mixed_precision(sch):
+    """FP32 fluxes (erff/expf) with FP64 accumulation: the FP64 state to 1e-5 after 5 steps"""
+    m, s, g, u = K.channel_case(24, 12, sch_hyp=sch)
+    res = {}
+    for prec in (S.PREC_FP64, S.PREC_MIXED):
+        gpu = Solver(m, dict(s, precision=prec), g)
+        gpu.set_state(u)
+        for _ in range(5):
+            gpu.rk_step(2e-3)
+        res[prec] = gpu.get_state()
+    assert K.rel_err(res[S.PREC_MIXED][:4], res[S.PREC_FP64][:4]) <= 1e-5
 
 
 # --- second order reference MUSCL family ---------------------------------------

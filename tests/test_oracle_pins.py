@@ -294,3 +294,115 @@ def test_hll_dissipation_of_a_density_jump():
     sl, sr = -max(at, np.sqrt(0.7)), max(at, np.sqrt(1.4))
     assert abs(f[0] - sl * sr / (sr - sl) * (qR[0] - qL[0])) < 1e-14
     assert f[0] > 0.0
+
+
+# --- kinetic wave speeds (HLLEK/HLLEKB, HLLCK/HLLCKB) -------------------------------------------
+# calc_flux_hlle.f90:114-123 and calc_flux_hllc.f90:116-125 bound the waves by the mean speeds of the
+# half Maxwellians of the two sides: with c = a / sqrt(g/2) and M = vn / c -+ sqrt(g/2) beta,
+#   S_L = min(c_L h_L(M_L), u~ - a~),  h_L(M) = -1/2 int_M^inf erfc(t) dt,
+#   S_R = max(c_R h_R(M_R), u~ + a~),  h_R(M) = +1/2 int_-inf^M erfc(-t) dt,
+# whose closed forms the reference evaluates with erf and exp.  The checks take the integrals by
+# quadrature, so a dropped term, a wrong sign or a swapped erf argument of the closed form fails.
+
+def _roe_normal(qL, qR, n, g=G):
+    rl, rr = qL[0], qR[0]
+    sq = np.sqrt(rr / rl)
+    ut = (qL[2:5] + sq * qR[2:5]) / (1 + sq)
+    hl = g / (g - 1) * qL[1] / rl + .5 * qL[2:5] @ qL[2:5]
+    hr = g / (g - 1) * qR[1] / rr + .5 * qR[2:5] @ qR[2:5]
+    ht = (hl + sq * hr) / (1 + sq)
+    return ut @ n, np.sqrt((g - 1) * (ht - .5 * ut @ ut))
+
+
+def kinetic_speeds(qL, qR, n, beta, g=G):
+    from scipy import integrate, special
+    vm, am = _roe_normal(qL, qR, n, g)
+    f1 = np.sqrt(g / 2)
+    cl, cr = np.sqrt(g * qL[1] / qL[0]) / f1, np.sqrt(g * qR[1] / qR[0]) / f1
+    ml, mr = (qL[2:5] @ n) / cl - f1 * beta, (qR[2:5] @ n) / cr + f1 * beta
+    hl = -0.5 * integrate.quad(special.erfc, ml, np.inf, epsabs=1e-15, epsrel=1e-13)[0]
+    hr = 0.5 * integrate.quad(lambda t: special.erfc(-t), -np.inf, mr, epsabs=1e-15, epsrel=1e-13)[0]
+    return min(cl * hl, vm - am), max(cr * hr, vm + am)
+
+
+def hll_form(qL, qR, n, sl, sr, g=G):
+    """Toro eq. 10.21 with the given speeds (S_L < 0 < S_R)"""
+    return (sr * _phys_flux(qL, n, g) - sl * _phys_flux(qR, n, g) + sl * sr * (_cons(qR, g) - _cons(qL, g))) / (sr - sl)
+
+
+def hllc_form(qL, qR, n, sl, sr, g=G):
+    """Toro's HLLC (Riemann Solvers 3rd ed., eqs. 10.37-10.39) with Batten's contact speed S*:
+    F*_K = F_K + S_K (U*_K - U_K)"""
+    if sl >= 0:
+        return _phys_flux(qL, n, g)
+    if sr <= 0:
+        return _phys_flux(qR, n, g)
+    rl, pl, ul = qL[0], qL[1], qL[2:5] @ n
+    rr, pr, ur = qR[0], qR[1], qR[2:5] @ n
+    ss = (pr - pl + rl * ul * (sl - ul) - rr * ur * (sr - ur)) / (rl * (sl - ul) - rr * (sr - ur))
+    q, s, un = (qL, sl, ul) if ss >= 0 else (qR, sr, ur)
+    U = _cons(q, g)
+    fac = q[0] * (s - un) / (s - ss)
+    vel = q[2:5] + (ss - un) * n
+    Us = fac * np.concatenate([[1.0, U[1] / q[0] + (ss - un) * (ss + q[1] / (q[0] * (s - un)))], vel])
+    return _phys_flux(q, n, g) + s * (Us - U)
+
+
+def _random_pairs(seed, count):
+    rng = np.random.default_rng(seed)
+    for k in range(count):
+        vmax = 3.0 if k % 3 == 0 else 1.0
+        qL = np.concatenate([[rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0)], rng.uniform(-vmax, vmax, 3)])
+        qR = np.concatenate([[rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0)], rng.uniform(-vmax, vmax, 3)])
+        n = rng.normal(size=3)
+        yield qL, qR, n / np.linalg.norm(n)
+
+
+BETA_B = np.sqrt(0.4 / 2.8)
+
+
+@pytest.mark.parametrize("sch,beta", [(S.SCH_HLLEK, 1.0), (S.SCH_HLLEKB, BETA_B)])
+def test_hllek_matches_hll_form_with_kinetic_speeds(sch, beta):
+    for qL, qR, n in _random_pairs(31, 200):
+        sl, sr = kinetic_speeds(qL, qR, n, beta)
+        assert sl < 0 < sr
+        np.testing.assert_allclose(O.flux_face(qL, qR, n, sch), hll_form(qL, qR, n, sl, sr), rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("sch,beta", [(S.SCH_HLLC, 1.0), (S.SCH_HLLCB, BETA_B), (S.SCH_HLLCK, 1.0),
+                                      (S.SCH_HLLCKB, BETA_B)])
+def test_hllc_matches_toro_form(sch, beta):
+    kin = sch in (S.SCH_HLLCK, S.SCH_HLLCKB)
+    for qL, qR, n in _random_pairs(32, 200):
+        if kin:
+            sl, sr = kinetic_speeds(qL, qR, n, beta)
+        else:
+            vm, am = _roe_normal(qL, qR, n)
+            sl = min(qL[2:5] @ n - beta * np.sqrt(G * qL[1] / qL[0]), vm - am)
+            sr = max(qR[2:5] @ n + beta * np.sqrt(G * qR[1] / qR[0]), vm + am)
+        np.testing.assert_allclose(O.flux_face(qL, qR, n, sch), hllc_form(qL, qR, n, sl, sr), rtol=1e-11, atol=1e-11)
+
+
+def test_kinetic_speeds_tend_to_the_classical_bounds():
+    """far from the sonic point the half Maxwellian's mean speed is the side's own bound: h(M) -> M
+    as M -> -inf (left), M -> +inf (right); so a strongly supersonic pair has the HLLC speeds"""
+    qL = np.array([1.0, 1.0, -9.0, 0.0, 0.0])
+    qR = np.array([1.2, 0.8, 9.5, 0.0, 0.0])
+    n = np.array([1.0, 0.0, 0.0])
+    for sk, sc in ((S.SCH_HLLCK, S.SCH_HLLC), (S.SCH_HLLEK, S.SCH_HLLE)):
+        np.testing.assert_allclose(O.flux_face(qL, qR, n, sk), O.flux_face(qL, qR, n, sc), rtol=1e-12, atol=1e-13)
+    # at the sonic point of the half Maxwellian (M_L = 0: vn_L = beta a_L) the left bound is
+    # c_L h_L(0) = -c_L / (2 sqrt(pi)) = -a_L / sqrt(2 pi g), since int_0^inf erfc = 1/sqrt(pi); a
+    # denser right side at the same p and u keeps the Roe bound u~ - a~ positive, so S_L is that
+    from scipy import integrate, special
+    aL = np.sqrt(G)
+    qL = np.array([1.0, 1.0, aL, 0.0, 0.0])
+    qR = np.array([2.0, 1.0, aL, 0.0, 0.0])
+    vm, am = _roe_normal(qL, qR, n)
+    assert vm - am > 0
+    sl = -aL / np.sqrt(2 * np.pi * G)
+    f1 = np.sqrt(G / 2)
+    cr = np.sqrt(G * 0.5) / f1
+    mr = aL / cr + f1
+    sr = max(cr * 0.5 * integrate.quad(lambda t: special.erfc(-t), -np.inf, mr, epsabs=1e-15)[0], vm + am)
+    np.testing.assert_allclose(O.flux_face(qL, qR, n, S.SCH_HLLEK), hll_form(qL, qR, n, sl, sr), rtol=1e-12)

@@ -47,6 +47,16 @@ __device__ __forceinline__ Dual copysign(Dual a, Dual b)
     const bool flip = (a.v < 0.0) != (b.v < 0.0);
     return flip ? -a : a;
 }
+__device__ __forceinline__ Dual exp(Dual a)
+{
+    const double e = ::exp(a.v);
+    return Dual(e, e * a.d);
+}
+// d erf(x) / dx = 2 / sqrt(pi) exp(-x^2)
+__device__ __forceinline__ Dual erf(Dual a)
+{
+    return Dual(::erf(a.v), 1.1283791670955125739 * ::exp(-a.v * a.v) * a.d);
+}
 // a^b with both operands carrying derivatives
 __device__ __forceinline__ Dual dpow(Dual a, Dual b)
 {

@@ -722,7 +722,7 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     if (!c->mesh_ok) return fail(c, TY_ERR_ARG, "ty_set_scheme before ty_mesh_upload");
     if (s->nstage < 1 || s->nstage > MAX_STAGES || !s->rk_coef) return fail(c, TY_ERR_ARG, "bad RK tableau");
     const int sch = s->sch_hyp;
-    if (sch != 5 && sch != 10 && sch != 21 && sch != 22 && sch != 25 && sch != 26)
+    if (sch != 5 && sch != 10 && (sch < 21 || sch > 28))
         return fail(c, TY_ERR_UNSUPPORTED, "numerical flux " + std::to_string(sch) + " not provided");
     if (!strchr("NMFU", s->method) || s->method == 0) return fail(c, TY_ERR_UNSUPPORTED, "high-resolution method");
     if (!strchr("XMAVSKLW", s->limiter) || s->limiter == 0) return fail(c, TY_ERR_UNSUPPORTED, "limiter");

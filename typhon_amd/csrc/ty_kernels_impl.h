@@ -271,6 +271,33 @@ __device__ __forceinline__ void roe_avg(R g, R gig1, const R* qL, const R* qR, R
     a2 = (g - R(1)) * (h - R(.5) * dotd<DIM>(vel, vel));
 }
 
+// Wave speeds of the HLL family (calc_flux_hlle.f90:64-123, calc_flux_hllc.f90:64-125): Einfeldt's
+// bounds from the sides' normal velocities and sound speeds and the Roe average (vm, am), with
+// beta = 1 or sqrt((g-1)/2g) (the B variants), HLLE's bounded by zero; the kinetic variants (23, 24,
+// 27, 28) take the mean speeds of each side's half Maxwellian, c h(M), c = a / sqrt(g/2),
+// M = vn / c -+ sqrt(g/2) beta, in place of vn -+ beta a.
+template <typename R>
+__device__ __forceinline__ void hll_speeds(int sch, R g, R vnl, R vnr, R al, R ar, R vm, R am, R& sl, R& sr)
+{
+    const bool kin = sch == 23 || sch == 24 || sch == 27 || sch == 28;
+    const R beta = (sch == 22 || sch == 24 || sch == 26 || sch == 28) ? sqrt((g - R(1)) / (R(2) * g)) : R(1);
+    if (kin) {
+        const R f1 = sqrt(R(.5) * g), f2 = R(.5) / sqrt(R(3.141592653589793238462643));
+        const R cl = al / f1, cr = ar / f1;
+        const R ml = vnl / cl - f1 * beta, mr = vnr / cr + f1 * beta;
+        const R kinwl = R(.5) * (R(1) - erf(ml)), kinwr = R(.5) * (R(1) + erf(mr));
+        const R kindl = -(f2 * exp(-ml * ml)), kindr = f2 * exp(-mr * mr);
+        sl = fmin_(cl * (ml * kinwl + kindl), vm - am);
+        sr = fmax_(cr * (mr * kinwr + kindr), vm + am);
+    } else if (sch == 21 || sch == 22) {
+        sl = fmin_(fmin_(R(0), vnl - beta * al), vm - am);
+        sr = fmax_(fmax_(R(0), vnr + beta * ar), vm + am);
+    } else {
+        sl = fmin_(vnl - beta * al, vm - am);
+        sr = fmax_(vnr + beta * ar, vm + am);
+    }
+}
+
 // ig1 = 1/(gamma - 1), gig1 = gamma/(gamma - 1), irL/irR the reciprocal densities
 template <int DIM, typename R>
 __device__ __forceinline__ void num_flux_pre(int sch, R g, R ig1, R gig1, const R* qL, const R* qR, R irL, R irR,
@@ -295,10 +322,9 @@ __device__ __forceinline__ void num_flux_pre(int sch, R g, R ig1, R gig1, const 
     roe_avg<DIM>(g, gig1, qL, qR, irL, irR, rrho, rv, H, a2);
     const R vm = dotd<DIM>(rv, n);
     const R am = asqrt(a2);
-    if (sch == 21 || sch == 22) {  // HLLE(B): calc_flux_hlle.f90:109-143
-        const R beta = (sch == 21) ? R(1) : sqrt((g - R(1)) / (R(2) * g));
-        const R sl = fmin_(fmin_(R(0), vnl - beta * al), vm - am);
-        const R sr = fmax_(fmax_(R(0), vnr + beta * ar), vm + am);
+    if (sch >= 21 && sch <= 24) {  // HLLE, HLLEB, HLLEK, HLLEKB: calc_flux_hlle.f90:109-143
+        R sl, sr;
+        hll_speeds(sch, g, vnl, vnr, al, ar, vm, am, sl, sr);
         const R iks = arcp(sr - sl);
         const R kl = sr * iks, kr = -sl * iks, ku = sl * sr * iks;
         F[0] = (kl * vnl - ku) * qL[0] + (kr * vnr + ku) * qR[0];
@@ -333,10 +359,9 @@ __device__ __forceinline__ void num_flux_pre(int sch, R g, R ig1, R gig1, const 
                                 - (w1 * (rv[d] - a * n[d]) + w2 * rv[d] + w3 * (rv[d] + a * n[d]) + w4 * dvt[d]));
         return;
     }
-    // HLLC (Batten), HLLCB: calc_flux_hllc.f90:61-171
-    const R beta = (sch == 26) ? sqrt((g - R(1)) / (R(2) * g)) : R(1);
-    const R sl = fmin_(vnl - beta * al, vm - am);
-    const R sr = fmax_(vnr + beta * ar, vm + am);
+    // HLLC (Batten), HLLCB, HLLCK, HLLCKB: calc_flux_hllc.f90:61-171
+    R sl, sr;
+    hll_speeds(sch, g, vnl, vnr, al, ar, vm, am, sl, sr);
     if (sl >= R(0)) {
         F[0] = vnl * qL[0];
         F[1] = vnl * (rel + qL[1]);
@@ -975,7 +1000,7 @@ __device__ __forceinline__ void viscous_face_pre(const Physics& ph, const R* qL,
 // ---------------------------------------------------------------------------
 // Face Jacobians for the implicit operator: calc_jacflux (calc_jacflux.f90:49-77), the
 // Euler flux Jacobian A(n) in (rho, rhoE, rho u..) at a primitive state, weighted by
-// calc_jac_hll (calc_jac_hll.f90:46-57) with the HLLC wave speeds clipped at zero
+// calc_jac_hll (calc_jac_hll.f90:46-57) with the flux's wave speeds clipped at zero
 // (calc_flux_hllc.f90:185-189).  Row-major NV x NV blocks, dF_i/dU_j.
 
 template <int DIM, typename R>
@@ -1008,7 +1033,6 @@ __device__ __forceinline__ void jac_hll(const Physics& ph, const R* qL, const R*
 {
     constexpr int NV = DIM + 2;
     const R g = R(ph.gamma);
-    const R beta = (ph.sch_hyp == 26) ? sqrt((g - R(1)) / (R(2) * g)) : R(1);
     const R vnl = dotd<DIM>(qL + 2, n), vnr = dotd<DIM>(qR + 2, n);
     const R irL = R(1) / qL[0], irR = R(1) / qR[0];
     const R al = sqrt(g * qL[1] * irL), ar = sqrt(g * qR[1] * irR);
@@ -1016,8 +1040,10 @@ __device__ __forceinline__ void jac_hll(const Physics& ph, const R* qL, const R*
     roe_avg<DIM>(g, R(ph.gig1), qL, qR, irL, irR, rrho, rv, H, a2);
     const R vm = dotd<DIM>(rv, n);
     const R am = sqrt(a2);
-    R sl = fmin_(vnl - beta * al, vm - am);
-    R sr = fmax_(vnr + beta * ar, vm + am);
+    // the flux's own wave speeds; HLLC's for the fluxes outside the HLL family, which the
+    // reference does not pair with HLL Jacobians
+    R sl, sr;
+    hll_speeds((ph.sch_hyp >= 21 && ph.sch_hyp <= 28) ? ph.sch_hyp : 25, g, vnl, vnr, al, ar, vm, am, sl, sr);
     sl = fmin_(sl, R(0));
     sr = fmax_(sr, R(0));
     jacflux<DIM>(g, qL, n, vnl, jL);

@@ -13,6 +13,7 @@ TPS_EXPL, TPS_RK2, TPS_RK2SSP, TPS_RK3SSP, TPS_RK4 = 10, 30, 31, 33, 35
 TPS_LSRK25BB, TPS_LSRK26BB, TPS_LSRK12BS, TPS_LSRK13BS = 41, 42, 51, 52
 # numerical fluxes (MENU_NUM.f90:42-63)
 SCH_RUSANOV, SCH_ROE, SCH_HLLE, SCH_HLLEB, SCH_HLLC, SCH_HLLCB = 5, 10, 21, 22, 25, 26
+SCH_HLLEK, SCH_HLLEKB, SCH_HLLCK, SCH_HLLCKB = 23, 24, 27, 28  # kinetic wave speeds (MENU_NUM.f90:49-54)
 # gradients (MENU_NUM.f90:128-132)
 GRAD_NONE, GRAD_LSQ, GRAD_LSQW = 5, 10, 12
 # post-limiters (MENU_NUM.f90:95-100)
