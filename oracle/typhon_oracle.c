@@ -630,6 +630,38 @@ static void postlimit_barth(or_ctx* o, double klim)
     }
 }
 
+/* A7: monotonic0/1/2 (SOURCE/MGRID/LIMITER.f90:427-477) of the face values x1 (left) and x2
+   (right) between the cell values a (left) and b (right); postlimit_monotonic
+   (SOURCE/MGRID/postlimit_monotonic.f90:40-110) applies them on every face, ghost cells on the
+   right, to rho, p and each velocity component. */
+static void monotonic(int code, double* x1, double* x2, double a, double b)
+{
+    double l = *x1, r = *x2;
+    if (code == OR_POSTLIM_MONOTONIC0) {
+        if (a < b) { l = FMAX(a, FMIN(l, b)); r = FMIN(b, FMAX(r, a)); }
+        else       { l = FMIN(a, FMAX(l, b)); r = FMAX(b, FMIN(r, a)); }
+    } else if (code == OR_POSTLIM_MONOTONIC1) {
+        double midx = 0.5 * (l + r);
+        if (a < b) { l = FMAX(a, FMIN(FMIN(l, midx), b)); r = FMIN(b, FMAX(FMAX(r, midx), a)); }
+        else       { l = FMIN(a, FMAX(FMAX(l, midx), b)); r = FMAX(b, FMIN(FMIN(r, midx), a)); }
+    } else {
+        double mid = 0.5 * (a + b);
+        if (a < b) { l = FMAX(a, FMIN(l, mid)); r = FMIN(b, FMAX(r, mid)); }
+        else       { l = FMIN(a, FMAX(l, mid)); r = FMAX(b, FMIN(r, mid)); }
+    }
+    *x1 = l; *x2 = r;
+}
+
+static void postlimit_monotonic(or_ctx* o, int code)
+{
+    const or_config* c = &o->c;
+    for (int f = 0; f < o->nf; ++f) {
+        int cl = c->ic1[f], cr = c->ic2[f];
+        for (int v = 0; v < NV; ++v)
+            monotonic(code, &QL(o, v, f), &QR(o, v, f), PRIM(o, v, cl), PRIM(o, v, cr));
+    }
+}
+
 /* A8: Venkatakrishnan limiter (no reference counterpart; Venkatakrishnan, J. Comput.
    Phys. 118 (1995) 120-130, eq. 4.4 smooth min with eps^2 = (K h)^3, h = V^(1/dim)).
    Written as phi = [(dm^2+eps2) + 2 d2 dm] / [dm^2 + 2 d2^2 + dm d2 + eps2]
@@ -1013,6 +1045,8 @@ static void calc_rhs_dt(or_ctx* o, double dt, const double* dtl)
         hres_states(o);
         if (c->postlimiter == OR_POSTLIM_BARTH) postlimit_barth(o, 0.5);
         else if (c->postlimiter == OR_POSTLIM_SUPERBARTH) postlimit_barth(o, 1.0);
+        else if (c->postlimiter >= OR_POSTLIM_MONOTONIC0 && c->postlimiter <= OR_POSTLIM_MONOTONIC2)
+            postlimit_monotonic(o, c->postlimiter);
     }
     boco_face_state(o);
     /* integration_ns_ust (EQNS/integration_ns_ust.f90:87-192): inviscid fluxes */

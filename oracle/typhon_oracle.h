@@ -54,6 +54,9 @@ extern "C" {
 
 /* post limiters: MENU_NUM.f90:95-100 */
 #define OR_POSTLIM_NONE       0
+#define OR_POSTLIM_MONOTONIC0 10
+#define OR_POSTLIM_MONOTONIC1 11
+#define OR_POSTLIM_MONOTONIC2 12
 #define OR_POSTLIM_BARTH      20
 #define OR_POSTLIM_SUPERBARTH 21
 

@@ -113,6 +113,32 @@ def test_barth_postlimiter(method, post):
     _residual_parity(lambda: K.vortex_case(4, L=4.0, method=method, limiter="X", postlimiter=post))
 
 
+MONO = [S.POSTLIM_MONOTONIC0, S.POSTLIM_MONOTONIC1, S.POSTLIM_MONOTONIC2]
+
+
+@pytest.mark.parametrize("post", MONO)
+@pytest.mark.parametrize("method", ["M", "F", "U"])
+def test_monotonic_postlimiter_channel(method, post):
+    """MONOTONIC0/1/2 on the channel's interior, slip-wall, inlet and outlet faces"""
+    _residual_parity(lambda: K.channel_case(24, 12, method=method, limiter="S", postlimiter=post))
+
+
+@pytest.mark.parametrize("post", MONO)
+def test_monotonic_postlimiter_tets_steps(post):
+    _step_parity(lambda: K.bump_case(6, 3, 3, noise=0.08, method="M", limiter="A", gradmeth=S.GRAD_LSQ,
+                                     postlimiter=post, tps_meth=S.TPS_RK2), nsteps=4)
+
+
+@pytest.mark.parametrize("post", MONO)
+def test_monotonic_postlimiter_periodic(post):
+    _residual_parity(lambda: K.vortex_case(4, L=4.0, method="U", limiter="A", postlimiter=post))
+
+
+def test_monotonic1_mixed_precision():
+    _step_parity(lambda: K.channel_case(24, 12, method="M", limiter="S", postlimiter=S.POSTLIM_MONOTONIC1,
+                                        precision=S.PREC_MIXED), nsteps=3, tol=2e-5)
+
+
 def test_weighted_lsq():
     _residual_parity(lambda: K.channel_case(24, 12, method="M", limiter="M", gradmeth=S.GRAD_LSQW))
 

@@ -406,3 +406,94 @@ def test_kinetic_speeds_tend_to_the_classical_bounds():
     mr = aL / cr + f1
     sr = max(cr * 0.5 * integrate.quad(lambda t: special.erfc(-t), -np.inf, mr, epsabs=1e-15)[0], vm + am)
     np.testing.assert_allclose(O.flux_face(qL, qR, n, S.SCH_HLLEK), hll_form(qL, qR, n, sl, sr), rtol=1e-12)
+
+
+# --- MONOTONIC0/1/2 post-limiters ---------------------------------------------------------------
+# LIMITER.f90:416-425 documents them on the face values x1 | x2 between the cell values a | b:
+#   (0) x1 and x2 always in the range of [a, b];
+#   (1) (0), and where a, x1, x2, b are not monotonic x1 and x2 go to their average;
+#   (2) (1), and |x1 - a|, |x2 - b| <= |b - a| / 2.
+# The checks restate (0) and (2) as interval clips (np.clip onto the range and onto the half
+# range of each side), take (1) on faces worked by hand, and compare with the face states the
+# oracle computes without the post-limiter.
+
+def _mono_states(mk, post):
+    out = []
+    for p in (S.POSTLIM_NONE, post):
+        m, s, g, u = mk()
+        o = K.make_oracle(m, dict(s, postlimiter=p), g)
+        o.set_cons(u)
+        o.calc_rhs(1e-3)
+        out.append(o)
+    (ql0, qr0), (ql, qr) = out[0].face_states(), out[1].face_states()
+    prim = out[0].prim()
+    return m, prim, ql0.copy(), qr0.copy(), ql.copy(), qr.copy()
+
+
+def _tets():
+    return K.bump_case(5, 3, 3, noise=0.08, method="M", limiter="A", gradmeth=S.GRAD_LSQ)
+
+
+def test_monotonic0_clips_to_the_cell_range():
+    m, prim, ql0, qr0, ql, qr = _mono_states(_tets, S.POSTLIM_MONOTONIC0)
+    a, b = prim[:, m.ic1], prim[:, m.ic2]
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    ni = m.nface_int
+    np.testing.assert_array_equal(ql, np.clip(ql0, lo, hi))
+    np.testing.assert_array_equal(qr[:, :ni], np.clip(qr0[:, :ni], lo[:, :ni], hi[:, :ni]))
+    # the noisy tet field overshoots on many faces, so the clip is exercised
+    assert (ql0 > hi).sum() + (ql0 < lo).sum() > 20
+
+
+def test_monotonic2_clips_to_the_half_ranges():
+    m, prim, ql0, qr0, ql, qr = _mono_states(_tets, S.POSTLIM_MONOTONIC2)
+    a, b = prim[:, m.ic1], prim[:, m.ic2]
+    mid = 0.5 * (a + b)
+    ni = m.nface_int
+    np.testing.assert_array_equal(ql, np.clip(ql0, np.minimum(a, mid), np.maximum(a, mid)))
+    np.testing.assert_array_equal(qr[:, :ni], np.clip(qr0[:, :ni], np.minimum(b, mid)[:, :ni], np.maximum(b, mid)[:, :ni]))
+    assert np.all(np.abs(ql - a) <= 0.5 * np.abs(b - a) + 4e-16 * np.maximum(np.abs(a), np.abs(b)))
+
+
+def test_monotonic1_on_uniform_rows_by_hand():
+    """superbee MUSCL on the row (0.2, 1.0, 1.2, 2.0, ...) with h = 1/8: at the face between
+    columns 1 and 2, D- = 0.8, D+ = 0.2, D++ = 0.8, superbee(0.8, 0.2) = 0.4, so x1 = 1.0 + 0.2 =
+    1.2 = b and x2 = 1.2 - 0.2 = 1.0 = a: the pair crosses, and MONOTONIC1 takes both to their
+    average 1.1; MONOTONIC0 leaves them (both in [1.0, 1.2]); MONOTONIC2 bounds each side by the
+    midpoint of the cells, also 1.1.  At the face between columns 0 and 1 (0.4 | 0.2 | 1.0 | 1.2):
+    x1 = 0.2 (D- < 0 < D+), x2 = 1.0 - 0.5 superbee(0.2, 0.8) = 0.8: monotonic, MONOTONIC1 keeps
+    both, MONOTONIC2 gives x2 = max(0.8, 0.6) = 0.8."""
+    m, h, nc = _row_mesh(2)
+    rho = np.array([0.2, 1.0, 1.2, 2.0, 1.5, 1.0, 0.6, 0.4])
+    zero = np.zeros(nc)
+    expect = {S.POSTLIM_NONE: {(1, 2): (1.2, 1.0), (0, 1): (0.2, 0.8)},
+              S.POSTLIM_MONOTONIC0: {(1, 2): (1.2, 1.0), (0, 1): (0.2, 0.8)},
+              S.POSTLIM_MONOTONIC1: {(1, 2): (1.1, 1.1), (0, 1): (0.2, 0.8)},
+              S.POSTLIM_MONOTONIC2: {(1, 2): (1.1, 1.1), (0, 1): (0.2, 0.8)}}
+    for post, want in expect.items():
+        o, col = _row_oracle(m, h, nc, S.default_scheme(method="M", limiter="S", postlimiter=post), rho, zero, zero)
+        qL, qR = o.face_states()
+        hits = 0
+        for f, a, b, _, _ in _x_faces(m, col, nc):
+            if (a, b) in want:
+                el, er = want[(a, b)]
+                assert abs(qL[0, f] - el) < 1e-14 and abs(qR[0, f] - er) < 1e-14, (post, a, b, qL[0, f], qR[0, f])
+                hits += 1
+        assert hits == 8
+
+
+def test_monotonic1_makes_every_interior_face_monotonic():
+    m, prim, ql0, qr0, ql, qr = _mono_states(_tets, S.POSTLIM_MONOTONIC1)
+    ni = m.nface_int
+    a, b = prim[:, m.ic1[:ni]], prim[:, m.ic2[:ni]]
+    x1, x2 = ql[:, :ni], qr[:, :ni]
+    up = a < b
+    tol = 1e-15 * np.abs(a).max()
+    assert np.all(np.where(up, (a <= x1 + tol) & (x1 <= x2 + tol) & (x2 <= b + tol),
+                           (a + tol >= x1) & (x1 + tol >= x2) & (x2 + tol >= b)))
+    # faces whose unlimited pair was already monotonic keep it
+    y1, y2 = ql0[:, :ni], qr0[:, :ni]
+    keep = np.where(up, (a <= y1) & (y1 <= y2) & (y2 <= b), (a >= y1) & (y1 >= y2) & (y2 >= b))
+    assert keep.sum() > 20 and (~keep).sum() > 20
+    np.testing.assert_array_equal(x1[keep], y1[keep])
+    np.testing.assert_array_equal(x2[keep], y2[keep])

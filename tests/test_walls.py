@@ -214,6 +214,17 @@ def test_gpu_subsonic_channel_ns(second):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("post", [S.POSTLIM_MONOTONIC0, S.POSTLIM_MONOTONIC1, S.POSTLIM_MONOTONIC2])
+def test_gpu_monotonic_postlimiters_at_walls(post):
+    """the ghost sides of isothermal, adiabatic, subsonic-inlet and outlet faces (MONOTONIC1 reads
+    the ghost cell's own extrapolation with the boundary map of its gradient)"""
+    def mk():
+        m, s, g, u = subsonic_channel(second=True)
+        return m, dict(s, limiter="S", postlimiter=post), g, u
+    _gpu_parity(mk)
+
+
+@pytest.mark.gpu
 def test_gpu_subsonic_channel_euler():
     _gpu_parity(lambda: subsonic_channel(second=True, viscous=False))
 

@@ -112,6 +112,7 @@ struct ty_ctx {
     void* d_qf = nullptr;
     void* work_grad[2] = {nullptr, nullptr};  // per precision: [0] FP64, [1] FP32
     void* work_qf[2] = {nullptr, nullptr};
+    void* work_qf2[2] = {nullptr, nullptr};  // MONOTONIC1: the pair-limited face states
     void* work_gbnd[2] = {nullptr, nullptr};  // ghost-side viscous records of the boundary faces
     // matrix-free implicit solve (allocated on first use)
     bool kry_ok = false;
@@ -188,6 +189,7 @@ void free_state(ty_ctx* c)
     c->d_us[0] = c->d_us[1] = nullptr;
     c->d_grad = c->d_qf = nullptr;
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
+    c->work_qf2[0] = c->work_qf2[1] = nullptr;
     c->work_gbnd[0] = c->work_gbnd[1] = nullptr;
     c->kry_ok = false;
     c->kry_base = false;
@@ -711,6 +713,11 @@ int ensure_work(ty_ctx* c, int prec)
         c->owned.push_back(p);
         c->work_qf[prec] = p;
     }
+    if (c->ph.mono == 11 && !c->work_qf2[prec]) {
+        HIPCHK(c, hipMalloc(&p, (size_t)c->ns * c->nv * c->n * rb));
+        c->owned.push_back(p);
+        c->work_qf2[prec] = p;
+    }
     return TY_OK;
 }
 
@@ -726,7 +733,7 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
         return fail(c, TY_ERR_UNSUPPORTED, "numerical flux " + std::to_string(sch) + " not provided");
     if (!strchr("NMFU", s->method) || s->method == 0) return fail(c, TY_ERR_UNSUPPORTED, "high-resolution method");
     if (!strchr("XMAVSKLW", s->limiter) || s->limiter == 0) return fail(c, TY_ERR_UNSUPPORTED, "limiter");
-    if (s->postlimiter != 0 && s->postlimiter != 20 && s->postlimiter != 21)
+    if (s->postlimiter != 0 && s->postlimiter != 20 && s->postlimiter != 21 && (s->postlimiter < 10 || s->postlimiter > 12))
         return fail(c, TY_ERR_UNSUPPORTED, "post-limiter");
     if (s->limiter == 'W' && s->postlimiter != 0)
         return fail(c, TY_ERR_UNSUPPORTED, "Venkatakrishnan limiter with a post-limiter");
@@ -759,6 +766,7 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     c->d_us[0] = c->d_us[1] = nullptr;
     c->d_grad = c->d_qf = nullptr;
     c->work_grad[0] = c->work_grad[1] = c->work_qf[0] = c->work_qf[1] = nullptr;
+    c->work_qf2[0] = c->work_qf2[1] = nullptr;
     c->work_gbnd[0] = c->work_gbnd[1] = nullptr;
     c->kry_ok = c->kry_base = false;
     c->kry_m = 0;
@@ -797,8 +805,10 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     ph.method = s->method;
     ph.limiter = s->method == 'N' ? 'X' : s->limiter;
     ph.lsqw = s->gradmeth == 12;
-    ph.barth = s->method != 'N' && s->limiter != 'W' && s->postlimiter != 0;
+    // post-limiters act on second-order face states only (on cell values they change nothing)
+    ph.barth = s->method != 'N' && s->limiter != 'W' && (s->postlimiter == 20 || s->postlimiter == 21);
     ph.klim = s->postlimiter == 21 ? 1.0 : 0.5;
+    ph.mono = (s->method != 'N' && s->limiter != 'W' && s->postlimiter >= 10 && s->postlimiter <= 12) ? s->postlimiter : 0;
     ph.ig1 = 1.0 / (gas->gamma - 1.0);
     ph.gig1 = gas->gamma / (gas->gamma - 1.0);
     ph.irgas = 1.0 / gas->r_const;
@@ -969,6 +979,7 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     a.gbnd_out = (R*)c->work_gbnd[prec];
     a.qf = (const R*)c->work_qf[prec];
     a.qf_out = (R*)c->work_qf[prec];  // null for first-order schemes
+    a.qf_mono = (R*)c->work_qf2[prec];  // null unless MONOTONIC1
     a.res_out = c->d_res;
     a.d_dt = c->d_dt;
     if (c->steady_on) {

@@ -80,8 +80,9 @@ struct Physics {
     char method;   // 'N','M','F','U'
     char limiter;  // 'X','M','A','V','S','K','L','W'
     int lsqw;      // weighted LSQ
-    int barth;     // post-limiter on
+    int barth;     // Barth / super-Barth post-limiter on
     double klim;
+    int mono;      // monotonic post-limiter 10, 11, 12 (MONOTONIC0/1/2) of second-order face states; 0 off
     // launch constants derived on the host: 1/(gamma-1), gamma/(gamma-1), 1/r,
     // conductivity factor cp/Pr, 1/Tref and Tref + S of Sutherland's law
     double ig1, gig1, irgas, cp_pr, itref, tsum;
@@ -118,6 +119,8 @@ struct KArgs {
     R* gbnd_out;          // written by K1 (vrec)
     const R* qf;          // [ns][n][nv] own-side face states (second order), one record per slot
     R* qf_out;            // written by K1 when not null
+    R* qf_mono;           // MONOTONIC1: the face states limited in pairs (k_mono1 from qf_out), which K2
+                          // then reads in place of qf; null otherwise
     double* res_out;      // [nv][n] residual-only mode
     R* dinv_out;          // [nv*nv][n] inverse diagonal blocks of the implicit operator (residual mode)
     double* diag_out;     // [nv*nv][n] the diagonal blocks themselves (optional)

@@ -685,7 +685,61 @@ __device__ __forceinline__ void write_visc_records(const KArgs<R>& a, int c, con
 }
 
 // reconstruction performed by K1 after the gradient
-enum Rec : int { REC_NONE = 0, REC_VENKAT = 1, REC_MUSCL = 2, REC_MUSCL_BARTH = 3 };
+enum Rec : int { REC_NONE = 0, REC_VENKAT = 1, REC_MUSCL = 2, REC_MUSCL_BARTH = 3, REC_MUSCL_MONO = 4 };
+
+// monotonic0/1/2 (LIMITER.f90:427-477) of the face values x1 (left side) and x2 (right side)
+// between the cell values a (left) and b (right): MONOTONIC0 keeps each in [a, b], MONOTONIC1 also
+// bounds each by the pair's average, MONOTONIC2 keeps each between its cell value and (a + b) / 2.
+// mono_left returns the limited x1, mono_right the limited x2 (0 and 2 read only their own side).
+template <typename R>
+__device__ __forceinline__ R mono_left(int m, R x1, R x2, R a, R b)
+{
+    if (m == 10) return a < b ? fmax_(a, fmin_(x1, b)) : fmin_(a, fmax_(x1, b));
+    if (m == 11) {
+        const R midx = R(.5) * (x1 + x2);
+        return a < b ? fmax_(a, fmin_(fmin_(x1, midx), b)) : fmin_(a, fmax_(fmax_(x1, midx), b));
+    }
+    const R mid = R(.5) * (a + b);
+    return a < b ? fmax_(a, fmin_(x1, mid)) : fmin_(a, fmax_(x1, mid));
+}
+template <typename R>
+__device__ __forceinline__ R mono_right(int m, R x1, R x2, R a, R b)
+{
+    if (m == 10) return a < b ? fmin_(b, fmax_(x2, a)) : fmax_(b, fmin_(x2, a));
+    if (m == 11) {
+        const R midx = R(.5) * (x1 + x2);
+        return a < b ? fmin_(b, fmax_(fmax_(x2, midx), a)) : fmax_(b, fmin_(fmin_(x2, midx), a));
+    }
+    const R mid = R(.5) * (a + b);
+    return a < b ? fmin_(b, fmax_(x2, mid)) : fmax_(b, fmin_(x2, mid));
+}
+
+// postlimit_monotonic (postlimit_monotonic.f90:40-110) on this cell's face state q of slot k in K1:
+// MONOTONIC0/2 from the two cell values; MONOTONIC1 of a boundary face with the ghost side's own
+// extrapolation (the hres of the ghost cell with the cell_grad boundary map of its gradient), its
+// interior faces are left to k_mono1, which sees both sides' face states.
+template <int DIM, typename R>
+__device__ __forceinline__ void mono_own(const KArgs<R>& a, int k, int c, int L, bool other, const R* qc, const R* qo,
+                                         const R* dk, const R* rk, const R* gr, R* q)
+{
+    constexpr int NV = DIM + 2;
+    const int m = a.ph.mono;
+    if (m == 11) {
+        if (!link_boundary(L)) return;
+        R nf[DIM], gb[NV * DIM], rg[DIM], x2[NV];
+        (void)slot_normal<DIM>(a.g, k, a.n, c, nf);
+        bc_grad<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, gr, gb);
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) rg[e] = rk[e] - dk[e];
+        muscl_own<DIM>(a.ph, qc, qo, dk, rg, gb, true, x2);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) q[v] = mono_left(11, q[v], x2[v], qc[v], qo[v]);
+        return;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        q[v] = other ? mono_right(m, R(0), q[v], qo[v], qc[v]) : mono_left(m, q[v], R(0), qc[v], qo[v]);
+}
 
 // DENSE: every cell uses all NSM == ns slots (closed tet/hex/quad meshes), so the slot
 // loops carry no guards and every slot's loads can be issued together.
@@ -781,7 +835,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
         }
         return;
     }
-    // MUSCL family, optionally Barth-limited
+    // MUSCL family, optionally Barth-limited or monotonic post-limited
     constexpr bool barth = REC == REC_MUSCL_BARTH;
     const R klim = R(a.ph.klim);
     R ratio[3] = {R(1), R(1), R(1)};
@@ -807,6 +861,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
                 ratio[gi] = fmin_(ratio[gi], lr);
             }
         } else {
+            if (REC == REC_MUSCL_MONO) mono_own<DIM>(a, k, c, L, other, qc, qo, dk, rk, gr, q);
             R* out = a.qf_out + ((size_t)k * n + c) * NV;
 #pragma unroll
             for (int v = 0; v < NV; ++v) out[v] = q[v];
@@ -1460,6 +1515,43 @@ __global__ void k_to_stage(const double* __restrict__ src, R* __restrict__ dst, 
     store_prim<R>(u, nv, g1, dst + (size_t)c * nv);
 }
 
+// MONOTONIC1 on interior and periodic faces: each cell limits the pair (own, far) of K1's face
+// states of every slot with the two cell values, in the face's canonical orientation, and keeps its
+// own member; both cells of a face compute the same pair.  Boundary slots were limited by K1.
+template <int DIM, typename R, int NSM, bool DENSE>
+__global__ __launch_bounds__(BLOCK) void k_mono1(KArgs<R> a)
+{
+    constexpr int NV = DIM + 2;
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= a.n || (a.stop && *a.stop)) return;
+    const int n = a.n;
+    R qc[NV];
+    load_prim<DIM>(a.us, c, qc);
+#pragma unroll
+    for (int k = 0; k < NSM; ++k) {
+        if (!DENSE && k >= a.ns) break;
+        const int L = a.link[(size_t)k * n + c];
+        if (!DENSE && L == SLOT_UNUSED) continue;
+        const R* own = a.qf_out + ((size_t)k * n + c) * NV;
+        R* out = a.qf_mono + ((size_t)k * n + c) * NV;
+        if (link_boundary(L)) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) out[v] = own[v];
+            continue;
+        }
+        const R* far = a.qf_out + ((size_t)link_mirror(L) * n + link_index(L)) * NV;
+        R qo[NV];
+        load_prim<DIM>(a.us, link_index(L), qo);
+        const bool other = link_other(L);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R x1 = other ? far[v] : own[v], x2 = other ? own[v] : far[v];
+            const R qa = other ? qo[v] : qc[v], qb = other ? qc[v] : qo[v];
+            out[v] = other ? mono_right(11, x1, x2, qa, qb) : mono_left(11, x1, x2, qa, qb);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 
@@ -1470,7 +1562,9 @@ static void gradient_variant(const KArgs<R>& a, hipStream_t s)
     if (!a.qf_out) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_NONE, DENSE>), grid, block, 0, s, a);
     else if (a.ph.limiter == 'W') hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_VENKAT, DENSE>), grid, block, 0, s, a);
     else if (a.ph.barth) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL_BARTH, DENSE>), grid, block, 0, s, a);
+    else if (a.ph.mono) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL_MONO, DENSE>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL, DENSE>), grid, block, 0, s, a);
+    if (a.qf_out && a.qf_mono) hipLaunchKernelGGL((k_mono1<DIM, R, NSM, DENSE>), grid, block, 0, s, a);
 }
 
 // slot-count variant: dense 4 (tets, quads) or 6 (hexes) slots per cell, otherwise guarded loops
@@ -1497,8 +1591,10 @@ static void flux_variant(const KArgs<R>& a, hipStream_t s)
 }
 
 template <int DIM, typename R>
-hipError_t launch_flux(const KArgs<R>& a, hipStream_t s)
+hipError_t launch_flux(const KArgs<R>& a_in, hipStream_t s)
 {
+    KArgs<R> a = a_in;
+    if (a.qf_mono) a.qf = a.qf_mono;  // MONOTONIC1: the pair-limited face states
     const bool jac = a.dinv_out != nullptr || a.diag_out != nullptr;
     if (jac) {  // implicit diagonal blocks: guarded loops only
         if (a.ns <= 4) flux_variant<DIM, R, 4, true, false>(a, s);

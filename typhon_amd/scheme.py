@@ -18,6 +18,7 @@ SCH_HLLEK, SCH_HLLEKB, SCH_HLLCK, SCH_HLLCKB = 23, 24, 27, 28  # kinetic wave sp
 GRAD_NONE, GRAD_LSQ, GRAD_LSQW = 5, 10, 12
 # post-limiters (MENU_NUM.f90:95-100)
 POSTLIM_NONE, POSTLIM_BARTH, POSTLIM_SUPERBARTH = 0, 20, 21
+POSTLIM_MONOTONIC0, POSTLIM_MONOTONIC1, POSTLIM_MONOTONIC2 = 10, 11, 12
 # dissipative fluxes (MENU_NUM.f90:120-125)
 DIS_NOFLUX, DIS_CELLAVG2, DIS_CELLFULL = 1, 5, 10
 # fluid models (EQNS.f90:27-32) and viscosity laws (EQNS.f90:39-43)
