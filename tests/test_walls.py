@@ -225,6 +225,24 @@ def test_gpu_monotonic_postlimiters_at_walls(post):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dis,why", [(S.DIS_COMPACT, "COMPACT"), (S.DIS_FACECENTERED, "FACE-AVERAGE"),
+                                     (S.DIS_FACEPENALTY, "PENALTY")])
+def test_gpu_unrun_viscous_discretisations_refused(dis, why):
+    """COMPACT stops in the reference's NS velocity-gradient interpolation; FACE-AVERAGE and PENALTY
+    need face gradients, which the reference offers only with its spectral-volume method: the
+    scheme is refused and the previous one stays"""
+    from typhon_amd.solver import Solver
+    from typhon_amd._lib import TyError
+    m, s, g, u = subsonic_channel(second=True)
+    gpu = Solver(m, s, g)
+    gpu.set_state(u)
+    with pytest.raises(TyError, match=why):
+        gpu.set_scheme(dict(s, sch_dis=dis), g)
+    r = gpu.calc_rhs(1e-6)
+    assert np.isfinite(r).all()
+
+
+@pytest.mark.gpu
 def test_gpu_subsonic_channel_euler():
     _gpu_parity(lambda: subsonic_channel(second=True, viscous=False))
 

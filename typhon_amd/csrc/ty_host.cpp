@@ -739,6 +739,16 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
         return fail(c, TY_ERR_UNSUPPORTED, "Venkatakrishnan limiter with a post-limiter");
     const bool viscous = gas->typ_fluid == 15;
     if (gas->typ_fluid != 10 && !viscous) return fail(c, TY_ERR_UNSUPPORTED, "fluid model");
+    // the reference's NS residual runs CELL-AVERAGE (5) and FULL (10) only: COMPACT (2) stops in
+    // interp_facegradient_vect (interpface_gradient_vect.f90:61-63, called by
+    // calc_flux_viscous_cell.f90:111); FACE-AVERAGE (20) and PENALTY (22) need face gradients, which
+    // get_gradientmethod refuses for first-order and MUSCL methods (MENU_NUM.f90:307-309, 318-319)
+    if (viscous && s->sch_dis == 2)
+        return fail(c, TY_ERR_UNSUPPORTED, "dissipative flux COMPACT: the NS velocity-gradient interpolation "
+                                           "does not authorise it (use CELL-AVERAGE or FULL)");
+    if (viscous && (s->sch_dis == 20 || s->sch_dis == 22))
+        return fail(c, TY_ERR_UNSUPPORTED, "dissipative flux FACE-AVERAGE/PENALTY: face based gradients are "
+                                           "unavailable for first-order and MUSCL methods");
     if (viscous && s->sch_dis != 5 && s->sch_dis != 10)
         return fail(c, TY_ERR_UNSUPPORTED, "dissipative flux scheme");
     if ((s->method != 'N' || viscous) && s->gradmeth != 10 && s->gradmeth != 12)

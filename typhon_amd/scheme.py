@@ -21,6 +21,7 @@ POSTLIM_NONE, POSTLIM_BARTH, POSTLIM_SUPERBARTH = 0, 20, 21
 POSTLIM_MONOTONIC0, POSTLIM_MONOTONIC1, POSTLIM_MONOTONIC2 = 10, 11, 12
 # dissipative fluxes (MENU_NUM.f90:120-125)
 DIS_NOFLUX, DIS_CELLAVG2, DIS_CELLFULL = 1, 5, 10
+DIS_COMPACT, DIS_FACECENTERED, DIS_FACEPENALTY = 2, 20, 22  # refused for NS, as by the reference
 # fluid models (EQNS.f90:27-32) and viscosity laws (EQNS.f90:39-43)
 EQ_EULER, EQ_NSLAM = 10, 15
 VISC_SUTH, VISC_DYNCST, VISC_KINCST = 11, 13, 14
