@@ -1,5 +1,6 @@
-// Instantiations of the residual kernels (ty_kernels_impl.h) for FP64 arithmetic,
-// plus the precision-independent time-step and steady-cycle kernels.
+// The precision-independent time-step and steady-cycle kernels; the residual kernels of
+// ty_kernels_impl.h are instantiated per precision, dimension and kernel in ty_inst_*.hip
+// (separate translation units so that they compile in parallel).
 #include "ty_kernels_impl.h"
 
 namespace ty {
@@ -10,11 +11,6 @@ int grid_blocks(int n)
     return (nb + 7) & ~7;
 }
 
-#define TY_INST(DIM, R)                                                          \
-    template hipError_t launch_gradient<DIM, R>(const KArgs<R>&, hipStream_t); \
-    template hipError_t launch_flux<DIM, R>(const KArgs<R>&, hipStream_t);
-TY_INST(2, double)
-TY_INST(3, double)
 template hipError_t launch_timestep<2>(const DtArgs&, double, double*, int*, hipStream_t);
 template hipError_t launch_timestep<3>(const DtArgs&, double, double*, int*, hipStream_t);
 template hipError_t launch_to_stage<double>(const double*, double*, int, int, double, hipStream_t);

@@ -743,7 +743,9 @@ __device__ __forceinline__ void mono_own(const KArgs<R>& a, int k, int c, int L,
 
 // DENSE: every cell uses all NSM == ns slots (closed tet/hex/quad meshes), so the slot
 // loops carry no guards and every slot's loads can be issued together.
-template <int DIM, typename R, int NSM, int REC, bool DENSE>
+// GOUT: the cell gradients (or the viscous records) are written too; the Euler reconstructions
+// launch without it, so they do not carry the viscous-record code's registers.
+template <int DIM, typename R, int NSM, int REC, bool DENSE, bool GOUT>
 __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 {
     constexpr int NV = DIM + 2;
@@ -780,7 +782,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
             }
         }
     }
-    if (a.grad_out) {
+    if (GOUT && a.grad_out) {
         if (a.vrec) write_visc_records<DIM>(a, c, qc, gr, a.grad_out, true);
         else {
             R* go = a.grad_out + (size_t)c * NV * DIM;
@@ -1555,15 +1557,28 @@ __global__ __launch_bounds__(BLOCK) void k_mono1(KArgs<R> a)
 // ---------------------------------------------------------------------------
 // launchers
 
-template <int DIM, typename R, int NSM, bool DENSE>
-static void gradient_variant(const KArgs<R>& a, hipStream_t s)
+// gradient_variant, rec_variant and flux_variant are instantiated one per translation unit by the
+// build (they are the expensive part to compile); launch_gradient/launch_flux only pick one.
+template <int DIM, typename R, int NSM, bool DENSE, bool GOUT>
+void rec_variant(const KArgs<R>& a, hipStream_t s)
 {
     const dim3 grid(grid_blocks(a.n)), block(BLOCK);
-    if (!a.qf_out) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_NONE, DENSE>), grid, block, 0, s, a);
-    else if (a.ph.limiter == 'W') hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_VENKAT, DENSE>), grid, block, 0, s, a);
-    else if (a.ph.barth) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL_BARTH, DENSE>), grid, block, 0, s, a);
-    else if (a.ph.mono) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL_MONO, DENSE>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL, DENSE>), grid, block, 0, s, a);
+    if (a.ph.limiter == 'W')
+        hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_VENKAT, DENSE, GOUT>), grid, block, 0, s, a);
+    else if (a.ph.barth)
+        hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL_BARTH, DENSE, GOUT>), grid, block, 0, s, a);
+    else if (a.ph.mono)
+        hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL_MONO, DENSE, GOUT>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_MUSCL, DENSE, GOUT>), grid, block, 0, s, a);
+}
+
+template <int DIM, typename R, int NSM, bool DENSE>
+void gradient_variant(const KArgs<R>& a, hipStream_t s)
+{
+    const dim3 grid(grid_blocks(a.n)), block(BLOCK);
+    if (!a.qf_out) hipLaunchKernelGGL((k_gradient<DIM, R, NSM, REC_NONE, DENSE, true>), grid, block, 0, s, a);
+    else if (a.grad_out) rec_variant<DIM, R, NSM, DENSE, true>(a, s);
+    else rec_variant<DIM, R, NSM, DENSE, false>(a, s);
     if (a.qf_out && a.qf_mono) hipLaunchKernelGGL((k_mono1<DIM, R, NSM, DENSE>), grid, block, 0, s, a);
 }
 
@@ -1579,7 +1594,7 @@ hipError_t launch_gradient(const KArgs<R>& a, hipStream_t s)
 }
 
 template <int DIM, typename R, int NSM, bool JAC, bool DENSE>
-static void flux_variant(const KArgs<R>& a, hipStream_t s)
+void flux_variant(const KArgs<R>& a, hipStream_t s)
 {
     const dim3 grid(grid_blocks(a.n)), block(BLOCK);
     const bool fs = a.ph.method != 'N';
