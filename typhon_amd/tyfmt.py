@@ -234,3 +234,149 @@ def _periodic_wrap(t: TyMesh, bocos):
     for _ in range(3):
         wrap = wrap[wrap]
     return wrap
+
+
+# ---------------------------------------------------------------------------
+# .tys solution files (TYFMT_SOL.f90, TYPHON_FMT.f90) and the restart initialisation
+# (INIT TYPE=TYPHON/TYS, def_init.f90:54-73 -> init_gridfield_ust.f90:89-104)
+
+TY_SOLUTION, TY_SCALAR, TY_VECTOR = 30, 31, 32          # section user types (TYPHON_FMT.f90:31-46)
+XTY_FILE_MESH, XTY_FILE_SOL = 10, 20                    # FILE_HEADER file types
+XTY_VERSION, MESH_FULL = 3, 11
+SOL_UNDEFINED, SOL_CELL = 1, 2                          # GENFIELD.f90:26-27
+QS_DENSITY, QS_PRESSURE, QV_VELOCITY = 1, 2, 101        # QUANTITY.f90:13-26
+QUANTITY_NAME = {QS_DENSITY: "RHO", QS_PRESSURE: "PS", QV_VELOCITY: "V"}
+
+
+@dataclass
+class TySolution:
+    version: int                 # TYPHON format version of the file
+    mesh_sections: list          # the file's mesh sections (USTMESH ... marks), as read
+    soltype: int
+    ncell: int
+    ndof: int
+    scalars: dict                # quantity id -> [nelem]
+    vectors: dict                # quantity id -> [nelem, 3]
+
+
+def mesh_sections(path) -> list:
+    """the mesh sections of a .tym file (everything after its FILE_HEADER), with the USTMESH
+    header in the form of TYPHON format 2-3 (typhonwrite_umeshheader, TYFMT_MESH.f90:94-140: user
+    type 10, parameters grid type, ncell, nface, nvtex, ncellsections, nfacesections, ncellmark,
+    nfacemark); a format-1 header (user type 1, typhonread_umeshheader :177-186: no ncellmark) is
+    converted, its other parameters kept"""
+    _, secs = read_xbin(path)
+    hdr = secs[0].params if secs and secs[0].usertype == TY_FILEDEF else [XTY_VERSION]
+    out = [s for s in secs[1:]] if secs and secs[0].usertype == TY_FILEDEF else list(secs)
+    if hdr[0] == 1:
+        for k, s in enumerate(out):
+            if s.name == "USTMESH":
+                p = s.params
+                out[k] = Section("USTMESH", DT_NODATA, TY_MESHDEF, [p[0], p[1], p[2], p[3], p[4], p[5], 0, p[6]])
+    return out
+
+
+def write_tys(path, msecs: list, rho, p, vel, xbin_version=2) -> None:
+    """A TYPHON solution file as Typhon's TYPHON output writes it (outputzone_open.f90:44-49,
+    outputzone_ustmesh.f90:52-64, typhonwrite_sol in TYFMT_SOL.f90:30-72): FILE_HEADER (format 3,
+    solution file, one mesh, one solution, full mesh), the mesh sections ``msecs`` (from
+    mesh_sections of the mesh's .tym), then SOLUTION and one section per primitive quantity of
+    the NS solver (def_model_ns.f90:97-99): RHO and PS (1-D real arrays), V (3 reals per element).
+
+    rho, p [nelem], vel [nelem, 3] in Typhon's cell order; Typhon writes all ncell elements of its
+    field (interior cells, then one ghost per boundary face), a caller that holds interior cells
+    only passes those followed by whatever it keeps for the ghosts.  The SOLUTION parameters are
+    (SOL_CELL, nelem, nelem, 1, 2, 1): Typhon's reader stores them and reads nelem values."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+    nelem = rho.shape[0]
+    if p.shape[0] != nelem or vel.shape[0] != nelem:
+        raise ValueError("write_tys: rho, p and vel must have the same number of elements")
+    secs = [Section("FILE_HEADER", DT_NODATA, TY_FILEDEF, [XTY_VERSION, XTY_FILE_SOL, 1, 1, MESH_FULL])]
+    secs += list(msecs)
+    secs.append(Section("SOLUTION", DT_NODATA, TY_SOLUTION, [SOL_CELL, nelem, nelem, 1, 2, 1]))
+    secs.append(Section(QUANTITY_NAME[QS_DENSITY], DT_REAL_ORD, TY_SCALAR, [QS_DENSITY], data=rho))
+    secs.append(Section(QUANTITY_NAME[QS_PRESSURE], DT_REAL_ORD, TY_SCALAR, [QS_PRESSURE], data=p))
+    secs.append(Section(QUANTITY_NAME[QV_VELOCITY], DT_REAL_ORD, TY_VECTOR, [QV_VELOCITY], data=vel))
+    write_xbin(path, secs, version=xbin_version)
+
+
+def read_tys(path) -> TySolution:
+    """typhon_openread + typhonread_ustmesh (kept as sections) + typhonread_sol (TYFMT_SOL.f90:77-170):
+    the SOLUTION section's parameters per format version (1-2: nelem, nscal, nvect; 3: soltype,
+    nelem, ncell, ndof, nscal, nvect), then nscal scalar and nvect vector sections"""
+    xver, secs = read_xbin(path)
+    if not secs or secs[0].usertype != TY_FILEDEF or not secs[0].params:
+        raise ValueError(f"{path}: no TYPHON FILE_HEADER")
+    hdr = secs[0].params
+    version = hdr[0]
+    if version > XTY_VERSION:
+        raise ValueError(f"{path}: TYPHON format version {version} is newer than {XTY_VERSION}")
+    nb_sol = hdr[3] if len(hdr) >= 4 else 0
+    if nb_sol < 1:
+        raise ValueError(f"{path}: missing solution in TYPHON input file")
+    isol = [k for k, s in enumerate(secs) if s.usertype == TY_SOLUTION]
+    if not isol:
+        raise ValueError(f"{path}: expecting SOLUTION data section")
+    k = isol[0]
+    par = secs[k].params
+    if version <= 2:
+        if len(par) != 3:
+            raise ValueError(f"{path}: expecting 3 parameters in SOLUTION data section")
+        nelem, nsca, nvec = par
+        soltype, ncell, ndof = SOL_CELL, nelem, 1
+    else:
+        if len(par) != 6:
+            raise ValueError(f"{path}: expecting 6 parameters in SOLUTION data section")
+        soltype, nelem, ncell, ndof, nsca, nvec = par
+    scalars, vectors = {}, {}
+    rest = secs[k + 1:]
+    if len(rest) < nsca + nvec:
+        raise ValueError(f"{path}: {nsca} scalar and {nvec} vector sections expected")
+    for s in rest[:nsca]:
+        if s.usertype != TY_SCALAR:
+            raise ValueError(f"{path}: expecting SCALAR SOLUTION data section")
+        scalars[s.params[0]] = np.asarray(s.data, dtype=np.float64).reshape(-1)[:nelem]
+    for s in rest[nsca:nsca + nvec]:
+        if s.usertype != TY_VECTOR:
+            raise ValueError(f"{path}: expecting VECTOR SOLUTION data section")
+        vectors[s.params[0]] = np.asarray(s.data, dtype=np.float64).reshape(-1, 3)[:nelem]
+    return TySolution(version, secs[1:k], soltype, ncell, ndof, scalars, vectors)
+
+
+def restart_state(path, ncell_int: int, gamma: float = 1.4) -> np.ndarray:
+    """The conservative state [5, ncell_int] (rho, rhoE, rho u) that INIT TYPE=TYPHON gives the NS
+    solver: density, pressure and velocity of the file's first ncell_int elements, then
+    calc_varcons_ns (calc_varcons_ns.f90:35-45): rhoE = p / (gamma - 1) + rho |u|^2 / 2."""
+    sol = read_tys(path)
+    try:
+        rho, p, vel = sol.scalars[QS_DENSITY], sol.scalars[QS_PRESSURE], sol.vectors[QV_VELOCITY]
+    except KeyError as e:
+        raise ValueError(f"{path}: the NS solver needs RHO, PS and V, missing quantity {e}") from None
+    if rho.shape[0] < ncell_int:
+        raise ValueError(f"{path}: {rho.shape[0]} elements for {ncell_int} cells")
+    rho, p, vel = rho[:ncell_int], p[:ncell_int], vel[:ncell_int]
+    mom = rho[:, None] * vel
+    rhoe = p / (gamma - 1.0) + 0.5 * rho * (vel * vel).sum(1)
+    return np.vstack([rho, rhoe, mom.T])
+
+
+def solution_fields(cons: np.ndarray, gamma: float = 1.4):
+    """(rho, p, vel [n, 3]) of a conservative state [5, n] (calc_varprim_ns.f90:47-52)"""
+    rho = cons[0]
+    vel = (cons[2:5] / rho).T
+    p = (gamma - 1.0) * (cons[1] - 0.5 * rho * (vel * vel).sum(1))
+    return rho, p, vel
+
+
+def write_state_tys(path, tym_path, mesh, cons: np.ndarray, gamma: float = 1.4) -> None:
+    """write_tys for the solver state cons [5, ncell_int] on ``mesh`` (built from ``tym_path``):
+    interior cells, then one ghost per boundary face holding its interior cell's values (the
+    restart recomputes the ghost states from the boundary conditions before using them)."""
+    rho, p, vel = solution_fields(cons, gamma)
+    ghost_of = np.empty(mesh.ncell - mesh.ncell_int, dtype=np.int64)
+    for b in mesh.bocos:
+        ghost_of[b.iface - mesh.nface_int] = mesh.ic1[b.iface]
+    idx = np.concatenate([np.arange(mesh.ncell_int), ghost_of])
+    write_tys(path, mesh_sections(tym_path), rho[idx], p[idx], vel[idx])
