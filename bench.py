@@ -1,13 +1,22 @@
 """Benchmark: residual Mcell-updates/s of the MI355X residual path.
 
-Workload (one GPU): 3D Euler, 100^3 hexes split into 6,000,000 tets on a periodic
-[0,10]^3 box (vertices jittered +-5% from seed 2, cells RCM-ordered on upload),
-isentropic vortex (strength 5) advected at M=0.5, second-order MUSCL with LSQ
-gradients and the Venkatakrishnan limiter (K=5), HLLC, classical RK4, CFL 0.8.
-A step is one RK4 step = 4 residual evaluations (time step on the device
-included); Mcell-updates/s = cells * 4 * steps / time.  `value` is the FP64
-form; the FP32+FP64-accumulate form and its density error after the same steps
-are reported beside it.
+Headline workload (one GPU): 3D Euler, 100^3 hexes split into 6,000,000 tets on a periodic
+[0,10]^3 box (vertices jittered +-5% from seed 2, cells in Hilbert order on upload; the
+reverse Cuthill-McKee order is timed beside it), isentropic vortex (strength 5) advected at
+M=0.5, second-order MUSCL with LSQ gradients and the Venkatakrishnan limiter (K=5), HLLC,
+classical RK4, CFL 0.8.  A step is one RK4 step = 4 residual evaluations (time step on the
+device included); Mcell-updates/s = cells * 4 * steps / time.  `value` is the FP64 form; the
+FP32+FP64-accumulate form, its per-field error after 1000 steps and the other workloads are
+reported beside it:
+
+- c1: 2D channel, 256 x 128 jittered quads, first-order HLLC + RK2, 500 steps, FP64 (the CPU
+  oracle runs it at full size beside the GPU);
+- adjoint / c4: the 249,480-tet bump channel (first-order HLLC + RK2): primal rate,
+  adjoint/primal kernel time, and the wall-force gradient (inflow and 64 wall vertices) against
+  central differences and the tangent-linear product;
+- implicit: 160^3 = 4.1M-hex Navier-Stokes Taylor-Green, matrix-free backward Euler;
+- ns_16m: 256^3 = 16.8M-hex Navier-Stokes, RK4, FP64 and mixed, its own roofline and CPU
+  baseline, adjoint/primal ratio and a 200-step checkpointed adjoint.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 100] [--no-cpu]
 """
@@ -94,17 +103,150 @@ def cpu_baseline(n_cpu, budget_s=20.0):
                        f"{m.ncell_int} periodic tets, {steps} RK4 steps ({el:.1f} s)")
 
 
+def field_errors(a, b, gamma=1.4):
+    """relative L2 and Linf errors of state a against state b [5, n] per field: density, total
+    energy, momentum (one vector field: norms of the vector difference over norms of the vector)
+    and pressure; L2 = ||a - b||_2 / ||b||_2, Linf = max |a - b| / max |b|"""
+    def p(u):
+        return (gamma - 1.0) * (u[1] - 0.5 * (u[2] ** 2 + u[3] ** 2 + u[4] ** 2) / u[0])
+
+    out = {}
+    for name, fa, fb in (("rho", a[0], b[0]), ("rhoE", a[1], b[1]), ("momentum", a[2:5], b[2:5]),
+                         ("p", p(a), p(b))):
+        d = fa - fb
+        if fa.ndim == 2:
+            dn, bn = np.sqrt((d * d).sum(0)), np.sqrt((fb * fb).sum(0))
+        else:
+            dn, bn = np.abs(d), np.abs(fb)
+        out[name] = dict(l2=float(np.sqrt((dn * dn).sum() / (bn * bn).sum())), linf=float(dn.max() / bn.max()))
+    return out
+
+
+def run_steps(solver, u, steps):
+    """state after ``steps`` device-resident steps from u (time step on the device)"""
+    solver.set_state(u)
+    solver.advance(steps)
+    solver.sync()
+    return solver.get_state()
+
+
+def mixed_accuracy(m, s, g, u, steps=1000, order="hilbert"):
+    """FP64 and mixed forms advanced ``steps`` steps from the same state; per-field errors of mixed
+    against FP64"""
+    from typhon_amd import scheme as S
+    from typhon_amd.solver import Solver
+    st = {}
+    for prec in (S.PREC_FP64, S.PREC_MIXED):
+        solver = Solver(m, dict(s, precision=prec), g, cell_order=order)
+        st[prec] = run_steps(solver, u, steps)
+        solver.close()
+    return dict(steps=steps, reference="the FP64 GPU form (parity-tied to the oracle at 1e-12)",
+                fields=field_errors(st[S.PREC_MIXED], st[S.PREC_FP64]))
+
+
+def c1_bench(steps=500, reps=3, cpu=True):
+    """2D channel (256 x 128 jittered quads, slip walls, Dirichlet inlet, pressure outlet, density
+    bump), first-order HLLC + RK2 (reference midpoint), CFL 0.8, global time step: ``steps`` steps
+    device-resident (best of ``reps``), and the CPU oracle over the same steps at full size"""
+    import cases as K
+    from typhon_amd.solver import Solver
+    m, s, g, u = K.channel_case()
+    solver = Solver(m, s, g)
+    run_steps(solver, u, 20)
+    best = np.inf
+    for _ in range(reps):
+        solver.set_state(u)
+        solver.sync()
+        t0 = time.perf_counter()
+        solver.advance(steps)
+        solver.sync()
+        best = min(best, time.perf_counter() - t0)
+    gpu_state = run_steps(solver, u, steps)
+    solver.close()
+    nc = m.ncell_int
+    out = dict(workload=f"2D Euler channel, 256 x 128 = {nc} jittered quads, first-order HLLC + RK2, CFL 0.8, "
+                        f"{steps} steps", cells=nc, steps=steps, value=round(nc * 2 * steps / best / 1e6, 1),
+               unit="Mcell-updates/s", ms_per_step=round(best / steps * 1e3, 4), dtype="f64",
+               us_per_cell_it=round(best / steps / nc * 1e6, 6))
+    if cpu:
+        from oracle import oracle as OR
+        o = K.make_oracle(m, s, g)
+        o.set_cons(u)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.rk_step(o.timestep(s["cfl"]))
+        el = time.perf_counter() - t0
+        threads = int(OR.lib().or_threads())
+        out["cpu_baseline"] = dict(value=round(nc * 2 * steps / el / 1e6, 2), unit="Mcell-updates/s", cores=threads,
+                                   kind="port", sample=f"oracle C restatement, {threads} OpenMP threads, the full "
+                                                       f"workload ({steps} steps, {el:.1f} s)",
+                                   us_per_cell_it_x_ncpu=round(threads * el / steps / nc * 1e6, 4))
+        out["gpu_vs_oracle_after_steps"] = field_errors(gpu_state, o.get_cons())
+    return out
+
+
+def c4_bench(rate_steps=500, nvert=64, log=None):
+    """the adjoint workload (249,480-tet bump channel, first-order HLLC + RK2): primal rate over
+    ``rate_steps`` device-resident steps, and the wall-force gradient check (bump_gradient.check)"""
+    import cases as K
+    import bump_gradient as BG
+    from typhon_amd.solver import Solver
+    m, s, g, u = K.bump_case()
+    solver = Solver(m, s, g)
+    run_steps(solver, u, 10)
+    solver.set_state(u)
+    solver.sync()
+    t0 = time.perf_counter()
+    solver.advance(rate_steps)
+    solver.sync()
+    el = time.perf_counter() - t0
+    solver.close()
+    nc = m.ncell_int
+    out = dict(primal=dict(value=round(nc * 2 * rate_steps / el / 1e6, 1), unit="Mcell-updates/s",
+                           ms_per_step=round(el / rate_steps * 1e3, 4), steps=rate_steps, dtype="f64"))
+    rep = BG.check(nvert=nvert, log=log)
+    rows = rep.pop("rows")
+    rep["table"] = [[r["param"], r["adjoint"], r["fd"], r["rel_diff"]] for r in rows]
+    out["gradient"] = rep
+    return out
+
+
+def ns_cpu_baseline(n=48, budget_s=15.0, max_steps=200):
+    """the oracle on a bounded sample of the 16.8M-hex NS workload: the same scheme on n^3 hexes"""
+    import cases as K
+    from oracle import oracle as OR
+    threads = int(OR.lib().or_threads())
+    m, s, g, u = K.tgv_case(n, walls=True)
+    o = K.make_oracle(m, s, g)
+    o.set_cons(u)
+    dt = o.timestep(s["cfl"])
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.rk_step(dt)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s or steps >= max_steps:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=round(m.ncell_int * 4 * steps / el / 1e6, 3), unit="Mcell-updates/s", cores=threads,
+                kind="port", us_per_cell_it_x_ncpu=round(threads * el / steps / m.ncell_int * 1e6, 4),
+                sample=f"oracle C restatement, {threads} OpenMP threads, same scheme on {n}^3 = {m.ncell_int} hexes, "
+                       f"{steps} RK4 steps ({el:.1f} s)")
+
+
+_LIGHT = "--no-cpu --no-implicit --no-c1 --no-rcm --accuracy-steps 0 --c4-vertices -1"
 PROFILE_CMDS = {
-    "trace": "rocprofv3 --kernel-trace --stats -f csv -d {d}/trace -o c2 -- python3 bench.py --steps 20 --warmup 2 "
-             "--no-cpu --no-implicit --no-ns",
-    "fetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/fetch -o c2 -- python3 bench.py --steps 2 --warmup 1 --no-cpu "
-             "--no-adjoint --no-implicit --no-ns",
-    "write": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/write -o c2 -- python3 bench.py --steps 2 --warmup 1 --no-cpu "
-             "--no-adjoint --no-implicit --no-ns",
-    "ns": "rocprofv3 --kernel-trace --stats -f csv -d {d}/ns -o ns -- python3 bench.py --steps 1 --warmup 1 --no-cpu "
-          "--no-mixed --no-adjoint --no-implicit --ns-steps 20 --ns-adj-steps 4",
+    "trace": "rocprofv3 --kernel-trace --stats -f csv -d {d}/trace -o a -- python3 bench.py --steps 20 --warmup 2 "
+             + _LIGHT + " --ns-steps 20 --ns-adj-steps 4",
+    "fetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/fetch -o a -- python3 bench.py --steps 2 --warmup 1 "
+             + _LIGHT + " --ns-steps 2 --ns-adj-steps 2",
+    "write": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/write -o a -- python3 bench.py --steps 2 --warmup 1 "
+             + _LIGHT + " --ns-steps 2 --ns-adj-steps 2",
+    "sq": "rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_WAVE_CYCLES SQ_BUSY_CYCLES "
+          "SQ_WAIT_INST_ANY SQ_ACTIVE_INST_VALU -f csv -d {d}/sq -o a -- python3 bench.py --steps 2 --warmup 1 "
+          + _LIGHT + " --ns-steps 2 --ns-adj-steps 2",
     "imp": "rocprofv3 --kernel-trace --stats -f csv -d {d}/imp -o imp -- python3 bench.py --steps 1 --warmup 1 "
-           "--no-cpu --no-mixed --no-adjoint --no-ns",
+           "--no-cpu --no-mixed --no-adjoint --no-ns --no-c1 --no-rcm --accuracy-steps 0",
     "cfetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d {d}/cfetch -o copy -- python3 bench.py --copy-calibration 1024",
     "cwrite": "rocprofv3 --pmc WRITE_SIZE -f csv -d {d}/cwrite -o copy -- python3 bench.py --copy-calibration 1024",
 }
@@ -116,97 +258,151 @@ def _csv_rows(path):
         return list(csv.DictReader(f))
 
 
-def _kernel_key(name):
-    """short kernel key from a demangled HIP kernel name: 'k_flux<3, double, ...>' -> ('flux', 'double')"""
-    for key, tag in (("k_flux<", "flux"), ("k_gradient<", "grad"), ("k_dt_local<", "dt")):
-        i = name.find(key)
-        if i >= 0:
-            args = name[i + len(key):].split(">")[0].split(",")
-            return tag, args[1].strip() if len(args) > 1 else ""
-    return None, None
+def _pmc(path, counter):
+    """kernel name -> per-dispatch values of one counter (in the counter's unit)"""
+    out = {}
+    for r in _csv_rows(path):
+        if r["Counter_Name"] == counter:
+            out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return out
 
 
-def summarize_profiles(d, n=100):
-    """Turn rocprofv3 CSVs under d (made by the PROFILE_CMDS commands) into the committed text
-    summaries under profiles/: per-kernel durations, and HBM bytes per launch from FETCH_SIZE /
-    WRITE_SIZE scaled by what the same counters read for a plain copy of known size."""
-    nc = 6 * n ** 3
-    nf = 2 * nc  # periodic Kuhn-split box: every face is shared by two tets
-    bm = byte_model(nc, nf, 0, mixed=False)
-    bmm = byte_model(nc, nf, 0, mixed=True)
-    stats = _csv_rows(os.path.join(d, "trace", "c2_kernel_stats.csv"))
+def _mean(v):
+    return sum(v) / len(v) if v else float("nan")
 
-    def pmc(sub, prefix, counter):
-        rows = _csv_rows(os.path.join(d, sub, f"{prefix}_counter_collection.csv"))
-        out = {}
-        for r in rows:
-            if r["Counter_Name"] != counter:
-                continue
-            out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]) * 1024.0)
-        return out
 
-    cf, cw = pmc("cfetch", "copy", "FETCH_SIZE"), pmc("cwrite", "copy", "WRITE_SIZE")
+def adjoint_byte_model(nc, nf, nb, dim=3, nv=5):
+    """Algorithmic HBM bytes of one adjoint evaluation (lambda -> (dR/dU)^T lambda) of the first-order
+    viscous residual, priced with the same rule as the primal's: what the primal residual reads
+    (gradient pass and flux pass, residual only) plus lambda read and the result written, FP64"""
+    bm = byte_model(nc, nf, nb, dim, nv, mixed=False)
+    return bm["residual_only"] + 2 * nc * nv * 8
+
+
+def summarize_profiles(d, n=100, ns_n=256):
+    """Turn the rocprofv3 CSVs under d (made by the PROFILE_CMDS commands) into the committed
+    summaries under profiles/: per-kernel durations, HBM bytes per launch from FETCH_SIZE (KiB) and
+    WRITE_SIZE (KiB) scaled by what the same counters read for a plain 1 GiB device copy, the
+    algorithmic bytes of each residual kernel and SQ issue counters."""
+    nc2, nf2 = 6 * n ** 3, 12 * n ** 3  # periodic Kuhn-split box: every face shared by two tets
+    ncn = ns_n ** 3
+    nfn, nbn = 3 * ncn - ns_n * ns_n, 2 * ns_n * ns_n  # periodic in x and z, slip walls at the y ends
+    stats = {r["Name"]: r for r in _csv_rows(os.path.join(d, "trace", "a_kernel_stats.csv"))}
     copy_bytes = 1024 << 20
 
-    def copy_factor(tab):
-        vals = [v for k, vs in tab.items() if "copy" in k.lower() for v in vs if v > copy_bytes / 4]
-        return copy_bytes / (sum(vals) / len(vals)) if vals else None
+    def copy_factor(sub, counter):
+        vals = [v * 1024 for k, vs in _pmc(os.path.join(d, sub, "copy_counter_collection.csv"), counter).items()
+                if "opy" in k for v in vs if v * 1024 > copy_bytes / 4]
+        return copy_bytes / _mean(vals)
 
-    f_fac, w_fac = copy_factor(cf), copy_factor(cw)
-    fe, wr = pmc("fetch", "c2", "FETCH_SIZE"), pmc("write", "c2", "WRITE_SIZE")
-    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
-    lines = ["# Residual kernels, 3D Euler MUSCL/LSQ/Venkatakrishnan + HLLC + RK4, 6,000,000 periodic tets (FP64 and FP32)",
-             "", "The trace run also holds the adjoint workload's kernels (first-order HLLC residual and its adjoint on "
-             "the 249,480-tet bump channel, 20 launches each).", "",
-             "Commands (from the repository root, output directory D):", ""]
-    lines += ["    " + c.format(d="D") for c in PROFILE_CMDS.values()]
-    lines += ["    python3 bench.py --summarize D", "",
-              "## Kernel durations (rocprofv3 --kernel-trace --stats)", "",
-              "| kernel | calls | avg us | min us | max us | share % |", "|---|---|---|---|---|---|"]
-    for r in stats:
-        if "ty::" not in r["Name"]:
-            continue
-        lines.append(f"| {r['Name'].replace('|', '/')} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
-                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
-    lines += ["", "## HBM traffic per launch (FP64 and FP32 residual kernels of the 6M-tet workload)", "",
-              f"Copy calibration (1 GiB device copy): FETCH_SIZE factor {f_fac:.3f}, WRITE_SIZE factor {w_fac:.3f} "
-              "(bytes moved / counter bytes).", "",
-              "| kernel | FETCH_SIZE GB (raw) | read GB (scaled) | WRITE_SIZE GB (raw) | written GB (scaled) | "
-              "HBM GB/launch | algorithmic GB/launch | ratio |", "|---|---|---|---|---|---|---|---|"]
+    f_fac, w_fac = copy_factor("cfetch", "FETCH_SIZE"), copy_factor("cwrite", "WRITE_SIZE")
+    fe = _pmc(os.path.join(d, "fetch", "a_counter_collection.csv"), "FETCH_SIZE")
+    wr = _pmc(os.path.join(d, "write", "a_counter_collection.csv"), "WRITE_SIZE")
+    sqf = os.path.join(d, "sq", "a_counter_collection.csv")
+    sq = {c: _pmc(sqf, c) for c in ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_WAVE_CYCLES",
+                                    "SQ_ACTIVE_INST_VALU")} if os.path.exists(sqf) else {}
+    models = {  # kernel name prefix -> (label, algorithmic bytes per launch)
+        "void ty::k_gradient<3, double, 4, 1, true, false>": ("6M tets K1 FP64 (LSQ + Venkatakrishnan, face states)",
+                                                            byte_model(nc2, nf2, 0)["grad"]),
+        "void ty::k_flux<3, double, true, false, 4, false, true>": ("6M tets K2 FP64 (HLLC, gather, RK4)",
+                                                                   byte_model(nc2, nf2, 0)["flux"]),
+        "void ty::k_gradient<3, float, 4, 1, true, false>": ("6M tets K1 FP32",
+                                                           byte_model(nc2, nf2, 0, mixed=True)["grad"]),
+        "void ty::k_flux<3, float, true, false, 4, false, true>": ("6M tets K2 FP32 (FP64 RK accumulation)",
+                                                                  byte_model(nc2, nf2, 0, mixed=True)["flux"]),
+        "void ty::k_gradient<3, double, 6, 0, true, true>": ("16.8M hexes NS K1 FP64 (LSQ, viscous records)",
+                                                           byte_model(ncn, nfn, nbn)["grad"]),
+        "void ty::k_flux<3, double, false, true, 6, false, true>": ("16.8M hexes NS K2 FP64 (HLLC + viscous, RK4)",
+                                                                   byte_model(ncn, nfn, nbn)["flux"]),
+        "void ty::k_gradient<3, float, 6, 0, true, true>": ("16.8M hexes NS K1 FP32",
+                                                          byte_model(ncn, nfn, nbn, mixed=True)["grad"]),
+        "void ty::k_flux<3, float, false, true, 6, false, true>": ("16.8M hexes NS K2 FP32 (FP64 RK accumulation)",
+                                                                  byte_model(ncn, nfn, nbn, mixed=True)["flux"]),
+    }
+    adj_kernels = ("k_adj_face_geom<3>", "k_res_adjoint_face<3, true, false>", "k_res_adjoint_face<3, true, true>",
+                   "k_res_adjoint_bnd_cells<3>", "k_res_adjoint_gather<3, true, 6>", "k_res_adjoint_tail<3>")
+
+    def row(name):
+        st = stats.get(name)
+        us = float(st["AverageNs"]) / 1e3 if st else float("nan")
+        f = _mean(fe.get(name, [])) * 1024 * f_fac
+        w = _mean(wr.get(name, [])) * 1024 * w_fac
+        return us, f, w
+
+    def sq_cols(name):
+        if not sq or name not in sq["SQ_WAVES"]:
+            return "| | |"
+        waves = _mean(sq["SQ_WAVES"][name])
+        valu = _mean(sq["SQ_INSTS_VALU"][name]) / waves
+        vmem = _mean(sq["SQ_INSTS_VMEM_RD"][name]) / waves
+        busy = _mean(sq["SQ_ACTIVE_INST_VALU"][name]) / _mean(sq["SQ_WAVE_CYCLES"][name])
+        return f"| {valu:.0f} / {vmem:.0f} | {100 * busy:.0f}% |"
+
+    head = ["Commands (from the repository root, output directory D):", ""]
+    head += ["    " + c.format(d="D") for c in PROFILE_CMDS.values()]
+    head += ["    python3 bench.py --summarize D", "",
+             f"Copy calibration (1 GiB device copy): FETCH_SIZE factor {f_fac:.3f}, WRITE_SIZE factor {w_fac:.3f} "
+             "(bytes moved / counter bytes).", ""]
+    lines = ["# Residual kernels: durations, HBM traffic and issue counters", "",
+             "6M periodic tets (100^3 hexes x 6; MUSCL + LSQ + Venkatakrishnan, HLLC, RK4) and 16.8M hexes (256^3, "
+             "NS first-order HLLC + cell-full viscous flux, LSQ gradients, RK4), FP64 and FP32 (mixed: FP64 U0 and "
+             "RK accumulator).  Algorithmic bytes: the byte model of bench.py (each array once per pass, int32 "
+             "indices).", ""] + head
+    lines += ["| kernel | avg us | read GB | written GB | HBM GB | HBM TB/s | algorithmic GB | traffic / algorithmic "
+              "| algorithmic TB/s (frac of 8) | VALU / VMEM-read instr per wave | VALU busy |",
+              "|---|---|---|---|---|---|---|---|---|---|---|"]
     traffic = {}
-    for name, vals in fe.items():
-        tag, prec = _kernel_key(name)
-        if tag not in ("flux", "grad") or prec not in ("double", "float"):
+    for prefix, (label, alg) in models.items():
+        name = next((k for k in stats if k.startswith(prefix)), None)
+        if name is None:
             continue
-        f_raw = sum(vals) / len(vals)
-        wv = wr.get(name, [])
-        w_raw = sum(wv) / len(wv) if wv else float("nan")
-        tot = f_raw * f_fac + w_raw * w_fac
-        alg = (bm if prec == "double" else bmm)[tag]
-        key = tag if prec == "double" else f"{tag}_fp32"
-        traffic[key] = dict(kernel=name, fetch_raw=f_raw, write_raw=w_raw, fetch_factor=f_fac, write_factor=w_fac,
-                            bytes_per_launch=int(tot), algorithmic_bytes=int(alg), launches=len(vals))
-        lines.append(f"| {name} | {f_raw / 1e9:.3f} | {f_raw * f_fac / 1e9:.3f} | {w_raw / 1e9:.3f} | "
-                     f"{w_raw * w_fac / 1e9:.3f} | {tot / 1e9:.3f} | {alg / 1e9:.3f} | {tot / alg:.2f} |")
-    with open(os.path.join(ROOT, "profiles", "residual_kernels_c2.md"), "w") as f:
-        f.write("\n".join(lines) + "\n")
-    ns_stats = os.path.join(d, "ns", "ns_kernel_stats.csv")
-    if os.path.exists(ns_stats):
-        nl = ["# NS kernels, 256^3 = 16,777,216 hexes (periodic x/z, slip walls y), first-order HLLC + cell-full "
-              "viscous flux, LSQ gradients, RK4", "",
-              "FP64 and mixed residual kernels (20 RK4 steps each) and the FP64 adjoint kernels of a 4-step "
-              "checkpointed unsteady adjoint (viscous face reverse sweep, gather with the LSQ-gradient transpose, "
-              "tail gather, FP64 stage residuals).  The run also holds one FP64 step of the 6M-tet workload.", "",
-              "Command (from the repository root, output directory D):", "",
-              "    " + PROFILE_CMDS["ns"].format(d="D"), "    python3 bench.py --summarize D", "",
+        us, f, w = row(name)
+        tb = (f + w) / (us * 1e-6) / 1e12
+        at = alg / (us * 1e-6) / 1e12
+        lines.append(f"| {label} | {us:.1f} | {f / 1e9:.3f} | {w / 1e9:.3f} | {(f + w) / 1e9:.3f} | {tb:.2f} | "
+                     f"{alg / 1e9:.3f} | {(f + w) / alg:.2f} | {at:.2f} ({at / 8:.3f}) {sq_cols(name)}")
+        traffic[prefix] = dict(kernel=name, bytes_per_launch=int(f + w), algorithmic_bytes=int(alg),
+                               fetch_factor=f_fac, write_factor=w_fac, avg_us=us)
+    lines += ["", "## All kernels of the trace run (rocprofv3 --kernel-trace --stats)", "",
               "| kernel | calls | avg us | min us | max us | share % |", "|---|---|---|---|---|---|"]
-        for r in _csv_rows(ns_stats):
-            if "ty::" not in r["Name"]:
-                continue
-            nl.append(f"| {r['Name'].replace('|', '/')} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
-                      f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
-        with open(os.path.join(ROOT, "profiles", "ns_kernels_16m.md"), "w") as f:
-            f.write("\n".join(nl) + "\n")
+    for name, r in stats.items():
+        if "ty::" not in name:
+            continue
+        lines.append(f"| {name.replace('|', '/')[:110]} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    with open(os.path.join(ROOT, "profiles", "residual_kernels.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    for prefix, name in ((list(models)[1], "traffic_flux_c2_fp64.json"), (list(models)[0], "traffic_grad_c2_fp64.json"),
+                         (list(models)[5], "traffic_flux_ns16m_fp64.json"),
+                         (list(models)[4], "traffic_grad_ns16m_fp64.json")):
+        if prefix in traffic:
+            with open(os.path.join(ROOT, "profiles", name), "w") as f:
+                json.dump(traffic[prefix], f, indent=1)
+    # adjoint kernels of the 16.8M-hex NS residual
+    alg = adjoint_byte_model(ncn, nfn, nbn)
+    al = ["# Adjoint kernels of the 16.8M-hex NS residual (FP64): durations and HBM traffic", "",
+          "One adjoint evaluation (lambda -> (dR/dU)^T lambda) runs the face kernel (two-sided faces, then boundary "
+          "faces), the boundary-cell kernel, the gather (with the LSQ-gradient transpose) and the tail gather; "
+          "k_adj_face_geom builds the per-face viscous geometry once per work buffer.  Algorithmic bytes of one "
+          "evaluation: the primal residual's reads (gradient and flux passes) plus lambda and the result, "
+          f"{alg / ncn:.0f} B per cell = {alg / 1e9:.2f} GB.", ""] + head
+    al += ["| kernel | avg us | read GB | written GB | HBM TB/s |", "|---|---|---|---|---|"]
+    tot_us = 0.0
+    for k in adj_kernels:
+        name = next((x for x in stats if k in x), None)
+        if name is None:
+            continue
+        us, f, w = row(name)
+        if "geom" not in k:
+            tot_us += us
+        al.append(f"| {name} | {us:.1f} | {f / 1e9:.3f} | {w / 1e9:.3f} | {(f + w) / (us * 1e-6) / 1e12:.2f} |")
+    if tot_us > 0:
+        at = alg / (tot_us * 1e-6) / 1e12
+        al += ["", f"Evaluation (all but the geometry kernel): {tot_us:.0f} us; algorithmic {at:.2f} TB/s, "
+                   f"{at / 8:.3f} of the 8 TB/s peak."]
+    with open(os.path.join(ROOT, "profiles", "adjoint_kernels_ns16m.md"), "w") as f:
+        f.write("\n".join(al) + "\n")
     imp_stats = os.path.join(d, "imp", "imp_kernel_stats.csv")
     if os.path.exists(imp_stats):
         il = ["# Implicit NS kernels, 160^3 = 4,096,000 periodic hexes (Taylor-Green, dt = 10 x explicit)", "",
@@ -224,11 +420,7 @@ def summarize_profiles(d, n=100):
                       f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
         with open(os.path.join(ROOT, "profiles", "implicit_kernels_c3.md"), "w") as f:
             f.write("\n".join(il) + "\n")
-    for key, t in traffic.items():
-        name = f"traffic_{key[:-5]}_c2_fp32.json" if key.endswith("_fp32") else f"traffic_{key}_c2_fp64.json"
-        with open(os.path.join(ROOT, "profiles", name), "w") as f:
-            json.dump(t, f, indent=1)
-    print("\n".join(lines))
+    print("\n".join(lines[:40]))
 
 
 def adjoint_ratio(reps=20):
@@ -306,11 +498,28 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
         solver.sync()
         el = time.perf_counter() - t0
         tm = solver.timing(False)
-        states[name] = solver.get_state()[0]
+        states[name] = solver.get_state()
         out[name] = dict(value=round(nc * 4 * steps / el / 1e6, 1), unit="Mcell-updates/s",
                          ms_per_step=round(el / steps * 1e3, 3),
                          grad_ms_avg=round(tm["grad_ms"] / max(1, tm["grad_n"]), 4),
                          flux_ms_avg=round(tm["flux_ms"] / max(1, tm["flux_n"]), 4))
+        info = solver.mesh_info()
+        bm = byte_model(nc, info["nface"], info["nbface"], mixed=prec == S.PREC_MIXED)
+        for kern in ("grad", "flux"):
+            out[name][f"{kern}_achieved_GBs"] = round(bm[kern] / (out[name][f"{kern}_ms_avg"] * 1e-3) / 1e9, 1)
+        out[name]["bytes_per_cell_eval"] = round(bm["total"] / nc, 1)
+        if name == "fp64":  # the dominant kernel of the FP64 form and its HBM roofline
+            kern = "flux" if out[name]["flux_ms_avg"] >= out[name]["grad_ms_avg"] else "grad"
+            ms_avg = out[name][f"{kern}_ms_avg"]
+            ach = bm[kern] / (ms_avg * 1e-3) / 1e9
+            traffic = load_traffic(os.path.join(ROOT, "profiles", f"traffic_{kern}_ns16m_fp64.json"))
+            out["roofline"] = {"bound": "hbm", "kernel": f"k_{'flux' if kern == 'flux' else 'gradient'}",
+                               "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(ach / HBM_PEAK_GBS, 4),
+                               "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                               "traffic_achieved_GBs": (round(traffic["bytes_per_launch"] / (ms_avg * 1e-3) / 1e9, 1)
+                                                        if traffic else None),
+                               "algorithmic_bytes_per_launch": int(bm[kern]), "avg_launch_ms": ms_avg}
         print(f"# ns {name}: {out[name]}", file=sys.stderr, flush=True)
     # adjoint/primal cost of one NS residual: device time of the FP64 primal kernels (LSQ gradient +
     # flux) vs the adjoint kernel chain (face reverse sweeps, gather with the gradient transpose, tail)
@@ -335,8 +544,8 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
     del lam
     r64, rmx = states.pop("fp64"), states.pop("mixed")
     out["speedup_mixed_vs_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
-    out["density_rel_err_l2"] = float(np.sqrt(np.mean((rmx - r64) ** 2)) / np.sqrt(np.mean(r64 ** 2)))
-    out["density_rel_err_linf"] = float(np.abs(rmx - r64).max() / np.abs(r64).max())
+    out["mixed_error_after_steps"] = dict(steps=steps, reference="the FP64 GPU form",
+                                          fields=field_errors(rmx, r64))
     out["dt"] = dt
     del r64, rmx
     if adj_steps > 0:
@@ -429,6 +638,12 @@ def main():
     ap.add_argument("--ns-n", type=int, default=256, help="hexes per edge of the NS section's box")
     ap.add_argument("--ns-steps", type=int, default=1000)
     ap.add_argument("--ns-adj-steps", type=int, default=200, help="RK steps of the NS adjoint (0: skip)")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps of the FP64-vs-mixed per-field error run of the headline workload (0: skip)")
+    ap.add_argument("--no-rcm", action="store_true", help="skip the reverse Cuthill-McKee order run")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 2D channel workload")
+    ap.add_argument("--c4-vertices", type=int, default=64,
+                    help="wall vertices in the bump-channel gradient check (-1: skip the check)")
     ap.add_argument("--order", default="hilbert", choices=["hilbert", "rcm", "none"],
                     help="device cell order set at mesh upload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -437,7 +652,7 @@ def main():
                     help="write the profiles/ summaries from rocprofv3 output under DIR and exit")
     args = ap.parse_args()
     if args.summarize:
-        summarize_profiles(args.summarize, args.n)
+        summarize_profiles(args.summarize, args.n, args.ns_n)
         return
     if args.copy_calibration:
         import torch
@@ -510,20 +725,40 @@ def main():
     if not args.no_mixed:
         tmx, tmm, stmx, _ = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_MIXED, args.order)
         bmm = byte_model(nc, nf_dev, nb_dev, mixed=True)
-        rho64, rhomx = st64[0], stmx[0]
-        err_l2 = float(np.sqrt(np.mean((rhomx - rho64) ** 2)) / np.sqrt(np.mean(rho64 ** 2)))
-        err_inf = float(np.abs(rhomx - rho64).max() / np.abs(rho64).max())
         fl = tmm["flux_ms"] / max(1, tmm["flux_n"])
         out["mixed"] = {"value": round(nc * evals / tmx / 1e6, 3), "unit": "Mcell-updates/s",
-                        "dtype": "fp32 flux/gradient/stage state, fp64 U0/accumulator",
+                        "dtype": "fp32 flux/gradient/stage state and face states; fp64 U0, RK accumulator, "
+                                 "primitive recovery from total energy",
                         "speedup_vs_fp64": round(t64 / tmx, 3),
-                        "density_rel_err_l2": err_l2, "density_rel_err_linf": err_inf,
-                        "after_steps": args.steps + args.warmup,
+                        "error_after_timed_steps": dict(steps=args.steps + args.warmup,
+                                                        fields=field_errors(stmx, st64)),
                         "flux_ms_avg": round(fl, 4),
                         "grad_ms_avg": round(tmm["grad_ms"] / max(1, tmm["grad_n"]), 4),
                         "flux_achieved_GBs": round(bmm["flux"] / (fl * 1e-3) / 1e9, 1)}
+        print(f"# mixed: {out['mixed']['value']} Mcell-updates/s", file=sys.stderr, flush=True)
+        if args.accuracy_steps > 0:
+            out["mixed"]["error_after_1000_steps" if args.accuracy_steps == 1000 else "error_after_steps"] = \
+                mixed_accuracy(m, s, g, u, args.accuracy_steps, args.order)
+            print("# mixed accuracy done", file=sys.stderr, flush=True)
+    if not args.no_rcm and args.order != "rcm":
+        tr, tmr, _, _ = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_FP64, "rcm")
+        out["rcm_order"] = {"value": round(nc * evals / tr / 1e6, 3), "unit": "Mcell-updates/s",
+                            "grad_ms_avg": round(tmr["grad_ms"] / max(1, tmr["grad_n"]), 4),
+                            "flux_ms_avg": round(tmr["flux_ms"] / max(1, tmr["flux_n"]), 4),
+                            "note": "same FP64 workload with the cells in reverse Cuthill-McKee order"}
+        print(f"# rcm: {out['rcm_order']}", file=sys.stderr, flush=True)
+    del m, u
+    if not args.no_c1:
+        out["c1"] = c1_bench(cpu=not args.no_cpu)
+        print(f"# c1: {out['c1']['value']} Mcell-updates/s", file=sys.stderr, flush=True)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_ratio()
+        if args.c4_vertices >= 0:
+            log = lambda msg: print("# " + msg, file=sys.stderr, flush=True)  # noqa: E731
+            out["c4"] = c4_bench(nvert=args.c4_vertices, log=log)
+            g4 = out["c4"]["gradient"]
+            print(f"# c4: primal {out['c4']['primal']['value']} Mcell-updates/s; gradient {g4['passed']}/{g4['checks']} "
+                  f"within {g4['tol']:g}", file=sys.stderr, flush=True)
     if not args.no_implicit:
         try:
             out["implicit"] = implicit_bench()
@@ -534,6 +769,8 @@ def main():
             out["ns_16m"] = ns_bench(args.ns_n, args.ns_steps, adj_steps=args.ns_adj_steps)
         except Exception as e:  # reported, not fatal
             out["ns_16m"] = {"error": str(e)}
+        if not args.no_cpu and "error" not in out["ns_16m"]:
+            out["ns_16m"]["cpu_baseline"] = ns_cpu_baseline()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
     print(json.dumps(out), flush=True)

@@ -49,6 +49,7 @@ struct ty_ctx {
     std::vector<int> perm, iperm;  // perm[new] = caller interior index
     std::vector<int> nbr, fw;      // [ns][n]
     std::vector<int> link;         // [ns][n] link words of the residual kernels
+    std::vector<unsigned> bnd_bits;  // bit g of word g / 32: a cell of [32 g, 32 g + 32) has a boundary slot
     std::vector<int> fcell;        // [2][nf] owner and other cell of each device face (-1: boundary)
     std::vector<double> fn, fc, fs;  // [3][nf], [3][nf], [nf]
     std::vector<int> fsrc;           // [nf] caller face (0-based) whose geometry device face k carries
@@ -74,6 +75,7 @@ struct ty_ctx {
     int* d_nbr = nullptr;
     int* d_fw = nullptr;
     int* d_link = nullptr;
+    unsigned* d_bnd_bits = nullptr;
     int* d_fcell = nullptr;
     void* d_geo = nullptr;  // one allocation for the geometry of the active precision
     Geo<double> geo_d{};
@@ -173,6 +175,7 @@ void free_state(ty_ctx* c)
     c->d_nbr = nullptr;
     c->d_fw = nullptr;
     c->d_link = nullptr;
+    c->d_bnd_bits = nullptr;
     c->d_fcell = nullptr;
     c->d_geo = nullptr;
     c->d_bboco = nullptr;
@@ -610,6 +613,18 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         for (int d = 0; d < 3; ++d) c->cc[(size_t)d * n + i] = m->centre[(size_t)3 * o + d];
         c->vol[i] = m->volume[o];
     });
+    // groups of 32 cells with a boundary slot: the residual kernels take a path without the boundary
+    // code for the wavefronts whose cells have none
+    c->bnd_bits.assign(((size_t)n + 1023) / 1024, 0u);
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < c->ns; ++k) {
+            const int L = c->link[(size_t)k * n + i];
+            if (L != SLOT_UNUSED && L < 0) {
+                c->bnd_bits[(size_t)i >> 10] |= 1u << ((i >> 5) & 31);
+                break;
+            }
+        }
+    }
     c->bcell.clear();
     for (int k = 0; k < c->nb; ++k) c->bcell.push_back(bfaces[k].a);  // bfaces are sorted by cell
     c->bcell.erase(std::unique(c->bcell.begin(), c->bcell.end()), c->bcell.end());
@@ -617,7 +632,7 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
     int rc;
     if ((rc = upload(c, &c->d_nbr, c->nbr)) || (rc = upload(c, &c->d_fw, c->fw)) ||
         (rc = upload(c, &c->d_bboco, c->bboco)) || (rc = upload(c, &c->d_vol, c->vol)) ||
-        (rc = upload(c, &c->d_link, c->link)) ||
+        (rc = upload(c, &c->d_link, c->link)) || (rc = upload(c, &c->d_bnd_bits, c->bnd_bits)) ||
         (rc = upload(c, &c->d_fcell, c->fcell)) || (rc = upload(c, &c->d_cc, c->cc)) ||
         (rc = upload(c, &c->d_fc, c->fc)) || (rc = upload(c, &c->d_fn, c->fn)) || (rc = upload(c, &c->d_fs, c->fs)) ||
         (rc = upload(c, &c->d_gc, c->gc)) || (rc = upload(c, &c->d_ps, c->ps)) ||
@@ -971,6 +986,7 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     a.nbr = c->d_nbr;
     a.fw = c->d_fw;
     a.link = c->d_link;
+    a.bnd_bits = c->d_bnd_bits;
     a.g = g;
     a.boco = c->d_boco;
     a.vol = c->d_vol;

@@ -98,6 +98,8 @@ struct KArgs {
                      // -1 unused (tangent/adjoint kernels)
     const int* link; // [ns][n] link word of the residual kernels: boundary << 31 | other-side << 30 |
                      // mirror slot << 27 | neighbour cell or boundary face; -1 unused
+    const unsigned* bnd_bits;  // bit g of word g / 32: cells [32 g, 32 g + 32) include one with a boundary
+                               // slot (null: assume every group does)
     Geo<R> g;
     const BocoDev* boco;
     const double* vol;    // [n]

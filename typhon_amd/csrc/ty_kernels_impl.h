@@ -579,10 +579,11 @@ __device__ __forceinline__ void ghost_state(const KArgs<R>& a, int k, int c, int
     }
 }
 
-template <int DIM, typename R>
+// BND false: the caller knows the slot is not a boundary slot (no boundary code is generated)
+template <int DIM, bool BND = true, typename R>
 __device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int L, const R* qc, R* qo, R& ir)
 {
-    if (!link_boundary(L)) {
+    if (!BND || !link_boundary(L)) {
         load_prim<DIM>(a.us, link_index(L), qo, ir);
     } else {
         ghost_state<DIM>(a, k, c, L, qc, qo);
@@ -590,10 +591,10 @@ __device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int L
     }
 }
 
-template <int DIM, typename R>
+template <int DIM, bool BND = true, typename R>
 __device__ __forceinline__ void far_state(const KArgs<R>& a, int k, int c, int L, const R* qc, R* qo)
 {
-    if (!link_boundary(L)) {
+    if (!BND || !link_boundary(L)) {
         load_prim<DIM>(a.us, link_index(L), qo);
     } else {
         ghost_state<DIM>(a, k, c, L, qc, qo);
@@ -654,7 +655,8 @@ __device__ __forceinline__ void side_temp(const Physics& ph, const R* q, R ir, c
 // gradient (calc_flux_viscous_cell.f90:96-109): K1 stores those per cell (dim^2 + dim values
 // instead of nv dim) and, for the ghost side of each boundary face, the gradients of the
 // cell_grad boundary map (bc_grad) with the ghost's temperature gradient.
-template <int DIM, typename R>
+// BND false: the cell has no boundary slot (no ghost records)
+template <int DIM, typename R, bool BND = true>
 __device__ __forceinline__ void write_visc_records(const KArgs<R>& a, int c, const R* qc, const R* gr, R* dst,
                                                    bool ghosts)
 {
@@ -666,7 +668,7 @@ __device__ __forceinline__ void write_visc_records(const KArgs<R>& a, int c, con
     side_temp<DIM>(a.ph, qc, arcp(qc[0]), gr, T, gT);
 #pragma unroll
     for (int e = 0; e < DIM; ++e) go[DIM * DIM + e] = gT[e];
-    if (!ghosts) return;
+    if (!BND || !ghosts) return;
     for (int k = 0; k < a.ns; ++k) {
         const int L = a.link[(size_t)k * a.n + c];
         if (L == SLOT_UNUSED || !link_boundary(L)) continue;
@@ -718,14 +720,14 @@ __device__ __forceinline__ R mono_right(int m, R x1, R x2, R a, R b)
 // MONOTONIC0/2 from the two cell values; MONOTONIC1 of a boundary face with the ghost side's own
 // extrapolation (the hres of the ghost cell with the cell_grad boundary map of its gradient), its
 // interior faces are left to k_mono1, which sees both sides' face states.
-template <int DIM, typename R>
+template <int DIM, bool BND = true, typename R>
 __device__ __forceinline__ void mono_own(const KArgs<R>& a, int k, int c, int L, bool other, const R* qc, const R* qo,
                                          const R* dk, const R* rk, const R* gr, R* q)
 {
     constexpr int NV = DIM + 2;
     const int m = a.ph.mono;
     if (m == 11) {
-        if (!link_boundary(L)) return;
+        if (!BND || !link_boundary(L)) return;
         R nf[DIM], gb[NV * DIM], rg[DIM], x2[NV];
         (void)slot_normal<DIM>(a.g, k, a.n, c, nf);
         bc_grad<DIM>(a.boco[a.g.bboco[link_index(L)]], nf, gr, gb);
@@ -745,12 +747,42 @@ __device__ __forceinline__ void mono_own(const KArgs<R>& a, int k, int c, int L,
 // loops carry no guards and every slot's loads can be issued together.
 // GOUT: the cell gradients (or the viscous records) are written too; the Euler reconstructions
 // launch without it, so they do not carry the viscous-record code's registers.
+// Kernels that take the interior path (a copy of the cell's code without the boundary branches) for
+// wavefronts without boundary slots: measured faster for the FP32 kernels and the plain-gradient
+// and viscous FP64 kernels, slower for the FP64 Euler second-order pair.
+template <typename R, int REC>
+constexpr bool grad_interior_path() { return sizeof(R) == 4 || REC == REC_NONE; }
+template <typename R, bool VISC>
+constexpr bool flux_interior_path() { return sizeof(R) == 4 || VISC; }
+
+// Whether the wavefront of cell c holds a cell with a boundary slot: bits 2w and 2w + 1 (its two
+// 32-cell groups) of the group bits, w = c / 64; uniform over the wavefront.
+__device__ __forceinline__ bool wave_boundary(const unsigned* bits, int c)
+{
+    if (!bits) return true;
+    const int w = __builtin_amdgcn_readfirstlane(c >> 6);
+    return (bits[w >> 4] & (3u << ((2 * w) & 31))) != 0u;
+}
+
+template <int DIM, typename R, int NSM, int REC, bool DENSE, bool GOUT, bool BND>
+__device__ __forceinline__ void grad_cell(const KArgs<R>& a, int c);
+
+// Meshes whose cells use every slot run the wavefronts without boundary slots through a copy of
+// the cell's code without the boundary branches, so that every slot's loads can be issued at once.
 template <int DIM, typename R, int NSM, int REC, bool DENSE, bool GOUT>
 __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
 {
-    constexpr int NV = DIM + 2;
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n || (a.stop && *a.stop)) return;
+    if (!DENSE || !grad_interior_path<R, REC>() || wave_boundary(a.bnd_bits, c))
+        grad_cell<DIM, R, NSM, REC, DENSE, GOUT, true>(a, c);
+    else grad_cell<DIM, R, NSM, REC, DENSE, GOUT, false>(a, c);
+}
+
+template <int DIM, typename R, int NSM, int REC, bool DENSE, bool GOUT, bool BND>
+__device__ __forceinline__ void grad_cell(const KArgs<R>& a, int c)
+{
+    constexpr int NV = DIM + 2;
     const int n = a.n;
     R qc[NV];
     load_prim<DIM>(a.us, c, qc);
@@ -769,7 +801,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
         const int L = a.link[(size_t)k * n + c];
         if (!DENSE && L == SLOT_UNUSED) continue;
         R qo[NV], wk[DIM];
-        far_state<DIM>(a, k, c, L, qc, qo);
+        far_state<DIM, BND>(a, k, c, L, qc, qo);
         slot_vec<DIM>(a.g.sw, k, n, c, wk);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -783,12 +815,12 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
         }
     }
     if (GOUT && a.grad_out) {
-        if (a.vrec) write_visc_records<DIM>(a, c, qc, gr, a.grad_out, true);
+        if (a.vrec) write_visc_records<DIM, R, BND>(a, c, qc, gr, a.grad_out, true);
         else {
             R* go = a.grad_out + (size_t)c * NV * DIM;
 #pragma unroll
             for (int k = 0; k < NV * DIM; ++k) go[k] = gr[k];
-            if (a.vrec_out) write_visc_records<DIM>(a, c, qc, gr, a.vrec_out, false);
+            if (a.vrec_out) write_visc_records<DIM, R, BND>(a, c, qc, gr, a.vrec_out, false);
         }
     }
     if (REC == REC_NONE) return;
@@ -849,7 +881,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
         if (!DENSE && L == SLOT_UNUSED) continue;
         const bool other = link_other(L);
         R qo[NV], dk[DIM], rk[DIM];
-        far_state<DIM>(a, k, c, L, qc, qo);
+        far_state<DIM, BND>(a, k, c, L, qc, qo);
         slot_vec<DIM>(a.g.sd, k, n, c, dk);
         slot_vec<DIM>(a.g.sr, k, n, c, rk);
         R* q = qfk[barth ? k : 0];
@@ -863,7 +895,7 @@ __global__ __launch_bounds__(BLOCK) void k_gradient(KArgs<R> a)
                 ratio[gi] = fmin_(ratio[gi], lr);
             }
         } else {
-            if (REC == REC_MUSCL_MONO) mono_own<DIM>(a, k, c, L, other, qc, qo, dk, rk, gr, q);
+            if (REC == REC_MUSCL_MONO) mono_own<DIM, BND>(a, k, c, L, other, qc, qo, dk, rk, gr, q);
             R* out = a.qf_out + ((size_t)k * n + c) * NV;
 #pragma unroll
             for (int v = 0; v < NV; ++v) out[v] = q[v];
@@ -1181,12 +1213,26 @@ __device__ __forceinline__ void invert_block(double* m, double* inv)
 template <typename R, bool FS, bool VISC, bool JAC>
 constexpr int flux_min_waves() { return (VISC && !FS && !JAC && sizeof(R) == 8) ? 3 : 1; }
 
+template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC, bool DENSE, bool BND>
+__device__ __forceinline__ void flux_cell(const KArgs<R>& a, int c);
+
+// Meshes whose cells use every slot run the wavefronts without boundary slots through a copy of
+// the cell's code without the boundary branches (as k_gradient).
 template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC, bool DENSE>
 __global__ __launch_bounds__(BLOCK, (flux_min_waves<R, FS, VISC, JAC>())) void k_flux(KArgs<R> a)
 {
-    constexpr int NV = DIM + 2;
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n || (a.stop && *a.stop)) return;
+    if (!DENSE || JAC || !flux_interior_path<R, VISC>() || wave_boundary(a.bnd_bits, c))
+        flux_cell<DIM, R, FS, VISC, NSM, JAC, DENSE, true>(a, c);
+    else
+        flux_cell<DIM, R, FS, VISC, NSM, JAC, DENSE, false>(a, c);
+}
+
+template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC, bool DENSE, bool BND>
+__device__ __forceinline__ void flux_cell(const KArgs<R>& a, int c)
+{
+    constexpr int NV = DIM + 2;
     const int n = a.n;
     const R gam = R(a.ph.gamma);
     R qc[NV], irc = R(0);
@@ -1212,13 +1258,13 @@ __global__ __launch_bounds__(BLOCK, (flux_min_waves<R, FS, VISC, JAC>())) void k
         if (!DENSE && k >= a.ns) break;
         const int L = a.link[(size_t)k * n + c];
         if (!DENSE && L == SLOT_UNUSED) continue;
-        const bool bnd = link_boundary(L);
+        const bool bnd = BND && link_boundary(L);
         const bool other = link_other(L);
         R nf[DIM];
         const R A = slot_normal<DIM>(a.g, k, n, c, nf);
         // the far cell's state (first order, viscous flux)
         R qo[NV], iro = R(0);
-        if (!FS || VISC) far_state<DIM>(a, k, c, L, qc, qo, iro);
+        if (!FS || VISC) far_state<DIM, BND>(a, k, c, L, qc, qo, iro);
         R qLf[NV], qRf[NV], irL, irR;
         if (FS) {
             const R* own = a.qf + ((size_t)k * n + c) * NV;

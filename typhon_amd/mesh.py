@@ -278,34 +278,59 @@ def build_ustmesh(dim: int, vtx: np.ndarray, cells: np.ndarray, etype: str,
         ghost = ncell_int + (faces - nface_int)
         c2[faces] = ghost
         btype = spec["type"]
-        inner = c1[faces]
         ipart = None
         if btype == BC_PERIOD:
             pf = spans[spec["partner"]]
             kp = {int(k): int(f) for k, f in zip(pkey[pf], pf)}
             ipart = np.array([kp[int(k)] for k in pkey[faces]], dtype=np.int32)
-            icper = c1[ipart]
-            dfc = ccent_i[icper] - fc[ipart]
-            cvol[ghost] = cvol_i[icper]
-            ccent[ghost] = fc[faces] + dfc
-        elif btype == BC_SYM:
-            gc = ccent_i[inner]
-            n = fn[faces]
-            k = 2.0 * np.einsum("ij,ij->i", fc[faces] - gc, n)
-            cvol[ghost] = cvol_i[inner]
-            ccent[ghost] = gc + k[:, None] * n
-        elif btype in _GHOSTCELL:
-            cvol[ghost] = cvol_i[inner]
-            ccent[ghost] = 2.0 * fc[faces] - ccent_i[inner]
-        else:
-            cvol[ghost] = 0.0
-            ccent[ghost] = fc[faces]
         param = np.zeros(8)
         p = np.asarray(spec.get("param", []), dtype=np.float64)
         param[:p.size] = p
         bocos.append(Boco(spec["family"], btype, faces, ipart, param))
+    _ghost_geometry(ncell_int, nface_int, c1, fn, fc, ccent, cvol, bocos)
     return UstMesh(dim, ncell_int, ncell, nface, nface_int, c1.astype(np.int32), c2.astype(np.int32),
                    fn, fc, fs, ccent, cvol, bocos, vtx, fvtx)
+
+
+def _ghost_geometry(ncell_int, nface_int, c1, fn, fc, ccent, cvol, bocos):
+    """ghost cell centres and volumes, one ghost per boundary face (init_ustboco_ghostcell.f90:46-82,
+    periodic: init_connect_grid.f90:226-231); ccent/cvol hold the interior cells and are completed
+    in place"""
+    for b in bocos:
+        faces = b.iface
+        ghost = ncell_int + (faces - nface_int)
+        inner = c1[faces]
+        if b.type == BC_PERIOD:
+            icper = c1[b.ipart]
+            cvol[ghost] = cvol[icper]
+            ccent[ghost] = fc[faces] + (ccent[icper] - fc[b.ipart])
+        elif b.type == BC_SYM:
+            gc = ccent[inner]
+            n = fn[faces]
+            k = 2.0 * np.einsum("ij,ij->i", fc[faces] - gc, n)
+            cvol[ghost] = cvol[inner]
+            ccent[ghost] = gc + k[:, None] * n
+        elif b.type in _GHOSTCELL:
+            cvol[ghost] = cvol[inner]
+            ccent[ghost] = 2.0 * fc[faces] - ccent[inner]
+        else:
+            cvol[ghost] = 0.0
+            ccent[ghost] = fc[faces]
+
+
+def moved(m: UstMesh, vtx: np.ndarray) -> UstMesh:
+    """The mesh ``m`` with its vertices moved to ``vtx`` (same numbering and connectivity): face,
+    cell and ghost geometry recomputed as build_ustmesh does, each face keeping its orientation
+    (the displacements are assumed small enough not to turn a face over)."""
+    vtx = np.asarray(vtx, dtype=np.float64)
+    fn, fc, fs = _face_geometry(m.dim, vtx, m.fvtx)
+    fn = np.where((np.einsum("ij,ij->i", fn, m.fnorm) < 0.0)[:, None], -fn, fn)
+    ccent = np.zeros((m.ncell, 3))
+    cvol = np.zeros(m.ncell)
+    ccent[:m.ncell_int], cvol[:m.ncell_int] = _cell_geometry(m.dim, m.ncell_int, m.ic1, m.ic2, fn, fc, fs)
+    _ghost_geometry(m.ncell_int, m.nface_int, m.ic1, fn, fc, ccent, cvol, m.bocos)
+    return UstMesh(m.dim, m.ncell_int, m.ncell, m.nface, m.nface_int, m.ic1, m.ic2, fn, fc, fs, ccent, cvol,
+                   m.bocos, vtx, m.fvtx)
 
 
 # ---------------------------------------------------------------------------

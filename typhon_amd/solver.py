@@ -85,6 +85,14 @@ class Solver:
         self._check(self.lib.ty_mesh_upload(self.h, C.byref(d)), "ty_mesh_upload")
         del keep
 
+    def upload(self, mesh, cell_order: str = "hilbert"):
+        """replace the mesh (e.g. the same mesh with moved vertices or other boundary parameters)
+        and set the current scheme again; the state must be set again afterwards"""
+        self._upload_mesh(mesh, self.CELL_ORDER[cell_order])
+        self.mesh = mesh
+        self.ni = mesh.ncell_int
+        self.set_scheme(self.scheme, self.gas)
+
     def set_scheme(self, scheme: dict, gas: dict):
         s = default_scheme(**scheme)
         g = default_gas(**gas)
