@@ -10,10 +10,13 @@ struct AdArgs {
     int nf, nb;           // device faces, device boundary faces
     const int* fcell;     // [2][nf] owner and other cell of each device face (other = -1 on boundary faces)
     int nfi;              // two-sided faces come first: device faces [0, nfi)
-    double* face;         // [ns][n][RW] work: per face side, at the slot where that side's cell keeps
-                          // the face, the cell's state adjoint, (viscous) the adjoint of its velocity
-                          // gradient and of its temperature gradient, and the side's share of its
-                          // volume sensitivity; RW = nv (+ dim^2 + dim) + 1
+    double* face;         // [ns][n][RS] work: per face side, at the slot where that side's cell keeps
+                          // the face, the cell's state adjoint and (with volume sensitivities) the
+                          // side's share of its volume sensitivity; RS = nv (+ 1)
+    double* fgrad;        // [ns][n][dim^2 + dim] work (viscous): at the owner's slot of a two-sided
+                          // face, the adjoint of the face's averaged velocity gradient and temperature
+                          // gradient (each side's gradient adjoint is its distance weight times it); at
+                          // a boundary face's slot, the interior cell's velocity-gradient adjoint
     const unsigned char* fslot;  // [2][nf] slot of each device face in its owner's / other cell's list
     const double* qrec;   // [n][nv] primitive records (rho, p, u..) of k.us (the HLLC face kernels)
     double* lrec;         // [n][nv] lambda dt / V per cell, one record per cell (HLLC face kernels),

@@ -463,7 +463,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_face_geom(AdArgs t)
 template <int DIM, bool VISC, bool BND>
 __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs t)
 {
-    constexpr int NV = DIM + 2, NG = NV * DIM, NC = DIM * DIM + DIM, RW = NV + (VISC ? NC : 0) + 1;
+    constexpr int NV = DIM + 2, NG = NV * DIM, NC = DIM * DIM + DIM, RS = NV + 1;
     const KArgs<double>& a = t.k;
     // two launches: two-sided faces [0, nfi), boundary faces [nfi, nf) (BND)
     const int f = (BND ? t.nfi : 0) + (int)(blockIdx.x * BLOCK + threadIdx.x);
@@ -495,12 +495,12 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
         load_q<DIM>(t, cr, qR);
     }
     hllc_rev<DIM>(a.ph.sch_hyp, a.ph.gamma, qL, qR, nv, mv, Fv, qLb, qRb, gn);
-    // side records at the slot each side keeps the face in: [slot][cell][rw]; the volume share
+    // side records at the slot each side keeps the face in: [slot][cell][rs]; the volume share
     // (last entry) only when volume sensitivities are asked for
     const bool vol = t.gvol != nullptr;
-    const int rw = vol ? RW : RW - 1;
-    double* out = t.face + ((size_t)t.fslot[f] * n + cl) * rw;
-    double* oto = bnd ? out : t.face + ((size_t)t.fslot[(size_t)t.nf + f] * n + cr) * rw;
+    const int rs = vol ? RS : RS - 1;
+    double* out = t.face + ((size_t)t.fslot[f] * n + cl) * rs;
+    double* oto = bnd ? out : t.face + ((size_t)t.fslot[(size_t)t.nf + f] * n + cr) * rs;
     // gradients: full records on boundary faces (the ghost gradient map acts on them), viscous
     // records on two-sided faces
     constexpr int NB = VISC ? (BND ? NG : NC) : 1;
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
         const double dHL = t.fgeo[f];
 #pragma unroll
         for (int e = 0; e < DIM; ++e) dk[e] = t.fgeo[(size_t)(1 + e) * t.nf + f];
-        viscous_rev<DIM, !BND>(a.ph, qL, qR, dHL, dk, nv, gL, gR, mv, Fv, qLb, qRb, gLb, gRb);
+        viscous_rev<DIM, !BND, !BND>(a.ph, qL, qR, dHL, dk, nv, gL, gR, mv, Fv, qLb, qRb, gLb, gRb);
         if (bnd) {
             // gLb += (dB_g/dgL)^T gRb: B_g copies the gradient, or reflects the scalar gradients
             // by S = I - 2 n n^T and the velocity gradient as S G S; both maps are self-adjoint
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
         double sL = 0.0;
 #pragma unroll
         for (int v = 0; v < NV; ++v) sL += lL[v] * Fv[v];
-        out[RW - 1] = -A * sL;
+        out[RS - 1] = -A * sL;
     }
     if (bnd) {
         const int bi = f - t.nfi;
@@ -575,15 +575,11 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
         prim_rev_q<DIM>(qR, g1d, qRb, gRc);
 #pragma unroll
         for (int v = 0; v < NV; ++v) oto[v] = gRc[v];
-        if (VISC) {
-#pragma unroll
-            for (int q = 0; q < NB; ++q) oto[NV + q] = gRb[q];
-        }
         if (vol) {
             double sR = 0.0;
 #pragma unroll
             for (int v = 0; v < NV; ++v) sR += lR[v] * Fv[v];
-            oto[RW - 1] = A * sR;
+            oto[RS - 1] = A * sR;
         }
     }
     double gLc[NV];
@@ -593,14 +589,15 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
 #pragma unroll
     for (int v = 0; v < NV; ++v) out[v] = gLc[v];
     if (VISC) {
+        double* og = t.fgrad + ((size_t)t.fslot[f] * n + cl) * NC;
         if (BND) {
 #pragma unroll
-            for (int q = 0; q < DIM * DIM; ++q) out[NV + q] = gLb[2 * DIM + q];
+            for (int q = 0; q < DIM * DIM; ++q) og[q] = gLb[2 * DIM + q];
 #pragma unroll
-            for (int e = 0; e < DIM; ++e) out[NV + DIM * DIM + e] = 0.0;
+            for (int e = 0; e < DIM; ++e) og[DIM * DIM + e] = 0.0;
         } else {
 #pragma unroll
-            for (int q = 0; q < NC; ++q) out[NV + q] = gLb[q];
+            for (int q = 0; q < NC; ++q) og[q] = gLb[q];
         }
     }
     if (!VISC && t.gs) face_sens<DIM>(t, f, nv, A, mv, Fv, gn);
@@ -617,13 +614,13 @@ __global__ __launch_bounds__(BLOCK, BND ? 1 : 2) void k_res_adjoint_face(AdArgs 
 template <int DIM, bool VISC, int NSM>
 __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
 {
-    constexpr int NV = DIM + 2, NG = NV * DIM, NC = DIM * DIM + DIM, RW = NV + (VISC ? NC : 0) + 1;
+    constexpr int NV = DIM + 2, NG = NV * DIM, NC = DIM * DIM + DIM, RS = NV + 1;
     const KArgs<double>& a = t.k;
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n) return;
     const int n = a.n;
     const bool vol = t.gvol != nullptr;
-    const int rw = vol ? RW : RW - 1;
+    const int rs = vol ? RS : RS - 1;
     double gu[NV], sv = 0.0, gc[VISC ? NC : 1], gx[VISC ? 2 * DIM : 1];
 #pragma unroll
     for (int v = 0; v < NV; ++v) gu[v] = 0.0;
@@ -639,19 +636,29 @@ __global__ __launch_bounds__(BLOCK) void k_res_adjoint_gather(AdArgs t)
         if (k >= a.ns) break;
         const int L = a.link[(size_t)k * n + c];
         if (L == SLOT_UNUSED) continue;
-        const double* in = t.face + ((size_t)k * n + c) * rw;
+        const double* in = t.face + ((size_t)k * n + c) * rs;
 #pragma unroll
         for (int v = 0; v < NV; ++v) gu[v] += in[v];
         if (VISC) {
-#pragma unroll
-            for (int q = 0; q < NC; ++q) gc[q] += in[NV + q];
             if (link_boundary(L)) {
+                const double* fg = t.fgrad + ((size_t)k * n + c) * NC;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) gc[q] += fg[q];
                 const double* bx = t.gbx + (size_t)link_index(L) * 2 * DIM;
 #pragma unroll
                 for (int q = 0; q < 2 * DIM; ++q) gx[q] += bx[q];
+            } else {
+                // the face's gradient adjoint, kept at its owner's slot, times this side's weight
+                // (dHR = 1 - dHL on the owner's side, dHL on the other's)
+                const bool oth = link_other(L);
+                const double dHL = a.g.sh[(size_t)k * n + c];
+                const double w = oth ? dHL : 1.0 - dHL;
+                const double* fg = t.fgrad + (oth ? (size_t)link_mirror(L) * n + link_index(L) : (size_t)k * n + c) * NC;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) gc[q] += w * fg[q];
             }
         }
-        if (vol) sv += in[RW - 1];
+        if (vol) sv += in[RS - 1];
     }
     if (VISC) {
         const double g1d = a.ph.gamma - 1.0, r = a.ph.rgas;

@@ -2007,17 +2007,18 @@ int ad_prepare(ty_ctx* c, double dt, AdArgs& t, DevTmp& tmp)
 int alloc_adjoint_work(ty_ctx* c, AdArgs& t, DevTmp& tmp)
 {
     const int n = c->n, nv = c->nv, dim = c->dim;
-    const int rw = nv + (c->ph.viscous ? dim * dim + dim : 0) + 1;
     int rc;
     double *d_face = nullptr, *d_lrec = nullptr;
-    if ((rc = tmp.alloc(&d_face, (size_t)c->ns * n * rw)) || (rc = tmp.alloc(&d_lrec, (size_t)nv * n))) return rc;
+    if ((rc = tmp.alloc(&d_face, (size_t)c->ns * n * (nv + 1))) || (rc = tmp.alloc(&d_lrec, (size_t)nv * n))) return rc;
     t.face = d_face;
     t.lrec = d_lrec;
     if (c->ph.viscous) {
-        double *d_fg = nullptr, *d_sl = nullptr, *d_gx = nullptr;
+        double *d_fg = nullptr, *d_sl = nullptr, *d_gx = nullptr, *d_fgr = nullptr;
         if ((rc = tmp.alloc(&d_fg, (size_t)4 * c->nf)) || (rc = tmp.alloc(&d_sl, (size_t)c->ns * n * nv)) ||
-            (rc = tmp.alloc(&d_gx, (size_t)2 * dim * std::max(1, c->nb))))
+            (rc = tmp.alloc(&d_gx, (size_t)2 * dim * std::max(1, c->nb))) ||
+            (rc = tmp.alloc(&d_fgr, (size_t)c->ns * n * (dim * dim + dim))))
             return rc;
+        t.fgrad = d_fgr;
         t.fgeo = d_fg;
         t.slot = d_sl;
         t.gbx = d_gx;

@@ -830,6 +830,7 @@ __device__ __forceinline__ void grad_cell(const KArgs<R>& a, int c)
         // minimum is tracked as a fraction by cross-multiplication and divided once.
         const R eps2 = a.g.eps2[c];
         R pn[NV], pd[NV];
+        R d2k[NSM][NV];  // grad(q_v).(x_f - x_c) of each slot, kept for the face states
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             pn[v] = R(1);
@@ -844,6 +845,7 @@ __device__ __forceinline__ void grad_cell(const KArgs<R>& a, int c)
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const R d2 = dotd<DIM>(gr + v * DIM, rk);
+                d2k[k][v] = d2;
                 if (d2 == R(0)) continue;
                 const R dm = (d2 > R(0)) ? qmax[v] - qc[v] : qmin[v] - qc[v];
                 const R num = (dm * dm + eps2) + R(2) * d2 * dm;
@@ -861,11 +863,9 @@ __device__ __forceinline__ void grad_cell(const KArgs<R>& a, int c)
         for (int k = 0; k < NSM; ++k) {
             if (!DENSE && k >= a.ns) break;
             if (!DENSE && a.link[(size_t)k * n + c] == SLOT_UNUSED) continue;
-            R rk[DIM];
-            slot_vec<DIM>(a.g.sr, k, n, c, rk);
             R* out = a.qf_out + ((size_t)k * n + c) * NV;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) out[v] = qc[v] + phi[v] * dotd<DIM>(gr + v * DIM, rk);
+            for (int v = 0; v < NV; ++v) out[v] = qc[v] + phi[v] * d2k[k][v];
         }
         return;
     }

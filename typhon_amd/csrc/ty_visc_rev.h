@@ -15,7 +15,10 @@
 
 namespace ty {
 
-template <int DIM, bool COMPACT = false>
+// SHARED (with COMPACT): gLb receives the adjoint of the face's averaged velocity gradient G0 and
+// temperature gradient (assigned, not weighted) and gRb is left alone: each side's adjoint is its
+// distance weight (dHR for L, dHL for R) times it.
+template <int DIM, bool COMPACT = false, bool SHARED = false>
 __device__ __forceinline__ void viscous_rev(const Physics& ph, const double* qL, const double* qR, double dHL,
                                             const double* vLR, const double* nf, const double* gL,
                                             const double* gR, const double* Fb, double* F, double* qLb, double* qRb,
@@ -159,15 +162,21 @@ __device__ __forceinline__ void viscous_rev(const Physics& ph, const double* qL,
     for (int i = 0; i < DIM; ++i)
 #pragma unroll
         for (int j = 0; j < DIM; ++j) {
-            gRb[GU + i * DIM + j] += dHL * Gb[i][j];
-            gLb[GU + i * DIM + j] += dHR * Gb[i][j];
+            if (SHARED) {
+                gLb[GU + i * DIM + j] = Gb[i][j];
+            } else {
+                gRb[GU + i * DIM + j] += dHL * Gb[i][j];
+                gLb[GU + i * DIM + j] += dHR * Gb[i][j];
+            }
         }
     // gavg = dHL gTR + dHR gTL; gT = k1 grad p - k2 grad rho
     double kl1_b = 0.0, kl2_b = 0.0, kr1_b = 0.0, kr2_b = 0.0;
 #pragma unroll
     for (int e = 0; e < DIM; ++e) {
         const double tLb = dHR * gavg_b[e], tRb = dHL * gavg_b[e];
-        if (COMPACT) {
+        if (SHARED) {
+            gLb[DIM * DIM + e] = gavg_b[e];
+        } else if (COMPACT) {
             gLb[DIM * DIM + e] += tLb;
             gRb[DIM * DIM + e] += tRb;
         } else {
