@@ -20,6 +20,7 @@
 #include <limits>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/typhon_gpu.h"
@@ -701,6 +702,16 @@ int build_geo(ty_ctx* c, Geo<R>& g)
     g.eps2 = dp + o_eps;
     g.sh = dp + o_sh;
     g.bboco = c->d_bboco;
+    g.pa = g.pb = g.pw = nullptr;
+    if constexpr (std::is_same<R, float>::value) {
+        // FP32: the packed slot records of the viscous kernels
+        float4* pk = nullptr;
+        if ((rc = dalloc(c, &pk, (size_t)3 * ns * n))) return rc;
+        HIPCHK(c, launch_geo_pack(n, ns, g, c->d_link, pk, c->stream));
+        g.pa = pk;
+        g.pb = pk + (size_t)ns * n;
+        g.pw = pk + (size_t)2 * ns * n;
+    }
     return TY_OK;
 }
 

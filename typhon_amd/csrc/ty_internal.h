@@ -68,6 +68,12 @@ struct Geo {
     const R* eps2;                  // [n] Venkatakrishnan (K h)^3
     const R* sh;                    // [ns][n] viscous face weight dHL = |x_f - x_L| / (|x_f - x_L| + |x_f - x_R|)
                                     // (canonical sides; the other side's weight is 1 - dHL)
+    // FP32 geometry only (null in FP64): the same data packed per slot, [ns][n] float4, one load per
+    // record in the FP32 viscous kernels: pa = (S, dHL), pb = (x_R - x_L, link word),
+    // pw = (LSQ weight, link word); the link word's bits are stored in the fourth float
+    const float4* pa;
+    const float4* pb;
+    const float4* pw;
 };
 
 struct Physics {
@@ -222,6 +228,8 @@ struct GeoSrc {
 // writes ss | sw | sr | sd ([ns][3][n] each) | fn ([3][nf]) | fs ([nf]) | eps2 ([n]) | sh ([ns][n]) in precision R
 template <typename R>
 hipError_t launch_geo(const GeoSrc& s, R* out, hipStream_t st);
+// the FP32 packed slot records of Geo<float> (pa, pb, pw: [ns][n] each in `out`) from its SoA arrays
+hipError_t launch_geo_pack(int n, int ns, const Geo<float>& g, const int* link, float4* out, hipStream_t st);
 // stable ascending device sort of n host keys over bits [0, end_bit): idx_out (host) gets the
 // input position of each key in sorted order (ty_setup.hip)
 hipError_t sort_keys_device(const uint64_t* keys, int n, int end_bit, int* idx_out, hipStream_t st);

@@ -515,6 +515,14 @@ __device__ __forceinline__ void slot_vec(const R* __restrict__ p, int k, int n, 
     for (int e = 0; e < DIM; ++e) x[e] = p[((size_t)k * 3 + e) * n + c];
 }
 
+// slot_vec of a stream read once per launch, non-temporal (it need not displace gathered data)
+template <int DIM, typename R>
+__device__ __forceinline__ void slot_vec_nt(const R* __restrict__ p, int k, int n, int c, R* x)
+{
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) x[e] = __builtin_nontemporal_load(p + ((size_t)k * 3 + e) * n + c);
+}
+
 template <int DIM, typename R>
 __device__ __forceinline__ void face_n(const Geo<R>& g, int f, R* n)
 {
@@ -755,6 +763,14 @@ constexpr bool grad_interior_path() { return sizeof(R) == 4 || REC == REC_NONE; 
 template <typename R, bool VISC>
 constexpr bool flux_interior_path() { return sizeof(R) == 4 || VISC; }
 
+// a streamed record read once (non-temporal: it need not displace gathered data in the caches)
+__device__ __forceinline__ float4 nt_load4(const float4* p)
+{
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Whether the wavefront of cell c holds a cell with a boundary slot: bits 2w and 2w + 1 (its two
 // 32-cell groups) of the group bits, w = c / 64; uniform over the wavefront.
 __device__ __forceinline__ bool wave_boundary(const unsigned* bits, int c)
@@ -795,14 +811,26 @@ __device__ __forceinline__ void grad_cell(const KArgs<R>& a, int c)
 #pragma unroll
         for (int e = 0; e < DIM; ++e) gr[v * DIM + e] = R(0);
     }
+    // FP32 interior path: the link word and the LSQ weight in one packed record (pw)
+    constexpr bool PW = DENSE && !BND && sizeof(R) == 4;
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
         if (!DENSE && k >= a.ns) break;
-        const int L = a.link[(size_t)k * n + c];
+        float4 pw;
+        if (PW) pw = nt_load4(a.g.pw + (size_t)k * n + c);
+        const int L = PW ? __float_as_int(pw.w) : a.link[(size_t)k * n + c];
         if (!DENSE && L == SLOT_UNUSED) continue;
         R qo[NV], wk[DIM];
         far_state<DIM, BND>(a, k, c, L, qc, qo);
-        slot_vec<DIM>(a.g.sw, k, n, c, wk);
+        if (PW) {
+            const R w3[3] = {R(pw.x), R(pw.y), R(pw.z)};
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) wk[e] = w3[e];
+        } else if (DENSE) {
+            slot_vec_nt<DIM>(a.g.sw, k, n, c, wk);
+        } else {
+            slot_vec<DIM>(a.g.sw, k, n, c, wk);
+        }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const R dq = qo[v] - qc[v];
@@ -841,7 +869,8 @@ __device__ __forceinline__ void grad_cell(const KArgs<R>& a, int c)
             if (!DENSE && k >= a.ns) break;
             if (!DENSE && a.link[(size_t)k * n + c] == SLOT_UNUSED) continue;
             R rk[DIM];
-            slot_vec<DIM>(a.g.sr, k, n, c, rk);
+            if (DENSE) slot_vec_nt<DIM>(a.g.sr, k, n, c, rk);
+            else slot_vec<DIM>(a.g.sr, k, n, c, rk);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const R d2 = dotd<DIM>(gr + v * DIM, rk);
@@ -1253,15 +1282,38 @@ __device__ __forceinline__ void flux_cell(const KArgs<R>& a, int c)
     R res[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) res[v] = R(0);
+    // FP32 interior path of the first-order viscous flux: per slot one packed record pb (centre
+    // difference, link word) and one pa (area vector, distance weight) in place of six scalar loads
+    constexpr bool PF = !FS && VISC && !JAC && DENSE && !BND && sizeof(R) == 4;
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
         if (!DENSE && k >= a.ns) break;
-        const int L = a.link[(size_t)k * n + c];
+        float4 pb;
+        if (PF) pb = nt_load4(a.g.pb + (size_t)k * n + c);
+        const int L = PF ? __float_as_int(pb.w) : a.link[(size_t)k * n + c];
         if (!DENSE && L == SLOT_UNUSED) continue;
         const bool bnd = BND && link_boundary(L);
         const bool other = link_other(L);
         R nf[DIM];
-        const R A = slot_normal<DIM>(a.g, k, n, c, nf);
+        R A, hpk = R(0);
+        if (PF) {
+            const float4 pa = nt_load4(a.g.pa + (size_t)k * n + c);
+            const R S[3] = {R(pa.x), R(pa.y), R(pa.z)};
+            A = asqrt(dotd<DIM>(S, S));
+            const R ia = arcp(A);
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) nf[e] = S[e] * ia;
+            hpk = R(pa.w);
+        } else if (VISC && DENSE) {
+            R S[DIM];
+            slot_vec_nt<DIM>(a.g.ss, k, n, c, S);
+            A = asqrt(dotd<DIM>(S, S));
+            const R ia = arcp(A);
+#pragma unroll
+            for (int e = 0; e < DIM; ++e) nf[e] = S[e] * ia;
+        } else {
+            A = slot_normal<DIM>(a.g, k, n, c, nf);
+        }
         // the far cell's state (first order, viscous flux)
         R qo[NV], iro = R(0);
         if (!FS || VISC) far_state<DIM, BND>(a, k, c, L, qc, qo, iro);
@@ -1302,9 +1354,19 @@ __device__ __forceinline__ void flux_cell(const KArgs<R>& a, int c)
         num_flux_pre<DIM>(a.ph.sch_hyp, gam, R(a.ph.ig1), R(a.ph.gig1), qLf, qRf, irL, irR, nf, flux);
         R dk[DIM], dHL = R(0);
         if (VISC) {
-            slot_vec<DIM>(a.g.sd, k, n, c, dk);
-            dHL = a.g.sh[(size_t)k * n + c];
             R go[VR];
+            if (PF) {
+                const R d3[3] = {R(pb.x), R(pb.y), R(pb.z)};
+#pragma unroll
+                for (int e = 0; e < DIM; ++e) dk[e] = d3[e];
+                dHL = hpk;
+            } else if (DENSE) {
+                slot_vec_nt<DIM>(a.g.sd, k, n, c, dk);
+                dHL = __builtin_nontemporal_load(a.g.sh + (size_t)k * n + c);
+            } else {
+                slot_vec<DIM>(a.g.sd, k, n, c, dk);
+                dHL = a.g.sh[(size_t)k * n + c];
+            }
             load_rec<VR>(bnd ? a.gbnd : a.grad, link_index(L), go);
             const R To = qo[1] * (iro * R(a.ph.irgas));
             // ns_bocoflux (ns_bocoflux.f90:47-111): on flux and adiabatic walls the imposed heat

@@ -146,6 +146,30 @@ hipError_t launch_geo(const GeoSrc& s, R* out, hipStream_t st)
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(BLOCK) void k_geo_pack(int n, int ns, Geo<float> g, const int* __restrict__ link,
+                                                   float4* __restrict__ out)
+{
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= n) return;
+    float4* pa = out;
+    float4* pb = out + (size_t)ns * n;
+    float4* pw = out + (size_t)2 * ns * n;
+    for (int k = 0; k < ns; ++k) {
+        const size_t i = (size_t)k * n + c;
+        const size_t v = (size_t)k * 3 * n + c;
+        const float lb = __int_as_float(link[i]);
+        pa[i] = make_float4(g.ss[v], g.ss[v + n], g.ss[v + 2 * (size_t)n], g.sh[i]);
+        pb[i] = make_float4(g.sd[v], g.sd[v + n], g.sd[v + 2 * (size_t)n], lb);
+        pw[i] = make_float4(g.sw[v], g.sw[v + n], g.sw[v + 2 * (size_t)n], lb);
+    }
+}
+
+hipError_t launch_geo_pack(int n, int ns, const Geo<float>& g, const int* link, float4* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_geo_pack, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, n, ns, g, link, out);
+    return hipGetLastError();
+}
+
 template hipError_t launch_geo<float>(const GeoSrc&, float*, hipStream_t);
 template hipError_t launch_geo<double>(const GeoSrc&, double*, hipStream_t);
 
