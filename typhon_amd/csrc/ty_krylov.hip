@@ -64,6 +64,10 @@ __global__ void k_block_apply(const T* __restrict__ dinv, const T* __restrict__ 
 // partial[b][i] = sum over block b of w . V_i, i < m (m <= MAXK): vectors in groups of MG held
 // in registers (compile-time indexed), each group one grid-stride sweep that re-reads w
 constexpr int MG = 8;
+constexpr int CH = 8;  // basis vectors loaded together by k_cgs_pass and k_lincomb
+// the fixed grid of the multi-dot and the fused CGS dot pass (the same grid, so their partial sums
+// agree bit for bit); 2048 and 4096 blocks measured no faster
+constexpr long MD_BLOCKS = 1024;
 template <typename T>
 __global__ __launch_bounds__(KB) void k_multidot(const T* __restrict__ w, const T* __restrict__ V, long stride,
                                                  int m, long len, double* __restrict__ partial)
@@ -78,9 +82,12 @@ __global__ __launch_bounds__(KB) void k_multidot(const T* __restrict__ w, const 
         for (int i = 0; i < MG; ++i) acc[i] = 0.0;
         for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long)gridDim.x * blockDim.x) {
             const double we = (double)w[e];
+            T vc[MG];  // loads issued together, indices clamped to the group's last vector
+#pragma unroll
+            for (int i = 0; i < MG; ++i) vc[i] = Vg[(long)min(i, mg - 1) * stride + e];
 #pragma unroll
             for (int i = 0; i < MG; ++i)
-                if (i < mg) acc[i] += we * (double)Vg[(long)i * stride + e];
+                if (i < mg) acc[i] += we * (double)vc[i];
         }
 #pragma unroll
         for (int i = 0; i < MG; ++i) {
@@ -139,11 +146,20 @@ __global__ __launch_bounds__(KB) void k_cgs_pass(T* __restrict__ w, const T* __r
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long)gridDim.x * blockDim.x) {
         T vi[MAXK];
         double s = (double)w[e];
+        // the basis in chunks of CH vectors: each chunk's loads issued together (indices clamped to
+        // the last vector, so no load waits behind a branch), applied in vector order
 #pragma unroll
-        for (int i = 0; i < MAXK; ++i) {
-            if (i < m) {
-                vi[i] = V[(long)i * stride + e];
-                s -= hr[i] * (double)vi[i];
+        for (int c0 = 0; c0 < MAXK; c0 += CH) {
+            if (c0 >= m) break;
+            T vc[CH];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) vc[j] = V[(long)min(c0 + j, m - 1) * stride + e];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                if (c0 + j < m) {
+                    vi[c0 + j] = vc[j];
+                    s -= hr[c0 + j] * (double)vc[j];
+                }
             }
         }
         const T ws = (T)s;
@@ -181,7 +197,15 @@ __global__ void k_lincomb(T* __restrict__ out, const T* __restrict__ V, long str
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= len) return;
     double s = 0.0;
-    for (int i = 0; i < m; ++i) s += y[i] * (double)V[(long)i * stride + e];
+    // in chunks of CH vectors whose loads are issued together (indices clamped), summed in order
+    for (int c0 = 0; c0 < m; c0 += CH) {
+        T vc[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) vc[j] = V[(long)min(c0 + j, m - 1) * stride + e];
+#pragma unroll
+        for (int j = 0; j < CH; ++j)
+            if (c0 + j < m) s += y[c0 + j] * (double)vc[j];
+    }
     out[e] = (T)s;
 }
 
@@ -214,7 +238,7 @@ static int nb_of(long len) { return (int)((len + KB - 1) / KB); }
 int multidot_blocks(long len)
 {
     long nb = (len + KB - 1) / KB;
-    return (int)(nb < 1024 ? nb : 1024);
+    return (int)(nb < MD_BLOCKS ? nb : MD_BLOCKS);
 }
 
 // the update + norm sweep is light on registers: a grid large enough for full occupancy
