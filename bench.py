@@ -452,6 +452,33 @@ def adjoint_ratio(reps=20):
                 adjoint_kernel_ms=round(adj, 4), ratio=round(adj / prim, 3))
 
 
+def mesh_build_bench(m, phases, sample_n=64):
+    """Mesh preprocessing (faces, face and cell geometry: create_face_connect + calc_meshgeom) of the
+    NS box ``m`` built on the device, beside the host numpy build of the same box at sample_n^3
+    hexes (the device build of that sample is checked equal to it, bit for bit)."""
+    import cases as K
+    from typhon_amd import mesh as M
+    nc = m.ncell_int
+    dev_s = phases["faces_and_geometry"]
+    kern_s = phases.get("device_faces", 0.0) + phases.get("device_finish", 0.0)
+    t0 = time.perf_counter()
+    hs = K.tgv_case(sample_n, walls=True)[0]
+    th = M.LAST_BUILD["faces_and_geometry"]
+    t_case = time.perf_counter() - t0
+    ds = K.tgv_case(sample_n, walls=True, device=0)[0]
+    same = all(np.array_equal(getattr(hs, k), getattr(ds, k)) for k in
+               ("ic1", "ic2", "fvtx", "fnorm", "fcent", "fsurf", "ccent", "cvol"))
+    return dict(workload=f"faces + face/cell geometry of the {round(nc ** (1 / 3))}^3-hex box ({nc} cells, "
+                         f"{m.nface} faces)",
+                device_s=round(dev_s, 2), device_Mcells_per_s=round(nc / dev_s / 1e6, 2),
+                device_calls_s=round(kern_s, 2),
+                device_calls_note="ty_meshbuild_faces + ty_meshbuild_finish, host-device copies included",
+                host_numpy=dict(sample=f"{sample_n}^3 hexes", s=round(th, 2),
+                                Mcells_per_s=round(sample_n ** 3 / th / 1e6, 3), case_s=round(t_case, 2)),
+                speedup_vs_host=round((nc / dev_s) / (sample_n ** 3 / th), 1),
+                device_equals_host_on_sample=bool(same))
+
+
 def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
     """The 16M-cell unsteady NS workload: compressible Taylor-Green field on n^3 hexes (Re 1e4,
     Sutherland, Pr 0.72, 1% seeded noise), periodic in x and z with slip walls at y = 0 and
@@ -468,16 +495,19 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
     from typhon_amd import adjoint as A
     from typhon_amd import scheme as S
     from typhon_amd.solver import Solver
+    from typhon_amd import mesh as M
     t0 = time.perf_counter()
-    m, s, g, u = K.tgv_case(n, walls=True)
+    m, s, g, u = K.tgv_case(n, walls=True, device=0)
     tmesh = time.perf_counter() - t0
+    phases = dict(M.LAST_BUILD)
     t0 = time.perf_counter()
     solver = Solver(m, s, g)
     tup = time.perf_counter() - t0
     nc = m.ncell_int
     out = dict(workload=f"3D NS Taylor-Green, {n}^3 = {nc} hexes (periodic x/z, slip walls y), first-order HLLC + "
                         f"viscous (cell-full), LSQ gradients, RK4, CFL 0.5", cells=nc, steps=steps,
-               host_setup_s=dict(mesh_build=round(tmesh, 1), upload=round(tup, 1)))
+               host_setup_s=dict(case_with_device_mesh_build=round(tmesh, 1), upload=round(tup, 1)),
+               mesh_build=mesh_build_bench(m, phases))
     states = {}
     dt = None
     for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
@@ -673,8 +703,9 @@ def main():
     from typhon_amd import scheme as S
 
     t0 = time.perf_counter()
-    m, s, g, u = K.vortex_case(args.n, L=10.0)
-    print(f"# mesh: {m.ncell_int} tets, {m.nface} faces (reference form), built in {time.perf_counter() - t0:.1f} s",
+    m, s, g, u = K.vortex_case(args.n, L=10.0, device=0)
+    print(f"# mesh: {m.ncell_int} tets, {m.nface} faces (reference form), built on the device in "
+          f"{time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
     t64, tm64, st64, info = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_FP64, args.order)
     print(f"# fp64: {t64:.3f} s for {args.steps} steps; kernels {tm64}", file=sys.stderr, flush=True)

@@ -17,6 +17,8 @@ implicit none
 integer(c_int), parameter :: TY_OK = 0, TY_ERR_ARG = 1, TY_ERR_HIP = 2, TY_ERR_STATE = 3, &
                              TY_ERR_UNSUPPORTED = 4, TY_ERR_NOT_CONVERGED = 5
 integer(c_int32_t), parameter :: TY_PREC_FP64 = 0, TY_PREC_MIXED = 1
+! element codes of ELEMVTEX.f90 taken by ty_meshbuild_faces
+integer(c_int32_t), parameter :: TY_ELEM_TRI3 = 13, TY_ELEM_QUAD4 = 24, TY_ELEM_TETRA4 = 101, TY_ELEM_HEXA8 = 141
 
 ! umesh%boco(ib) with its type code and resolved uniform parameters
 type, bind(C) :: ty_boco_desc
@@ -294,6 +296,52 @@ interface
     integer(c_int32_t), value   :: nprobe
     integer(c_int32_t)          :: ids(*)
     real(c_double)              :: results(*)
+  endfunction
+
+  ! device mesh preprocessing: create_face_connect + calc_meshgeom (MESHCONNECT.f90:34-163, MESHGEOM.f90:421-461)
+  integer(c_int) function ty_meshbuild_create(mb, device) bind(C, name="ty_meshbuild_create")
+    import :: c_int, c_ptr
+    type(c_ptr)           :: mb
+    integer(c_int), value :: device
+  endfunction
+
+  integer(c_int) function ty_meshbuild_destroy(mb) bind(C, name="ty_meshbuild_destroy")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: mb
+  endfunction
+
+  type(c_ptr) function ty_meshbuild_error(mb) bind(C, name="ty_meshbuild_error")
+    import :: c_ptr
+    type(c_ptr), value :: mb
+  endfunction
+
+  ! elemvtex: cellvtex%elem(1)%elemvtex(1:ncell, 1:nv); vertex: umesh%mesh%vertex(1:nvtex,1,1)
+  integer(c_int) function ty_meshbuild_faces(mb, dim, elemtype, ncell, elemvtex, nvtex, vertex, nface, nface_int) &
+      bind(C, name="ty_meshbuild_faces")
+    import :: c_int, c_ptr, c_int32_t, c_double
+    type(c_ptr), value         :: mb
+    integer(c_int32_t), value  :: dim, elemtype, ncell, nvtex
+    integer(c_int32_t)         :: elemvtex(*)
+    real(c_double)             :: vertex(*)
+    integer(c_int32_t)         :: nface, nface_int
+  endfunction
+
+  ! boundary faces in order of first appearance: bfacevtex(1:nbface, 1:4), bcentre(1:nbface) of v3d
+  integer(c_int) function ty_meshbuild_boundary(mb, bfacevtex, bcentre) bind(C, name="ty_meshbuild_boundary")
+    import :: c_int, c_ptr, c_int32_t, c_double
+    type(c_ptr), value :: mb
+    integer(c_int32_t) :: bfacevtex(*)
+    real(c_double)     :: bcentre(*)
+  endfunction
+
+  ! border(j): boundary-list position of face nface_int + j; outputs as umesh%facecell%fils(1:nface,1:2),
+  ! umesh%facevtex%fils(1:nface,1:4), umesh%mesh%iface(1:nface,1,1), centre and volume of the elements
+  integer(c_int) function ty_meshbuild_finish(mb, border, facecell, facevtex, face, centre, volume) &
+      bind(C, name="ty_meshbuild_finish")
+    import :: c_int, c_ptr, c_int32_t, c_double
+    type(c_ptr), value :: mb
+    integer(c_int32_t) :: border(*), facecell(*), facevtex(*)
+    real(c_double)     :: face(*), centre(*), volume(*)
   endfunction
 
 endinterface

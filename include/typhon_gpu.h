@@ -326,6 +326,48 @@ int ty_timing(ty_ctx* ctx, int32_t enable, double* ms, int32_t* count);
 int ty_mesh_info(const ty_ctx* ctx, int32_t* ncell, int32_t* nface_dev, int32_t* nslot,
                  int32_t* nbface_dev);
 
+/* Mesh preprocessing on the device: the faces and geometry of an element mesh, what
+ * create_face_connect (CFDTOOLS/Mesh/MESHCONNECT.f90:34-163: create_facevtex.f90:15-220, the face
+ * order of reorder_ustconnect MESHCONNECT.f90:511-606) and calc_meshgeom (MESHGEOM.f90:421-461)
+ * compute on the host in the reference.  Three calls on a builder handle:
+ *   ty_meshbuild_faces     the faces: each distinct set of element-face vertices once, numbered in
+ *                          order of first appearance (element by element, local faces in the
+ *                          element's order), its first element the left cell; returns the face
+ *                          and internal-face counts;
+ *   ty_meshbuild_boundary  the boundary faces in that order (vertices and centres), for the caller
+ *                          to sort them into boundary conditions (create_ustboco,
+ *                          MESHCONNECT.f90:186-278, works from the mesh file's vertex lists);
+ *   ty_meshbuild_finish    with the caller's boundary order: internal faces first (in order of
+ *                          appearance), then the boundary faces in that order; left cell < right
+ *                          cell on internal faces (calc_ust_checkface, MESHGEOM.f90:362-410); face
+ *                          geometry (calc_ust_face, MESHGEOM.f90:37-139); cell centres and volumes
+ *                          (calc_ust_midcell/elemvol/cell, MESHGEOM.f90:148-352); each normal
+ *                          turned to point from the left cell to the right cell (out of the domain
+ *                          on the boundary).  The handle's device memory is released after it.
+ * One element type per mesh.  elemvtex: cellvtex%elem(1)%elemvtex(1:ncell, 1:nv), column-major,
+ * 1-based (ELEMVTEX.f90 vertex order); vertex: umesh%mesh%vertex(1:nvtex) of v3d.
+ * bfacevtex(1:nbface, 1:4) column-major, 1-based, 0 where a face has fewer vertices; bcentre:
+ * nbface v3d.  border[j] (j = 1..nbface): the boundary-list position (1-based) of the face that
+ * becomes face nface_int + j.  Outputs of ty_meshbuild_finish in the layouts ty_mesh_desc takes:
+ * facecell(1:nface, 1:2) column-major 1-based with 0 as the right cell of boundary faces (their
+ * ghost cells are the caller's, init_connect_grid.f90:82-88), facevtex(1:nface, 1:4) as bfacevtex,
+ * face(1:nface) st_face {normale(3), centre(3), surface}, centre(1:ncell) v3d, volume(1:ncell) of
+ * the elements.  TY_ERR_ARG for a vertex number outside 1..nvtex, a face shared by more than two
+ * elements, or a border that is not a permutation of 1..nbface. */
+#define TY_ELEM_TRI3 13   /* ELEMVTEX.f90 element codes */
+#define TY_ELEM_QUAD4 24
+#define TY_ELEM_TETRA4 101
+#define TY_ELEM_HEXA8 141
+typedef struct ty_meshbuild ty_meshbuild;
+int ty_meshbuild_create(ty_meshbuild** mb, int device);
+int ty_meshbuild_destroy(ty_meshbuild* mb);
+const char* ty_meshbuild_error(const ty_meshbuild* mb);
+int ty_meshbuild_faces(ty_meshbuild* mb, int32_t dim, int32_t elemtype, int32_t ncell, const int32_t* elemvtex,
+                       int32_t nvtex, const double* vertex, int32_t* nface, int32_t* nface_int);
+int ty_meshbuild_boundary(ty_meshbuild* mb, int32_t* bfacevtex, double* bcentre);
+int ty_meshbuild_finish(ty_meshbuild* mb, const int32_t* border, int32_t* facecell, int32_t* facevtex,
+                        double* face, double* centre, double* volume);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,10 +60,11 @@ def vortex_state(x, L, beta=5.0, gamma=GAMMA):
     return prim_to_cons(rho, p, vel, gamma)
 
 
-def vortex_case(n=100, L=10.0, seed=2, jitter=0.05, **scheme):
+def vortex_case(n=100, L=10.0, seed=2, jitter=0.05, device=None, **scheme):
     """3D Euler, n^3 hexes x 6 tets, periodic in x/y/z, isentropic vortex (strength 5),
-    second-order MUSCL with LSQ gradients and the Venkatakrishnan limiter (K=5), RK4."""
-    m = M.box_tets(n, L, jitter, seed, periodic=True)
+    second-order MUSCL with LSQ gradients and the Venkatakrishnan limiter (K=5), RK4.
+    device: GPU index to build the mesh's faces and geometry on (None: on the host)."""
+    m = M.box_tets(n, L, jitter, seed, periodic=True, device=device)
     cons = vortex_state(m.ccent[:m.ncell_int], L)
     sch = dict(sch_hyp=S.SCH_HLLC, method="M", limiter="W", gradmeth=S.GRAD_LSQ, venkat_k=5.0,
                tps_meth=S.TPS_RK4, cfl=0.8)
@@ -86,11 +87,11 @@ def periodic_quad_case(nx=20, ny=20, jitter=0.0, seed=1, **scheme):
     return m, S.default_scheme(**sch), S.default_gas(), cons
 
 
-def tgv_case(n=16, seed=3, mach=0.3, re=1e4, noise=0.01, jitter=0.0, walls=False, **scheme):
+def tgv_case(n=16, seed=3, mach=0.3, re=1e4, noise=0.01, jitter=0.0, walls=False, device=None, **scheme):
     """3D compressible Navier-Stokes Taylor-Green field on a periodic (2 pi L)^3 hex box, air with
     Sutherland's law (T0 = 300 K, p0 = 1e5 Pa), Re = rho0 U0 L / mu(T0), Pr = 0.72, 1% seeded noise.
     walls: slip walls at y = 0 and y = 2 pi L instead of periodicity in y (the field's v vanishes
-    there)."""
+    there).  device: GPU index to build the mesh's faces and geometry on (None: on the host)."""
     gamma, r = GAMMA, R_AIR
     T0, p0 = 300.0, 1e5
     rho0 = p0 / (r * T0)
@@ -99,7 +100,7 @@ def tgv_case(n=16, seed=3, mach=0.3, re=1e4, noise=0.01, jitter=0.0, walls=False
     mu0, tref, ts = 1.68e-5, 273.0, 110.5
     mu = mu0 * (T0 / tref) ** 1.5 * (tref + ts) / (T0 + ts)
     Lr = re * mu / (rho0 * U0)
-    m = M.box_hex(n, 2 * np.pi * Lr, jitter, seed, periodic=(True, not walls, True))
+    m = M.box_hex(n, 2 * np.pi * Lr, jitter, seed, periodic=(True, not walls, True), device=device)
     x = m.ccent[:m.ncell_int] / Lr
     ni = m.ncell_int
     ids = np.arange(ni)

@@ -17,7 +17,9 @@ EXPORTS = ["ty_create", "ty_destroy", "ty_last_error", "ty_mesh_upload", "ty_set
            "ty_state_get", "ty_timestep", "ty_residual", "ty_rk_step", "ty_advance", "ty_sync", "ty_timing",
            "ty_mesh_info", "ty_jacobian_diag", "ty_jvp", "ty_linear_solve", "ty_implicit_step",
            "ty_implicit_bdlu_step", "ty_residual_tangent", "ty_residual_adjoint", "ty_unsteady_adjoint",
-           "ty_refcons", "ty_local_timestep", "ty_steady_cycle", "ty_probe_define", "ty_probe_eval", "ty_probe_eval_noalloc"]
+           "ty_refcons", "ty_local_timestep", "ty_steady_cycle", "ty_probe_define", "ty_probe_eval", "ty_probe_eval_noalloc",
+           "ty_meshbuild_create", "ty_meshbuild_destroy", "ty_meshbuild_error", "ty_meshbuild_faces",
+           "ty_meshbuild_boundary", "ty_meshbuild_finish"]
 
 
 class TyBocoDesc(C.Structure):
@@ -119,8 +121,15 @@ def load():
     lib.ty_probe_define.argtypes = [vp, C.c_char_p, C.c_int32, I32P]
     lib.ty_probe_eval.argtypes = [vp, C.c_double, C.c_int32, I32P, F64P]
     lib.ty_probe_eval_noalloc.argtypes = [vp, C.c_double, C.c_int32, I32P, F64P]
+    lib.ty_meshbuild_create.argtypes = [C.POINTER(vp), C.c_int]
+    lib.ty_meshbuild_destroy.argtypes = [vp]
+    lib.ty_meshbuild_error.restype = C.c_char_p
+    lib.ty_meshbuild_error.argtypes = [vp]
+    lib.ty_meshbuild_faces.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, I32P, C.c_int32, F64P, I32P, I32P]
+    lib.ty_meshbuild_boundary.argtypes = [vp, I32P, F64P]
+    lib.ty_meshbuild_finish.argtypes = [vp, I32P, I32P, I32P, F64P, F64P, F64P]
     for name in EXPORTS:
-        if name != "ty_last_error":
+        if name not in ("ty_last_error", "ty_meshbuild_error"):
             getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

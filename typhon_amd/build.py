@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtyphon_gpu.so")
 SOURCES = ["ty_adjoint.hip", "ty_kernels_f64.hip", "ty_kernels_f32.hip", "ty_krylov.hip", "ty_setup.hip",
-           "ty_probe.hip", "ty_host.cpp"]
+           "ty_probe.hip", "ty_meshbuild.hip", "ty_host.cpp"]
 GEN = os.path.join(HERE, "_build")  # generated instantiation units (not sources of their own)
 
 # The residual kernels' variants (ty_kernels_impl.h: rec_variant<DIM, R, NSM, DENSE, GOUT>,
@@ -64,7 +64,9 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 # The derivative kernels (tangent and adjoint) have no reference counterpart to match bit for bit:
 # let them multiply by reciprocals and use the rcp + Newton FP64 division (about 1 ulp).  The
 # primal kernels keep IEEE division and square root.
-EXTRA = {"ty_adjoint.hip": ["-freciprocal-math", "-fapprox-func"]}
+EXTRA = {"ty_adjoint.hip": ["-freciprocal-math", "-fapprox-func"],
+         # the device mesh build reproduces the host build's geometry bit for bit: no FMA contraction
+         "ty_meshbuild.hip": ["-ffp-contract=off"]}
 
 
 def build(verbose: bool = False) -> str:

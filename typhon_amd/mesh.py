@@ -13,6 +13,7 @@ All indices here are 0-based; ``ic2`` of a boundary face is its ghost cell
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -101,13 +102,11 @@ _LOCAL_FACES = {
 }
 
 
-def _extract_faces(cells: np.ndarray, etype: str, wrap: np.ndarray | None):
-    """Unique faces of an element mesh.
+def _extract_faces(cells: np.ndarray, etype: str):
+    """Unique faces of an element mesh (create_facevtex.f90:15-220).
 
     Returns fvtx [nf,4] (first occurrence's vertex order), cell1, cell2 (-1 when
-    boundary), in order of first appearance (cell-major).  ``wrap`` maps vertex
-    ids to their periodic representative and is only used to build the periodic
-    pairing key of boundary faces."""
+    boundary), in order of first appearance (cell-major)."""
     lf = _LOCAL_FACES[etype]
     nloc = len(lf)
     nvf = len(lf[0])
@@ -146,23 +145,25 @@ def _extract_faces(cells: np.ndarray, etype: str, wrap: np.ndarray | None):
         raise ValueError("non-manifold mesh: a face is shared by more than two cells")
     # order of first appearance
     app = np.argsort(fidx, kind="stable")
-    fvtx = fv[fidx[app]]
-    c1 = c1[app]
-    c2 = c2[app]
-    pkey = None
-    if wrap is not None:
-        # one id per set of wrapped vertex ids (boundary faces only are ever compared)
-        ws = np.sort(wrap[fvtx[:, :nvf]], axis=1)
-        bnd = c2 < 0
-        pkey = np.full(ws.shape[0], -1, dtype=np.int64)
-        if bnd.any():
-            _, inv = np.unique(ws[bnd], axis=0, return_inverse=True)
-            pkey[bnd] = inv.ravel()
-    return fvtx, c1, c2, pkey
+    return fv[fidx[app]], c1[app], c2[app]
+
+
+def _periodic_keys(bfvtx: np.ndarray, nvf: int, wrap: np.ndarray) -> np.ndarray:
+    """one id per set of wrapped vertex ids of each boundary face: periodic partners share it"""
+    ws = np.sort(wrap[bfvtx[:, :nvf]], axis=1)
+    if ws.shape[0] == 0:
+        return np.zeros(0, dtype=np.int64)
+    _, inv = np.unique(ws, axis=0, return_inverse=True)
+    return inv.ravel()
 
 
 # ---------------------------------------------------------------------------
 # geometry: CFDTOOLS/Mesh/MESHGEOM.f90
+
+def _dot(a, b):
+    """row-wise dot product of [n,3] arrays, summed x, y, z in that order"""
+    return a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1] + a[:, 2] * b[:, 2]
+
 
 def _face_geometry(dim: int, vtx: np.ndarray, fvtx: np.ndarray):
     """calc_ust_face (MESHGEOM.f90:37-139)."""
@@ -184,14 +185,14 @@ def _face_geometry(dim: int, vtx: np.ndarray, fvtx: np.ndarray):
         cen = (v1 + v2 + v3) / 3.0
         if quad.any():
             v4 = vtx[np.where(quad, fvtx[:, 3], fvtx[:, 0])]
-            s1 = np.sqrt(np.einsum("ij,ij->i", n1, n1))
+            s1 = np.sqrt(_dot(n1, n1))
             n2 = .5 * np.cross(b, v4 - v1)
-            s2 = np.sqrt(np.einsum("ij,ij->i", n2, n2))
+            s2 = np.sqrt(_dot(n2, n2))
             cg2 = (v1 + v3 + v4) / 3.0
             qc = (s1[:, None] * cen + s2[:, None] * cg2) / (s1 + s2)[:, None]
             nrm = np.where(quad[:, None], n1 + n2, n1)
             cen = np.where(quad[:, None], qc, cen)
-        surf = np.sqrt(nrm[:, 0] * nrm[:, 0] + nrm[:, 1] * nrm[:, 1] + nrm[:, 2] * nrm[:, 2])
+        surf = np.sqrt(_dot(nrm, nrm))
     return nrm / surf[:, None], cen, surf
 
 
@@ -209,7 +210,7 @@ def _cell_geometry(dim, ncell_int, ic1, ic2, fnorm, fcent, fsurf):
     cen = np.zeros((ncell_int, 3))
     for side, cells, sel in ((0, ic1, slice(None)), (1, ic2[inner], inner)):
         kh = fcent[sel] - mid[cells]
-        h = np.abs(np.einsum("ij,ij->i", kh, fnorm[sel]))
+        h = np.abs(_dot(kh, fnorm[sel]))
         v = kvol * h * fsurf[sel]
         cg = fcent[sel] - kcg * kh
         vol += np.bincount(cells, weights=v, minlength=ncell_int)
@@ -219,36 +220,35 @@ def _cell_geometry(dim, ncell_int, ic1, ic2, fnorm, fcent, fsurf):
     return cen, vol
 
 
-def build_ustmesh(dim: int, vtx: np.ndarray, cells: np.ndarray, etype: str,
-                  face_family, bocos_spec: list, wrap: np.ndarray | None = None) -> UstMesh:
-    """Build Typhon's ``st_ustmesh`` from an element mesh.
+ELEMTYPE = {"tri": 13, "quad": 24, "tet": 101, "hex": 141}  # ELEMVTEX.f90 element codes
 
-    face_family(fvtx_boundary, centres) -> array of family names per boundary face.
-    bocos_spec: list of dicts {family, type, param[, partner]} in boundary order;
-    a periodic boco names its partner family, paired through ``wrap``.
-    """
-    fvtx, c1, c2, pkey = _extract_faces(cells, etype, wrap)
+# wall-clock seconds of the phases of the last build_ustmesh call (for benchmarks)
+LAST_BUILD: dict = {}
+
+
+def _boco_groups(face_family, bocos_spec, bfvtx, bcent):
+    """boundary faces (in order of first appearance) of each boco of the spec, as index arrays"""
+    fam = np.asarray(face_family(bfvtx, bcent))
+    groups = [np.nonzero(fam == spec["family"])[0] for spec in bocos_spec]
+    if sum(len(g) for g in groups) != bfvtx.shape[0]:
+        raise ValueError("boundary faces without a boundary condition family")
+    return groups
+
+
+def _host_faces(dim, vtx, cells, etype, face_family, bocos_spec):
+    """faces in the final order and geometry, on the host (numpy)"""
+    fvtx, c1, c2 = _extract_faces(cells, etype)
     ncell_int = cells.shape[0]
     internal = c2 >= 0
-    # face geometry needs the face vertex list in hand
     fn, fc, fs = _face_geometry(dim, vtx, fvtx)
-    fam = np.asarray(face_family(fvtx[~internal], fc[~internal]))
     bidx = np.nonzero(~internal)[0]
+    groups = _boco_groups(face_family, bocos_spec, fvtx[bidx], fc[bidx])
     # internal faces first, then boundary faces grouped by boco in spec order
-    groups = []
-    for spec in bocos_spec:
-        groups.append(bidx[fam == spec["family"]])
-    nb_listed = sum(len(g) for g in groups)
-    if nb_listed != len(bidx):
-        raise ValueError("boundary faces without a boundary condition family")
-    order = np.concatenate([np.nonzero(internal)[0]] + groups)
+    order = np.concatenate([np.nonzero(internal)[0]] + [bidx[g] for g in groups])
     fvtx = fvtx[order].astype(np.int32)
     c1 = c1[order]
     c2 = c2[order]
     fn, fc, fs = fn[order], fc[order], fs[order]
-    if pkey is not None:
-        pkey = pkey[order]
-    nface = order.shape[0]
     nface_int = int(internal.sum())
     # calc_ust_checkface (MESHGEOM.f90:362-410): ic1 < ic2 for internal faces
     swap = (c2 >= 0) & (c1 > c2)
@@ -257,8 +257,84 @@ def build_ustmesh(dim: int, vtx: np.ndarray, cells: np.ndarray, etype: str,
     c2[swap] = t
     ccent_i, cvol_i = _cell_geometry(dim, ncell_int, c1, c2, fn, fc, fs)
     v12 = np.where((c2 >= 0)[:, None], ccent_i[np.maximum(c2, 0)], fc) - ccent_i[c1]
-    flip = np.einsum("ij,ij->i", v12, fn) < 0.0
+    flip = _dot(v12, fn) < 0.0
     fn[flip] = -fn[flip]
+    return fvtx, c1, c2, fn, fc, fs, ccent_i, cvol_i, nface_int, [len(g) for g in groups]
+
+
+def _device_faces(device, dim, vtx, cells, etype, face_family, bocos_spec):
+    """the same arrays from the library's device mesh build (ty_meshbuild_*); no host fallback"""
+    import ctypes as C
+    from . import _lib
+    lib = _lib.load()
+    mb = C.c_void_p()
+    rc = lib.ty_meshbuild_create(C.byref(mb), int(device))
+    if rc:
+        raise _lib.TyError(rc, "ty_meshbuild_create")
+
+    def check(rc, what):
+        if rc:
+            raise _lib.TyError(rc, f"{what}: {lib.ty_meshbuild_error(mb).decode()}")
+    i32 = lambda a: a.ctypes.data_as(_lib.I32P)  # noqa: E731
+    f64 = lambda a: a.ctypes.data_as(_lib.F64P)  # noqa: E731
+    try:
+        ncell = cells.shape[0]
+        ev = np.ascontiguousarray(cells.T, dtype=np.int32) + 1  # elemvtex(1:ncell, 1:nv), 1-based
+        vt = np.ascontiguousarray(vtx, dtype=np.float64)
+        nf, nfi = C.c_int32(), C.c_int32()
+        t0 = time.perf_counter()
+        check(lib.ty_meshbuild_faces(mb, dim, ELEMTYPE[etype], ncell, i32(ev), vt.shape[0], f64(vt), C.byref(nf),
+                                     C.byref(nfi)), "ty_meshbuild_faces")
+        LAST_BUILD["device_faces"] = time.perf_counter() - t0
+        del ev
+        nface, nface_int = nf.value, nfi.value
+        nb = nface - nface_int
+        bfv = np.empty((4, nb), dtype=np.int32)
+        bc = np.empty((nb, 3))
+        check(lib.ty_meshbuild_boundary(mb, i32(bfv), f64(bc)), "ty_meshbuild_boundary")
+        groups = _boco_groups(face_family, bocos_spec, bfv.T - 1, bc)
+        border = (np.concatenate(groups) + 1).astype(np.int32) if nb else np.zeros(1, dtype=np.int32)
+        facecell = np.empty((2, nface), dtype=np.int32)
+        facevtex = np.empty((4, nface), dtype=np.int32)
+        face = np.empty((nface, 7))
+        cen = np.empty((ncell, 3))
+        vol = np.empty(ncell)
+        t0 = time.perf_counter()
+        check(lib.ty_meshbuild_finish(mb, i32(border), i32(facecell), i32(facevtex), f64(face), f64(cen), f64(vol)),
+              "ty_meshbuild_finish")
+        LAST_BUILD["device_finish"] = time.perf_counter() - t0
+    finally:
+        lib.ty_meshbuild_destroy(mb)
+    c1 = facecell[0].astype(np.int64) - 1
+    c2 = facecell[1].astype(np.int64) - 1
+    fvtx = np.ascontiguousarray(facevtex.T) - 1
+    fn = np.ascontiguousarray(face[:, 0:3])
+    fc = np.ascontiguousarray(face[:, 3:6])
+    fs = np.ascontiguousarray(face[:, 6])
+    return fvtx, c1, c2, fn, fc, fs, cen, vol, nface_int, [len(g) for g in groups]
+
+
+def build_ustmesh(dim: int, vtx: np.ndarray, cells: np.ndarray, etype: str,
+                  face_family, bocos_spec: list, wrap: np.ndarray | None = None, device: int | None = None) -> UstMesh:
+    """Build Typhon's ``st_ustmesh`` from an element mesh.
+
+    face_family(fvtx_boundary, centres) -> array of family names per boundary face.
+    bocos_spec: list of dicts {family, type, param[, partner]} in boundary order;
+    a periodic boco names its partner family, paired through ``wrap``.
+    device: None builds faces and geometry on the host (numpy); a GPU index builds them there
+    (ty_meshbuild_*: the same arrays bit for bit).  Boundary grouping, periodic pairing and ghost
+    cells (boundary-sized work) stay on the host either way.
+    """
+    t0 = time.perf_counter()
+    LAST_BUILD.clear()
+    if device is None:
+        out = _host_faces(dim, vtx, cells, etype, face_family, bocos_spec)
+    else:
+        out = _device_faces(device, dim, vtx, cells, etype, face_family, bocos_spec)
+    LAST_BUILD["faces_and_geometry"] = time.perf_counter() - t0
+    fvtx, c1, c2, fn, fc, fs, ccent_i, cvol_i, nface_int, sizes = out
+    ncell_int = cells.shape[0]
+    nface = fvtx.shape[0]
     # ghost cells, one per boundary face, in boco order (init_connect_grid.f90:82-88)
     nghost = nface - nface_int
     ncell = ncell_int + nghost
@@ -266,28 +342,35 @@ def build_ustmesh(dim: int, vtx: np.ndarray, cells: np.ndarray, etype: str,
     cvol = np.zeros(ncell)
     ccent[:ncell_int] = ccent_i
     cvol[:ncell_int] = cvol_i
-    bocos = []
-    start = nface_int
+    c2[nface_int:] = ncell_int + np.arange(nghost)
+    pkey = None
+    if wrap is not None:
+        pkey = _periodic_keys(fvtx[nface_int:], len(_LOCAL_FACES[etype][0]), wrap)
     spans = {}
-    for spec, g in zip(bocos_spec, groups):
-        faces = np.arange(start, start + len(g), dtype=np.int32)
-        spans[spec["family"]] = faces
-        start += len(g)
+    start = nface_int
+    for spec, n in zip(bocos_spec, sizes):
+        spans[spec["family"]] = np.arange(start, start + n, dtype=np.int32)
+        start += n
+    bocos = []
     for spec in bocos_spec:
         faces = spans[spec["family"]]
-        ghost = ncell_int + (faces - nface_int)
-        c2[faces] = ghost
         btype = spec["type"]
         ipart = None
         if btype == BC_PERIOD:
+            # partner face: the face of the partner family with the same wrapped vertex set
             pf = spans[spec["partner"]]
-            kp = {int(k): int(f) for k, f in zip(pkey[pf], pf)}
-            ipart = np.array([kp[int(k)] for k in pkey[faces]], dtype=np.int32)
+            kp = pkey[pf - nface_int]
+            srt = np.argsort(kp, kind="stable")
+            pos = np.minimum(np.searchsorted(kp[srt], pkey[faces - nface_int]), max(len(pf) - 1, 0))
+            if len(pf) != len(faces) or not np.array_equal(kp[srt][pos], pkey[faces - nface_int]):
+                raise ValueError(f"periodic boco {spec['family']}: faces without a partner in {spec['partner']}")
+            ipart = pf[srt[pos]].astype(np.int32)
         param = np.zeros(8)
         p = np.asarray(spec.get("param", []), dtype=np.float64)
         param[:p.size] = p
         bocos.append(Boco(spec["family"], btype, faces, ipart, param))
     _ghost_geometry(ncell_int, nface_int, c1, fn, fc, ccent, cvol, bocos)
+    LAST_BUILD["total"] = time.perf_counter() - t0
     return UstMesh(dim, ncell_int, ncell, nface, nface_int, c1.astype(np.int32), c2.astype(np.int32),
                    fn, fc, fs, ccent, cvol, bocos, vtx, fvtx)
 
@@ -336,7 +419,7 @@ def moved(m: UstMesh, vtx: np.ndarray) -> UstMesh:
 # ---------------------------------------------------------------------------
 # synthetic meshes
 
-def channel_quad(nx=256, ny=128, lx=1.0, ly=0.5, jitter=0.1, seed=1, bocos=None):
+def channel_quad(nx=256, ny=128, lx=1.0, ly=0.5, jitter=0.1, seed=1, bocos=None, device=None):
     """2D channel of nx*ny quads, interior vertices jittered by +-jitter*h.
 
     Families IMIN/IMAX (x=0/lx) and JMIN/JMAX (y=0/ly)."""
@@ -373,7 +456,7 @@ def channel_quad(nx=256, ny=128, lx=1.0, ly=0.5, jitter=0.1, seed=1, bocos=None)
         out[(jv == ny) & (jw == ny)] = "JMAX"
         return out
 
-    return build_ustmesh(2, vtx, cells, "quad", fam, bocos, wrap)
+    return build_ustmesh(2, vtx, cells, "quad", fam, bocos, wrap, device)
 
 
 def _axes(periodic):
@@ -445,7 +528,7 @@ def _box_family(n):
     return fam
 
 
-def box_tets(n=100, L=10.0, jitter=0.05, seed=2, periodic=True, bocos=None):
+def box_tets(n=100, L=10.0, jitter=0.05, seed=2, periodic=True, bocos=None, device=None):
     """n^3 hexes on [0,L]^3, each split into 6 tets about the main diagonal
     (Kuhn/Freudenthal split, conforming between neighbours)."""
     vtx, wrap = _box_vertices(n, L, jitter, seed, periodic)
@@ -459,7 +542,7 @@ def box_tets(n=100, L=10.0, jitter=0.05, seed=2, periodic=True, bocos=None):
     cells = np.stack([np.stack([h[a], h[b], h[c], h[d]], axis=1) for a, b, c, d in kuhn], axis=1).reshape(-1, 4)
     if bocos is None:
         bocos = _box_bocos(periodic)
-    return build_ustmesh(3, vtx, cells, "tet", _box_family(n), bocos, wrap)
+    return build_ustmesh(3, vtx, cells, "tet", _box_family(n), bocos, wrap, device)
 
 
 def bump_channel_vertices(nx=63, ny=33, nz=20, lx=3.0, ly=1.0, lz=0.6, jitter=0.05, seed=4, amp=0.05):
@@ -481,7 +564,7 @@ def bump_channel_vertices(nx=63, ny=33, nz=20, lx=3.0, ly=1.0, lz=0.6, jitter=0.
 
 
 def bump_channel_tets(nx=63, ny=33, nz=20, lx=3.0, ly=1.0, lz=0.6, jitter=0.05, seed=4, amp=0.05, bocos=None,
-                      vtx=None):
+                      vtx=None, device=None):
     """Tet channel with a bottom-wall bump (6 Kuhn tets per hex): families IMIN/IMAX (x = 0 / lx),
     JMIN (bump wall) / JMAX, KMIN/KMAX.  ``vtx`` replaces the generated vertex coordinates (same
     numbering), e.g. with displaced wall vertices."""
@@ -507,10 +590,10 @@ def bump_channel_tets(nx=63, ny=33, nz=20, lx=3.0, ly=1.0, lz=0.6, jitter=0.05, 
         return out
     if bocos is None:
         bocos = [dict(family=f, type=BC_SYM) for f in ("IMIN", "IMAX", "JMIN", "JMAX", "KMIN", "KMAX")]
-    return build_ustmesh(3, np.asarray(vtx, dtype=np.float64), cells, "tet", fam, bocos, None)
+    return build_ustmesh(3, np.asarray(vtx, dtype=np.float64), cells, "tet", fam, bocos, None, device)
 
 
-def box_hex(n=16, L=2 * np.pi, jitter=0.0, seed=3, periodic=True, bocos=None):
+def box_hex(n=16, L=2 * np.pi, jitter=0.0, seed=3, periodic=True, bocos=None, device=None):
     """n^3 hexes on [0,L]^3; periodic: bool or per-axis (x, y, z) flags (slip walls elsewhere)."""
     vtx, wrap = _box_vertices(n, L, jitter, seed, periodic)
     ci, cj, ck = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
@@ -522,10 +605,10 @@ def box_hex(n=16, L=2 * np.pi, jitter=0.0, seed=3, periodic=True, bocos=None):
                       v(0, 0, 1), v(1, 0, 1), v(1, 1, 1), v(0, 1, 1)], axis=1)
     if bocos is None:
         bocos = _box_bocos(periodic)
-    return build_ustmesh(3, vtx, cells, "hex", _box_family(n), bocos, wrap)
+    return build_ustmesh(3, vtx, cells, "hex", _box_family(n), bocos, wrap, device)
 
 
-def periodic_quad(nx=20, ny=20, lx=1.0, ly=1.0, jitter=0.0, seed=1):
+def periodic_quad(nx=20, ny=20, lx=1.0, ly=1.0, jitter=0.0, seed=1, device=None):
     """2D doubly periodic quad mesh (the NRG TIME/RUNGEKUTTA layout)."""
     bocos = [dict(family="IMIN", type=BC_PERIOD, partner="IMAX"),
              dict(family="IMAX", type=BC_PERIOD, partner="IMIN"),
@@ -554,4 +637,4 @@ def periodic_quad(nx=20, ny=20, lx=1.0, ly=1.0, jitter=0.0, seed=1):
         out[(jv == ny) & (jw == ny)] = "JMAX"
         return out
 
-    return build_ustmesh(2, vtx, cells, "quad", fam, bocos, wrap)
+    return build_ustmesh(2, vtx, cells, "quad", fam, bocos, wrap, device)

@@ -178,9 +178,9 @@ def read_tym(path) -> TyMesh:
 _ETYPE = {ELEM_QUAD4: "quad", ELEM_TRI3: "tri", ELEM_TETRA4: "tet", ELEM_HEXA8: "hex"}
 
 
-def tym_ustmesh(path, bocos: list, dim: int | None = None):
-    """Build the solver mesh (typhon_amd.mesh.UstMesh) of a .tym file; ``bocos`` as for
-    mesh.build_ustmesh (families named after the file's marks; periodic bocos name their
+def tym_ustmesh(path, bocos: list, dim: int | None = None, device: int | None = None):
+    """Build the solver mesh (typhon_amd.mesh.UstMesh) of a .tym file; ``bocos`` and ``device`` as
+    for mesh.build_ustmesh (families named after the file's marks; periodic bocos name their
     partner and are paired through the translation ``period`` = (dx, dy, dz) of the spec)."""
     from . import mesh as M
     t = read_tym(path)
@@ -207,7 +207,7 @@ def tym_ustmesh(path, bocos: list, dim: int | None = None):
             key = tuple(sorted(int(v) for v in fv[k] if v >= 0))
             out[k] = key_fam.get(key)
         return out
-    return M.build_ustmesh(dim, t.vtx.copy(), cv, etype, family, bocos, wrap)
+    return M.build_ustmesh(dim, t.vtx.copy(), cv, etype, family, bocos, wrap, device)
 
 
 def _periodic_wrap(t: TyMesh, bocos):
