@@ -702,16 +702,16 @@ int build_geo(ty_ctx* c, Geo<R>& g)
     g.eps2 = dp + o_eps;
     g.sh = dp + o_sh;
     g.bboco = c->d_bboco;
-    g.pa = g.pb = g.pw = nullptr;
-    if constexpr (std::is_same<R, float>::value) {
-        // FP32: the packed slot records of the viscous kernels
-        float4* pk = nullptr;
-        if ((rc = dalloc(c, &pk, (size_t)3 * ns * n))) return rc;
-        HIPCHK(c, launch_geo_pack(n, ns, g, c->d_link, pk, c->stream));
-        g.pa = pk;
-        g.pb = pk + (size_t)ns * n;
-        g.pw = pk + (size_t)2 * ns * n;
-    }
+    // the packed slot records of the interior paths (viscous and plain-gradient kernels)
+    // (FP32: pw, pa, pb; FP64: pw, the FP64 flux kernel keeps its scalar loads)
+    typename Pack4<R>::S* pk = nullptr;
+    const size_t arr = (size_t)Pack4<R>::planes * ns * n;
+    const bool fp32 = std::is_same<R, float>::value;
+    if ((rc = dalloc(c, &pk, (fp32 ? 3 : 1) * arr))) return rc;
+    HIPCHK(c, launch_geo_pack<R>(n, ns, g, c->d_link, pk, c->stream));
+    g.pw = pk;
+    g.pa = fp32 ? pk + arr : nullptr;
+    g.pb = fp32 ? pk + 2 * arr : nullptr;
     return TY_OK;
 }
 

@@ -57,6 +57,21 @@ struct BocoDev {
 // the per-face arrays.  "Canonical" quantities are those of the face's owner
 // (the reference's ic1 side): the normal points owner -> other and
 // d = x_R - x_L is taken in the owner's frame (periodic far sides translated).
+// four values of precision R per record: FP32 as one float4 array, FP64 as two planes of double2
+// (x, y | z, w), so that each load of a wavefront reads one contiguous block
+template <typename R>
+struct Pack4;
+template <>
+struct Pack4<float> {
+    using S = float4;
+    static constexpr int planes = 1;
+};
+template <>
+struct Pack4<double> {
+    using S = double2;
+    static constexpr int planes = 2;
+};
+
 template <typename R>
 struct Geo {
     const R* ss;    // [ns][3][n] canonical area vector S = A n of the slot's face
@@ -68,12 +83,13 @@ struct Geo {
     const R* eps2;                  // [n] Venkatakrishnan (K h)^3
     const R* sh;                    // [ns][n] viscous face weight dHL = |x_f - x_L| / (|x_f - x_L| + |x_f - x_R|)
                                     // (canonical sides; the other side's weight is 1 - dHL)
-    // FP32 geometry only (null in FP64): the same data packed per slot, [ns][n] float4, one load per
-    // record in the FP32 viscous kernels: pa = (S, dHL), pb = (x_R - x_L, link word),
-    // pw = (LSQ weight, link word); the link word's bits are stored in the fourth float
-    const float4* pa;
-    const float4* pb;
-    const float4* pw;
+    // the same data packed per slot, [ns][n] records of four R (Pack4), read in the interior paths
+    // of the FP32 viscous flux (pa = (S, dHL), pb = (x_R - x_L, link word); null in FP64) and of
+    // the plain-gradient kernels (pw = (LSQ weight, link word)); the link word's bits are stored in
+    // the fourth element
+    const typename Pack4<R>::S* pa;
+    const typename Pack4<R>::S* pb;
+    const typename Pack4<R>::S* pw;
 };
 
 struct Physics {
@@ -229,7 +245,8 @@ struct GeoSrc {
 template <typename R>
 hipError_t launch_geo(const GeoSrc& s, R* out, hipStream_t st);
 // the FP32 packed slot records of Geo<float> (pa, pb, pw: [ns][n] each in `out`) from its SoA arrays
-hipError_t launch_geo_pack(int n, int ns, const Geo<float>& g, const int* link, float4* out, hipStream_t st);
+template <typename R>
+hipError_t launch_geo_pack(int n, int ns, const Geo<R>& g, const int* link, typename Pack4<R>::S* out, hipStream_t st);
 // stable ascending device sort of n host keys over bits [0, end_bit): idx_out (host) gets the
 // input position of each key in sorted order (ty_setup.hip)
 hipError_t sort_keys_device(const uint64_t* keys, int n, int end_bit, int* idx_out, hipStream_t st);

@@ -770,6 +770,22 @@ __device__ __forceinline__ float4 nt_load4(const float4* p)
     const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
     return make_float4(v.x, v.y, v.z, v.w);
 }
+__device__ __forceinline__ double2 nt_load2(const double2* p)
+{
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return make_double2(v.x, v.y);
+}
+// record i of a packed slot array (Pack4): FP32 one float4, FP64 two double2 planes
+__device__ __forceinline__ float4 ld_pack(const float4* p, size_t i, size_t) { return nt_load4(p + i); }
+__device__ __forceinline__ double4 ld_pack(const double2* p, size_t i, size_t plane)
+{
+    const double2 a = nt_load2(p + i), b = nt_load2(p + plane + i);
+    return make_double4(a.x, a.y, b.x, b.y);
+}
+// the link word stored in the fourth element of a packed record
+__device__ __forceinline__ int link_of(float w) { return __float_as_int(w); }
+__device__ __forceinline__ int link_of(double w) { return (int)__double_as_longlong(w); }
 
 // Whether the wavefront of cell c holds a cell with a boundary slot: bits 2w and 2w + 1 (its two
 // 32-cell groups) of the group bits, w = c / 64; uniform over the wavefront.
@@ -811,14 +827,14 @@ __device__ __forceinline__ void grad_cell(const KArgs<R>& a, int c)
 #pragma unroll
         for (int e = 0; e < DIM; ++e) gr[v * DIM + e] = R(0);
     }
-    // FP32 interior path: the link word and the LSQ weight in one packed record (pw)
-    constexpr bool PW = DENSE && !BND && sizeof(R) == 4;
+    // interior path: the link word and the LSQ weight in one packed record (pw)
+    constexpr bool PW = DENSE && !BND;
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
         if (!DENSE && k >= a.ns) break;
-        float4 pw;
-        if (PW) pw = nt_load4(a.g.pw + (size_t)k * n + c);
-        const int L = PW ? __float_as_int(pw.w) : a.link[(size_t)k * n + c];
+        decltype(ld_pack(a.g.pw, 0, 0)) pw;
+        if (PW) pw = ld_pack(a.g.pw, (size_t)k * n + c, (size_t)a.ns * n);
+        const int L = PW ? link_of(pw.w) : a.link[(size_t)k * n + c];
         if (!DENSE && L == SLOT_UNUSED) continue;
         R qo[NV], wk[DIM];
         far_state<DIM, BND>(a, k, c, L, qc, qo);
@@ -1283,21 +1299,22 @@ __device__ __forceinline__ void flux_cell(const KArgs<R>& a, int c)
 #pragma unroll
     for (int v = 0; v < NV; ++v) res[v] = R(0);
     // FP32 interior path of the first-order viscous flux: per slot one packed record pb (centre
-    // difference, link word) and one pa (area vector, distance weight) in place of six scalar loads
+    // difference, link word) and one pa (area vector, distance weight) in place of six scalar loads.
+    // (The FP64 kernel measured 1% slower with the two-plane FP64 records and keeps its scalar loads.)
     constexpr bool PF = !FS && VISC && !JAC && DENSE && !BND && sizeof(R) == 4;
 #pragma unroll
     for (int k = 0; k < NSM; ++k) {
         if (!DENSE && k >= a.ns) break;
-        float4 pb;
-        if (PF) pb = nt_load4(a.g.pb + (size_t)k * n + c);
-        const int L = PF ? __float_as_int(pb.w) : a.link[(size_t)k * n + c];
+        decltype(ld_pack(a.g.pb, 0, 0)) pb;
+        if (PF) pb = ld_pack(a.g.pb, (size_t)k * n + c, (size_t)a.ns * n);
+        const int L = PF ? link_of(pb.w) : a.link[(size_t)k * n + c];
         if (!DENSE && L == SLOT_UNUSED) continue;
         const bool bnd = BND && link_boundary(L);
         const bool other = link_other(L);
         R nf[DIM];
         R A, hpk = R(0);
         if (PF) {
-            const float4 pa = nt_load4(a.g.pa + (size_t)k * n + c);
+            const auto pa = ld_pack(a.g.pa, (size_t)k * n + c, (size_t)a.ns * n);
             const R S[3] = {R(pa.x), R(pa.y), R(pa.z)};
             A = asqrt(dotd<DIM>(S, S));
             const R ia = arcp(A);

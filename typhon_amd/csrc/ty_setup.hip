@@ -146,29 +146,53 @@ hipError_t launch_geo(const GeoSrc& s, R* out, hipStream_t st)
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(BLOCK) void k_geo_pack(int n, int ns, Geo<float> g, const int* __restrict__ link,
-                                                   float4* __restrict__ out)
+// link word bits in the fourth element of a packed record
+__device__ __forceinline__ float link_as(float, int L) { return __int_as_float(L); }
+__device__ __forceinline__ double link_as(double, int L) { return __longlong_as_double((long long)(unsigned)L); }
+
+__device__ __forceinline__ void st_pack(float4* p, size_t i, size_t, float x, float y, float z, float w)
 {
+    p[i] = make_float4(x, y, z, w);
+}
+__device__ __forceinline__ void st_pack(double2* p, size_t i, size_t plane, double x, double y, double z, double w)
+{
+    p[i] = make_double2(x, y);
+    p[plane + i] = make_double2(z, w);
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_geo_pack(int n, int ns, Geo<R> g, const int* __restrict__ link,
+                                                   typename Pack4<R>::S* __restrict__ out)
+{
+    using P = typename Pack4<R>::S;
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= n) return;
-    float4* pa = out;
-    float4* pb = out + (size_t)ns * n;
-    float4* pw = out + (size_t)2 * ns * n;
+    // FP32: pw | pa | pb; FP64: pw only
+    const size_t plane = (size_t)ns * n, arr = Pack4<R>::planes * plane;
+    P* pw = out;
+    P* pa = out + arr;
+    P* pb = out + 2 * arr;
     for (int k = 0; k < ns; ++k) {
         const size_t i = (size_t)k * n + c;
         const size_t v = (size_t)k * 3 * n + c;
-        const float lb = __int_as_float(link[i]);
-        pa[i] = make_float4(g.ss[v], g.ss[v + n], g.ss[v + 2 * (size_t)n], g.sh[i]);
-        pb[i] = make_float4(g.sd[v], g.sd[v + n], g.sd[v + 2 * (size_t)n], lb);
-        pw[i] = make_float4(g.sw[v], g.sw[v + n], g.sw[v + 2 * (size_t)n], lb);
+        const R lb = link_as(R(0), link[i]);
+        st_pack(pw, i, plane, g.sw[v], g.sw[v + n], g.sw[v + 2 * (size_t)n], lb);
+        if (sizeof(R) == 4) {
+            st_pack(pa, i, plane, g.ss[v], g.ss[v + n], g.ss[v + 2 * (size_t)n], g.sh[i]);
+            st_pack(pb, i, plane, g.sd[v], g.sd[v + n], g.sd[v + 2 * (size_t)n], lb);
+        }
     }
 }
 
-hipError_t launch_geo_pack(int n, int ns, const Geo<float>& g, const int* link, float4* out, hipStream_t st)
+template <typename R>
+hipError_t launch_geo_pack(int n, int ns, const Geo<R>& g, const int* link, typename Pack4<R>::S* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_geo_pack, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, n, ns, g, link, out);
+    hipLaunchKernelGGL(k_geo_pack<R>, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, n, ns, g, link, out);
     return hipGetLastError();
 }
+
+template hipError_t launch_geo_pack<float>(int, int, const Geo<float>&, const int*, float4*, hipStream_t);
+template hipError_t launch_geo_pack<double>(int, int, const Geo<double>&, const int*, double2*, hipStream_t);
 
 template hipError_t launch_geo<float>(const GeoSrc&, float*, hipStream_t);
 template hipError_t launch_geo<double>(const GeoSrc&, double*, hipStream_t);
