@@ -38,16 +38,30 @@ METRIC = "residual Mcell-updates/s, FP64 vs FP32+refine at stated err; adjoint/p
 HBM_PEAK_GBS = 8000.0
 
 
-def byte_model(nc, nf, nb, dim=3, nv=5, mixed=False, rk4=True):
+def byte_model(nc, nf, nb, dim=3, nv=5, mixed=False, rk4=True, gradients=True):
     """Algorithmic HBM bytes per launch of the two residual kernels (compulsory traffic, each
-    array once per pass, int32 indices): gradient+limiter pass and flux+gather+RK-update pass."""
+    array once per pass, int32 indices): gradient+limiter pass and flux+gather+RK-update pass.
+    gradients=False: a first-order inviscid residual (no gradient pass); rk4=False: the RK2
+    (midpoint) combine instead of RK4's."""
     ps = pg = px = 4 if mixed else 8
     E = 2 * nf - nb
     b_grad = nc * (nv * ps + 8 * dim + dim * (dim + 1) // 2 * pg + dim * nv * pg + 4) + 8 * nf + 4 * E
     b_flux = nc * (nv * ps + 8 + 4) + nf * (dim * px + 8) + 4 * E
-    b_flux += nc * (dim * nv * pg + 8 * dim) + nf * dim * px
-    b_rk = nc * nv * (22 + 0.75 * ps) if rk4 else 0.0
+    if gradients:
+        b_flux += nc * (dim * nv * pg + 8 * dim) + nf * dim * px
+    else:
+        b_grad = 0
+    b_rk = nc * nv * (22 + 0.75 * ps) if rk4 else nc * nv * (ps + 16) / 2
     return dict(grad=b_grad, flux=b_flux + b_rk, residual_only=b_grad + b_flux, total=b_grad + b_flux + b_rk)
+
+
+def flux_roofline(bytes_per_launch, tm, what):
+    """roofline object of the flux kernel from its event-timed launches"""
+    ms = tm["flux_ms"] / max(1, tm["flux_n"])
+    ach = bytes_per_launch / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "k_flux", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes_per_launch": int(bytes_per_launch),
+            "avg_launch_ms": round(ms, 5), "model": what}
 
 
 def run_gpu(m, s, g, u, steps, warmup, precision, order="hilbert"):
@@ -161,13 +175,23 @@ def c1_bench(steps=500, reps=3, cpu=True):
         solver.advance(steps)
         solver.sync()
         best = min(best, time.perf_counter() - t0)
+    solver.set_state(u)
+    solver.sync()
+    solver.timing(True)
+    solver.advance(steps)
+    solver.sync()
+    tm = solver.timing(False)
+    info = solver.mesh_info()
     gpu_state = run_steps(solver, u, steps)
     solver.close()
     nc = m.ncell_int
+    bm = byte_model(nc, info["nface"], info["nbface"], dim=2, nv=4, rk4=False, gradients=False)
     out = dict(workload=f"2D Euler channel, 256 x 128 = {nc} jittered quads, first-order HLLC + RK2, CFL 0.8, "
                         f"{steps} steps", cells=nc, steps=steps, value=round(nc * 2 * steps / best / 1e6, 1),
                unit="Mcell-updates/s", ms_per_step=round(best / steps * 1e3, 4), dtype="f64",
-               us_per_cell_it=round(best / steps / nc * 1e6, 6))
+               us_per_cell_it=round(best / steps / nc * 1e6, 6),
+               roofline=flux_roofline(bm["flux"], tm, f"first-order flux + RK2 combine, "
+                                                      f"{bm['flux'] / nc:.1f} B/cell per launch"))
     if cpu:
         from oracle import oracle as OR
         o = K.make_oracle(m, s, g)
@@ -187,7 +211,9 @@ def c1_bench(steps=500, reps=3, cpu=True):
 
 def c4_bench(rate_steps=500, nvert=64, log=None):
     """the adjoint workload (249,480-tet bump channel, first-order HLLC + RK2): primal rate over
-    ``rate_steps`` device-resident steps, and the wall-force gradient check (bump_gradient.check)"""
+    ``rate_steps`` device-resident steps with its flux kernel's roofline, and the wall-force gradient
+    checks (bump_gradient.check): ``nvert`` bump-wall vertices after 20 steps, the inflow rho, u, p
+    after 400 steps"""
     import cases as K
     import bump_gradient as BG
     from typhon_amd.solver import Solver
@@ -200,14 +226,26 @@ def c4_bench(rate_steps=500, nvert=64, log=None):
     solver.advance(rate_steps)
     solver.sync()
     el = time.perf_counter() - t0
+    solver.set_state(u)
+    solver.sync()
+    solver.timing(True)
+    solver.advance(rate_steps)
+    solver.sync()
+    tm = solver.timing(False)
+    info = solver.mesh_info()
     solver.close()
     nc = m.ncell_int
+    bm = byte_model(nc, info["nface"], info["nbface"], rk4=False, gradients=False)
     out = dict(primal=dict(value=round(nc * 2 * rate_steps / el / 1e6, 1), unit="Mcell-updates/s",
-                           ms_per_step=round(el / rate_steps * 1e3, 4), steps=rate_steps, dtype="f64"))
-    rep = BG.check(nvert=nvert, log=log)
-    rows = rep.pop("rows")
-    rep["table"] = [[r["param"], r["adjoint"], r["fd"], r["rel_diff"]] for r in rows]
-    out["gradient"] = rep
+                           ms_per_step=round(el / rate_steps * 1e3, 4), steps=rate_steps, dtype="f64",
+                           roofline=flux_roofline(bm["flux"], tm, f"first-order flux + RK2 combine, "
+                                                                  f"{bm['flux'] / nc:.1f} B/cell per launch")))
+    for key, kw in (("gradient", dict(nvert=nvert)),
+                    ("gradient_inflow", dict(nsteps=400, nvert=0, inflow=True))):
+        rep = BG.check(log=log, **kw)
+        rows = rep.pop("rows")
+        rep["table"] = [[r["param"], r["adjoint"], r["fd"], r["rel_diff"], r["rel_to_norm"]] for r in rows]
+        out[key] = rep
     return out
 
 
@@ -622,7 +660,7 @@ def implicit_bench(n=160, dt_mult=10.0, steps=2):
     import cases as K
     from typhon_amd.solver import Solver, krylov
     t0 = time.perf_counter()
-    m, s, g, u = K.tgv_case(n)
+    m, s, g, u = K.tgv_case(n, device=0)
     tmesh = time.perf_counter() - t0
     solver = Solver(m, s, g)
     solver.set_state(u)
@@ -636,13 +674,37 @@ def implicit_bench(n=160, dt_mult=10.0, steps=2):
         stats.append(solver.implicit_step(dt, kry))
     solver.sync()
     el = time.perf_counter() - t0
+    # the residual kernels inside the Krylov loop (FP32 JVPs, FP64 outer residuals), event-timed on a
+    # repeat of the same steps
+    solver.set_state(u)
+    solver.implicit_step(dt, kry)
+    solver.sync()
+    solver.timing(True)
+    for _ in range(steps):
+        solver.implicit_step(dt, kry)
+    solver.sync()
+    tm = solver.timing(False)
+    info = solver.mesh_info()
     solver.close()
     inner = sum(st["inner_iters"] for st in stats)
+    nc = m.ncell_int
+    # one FP32 JVP = one residual evaluation (gradient + flux passes, no RK combine); the average
+    # also holds the few slower FP64 launches of the outer passes, priced at the FP32 bytes, so the
+    # achieved rate is understated
+    bm = byte_model(nc, info["nface"], info["nbface"], mixed=True)
+    jvp_ms = tm["grad_ms"] / max(1, tm["grad_n"]) + tm["flux_ms"] / max(1, tm["flux_n"])
+    res_bytes = bm["residual_only"]
+    ach = res_bytes / (jvp_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "k_gradient + k_flux (residual of a JVP)", "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_launch": int(res_bytes), "avg_launch_ms": round(jvp_ms, 4),
+            "model": f"mixed residual-only, {res_bytes / nc:.1f} B/cell",
+            "residual_kernel_share_of_step": round((tm["grad_ms"] + tm["flux_ms"]) / (el * 1e3), 3)}
     return dict(workload=f"3D NS Taylor-Green, {n}^3 = {m.ncell_int} periodic hexes, implicit backward Euler, "
                          f"dt = {dt_mult:g} x explicit", ms_per_implicit_step=round(el / steps * 1e3, 2),
                 outer_iters=[st["outer_iters"] for st in stats], inner_iters=[st["inner_iters"] for st in stats],
                 rel_res=[st["rel_res"] for st in stats], ms_per_arnoldi_step=round(el / max(1, inner) * 1e3, 3),
-                mesh_build_s=round(tmesh, 1))
+                mesh_build_s=round(tmesh, 1), roofline=roof)
 
 
 def load_traffic(path):
