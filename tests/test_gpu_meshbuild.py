@@ -3,8 +3,9 @@ MESHCONNECT.f90:34-163, and the geometry of calc_meshgeom, MESHGEOM.f90:421-461)
 build (typhon_amd/mesh.py, numpy).
 
 The host build is the checker here: the meshes it builds feed the residual tests that match the
-reference's NRG golden histories, and tests/test_oracle_pins.py checks its geometry against closed
-forms.  Bar: bit for bit on every array -- face order, facecell, facevtex, boundary faces and periodic
+reference's NRG golden histories, and tests/test_oracle.py checks its geometry against closed forms
+(test_geometry_closed_forms) and against the oracle's C restatement of MESHGEOM.f90
+(test_meshgeom_c_restatement_matches_builder).  Bar: bit for bit on every array -- face order, facecell, facevtex, boundary faces and periodic
 partners, and every geometric array (the device evaluates the host's expressions in the host's
 order, without FMA contraction).  At the 256^3-hex size the host build takes minutes, so there the
 device build is checked through properties that fix it: volumes add up to the box, every cell is
