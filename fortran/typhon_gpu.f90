@@ -298,6 +298,13 @@ interface
     real(c_double)              :: results(*)
   endfunction
 
+  ! the device cell order: cell(i) = the caller's cell number of device cell i
+  integer(c_int) function ty_cell_order(ctx, cell) bind(C, name="ty_cell_order")
+    import :: c_int, c_ptr, c_int32_t
+    type(c_ptr), value :: ctx
+    integer(c_int32_t) :: cell(*)
+  endfunction
+
   ! device mesh preprocessing: create_face_connect + calc_meshgeom (MESHCONNECT.f90:34-163, MESHGEOM.f90:421-461)
   integer(c_int) function ty_meshbuild_create(mb, device) bind(C, name="ty_meshbuild_create")
     import :: c_int, c_ptr

@@ -325,6 +325,10 @@ int ty_timing(ty_ctx* ctx, int32_t enable, double* ms, int32_t* count);
 /* layout facts for the caller / harness: number of slots per cell, merged face count */
 int ty_mesh_info(const ty_ctx* ctx, int32_t* ncell, int32_t* nface_dev, int32_t* nslot,
                  int32_t* nbface_dev);
+/* the device cell order chosen by ty_mesh_upload (no reference counterpart; the layout a caller needs
+ * to address device-ordered data): cell[i] = the caller's cell number (1-based) of the i-th device
+ * cell, i = 0..ncell_int-1 */
+int ty_cell_order(const ty_ctx* ctx, int32_t* cell);
 
 /* Mesh preprocessing on the device: the faces and geometry of an element mesh, what
  * create_face_connect (CFDTOOLS/Mesh/MESHCONNECT.f90:34-163: create_facevtex.f90:15-220, the face

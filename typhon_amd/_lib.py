@@ -18,7 +18,7 @@ EXPORTS = ["ty_create", "ty_destroy", "ty_last_error", "ty_mesh_upload", "ty_set
            "ty_mesh_info", "ty_jacobian_diag", "ty_jvp", "ty_linear_solve", "ty_implicit_step",
            "ty_implicit_bdlu_step", "ty_residual_tangent", "ty_residual_adjoint", "ty_unsteady_adjoint",
            "ty_refcons", "ty_local_timestep", "ty_steady_cycle", "ty_probe_define", "ty_probe_eval", "ty_probe_eval_noalloc",
-           "ty_meshbuild_create", "ty_meshbuild_destroy", "ty_meshbuild_error", "ty_meshbuild_faces",
+           "ty_cell_order", "ty_meshbuild_create", "ty_meshbuild_destroy", "ty_meshbuild_error", "ty_meshbuild_faces",
            "ty_meshbuild_boundary", "ty_meshbuild_finish"]
 
 
@@ -106,6 +106,7 @@ def load():
     lib.ty_sync.argtypes = [vp]
     lib.ty_timing.argtypes = [vp, C.c_int32, F64P, I32P]
     lib.ty_mesh_info.argtypes = [vp, I32P, I32P, I32P, I32P]
+    lib.ty_cell_order.argtypes = [vp, I32P]
     lib.ty_jacobian_diag.argtypes = [vp, C.c_double, F64P]
     lib.ty_jvp.argtypes = [vp, C.c_double, F64P, F64P]
     lib.ty_linear_solve.argtypes = [vp, C.c_double, C.POINTER(TyKrylov), F64P, F64P, C.POINTER(TyKrylovStats)]

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtyphon_gpu.so")
 SOURCES = ["ty_adjoint.hip", "ty_kernels_f64.hip", "ty_kernels_f32.hip", "ty_krylov.hip", "ty_setup.hip",
-           "ty_probe.hip", "ty_meshbuild.hip", "ty_host.cpp"]
+           "ty_probe.hip", "ty_meshbuild.hip", "ty_order.hip", "ty_host.cpp"]
 GEN = os.path.join(HERE, "_build")  # generated instantiation units (not sources of their own)
 
 # The residual kernels' variants (ty_kernels_impl.h: rec_variant<DIM, R, NSM, DENSE, GOUT>,

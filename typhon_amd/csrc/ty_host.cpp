@@ -248,60 +248,6 @@ void parallel_for(int n, Fn fn)
     for (auto& x : th) x.join();
 }
 
-// reverse Cuthill-McKee on the interior-cell graph
-std::vector<int> rcm(int n, const std::vector<int>& xadj, const std::vector<int>& adj)
-{
-    std::vector<int> deg(n);
-    for (int i = 0; i < n; ++i) deg[i] = xadj[i + 1] - xadj[i];
-    std::vector<int> mark(n, -1), order;
-    order.reserve(n);
-    std::vector<int> q(n), lvl(n);
-    int stamp = 0, scan = 0;
-    std::vector<char> done(n, 0);
-    auto bfs_far = [&](int s) {
-        // BFS from s over unfinished nodes; returns min-degree node of the last level
-        ++stamp;
-        int h = 0, t = 0;
-        q[t++] = s;
-        mark[s] = stamp;
-        lvl[s] = 0;
-        int last = 0;
-        while (h < t) {
-            int u = q[h++];
-            last = lvl[u];
-            for (int k = xadj[u]; k < xadj[u + 1]; ++k) {
-                int v = adj[k];
-                if (done[v] || mark[v] == stamp) continue;
-                mark[v] = stamp;
-                lvl[v] = lvl[u] + 1;
-                q[t++] = v;
-            }
-        }
-        int best = s;
-        for (int i = 0; i < t; ++i)
-            if (lvl[q[i]] == last && deg[q[i]] < deg[best]) best = q[i];
-        return best;
-    };
-    std::vector<int> nb;
-    while ((int)order.size() < n) {
-        while (done[scan]) ++scan;
-        int s = bfs_far(bfs_far(scan));
-        size_t head = order.size();
-        order.push_back(s);
-        done[s] = 1;
-        while (head < order.size()) {
-            int u = order[head++];
-            nb.clear();
-            for (int k = xadj[u]; k < xadj[u + 1]; ++k)
-                if (!done[adj[k]]) { nb.push_back(adj[k]); done[adj[k]] = 1; }
-            std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] < deg[b]; });
-            for (int v : nb) order.push_back(v);
-        }
-    }
-    std::reverse(order.begin(), order.end());
-    return order;
-}
-
 // Hilbert-curve order of the cell centres (J. Skilling, "Programming the Hilbert
 // curve", AIP Conf. Proc. 707 (2004) 381: axes -> transposed index, then bit interleave)
 int hilbert_order(ty_ctx* c, int n, const double* centre, int dim, std::vector<int>& order)
@@ -458,15 +404,19 @@ int ty_mesh_upload(ty_ctx* c, const ty_mesh_desc* m)
         order.resize(ni);
         for (int i = 0; i < ni; ++i) order[i] = i;
     } else {
-        std::vector<int> xadj(ni + 1, 0), adj;
+        // reverse Cuthill-McKee over the graph of interior faces and merged periodic pairs, on the
+        // device; adjacency lists in this edge order
+        std::vector<int> ea, eb;
+        ea.reserve(faces.size() + pfaces.size());
+        eb.reserve(faces.size() + pfaces.size());
         for (auto* L : {&faces, &pfaces})
-            for (auto& d : *L) { xadj[d.a + 1]++; xadj[d.b + 1]++; }
-        for (int i = 0; i < ni; ++i) xadj[i + 1] += xadj[i];
-        adj.resize(xadj[ni]);
-        std::vector<int> pos(xadj.begin(), xadj.end() - 1);
-        for (auto* L : {&faces, &pfaces})
-            for (auto& d : *L) { adj[pos[d.a]++] = d.b; adj[pos[d.b]++] = d.a; }
-        order = rcm(ni, xadj, adj);
+            for (auto& d : *L) {
+                ea.push_back(d.a);
+                eb.push_back(d.b);
+            }
+        bool degree_ok = true;
+        HIPCHK(c, rcm_order_device(ni, ea, eb, order, degree_ok, c->stream));
+        if (!degree_ok) return fail(c, TY_ERR_UNSUPPORTED, "reverse Cuthill-McKee order: a cell with 4096 neighbours or more");
     }
     phase("mesh: cell order");
     c->perm = order;
@@ -653,6 +603,13 @@ int ty_mesh_info(const ty_ctx* c, int32_t* ncell, int32_t* nface_dev, int32_t* n
     if (nface_dev) *nface_dev = c->nf;
     if (nslot) *nslot = c->ns;
     if (nbface_dev) *nbface_dev = c->nb;
+    return TY_OK;
+}
+
+int ty_cell_order(const ty_ctx* c, int32_t* cell)
+{
+    if (!c || !c->mesh_ok || !cell) return TY_ERR_ARG;
+    for (int i = 0; i < c->n; ++i) cell[i] = c->perm[i] + 1;
     return TY_OK;
 }
 

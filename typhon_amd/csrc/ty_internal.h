@@ -1,6 +1,7 @@
 // Internal declarations shared by the HIP kernels (ty_kernels_*.hip) and the
 // host side of the C-ABI (ty_host.cpp).  gfx950 only.
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -250,5 +251,9 @@ hipError_t launch_geo_pack(int n, int ns, const Geo<R>& g, const int* link, type
 // stable ascending device sort of n host keys over bits [0, end_bit): idx_out (host) gets the
 // input position of each key in sorted order (ty_setup.hip)
 hipError_t sort_keys_device(const uint64_t* keys, int n, int end_bit, int* idx_out, hipStream_t st);
+// reverse Cuthill-McKee order of an n-node graph given by its edges (ea[e], eb[e]): order[i] = the node
+// placed i-th (ty_order.hip); degree_ok false (and no order) if a node has 4096 edges or more
+hipError_t rcm_order_device(int n, const std::vector<int>& ea, const std::vector<int>& eb, std::vector<int>& order,
+                            bool& degree_ok, hipStream_t st);
 
 }  // namespace ty

@@ -317,6 +317,12 @@ class Solver:
         self._check(self.lib.ty_mesh_info(self.h, *[C.byref(x) for x in v]), "ty_mesh_info")
         return dict(ncell=v[0].value, nface=v[1].value, nslot=v[2].value, nbface=v[3].value)
 
+    def cell_order(self):
+        """the device cell order: the caller's (0-based) cell index of each device cell"""
+        out = np.empty(self.ni, dtype=np.int32)
+        self._check(self.lib.ty_cell_order(self.h, _p32(out)), "ty_cell_order")
+        return out - 1
+
 
 def krylov(nkrylov=30, max_restart=4, max_outer=8, inner_precision=1, precond=1, inner_tol=1e-3, outer_tol=1e-8):
     """Settings of the matrix-free implicit solve (ty_krylov): GMRES(nkrylov) with block-Jacobi,
