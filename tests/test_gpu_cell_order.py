@@ -108,7 +108,19 @@ MESHES = {
     "periodic hexes": lambda: M.box_hex(8, jitter=0.1),
     "channel hexes": lambda: M.box_hex(7, periodic=(True, False, True)),
     "two boxes": two_boxes,
+    "quad20x20.tym": lambda: _tym("quad20x20.tym"),
 }
+
+
+def _tym(name):
+    import os
+    from typhon_amd import tyfmt
+    here = os.path.dirname(os.path.abspath(__file__))
+    bocos = [dict(family="IMIN", type=M.BC_PERIOD, partner="IMAX", period=(1.0, 0.0, 0.0)),
+             dict(family="IMAX", type=M.BC_PERIOD, partner="IMIN", period=(-1.0, 0.0, 0.0)),
+             dict(family="JMIN", type=M.BC_PERIOD, partner="JMAX", period=(0.0, 1.0, 0.0)),
+             dict(family="JMAX", type=M.BC_PERIOD, partner="JMIN", period=(0.0, -1.0, 0.0))]
+    return tyfmt.tym_ustmesh(os.path.join(here, "golden", "meshes", name), bocos)
 
 
 def _solver(m, order):

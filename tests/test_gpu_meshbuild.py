@@ -59,6 +59,15 @@ def tri_channel(nx=14, ny=9, lx=1.4, ly=0.9, jitter=0.2, seed=5, device=None):
     return M.build_ustmesh(2, vtx, cells, "tri", fam, bocos, None, device)
 
 
+def _tym_mixing_layer(device=None):
+    """the reference's mxlay2x20quad-h50.tym: periodic in x, moving adiabatic walls"""
+    bocos = [dict(family="IMIN", type=M.BC_PERIOD, partner="IMAX", period=(1.0, 0.0, 0.0)),
+             dict(family="IMAX", type=M.BC_PERIOD, partner="IMIN", period=(-1.0, 0.0, 0.0)),
+             dict(family="JMIN", type=M.BC_WALL_ADIAB, param=[0.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0.0]),
+             dict(family="JMAX", type=M.BC_WALL_ADIAB, param=[0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0])]
+    return tyfmt.tym_ustmesh(os.path.join(HERE, "golden", "meshes", "mxlay2x20quad-h50.tym"), bocos, device=device)
+
+
 def _tym_periodic(device=None):
     bocos = [dict(family="IMIN", type=M.BC_PERIOD, partner="IMAX", period=(1.0, 0.0, 0.0)),
              dict(family="IMAX", type=M.BC_PERIOD, partner="IMIN", period=(-1.0, 0.0, 0.0)),
@@ -72,6 +81,7 @@ MAKERS = {
     "quad channel": lambda device=None: M.channel_quad(24, 12, device=device),
     "periodic quad": lambda device=None: M.periodic_quad(10, 6, jitter=0.1, device=device),
     "quad20x20.tym periodic": _tym_periodic,
+    "mxlay2x20quad-h50.tym walls": _tym_mixing_layer,
     "periodic tets": lambda device=None: M.box_tets(6, device=device),
     "walled tets": lambda device=None: M.box_tets(5, periodic=False, device=device),
     "bump tets": lambda device=None: M.bump_channel_tets(12, 6, 4, device=device),
