@@ -317,6 +317,63 @@ def adjoint_byte_model(nc, nf, nb, dim=3, nv=5):
     return bm["residual_only"] + 2 * nc * nv * 8
 
 
+def krylov_bytes_section(trace, ncell=160 ** 3, nv=5, pk=4):
+    """Bytes of the FP32 Krylov vector kernels of the implicit profile, launch by launch.  An
+    Arnoldi step is: the preconditioner, the JVP (its FD step size takes one multi-dot of a single
+    vector), the Gram-Schmidt multi-dot over the step's j + 1 basis vectors, two CGS sweeps; a cycle
+    ends with one linear combination.  So the basis size of a step is the number of first CGS sweeps
+    since the cycle's last linear combination, and the multi-dot right before each first sweep is the
+    Gram-Schmidt one."""
+    if not os.path.exists(trace):
+        return []
+    rows = sorted(_csv_rows(trace), key=lambda r: int(r["Start_Timestamp"]))
+    vec = ncell * nv * pk
+    impl = alg = 0.0
+    t_ns = 0
+    steps = 0
+    j1 = 0  # basis vectors of the current Arnoldi step
+    last_dot = None
+    for r in rows:
+        name = r["Kernel_Name"]
+        dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        if "k_multidot<float>" in name:
+            if last_dot is not None:  # the previous one was a single-vector dot
+                impl += vec
+            last_dot = dur
+            t_ns += dur
+        elif "k_cgs_pass<float, true>" in name:
+            j1 += 1
+            steps += 1
+            impl += (j1 + 1) * vec + (j1 + 2) * vec  # its multi-dot (w, V_0..V_j) and this sweep
+            alg += 2 * j1 * vec  # the model's orthogonalisation: 2 j vectors per step
+            last_dot = None
+            t_ns += dur
+        elif "k_cgs_pass<float, false>" in name:
+            impl += (j1 + 2) * vec
+            t_ns += dur
+        elif "k_lincomb<float>" in name:
+            impl += (j1 + 1) * vec
+            alg += (j1 + 1) * vec
+            t_ns += dur
+            j1 = 0
+    if last_dot is not None:
+        impl += vec
+    if not steps or not t_ns:
+        return []
+    t = t_ns * 1e-9
+    return ["", "## FP32 Krylov vector kernels (multi-dots, the two CGS2 sweeps, the cycle's linear combination)", "",
+            f"{steps} Arnoldi steps in this run, {t * 1e3:.1f} ms in these kernels ({t / steps * 1e3:.3f} ms per step); "
+            f"a vector is {vec / 1e6:.1f} MB (5 x 4,096,000 FP32).",
+            f"- bytes of the implementation (each Gram-Schmidt launch reads its j + 1 basis vectors and w, the "
+            f"sweeps also write w): {impl / 1e9:.1f} GB, {impl / t / 1e12:.2f} TB/s (the first basis vectors of a "
+            f"cycle are still in the 256 MB last-level cache);",
+            f"- algorithmic bytes of the Krylov model (2 j vectors per step for the orthogonalisation, the "
+            f"correction's linear combination): {alg / 1e9:.1f} GB, {alg / t / 1e12:.2f} TB/s, "
+            f"{alg / t / 8e12:.3f} of the 8 TB/s peak.",
+            "CGS2 reads the basis three times per step where the model's modified Gram-Schmidt reads it twice; "
+            "it needs one host round trip per step instead of j + 1."]
+
+
 def summarize_profiles(d, n=100, ns_n=256):
     """Turn the rocprofv3 CSVs under d (made by the PROFILE_CMDS commands) into the committed
     summaries under profiles/: per-kernel durations, HBM bytes per launch from FETCH_SIZE (KiB) and
@@ -456,6 +513,7 @@ def summarize_profiles(d, n=100, ns_n=256):
                 continue
             il.append(f"| {r['Name'].replace('|', '/')[:120]} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                       f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+        il += krylov_bytes_section(os.path.join(d, "imp", "imp_kernel_trace.csv"))
         with open(os.path.join(ROOT, "profiles", "implicit_kernels_c3.md"), "w") as f:
             f.write("\n".join(il) + "\n")
     print("\n".join(lines[:40]))

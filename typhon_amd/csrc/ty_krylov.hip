@@ -130,26 +130,28 @@ __global__ __launch_bounds__(KB) void k_reduce_partials(const double* __restrict
 // element is owned by the thread k_multidot gives it and the partials equal what k_multidot
 // would return on the stored w, bit for bit; the basis is read once instead of twice.  The norm
 // form runs on a larger grid (fixed, so its sum is repeatable too).
-template <typename T, bool DOTS>
+// NM >= m bounds the basis held in registers: the launcher picks 8, 16, 24 or MAXK, so that short
+// bases (the first steps of every cycle) run with fewer registers and more waves in flight.
+template <typename T, bool DOTS, int NM>
 __global__ __launch_bounds__(KB) void k_cgs_pass(T* __restrict__ w, const T* __restrict__ V, long stride,
                                                  const double* __restrict__ h, int m, long len,
                                                  double* __restrict__ partial)
 {
-    constexpr int NA = DOTS ? MAXK : 1;
+    constexpr int NA = DOTS ? NM : 1;
     __shared__ double red[KB / 64][NA];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double hr[MAXK], acc[NA];
+    double hr[NM], acc[NA];
 #pragma unroll
-    for (int i = 0; i < MAXK; ++i) hr[i] = i < m ? h[i] : 0.0;
+    for (int i = 0; i < NM; ++i) hr[i] = i < m ? h[i] : 0.0;
 #pragma unroll
     for (int i = 0; i < NA; ++i) acc[i] = 0.0;
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long)gridDim.x * blockDim.x) {
-        T vi[MAXK];
+        T vi[NM];
         double s = (double)w[e];
         // the basis in chunks of CH vectors: each chunk's loads issued together (indices clamped to
         // the last vector, so no load waits behind a branch), applied in vector order
 #pragma unroll
-        for (int c0 = 0; c0 < MAXK; c0 += CH) {
+        for (int c0 = 0; c0 < NM; c0 += CH) {
             if (c0 >= m) break;
             T vc[CH];
 #pragma unroll
@@ -287,8 +289,15 @@ hipError_t cgs_pass(T* w, const T* V, long stride, const double* h, int m, bool 
                     double* out, hipStream_t s)
 {
     const int nb = dots ? multidot_blocks(len) : norm_pass_blocks(len);
-    if (dots) hipLaunchKernelGGL((k_cgs_pass<T, true>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
-    else hipLaunchKernelGGL((k_cgs_pass<T, false>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
+    if (dots && m <= 8)
+        hipLaunchKernelGGL((k_cgs_pass<T, true, 8>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
+    else if (dots && m <= 16)
+        hipLaunchKernelGGL((k_cgs_pass<T, true, 16>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
+    else if (dots && m <= 24)
+        hipLaunchKernelGGL((k_cgs_pass<T, true, 24>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
+    else if (dots)
+        hipLaunchKernelGGL((k_cgs_pass<T, true, MAXK>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
+    else hipLaunchKernelGGL((k_cgs_pass<T, false, MAXK>), dim3(nb), dim3(KB), 0, s, w, V, stride, h, m, len, partial);
     hipLaunchKernelGGL(k_reduce_partials, dim3(dots ? m : 1), dim3(KB), 0, s, (const double*)partial, nb,
                        dots ? m : 1, out);
     return hipGetLastError();
