@@ -9,14 +9,19 @@ device included); Mcell-updates/s = cells * 4 * steps / time.  `value` is the FP
 FP32+FP64-accumulate form, its per-field error after 1000 steps and the other workloads are
 reported beside it:
 
-- c1: 2D channel, 256 x 128 jittered quads, first-order HLLC + RK2, 500 steps, FP64 (the CPU
-  oracle runs it at full size beside the GPU);
-- adjoint / c4: the 249,480-tet bump channel (first-order HLLC + RK2): primal rate,
-  adjoint/primal kernel time, and the wall-force gradient (inflow and 64 wall vertices) against
-  central differences and the tangent-linear product;
-- implicit: 160^3 = 4.1M-hex Navier-Stokes Taylor-Green, matrix-free backward Euler;
+- c1: 2D channel, 256 x 128 jittered quads, first-order HLLC + RK2, 500 steps, FP64, with its
+  flux kernel's roofline (the CPU oracle runs it at full size beside the GPU);
+- adjoint / c4: the 249,480-tet bump channel (first-order HLLC + RK2): primal rate and roofline,
+  adjoint/primal kernel time, and the wall-force gradient (64 wall vertices after 20 steps, the
+  inflow state after 400) against central differences and the tangent-linear product;
+- implicit: 160^3 = 4.1M-hex Navier-Stokes Taylor-Green, matrix-free backward Euler, with the
+  roofline of the JVP's residual evaluation;
 - ns_16m: 256^3 = 16.8M-hex Navier-Stokes, RK4, FP64 and mixed, its own roofline and CPU
-  baseline, adjoint/primal ratio and a 200-step checkpointed adjoint.
+  baseline, adjoint/primal ratio, a 200-step checkpointed adjoint, and the device mesh build of
+  its box beside the host (numpy) build of a 64^3 sample.
+
+The large meshes are built on the device (faces and geometry: mesh.build_ustmesh(..., device=0),
+the host build's arrays bit for bit).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 100] [--no-cpu]
 """
