@@ -346,14 +346,14 @@ def krylov_bytes_section(trace, ncell=160 ** 3, nv=5, pk=4):
                 impl += vec
             last_dot = dur
             t_ns += dur
-        elif "k_cgs_pass<float, true>" in name:
+        elif "k_cgs_pass<float, true" in name:
             j1 += 1
             steps += 1
             impl += (j1 + 1) * vec + (j1 + 2) * vec  # its multi-dot (w, V_0..V_j) and this sweep
             alg += 2 * j1 * vec  # the model's orthogonalisation: 2 j vectors per step
             last_dot = None
             t_ns += dur
-        elif "k_cgs_pass<float, false>" in name:
+        elif "k_cgs_pass<float, false" in name:
             impl += (j1 + 2) * vec
             t_ns += dur
         elif "k_lincomb<float>" in name:

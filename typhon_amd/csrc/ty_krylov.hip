@@ -66,8 +66,10 @@ __global__ void k_block_apply(const T* __restrict__ dinv, const T* __restrict__ 
 constexpr int MG = 8;
 constexpr int CH = 8;  // basis vectors loaded together by k_cgs_pass and k_lincomb
 // the fixed grid of the multi-dot and the fused CGS dot pass (the same grid, so their partial sums
-// agree bit for bit); 2048 and 4096 blocks measured no faster
-constexpr long MD_BLOCKS = 1024;
+// agree bit for bit).  With the dot pass's registers bounded by the basis size, 4096 blocks keep more
+// waves in flight than 1024 (implicit 4.1M-hex step 117.9 -> 113.2 ms); 2048, 8192 and 16384 measured
+// 117.4, 114.7 and 119.1 ms
+constexpr long MD_BLOCKS = 4096;
 template <typename T>
 __global__ __launch_bounds__(KB) void k_multidot(const T* __restrict__ w, const T* __restrict__ V, long stride,
                                                  int m, long len, double* __restrict__ partial)
