@@ -126,9 +126,10 @@ int ty_create(ty_ctx** ctx, int device);
 int ty_destroy(ty_ctx* ctx);
 const char* ty_last_error(const ty_ctx* ctx);
 
-/* mesh upload once (after init_world); builds the device layout: RCM-ordered
- * cells, periodic face pairs merged, cell->face slot lists.  Replaces the
- * mesh side of calc_rhs (calc_rhs.f90:48-145). */
+/* mesh upload once (after init_world); builds the device layout: cells renumbered
+ * (mesh->cell_order: reverse Cuthill-McKee or Hilbert, computed on the device),
+ * periodic face pairs merged, cell->face slot lists.  Replaces the mesh side of
+ * calc_rhs (calc_rhs.f90:48-145). */
 int ty_mesh_upload(ty_ctx* ctx, const ty_mesh_desc* mesh);
 
 /* model and scheme: the resolved mnu_solver / mnu_spat / mnu_time menus */
