@@ -1507,7 +1507,7 @@ int alloc_krylov(ty_ctx* c, int m, int prec)
         if ((rc = dalloc(c, &c->k_b, len)) || (rc = dalloc(c, &c->k_x, len)) || (rc = dalloc(c, &c->k_r, len)) ||
             (rc = dalloc(c, &c->k_r0, len)) || (rc = dalloc(c, &c->k_r1, len)) || (rc = dalloc(c, &c->k_r2, len)) ||
             (rc = dalloc(c, &c->k_pd, len)) || (rc = dalloc(c, &c->k_dx, len)) || (rc = dalloc(c, &c->k_pf, len)) ||
-            (rc = dalloc(c, &c->k_part, (size_t)krylov_partials(len))) || (rc = dalloc(c, &c->k_h, 3 * MAXK)))
+            (rc = dalloc(c, &c->k_part, (size_t)krylov_partials(len))) || (rc = dalloc(c, &c->k_h, 3 * MAXK + 4)))
             return rc;
         c->kry_base = true;
     }
@@ -1578,19 +1578,17 @@ int gmres(ty_ctx* c, const double* rhs, const ty_krylov* k, double unorm, int* i
     std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m), hh(MAXK), h2(MAXK), hbuf(3 * MAXK);
     double res = beta0;
     HIPCHK(c, hipMemsetAsync(c->k_dx, 0, len * sizeof(double), c->stream));
-    auto A_apply = [&](const P* zin, P* wout) -> int {  // wout = zin - J zin
-        double zz;
-        int r2;
-        if ((r2 = dot_host<P>(c, zin, zin, len, &zz))) return r2;
-        const double zn = std::sqrt(zz / (double)len);
-        if (zn == 0.0) {
-            HIPCHK(c, hipMemcpyAsync(wout, zin, len * sizeof(P), hipMemcpyDeviceToDevice, c->stream));
-            return TY_OK;
-        }
-        const double eps = sq * (1.0 + unorm) / zn;
-        HIPCHK(c, (fill_state<P, P>(pert, c->d_u0, eps, zin, c->nv, c->n, c->gas.gamma, c->stream)));
+    // wout = zin - J zin; the difference step eps = sqrt(machine eps) (1 + ||u||) / ||zin|| (RMS norms)
+    // is formed on the device from the device norm, so the step needs no host round trip; for
+    // zin = 0 both eps and 1/eps are 0 and wout = zin
+    double* d_zz = c->k_h + 3 * MAXK;
+    double* d_eps = d_zz + 1;  // eps, 1 / eps
+    auto A_apply = [&](const P* zin, P* wout) -> int {
+        HIPCHK(c, multidot<P>(zin, zin, 0, 1, len, c->k_part, d_zz, c->stream));
+        HIPCHK(c, jvp_step(d_zz, len, sq * (1.0 + unorm), d_eps, c->stream));
+        HIPCHK(c, (fill_state<P, P>(pert, c->d_u0, 0.0, zin, c->nv, c->n, c->gas.gamma, c->stream, d_eps)));
         HIPCHK(c, eval_residual<P>(c, pert, c->k_r1, nullptr, nullptr));
-        HIPCHK(c, apply_a<P>(wout, zin, c->k_r1, c->k_r0, 1.0 / eps, len, c->stream));
+        HIPCHK(c, apply_a<P>(wout, zin, c->k_r1, c->k_r0, 0.0, len, c->stream, d_eps + 1));
         return TY_OK;
     };
     for (int cycle = 0; cycle < std::max(1, k->max_restart); ++cycle) {

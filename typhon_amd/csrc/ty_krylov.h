@@ -8,11 +8,16 @@ constexpr int MAXK = 32;  // Krylov basis size limit (GMRES restart length + 1)
 
 int multidot_blocks(long len);
 long krylov_partials(long len);  // doubles of partial-sum space the reductions below need
-// stage-state records dst[c][v] = (P)(u[v][c] + a v[v][c]) for the residual kernels
+// stage-state records dst[c][v] = (P)(u[v][c] + a v[v][c]) for the residual kernels (ap: a on the device)
 template <typename P, typename T>
-hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, double gamma, hipStream_t s);
+hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, double gamma, hipStream_t s,
+                      const double* ap = nullptr);
+// w = z - (r1 - r0) sc (sp: sc on the device)
 template <typename T>
-hipError_t apply_a(T* w, const T* z, const double* r1, const double* r0, double sc, long len, hipStream_t s);
+hipError_t apply_a(T* w, const T* z, const double* r1, const double* r0, double sc, long len, hipStream_t s,
+                   const double* sp = nullptr);
+// e[0] = sq1u / sqrt(*zz / len) and e[1] = 1 / e[0] on the device (0 and 0 for *zz = 0)
+hipError_t jvp_step(const double* zz, long len, double sq1u, double* e, hipStream_t s);
 template <typename T>
 hipError_t block_apply(int nv, const T* dinv, const T* v, T* z, int n, hipStream_t s);
 template <typename T>

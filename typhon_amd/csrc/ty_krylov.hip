@@ -19,12 +19,14 @@ __device__ __forceinline__ double wsum(double v)
 
 // dst[c] = the stage-state (primitive) record of u[.][c] + a * v[.][c]: a perturbed state,
 // pressure recovered in FP64 (g1 = gamma - 1)
+// (ap, when given, holds a on the device)
 template <typename P, typename T>
-__global__ void k_fill_state(P* __restrict__ dst, const double* __restrict__ u, double a, const T* __restrict__ v,
-                             int nv, int n, double g1)
+__global__ void k_fill_state(P* __restrict__ dst, const double* __restrict__ u, double a, const double* __restrict__ ap,
+                             const T* __restrict__ v, int nv, int n, double g1)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
+    if (ap) a = *ap;
     double w[NVMAX];
     for (int k = 0; k < nv; ++k) {
         const size_t i = (size_t)k * n + c;
@@ -35,12 +37,26 @@ __global__ void k_fill_state(P* __restrict__ dst, const double* __restrict__ u, 
 
 // w = z - (r1 - r0) * s     (A z = z - J z with J z from a one-sided difference)
 // central form: w = z - (r1 - r0) * s with r0 the residual at u - eps z
+// (sp, when given, holds s on the device)
 template <typename T>
 __global__ void k_apply_a(T* __restrict__ w, const T* __restrict__ z, const double* __restrict__ r1,
-                          const double* __restrict__ r0, double s, long len)
+                          const double* __restrict__ r0, double s, const double* __restrict__ sp, long len)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sp) s = *sp;
     if (i < len) w[i] = (T)((double)z[i] - (r1[i] - r0[i]) * s);
+}
+
+// the one-sided JVP's step from a norm on the device: e[0] = eps = sq1u / sqrt(zz / len) (sq1u =
+// sqrt(machine eps) (1 + ||u||)), e[1] = 1 / eps; both 0 for zz = 0, where the difference then
+// vanishes and A z = z
+__global__ void k_jvp_step(const double* __restrict__ zz, double len, double sq1u, double* __restrict__ e)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double zn = sqrt(*zz / len);
+    const double eps = zn == 0.0 ? 0.0 : sq1u / zn;
+    e[0] = eps;
+    e[1] = zn == 0.0 ? 0.0 : 1.0 / eps;
 }
 
 // z = Dinv v per cell (nv x nv blocks, [nv*nv][n] row-major entries)
@@ -259,15 +275,22 @@ long krylov_partials(long len)
 }
 
 template <typename P, typename T>
-hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, double gamma, hipStream_t s)
+hipError_t fill_state(P* dst, const double* u, double a, const T* v, int nv, int n, double gamma, hipStream_t s,
+                      const double* ap)
 {
-    hipLaunchKernelGGL((k_fill_state<P, T>), dim3(nb_of(n)), dim3(KB), 0, s, dst, u, a, v, nv, n, gamma - 1.0);
+    hipLaunchKernelGGL((k_fill_state<P, T>), dim3(nb_of(n)), dim3(KB), 0, s, dst, u, a, ap, v, nv, n, gamma - 1.0);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t apply_a(T* w, const T* z, const double* r1, const double* r0, double sc, long len, hipStream_t s)
+hipError_t apply_a(T* w, const T* z, const double* r1, const double* r0, double sc, long len, hipStream_t s,
+                   const double* sp)
 {
-    hipLaunchKernelGGL((k_apply_a<T>), dim3(nb_of(len)), dim3(KB), 0, s, w, z, r1, r0, sc, len);
+    hipLaunchKernelGGL((k_apply_a<T>), dim3(nb_of(len)), dim3(KB), 0, s, w, z, r1, r0, sc, sp, len);
+    return hipGetLastError();
+}
+hipError_t jvp_step(const double* zz, long len, double sq1u, double* e, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_jvp_step, dim3(1), dim3(64), 0, s, zz, (double)len, sq1u, e);
     return hipGetLastError();
 }
 template <typename T>
@@ -447,7 +470,7 @@ hipError_t asum(const double* x, long len, double* partial, double* out, hipStre
 }
 
 #define TY_KINST(T)                                                                                       \
-    template hipError_t apply_a<T>(T*, const T*, const double*, const double*, double, long, hipStream_t); \
+    template hipError_t apply_a<T>(T*, const T*, const double*, const double*, double, long, hipStream_t, const double*); \
     template hipError_t block_apply<T>(int, const T*, const T*, T*, int, hipStream_t);                     \
     template hipError_t multidot<T>(const T*, const T*, long, int, long, double*, double*, hipStream_t);   \
     template hipError_t cgs_pass<T>(T*, const T*, long, const double*, int, bool, long, double*, double*,   \
@@ -456,10 +479,10 @@ hipError_t asum(const double* x, long len, double* partial, double* out, hipStre
     template hipError_t accumulate<T>(double*, const T*, long, hipStream_t);
 TY_KINST(float)
 TY_KINST(double)
-template hipError_t fill_state<float, float>(float*, const double*, double, const float*, int, int, double, hipStream_t);
-template hipError_t fill_state<float, double>(float*, const double*, double, const double*, int, int, double, hipStream_t);
-template hipError_t fill_state<double, double>(double*, const double*, double, const double*, int, int, double, hipStream_t);
-template hipError_t fill_state<double, float>(double*, const double*, double, const float*, int, int, double, hipStream_t);
+template hipError_t fill_state<float, float>(float*, const double*, double, const float*, int, int, double, hipStream_t, const double*);
+template hipError_t fill_state<float, double>(float*, const double*, double, const double*, int, int, double, hipStream_t, const double*);
+template hipError_t fill_state<double, double>(double*, const double*, double, const double*, int, int, double, hipStream_t, const double*);
+template hipError_t fill_state<double, float>(double*, const double*, double, const float*, int, int, double, hipStream_t, const double*);
 template hipError_t scale<float, float>(float*, const float*, double, long, hipStream_t);
 template hipError_t scale<float, double>(float*, const double*, double, long, hipStream_t);
 template hipError_t scale<double, double>(double*, const double*, double, long, hipStream_t);
