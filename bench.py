@@ -484,7 +484,7 @@ def summarize_profiles(d, n=100, ns_n=256):
     al = ["# Adjoint kernels of the 16.8M-hex NS residual (FP64): durations and HBM traffic", "",
           "One adjoint evaluation (lambda -> (dR/dU)^T lambda) runs the face kernel (two-sided faces, then boundary "
           "faces), the boundary-cell kernel, the gather (with the LSQ-gradient transpose) and the tail gather; "
-          "k_adj_face_geom builds the per-face viscous geometry once per work buffer.  Algorithmic bytes of one "
+          "k_adj_face_geom builds the per-face viscous geometry on the first call after ty_set_scheme.  Algorithmic bytes of one "
           "evaluation: the primal residual's reads (gradient and flux passes) plus lambda and the result, "
           f"{alg / ncn:.0f} B per cell = {alg / 1e9:.2f} GB.", ""] + head
     al += ["| kernel | avg us | read GB | written GB | HBM TB/s |", "|---|---|---|---|---|"]

@@ -94,6 +94,38 @@ def test_adjoint_without_volume_sensitivities(name):
     assert np.array_equal(full["u"], part["u"]) and np.array_equal(full["q"], part["q"])
 
 
+def test_adjoint_face_geometry_kept_across_calls():
+    """the per-face viscous geometry is built on the first adjoint call and kept: a second call, a
+    call after ty_set_scheme and one after a re-upload of another mesh all give a fresh context's
+    adjoint, bit for bit"""
+    m, s, g, u, gpu, dt = _setup("channel_full")
+    lam = np.random.default_rng(13).normal(size=(5, m.ncell_int))
+    first = gpu.residual_adjoint(dt, lam, area_vectors=False)
+    again = gpu.residual_adjoint(dt, lam, area_vectors=False)
+    gpu.set_scheme(gpu.scheme, gpu.gas)
+    gpu.set_state(u)
+    reset = gpu.residual_adjoint(dt, lam, area_vectors=False)
+    for k in ("u", "vol", "q"):
+        assert np.array_equal(first[k], again[k]) and np.array_equal(first[k], reset[k]), k
+    m2, s2, g2, u2 = CASES["bump_full"]()
+    u2 = _noisy(m2, u2)
+    fresh = Solver(m2, s2, g2)
+    try:
+        fresh.set_state(u2)
+        dt2 = fresh.timestep()
+        lam2 = np.random.default_rng(14).normal(size=(5, m2.ncell_int))
+        want = fresh.residual_adjoint(dt2, lam2, area_vectors=False)
+    finally:
+        fresh.close()
+    gpu.set_scheme(s2, g2)
+    gpu.upload(m2)
+    gpu.set_state(u2)
+    got = gpu.residual_adjoint(dt2, lam2, area_vectors=False)
+    gpu.close()
+    for k in ("u", "vol", "q"):
+        assert np.array_equal(got[k], want[k]), k
+
+
 def test_viscous_area_vector_derivatives_are_refused():
     m, s, g, u, gpu, dt = _setup("channel_full")
     with pytest.raises(Exception, match="area-vector"):

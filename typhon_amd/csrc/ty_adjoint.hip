@@ -811,7 +811,7 @@ hipError_t launch_res_tangent(const AdArgs& t, hipStream_t s)
     return hipGetLastError();
 }
 
-// t.fgeo from the slot geometry: once per work buffer, before the viscous adjoint launches
+// t.fgeo from the slot geometry: once per scheme upload, before the first viscous adjoint launch
 template <int DIM>
 hipError_t launch_adj_face_geom(const AdArgs& t, hipStream_t s)
 {

@@ -127,6 +127,7 @@ struct ty_ctx {
     void *k_V = nullptr, *k_z = nullptr, *k_w = nullptr, *k_t = nullptr, *k_dinv = nullptr;
     double* d_res = nullptr;
     double* d_part = nullptr;
+    double* d_adj_fgeo = nullptr;  // per-face viscous geometry of the adjoint ([4][nf], built on first use)
     int* d_bad = nullptr;
     double* d_dt = nullptr;
     int* d_err = nullptr;
@@ -200,6 +201,7 @@ void free_state(ty_ctx* c)
     c->kry_m = 0;
     c->kry_prec = -1;
     c->d_bad = c->d_err = nullptr;
+    c->d_adj_fgeo = nullptr;
     for (auto* v : {&c->ev_grad, &c->ev_flux, &c->ev_dt, &c->ev_tan, &c->ev_adj}) {
         for (auto& e : *v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         v->clear();
@@ -765,6 +767,7 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
     c->kry_m = 0;
     c->kry_prec = -1;
     c->d_bad = c->d_err = nullptr;
+    c->d_adj_fgeo = nullptr;
     // keep the mesh buffers, drop the rest
     {
         std::vector<void*> keep;
@@ -1979,16 +1982,20 @@ int alloc_adjoint_work(ty_ctx* c, AdArgs& t, DevTmp& tmp)
     t.face = d_face;
     t.lrec = d_lrec;
     if (c->ph.viscous) {
-        double *d_fg = nullptr, *d_sl = nullptr, *d_gx = nullptr, *d_fgr = nullptr;
-        if ((rc = tmp.alloc(&d_fg, (size_t)4 * c->nf)) || (rc = tmp.alloc(&d_sl, (size_t)c->ns * n * nv)) ||
+        double *d_sl = nullptr, *d_gx = nullptr, *d_fgr = nullptr;
+        if ((rc = tmp.alloc(&d_sl, (size_t)c->ns * n * nv)) ||
             (rc = tmp.alloc(&d_gx, (size_t)2 * dim * std::max(1, c->nb))) ||
             (rc = tmp.alloc(&d_fgr, (size_t)c->ns * n * (dim * dim + dim))))
             return rc;
         t.fgrad = d_fgr;
-        t.fgeo = d_fg;
         t.slot = d_sl;
         t.gbx = d_gx;
-        HIPCHK(c, dim == 2 ? launch_adj_face_geom<2>(t, c->stream) : launch_adj_face_geom<3>(t, c->stream));
+        // the per-face geometry depends on the mesh and the scheme's slot geometry only: built on the
+        // first adjoint call after ty_set_scheme and kept (the context's buffer, freed with the scheme)
+        const bool fresh = c->d_adj_fgeo == nullptr;
+        if (fresh && (rc = dalloc(c, &c->d_adj_fgeo, (size_t)4 * c->nf))) return rc;
+        t.fgeo = c->d_adj_fgeo;
+        if (fresh) HIPCHK(c, dim == 2 ? launch_adj_face_geom<2>(t, c->stream) : launch_adj_face_geom<3>(t, c->stream));
     }
     return TY_OK;
 }
