@@ -6,8 +6,8 @@ reverse Cuthill-McKee order is timed beside it), isentropic vortex (strength 5) 
 M=0.5, second-order MUSCL with LSQ gradients and the Venkatakrishnan limiter (K=5), HLLC,
 classical RK4, CFL 0.8.  A step is one RK4 step = 4 residual evaluations (time step on the
 device included); Mcell-updates/s = cells * 4 * steps / time.  `value` is the FP64 form; the
-FP32+FP64-accumulate form, its per-field error after 1000 steps and the other workloads are
-reported beside it:
+mixed form (FP32 fluxes, gradients and stage increments; FP64 state and update), its per-field
+error after 1000 steps and the other workloads are reported beside it:
 
 - c1: 2D channel, 256 x 128 jittered quads, first-order HLLC + RK2, 500 steps, FP64, with its
   flux kernel's roofline (the CPU oracle runs it at full size beside the GPU);
@@ -47,8 +47,10 @@ def byte_model(nc, nf, nb, dim=3, nv=5, mixed=False, rk4=True, gradients=True):
     """Algorithmic HBM bytes per launch of the two residual kernels (compulsory traffic, each
     array once per pass, int32 indices): gradient+limiter pass and flux+gather+RK-update pass.
     gradients=False: a first-order inviscid residual (no gradient pass); rk4=False: the RK2
-    (midpoint) combine instead of RK4's."""
-    ps = pg = px = 4 if mixed else 8
+    (midpoint) combine instead of RK4's.  RK4 per step and value: U0 read by each stage and written
+    by the last (5 x 8 B), the stage-increment sum written by stages 1-3 and read by 2-4 (6 x 4 B
+    mixed, 6 x 8 B FP64) and three stage states (3 x 4 or 8 B), a quarter of it per launch."""
+    ps = pg = px = pa = 4 if mixed else 8
     E = 2 * nf - nb
     b_grad = nc * (nv * ps + 8 * dim + dim * (dim + 1) // 2 * pg + dim * nv * pg + 4) + 8 * nf + 4 * E
     b_flux = nc * (nv * ps + 8 + 4) + nf * (dim * px + 8) + 4 * E
@@ -56,7 +58,7 @@ def byte_model(nc, nf, nb, dim=3, nv=5, mixed=False, rk4=True, gradients=True):
         b_flux += nc * (dim * nv * pg + 8 * dim) + nf * dim * px
     else:
         b_grad = 0
-    b_rk = nc * nv * (22 + 0.75 * ps) if rk4 else nc * nv * (ps + 16) / 2
+    b_rk = nc * nv * (10 + 1.5 * pa + 0.75 * ps) if rk4 else nc * nv * (ps + 16) / 2
     return dict(grad=b_grad, flux=b_flux + b_rk, residual_only=b_grad + b_flux, total=b_grad + b_flux + b_rk)
 
 
@@ -408,7 +410,7 @@ def summarize_profiles(d, n=100, ns_n=256):
                                                                    byte_model(nc2, nf2, 0)["flux"]),
         "void ty::k_gradient<3, float, 4, 1, true, false>": ("6M tets K1 FP32",
                                                            byte_model(nc2, nf2, 0, mixed=True)["grad"]),
-        "void ty::k_flux<3, float, true, false, 4, false, true>": ("6M tets K2 FP32 (FP64 RK accumulation)",
+        "void ty::k_flux<3, float, true, false, 4, false, true>": ("6M tets K2 FP32 (FP64 U0 update)",
                                                                   byte_model(nc2, nf2, 0, mixed=True)["flux"]),
         "void ty::k_gradient<3, double, 6, 0, true, true>": ("16.8M hexes NS K1 FP64 (LSQ, viscous records)",
                                                            byte_model(ncn, nfn, nbn)["grad"]),
@@ -416,7 +418,7 @@ def summarize_profiles(d, n=100, ns_n=256):
                                                                    byte_model(ncn, nfn, nbn)["flux"]),
         "void ty::k_gradient<3, float, 6, 0, true, true>": ("16.8M hexes NS K1 FP32",
                                                           byte_model(ncn, nfn, nbn, mixed=True)["grad"]),
-        "void ty::k_flux<3, float, false, true, 6, false, true>": ("16.8M hexes NS K2 FP32 (FP64 RK accumulation)",
+        "void ty::k_flux<3, float, false, true, 6, false, true>": ("16.8M hexes NS K2 FP32 (FP64 U0 update)",
                                                                   byte_model(ncn, nfn, nbn, mixed=True)["flux"]),
     }
     adj_kernels = ("k_adj_face_geom<3>", "k_res_adjoint_face<3, true, false>", "k_res_adjoint_face<3, true, true>",
@@ -446,7 +448,7 @@ def summarize_profiles(d, n=100, ns_n=256):
     lines = ["# Residual kernels: durations, HBM traffic and issue counters", "",
              "6M periodic tets (100^3 hexes x 6; MUSCL + LSQ + Venkatakrishnan, HLLC, RK4) and 16.8M hexes (256^3, "
              "NS first-order HLLC + cell-full viscous flux, LSQ gradients, RK4), FP64 and FP32 (mixed: FP64 U0 and "
-             "RK accumulator).  Algorithmic bytes: the byte model of bench.py (each array once per pass, int32 "
+             "update, FP32 stage-increment sum).  Algorithmic bytes: the byte model of bench.py (each array once per pass, int32 "
              "indices).", ""] + head
     lines += ["| kernel | avg us | read GB | written GB | HBM GB | HBM TB/s | algorithmic GB | traffic / algorithmic "
               "| algorithmic TB/s (frac of 8) | VALU / VMEM-read instr per wave | VALU busy |",
@@ -586,7 +588,7 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
     y = 2 pi L (the field's v vanishes there), first-order HLLC + cell-full viscous flux with LSQ
     gradients, RK4 at CFL 0.5, one uploaded mesh for everything below.
 
-    - throughput: FP64 and mixed (FP32 flux/gradient/stage state, FP64 U0 and RK accumulator)
+    - throughput: FP64 and mixed (FP32 flux/gradient/stage state, FP64 U0, FP32 stage-increment sum)
       advance the same initial field by ``steps`` steps; density error of mixed vs FP64;
     - adjoint: J = pressure force normal to the y = 0 wall after ``adj_steps`` FP64 steps,
       dJ/dU0 by the device-resident checkpointed sweep (``snaps`` snapshots, FP64), checked against a central difference of two FP64 forward runs along a seeded
@@ -883,8 +885,8 @@ def main():
         bmm = byte_model(nc, nf_dev, nb_dev, mixed=True)
         fl = tmm["flux_ms"] / max(1, tmm["flux_n"])
         out["mixed"] = {"value": round(nc * evals / tmx / 1e6, 3), "unit": "Mcell-updates/s",
-                        "dtype": "fp32 flux/gradient/stage state and face states; fp64 U0, RK accumulator, "
-                                 "primitive recovery from total energy",
+                        "dtype": "fp32 flux/gradient/stage state, face states and stage-increment sum; fp64 U0, "
+                                 "its update and the primitive recovery from total energy",
                         "speedup_vs_fp64": round(t64 / tmx, 3),
                         "error_after_timed_steps": dict(steps=args.steps + args.warmup,
                                                         fields=field_errors(stmx, st64)),

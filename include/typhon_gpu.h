@@ -27,7 +27,7 @@ extern "C" {
 #define TY_ERR_NOT_CONVERGED 5
 
 #define TY_PREC_FP64 0        /* everything in FP64 */
-#define TY_PREC_MIXED 1       /* FP32 stage state, gradients, fluxes, geometry; FP64 U0/accumulator */
+#define TY_PREC_MIXED 1       /* FP32 stage state, gradients, fluxes, geometry, stage-increment sum; FP64 U0 and update */
 
 typedef struct ty_ctx ty_ctx;
 

@@ -3,8 +3,8 @@
 FP64 bar: 1e-12 relative (max-norm per field) on residuals and on states after
 a few RK steps -- the summation order differs (per-cell slot order on the GPU,
 face order in the oracle, colour order in the reference), so results agree to
-a few ulps, not bitwise.  Mixed precision (FP32 fluxes/gradients, FP64
-accumulation) is checked at the tolerance written in each test.
+a few ulps, not bitwise.  Mixed precision (FP32 fluxes/gradients, FP64 state
+and update) is checked at the tolerance written in each test.
 """
 import numpy as np
 import pytest
@@ -81,7 +81,7 @@ def test_kinetic_fluxes_second_order_steps(sch):
 
 @pytest.mark.parametrize("sch", [S.SCH_HLLCK, S.SCH_HLLEKB])
 def test_kinetic_fluxes_mixed_precision(sch):
-    """FP32 fluxes (erff/expf) with FP64 accumulation: the FP64 state to 1e-5 after 5 steps"""
+    """FP32 fluxes (erff/expf) with the FP64 state update: the FP64 state to 1e-5 after 5 steps"""
     m, s, g, u = K.channel_case(24, 12, sch_hyp=sch)
     res = {}
     for prec in (S.PREC_FP64, S.PREC_MIXED):
@@ -198,7 +198,7 @@ def _field_errors(a, b):
 
 
 def test_mixed_precision_steps():
-    # FP64 accumulation and FP64 pressure recovery: every field within 1e-6 relative of the FP64
+    # FP64 state update and FP64 pressure recovery: every field within 1e-6 relative of the FP64
     # run after 20 RK4 steps (second-order MUSCL + Venkatakrishnan)
     m, s, g, u = K.vortex_case(6, L=6.0)
     a = Solver(m, s, g)

@@ -965,7 +965,7 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     a.us = (const R*)c->d_us[c->cur];
     a.us_next = (R*)c->d_us[c->cur ^ 1];
     a.u0 = c->d_u0;
-    a.acc = c->d_acc;
+    a.acc = (R*)c->d_acc;  // sized for FP64; the FP32 kernels use its first half
     a.rstore = c->d_rstore;
     const int prec = sizeof(R) == sizeof(float) ? 1 : 0;
     a.grad = (const R*)c->work_grad[prec];
