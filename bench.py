@@ -6,8 +6,9 @@ reverse Cuthill-McKee order is timed beside it), isentropic vortex (strength 5) 
 M=0.5, second-order MUSCL with LSQ gradients and the Venkatakrishnan limiter (K=5), HLLC,
 classical RK4, CFL 0.8.  A step is one RK4 step = 4 residual evaluations (time step on the
 device included); Mcell-updates/s = cells * 4 * steps / time.  `value` is the FP64 form; the
-mixed form (FP32 fluxes, gradients and stage increments; FP64 state and update), its per-field
-error after 1000 steps and the other workloads are reported beside it:
+mixed form (FP32 fluxes and gradients; FP64 state, RK accumulation and primitive recovery), the
+mixed form that keeps the stage-increment sum in FP32, their per-field errors after 1000 steps and
+the other workloads are reported beside it:
 
 - c1: 2D channel, 256 x 128 jittered quads, first-order HLLC + RK2, 500 steps, FP64, with its
   flux kernel's roofline (the CPU oracle runs it at full size beside the GPU);
@@ -43,14 +44,16 @@ METRIC = "residual Mcell-updates/s, FP64 vs FP32+refine at stated err; adjoint/p
 HBM_PEAK_GBS = 8000.0
 
 
-def byte_model(nc, nf, nb, dim=3, nv=5, mixed=False, rk4=True, gradients=True):
+def byte_model(nc, nf, nb, dim=3, nv=5, mixed=False, rk4=True, gradients=True, f32sum=False):
     """Algorithmic HBM bytes per launch of the two residual kernels (compulsory traffic, each
     array once per pass, int32 indices): gradient+limiter pass and flux+gather+RK-update pass.
     gradients=False: a first-order inviscid residual (no gradient pass); rk4=False: the RK2
     (midpoint) combine instead of RK4's.  RK4 per step and value: U0 read by each stage and written
-    by the last (5 x 8 B), the stage-increment sum written by stages 1-3 and read by 2-4 (6 x 4 B
-    mixed, 6 x 8 B FP64) and three stage states (3 x 4 or 8 B), a quarter of it per launch."""
-    ps = pg = px = pa = 4 if mixed else 8
+    by the last (5 x 8 B), the stage-increment sum written by stages 1-3 and read by 2-4 (6 x 8 B;
+    6 x 4 B with f32sum, the mixed form that keeps that sum in FP32) and three stage states
+    (3 x 4 or 8 B), a quarter of it per launch."""
+    ps = pg = px = 4 if mixed else 8
+    pa = 4 if mixed and f32sum else 8
     E = 2 * nf - nb
     b_grad = nc * (nv * ps + 8 * dim + dim * (dim + 1) // 2 * pg + dim * nv * pg + 4) + 8 * nf + 4 * E
     b_flux = nc * (nv * ps + 8 + 4) + nf * (dim * px + 8) + 4 * E
@@ -151,18 +154,18 @@ def run_steps(solver, u, steps):
     return solver.get_state()
 
 
-def mixed_accuracy(m, s, g, u, steps=1000, order="hilbert"):
-    """FP64 and mixed forms advanced ``steps`` steps from the same state; per-field errors of mixed
-    against FP64"""
+def mixed_accuracy(m, s, g, u, steps=1000, order="hilbert", forms=(1,)):
+    """FP64 and the mixed ``forms`` (precision codes) advanced ``steps`` steps from the same state;
+    per-field errors of each mixed form against FP64, one dict per form"""
     from typhon_amd import scheme as S
     from typhon_amd.solver import Solver
     st = {}
-    for prec in (S.PREC_FP64, S.PREC_MIXED):
+    for prec in (S.PREC_FP64,) + tuple(forms):
         solver = Solver(m, dict(s, precision=prec), g, cell_order=order)
         st[prec] = run_steps(solver, u, steps)
         solver.close()
-    return dict(steps=steps, reference="the FP64 GPU form (parity-tied to the oracle at 1e-12)",
-                fields=field_errors(st[S.PREC_MIXED], st[S.PREC_FP64]))
+    return [dict(steps=steps, reference="the FP64 GPU form (parity-tied to the oracle at 1e-12)",
+                 fields=field_errors(st[f], st[S.PREC_FP64])) for f in forms]
 
 
 def c1_bench(steps=500, reps=3, cpu=True):
@@ -279,7 +282,7 @@ def ns_cpu_baseline(n=48, budget_s=15.0, max_steps=200):
                        f"{steps} RK4 steps ({el:.1f} s)")
 
 
-_LIGHT = "--no-cpu --no-implicit --no-c1 --no-rcm --accuracy-steps 0 --c4-vertices -1"
+_LIGHT = "--no-cpu --no-implicit --no-c1 --no-rcm --no-f32sum --accuracy-steps 0 --c4-vertices -1"
 PROFILE_CMDS = {
     "trace": "rocprofv3 --kernel-trace --stats -f csv -d {d}/trace -o a -- python3 bench.py --steps 20 --warmup 2 "
              + _LIGHT + " --ns-steps 20 --ns-adj-steps 4",
@@ -410,7 +413,7 @@ def summarize_profiles(d, n=100, ns_n=256):
                                                                    byte_model(nc2, nf2, 0)["flux"]),
         "void ty::k_gradient<3, float, 4, 1, true, false>": ("6M tets K1 FP32",
                                                            byte_model(nc2, nf2, 0, mixed=True)["grad"]),
-        "void ty::k_flux<3, float, true, false, 4, false, true>": ("6M tets K2 FP32 (FP64 U0 update)",
+        "void ty::k_flux<3, float, true, false, 4, false, true>": ("6M tets K2 FP32 (FP64 RK accumulation)",
                                                                   byte_model(nc2, nf2, 0, mixed=True)["flux"]),
         "void ty::k_gradient<3, double, 6, 0, true, true>": ("16.8M hexes NS K1 FP64 (LSQ, viscous records)",
                                                            byte_model(ncn, nfn, nbn)["grad"]),
@@ -418,7 +421,7 @@ def summarize_profiles(d, n=100, ns_n=256):
                                                                    byte_model(ncn, nfn, nbn)["flux"]),
         "void ty::k_gradient<3, float, 6, 0, true, true>": ("16.8M hexes NS K1 FP32",
                                                           byte_model(ncn, nfn, nbn, mixed=True)["grad"]),
-        "void ty::k_flux<3, float, false, true, 6, false, true>": ("16.8M hexes NS K2 FP32 (FP64 U0 update)",
+        "void ty::k_flux<3, float, false, true, 6, false, true>": ("16.8M hexes NS K2 FP32 (FP64 RK accumulation)",
                                                                   byte_model(ncn, nfn, nbn, mixed=True)["flux"]),
     }
     adj_kernels = ("k_adj_face_geom<3>", "k_res_adjoint_face<3, true, false>", "k_res_adjoint_face<3, true, true>",
@@ -448,7 +451,7 @@ def summarize_profiles(d, n=100, ns_n=256):
     lines = ["# Residual kernels: durations, HBM traffic and issue counters", "",
              "6M periodic tets (100^3 hexes x 6; MUSCL + LSQ + Venkatakrishnan, HLLC, RK4) and 16.8M hexes (256^3, "
              "NS first-order HLLC + cell-full viscous flux, LSQ gradients, RK4), FP64 and FP32 (mixed: FP64 U0 and "
-             "update, FP32 stage-increment sum).  Algorithmic bytes: the byte model of bench.py (each array once per pass, int32 "
+             "RK accumulator).  Algorithmic bytes: the byte model of bench.py (each array once per pass, int32 "
              "indices).", ""] + head
     lines += ["| kernel | avg us | read GB | written GB | HBM GB | HBM TB/s | algorithmic GB | traffic / algorithmic "
               "| algorithmic TB/s (frac of 8) | VALU / VMEM-read instr per wave | VALU busy |",
@@ -582,14 +585,15 @@ def mesh_build_bench(m, phases, sample_n=64):
                 device_equals_host_on_sample=bool(same))
 
 
-def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
+def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20, f32sum=True):
     """The 16M-cell unsteady NS workload: compressible Taylor-Green field on n^3 hexes (Re 1e4,
     Sutherland, Pr 0.72, 1% seeded noise), periodic in x and z with slip walls at y = 0 and
     y = 2 pi L (the field's v vanishes there), first-order HLLC + cell-full viscous flux with LSQ
     gradients, RK4 at CFL 0.5, one uploaded mesh for everything below.
 
-    - throughput: FP64 and mixed (FP32 flux/gradient/stage state, FP64 U0, FP32 stage-increment sum)
-      advance the same initial field by ``steps`` steps; density error of mixed vs FP64;
+    - throughput: FP64, mixed (FP32 flux/gradient/stage state, FP64 U0 and RK accumulator) and, with
+      ``f32sum``, the mixed form that keeps the stage-increment sum in FP32 advance the same initial
+      field by ``steps`` steps; per-field errors of each mixed form vs FP64;
     - adjoint: J = pressure force normal to the y = 0 wall after ``adj_steps`` FP64 steps,
       dJ/dU0 by the device-resident checkpointed sweep (``snaps`` snapshots, FP64), checked against a central difference of two FP64 forward runs along a seeded
       direction."""
@@ -613,7 +617,8 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
                mesh_build=mesh_build_bench(m, phases))
     states = {}
     dt = None
-    for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
+    forms = [("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)] + ([("mixed_f32sum", S.PREC_MIXED_F32SUM)] if f32sum else [])
+    for name, prec in forms:
         sc = dict(s)
         sc["precision"] = prec
         solver.set_scheme(sc, g)
@@ -637,7 +642,7 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
                          grad_ms_avg=round(tm["grad_ms"] / max(1, tm["grad_n"]), 4),
                          flux_ms_avg=round(tm["flux_ms"] / max(1, tm["flux_n"]), 4))
         info = solver.mesh_info()
-        bm = byte_model(nc, info["nface"], info["nbface"], mixed=prec == S.PREC_MIXED)
+        bm = byte_model(nc, info["nface"], info["nbface"], mixed=prec != S.PREC_FP64, f32sum=prec == S.PREC_MIXED_F32SUM)
         for kern in ("grad", "flux"):
             out[name][f"{kern}_achieved_GBs"] = round(bm[kern] / (out[name][f"{kern}_ms_avg"] * 1e-3) / 1e9, 1)
         out[name]["bytes_per_cell_eval"] = round(bm["total"] / nc, 1)
@@ -679,6 +684,12 @@ def ns_bench(n=256, steps=1000, warmup=2, adj_steps=200, snaps=20):
     out["speedup_mixed_vs_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
     out["mixed_error_after_steps"] = dict(steps=steps, reference="the FP64 GPU form",
                                           fields=field_errors(rmx, r64))
+    if "mixed_f32sum" in states:
+        rfx = states.pop("mixed_f32sum")
+        out["speedup_mixed_f32sum_vs_fp64"] = round(out["mixed_f32sum"]["value"] / out["fp64"]["value"], 3)
+        out["mixed_f32sum_error_after_steps"] = dict(steps=steps, reference="the FP64 GPU form",
+                                                     fields=field_errors(rfx, r64))
+        del rfx
     out["dt"] = dt
     del r64, rmx
     if adj_steps > 0:
@@ -798,6 +809,8 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="steps of the FP64-vs-mixed per-field error run of the headline workload (0: skip)")
     ap.add_argument("--no-rcm", action="store_true", help="skip the reverse Cuthill-McKee order run")
+    ap.add_argument("--no-f32sum", action="store_true",
+                    help="skip the mixed form that keeps the RK stage-increment sum in FP32")
     ap.add_argument("--no-c1", action="store_true", help="skip the 2D channel workload")
     ap.add_argument("--c4-vertices", type=int, default=64,
                     help="wall vertices in the bump-channel gradient check (-1: skip the check)")
@@ -885,8 +898,8 @@ def main():
         bmm = byte_model(nc, nf_dev, nb_dev, mixed=True)
         fl = tmm["flux_ms"] / max(1, tmm["flux_n"])
         out["mixed"] = {"value": round(nc * evals / tmx / 1e6, 3), "unit": "Mcell-updates/s",
-                        "dtype": "fp32 flux/gradient/stage state, face states and stage-increment sum; fp64 U0, "
-                                 "its update and the primitive recovery from total energy",
+                        "dtype": "fp32 flux/gradient/stage state and face states; fp64 U0, RK accumulator, "
+                                 "primitive recovery from total energy",
                         "speedup_vs_fp64": round(t64 / tmx, 3),
                         "error_after_timed_steps": dict(steps=args.steps + args.warmup,
                                                         fields=field_errors(stmx, st64)),
@@ -894,9 +907,25 @@ def main():
                         "grad_ms_avg": round(tmm["grad_ms"] / max(1, tmm["grad_n"]), 4),
                         "flux_achieved_GBs": round(bmm["flux"] / (fl * 1e-3) / 1e9, 1)}
         print(f"# mixed: {out['mixed']['value']} Mcell-updates/s", file=sys.stderr, flush=True)
+        forms = [("mixed", S.PREC_MIXED)]
+        if not args.no_f32sum:
+            tfx, tmf, stfx, _ = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_MIXED_F32SUM, args.order)
+            bmf = byte_model(nc, nf_dev, nb_dev, mixed=True, f32sum=True)
+            fl = tmf["flux_ms"] / max(1, tmf["flux_n"])
+            out["mixed_f32sum"] = {
+                "value": round(nc * evals / tfx / 1e6, 3), "unit": "Mcell-updates/s",
+                "dtype": "the mixed form with the RK stage-increment sum kept in fp32 between stages (fp64 U0, "
+                         "its update and the primitive recovery)",
+                "speedup_vs_fp64": round(t64 / tfx, 3),
+                "error_after_timed_steps": dict(steps=args.steps + args.warmup, fields=field_errors(stfx, st64)),
+                "flux_ms_avg": round(fl, 4), "grad_ms_avg": round(tmf["grad_ms"] / max(1, tmf["grad_n"]), 4),
+                "flux_achieved_GBs": round(bmf["flux"] / (fl * 1e-3) / 1e9, 1)}
+            print(f"# mixed_f32sum: {out['mixed_f32sum']['value']} Mcell-updates/s", file=sys.stderr, flush=True)
+            forms.append(("mixed_f32sum", S.PREC_MIXED_F32SUM))
         if args.accuracy_steps > 0:
-            out["mixed"]["error_after_1000_steps" if args.accuracy_steps == 1000 else "error_after_steps"] = \
-                mixed_accuracy(m, s, g, u, args.accuracy_steps, args.order)
+            errs = mixed_accuracy(m, s, g, u, args.accuracy_steps, args.order, tuple(p for _, p in forms))
+            for (name, _), e in zip(forms, errs):
+                out[name]["error_after_1000_steps" if args.accuracy_steps == 1000 else "error_after_steps"] = e
             print("# mixed accuracy done", file=sys.stderr, flush=True)
     if not args.no_rcm and args.order != "rcm":
         tr, tmr, _, _ = run_gpu(m, s, g, u, args.steps, args.warmup, S.PREC_FP64, "rcm")
@@ -924,7 +953,7 @@ def main():
             out["implicit"] = {"error": str(e)}
     if not args.no_ns:
         try:
-            out["ns_16m"] = ns_bench(args.ns_n, args.ns_steps, adj_steps=args.ns_adj_steps)
+            out["ns_16m"] = ns_bench(args.ns_n, args.ns_steps, adj_steps=args.ns_adj_steps, f32sum=not args.no_f32sum)
         except Exception as e:  # reported, not fatal
             out["ns_16m"] = {"error": str(e)}
         if not args.no_cpu and "error" not in out["ns_16m"]:

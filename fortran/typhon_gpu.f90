@@ -16,7 +16,7 @@ implicit none
 
 integer(c_int), parameter :: TY_OK = 0, TY_ERR_ARG = 1, TY_ERR_HIP = 2, TY_ERR_STATE = 3, &
                              TY_ERR_UNSUPPORTED = 4, TY_ERR_NOT_CONVERGED = 5
-integer(c_int32_t), parameter :: TY_PREC_FP64 = 0, TY_PREC_MIXED = 1
+integer(c_int32_t), parameter :: TY_PREC_FP64 = 0, TY_PREC_MIXED = 1, TY_PREC_MIXED_F32SUM = 2
 ! element codes of ELEMVTEX.f90 taken by ty_meshbuild_faces
 integer(c_int32_t), parameter :: TY_ELEM_TRI3 = 13, TY_ELEM_QUAD4 = 24, TY_ELEM_TETRA4 = 101, TY_ELEM_HEXA8 = 141
 

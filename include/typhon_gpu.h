@@ -27,7 +27,10 @@ extern "C" {
 #define TY_ERR_NOT_CONVERGED 5
 
 #define TY_PREC_FP64 0        /* everything in FP64 */
-#define TY_PREC_MIXED 1       /* FP32 stage state, gradients, fluxes, geometry, stage-increment sum; FP64 U0 and update */
+#define TY_PREC_MIXED 1       /* FP32 stage state, gradients, fluxes, geometry; FP64 U0/accumulator */
+#define TY_PREC_MIXED_F32SUM 2 /* TY_PREC_MIXED with the RK stage-increment sum kept in FP32 between
+                                  stages (U0 and its update stay FP64): 30 B less per cell and RK4
+                                  stage; no counterpart in the reference, which is FP64 throughout */
 
 typedef struct ty_ctx ty_ctx;
 
@@ -94,7 +97,7 @@ typedef struct {
     int32_t nstage;          /* rk%nstage (1..8) */
     const double* rk_coef;   /* rk%coef(1:nstage,1:nstage), column-major */
     double cfl;              /* deftime%stabnb */
-    int32_t precision;       /* TY_PREC_FP64 or TY_PREC_MIXED */
+    int32_t precision;       /* TY_PREC_FP64, TY_PREC_MIXED or TY_PREC_MIXED_F32SUM */
 } ty_scheme;
 
 /* Matrix-free implicit solve (no reference counterpart: the reference's INVERSION=GMRESFREE

@@ -3,8 +3,8 @@
 FP64 bar: 1e-12 relative (max-norm per field) on residuals and on states after
 a few RK steps -- the summation order differs (per-cell slot order on the GPU,
 face order in the oracle, colour order in the reference), so results agree to
-a few ulps, not bitwise.  Mixed precision (FP32 fluxes/gradients, FP64 state
-and update) is checked at the tolerance written in each test.
+a few ulps, not bitwise.  Mixed precision (FP32 fluxes/gradients, FP64
+accumulation) is checked at the tolerance written in each test.
 """
 import numpy as np
 import pytest
@@ -81,7 +81,7 @@ def test_kinetic_fluxes_second_order_steps(sch):
 
 @pytest.mark.parametrize("sch", [S.SCH_HLLCK, S.SCH_HLLEKB])
 def test_kinetic_fluxes_mixed_precision(sch):
-    """FP32 fluxes (erff/expf) with the FP64 state update: the FP64 state to 1e-5 after 5 steps"""
+    """FP32 fluxes (erff/expf) with FP64 accumulation: the FP64 state to 1e-5 after 5 steps"""
     m, s, g, u = K.channel_case(24, 12, sch_hyp=sch)
     res = {}
     for prec in (S.PREC_FP64, S.PREC_MIXED):
@@ -197,13 +197,15 @@ def _field_errors(a, b):
     return out
 
 
-def test_mixed_precision_steps():
-    # FP64 state update and FP64 pressure recovery: every field within 1e-6 relative of the FP64
-    # run after 20 RK4 steps (second-order MUSCL + Venkatakrishnan)
+@pytest.mark.parametrize("form", [S.PREC_MIXED, S.PREC_MIXED_F32SUM])
+def test_mixed_precision_steps(form):
+    # FP64 RK accumulation (or, PREC_MIXED_F32SUM, the stage-increment sum kept in FP32 and the FP64
+    # state update) and FP64 pressure recovery: every field within 1e-6 relative of the FP64 run
+    # after 20 RK4 steps (second-order MUSCL + Venkatakrishnan)
     m, s, g, u = K.vortex_case(6, L=6.0)
     a = Solver(m, s, g)
     s2 = dict(s)
-    s2["precision"] = S.PREC_MIXED
+    s2["precision"] = form
     b = Solver(m, s2, g)
     a.set_state(u)
     b.set_state(u)
@@ -214,6 +216,27 @@ def test_mixed_precision_steps():
     err = _field_errors(b.get_state(), a.get_state())
     print("mixed vs fp64 after 20 RK4 steps:", err)
     assert max(err.values()) <= 1e-6, err
+
+
+def test_f32sum_form_differs_from_mixed_by_the_sum_rounding_only():
+    """the two mixed forms run the same FP32 kernels; they differ only in the FP32 rounding of the
+    stage-increment sum, a relative 6e-8 of an increment: after 20 RK4 steps they agree far more
+    closely than either agrees with FP64, and are not bitwise equal (the FP32 sum is in use)"""
+    m, s, g, u = K.vortex_case(6, L=6.0)
+    sv = {f: Solver(m, dict(s, precision=f), g) for f in (S.PREC_FP64, S.PREC_MIXED, S.PREC_MIXED_F32SUM)}
+    for x in sv.values():
+        x.set_state(u)
+    for _ in range(20):
+        dt = sv[S.PREC_FP64].timestep()
+        for x in sv.values():
+            x.rk_step(dt)
+    st = {f: x.get_state() for f, x in sv.items()}
+    for x in sv.values():
+        x.close()
+    d_forms = max(_field_errors(st[S.PREC_MIXED_F32SUM], st[S.PREC_MIXED]).values())
+    d_fp64 = max(_field_errors(st[S.PREC_MIXED], st[S.PREC_FP64]).values())
+    print(f"f32sum vs mixed {d_forms:.2e}, mixed vs fp64 {d_fp64:.2e}")
+    assert 0.0 < d_forms <= 1e-6 and d_forms <= d_fp64
 
 
 def test_mixed_precision_recovers_pressure_in_fp64():

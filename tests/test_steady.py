@@ -183,7 +183,7 @@ def test_gpu_steady_host_checks_do_not_change_the_cycle():
 
 @pytest.mark.gpu
 def test_gpu_steady_cycle_mixed_precision():
-    """FP32 fluxes with the FP64 state update and residual: the history follows the FP64 one while the
+    """FP32 fluxes with FP64 accumulation and residual: the history follows the FP64 one while the
     residual is above the FP32 round-off of the fluxes"""
     from typhon_amd.solver import Solver
     m, s, g, u = K.channel_case(24, 12, tps_meth=S.TPS_RK4)

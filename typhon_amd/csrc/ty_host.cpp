@@ -69,6 +69,7 @@ struct ty_ctx {
     std::vector<double> rk;  // coef(i,j) row-major [ns][ns]
     int nv = 0;
     bool mixed = false;
+    bool acc32 = false;  // TY_PREC_MIXED_F32SUM
     Physics ph{};
     // device buffers
     std::vector<void*> owned;
@@ -738,7 +739,8 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
         return fail(c, TY_ERR_UNSUPPORTED, "dissipative flux scheme");
     if ((s->method != 'N' || viscous) && s->gradmeth != 10 && s->gradmeth != 12)
         return fail(c, TY_ERR_UNSUPPORTED, "gradient method (LSQ or W-LSQ required)");
-    if (s->precision != TY_PREC_FP64 && s->precision != TY_PREC_MIXED) return fail(c, TY_ERR_ARG, "precision");
+    if (s->precision != TY_PREC_FP64 && s->precision != TY_PREC_MIXED && s->precision != TY_PREC_MIXED_F32SUM)
+        return fail(c, TY_ERR_ARG, "precision");
     for (const ty_boco_desc& d : c->bocos)
         if ((d.type == 20 || d.type == 22) && !viscous)
             return fail(c, TY_ERR_UNSUPPORTED, "flux and adiabatic walls need the laminar NS model "
@@ -785,7 +787,8 @@ int ty_set_scheme(ty_ctx* c, const ty_gas* gas, const ty_scheme* s)
         for (int j = 0; j < ns; ++j) c->rk[(size_t)i * ns + j] = s->rk_coef[(size_t)j * ns + i];
     c->sch.rk_coef = nullptr;
     c->nv = c->dim + 2;
-    c->mixed = s->precision == TY_PREC_MIXED;
+    c->mixed = s->precision != TY_PREC_FP64;
+    c->acc32 = s->precision == TY_PREC_MIXED_F32SUM;
     Physics& ph = c->ph;
     ph.gamma = gas->gamma;
     ph.rgas = gas->r_const;
@@ -965,7 +968,9 @@ KArgs<R> make_args(ty_ctx* c, const Geo<R>& g)
     a.us = (const R*)c->d_us[c->cur];
     a.us_next = (R*)c->d_us[c->cur ^ 1];
     a.u0 = c->d_u0;
-    a.acc = (R*)c->d_acc;  // sized for FP64; the FP32 kernels use its first half
+    a.acc = c->d_acc;
+    // TY_PREC_MIXED_F32SUM: the FP32 kernels keep the sum in the first half of the same buffer
+    a.acc32 = sizeof(R) == sizeof(float) && c->acc32 ? (float*)c->d_acc : nullptr;
     a.rstore = c->d_rstore;
     const int prec = sizeof(R) == sizeof(float) ? 1 : 0;
     a.grad = (const R*)c->work_grad[prec];

@@ -132,7 +132,8 @@ struct KArgs {
                           // recovered from the FP64 conservative state by its producer
     R* us_next;           // [n][nv]
     double* u0;           // [nv][n]
-    R* acc;               // [nv][n] the step's sum of b_j dt/V R_j so far: FP64, FP32 in the mixed form
+    double* acc;          // [nv][n] the step's sum of b_j dt/V R_j over the stages so far
+    float* acc32;         // non-null (FP32 kernels of TY_PREC_MIXED_F32SUM): that sum kept in FP32 here
     double* rstore;       // [nstore][nv][n]
     const R* grad;        // [n][nv*dim] LSQ gradients of (rho, p, u..), one record per cell; with vrec
                           // set, the viscous records [n][dim*dim + dim] instead: the velocity

@@ -1505,16 +1505,18 @@ __device__ __forceinline__ void flux_cell(const KArgs<R>& a, int c)
         return;
     }
     // RK bookkeeping (treelevel_rungekutta.f90:60-105).  The increments are summed in FP64 and
-    // kept between stages in R: in the mixed form each R_j comes from FP32 face fluxes, whose
-    // rounding (relative to the sum of |face flux|) already exceeds the FP32 rounding of the
-    // partial sum; the state itself stays FP64 (U = U0 + sum, in FP64, at the last stage).
+    // kept between stages in FP64, or in FP32 with a.acc32 (TY_PREC_MIXED_F32SUM: each R_j comes
+    // from FP32 face fluxes, whose rounding, relative to the sum of |face flux|, already exceeds
+    // the FP32 rounding of the partial sum); the state stays FP64 (U = U0 + sum at the last stage).
+    const bool a32 = sizeof(R) == sizeof(float) && a.acc32 != nullptr;
     double acc[NV];
     if (a.stage == 1) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = Rv[v] * a.b;
     } else {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] = (double)a.acc[(size_t)v * n + c] + a.b * Rv[v];
+        for (int v = 0; v < NV; ++v)
+            acc[v] = (a32 ? (double)a.acc32[(size_t)v * n + c] : a.acc[(size_t)v * n + c]) + a.b * Rv[v];
     }
     if (a.store_slot >= 0) {
         double* rs = a.rstore + (size_t)a.store_slot * NV * n;
@@ -1525,7 +1527,12 @@ __device__ __forceinline__ void flux_cell(const KArgs<R>& a, int c)
     double un[NV];
     if (a.stage < a.nstage) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) a.acc[(size_t)v * n + c] = (R)acc[v];
+        for (int v = 0; v < NV; ++v) {
+            if (a32)
+                a.acc32[(size_t)v * n + c] = (float)acc[v];
+            else
+                a.acc[(size_t)v * n + c] = acc[v];
+        }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             double u = a.u0[(size_t)v * n + c];

@@ -26,7 +26,7 @@ DIS_COMPACT, DIS_FACECENTERED, DIS_FACEPENALTY = 2, 20, 22  # refused for NS, as
 EQ_EULER, EQ_NSLAM = 10, 15
 VISC_SUTH, VISC_DYNCST, VISC_KINCST = 11, 13, 14
 # precision forms
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_F32SUM = 0, 1, 2
 
 
 def rk_tableau(method: int) -> np.ndarray:
