@@ -409,19 +409,19 @@ def summarize_profiles(d, n=100, ns_n=256):
     models = {  # kernel name prefix -> (label, algorithmic bytes per launch)
         "void ty::k_gradient<3, double, 4, 1, true, false>": ("6M tets K1 FP64 (LSQ + Venkatakrishnan, face states)",
                                                             byte_model(nc2, nf2, 0)["grad"]),
-        "void ty::k_flux<3, double, true, false, 4, false, true>": ("6M tets K2 FP64 (HLLC, gather, RK4)",
+        "void ty::k_flux<3, double, true, false, 4, false, true, false>": ("6M tets K2 FP64 (HLLC, gather, RK4)",
                                                                    byte_model(nc2, nf2, 0)["flux"]),
         "void ty::k_gradient<3, float, 4, 1, true, false>": ("6M tets K1 FP32",
                                                            byte_model(nc2, nf2, 0, mixed=True)["grad"]),
-        "void ty::k_flux<3, float, true, false, 4, false, true>": ("6M tets K2 FP32 (FP64 RK accumulation)",
+        "void ty::k_flux<3, float, true, false, 4, false, true, false>": ("6M tets K2 FP32 (FP64 RK accumulation)",
                                                                   byte_model(nc2, nf2, 0, mixed=True)["flux"]),
         "void ty::k_gradient<3, double, 6, 0, true, true>": ("16.8M hexes NS K1 FP64 (LSQ, viscous records)",
                                                            byte_model(ncn, nfn, nbn)["grad"]),
-        "void ty::k_flux<3, double, false, true, 6, false, true>": ("16.8M hexes NS K2 FP64 (HLLC + viscous, RK4)",
+        "void ty::k_flux<3, double, false, true, 6, false, true, false>": ("16.8M hexes NS K2 FP64 (HLLC + viscous, RK4)",
                                                                    byte_model(ncn, nfn, nbn)["flux"]),
         "void ty::k_gradient<3, float, 6, 0, true, true>": ("16.8M hexes NS K1 FP32",
                                                           byte_model(ncn, nfn, nbn, mixed=True)["grad"]),
-        "void ty::k_flux<3, float, false, true, 6, false, true>": ("16.8M hexes NS K2 FP32 (FP64 RK accumulation)",
+        "void ty::k_flux<3, float, false, true, 6, false, true, false>": ("16.8M hexes NS K2 FP32 (FP64 RK accumulation)",
                                                                   byte_model(ncn, nfn, nbn, mixed=True)["flux"]),
     }
     adj_kernels = ("k_adj_face_geom<3>", "k_res_adjoint_face<3, true, false>", "k_res_adjoint_face<3, true, true>",

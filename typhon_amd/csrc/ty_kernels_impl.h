@@ -1258,25 +1258,28 @@ __device__ __forceinline__ void invert_block(double* m, double* inv)
 template <typename R, bool FS, bool VISC, bool JAC>
 constexpr int flux_min_waves() { return (VISC && !FS && !JAC && sizeof(R) == 8) ? 3 : 1; }
 
-template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC, bool DENSE, bool BND>
+template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC, bool DENSE, bool BND, bool A32>
 __device__ __forceinline__ void flux_cell(const KArgs<R>& a, int c);
 
 // Meshes whose cells use every slot run the wavefronts without boundary slots through a copy of
-// the cell's code without the boundary branches (as k_gradient).
-template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC, bool DENSE>
+// the cell's code without the boundary branches (as k_gradient).  A32: the FP32 kernels of
+// TY_PREC_MIXED_F32SUM, which keep the RK stage-increment sum in a.acc32 (an instance of its own:
+// a run-time choice inside one kernel cost the 16.8M-hex NS kernel most of the bytes it saves).
+template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC, bool DENSE, bool A32>
 __global__ __launch_bounds__(BLOCK, (flux_min_waves<R, FS, VISC, JAC>())) void k_flux(KArgs<R> a)
 {
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= a.n || (a.stop && *a.stop)) return;
     if (!DENSE || JAC || !flux_interior_path<R, VISC>() || wave_boundary(a.bnd_bits, c))
-        flux_cell<DIM, R, FS, VISC, NSM, JAC, DENSE, true>(a, c);
+        flux_cell<DIM, R, FS, VISC, NSM, JAC, DENSE, true, A32>(a, c);
     else
-        flux_cell<DIM, R, FS, VISC, NSM, JAC, DENSE, false>(a, c);
+        flux_cell<DIM, R, FS, VISC, NSM, JAC, DENSE, false, A32>(a, c);
 }
 
-template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC, bool DENSE, bool BND>
+template <int DIM, typename R, bool FS, bool VISC, int NSM, bool JAC, bool DENSE, bool BND, bool A32>
 __device__ __forceinline__ void flux_cell(const KArgs<R>& a, int c)
 {
+    static_assert(!A32 || sizeof(R) == sizeof(float), "the FP32 increment sum is a form of the FP32 kernels");
     constexpr int NV = DIM + 2;
     const int n = a.n;
     const R gam = R(a.ph.gamma);
@@ -1505,10 +1508,10 @@ __device__ __forceinline__ void flux_cell(const KArgs<R>& a, int c)
         return;
     }
     // RK bookkeeping (treelevel_rungekutta.f90:60-105).  The increments are summed in FP64 and
-    // kept between stages in FP64, or in FP32 with a.acc32 (TY_PREC_MIXED_F32SUM: each R_j comes
+    // kept between stages in FP64, or in FP32 in a.acc32 (A32, TY_PREC_MIXED_F32SUM: each R_j comes
     // from FP32 face fluxes, whose rounding, relative to the sum of |face flux|, already exceeds
     // the FP32 rounding of the partial sum); the state stays FP64 (U = U0 + sum at the last stage).
-    const bool a32 = sizeof(R) == sizeof(float) && a.acc32 != nullptr;
+    constexpr bool a32 = A32;
     double acc[NV];
     if (a.stage == 1) {
 #pragma unroll
@@ -1728,16 +1731,28 @@ hipError_t launch_gradient(const KArgs<R>& a, hipStream_t s)
     return hipGetLastError();
 }
 
-template <int DIM, typename R, int NSM, bool JAC, bool DENSE>
-void flux_variant(const KArgs<R>& a, hipStream_t s)
+template <int DIM, typename R, int NSM, bool JAC, bool DENSE, bool A32>
+void flux_launch(const KArgs<R>& a, hipStream_t s)
 {
     const dim3 grid(grid_blocks(a.n)), block(BLOCK);
     const bool fs = a.ph.method != 'N';
     const bool visc = a.ph.viscous != 0;
-    if (fs && visc) hipLaunchKernelGGL((k_flux<DIM, R, true, true, NSM, JAC, DENSE>), grid, block, 0, s, a);
-    else if (fs) hipLaunchKernelGGL((k_flux<DIM, R, true, false, NSM, JAC, DENSE>), grid, block, 0, s, a);
-    else if (visc) hipLaunchKernelGGL((k_flux<DIM, R, false, true, NSM, JAC, DENSE>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_flux<DIM, R, false, false, NSM, JAC, DENSE>), grid, block, 0, s, a);
+    if (fs && visc) hipLaunchKernelGGL((k_flux<DIM, R, true, true, NSM, JAC, DENSE, A32>), grid, block, 0, s, a);
+    else if (fs) hipLaunchKernelGGL((k_flux<DIM, R, true, false, NSM, JAC, DENSE, A32>), grid, block, 0, s, a);
+    else if (visc) hipLaunchKernelGGL((k_flux<DIM, R, false, true, NSM, JAC, DENSE, A32>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_flux<DIM, R, false, false, NSM, JAC, DENSE, A32>), grid, block, 0, s, a);
+}
+
+template <int DIM, typename R, int NSM, bool JAC, bool DENSE>
+void flux_variant(const KArgs<R>& a, hipStream_t s)
+{
+    if constexpr (sizeof(R) == sizeof(float) && !JAC) {
+        if (a.acc32 && a.mode != 1) {
+            flux_launch<DIM, R, NSM, JAC, DENSE, true>(a, s);
+            return;
+        }
+    }
+    flux_launch<DIM, R, NSM, JAC, DENSE, false>(a, s);
 }
 
 template <int DIM, typename R>
